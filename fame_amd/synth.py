@@ -1,0 +1,226 @@
+"""Seeded synthetic WGBS workloads: reference genomes and bisulfite reads.
+
+Genome model: i.i.d. bases with GC 41 % (A = T = 0.295, C = G = 0.205); CpG
+dinucleotides are depleted to about 1 % of positions by turning the G of a
+CpG into A.  Written as FASTA with ``>chrN`` headers and 80 columns, the form
+FAME's reader parses (RefReader_istr.cpp:25-160).
+
+Read model (directional library): a read is sampled uniformly from the
+original top strand (OT) or the original bottom strand (OB, the reverse
+complement).  Each genomic CpG carries a methylation level drawn from
+Beta(0.5, 0.5); a read C in CpG context stays C with that probability and
+becomes T otherwise; every other C becomes T with probability 0.995.  Then
+sequencing noise: substitutions at ``sub_rate`` and 1-bp insertions or
+deletions at ``indel_rate`` per base; qualities uniform in Phred 30-40.
+The bisulfite toss follows the reference's simulator
+(Synth/SynthDS.cpp:333-346) in spirit; the generator itself is our own.
+
+Everything is numpy and vectorised so that a 300 Mbp genome and 10 M reads
+are produced in tens of seconds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+_BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
+_COMP = np.zeros(256, dtype=np.uint8)
+for _a, _b in zip(b"ACGTN", b"TGCAN"):
+    _COMP[_a] = _b
+
+
+def make_genome(chrom_lengths, seed=1, cpg_target=0.01):
+    """Return a list of uint8 arrays (one per chromosome) over b'ACGT'."""
+    rng = np.random.default_rng(seed)
+    p = np.array([0.295, 0.205, 0.205, 0.295])
+    chroms = []
+    for n in chrom_lengths:
+        seq = _BASES[rng.choice(4, size=int(n), p=p)]
+        is_cpg = (seq[:-1] == ord("C")) & (seq[1:] == ord("G"))
+        natural = max(float(is_cpg.mean()), 1e-12)
+        keep = min(1.0, cpg_target / natural)
+        drop = is_cpg & (rng.random(is_cpg.size) >= keep)
+        g_idx = np.nonzero(drop)[0] + 1
+        seq[g_idx] = ord("A")
+        chroms.append(seq)
+    return chroms
+
+
+def write_fasta(path, chroms, names=None, width=80):
+    with open(path, "wb") as f:
+        for i, seq in enumerate(chroms):
+            name = names[i] if names else f"chr{i + 1}"
+            f.write(b">" + name.encode() + b"\n")
+            b = seq.tobytes()
+            for s in range(0, len(b), width):
+                f.write(b[s:s + width] + b"\n")
+
+
+def revcomp(a):
+    return _COMP[a[..., ::-1]]
+
+
+class ReadSet:
+    """Reads of one length class held as a dense (n, L) uint8 matrix."""
+
+    def __init__(self, seqs, lens, quals=None, truth=None):
+        self.seqs = seqs          # (n, Lmax) uint8, rows padded with 0 past lens
+        self.lens = lens          # (n,) uint32
+        self.quals = quals
+        self.truth = truth        # dict of per-read generator truth
+
+    def __len__(self):
+        return int(self.lens.shape[0])
+
+    def flat(self):
+        """Concatenated bytes plus u64 offsets / u32 lengths (the C-ABI input form)."""
+        n = len(self)
+        lens = self.lens.astype(np.uint64)
+        offs = np.zeros(n, dtype=np.uint64)
+        if n:
+            offs[1:] = np.cumsum(lens)[:-1]
+        if n and np.all(self.lens == self.seqs.shape[1]):
+            buf = np.ascontiguousarray(self.seqs).reshape(-1)
+        else:
+            mask = np.arange(self.seqs.shape[1])[None, :] < self.lens[:, None]
+            buf = self.seqs[mask]
+        return np.ascontiguousarray(buf, dtype=np.uint8), offs, self.lens.astype(np.uint32)
+
+    def write_fastq(self, path, start=0, stop=None, name_prefix="r"):
+        stop = len(self) if stop is None else stop
+        with open(path, "wb") as f:
+            for i in range(start, stop):
+                L = int(self.lens[i])
+                s = self.seqs[i, :L].tobytes()
+                q = (self.quals[i, :L] if self.quals is not None
+                     else np.full(L, 73, np.uint8)).tobytes()
+                f.write(b"@" + f"{name_prefix}{i}".encode() + b"\n" + s + b"\n+\n" + q + b"\n")
+
+    def subset(self, idx):
+        t = None if self.truth is None else {k: v[idx] for k, v in self.truth.items()}
+        return ReadSet(self.seqs[idx], self.lens[idx],
+                       None if self.quals is None else self.quals[idx], t)
+
+
+def meth_levels(chroms, seed=7):
+    """Per-position methylation level arrays (only CpG C positions are used)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for seq in chroms:
+        lv = np.zeros(seq.size, dtype=np.float32)
+        c = np.nonzero((seq[:-1] == ord("C")) & (seq[1:] == ord("G")))[0]
+        lv[c] = rng.beta(0.5, 0.5, size=c.size).astype(np.float32)
+        out.append(lv)
+    return out
+
+
+def _bisulfite(reads, ctx_next, level_at, rng, non_cpg_conv=0.995):
+    """Convert C -> T in place on (n, L) reads.
+
+    ctx_next: (n, L) base following each read base on the read's strand.
+    level_at: (n, L) methylation level of the CpG a C belongs to (0 if none).
+    """
+    is_c = reads == ord("C")
+    in_cpg = is_c & (ctx_next == ord("G"))
+    u = rng.random(reads.shape, dtype=np.float32)
+    conv = np.where(in_cpg, u >= level_at, u < non_cpg_conv) & is_c
+    reads[conv] = ord("T")
+
+
+def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_rate=0.0005,
+               len_range=None, ob_fraction=0.5, batch=1 << 20, regions=None):
+    """Sample `n` bisulfite reads.
+
+    read_len   nominal (maximum) length; with `len_range=(lo, hi)` each read's
+               length is uniform in [lo, hi] (hi <= read_len).
+    regions    optional (chrom_idx, start, end) arrays restricting the origin
+               of every read (used for repeat-heavy mixes).
+    """
+    rng = np.random.default_rng(seed)
+    if levels is None:
+        levels = meth_levels(chroms, seed=seed + 11)
+    lens_c = np.array([c.size for c in chroms], dtype=np.int64)
+    pad = read_len + 8
+    seqs = np.zeros((n, read_len), dtype=np.uint8)
+    lens = np.zeros(n, dtype=np.uint32)
+    t_chrom = np.zeros(n, dtype=np.int32)
+    t_pos = np.zeros(n, dtype=np.int64)
+    t_ob = np.zeros(n, dtype=bool)
+    for s in range(0, n, batch):
+        m = min(batch, n - s)
+        if regions is None:
+            ci = rng.choice(lens_c.size, size=m, p=lens_c / lens_c.sum())
+            pos = (rng.random(m) * (lens_c[ci] - pad - 2)).astype(np.int64) + 1
+        else:
+            rc, rs, re_ = regions
+            k = rng.integers(0, rc.size, size=m)
+            ci = rc[k]
+            span = np.maximum(re_[k] - rs[k] - pad - 2, 1)
+            pos = rs[k] + (rng.random(m) * span).astype(np.int64) + 1
+        ob = rng.random(m) < ob_fraction
+        if len_range is None:
+            L = np.full(m, read_len, dtype=np.int64)
+        else:
+            L = rng.integers(len_range[0], len_range[1] + 1, size=m)
+        # fetch a window of pad+1 bases (+1 for CpG context past the end)
+        win = np.empty((m, pad + 2), dtype=np.uint8)
+        lvl = np.empty((m, pad + 2), dtype=np.float32)
+        for c in np.unique(ci):
+            sel = np.nonzero(ci == c)[0]
+            idx = pos[sel, None] + np.arange(-1, pad + 1)[None, :]
+            win[sel] = chroms[c][idx]
+            lvl[sel] = levels[c][idx]
+        # OT: bases win[1 : 1+pad], next-base context win[2 : 2+pad]
+        # OB: reverse complement of the same span; a CpG C on the bottom
+        # strand is the complement of the G of a top-strand CpG whose C sits
+        # one position to the left, so its level is read from there.
+        fwd = win[:, 1:1 + pad].copy()
+        fwd_next = win[:, 2:2 + pad]
+        fwd_lvl = lvl[:, 1:1 + pad]
+        rev = revcomp(win[:, 1:1 + pad])
+        rev_next = revcomp(win[:, 0:pad])
+        rev_lvl = lvl[:, 0:pad][:, ::-1]
+        read = np.where(ob[:, None], rev, fwd)
+        nxt = np.where(ob[:, None], rev_next, fwd_next)
+        lv = np.where(ob[:, None], rev_lvl, fwd_lvl)
+        _bisulfite(read, nxt, lv, rng)
+        # substitutions
+        sub = rng.random(read.shape, dtype=np.float32) < sub_rate
+        alt = _BASES[(np.searchsorted(_BASES, read) + rng.integers(1, 4, size=read.shape)) % 4]
+        read = np.where(sub, alt, read)
+        # 1-bp indels: at most one per read (the first drawn event), applied
+        # by a gather so that the whole batch stays vectorised
+        ev = rng.random(read.shape, dtype=np.float32) < indel_rate
+        has = ev.any(axis=1)
+        p_ev = np.where(has, ev.argmax(axis=1), pad + 1)[:, None]
+        is_del = (rng.random(m) < 0.5)[:, None]
+        j = np.arange(read_len)[None, :]
+        src = np.where(is_del, j + (j >= p_ev), j - (j > p_ev))
+        out = np.take_along_axis(read, np.minimum(src, pad - 1), axis=1)
+        ins_here = (~is_del) & (j == p_ev)
+        out = np.where(ins_here, _BASES[rng.integers(0, 4, size=out.shape)], out)
+        col = np.arange(read_len)[None, :]
+        out[col >= L[:, None]] = 0
+        seqs[s:s + m] = out
+        lens[s:s + m] = L
+        t_chrom[s:s + m] = ci
+        t_pos[s:s + m] = pos
+        t_ob[s:s + m] = ob
+    quals = (rng.integers(30, 41, size=seqs.shape) + 33).astype(np.uint8)
+    return ReadSet(seqs, lens, quals, {"chrom": t_chrom, "pos": t_pos, "ob": t_ob})
+
+
+def config_genome(name, scale=1.0, seed=1):
+    """Chromosome lengths for the named workloads.
+
+    small   : 2 chromosomes x 5 Mbp (the reference's CPU-runnable case)
+    genome  : 24 chromosomes, 300 Mbp in total
+    """
+    if name == "small":
+        return [int(5_000_000 * scale)] * 2
+    if name == "genome":
+        rng = np.random.default_rng(seed + 100)
+        w = rng.uniform(0.5, 1.5, size=24)
+        L = (w / w.sum() * 300_000_000 * scale).astype(np.int64)
+        return [int(x) for x in L]
+    raise ValueError(name)

@@ -1,0 +1,92 @@
+/* CPU restatement of FAME's single-end read-matching path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it; the product (libfamehip.so) never links or calls it.
+ *
+ * Every function restates the reference algorithm (file:line in
+ * FischerJo/FAME) with the same integer types, so that its outputs are
+ * bit-identical to the reference CPU path on the same index and reads.
+ */
+#ifndef FAME_ORACLE_H
+#define FAME_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_index orc_index;
+typedef struct orc_ctx orc_ctx;
+
+/* RefGenome::load RefGenome.cpp:230-387 + read_strands :414-433 */
+orc_index* orc_load_index(const char* path, char* err, size_t errlen);
+void orc_free_index(orc_index* ix);
+/* header and table sizes: htab, readlen, winlen, kmerlen, kmercutoff,
+ * seedbits, ncpg, nchrom, nkmer, nwin, nfiltered */
+void orc_index_info(const orc_index* ix, uint64_t out[11]);
+
+/* ReadQueue ctor ReadQueue.cpp:25-78 (single-end) */
+orc_ctx* orc_ctx_new(orc_index* ix, int both_strands);
+void orc_ctx_free(orc_ctx* c);
+
+/* ReadQueue::matchReads ReadQueue.cpp:404-933 over n reads given as a byte
+ * buffer + per-read offsets / lengths.  Counters are accumulated (+=) into
+ * counters[0..2] = succ, nonunique, unsucc.  valid[i] = !isInvalid and
+ * mat[i] = Read::mat (0 when invalid).  nthreads <= 0 uses OpenMP's default. */
+int orc_match_reads(orc_ctx* c, const uint8_t* seqs, const uint64_t* offs, const uint32_t* lens,
+                    uint32_t n, int get_stranded, uint64_t counters[3], uint8_t* valid,
+                    uint64_t* mat, int nthreads);
+/* ReadQueue::decideStrand ReadQueue.cpp:376-400; returns matchR1Fwd */
+int orc_decide_strand(orc_ctx* c, uint64_t* fwd, uint64_t* rev);
+/* 4 x u16 per CpG (methFwd, unmethFwd, methRev, unmethRev) in cpgTable order */
+void orc_get_meth(const orc_ctx* c, uint16_t* out);
+/* ReadQueue::printMethylationLevels ReadQueue.cpp:1860-1893 */
+int orc_write_tsv(const orc_ctx* c, const char* path);
+/* number of reads whose DP read reference bytes outside their chromosome
+ * (undefined behaviour in the reference; see orc_dp_align) */
+uint64_t orc_ub_reads(const orc_ctx* c);
+
+/* ---- unit-level entry points (pinned individually by the tests) ---- */
+
+/* ntHash::NTPS64 spaced_nthash/nthash.hpp:480-498; returns the spaced hash of
+ * the 32-mer at seq[0..32) under the seed given as SEEDBITS (MSB = pos 0). */
+uint64_t orc_ntps64(const uint8_t* kmer, uint32_t seedbits);
+/* same value computed with the rolling update (nthash.hpp:492-498) for every
+ * k-mer of seq; out[k] = hash of seq[k..k+32) */
+void orc_ntps64_rolling(const uint8_t* seq, uint32_t len, uint32_t seedbits, uint64_t* out);
+
+/* ShiftAnd<6>::querySeq / queryRevSeq (ShiftAnd.h:130-255).  text[0..tlen) is
+ * scanned forward (rev=0) or from text[tlen-1] down to text[0] with
+ * complementing (rev=1).  Positions are reported exactly as the reference
+ * does relative to its iterator pair; see the implementation.  Returns the
+ * number of matches written (<= cap); *total receives the full count. */
+int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
+                 uint64_t* pos, uint8_t* err, int cap, int* total);
+
+/* LevenshtDP<uint16_t,8> fill + backtrace (LevenshtDP.h:207-544) of read
+ * `row` (len L) against reference bytes ending at ref[ref_end] read
+ * backwards; ref[0..ref_len) is the chromosome.  rev selects runDPFillRev /
+ * backtrackDPRev with CompiRev.  Writes the ERROR_T trace (0 MATCHING,
+ * 1 MISMATCH, 2 INSERTION, 3 DELETION) and returns its length; *ub is set if
+ * the reference was read outside [0, ref_len). */
+int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len,
+                 int64_t ref_end, int rev, uint8_t* trace, int cap, int* ub);
+
+/* tsl::hopscotch_map<uint32_t,uint16_t,identity> emulation: inserts keys in
+ * the given order into a map constructed with `bucket_count`, and writes the
+ * keys in iteration order.  Returns the number of keys written. */
+int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count, uint32_t* out);
+
+/* getSeedRefs ReadQueue.cpp:2329-2461 for one sequence: writes the fwd and
+ * rev maps in iteration order as (metaId, count) pairs. */
+int orc_seed_maps(orc_ctx* c, const uint8_t* seq, uint32_t len, uint32_t* fwd_keys,
+                  uint16_t* fwd_cnt, int* nfwd, uint32_t* rev_keys, uint16_t* rev_cnt, int* nrev,
+                  int cap, uint32_t* bucket_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,187 @@
+// Reference-bench harness: links FAME's own translation units (compiled
+// unmodified from their source tree, with a configured CONST.h) and replaces
+// only main.cpp, so that the reference CPU path can be timed with
+// steady_clock and its per-read results dumped for parity checks.
+//
+// This file is test infrastructure: it is the checker and the CPU baseline,
+// never the product path.
+//
+// Modes
+//   build <genome.fa> <index> [--no_loss] [--human_opt]
+//       mirrors main.cpp:318-351 (readReference -> RefGenome ctor -> save)
+//   match <index> <reads.fq> <out_prefix> [--unord_reads] [--max_reads N]
+//       mirrors queryRoutine main.cpp:363-398; writes
+//         <out_prefix>_cpg.tsv      via ReadQueue::printMethylationLevels
+//         <out_prefix>_reads.bin    "FAMERES1", u64 n, u8 valid[n], u64 mat[n]
+//         <out_prefix>_stats.json   counters, strand tallies, per-chunk timings
+//
+// ReadQueue keeps readBuffer / strand tallies private; they are reached by
+// compiling this translation unit only with `private` defined as `public`.
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <list>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include <omp.h>
+
+#define private public
+#include "ReadQueue.h"
+#include "RefReader_istr.h"
+#undef private
+
+using clk = std::chrono::steady_clock;
+
+static double secs(clk::time_point a, clk::time_point b)
+{
+    return std::chrono::duration<double>(b - a).count();
+}
+
+struct ChunkTime {
+    unsigned n;
+    double s;
+    bool stranded;
+};
+
+static void dump_reads(const std::string& path, const std::vector<uint8_t>& valid,
+                       const std::vector<uint64_t>& mat)
+{
+    std::ofstream of(path, std::ios::binary | std::ios::trunc);
+    const char magic[8] = {'F', 'A', 'M', 'E', 'R', 'E', 'S', '1'};
+    of.write(magic, 8);
+    uint64_t n = valid.size();
+    of.write(reinterpret_cast<const char*>(&n), 8);
+    of.write(reinterpret_cast<const char*>(valid.data()), n);
+    of.write(reinterpret_cast<const char*>(mat.data()), 8 * n);
+}
+
+static void collect(std::vector<Read>& buf, unsigned n, std::vector<uint8_t>& valid,
+                    std::vector<uint64_t>& mat)
+{
+    for (unsigned i = 0; i < n; ++i) {
+        const Read& r = buf[i];
+        valid.push_back(r.isInvalid ? 0 : 1);
+        mat.push_back(r.isInvalid ? 0 : r.mat);
+    }
+}
+
+static void write_stats(const std::string& path, const char* mode, uint64_t nreads,
+                        uint64_t succ, uint64_t nonu, uint64_t unsucc, uint64_t succPaired,
+                        uint64_t tooShort, const ReadQueue& q, double load_s,
+                        const std::vector<ChunkTime>& chunks)
+{
+    double total = 0;
+    for (auto& c : chunks) total += c.s;
+    std::ofstream of(path, std::ios::trunc);
+    of << "{\"mode\": \"" << mode << "\", \"reads\": " << nreads << ", \"succ\": " << succ
+       << ", \"nonunique\": " << nonu << ", \"unsucc\": " << unsucc
+       << ", \"succ_paired\": " << succPaired << ", \"too_short\": " << tooShort
+       << ", \"r1_fwd\": " << q.r1FwdMatches << ", \"r1_rev\": " << q.r1RevMatches
+       << ", \"match_r1_fwd\": " << (q.matchR1Fwd ? "true" : "false")
+       << ", \"corenum\": " << CORENUM << ", \"htabsize\": " << MyConst::HTABSIZE
+       << ", \"readlen\": " << MyConst::READLEN << ", \"chunksize\": " << MyConst::CHUNKSIZE
+       << ", \"load_s\": " << load_s << ", \"match_s\": " << total << ", \"chunks\": [";
+    for (size_t i = 0; i < chunks.size(); ++i) {
+        of << (i ? ", " : "") << "{\"n\": " << chunks[i].n << ", \"s\": " << chunks[i].s
+           << ", \"stranded\": " << (chunks[i].stranded ? "true" : "false") << "}";
+    }
+    of << "]}\n";
+}
+
+static int do_build(int argc, char** argv)
+{
+    if (argc < 4) return 2;
+    bool noloss = false, human = false;
+    for (int i = 4; i < argc; ++i) {
+        if (!strcmp(argv[i], "--no_loss")) noloss = true;
+        if (!strcmp(argv[i], "--human_opt")) human = true;
+    }
+    std::vector<struct CpG> cpgTab;
+    std::vector<struct CpG> cpgStartTab;
+    std::vector<std::vector<char>> genSeq;
+    std::unordered_map<uint8_t, std::string> chrMap;
+    readReference(argv[2], cpgTab, cpgStartTab, genSeq, chrMap, human);
+    RefGenome ref(std::move(cpgTab), std::move(cpgStartTab), genSeq, noloss, chrMap);
+    ref.save(argv[3]);
+    return 0;
+}
+
+// parseChunk, truncated so that at most `budget` reads are taken from the
+// file in total (budget 0 = no limit); used for bounded CPU samples.
+static bool parse_bounded(ReadQueue& q, unsigned& n, uint64_t& seen, uint64_t budget)
+{
+    bool more = q.parseChunk(n);
+    if (budget && seen + n >= budget) {
+        n = static_cast<unsigned>(budget - seen);
+        more = false;
+    }
+    seen += n;
+    return more;
+}
+
+static int do_match(int argc, char** argv)
+{
+    if (argc < 5) return 2;
+    bool both = false;
+    uint64_t budget = 0;
+    for (int i = 5; i < argc; ++i) {
+        if (!strcmp(argv[i], "--unord_reads")) both = true;
+        if (!strcmp(argv[i], "--max_reads") && i + 1 < argc) budget = std::stoull(argv[++i]);
+    }
+    auto t0 = clk::now();
+    RefGenome ref{std::string(argv[2])};
+    double load_s = secs(t0, clk::now());
+    ReadQueue q(argv[3], ref, false, both);
+    std::vector<uint8_t> valid;
+    std::vector<uint64_t> mat;
+    std::vector<ChunkTime> chunks;
+    uint64_t succ = 0, nonu = 0, unsucc = 0, seen = 0;
+    unsigned n = 0;
+    auto run = [&](bool stranded) {
+        auto a = clk::now();
+        q.matchReads(n, succ, nonu, unsucc, stranded);
+        chunks.push_back({n, secs(a, clk::now()), stranded});
+        collect(q.readBuffer, n, valid, mat);
+    };
+    // queryRoutine, main.cpp:374-391: the first chunk is matched in both
+    // orientations and decides the strand, full chunks follow, then the
+    // remainder of the last parse (possibly empty) is matched.
+    if (!both) {
+        parse_bounded(q, n, seen, budget);
+        run(true);
+        q.decideStrand();
+    }
+    while (parse_bounded(q, n, seen, budget)) run(false);
+    run(false);
+    std::string out = argv[4];
+    q.printMethylationLevels(out);
+    dump_reads(out + "_reads.bin", valid, mat);
+    write_stats(out + "_stats.json", "se", valid.size(), succ, nonu, unsucc, 0, 0, q, load_s,
+                chunks);
+    std::cout << "HARNESS reads=" << valid.size() << " succ=" << succ << " nonunique=" << nonu
+              << " unsucc=" << unsucc << "\n";
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::cerr << "usage: build|match ...\n";
+        return 2;
+    }
+    MyConst::sanityChecks();
+    std::string mode = argv[1];
+    if (mode == "build") return do_build(argc, argv);
+    if (mode == "match") return do_match(argc, argv);
+    std::cerr << "unknown mode\n";
+    return 2;
+}
