@@ -1,0 +1,142 @@
+"""ctypes binding of libfamehip.so (the C ABI of include/fame_hip.h).
+
+The library must have been built in-tree (python -m fame_amd.build); a missing
+library is an error, never a silent fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfamehip.so")
+
+FAME_OK = 0
+ERRORS = {-1: "FAME_ERR_IO", -2: "FAME_ERR_FORMAT", -3: "FAME_ERR_HIP", -4: "FAME_ERR_ARG",
+          -5: "FAME_ERR_CAPACITY"}
+INFO_FIELDS = ("htab", "readlen", "winlen", "kmerlen", "kmercutoff", "seedbits", "ncpg", "nchrom",
+               "nkmer", "nwin", "nfiltered")
+MAX_READ = 255
+
+# every entry point declared in include/fame_hip.h
+EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
+           "fame_match_reads_device", "fame_decide_strand", "fame_get_meth", "fame_reset",
+           "fame_print_methylation_levels", "fame_last_timings")
+
+_u8p = C.POINTER(C.c_uint8)
+_u16p = C.POINTER(C.c_uint16)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_dbl = C.POINTER(C.c_double)
+
+_lib = None
+
+
+class FameError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FameError(f"{LIB_PATH} is not built; run `python -m fame_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.fame_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.fame_close.argtypes = [C.c_void_p]
+        L.fame_last_error.restype = C.c_char_p
+        L.fame_last_error.argtypes = [C.c_void_p]
+        L.fame_index_info.argtypes = [C.c_void_p, _u64p]
+        L.fame_match_reads.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, C.c_uint32, C.c_int, _u64p,
+                                       _u8p, _u64p]
+        L.fame_match_reads_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_uint32, C.c_int, _u64p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
+        L.fame_decide_strand.argtypes = [C.c_void_p, _u64p, _u64p, C.POINTER(C.c_int)]
+        L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
+        L.fame_reset.argtypes = [C.c_void_p]
+        L.fame_print_methylation_levels.argtypes = [C.c_void_p, C.c_char_p]
+        L.fame_last_timings.argtypes = [C.c_void_p, _dbl]
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    """An index resident in HBM plus the matching state of one ReadQueue."""
+
+    def __init__(self, index_path, both_strands=False):
+        L = load()
+        h = C.c_void_p()
+        rc = L.fame_open(index_path.encode(), int(both_strands), C.byref(h))
+        self.h = h.value
+        if rc != FAME_OK:
+            msg = L.fame_last_error(self.h).decode() if self.h else "fame_open failed"
+            self.close()
+            raise FameError(f"{ERRORS.get(rc, rc)}: {msg}")
+        info = np.zeros(11, np.uint64)
+        self._check(L.fame_index_info(self.h, ptr(info, _u64p)))
+        self.info = dict(zip(INFO_FIELDS, (int(x) for x in info)))
+
+    def _check(self, rc):
+        if rc != FAME_OK:
+            msg = load().fame_last_error(self.h).decode()
+            raise FameError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+    def match(self, seqs, offs, lens, get_stranded, counters):
+        n = int(len(lens))
+        valid = np.zeros(max(n, 1), np.uint8)
+        mat = np.zeros(max(n, 1), np.uint64)
+        seqs = np.ascontiguousarray(seqs, np.uint8)
+        if seqs.size == 0:
+            seqs = np.zeros(1, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.uint32)
+        self._check(load().fame_match_reads(self.h, ptr(seqs, _u8p), ptr(offs, _u64p),
+                                            ptr(lens, _u32p), n, int(get_stranded),
+                                            ptr(counters, _u64p), ptr(valid, _u8p), ptr(mat, _u64p)))
+        return valid[:n], mat[:n]
+
+    def match_device(self, d_seqs, d_offs, d_lens, n, get_stranded, counters, d_valid=0, d_mat=0,
+                     stream=0):
+        self._check(load().fame_match_reads_device(self.h, d_seqs, d_offs, d_lens, int(n),
+                                                   int(get_stranded), ptr(counters, _u64p),
+                                                   d_valid or None, d_mat or None, stream or None))
+
+    def decide_strand(self):
+        f = C.c_uint64()
+        r = C.c_uint64()
+        m = C.c_int()
+        self._check(load().fame_decide_strand(self.h, C.byref(f), C.byref(r), C.byref(m)))
+        return bool(m.value), int(f.value), int(r.value)
+
+    def meth(self):
+        out = np.zeros((self.info["ncpg"], 4), np.uint16)
+        if self.info["ncpg"]:
+            self._check(load().fame_get_meth(self.h, ptr(out, _u16p), self.info["ncpg"]))
+        return out
+
+    def reset(self):
+        self._check(load().fame_reset(self.h))
+
+    def print_methylation_levels(self, basename):
+        self._check(load().fame_print_methylation_levels(self.h, basename.encode()))
+
+    def timings(self):
+        t = np.zeros(6, np.float64)
+        self._check(load().fame_last_timings(self.h, ptr(t, _dbl)))
+        return dict(zip(("prep", "seed", "verify", "select", "meth", "total"), t.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fame_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

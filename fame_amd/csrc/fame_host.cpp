@@ -1,0 +1,735 @@
+// Host side of libfamehip.so: index loading into HBM, per-batch
+// orchestration of the kernels, and the C ABI declared in include/fame_hip.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fame_hip.h"
+#include "fame_internal.h"
+
+using namespace fame;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // grows (never shrinks); contents are not preserved
+    hipError_t reserve(size_t want)
+    {
+        if (want <= bytes && p) return hipSuccess;
+        release();
+        size_t b = want ? want : 1;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPC(call)                                                                                           \
+    do {                                                                                                     \
+        hipError_t _e = (call);                                                                              \
+        if (_e != hipSuccess) throw HipError{_e, #call};                                                     \
+    } while (0)
+
+struct FameError {
+    int code;
+    std::string msg;
+};
+
+} // namespace
+
+struct fame_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool both = false;
+    bool match_r1_fwd = true;
+    // header
+    uint64_t htab = 0, kmercutoff = 0, ncpg = 0, nchrom = 0, nkmer = 0, nwin = 0, nfilt = 0;
+    uint32_t readlen = 0, winlen = 0, kmerlen = 0, seedbits = 0;
+    // host copies needed for the report
+    std::vector<uint32_t> cpg_pos;
+    std::vector<uint8_t> cpg_chrom;
+    std::vector<std::string> chr_name; // 256 entries, "" when unmapped
+    // device index
+    DevBuf d_tab, d_kent, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
+    DevBuf d_meth, d_counters, d_flags;
+    DevIndex ix{};
+    // batch buffers
+    DevBuf b_seqs, b_offs, b_lens, b_rev, b_pre, b_cand_off, b_cand_nf, b_cand_nr, b_pool_n;
+    DevBuf b_job_meta[FAME_NPOOL], b_job_ro[FAME_NPOOL], b_job_nm[FAME_NPOOL], b_job_match[FAME_NPOOL],
+        b_job_ovf_off[FAME_NPOOL];
+    DevBuf b_ovf_n, b_ovf_pool, b_ovf_job, b_ovf_buf, b_ovf_total;
+    DevBuf b_heavy_n, b_heavy_ro, b_heavy_e, b_heavy_bc, b_heavy_scratch;
+    DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_trace;
+    uint32_t pool_cap[FAME_NPOOL] = {0, 0, 0, 0};
+    uint32_t match_cap = 4;
+    uint32_t meth_threads = 65536;
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t last_heavy = 0, last_ovf = 0;
+};
+
+namespace {
+
+void set_err(fame_ctx* c, const std::string& m)
+{
+    if (c) c->err = m;
+}
+
+// ---------------------------------------------------------------------------
+// index loading (RefGenome::load RefGenome.cpp:230-387, read_strands :414-433)
+// ---------------------------------------------------------------------------
+
+struct File {
+    FILE* f = nullptr;
+    explicit File(const std::string& p) { f = fopen(p.c_str(), "rb"); }
+    ~File()
+    {
+        if (f) fclose(f);
+    }
+    void read(void* p, size_t n)
+    {
+        if (n && fread(p, 1, n, f) != n) throw FameError{FAME_ERR_IO, "truncated index file"};
+    }
+    template <typename T>
+    T get()
+    {
+        T v;
+        read(&v, sizeof v);
+        return v;
+    }
+    void skip(uint64_t n)
+    {
+        if (n && fseeko(f, (off_t)n, SEEK_CUR) != 0) throw FameError{FAME_ERR_IO, "truncated index file"};
+    }
+};
+
+constexpr size_t kStage = 256ull << 20; // host staging chunk for large tables
+
+void load_index(fame_ctx* c, const std::string& path)
+{
+    File f(path);
+    if (!f.f) throw FameError{FAME_ERR_IO, "cannot open index " + path};
+    c->htab = f.get<uint64_t>();
+    c->readlen = f.get<uint32_t>();
+    c->winlen = f.get<uint32_t>();
+    c->kmerlen = f.get<uint32_t>();
+    c->kmercutoff = f.get<uint64_t>();
+    c->seedbits = f.get<uint32_t>();
+    if (c->kmerlen != 32) throw FameError{FAME_ERR_FORMAT, "index k-mer length is not 32"};
+    if (c->htab == 0) throw FameError{FAME_ERR_FORMAT, "index hash table size is 0"};
+    if (c->readlen <= 32 || c->readlen > FAME_MAX_READ)
+        throw FameError{FAME_ERR_FORMAT, "index READLEN outside (32, 255]"};
+    hipStream_t st = c->stream;
+    // CpG table: {u8 chrom; 3 pad; u32 pos}
+    c->ncpg = f.get<uint64_t>();
+    {
+        std::vector<uint8_t> raw(c->ncpg * 8);
+        f.read(raw.data(), raw.size());
+        c->cpg_pos.resize(c->ncpg);
+        c->cpg_chrom.resize(c->ncpg);
+        for (uint64_t i = 0; i < c->ncpg; ++i) {
+            c->cpg_chrom[i] = raw[8 * i];
+            memcpy(&c->cpg_pos[i], &raw[8 * i + 4], 4);
+        }
+        HIPC(c->d_cpg_pos.reserve(4 * c->ncpg + 4));
+        HIPC(c->d_cpg_chrom.reserve(c->ncpg + 1));
+        HIPC(hipMemcpy(c->d_cpg_pos.p, c->cpg_pos.data(), 4 * c->ncpg, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(c->d_cpg_chrom.p, c->cpg_chrom.data(), c->ncpg, hipMemcpyHostToDevice));
+    }
+    f.skip(8 * f.get<uint64_t>()); // cpgStartTable (only used by dead code)
+    // chromosomes
+    c->nchrom = f.get<uint64_t>();
+    if (c->nchrom > 256) throw FameError{FAME_ERR_FORMAT, "more than 256 chromosomes"};
+    std::vector<uint64_t> coff(c->nchrom), clen(c->nchrom);
+    {
+        long pos0 = ftello(f.f);
+        uint64_t total = 0;
+        for (uint64_t i = 0; i < c->nchrom; ++i) {
+            clen[i] = f.get<uint64_t>();
+            coff[i] = total;
+            total += clen[i];
+            f.skip(clen[i]);
+        }
+        HIPC(c->d_seq.reserve(total + 64));
+        HIPC(hipMemset(c->d_seq.p, 0, total + 64));
+        fseeko(f.f, pos0, SEEK_SET);
+        std::vector<uint8_t> buf;
+        for (uint64_t i = 0; i < c->nchrom; ++i) {
+            f.get<uint64_t>();
+            uint64_t left = clen[i], o = coff[i];
+            while (left) {
+                size_t n = (size_t)std::min<uint64_t>(left, kStage);
+                buf.resize(n);
+                f.read(buf.data(), n);
+                HIPC(hipMemcpy(c->d_seq.as<uint8_t>() + o, buf.data(), n, hipMemcpyHostToDevice));
+                o += n;
+                left -= n;
+            }
+        }
+        HIPC(c->d_coff.reserve(8 * c->nchrom + 8));
+        HIPC(c->d_clen.reserve(8 * c->nchrom + 8));
+        HIPC(hipMemcpy(c->d_coff.p, coff.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(c->d_clen.p, clen.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
+    }
+    // tabIndex: u64[htab + 1], narrowed to u32 when every prefix fits
+    const uint64_t tablen = f.get<uint64_t>();
+    if (tablen != c->htab + 1) throw FameError{FAME_ERR_FORMAT, "tabIndex length != HTABSIZE + 1"};
+    DevBuf tab64;
+    HIPC(tab64.reserve(8 * tablen));
+    {
+        std::vector<uint8_t> buf;
+        uint64_t left = 8 * tablen, o = 0;
+        while (left) {
+            size_t n = (size_t)std::min<uint64_t>(left, kStage);
+            buf.resize(n);
+            f.read(buf.data(), n);
+            HIPC(hipMemcpy(tab64.as<uint8_t>() + o, buf.data(), n, hipMemcpyHostToDevice));
+            o += n;
+            left -= n;
+        }
+    }
+    c->nkmer = f.get<uint64_t>();
+    uint64_t last = 0;
+    HIPC(hipMemcpy(&last, tab64.as<uint64_t>() + c->htab, 8, hipMemcpyDeviceToHost));
+    if (last > c->nkmer) throw FameError{FAME_ERR_FORMAT, "tabIndex points past the k-mer table"};
+    if (c->nkmer < (1ull << 32)) {
+        HIPC(c->d_tab.reserve(4 * tablen));
+        launch_narrow_tab(tab64.as<uint64_t>(), tablen, c->d_tab.as<uint32_t>(), st);
+        HIPC(hipStreamSynchronize(st));
+        tab64.release();
+        c->ix.tab_is64 = 0;
+    } else {
+        std::swap(c->d_tab.p, tab64.p);
+        std::swap(c->d_tab.bytes, tab64.bytes);
+        c->ix.tab_is64 = 1;
+    }
+    // strands companion file, then the k-mer table packed with it
+    DevBuf sbits;
+    {
+        File g(path + "_strands");
+        if (!g.f) throw FameError{FAME_ERR_IO, "missing strand file " + path + "_strands"};
+        uint64_t n = g.get<uint64_t>();
+        if (n != c->nkmer) throw FameError{FAME_ERR_FORMAT, "strand file size differs from k-mer table"};
+        std::vector<uint8_t> bits((n + 7) / 8);
+        g.read(bits.data(), bits.size());
+        HIPC(sbits.reserve(bits.size() + 1));
+        HIPC(hipMemcpy(sbits.p, bits.data(), bits.size(), hipMemcpyHostToDevice));
+    }
+    HIPC(c->d_kent.reserve(4 * c->nkmer + 4));
+    {
+        DevBuf stage;
+        HIPC(stage.reserve(kStage));
+        std::vector<uint8_t> buf;
+        const uint64_t per = kStage / 8;
+        for (uint64_t first = 0; first < c->nkmer; first += per) {
+            uint64_t n = std::min<uint64_t>(per, c->nkmer - first);
+            buf.resize(8 * n);
+            f.read(buf.data(), 8 * n);
+            HIPC(hipMemcpy(stage.p, buf.data(), 8 * n, hipMemcpyHostToDevice));
+            launch_pack_kmers(stage.as<uint32_t>(), sbits.as<uint8_t>(), first, n, c->d_kent.as<uint32_t>(), st);
+            HIPC(hipStreamSynchronize(st));
+        }
+    }
+    f.skip(8 * f.get<uint64_t>()); // metaCpGs (empty in current builds)
+    f.skip(8 * f.get<uint64_t>()); // metaStartCpGs
+    // windows {u32 startPos; u8 chrom; 3 pad; u32 startInd; u32 endInd}
+    c->nwin = f.get<uint64_t>();
+    {
+        std::vector<uint32_t> raw(4 * c->nwin);
+        f.read(raw.data(), 16 * c->nwin);
+        for (uint64_t i = 0; i < c->nwin; ++i) {
+            raw[4 * i + 1] &= 0xFFu;
+            if (raw[4 * i + 1] >= c->nchrom) throw FameError{FAME_ERR_FORMAT, "window on unknown chromosome"};
+        }
+        HIPC(c->d_win.reserve(16 * c->nwin + 16));
+        HIPC(hipMemcpy(c->d_win.p, raw.data(), 16 * c->nwin, hipMemcpyHostToDevice));
+    }
+    c->nfilt = f.get<uint64_t>();
+    f.skip(8 * c->nfilt); // filteredKmers (paired-end filter only)
+    // chrMap
+    c->chr_name.assign(256, std::string());
+    uint64_t nmap = 0;
+    if (fread(&nmap, 8, 1, f.f) == 1) {
+        for (uint64_t i = 0; i < nmap; ++i) {
+            uint8_t id = f.get<uint8_t>();
+            uint64_t sl = f.get<uint64_t>();
+            std::string s(sl, '\0');
+            f.read(&s[0], sl);
+            if (c->chr_name[id].empty()) c->chr_name[id] = s; // unordered_map::insert keeps the first
+        }
+    }
+    // methylation accumulators and counters
+    HIPC(c->d_meth.reserve(16 * c->ncpg + 16));
+    HIPC(hipMemset(c->d_meth.p, 0, 16 * c->ncpg + 16));
+    HIPC(c->d_counters.reserve(8 * 8));
+    HIPC(hipMemset(c->d_counters.p, 0, 64));
+    HIPC(c->d_flags.reserve(16));
+    // device index view
+    DevIndex& ix = c->ix;
+    ix.tab = c->d_tab.p;
+    ix.htab = c->htab;
+    ix.hpow2 = (c->htab & (c->htab - 1)) == 0;
+    ix.hmask = c->htab - 1;
+    ix.kent = c->d_kent.as<uint32_t>();
+    ix.win = c->d_win.as<uint4>();
+    ix.seq = c->d_seq.as<uint8_t>();
+    ix.coff = c->d_coff.as<uint64_t>();
+    ix.clen = c->d_clen.as<uint64_t>();
+    ix.cpg_pos = c->d_cpg_pos.as<uint32_t>();
+    ix.cpg_chrom = c->d_cpg_chrom.as<uint8_t>();
+    ix.readlen = c->readlen;
+    ix.winlen = c->winlen;
+    ix.seedbits = c->seedbits;
+}
+
+// ---------------------------------------------------------------------------
+// one batch
+// ---------------------------------------------------------------------------
+
+template <typename T>
+T* grow(DevBuf& b, size_t count)
+{
+    HIPC(b.reserve(sizeof(T) * count));
+    return b.as<T>();
+}
+
+struct Timer {
+    hipEvent_t e[7];
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s)
+    {
+        for (auto& x : e) (void)hipEventCreate(&x);
+    }
+    ~Timer()
+    {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    }
+    void mark(int i) { (void)hipEventRecord(e[i], st); }
+    float ms(int a, int b)
+    {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e[a], e[b]);
+        return t;
+    }
+};
+
+int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+              uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Timer tm(st);
+    BatchDev bd{};
+    bd.seqs = d_seqs;
+    bd.offs = d_offs;
+    bd.lens = d_lens;
+    bd.rev = grow<uint8_t>(c->b_rev, total_bytes + 64);
+    bd.pre = grow<uint8_t>(c->b_pre, n + 1);
+    const bool o0 = c->both || get_stranded || c->match_r1_fwd;
+    const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
+    bd.ori_mask = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
+    const uint32_t n_ro = 2 * n;
+    bd.cand_off = grow<uint32_t>(c->b_cand_off, n_ro + 1);
+    bd.cand_nf = grow<uint32_t>(c->b_cand_nf, n_ro + 1);
+    bd.cand_nr = grow<uint32_t>(c->b_cand_nr, n_ro + 1);
+    bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
+    bd.match_cap = c->match_cap;
+    const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n);
+    for (int p = 0; p < FAME_NPOOL; ++p)
+        if (c->pool_cap[p] < want_cap) c->pool_cap[p] = want_cap;
+    bd.ovf_n = grow<uint32_t>(c->b_ovf_n, 1);
+    bd.ovf_list_cap = std::max<uint32_t>(1024, n / 4);
+    bd.ovf_pool = grow<uint32_t>(c->b_ovf_pool, bd.ovf_list_cap);
+    bd.ovf_job = grow<uint32_t>(c->b_ovf_job, bd.ovf_list_cap);
+    bd.heavy_n = grow<uint32_t>(c->b_heavy_n, 1);
+    bd.heavy_cap = std::max<uint32_t>(1024, n_ro);
+    bd.heavy_ro = grow<uint32_t>(c->b_heavy_ro, bd.heavy_cap);
+    bd.heavy_e = grow<uint32_t>(c->b_heavy_e, bd.heavy_cap);
+    bd.heavy_bc = grow<uint32_t>(c->b_heavy_bc, bd.heavy_cap);
+    bd.status = grow<uint8_t>(c->b_status, n + 1);
+    bd.use = grow<uint8_t>(c->b_use, n + 1);
+    bd.mat = grow<uint64_t>(c->b_mat, n + 1);
+    bd.valid = grow<uint8_t>(c->b_valid, n + 1);
+    bd.counters = c->d_counters.as<uint64_t>();
+    bd.meth = c->d_meth.as<uint32_t>();
+    bd.flags = c->d_flags.as<uint32_t>();
+
+    tm.mark(0);
+    HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    launch_prep(bd, n, c->readlen, st);
+    tm.mark(1);
+    // seed: retried with larger pools when a pool overflowed
+    uint32_t pool_n[FAME_NPOOL];
+    uint32_t heavy_n = 0, flags = 0;
+    for (int attempt = 0;; ++attempt) {
+        for (int p = 0; p < FAME_NPOOL; ++p) {
+            bd.pool_cap[p] = c->pool_cap[p];
+            bd.job_meta[p] = grow<uint32_t>(c->b_job_meta[p], c->pool_cap[p]);
+            bd.job_ro[p] = grow<uint32_t>(c->b_job_ro[p], c->pool_cap[p]);
+        }
+        HIPC(hipMemsetAsync(bd.pool_n, 0, 4 * FAME_NPOOL, st));
+        HIPC(hipMemsetAsync(bd.heavy_n, 0, 4, st));
+        launch_seed_light(c->ix, bd, n_ro, st);
+        HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&heavy_n, bd.heavy_n, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (flags & FLAG_TOO_LONG) {
+            set_err(c, "read longer than FAME_MAX_READ");
+            return FAME_ERR_ARG;
+        }
+        if (flags & FLAG_HEAVY_FULL) {
+            set_err(c, "heavy list overflow");
+            return FAME_ERR_CAPACITY;
+        }
+        // heavy (read, orientation) pairs: global-memory scratch, in slices
+        if (heavy_n) {
+            std::vector<uint32_t> he(heavy_n), hb(heavy_n);
+            HIPC(hipMemcpy(he.data(), bd.heavy_e, 4 * heavy_n, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(hb.data(), bd.heavy_bc, 4 * heavy_n, hipMemcpyDeviceToHost));
+            uint64_t emax = 1, bcmax = 1;
+            for (uint32_t i = 0; i < heavy_n; ++i) {
+                emax = std::max<uint64_t>(emax, he[i]);
+                bcmax = std::max<uint64_t>(bcmax, hb[i]);
+            }
+            if (emax > (1ull << 26)) {
+                set_err(c, "a read has more than 2^26 index hits");
+                return FAME_ERR_CAPACITY;
+            }
+            auto rup = [](uint64_t v) {
+                uint64_t p = 1;
+                while (p < v) p <<= 1;
+                return p;
+            };
+            HeavyScratch hs{};
+            hs.cap_e = (uint32_t)emax;
+            hs.hsize = (uint32_t)rup(2 * emax);
+            uint64_t tbb = std::max(rup(bcmax), rup(2 * emax));
+            hs.tb_cap = (uint32_t)(2 * tbb + 64);
+            size_t stride = 0;
+            auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+            add(8 * FAME_DEV_MAX_READ);
+            add(4 * FAME_DEV_MAX_READ);
+            add(4 * FAME_DEV_MAX_READ);
+            add(4 * emax);
+            add(emax);
+            add(4 * emax);
+            for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
+            for (int i = 0; i < 4; ++i) add(4 * emax);
+            for (int i = 0; i < 2; ++i) add(4ull * hs.tb_cap);
+            for (int i = 0; i < 4; ++i) add(4 * emax);
+            hs.stride = stride;
+            const size_t budget = 2ull << 30;
+            uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(heavy_n, budget / stride));
+            HIPC(c->b_heavy_scratch.reserve(stride * per));
+            hs.base = c->b_heavy_scratch.as<uint8_t>();
+            for (uint32_t first = 0; first < heavy_n; first += per) {
+                hs.first = first;
+                hs.count = std::min(per, heavy_n - first);
+                launch_seed_heavy(c->ix, bd, hs, st);
+            }
+            HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            if (flags & FLAG_CAPACITY) {
+                set_err(c, "a read's candidate map exceeded the device emulation limits");
+                return FAME_ERR_CAPACITY;
+            }
+        }
+        if (!(flags & FLAG_POOL_FULL)) break;
+        if (attempt > 3) {
+            set_err(c, "candidate pools keep overflowing");
+            return FAME_ERR_CAPACITY;
+        }
+        for (int p = 0; p < FAME_NPOOL; ++p)
+            c->pool_cap[p] = std::max<uint32_t>(c->pool_cap[p], pool_n[p] + (pool_n[p] >> 2) + 1024);
+        HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    }
+    c->last_heavy = heavy_n;
+    tm.mark(2);
+    // verify
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        const uint32_t cap = std::max<uint32_t>(pool_n[p], 1);
+        bd.job_nm[p] = grow<uint32_t>(c->b_job_nm[p], cap);
+        bd.job_match[p] = grow<uint32_t>(c->b_job_match[p], (size_t)cap * c->match_cap);
+        bd.job_ovf_off[p] = grow<uint32_t>(c->b_job_ovf_off[p], cap);
+    }
+    bd.ovf_buf = grow<uint32_t>(c->b_ovf_buf, 1);
+    HIPC(hipMemsetAsync(bd.ovf_n, 0, 4, st));
+    for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, pool_n[p], nullptr, st);
+    uint32_t ovf_n = 0;
+    HIPC(hipMemcpyAsync(&ovf_n, bd.ovf_n, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_OVF_FULL) {
+        set_err(c, "match overflow list full");
+        return FAME_ERR_CAPACITY;
+    }
+    c->last_ovf = ovf_n;
+    if (ovf_n) {
+        uint32_t* d_total = grow<uint32_t>(c->b_ovf_total, 2);
+        launch_ovf_offsets(bd, ovf_n, d_total, st);
+        uint32_t total = 0;
+        HIPC(hipMemcpyAsync(&total, d_total, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        bd.ovf_buf = grow<uint32_t>(c->b_ovf_buf, total + 1);
+        for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, ovf_n, bd.ovf_job, st);
+    }
+    tm.mark(3);
+    launch_select(c->ix, bd, n, get_stranded, st);
+    tm.mark(4);
+    {
+        const size_t cells = (size_t)(FAME_DEV_MAX_READ + 1) * (2 * (FAME_DPBAND + 1) + 1);
+        uint16_t* dp = grow<uint16_t>(c->b_dp, cells * c->meth_threads);
+        uint8_t* tr = grow<uint8_t>(c->b_trace, (size_t)FAME_DEV_TRACE_CAP * c->meth_threads);
+        launch_meth(c->ix, bd, n, dp, c->meth_threads, tr, c->meth_threads, st);
+    }
+    tm.mark(5);
+    uint64_t cnt[5];
+    HIPC(hipMemcpyAsync(cnt, bd.counters, 40, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_CAPACITY) {
+        set_err(c, "alignment trace exceeded its capacity");
+        return FAME_ERR_CAPACITY;
+    }
+    // per-call counters: the device accumulates; report the increment
+    counters[0] += cnt[0];
+    counters[1] += cnt[1];
+    counters[2] += cnt[2];
+    HIPC(hipMemsetAsync(bd.counters, 0, 24, st));
+    HIPC(hipStreamSynchronize(st));
+    c->timings[0] = tm.ms(0, 1);
+    c->timings[1] = tm.ms(1, 2);
+    c->timings[2] = tm.ms(2, 3);
+    c->timings[3] = tm.ms(3, 4);
+    c->timings[4] = tm.ms(4, 5);
+    c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FAME_OK;
+}
+
+template <typename F>
+int guarded(fame_ctx* c, F&& f)
+{
+    try {
+        return f();
+    } catch (const HipError& e) {
+        set_err(c, std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.e));
+        return FAME_ERR_HIP;
+    } catch (const FameError& e) {
+        set_err(c, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(c, "host out of memory");
+        return FAME_ERR_CAPACITY;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int fame_open(const char* index_path, int both_strands, fame_ctx** out)
+{
+    if (!out || !index_path) return FAME_ERR_ARG;
+    *out = nullptr;
+    auto c = std::make_unique<fame_ctx>();
+    c->both = both_strands != 0;
+    int rc = guarded(c.get(), [&]() {
+        HIPC(hipGetDevice(&c->device));
+        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        load_index(c.get(), index_path);
+        return FAME_OK;
+    });
+    if (rc != FAME_OK) {
+        // keep the context so that fame_last_error can report
+        *out = c.release();
+        return rc;
+    }
+    *out = c.release();
+    return FAME_OK;
+}
+
+int fame_close(fame_ctx* c)
+{
+    if (!c) return FAME_OK;
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FAME_OK;
+}
+
+const char* fame_last_error(const fame_ctx* c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+int fame_index_info(const fame_ctx* c, uint64_t out[FAME_INFO_FIELDS])
+{
+    if (!c || !out) return FAME_ERR_ARG;
+    const uint64_t v[FAME_INFO_FIELDS] = {c->htab, c->readlen, c->winlen, c->kmerlen, c->kmercutoff, c->seedbits,
+                                          c->ncpg, c->nchrom,  c->nkmer,  c->nwin,    c->nfilt};
+    memcpy(out, v, sizeof v);
+    return FAME_OK;
+}
+
+int fame_match_reads(fame_ctx* c, const uint8_t* seqs, const uint64_t* offs, const uint32_t* lens, uint32_t n,
+                     int get_stranded, uint64_t counters[3], uint8_t* valid, uint64_t* mat)
+{
+    if (!c || !counters || (n && (!seqs || !offs || !lens))) return FAME_ERR_ARG;
+    if (!c->d_tab.p) {
+        set_err(c, "context has no index");
+        return FAME_ERR_ARG;
+    }
+    return guarded(c, [&]() {
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (lens[i] > FAME_MAX_READ) {
+                set_err(c, "read " + std::to_string(i) + " is longer than FAME_MAX_READ");
+                return (int)FAME_ERR_ARG;
+            }
+            total = std::max<uint64_t>(total, offs[i] + lens[i]);
+        }
+        hipStream_t st = c->stream;
+        uint8_t* d_s = grow<uint8_t>(c->b_seqs, total + 64);
+        uint64_t* d_o = grow<uint64_t>(c->b_offs, n + 1);
+        uint32_t* d_l = grow<uint32_t>(c->b_lens, n + 1);
+        if (total) HIPC(hipMemcpyAsync(d_s, seqs, total, hipMemcpyHostToDevice, st));
+        if (n) {
+            HIPC(hipMemcpyAsync(d_o, offs, 8ull * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_l, lens, 4ull * n, hipMemcpyHostToDevice, st));
+        }
+        int rc = run_batch(c, d_s, d_o, d_l, n, total, get_stranded, counters, st);
+        if (rc != FAME_OK) return rc;
+        if (n && valid) HIPC(hipMemcpyAsync(valid, c->b_valid.p, n, hipMemcpyDeviceToHost, st));
+        if (n && mat) HIPC(hipMemcpyAsync(mat, c->b_mat.p, 8ull * n, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_match_reads_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens,
+                            uint32_t n, int get_stranded, uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat,
+                            void* stream)
+{
+    if (!c || !counters || (n && (!d_seqs || !d_offs || !d_lens))) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        uint64_t total = 0;
+        if (n) {
+            // the reverse complements use the same offsets: size by the
+            // furthest read end
+            unsigned long long* d_end = grow<unsigned long long>(c->b_ovf_total, 1);
+            HIPC(hipMemsetAsync(d_end, 0, 8, st));
+            launch_max_end(d_offs, d_lens, n, d_end, st);
+            unsigned long long e = 0;
+            HIPC(hipMemcpyAsync(&e, d_end, 8, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            total = e;
+        }
+        int rc = run_batch(c, d_seqs, d_offs, d_lens, n, total, get_stranded, counters, st);
+        if (rc != FAME_OK) return rc;
+        if (n && d_valid) HIPC(hipMemcpyAsync(d_valid, c->b_valid.p, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat) HIPC(hipMemcpyAsync(d_mat, c->b_mat.p, 8ull * n, hipMemcpyDeviceToDevice, st));
+        HIPC(hipStreamSynchronize(st));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_decide_strand(fame_ctx* c, uint64_t* r1_fwd, uint64_t* r1_rev, int* match_r1_fwd)
+{
+    if (!c) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        uint64_t t[2];
+        HIPC(hipMemcpy(t, c->d_counters.as<uint64_t>() + 3, 16, hipMemcpyDeviceToHost));
+        c->match_r1_fwd = t[0] > t[1];
+        if (r1_fwd) *r1_fwd = t[0];
+        if (r1_rev) *r1_rev = t[1];
+        if (match_r1_fwd) *match_r1_fwd = c->match_r1_fwd ? 1 : 0;
+        return (int)FAME_OK;
+    });
+}
+
+int fame_get_meth(fame_ctx* c, uint16_t* out, uint64_t n_cpg)
+{
+    if (!c || !out || n_cpg != c->ncpg) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        DevBuf tmp;
+        HIPC(tmp.reserve(8 * c->ncpg + 8));
+        launch_meth_u16(c->d_meth.as<uint32_t>(), 4 * c->ncpg, tmp.as<uint16_t>(), c->stream);
+        HIPC(hipMemcpyAsync(out, tmp.p, 8 * c->ncpg, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_reset(fame_ctx* c)
+{
+    if (!c) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        HIPC(hipMemset(c->d_meth.p, 0, 16 * c->ncpg + 16));
+        HIPC(hipMemset(c->d_counters.p, 0, 64));
+        c->match_r1_fwd = true;
+        return (int)FAME_OK;
+    });
+}
+
+int fame_print_methylation_levels(fame_ctx* c, const char* basename)
+{
+    if (!c || !basename) return FAME_ERR_ARG;
+    std::vector<uint16_t> m(4 * c->ncpg + 4);
+    int rc = fame_get_meth(c, m.data(), c->ncpg);
+    if (rc != FAME_OK) return rc;
+    std::string path = std::string(basename) + "_cpg.tsv";
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) {
+        set_err(c, "cannot write " + path);
+        return FAME_ERR_IO;
+    }
+    std::vector<char> buf(1 << 20);
+    setvbuf(f, buf.data(), _IOFBF, buf.size());
+    for (uint64_t i = 0; i < c->ncpg; ++i) {
+        const std::string& nm = c->chr_name[c->cpg_chrom[i]];
+        fprintf(f, "%s\t%u\t%u\t%u\t%u\t%u\n", nm.c_str(), (unsigned)(c->cpg_pos[i] + c->readlen - 2),
+                (unsigned)m[4 * i], (unsigned)m[4 * i + 1], (unsigned)m[4 * i + 2], (unsigned)m[4 * i + 3]);
+    }
+    fclose(f);
+    return FAME_OK;
+}
+
+int fame_last_timings(const fame_ctx* c, double out[6])
+{
+    if (!c || !out) return FAME_ERR_ARG;
+    memcpy(out, c->timings, sizeof c->timings);
+    return FAME_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,111 @@
+// Internal device-side data structures shared by fame_device.hip and
+// fame_host.cpp.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define FAME_DEV_MAX_READ 255
+#define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
+#define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
+#define FAME_DPBAND 8 // 2*MISCOUNT + ADDMIS (ReadQueue.cpp:3640)
+
+namespace fame {
+
+enum : uint32_t {
+    FLAG_TOO_LONG = 1u,
+    FLAG_POOL_FULL = 2u,
+    FLAG_HEAVY_FULL = 4u,
+    FLAG_CAPACITY = 8u,
+    FLAG_OVF_FULL = 16u,
+};
+
+// The index as resident in HBM.
+//   tab      tabIndex, u32 when every prefix fits (nkmer < 2^32), else u64
+//   kent     one u32 per k-mer table entry: metaId << 1 | forward-strand bit
+//            (KMER_S::getMetaCpG + strandTable, structs.h:118-151)
+//   win      metaWindow as {startPos, chrom, startInd, endInd}
+//   seq      all chromosomes back to back, coff / clen per chromosome
+//   cpg_pos  cpgTable[i].pos, cpg_chrom cpgTable[i].chrom
+struct DevIndex {
+    const void* tab;
+    int tab_is64;
+    uint64_t htab, hmask;
+    int hpow2;
+    const uint32_t* kent;
+    const uint4* win;
+    const uint8_t* seq;
+    const uint64_t* coff;
+    const uint64_t* clen;
+    const uint32_t* cpg_pos;
+    const uint8_t* cpg_chrom;
+    uint32_t readlen, winlen, seedbits;
+};
+
+// Per-batch device buffers.  A (read, orientation) pair is ro = 2*read + ori
+// (ori 0 = the read as given, 1 = its reverse complement).
+struct BatchDev {
+    const uint8_t* seqs;
+    const uint64_t* offs;
+    const uint32_t* lens;
+    uint8_t* rev;
+    uint8_t* pre;      // 0 = to match, 3 = invalid (short, N, too long)
+    uint32_t ori_mask; // bit0: query ori 0, bit1: query ori 1
+    // candidate windows per ro, in hopscotch iteration order (fwd map then rev map)
+    uint32_t* cand_off;
+    uint32_t* cand_nf;
+    uint32_t* cand_nr;
+    // verify job pools, by 64-bit words of the pattern
+    uint32_t* pool_n; // [FAME_NPOOL] device counters
+    uint32_t pool_cap[FAME_NPOOL];
+    uint32_t* job_meta[FAME_NPOOL]; // metaId | rev-map << 31
+    uint32_t* job_ro[FAME_NPOOL];
+    uint32_t* job_nm[FAME_NPOOL];    // matches found
+    uint32_t* job_match[FAME_NPOOL]; // first match_cap matches: pos << 8 | err
+    uint32_t* job_ovf_off[FAME_NPOOL];
+    uint32_t match_cap;
+    uint32_t* ovf_n;
+    uint32_t* ovf_pool;
+    uint32_t* ovf_job;
+    uint32_t ovf_list_cap;
+    uint32_t* ovf_buf;
+    // heavy (read, orientation) pairs deferred from the LDS seed kernel
+    uint32_t* heavy_n;
+    uint32_t* heavy_ro;
+    uint32_t* heavy_e;
+    uint32_t* heavy_bc;
+    uint32_t heavy_cap;
+    // results
+    uint8_t* status; // 0 matched, 1 nonunique, 2 unmatched, 3 invalid
+    uint8_t* use;    // 1 = r.seq, 2 = revSeq carries the methylation call
+    uint64_t* mat;
+    uint8_t* valid;
+    uint64_t* counters; // succ, nonunique, unsucc, r1fwd, r1rev (accumulated)
+    uint32_t* meth;     // 4 x u32 per CpG
+    uint32_t* flags;
+};
+
+struct HeavyScratch {
+    uint8_t* base;
+    size_t stride;
+    uint32_t first, count;
+    uint32_t cap_e, hsize, tb_cap;
+};
+
+void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st);
+void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st);
+void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st);
+void launch_verify(const DevIndex& ix, const BatchDev& bd, uint32_t pool, uint32_t njobs,
+                   const uint32_t* only, hipStream_t st);
+void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_stranded, hipStream_t st);
+void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, size_t dp_stride,
+                 uint8_t* trace, uint32_t grid_threads, hipStream_t st);
+void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
+                       hipStream_t st);
+void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
+void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st);
+void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st);
+void launch_max_end(const uint64_t* offs, const uint32_t* lens, uint32_t n, unsigned long long* out, hipStream_t st);
+
+} // namespace fame

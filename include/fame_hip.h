@@ -1,0 +1,102 @@
+/* fame_hip.h -- C ABI of the MI355X read-matching library (libfamehip.so).
+ *
+ * The reference's matching boundary is a C++ member API (FischerJo/FAME
+ * ReadQueue.h:58-129, RefGenome.h:55).  Each entry point below replaces one
+ * of its members; the C++ mirror in fame_amd/csrc/readqueue.hpp and the
+ * Python mirror in fame_amd/readqueue.py call only these functions.
+ *
+ * Conventions: integer status codes (FAME_OK = 0, negative on error, message
+ * in fame_last_error); caller-owned host buffers; device memory owned by the
+ * context; one host thread per context.  Counters are accumulated (+=) into
+ * the caller's array like the reference's uint64_t& arguments
+ * (ReadQueue.cpp:926-931).
+ */
+#ifndef FAME_HIP_H
+#define FAME_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fame_ctx fame_ctx;
+
+enum {
+    FAME_OK = 0,
+    FAME_ERR_IO = -1,       /* index file missing / truncated */
+    FAME_ERR_FORMAT = -2,   /* header values this library cannot serve */
+    FAME_ERR_HIP = -3,      /* a HIP runtime call failed */
+    FAME_ERR_ARG = -4,      /* bad arguments (e.g. read longer than FAME_MAX_READ) */
+    FAME_ERR_CAPACITY = -5  /* a per-read structure outgrew the device limits */
+};
+
+/* longest read the device path accepts; longer reads make fame_match_reads
+ * fail with FAME_ERR_ARG (the reference has no limit) */
+#define FAME_MAX_READ 255
+
+/* index header / table sizes, in this order:
+ * htab, readlen, winlen, kmerlen, kmercutoff, seedbits, ncpg, nchrom, nkmer,
+ * nwin, nfiltered */
+#define FAME_INFO_FIELDS 11
+
+/* RefGenome(std::string filePath) RefGenome.cpp:77-80 (load :230-387 plus the
+ * <index>_strands companion :414-433) followed by the single-end ReadQueue
+ * constructor ReadQueue.cpp:25-78: loads the --load_index file, validates
+ * the header and makes the index resident in HBM on the current device.
+ * both_strands is FAME's --unord_reads flag (bsFlag).  On failure *out
+ * still receives a context that only carries the message for
+ * fame_last_error; it must be released with fame_close. */
+int fame_open(const char* index_path, int both_strands, fame_ctx** out);
+
+/* releases device and host memory */
+int fame_close(fame_ctx* ctx);
+
+/* message of the last failing call (never NULL) */
+const char* fame_last_error(const fame_ctx* ctx);
+
+int fame_index_info(const fame_ctx* ctx, uint64_t out[FAME_INFO_FIELDS]);
+
+/* ReadQueue::matchReads ReadQueue.cpp:404-933.  Reads are n byte strings
+ * seqs[offs[i] .. offs[i]+lens[i]).  counters[0..2] += succMatch,
+ * nonUniqueMatch, unSuccMatch.  valid[i] = !Read::isInvalid and mat[i] =
+ * Read::mat (0 for invalid reads).  valid / mat may be NULL. */
+int fame_match_reads(fame_ctx* ctx, const uint8_t* seqs, const uint64_t* offs,
+                     const uint32_t* lens, uint32_t n, int get_stranded, uint64_t counters[3],
+                     uint8_t* valid, uint64_t* mat);
+
+/* The same with every buffer already in device memory (HBM); d_valid /
+ * d_mat receive the per-read results on the device.  `stream` is a
+ * hipStream_t (NULL = the context's own stream); the call returns after the
+ * work completed.  No counterpart in the reference: it is the resident-input
+ * form of matchReads used by benchmarks and serving loops. */
+int fame_match_reads_device(fame_ctx* ctx, const uint8_t* d_seqs, const uint64_t* d_offs,
+                            const uint32_t* d_lens, uint32_t n, int get_stranded,
+                            uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat,
+                            void* stream);
+
+/* ReadQueue::decideStrand ReadQueue.cpp:376-400 (without the console
+ * report): matchR1Fwd = r1FwdMatches > r1RevMatches.  Any pointer may be
+ * NULL. */
+int fame_decide_strand(fame_ctx* ctx, uint64_t* r1_fwd, uint64_t* r1_rev, int* match_r1_fwd);
+
+/* methLevels (ReadQueue.h:354-371): 4 x u16 per CpG in cpgTable order
+ * (methFwd, unmethFwd, methRev, unmethRev), counts modulo 2^16 */
+int fame_get_meth(fame_ctx* ctx, uint16_t* out, uint64_t n_cpg);
+
+/* zeroes the CpG counts and strand tallies (a fresh ReadQueue) */
+int fame_reset(fame_ctx* ctx);
+
+/* ReadQueue::printMethylationLevels ReadQueue.cpp:1860-1893: writes
+ * <basename>_cpg.tsv, byte-identical to the reference's report */
+int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
+
+/* per-stage device time of the last fame_match_reads* call, milliseconds:
+ * prep, seed, verify, select, meth, and the total including host waits */
+int fame_last_timings(const fame_ctx* ctx, double out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

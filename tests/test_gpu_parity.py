@@ -1,0 +1,63 @@
+"""HIP path against the reference's own outputs (golden fixtures) and the
+oracle, through the C ABI.  Bit-exact: per-read validity and MATCH words,
+counters, strand tallies, CpG counts and the CpG report."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from fame_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def run_hip(c, chunk=None):
+    """queryRoutine main.cpp:363-398 over the case's reads."""
+    chunk = chunk or c.chunk
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    n = len(c.lens)
+    valid = np.zeros(n, np.uint8)
+    mat = np.zeros(n, np.uint64)
+    cnt = np.zeros(3, np.uint64)
+    s = 0
+    tally = None
+    if not c.unord:
+        e = min(n, chunk)
+        valid[s:e], mat[s:e] = ctx.match(c.seqs, c.offs[s:e], c.lens[s:e], True, cnt)
+        tally = ctx.decide_strand()
+        s = e
+    while s < n:
+        e = min(n, s + chunk)
+        valid[s:e], mat[s:e] = ctx.match(c.seqs, c.offs[s:e], c.lens[s:e], False, cnt)
+        s = e
+    return ctx, valid, mat, cnt, tally
+
+
+def test_hip_matches_reference(golden_case, tmp_path):
+    c = golden_case
+    print(f"case {c.name}: {len(c.lens)} reads", flush=True)
+    ctx, valid, mat, cnt, tally = run_hip(c)
+    bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
+    assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    if tally is not None:
+        assert (tally[1], tally[2], tally[0]) == (c.ref["r1_fwd"], c.ref["r1_rev"], c.ref["match_r1_fwd"])
+    assert np.array_equal(ctx.meth(), c.meth)
+    base = os.path.join(str(tmp_path), "hip")
+    ctx.print_methylation_levels(base)
+    assert open(base + "_cpg.tsv", "rb").read() == c.tsv
+    ctx.close()
+
+
+def test_hip_matches_oracle_single_batch(golden_case):
+    """All reads in one batch after a stranded first chunk: batch size does
+    not change per-read results."""
+    c = golden_case
+    ctx, valid, mat, cnt, _ = run_hip(c, chunk=max(len(c.lens) // 3, 1))
+    ix = orc.Index(c.index)
+    m, ov, om = orc.match_stream(ix, c.seqs, c.offs, c.lens, chunk=max(len(c.lens) // 3, 1),
+                                 both_strands=c.unord)
+    assert np.array_equal(valid, ov) and np.array_equal(mat, om)
+    assert np.array_equal(ctx.meth(), m.meth())
+    ctx.close()
