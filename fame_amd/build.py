@@ -9,6 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfamehip.so")
+INDEX_LIB = os.path.join(HERE, "libfameindex.so")
 SOURCES = [os.path.join(HERE, "csrc", "fame_device.hip"), os.path.join(HERE, "csrc", "fame_host.cpp")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
          # packed FP32 instructions are kept out of all device code
@@ -26,6 +27,14 @@ def build(verbose=False):
     if verbose:
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    # host-only index builder (C++ / OpenMP)
+    tmp = INDEX_LIB + ".tmp"
+    r = subprocess.run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-o", tmp,
+                        os.path.join(HERE, "csrc", "fame_index.cpp")], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building libfameindex.so")
+    os.replace(tmp, INDEX_LIB)
     return LIB
 
 
