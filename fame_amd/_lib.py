@@ -21,7 +21,7 @@ MAX_READ = 255
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
            "fame_match_reads_device", "fame_decide_strand", "fame_get_meth", "fame_reset",
-           "fame_print_methylation_levels", "fame_last_timings")
+           "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
 _u16p = C.POINTER(C.c_uint16)
@@ -57,6 +57,7 @@ def load():
         L.fame_reset.argtypes = [C.c_void_p]
         L.fame_print_methylation_levels.argtypes = [C.c_void_p, C.c_char_p]
         L.fame_last_timings.argtypes = [C.c_void_p, _dbl]
+        L.fame_last_stats.argtypes = [C.c_void_p, _u64p]
         _lib = L
     return _lib
 
@@ -129,6 +130,12 @@ class Context:
         t = np.zeros(6, np.float64)
         self._check(load().fame_last_timings(self.h, ptr(t, _dbl)))
         return dict(zip(("prep", "seed", "verify", "select", "meth", "total"), t.tolist()))
+
+    def stats(self):
+        t = np.zeros(8, np.uint64)
+        self._check(load().fame_last_stats(self.h, ptr(t, _u64p)))
+        return dict(zip(("kmers", "entries", "seeded", "jobs", "window_bytes", "dp", "read_bytes",
+                         "tab_bytes"), (int(x) for x in t)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -412,6 +412,12 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     const uint64_t E = carry;
     *E_out = E;
     *bc_out = bucketCount;
+    if (lane == 0) {
+        atomicAdd(&bd.stats[0], (unsigned long long)K);
+        atomicAdd(&bd.stats[1], (unsigned long long)E);
+        atomicAdd(&bd.stats[2], 1ull);
+        atomicAdd(&bd.stats[6], (unsigned long long)L);
+    }
     if (big || E > w.cap_e) return 1;
     // entry gather: ent[koff[k] + i] = kmer[lo + i]
     for (uint32_t k = lane; k < K; k += 64) {
@@ -718,15 +724,16 @@ __device__ __forceinline__ uint32_t base_code(uint8_t c)
     return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 3u : 4u;
 }
 
+// returns the number of window bytes scanned (0 when the lane has no job)
 template <int W>
-__global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
-                                                const uint32_t* __restrict__ only, uint32_t n_only)
+__device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
+                                               const uint32_t* __restrict__ only, uint32_t n_only,
+                                               uint64_t* peq)
 {
-    __shared__ uint64_t s_peq[256][4 * W];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t njobs = only ? n_only : bd.pool_n[pool];
-    if (tid >= njobs) return;
-    if (only && bd.ovf_pool[tid] != pool) return;
+    if (tid >= njobs) return 0;
+    if (only && bd.ovf_pool[tid] != pool) return 0;
     const uint32_t j = only ? only[tid] : tid;
     const uint32_t meta = bd.job_meta[pool][j];
     const uint32_t ro = bd.job_ro[pool][j];
@@ -735,7 +742,6 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
     const uint32_t id = meta & 0x7FFFFFFFu;
     const uint32_t m = bd.lens[r];
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-    uint64_t* peq = s_peq[threadIdx.x];
     // Peq masks of the pattern (ShiftAnd loadBitmasks ShiftAnd.h:345-432):
     // pattern T also matches text C (bisulfite asymmetry)
 #pragma unroll
@@ -855,6 +861,24 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
             }
         }
     }
+    return tlen ? tlen : 1;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
+                                                const uint32_t* __restrict__ only, uint32_t n_only)
+{
+    __shared__ uint64_t s_peq[256][4 * W];
+    __shared__ unsigned long long s_work[2];
+    if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t t = verify_one<W>(ix, bd, pool, only, n_only, s_peq[threadIdx.x]);
+    if (t && !only) {
+        atomicAdd(&s_work[0], 1ull);
+        atomicAdd(&s_work[1], (unsigned long long)t);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&bd.stats[3 + threadIdx.x], s_work[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1115,6 +1139,7 @@ __global__ __launch_bounds__(256) void k_meth(DevIndex ix, BatchDev bd, uint32_t
             continue;
         }
         // DP branch ReadQueue.cpp:3625-3912
+        atomicAdd(&bd.stats[5], 1ull);
         const uint32_t chrom = ix.cpg_chrom[m.z];
         const uint8_t* cs = ix.seq + ix.coff[chrom];
         const uint64_t clen = ix.clen[chrom];

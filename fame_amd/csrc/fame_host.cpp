@@ -78,7 +78,8 @@ struct fame_ctx {
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
     DevBuf d_tab, d_kent, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
-    DevBuf d_meth, d_counters, d_flags;
+    DevBuf d_meth, d_counters, d_flags, d_stats;
+    uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
     // batch buffers
     DevBuf b_seqs, b_offs, b_lens, b_rev, b_pre, b_cand_off, b_cand_nf, b_cand_nr, b_pool_n;
@@ -378,6 +379,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.counters = c->d_counters.as<uint64_t>();
     bd.meth = c->d_meth.as<uint32_t>();
     bd.flags = c->d_flags.as<uint32_t>();
+    bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
+    HIPC(hipMemsetAsync(bd.stats, 0, 8 * FAME_NSTATS, st));
 
     tm.mark(0);
     HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
@@ -512,6 +515,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     tm.mark(5);
     uint64_t cnt[5];
     HIPC(hipMemcpyAsync(cnt, bd.counters, 40, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(c->stats, bd.stats, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (flags & FLAG_CAPACITY) {
@@ -722,6 +726,15 @@ int fame_print_methylation_levels(fame_ctx* c, const char* basename)
                 (unsigned)m[4 * i], (unsigned)m[4 * i + 1], (unsigned)m[4 * i + 2], (unsigned)m[4 * i + 3]);
     }
     fclose(f);
+    return FAME_OK;
+}
+
+int fame_last_stats(const fame_ctx* c, uint64_t out[FAME_STATS_FIELDS])
+{
+    if (!c || !out) return FAME_ERR_ARG;
+    static_assert(FAME_STATS_FIELDS == FAME_NSTATS, "stats width");
+    memcpy(out, c->stats, sizeof c->stats);
+    out[7] = c->ix.tab_is64 ? 8 : 4;
     return FAME_OK;
 }
 

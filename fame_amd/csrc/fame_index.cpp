@@ -27,11 +27,13 @@
 //   fame_build_index_fasta(fasta, ..., out_path)  from a FASTA file
 
 #include <omp.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -354,6 +356,52 @@ uint32_t save_tmask(const Genome& g, const Win& w, uint64_t off, bool strand_of_
     return m;
 }
 
+// large uninitialised array on transparent huge pages (first touch happens
+// in the parallel loops that fill it)
+template <typename T>
+struct BigArray {
+    T* p = nullptr;
+    size_t n = 0;
+    explicit BigArray(size_t count) : n(count)
+    {
+        size_t bytes = ((count * sizeof(T)) + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+        p = static_cast<T*>(aligned_alloc(2u << 20, bytes ? bytes : (2u << 20)));
+        if (!p) throw std::bad_alloc();
+        madvise(p, bytes, MADV_HUGEPAGE);
+    }
+    ~BigArray() { free(p); }
+    BigArray(const BigArray&) = delete;
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+    T* data() { return p; }
+};
+
+// exclusive prefix sum out[i] = sum(v(j), j < i) for i <= n, in parallel blocks
+template <typename V, typename O>
+void par_prefix(uint64_t n, V&& v, O& out)
+{
+    const int T = omp_get_max_threads();
+    std::vector<uint64_t> part(T + 1, 0);
+    const uint64_t blk = (n + T - 1) / T;
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = std::min<uint64_t>(n, blk * t), b = std::min<uint64_t>(n, a + blk);
+        uint64_t s = 0;
+        for (uint64_t i = a; i < b; ++i) s += v(i);
+        part[t + 1] = s;
+#pragma omp barrier
+#pragma omp single
+        for (int k = 0; k < T; ++k) part[k + 1] += part[k];
+        s = part[t];
+        for (uint64_t i = a; i < b; ++i) {
+            out[i] = s;
+            s += v(i);
+        }
+    }
+    out[n] = part[T];
+}
+
 struct Entry {
     uint32_t meta; // metaId (isStart is always 0)
     uint16_t off;
@@ -363,6 +411,11 @@ struct Entry {
 int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool noloss, const char* out,
           char* err, size_t errlen)
 {
+    const bool verbose = getenv("FAME_INDEX_VERBOSE") != nullptr;
+    double t0 = omp_get_wtime();
+    auto phase = [&](const char* what) {
+        if (verbose) fprintf(stderr, "[index] %-12s %.2fs\n", what, omp_get_wtime() - t0);
+    };
     if (readlen <= KMERLEN || htab == 0) {
         snprintf(err, errlen, "bad parameters");
         return -4;
@@ -385,6 +438,7 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
     }
     for (uint64_t w = 0; w < nwin; ++w) wbase[w + 1] += wbase[w];
     const uint64_t nent = wbase[nwin];
+    phase("windows+count");
     // pass 2: bucket and payload of every entry, indexed by insertion order g
     std::vector<uint32_t> ekey(nent);
     std::vector<Entry> eval(nent);
@@ -404,25 +458,21 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
         }
         delete s;
     }
+    phase("hash");
     if (htab > 0xFFFFFFFFull) {
         snprintf(err, errlen, "HTABSIZE above 2^32 is not supported by this builder");
         return -4;
     }
     // counting sort; inside a bucket descending insertion order (the
     // reference decrements tabIndex while inserting in ascending order)
-    std::vector<uint64_t> tab(htab + 1, 0);
+    BigArray<uint64_t> tab(htab + 1);
     {
-        std::vector<std::atomic<uint32_t>>* cnt = new std::vector<std::atomic<uint32_t>>(htab);
+        BigArray<std::atomic<uint32_t>>* cnt = new BigArray<std::atomic<uint32_t>>(htab);
 #pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < htab; ++i) (*cnt)[i].store(0, std::memory_order_relaxed);
 #pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < nent; ++i) (*cnt)[ekey[i]].fetch_add(1, std::memory_order_relaxed);
-        uint64_t sum = 0;
-        for (uint64_t i = 0; i < htab; ++i) {
-            tab[i] = sum;
-            sum += (*cnt)[i].load(std::memory_order_relaxed);
-        }
-        tab[htab] = sum;
+        par_prefix(htab, [&](uint64_t i) { return (uint64_t)(*cnt)[i].load(std::memory_order_relaxed); }, tab);
         // fill slots in any order, then order each bucket by descending g
         std::vector<uint64_t> slot_g(nent);
 #pragma omp parallel for schedule(static)
@@ -436,7 +486,8 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
 #pragma omp parallel for schedule(dynamic, 65536)
         for (uint64_t b = 0; b < htab; ++b)
             if (tab[b + 1] - tab[b] > 1)
-                std::sort(slot_g.begin() + tab[b], slot_g.begin() + tab[b + 1], std::greater<uint64_t>());
+                std::sort(slot_g.begin() + (ptrdiff_t)tab[b], slot_g.begin() + (ptrdiff_t)tab[b + 1],
+                          std::greater<uint64_t>());
         std::vector<uint32_t>().swap(ekey);
         // payload in table order
         std::vector<Entry> tbl(nent);
@@ -444,6 +495,7 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
         for (uint64_t i = 0; i < nent; ++i) tbl[i] = eval[slot_g[i]];
         eval.swap(tbl);
     }
+    phase("sort");
     std::vector<Entry>& kt = eval;
     // filterHashTable: buckets with >= KMERCUTOFF entries lose the k-mers
     // occurring >= KMERCUTOFF times in them
@@ -451,8 +503,16 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
     std::vector<uint8_t> keep(nent, 1);
     if (!noloss) {
         std::vector<uint64_t> big;
-        for (uint64_t b = 0; b < htab; ++b)
-            if (tab[b + 1] - tab[b] >= kmercutoff) big.push_back(b);
+#pragma omp parallel
+        {
+            std::vector<uint64_t> mine;
+#pragma omp for schedule(static) nowait
+            for (uint64_t b = 0; b < htab; ++b)
+                if (tab[b + 1] - tab[b] >= kmercutoff) mine.push_back(b);
+#pragma omp critical
+            big.insert(big.end(), mine.begin(), mine.end());
+        }
+        std::sort(big.begin(), big.end());
 #pragma omp parallel
         {
             std::set<uint64_t> mine;
@@ -474,6 +534,7 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
             filtered.insert(mine.begin(), mine.end());
         }
     }
+    phase("cutoff");
     // filterRedundancyInHashTable: inside a run of entries with the same
     // (window, strand), keep the first entry of every distinct tmask
 #pragma omp parallel for schedule(dynamic, 65536)
@@ -498,26 +559,25 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
             }
         }
     }
+    phase("redundancy");
     // compaction
-    std::vector<uint64_t> ntab(htab + 1);
-    {
-        uint64_t s = 0;
-        for (uint64_t b = 0; b < htab; ++b) {
-            ntab[b] = s;
-            for (uint64_t j = tab[b]; j < tab[b + 1]; ++j) s += keep[j];
-        }
-        ntab[htab] = s;
-    }
+    BigArray<uint64_t> ntab(htab + 1);
+    par_prefix(htab, [&](uint64_t b) {
+        uint64_t k = 0;
+        for (uint64_t j = tab[b]; j < tab[b + 1]; ++j) k += keep[j];
+        return k;
+    }, ntab);
     const uint64_t nk = ntab[htab];
     std::vector<Entry> fk(nk);
-    {
-        uint64_t o = 0;
-        for (uint64_t j = 0; j < nent; ++j)
+#pragma omp parallel for schedule(dynamic, 65536)
+    for (uint64_t b = 0; b < htab; ++b) {
+        uint64_t o = ntab[b];
+        for (uint64_t j = tab[b]; j < tab[b + 1]; ++j)
             if (keep[j]) fk[o++] = kt[j];
     }
     std::vector<Entry>().swap(kt);
     std::vector<uint8_t>().swap(keep);
-    std::vector<uint64_t>().swap(tab);
+    phase("compact");
     // save RefGenome.cpp:83-228
     FILE* f = fopen(out, "wb");
     if (!f) {
@@ -600,6 +660,7 @@ int build(Genome& g, uint32_t readlen, uint64_t htab, uint64_t kmercutoff, bool 
     }
     const bool ok = !ferror(f);
     fclose(f);
+    phase("write");
     // <index>_strands, write_strands RefGenome.cpp:392-412
     std::string sp = std::string(out) + "_strands";
     FILE* s = fopen(sp.c_str(), "wb");

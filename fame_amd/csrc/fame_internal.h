@@ -9,6 +9,7 @@
 #define FAME_DEV_MAX_READ 255
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
+#define FAME_NSTATS 8
 #define FAME_DPBAND 8 // 2*MISCOUNT + ADDMIS (ReadQueue.cpp:3640)
 
 namespace fame {
@@ -84,6 +85,10 @@ struct BatchDev {
     uint64_t* counters; // succ, nonunique, unsucc, r1fwd, r1rev (accumulated)
     uint32_t* meth;     // 4 x u32 per CpG
     uint32_t* flags;
+    // work counters: k-mers hashed, bucket entries gathered, (read,
+    // orientation) pairs seeded, verify jobs, window bytes scanned, DP
+    // alignments, read bytes seeded
+    unsigned long long* stats;
 };
 
 struct HeavyScratch {

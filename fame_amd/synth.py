@@ -21,6 +21,8 @@ are produced in tens of seconds.
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 _BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
@@ -54,6 +56,10 @@ def write_fasta(path, chroms, names=None, width=80):
             b = seq.tobytes()
             for s in range(0, len(b), width):
                 f.write(b[s:s + width] + b"\n")
+
+
+# Phred 30-40 quality lines (qualities do not enter the matching path)
+_QUAL = (np.random.default_rng(5).integers(30, 41, size=(4096, 256)) + 33).astype(np.uint8)
 
 
 def revcomp(a):
@@ -93,7 +99,7 @@ class ReadSet:
                 L = int(self.lens[i])
                 s = self.seqs[i, :L].tobytes()
                 q = (self.quals[i, :L] if self.quals is not None
-                     else np.full(L, 73, np.uint8)).tobytes()
+                     else _QUAL[i % _QUAL.shape[0], :L]).tobytes()
                 f.write(b"@" + f"{name_prefix}{i}".encode() + b"\n" + s + b"\n+\n" + q + b"\n")
 
     def subset(self, idx):
@@ -114,39 +120,42 @@ def meth_levels(chroms, seed=7):
     return out
 
 
-def _bisulfite(reads, ctx_next, level_at, rng, non_cpg_conv=0.995):
-    """Convert C -> T in place on (n, L) reads.
-
-    ctx_next: (n, L) base following each read base on the read's strand.
-    level_at: (n, L) methylation level of the CpG a C belongs to (0 if none).
-    """
-    is_c = reads == ord("C")
-    in_cpg = is_c & (ctx_next == ord("G"))
-    u = rng.random(reads.shape, dtype=np.float32)
-    conv = np.where(in_cpg, u >= level_at, u < non_cpg_conv) & is_c
-    reads[conv] = ord("T")
+_CODE = np.full(256, 0, dtype=np.uint8)
+for _i, _b in enumerate(b"ACGT"):
+    _CODE[_b] = _i
 
 
 def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_rate=0.0005,
-               len_range=None, ob_fraction=0.5, batch=1 << 20, regions=None):
+               len_range=None, ob_fraction=0.5, batch=1 << 18, regions=None,
+               non_cpg_conv=0.995, threads=None):
     """Sample `n` bisulfite reads.
 
     read_len   nominal (maximum) length; with `len_range=(lo, hi)` each read's
                length is uniform in [lo, hi] (hi <= read_len).
     regions    optional (chrom_idx, start, end) arrays restricting the origin
                of every read (used for repeat-heavy mixes).
+    Batches are generated on a thread pool, each from its own seeded stream,
+    so the output depends only on the arguments.
     """
-    rng = np.random.default_rng(seed)
+    from numpy.lib.stride_tricks import sliding_window_view
+
     if levels is None:
         levels = meth_levels(chroms, seed=seed + 11)
     lens_c = np.array([c.size for c in chroms], dtype=np.int64)
+    coff = np.zeros(len(chroms), np.int64)
+    coff[1:] = np.cumsum(lens_c)[:-1]
+    G = np.concatenate(chroms)
+    LV = np.concatenate(levels)
     pad = read_len + 8
+    Gw = sliding_window_view(G, pad + 2)
+    Lw = sliding_window_view(LV, pad + 2)
     seqs = np.zeros((n, read_len), dtype=np.uint8)
     lens = np.zeros(n, dtype=np.uint32)
     t_chrom = np.zeros(n, dtype=np.int32)
     t_pos = np.zeros(n, dtype=np.int64)
     t_ob = np.zeros(n, dtype=bool)
-    for s in range(0, n, batch):
+    def one(s):
+        rng = np.random.default_rng([seed, s])
         m = min(batch, n - s)
         if regions is None:
             ci = rng.choice(lens_c.size, size=m, p=lens_c / lens_c.sum())
@@ -162,51 +171,50 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
             L = np.full(m, read_len, dtype=np.int64)
         else:
             L = rng.integers(len_range[0], len_range[1] + 1, size=m)
-        # fetch a window of pad+1 bases (+1 for CpG context past the end)
-        win = np.empty((m, pad + 2), dtype=np.uint8)
-        lvl = np.empty((m, pad + 2), dtype=np.float32)
-        for c in np.unique(ci):
-            sel = np.nonzero(ci == c)[0]
-            idx = pos[sel, None] + np.arange(-1, pad + 1)[None, :]
-            win[sel] = chroms[c][idx]
-            lvl[sel] = levels[c][idx]
-        # OT: bases win[1 : 1+pad], next-base context win[2 : 2+pad]
-        # OB: reverse complement of the same span; a CpG C on the bottom
-        # strand is the complement of the G of a top-strand CpG whose C sits
-        # one position to the left, so its level is read from there.
-        fwd = win[:, 1:1 + pad].copy()
-        fwd_next = win[:, 2:2 + pad]
-        fwd_lvl = lvl[:, 1:1 + pad]
-        rev = revcomp(win[:, 1:1 + pad])
-        rev_next = revcomp(win[:, 0:pad])
-        rev_lvl = lvl[:, 0:pad][:, ::-1]
-        read = np.where(ob[:, None], rev, fwd)
-        nxt = np.where(ob[:, None], rev_next, fwd_next)
-        lv = np.where(ob[:, None], rev_lvl, fwd_lvl)
-        _bisulfite(read, nxt, lv, rng)
-        # substitutions
-        sub = rng.random(read.shape, dtype=np.float32) < sub_rate
-        alt = _BASES[(np.searchsorted(_BASES, read) + rng.integers(1, 4, size=read.shape)) % 4]
-        read = np.where(sub, alt, read)
-        # 1-bp indels: at most one per read (the first drawn event), applied
-        # by a gather so that the whole batch stays vectorised
-        ev = rng.random(read.shape, dtype=np.float32) < indel_rate
-        has = ev.any(axis=1)
-        p_ev = np.where(has, ev.argmax(axis=1), pad + 1)[:, None]
+        # genome span [pos-1, pos+pad] of every read (one base of context on
+        # both sides for CpG calls at the read ends)
+        g0 = coff[ci] + pos - 1
+        win = Gw[g0]
+        lvl = Lw[g0]
+        # OT: bases win[1 : 1+pad], next base win[2 : 2+pad].  OB: the reverse
+        # complement of the same span; a bottom-strand CpG C is the complement
+        # of the G of a top-strand CpG whose C sits one position to the left,
+        # so its level is read there.
+        read = np.where(ob[:, None], revcomp(win[:, 1:1 + pad]), win[:, 1:1 + pad])
+        nxt = np.where(ob[:, None], revcomp(win[:, 0:pad]), win[:, 2:2 + pad])
+        lv = np.where(ob[:, None], lvl[:, 0:pad][:, ::-1], lvl[:, 1:1 + pad])
+        # bisulfite: CpG C kept with the CpG's level, other C -> T at non_cpg_conv
+        cpos = np.flatnonzero(read == ord("C"))
+        u = rng.random(cpos.size, dtype=np.float32)
+        in_cpg = nxt.reshape(-1)[cpos] == ord("G")
+        conv = np.where(in_cpg, u >= lv.reshape(-1)[cpos], u < non_cpg_conv)
+        flat = read.reshape(-1)
+        flat[cpos[conv]] = ord("T")
+        # substitutions: Binomial count, uniform positions, a different base
+        ns = rng.binomial(flat.size, sub_rate)
+        sp = rng.integers(0, flat.size, size=ns)
+        flat[sp] = _BASES[(_CODE[flat[sp]] + rng.integers(1, 4, size=ns)) % 4]
+        # at most one 1-bp indel per read, applied by a gather
+        has = rng.random(m) < 1.0 - (1.0 - indel_rate) ** pad
+        p_ev = np.where(has, rng.integers(0, read_len, size=m), pad + 1)[:, None]
         is_del = (rng.random(m) < 0.5)[:, None]
         j = np.arange(read_len)[None, :]
         src = np.where(is_del, j + (j >= p_ev), j - (j > p_ev))
         out = np.take_along_axis(read, np.minimum(src, pad - 1), axis=1)
-        ins_here = (~is_del) & (j == p_ev)
-        out = np.where(ins_here, _BASES[rng.integers(0, 4, size=out.shape)], out)
-        col = np.arange(read_len)[None, :]
-        out[col >= L[:, None]] = 0
+        rows = np.flatnonzero(has & ~is_del[:, 0])
+        out[rows, p_ev[rows, 0]] = _BASES[rng.integers(0, 4, size=rows.size)]
+        if len_range is not None:
+            out[j >= L[:, None]] = 0
         seqs[s:s + m] = out
         lens[s:s + m] = L
         t_chrom[s:s + m] = ci
         t_pos[s:s + m] = pos
         t_ob[s:s + m] = ob
-    quals = (rng.integers(30, 41, size=seqs.shape) + 33).astype(np.uint8)
+
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(threads or min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(one, range(0, n, batch)))
+    quals = None
     return ReadSet(seqs, lens, quals, {"chrom": t_chrom, "pos": t_pos, "ob": t_ob})
 
 

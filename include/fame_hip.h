@@ -96,6 +96,14 @@ int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
  * prep, seed, verify, select, meth, and the total including host waits */
 int fame_last_timings(const fame_ctx* ctx, double out[6]);
 
+/* work done by the last fame_match_reads* call, for roofline accounting:
+ * k-mers hashed, index bucket entries gathered, (read, orientation) pairs
+ * seeded, window verifications, window bytes scanned, banded alignments,
+ * read bytes seeded, and the byte width of one tabIndex entry on the device.
+ * No counterpart in the reference (instrumentation). */
+#define FAME_STATS_FIELDS 8
+int fame_last_stats(const fame_ctx* ctx, uint64_t out[FAME_STATS_FIELDS]);
+
 #ifdef __cplusplus
 }
 #endif
