@@ -46,6 +46,10 @@ static_assert(DPBAND == 2 * MISCOUNT + ADDMIS, "DP band");
 constexpr uint32_t CPGDUMMY = 0xFFFFFFFFu;
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t NBH = 62;             // hopscotch NeighborhoodSize
+constexpr uint32_t JOB_GAP = 0xFFFFFFFFu; // unused verify slot
+// spaced seed (CONST.h:57-58); RefGenome::load rejects other seeds
+// (RefGenome.cpp:275-280), and so does fame_open
+constexpr uint32_t SEEDBITS = FAME_SEEDBITS;
 constexpr uint32_t MAX_PROBES = 12 * NBH; // MAX_PROBES_FOR_EMPTY_BUCKET
 
 // ---------------------------------------------------------------------------
@@ -131,143 +135,186 @@ __device__ __forceinline__ uint64_t hs_ceil_div09(uint64_t n)
 }
 
 // ---------------------------------------------------------------------------
-// tsl::hopscotch_map<uint32_t, uint16_t, identity> placement emulation.
-// Only the order of iteration is reproduced: keys are inserted in the order
-// in which operator[] first created them (counts never move an element).
-// tb[] holds, per physical bucket, the index of the key stored there.
+// tsl::hopscotch_map<uint32_t, uint16_t, identity> placement emulation,
+// executed by a whole wavefront: the probes for an empty bucket, the
+// neighbourhood checks, the rehash collection and the final iteration are
+// ballots over 64 buckets at a time; the decisions (and so the control flow)
+// are wave-uniform.  Only the order of iteration is reproduced: keys are
+// inserted in the order in which operator[] first created them (count
+// updates never move an element).  tb[] holds, per physical bucket, the
+// index of the key stored there.
 // ---------------------------------------------------------------------------
 
+template <typename IdxT>
 struct Hs {
-    uint32_t* tb;
+    IdxT* tb;
     uint32_t tb_cap;
-    uint32_t* over;
+    IdxT* over;
     uint32_t over_cap;
-    uint32_t* tmp;
+    IdxT* tmp;
     uint32_t tmp_cap;
     const uint32_t* key;
     uint64_t B, mask, arr;
     uint32_t nelem, nover, max_thr, min_thr;
+    static constexpr IdxT NONE = (IdxT)~(IdxT)0;
 };
 
-enum { HS_OK = 0, HS_FAIL = 1 };
+enum { HS_OK = 0, HS_FAIL = 1, HS_NEED_REHASH = 2 };
+
+// lanes see each other's LDS / global writes (one wavefront per block)
+__device__ __forceinline__ void wave_sync()
+{
+    __threadfence_block();
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
 
 // hopscotch_hash ctor (hopscotch_hash.h:599-628) on a fresh bucket array;
 // the overflow list is kept (rehash_impl moves it to the new table)
-__device__ bool hs_setup(Hs& h, uint64_t bucket_count)
+template <typename IdxT>
+__device__ bool hs_setup(Hs<IdxT>& h, uint64_t bucket_count, int lane)
 {
-    uint64_t B = bucket_count ? rup_pow2(bucket_count) : 0;
-    uint64_t arr = B ? B + NBH - 1 : 0;
+    const uint64_t B = bucket_count ? rup_pow2(bucket_count) : 0;
+    const uint64_t arr = B ? B + NBH - 1 : 0;
     if (arr > h.tb_cap) return false;
     h.B = B;
     h.mask = B ? B - 1 : 0;
     h.arr = arr;
-    for (uint64_t i = 0; i < arr; ++i) h.tb[i] = EMPTY;
+    for (uint64_t i = lane; i < arr; i += 64) h.tb[i] = Hs<IdxT>::NONE;
     h.max_thr = hs_thr(B, 0.9f);
     h.min_thr = hs_thr(B, 0.1f);
+    wave_sync();
     return true;
 }
 
-// swap_empty_bucket_closer hopscotch_hash.h:1545-1578; the neighbourhood
-// bitmap of bucket tc is "the buckets tc+i holding a key whose home is tc"
-__device__ bool hs_swap_closer(Hs& h, uint64_t& e)
+// find_empty_bucket hopscotch_hash.h:1491-1500
+template <typename IdxT>
+__device__ uint64_t hs_find_empty(const Hs<IdxT>& h, uint64_t ib, int lane)
 {
-    const uint64_t start = e - NBH + 1;
-    for (uint64_t tc = start; tc < e; ++tc) {
-        for (uint64_t ts = tc; ts < e; ++ts) {
-            uint32_t i = h.tb[ts];
-            if (i != EMPTY && (h.key[i] & h.mask) == tc) {
-                h.tb[e] = i;
-                h.tb[ts] = EMPTY;
-                e = ts;
-                return true;
+    const uint64_t lim = ib + MAX_PROBES < h.arr ? ib + MAX_PROBES : h.arr;
+    for (uint64_t b0 = ib; b0 < lim; b0 += 64) {
+        const uint64_t b = b0 + lane;
+        const uint64_t bal = __ballot(b < lim && h.tb[b] == Hs<IdxT>::NONE);
+        if (bal) return b0 + __ffsll((long long)bal) - 1;
+    }
+    return h.arr;
+}
+
+// swap_empty_bucket_closer hopscotch_hash.h:1545-1578 (rare: the
+// neighbourhood of a home bucket is full); computed by lane 0, where the
+// neighbourhood bitmap of bucket tc is "the buckets tc+i holding a key whose
+// home is tc"
+template <typename IdxT>
+__device__ bool hs_swap_closer(Hs<IdxT>& h, uint64_t& e, int lane)
+{
+    uint64_t ne = e;
+    int found = 0;
+    if (lane == 0) {
+        const uint64_t start = e - NBH + 1;
+        for (uint64_t tc = start; tc < e && !found; ++tc) {
+            for (uint64_t ts = tc; ts < e; ++ts) {
+                const IdxT i = h.tb[ts];
+                if (i != Hs<IdxT>::NONE && (h.key[i] & h.mask) == tc) {
+                    h.tb[e] = i;
+                    h.tb[ts] = Hs<IdxT>::NONE;
+                    ne = ts;
+                    found = 1;
+                    break;
+                }
             }
         }
     }
-    return false;
+    wave_sync();
+    found = __shfl(found, 0, 64);
+    e = __shfl((unsigned long long)ne, 0, 64);
+    return found != 0;
 }
 
 // will_neighborhood_change_on_rehash hopscotch_hash.h:1466-1486
-__device__ bool hs_will_change(const Hs& h, uint64_t ib)
+template <typename IdxT>
+__device__ bool hs_will_change(const Hs<IdxT>& h, uint64_t ib, int lane)
 {
     const uint64_t mask2 = rup_pow2((h.mask + 1) * 2) - 1;
-    for (uint64_t b = ib; b < h.arr && (b - ib) < NBH; ++b) {
-        uint32_t i = h.tb[b];
-        if (i == EMPTY) continue;
-        uint64_t k = h.key[i];
-        if ((k & h.mask) != (k & mask2)) return true;
+    const uint64_t b = ib + lane;
+    bool ch = false;
+    if (lane < (int)NBH && b < h.arr) {
+        const IdxT i = h.tb[b];
+        if (i != Hs<IdxT>::NONE) {
+            const uint64_t k = h.key[i];
+            ch = (k & h.mask) != (k & mask2);
+        }
     }
-    return false;
+    return __ballot(ch) != 0;
 }
 
-enum { HS_PLACED = 0, HS_NEED_REHASH = 2 };
-
 // one pass of insert_impl hopscotch_hash.h:1426-1458 without its rehash
-// calls: returns HS_NEED_REHASH where the reference rehashes (load
-// threshold reached first, or no bucket within the neighbourhood and the
-// element may not go to the overflow list)
-__device__ int hs_place(Hs& h, uint32_t idx)
+// calls: HS_NEED_REHASH where the reference rehashes (load threshold reached
+// first, or no bucket in the neighbourhood and no overflow allowed)
+template <typename IdxT>
+__device__ int hs_place(Hs<IdxT>& h, uint32_t idx, int lane)
 {
     if (h.nelem - h.nover >= h.max_thr) return HS_NEED_REHASH;
     const uint64_t ib = h.key[idx] & h.mask;
-    uint64_t e = h.arr;
-    {
-        const uint64_t lim = ib + MAX_PROBES < h.arr ? ib + MAX_PROBES : h.arr;
-        for (uint64_t b = ib; b < lim; ++b)
-            if (h.tb[b] == EMPTY) {
-                e = b;
-                break;
-            }
-    }
+    uint64_t e = hs_find_empty(h, ib, lane);
     if (e < h.arr) {
         for (;;) {
             if (e - ib < NBH) {
-                h.tb[e] = idx;
+                if (lane == 0) h.tb[e] = (IdxT)idx;
+                wave_sync();
                 h.nelem++;
-                return HS_PLACED;
+                return HS_OK;
             }
-            if (!hs_swap_closer(h, e)) break;
+            if (!hs_swap_closer(h, e, lane)) break;
         }
     }
-    if (h.nelem < h.min_thr || !hs_will_change(h, ib)) {
+    if (h.nelem < h.min_thr || !hs_will_change(h, ib, lane)) {
         if (h.nover >= h.over_cap) return HS_FAIL;
-        h.over[h.nover++] = idx;
+        if (lane == 0) h.over[h.nover] = (IdxT)idx;
+        wave_sync();
+        h.nover++;
         h.nelem++;
-        return HS_PLACED;
+        return HS_OK;
     }
     return HS_NEED_REHASH;
 }
 
 // rehash_impl hopscotch_hash.h:1244-1290: a new table, the overflow list
 // carried over, then the old buckets re-inserted in physical order.  A
-// rehash that would be needed inside this re-insertion is reported as a
-// capacity failure.
-__device__ int hs_rehash(Hs& h, uint64_t count)
+// rehash needed inside this re-insertion is reported as a capacity failure.
+template <typename IdxT>
+__device__ int hs_rehash(Hs<IdxT>& h, uint64_t count, int lane)
 {
     const uint64_t need = hs_ceil_div09(h.nelem);
     if (need > count) count = need;
     uint32_t n = 0;
-    for (uint64_t b = 0; b < h.arr; ++b) {
-        uint32_t i = h.tb[b];
-        if (i != EMPTY) {
-            if (n >= h.tmp_cap) return HS_FAIL;
-            h.tmp[n++] = i;
-        }
+    for (uint64_t b0 = 0; b0 < h.arr; b0 += 64) {
+        const uint64_t b = b0 + lane;
+        const IdxT i = b < h.arr ? h.tb[b] : Hs<IdxT>::NONE;
+        const bool occ = i != Hs<IdxT>::NONE;
+        const uint64_t bal = __ballot(occ);
+        const uint32_t p = n + __popcll(bal & lanes_below(lane));
+        if (occ && p < h.tmp_cap) h.tmp[p] = i;
+        n += __popcll(bal);
     }
-    if (!hs_setup(h, count)) return HS_FAIL;
+    if (n > h.tmp_cap) return HS_FAIL;
+    wave_sync();
+    if (!hs_setup(h, count, lane)) return HS_FAIL;
     h.nelem = h.nover;
     for (uint32_t t = 0; t < n; ++t)
-        if (hs_place(h, h.tmp[t]) != HS_PLACED) return HS_FAIL;
+        if (hs_place(h, (uint32_t)h.tmp[t], lane) != HS_OK) return HS_FAIL;
     return HS_OK;
 }
 
 // operator[] creating a key: insert_impl with its rehash-and-retry loop
-__device__ int hs_insert(Hs& h, uint32_t idx)
+template <typename IdxT>
+__device__ int hs_insert(Hs<IdxT>& h, uint32_t idx, int lane)
 {
     for (;;) {
-        const int r = hs_place(h, idx);
-        if (r == HS_PLACED) return HS_OK;
+        const int r = hs_place(h, idx, lane);
+        if (r == HS_OK) return HS_OK;
         if (r == HS_FAIL) return HS_FAIL;
-        if (hs_rehash(h, (h.mask + 1) * 2) != HS_OK) return HS_FAIL;
+        if (hs_rehash(h, (h.mask + 1) * 2, lane) != HS_OK) return HS_FAIL;
     }
 }
 
@@ -315,77 +362,135 @@ __global__ void k_prep(const uint8_t* __restrict__ seqs, const uint64_t* __restr
 // k_seed: seed + filter + map order for one (read, orientation) per wave
 // ---------------------------------------------------------------------------
 
+template <typename TabT, typename IdxT>
 struct SeedWork {
-    // phase A (gather / count)
-    uint64_t* klo;     // [K]
-    uint32_t* klen;    // [K]
-    uint32_t* koff;    // [K] exclusive prefix of klen
-    uint32_t* ent;     // [cap_e] packed metaId<<1 | fwd
-    uint8_t* ekm;      // [cap_e] k-mer index of each entry
-    uint32_t* eaux;    // [cap_e] flags (bit0 kept, bit1 first) or hash slot (heavy)
+    TabT* klo;      // [K] bucket start
+    uint32_t* klen; // [K] bucket length
+    uint32_t* koff; // [K] exclusive prefix of klen
+    uint32_t* ent;  // [cap_e] packed metaId<<1 | fwd
+    uint8_t* ekm;   // [cap_e] k-mer index of each entry
+    uint32_t* eaux; // [cap_e] kept / first / count (or hash slot, heavy)
     uint32_t cap_e;
-    // heavy path only: hash table keyed by packed entry
-    uint32_t* hkey;
+    uint32_t* hkey; // heavy path: hash table keyed by packed entry
     uint32_t* hfirst;
     uint32_t* hcnt;
-    uint32_t hsize; // power of two, 0 = use pairwise compare
-    // insertion lists (alive across phases)
-    uint32_t* ins_key[2];
-    uint32_t* ins_cnt[2];
+    uint32_t hsize; // power of two; 0 = pairwise comparison
+    uint32_t* ins_key; // [2][cap_ins] keys in first-insertion order per map
+    uint16_t* ins_cnt; // [2][cap_ins] their u16 counts
     uint32_t cap_ins;
-    // phase B (hopscotch)
-    uint32_t* tb[2];
+    IdxT* tb;          // hopscotch buckets (one map at a time)
     uint32_t tb_cap;
-    uint32_t* tmp[2];
-    uint32_t* over[2];
-    uint32_t cap_list;
+    IdxT* tmp;         // rehash / output list
+    IdxT* over;        // overflow list
+    uint32_t* cand;    // [2][cap_ins] candidate windows per map
 };
 
 struct SeedShared {
-    uint8_t seq[FAME_DEV_MAX_READ + 1];
+    // rot[i][c] = rol(seed_c, 31 - i): the term of position i of a spaced
+    // k-mer (nthash.hpp:480-489); code 3 (N and others) contributes 0
+    uint64_t rot[KMERLEN][4];
     uint8_t code[FAME_DEV_MAX_READ + 1];
-    uint32_t nins[2];
-    uint32_t ncand[2];
-    uint32_t base;
-    uint32_t fail;
 };
 
-template <typename TabT>
-__device__ __forceinline__ uint64_t tab_at(const void* tab, uint64_t i)
+__device__ __forceinline__ void seed_table_init(SeedShared& sh, int lane)
 {
-    return (uint64_t)((const TabT*)tab)[i];
+    for (int t = lane; t < (int)KMERLEN * 4; t += 64) {
+        const int i = t >> 2, c = t & 3;
+        sh.rot[i][c] = rol64(red_seed((uint32_t)c), (int)(KMERLEN - 1 - i));
+    }
 }
 
-// Returns 0 done, 1 = needs the heavy path (capacity), with E / bucketCount
-// reported through the out pointers.
-template <typename TabT>
+// Returns 0 when done, 1 when the (read, orientation) needs the heavy path
+// (capacity); E / bucketCount are reported through the out pointers.
+// Candidate slots of the verify pools.  Persistent seed waves take pool
+// space in chunks (one atomic per chunk instead of one per read); the slots
+// a wave leaves unused are marked JOB_GAP.  `exact` takes exactly what a
+// read needs (heavy path).
+struct PoolChunk {
+    uint32_t pool, base, left; // one open chunk at a time (a batch rarely mixes pools)
+};
+constexpr uint32_t POOL_CHUNK = 64;
+
+__device__ __forceinline__ uint32_t pc_get_cap(const BatchDev& bd, uint32_t p)
+{
+    uint32_t v = bd.pool_cap[0];
+#pragma unroll
+    for (uint32_t q = 1; q < FAME_NPOOL; ++q) v = p == q ? bd.pool_cap[q] : v;
+    return v;
+}
+__device__ __forceinline__ uint32_t* pc_meta(const BatchDev& bd, uint32_t p)
+{
+    uint32_t* v = bd.job_meta[0];
+#pragma unroll
+    for (uint32_t q = 1; q < FAME_NPOOL; ++q) v = p == q ? bd.job_meta[q] : v;
+    return v;
+}
+
+__device__ void pool_gap_fill(const BatchDev& bd, uint32_t p, uint32_t base, uint32_t n, int lane)
+{
+    if (base + n > pc_get_cap(bd, p)) return;
+    uint32_t* meta = pc_meta(bd, p);
+    for (uint32_t t = lane; t < n; t += 64) meta[base + t] = JOB_GAP;
+}
+
+// returns the first slot of `need` consecutive slots, or ~0u if the pool is full
+__device__ uint32_t pool_take(const BatchDev& bd, PoolChunk& pc, uint32_t p, uint32_t need, bool exact, int lane)
+{
+    if (exact) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&bd.pool_n[p], need);
+        b = __shfl(b, 0, 64);
+        return b + need <= bd.pool_cap[p] ? b : ~0u;
+    }
+    if (pc.pool != p || pc.left < need) {
+        if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
+        const uint32_t sz = need > POOL_CHUNK ? need : POOL_CHUNK;
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&bd.pool_n[p], sz);
+        b = __shfl(b, 0, 64);
+        pc.pool = p;
+        pc.base = b;
+        pc.left = sz;
+        if (b + sz > pc_get_cap(bd, p)) {
+            pc.left = 0;
+            return ~0u;
+        }
+    }
+    const uint32_t b = pc.base;
+    pc.base += need;
+    pc.left -= need;
+    return b;
+}
+
+struct SeedStats {
+    uint64_t kmers, entries, seeded, bytes;
+};
+
+template <typename TabT, typename IdxT>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
-                        SeedWork& w, uint64_t* E_out, uint32_t* bc_out)
+                        const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
+                        uint64_t* E_out, uint32_t* bc_out)
 {
     const int lane = threadIdx.x;
     const uint32_t r = ro >> 1, ori = ro & 1;
     const uint32_t L = bd.lens[r];
     const uint8_t* s = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-    for (uint32_t i = lane; i < L; i += 64) {
-        uint8_t c = s[i];
-        sh.seq[i] = c;
-        sh.code[i] = (uint8_t)red_code(c);
-    }
-    __syncthreads();
+    for (uint32_t i = lane; i < L; i += 64) sh.code[i] = (uint8_t)red_code(s[i]);
+    wave_sync();
     const uint32_t K = L - KMERLEN + 1;
-    const uint32_t sb = ix.seedbits;
+    const TabT* tab = (const TabT*)ix.tab;
     // hashes + tabIndex gathers (NTPS64 closed form; key = h % HTABSIZE)
     uint32_t bsum = 0;
     for (uint32_t k = lane; k < K; k += 64) {
         uint64_t h = 0;
 #pragma unroll
         for (uint32_t i = 0; i < KMERLEN; ++i)
-            if ((sb >> (KMERLEN - 1 - i)) & 1) h ^= rol64(red_seed(sh.code[k + i]), (int)(KMERLEN - 1 - i));
+            if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][sh.code[k + i]];
         const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
-        const uint64_t lo = tab_at<TabT>(ix.tab, key);
-        const uint64_t hi = tab_at<TabT>(ix.tab, key + 1);
+        const TabT lo = tab[key];
+        const TabT hi = tab[key + 1];
         w.klo[k] = lo;
-        const uint64_t len = hi - lo;
+        const uint64_t len = (uint64_t)(hi - lo);
         w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
         bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
     }
@@ -397,13 +502,12 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     for (uint32_t k0 = 0; k0 < K; k0 += 64) {
         const uint32_t k = k0 + lane;
         uint32_t v = k < K ? w.klen[k] : 0;
-        // buckets beyond 2^24 entries cannot be served (reported as capacity)
-        if (v > 0xFFFFFFu) {
+        if (v > 0xFFFFFFu) { // beyond what the device path serves
             v = 0xFFFFFFu;
             big = true;
         }
-        uint32_t inc = wave_iscan_u32(v, lane);
-        uint32_t tot = __shfl(inc, 63, 64);
+        const uint32_t inc = wave_iscan_u32(v, lane);
+        const uint32_t tot = __shfl(inc, 63, 64);
         if (k < K) w.koff[k] = (uint32_t)(carry + inc - v);
         carry += tot;
         if (carry > 0x7FFFFFFFull) big = true;
@@ -412,30 +516,34 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     const uint64_t E = carry;
     *E_out = E;
     *bc_out = bucketCount;
-    if (lane == 0) {
-        atomicAdd(&bd.stats[0], (unsigned long long)K);
-        atomicAdd(&bd.stats[1], (unsigned long long)E);
-        atomicAdd(&bd.stats[2], 1ull);
-        atomicAdd(&bd.stats[6], (unsigned long long)L);
-    }
     if (big || E > w.cap_e) return 1;
-    // entry gather: ent[koff[k] + i] = kmer[lo + i]
-    for (uint32_t k = lane; k < K; k += 64) {
-        const uint64_t lo = w.klo[k];
-        const uint32_t n = w.klen[k], o = w.koff[k];
-        for (uint32_t i = 0; i < n; ++i) {
-            w.ent[o + i] = ix.kent[lo + i];
-            w.ekm[o + i] = (uint8_t)k;
+    ss.kmers += K;
+    ss.entries += E;
+    ss.seeded += 1;
+    ss.bytes += L;
+    wave_sync();
+    // entry gather, one entry per lane: entry j belongs to the k-mer whose
+    // [koff, koff + klen) contains j (binary search over koff)
+    for (uint32_t j = lane; j < E; j += 64) {
+        uint32_t lo = 0, hi = K; // last k with koff[k] <= j and klen[k] > 0
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (w.koff[mid] <= j) lo = mid;
+            else hi = mid;
         }
+        uint32_t k = lo;
+        while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
+        w.ent[j] = ix.kent[(uint64_t)w.klo[k] + (j - w.koff[k])];
+        w.ekm[j] = (uint8_t)k;
     }
-    __syncthreads();
+    wave_sync();
     // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
     for (uint32_t j = lane; j < E; j += 64) {
         const bool skip = j > 0 && w.ekm[j] == w.ekm[j - 1] && w.ent[j] == w.ent[j - 1];
         w.eaux[j] = skip ? 0u : 1u;
     }
-    __syncthreads();
-    // first occurrence and total count of every kept key
+    wave_sync();
+    // first occurrence and total count (u16) of every kept key
     if (w.hsize == 0) {
         for (uint32_t j = lane; j < E; j += 64) {
             if (!(w.eaux[j] & 1)) continue;
@@ -448,7 +556,6 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
                     if (i < j) first = false;
                 }
             }
-            // count is kept in bits 16..31 (u16 as in the reference's map)
             w.eaux[j] = 1u | (first ? 2u : 0u) | ((cnt & 0xFFFFu) << 16);
         }
     } else {
@@ -458,7 +565,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
             const uint32_t key = w.ent[j];
             uint32_t slot = (key * 2654435761u) & hm;
             for (;;) {
-                uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
+                const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
                 if (old == EMPTY || old == key) break;
                 slot = (slot + 1) & hm;
             }
@@ -466,21 +573,20 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
             atomicAdd(&w.hcnt[slot], 1u);
             w.eaux[j] = 1u | (slot << 8);
         }
-        __threadfence_block();
-        __syncthreads();
+        wave_sync();
         for (uint32_t j = lane; j < E; j += 64) {
-            uint32_t a = w.eaux[j];
+            const uint32_t a = w.eaux[j];
             if (!(a & 1)) continue;
             const uint32_t slot = a >> 8;
             const bool first = w.hfirst[slot] == j;
             w.eaux[j] = 1u | (first ? 2u : 0u) | ((w.hcnt[slot] & 0xFFFFu) << 16);
         }
     }
-    __syncthreads();
+    wave_sync();
     // insertion sequences: a key enters its map at its first occurrence if
     // that occurrence may insert (k-mer 0, or k-mer cIdx+1 with cIdx < maxQPos)
     const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
-    uint32_t nins0 = 0, nins1 = 0;
+    uint32_t nins[2] = {0, 0};
     for (uint32_t j0 = 0; j0 < E; j0 += 64) {
         const uint32_t j = j0 + lane;
         bool take0 = false, take1 = false;
@@ -498,124 +604,118 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
             }
         }
         const uint64_t b0 = __ballot(take0), b1 = __ballot(take1);
-        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const uint64_t below = lanes_below(lane);
         if (take0) {
-            uint32_t p = nins0 + __popcll(b0 & below);
+            const uint32_t p = nins[0] + __popcll(b0 & below);
             if (p < w.cap_ins) {
-                w.ins_key[0][p] = key;
-                w.ins_cnt[0][p] = cnt;
+                w.ins_key[p] = key;
+                w.ins_cnt[p] = (uint16_t)cnt;
             }
         }
         if (take1) {
-            uint32_t p = nins1 + __popcll(b1 & below);
+            const uint32_t p = nins[1] + __popcll(b1 & below);
             if (p < w.cap_ins) {
-                w.ins_key[1][p] = key;
-                w.ins_cnt[1][p] = cnt;
+                w.ins_key[w.cap_ins + p] = key;
+                w.ins_cnt[w.cap_ins + p] = (uint16_t)cnt;
             }
         }
-        nins0 += __popcll(b0);
-        nins1 += __popcll(b1);
+        nins[0] += __popcll(b0);
+        nins[1] += __popcll(b1);
     }
-    if (nins0 > w.cap_ins || nins1 > w.cap_ins) return 1;
-    __syncthreads();
-    // hopscotch placement, one lane per map
-    if (lane == 0) sh.fail = 0;
-    __syncthreads();
-    if (lane < 2) {
-        Hs h;
-        h.tb = w.tb[lane];
+    if (nins[0] > w.cap_ins || nins[1] > w.cap_ins) return 1;
+    wave_sync();
+    // hopscotch placement of each map, then its windows with count >= QTHRESH
+    // in iteration order (buckets, then the overflow list)
+    uint32_t ncand[2];
+#pragma unroll
+    for (int mp = 0; mp < 2; ++mp) {
+        Hs<IdxT> h;
+        h.tb = w.tb;
         h.tb_cap = w.tb_cap;
-        h.over = w.over[lane];
-        h.over_cap = w.cap_list;
-        h.tmp = w.tmp[lane];
-        h.tmp_cap = w.cap_list;
-        h.key = w.ins_key[lane];
+        h.over = w.over;
+        h.over_cap = w.cap_ins;
+        h.tmp = w.tmp;
+        h.tmp_cap = w.cap_ins;
+        h.key = w.ins_key + mp * w.cap_ins;
         h.nelem = h.nover = 0;
-        const uint32_t ni = lane ? nins1 : nins0;
-        bool ok = hs_setup(h, bucketCount);
-        for (uint32_t i = 0; ok && i < ni; ++i) ok = hs_insert(h, i) == HS_OK;
+        if (!hs_setup(h, bucketCount, lane)) return 1;
+        for (uint32_t i = 0; i < nins[mp]; ++i)
+            if (hs_insert(h, i, lane) != HS_OK) return 1;
+        const uint16_t* cnt = w.ins_cnt + mp * w.cap_ins;
+        uint32_t* cand = w.cand + mp * w.cap_ins;
         uint32_t nc = 0;
-        if (ok) {
-            // iteration order: buckets, then the overflow list; keep the
-            // windows passing the q-gram threshold (count is a u16)
-            for (uint64_t b = 0; b < h.arr; ++b) {
-                uint32_t i = h.tb[b];
-                if (i != EMPTY && w.ins_cnt[lane][i] >= QTHRESH) h.tmp[nc++] = w.ins_key[lane][i];
-            }
-            for (uint32_t t = 0; t < h.nover; ++t) {
-                uint32_t i = h.over[t];
-                if (w.ins_cnt[lane][i] >= QTHRESH) h.tmp[nc++] = w.ins_key[lane][i];
-            }
-        } else {
-            atomicOr(&sh.fail, 1u);
+        for (uint64_t b0 = 0; b0 < h.arr + h.nover; b0 += 64) {
+            const uint64_t b = b0 + lane;
+            IdxT i = Hs<IdxT>::NONE;
+            if (b < h.arr) i = h.tb[b];
+            else if (b < h.arr + h.nover) i = h.over[b - h.arr];
+            const bool keep = i != Hs<IdxT>::NONE && cnt[i] >= QTHRESH;
+            const uint64_t bal = __ballot(keep);
+            if (keep) cand[nc + __popcll(bal & lanes_below(lane))] = h.key[i];
+            nc += __popcll(bal);
         }
-        sh.ncand[lane] = nc;
+        ncand[mp] = nc;
+        wave_sync();
     }
-    __syncthreads();
-    if (sh.fail) return 1;
     // emit candidates: fwd-map windows then rev-map windows
-    const uint32_t nf = sh.ncand[0], nr = sh.ncand[1];
+    const uint32_t nf = ncand[0], nr = ncand[1];
     const uint32_t pool = (L + 63) / 64 - 1;
+    uint32_t base = 0;
+    if (nf + nr) {
+        base = pool_take(bd, pc, pool, nf + nr, exact, lane);
+        if (base == ~0u) {
+            if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
+            base = 0;
+        } else {
+            uint32_t* jm = pc_meta(bd, pool);
+            uint32_t* jr = bd.job_ro[0];
+#pragma unroll
+            for (uint32_t q = 1; q < FAME_NPOOL; ++q) jr = pool == q ? bd.job_ro[q] : jr;
+            for (uint32_t t = lane; t < nf + nr; t += 64) {
+                const bool isrev = t >= nf;
+                const uint32_t id = isrev ? w.cand[w.cap_ins + (t - nf)] : w.cand[t];
+                jm[base + t] = id | (isrev ? 0x80000000u : 0u);
+                jr[base + t] = ro;
+            }
+        }
+    }
     if (lane == 0) {
-        uint32_t base = (nf + nr) ? atomicAdd(&bd.pool_n[pool], nf + nr) : 0;
-        if (base + nf + nr > bd.pool_cap[pool]) atomicOr(bd.flags, FLAG_POOL_FULL);
-        sh.base = base;
         bd.cand_off[ro] = base;
         bd.cand_nf[ro] = nf;
         bd.cand_nr[ro] = nr;
     }
-    __syncthreads();
-    const uint32_t base = sh.base;
-    if (base + nf + nr <= bd.pool_cap[pool]) {
-        for (uint32_t t = lane; t < nf + nr; t += 64) {
-            const bool isrev = t >= nf;
-            const uint32_t id = isrev ? w.tmp[1][t - nf] : w.tmp[0][t];
-            bd.job_meta[pool][base + t] = id | (isrev ? 0x80000000u : 0u);
-            bd.job_ro[pool][base + t] = ro;
-        }
-    }
     return 0;
 }
 
-constexpr uint32_t L_CAP_E = 256;   // entries per (read, orientation) in LDS
-constexpr uint32_t L_CAP_INS = 256; // keys per map in LDS
-constexpr uint32_t L_TB = 512 + NBH - 1;
+constexpr uint32_t L_CAP_E = 256;   // index hits per (read, orientation) in LDS
+constexpr uint32_t L_CAP_INS = 128; // keys per map in LDS
+constexpr uint32_t L_TB = 256 + NBH - 1;
 
 template <typename TabT>
 __global__ __launch_bounds__(64) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
 {
     __shared__ SeedShared sh;
-    __shared__ uint64_t s_klo[FAME_DEV_MAX_READ];
+    __shared__ TabT s_klo[FAME_DEV_MAX_READ];
     __shared__ uint32_t s_klen[FAME_DEV_MAX_READ];
     __shared__ uint32_t s_koff[FAME_DEV_MAX_READ];
-    // phase A arrays and phase B arrays share storage
+    // gather / count arrays and hopscotch arrays share storage
     __shared__ union {
         struct {
             uint32_t ent[L_CAP_E];
-            uint8_t ekm[L_CAP_E];
             uint32_t eaux[L_CAP_E];
+            uint8_t ekm[L_CAP_E];
         } a;
         struct {
-            uint32_t tb[2][L_TB];
-            uint32_t tmp[2][L_CAP_INS];
-            uint32_t over[2][L_CAP_INS];
+            uint16_t tb[L_TB];
+            uint16_t tmp[L_CAP_INS];
+            uint16_t over[L_CAP_INS];
         } b;
     } u;
-    __shared__ uint32_t s_ins_key[2][L_CAP_INS];
-    __shared__ uint32_t s_ins_cnt[2][L_CAP_INS];
+    __shared__ uint32_t s_ins_key[2 * L_CAP_INS];
+    __shared__ uint16_t s_ins_cnt[2 * L_CAP_INS];
+    __shared__ uint32_t s_cand[2 * L_CAP_INS];
 
-    const uint32_t ro = blockIdx.x;
-    if (ro >= n_ro) return;
-    const uint32_t r = ro >> 1, ori = ro & 1;
-    if (!((bd.ori_mask >> ori) & 1) || bd.pre[r] != 0) {
-        if (threadIdx.x == 0) {
-            bd.cand_nf[ro] = 0;
-            bd.cand_nr[ro] = 0;
-            bd.cand_off[ro] = 0;
-        }
-        return;
-    }
-    SeedWork w;
+    SeedWork<TabT, uint16_t> w;
     w.klo = s_klo;
     w.klen = s_klen;
     w.koff = s_koff;
@@ -625,34 +725,53 @@ __global__ __launch_bounds__(64) void k_seed_light(DevIndex ix, BatchDev bd, uin
     w.cap_e = L_CAP_E;
     w.hsize = 0;
     w.hkey = w.hfirst = w.hcnt = nullptr;
-    w.ins_key[0] = s_ins_key[0];
-    w.ins_key[1] = s_ins_key[1];
-    w.ins_cnt[0] = s_ins_cnt[0];
-    w.ins_cnt[1] = s_ins_cnt[1];
+    w.ins_key = s_ins_key;
+    w.ins_cnt = s_ins_cnt;
     w.cap_ins = L_CAP_INS;
-    w.tb[0] = u.b.tb[0];
-    w.tb[1] = u.b.tb[1];
+    w.tb = u.b.tb;
     w.tb_cap = L_TB;
-    w.tmp[0] = u.b.tmp[0];
-    w.tmp[1] = u.b.tmp[1];
-    w.over[0] = u.b.over[0];
-    w.over[1] = u.b.over[1];
-    w.cap_list = L_CAP_INS;
-    uint64_t E;
-    uint32_t bc;
-    int rc = seed_one<TabT>(ix, bd, ro, sh, w, &E, &bc);
-    if (rc != 0 && threadIdx.x == 0) {
-        uint32_t h = atomicAdd(bd.heavy_n, 1u);
-        if (h < bd.heavy_cap) {
-            bd.heavy_ro[h] = ro;
-            bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
-            bd.heavy_bc[h] = bc;
-        } else {
-            atomicOr(bd.flags, FLAG_HEAVY_FULL);
+    w.tmp = u.b.tmp;
+    w.over = u.b.over;
+    w.cand = s_cand;
+    const int lane = threadIdx.x;
+    seed_table_init(sh, lane);
+    PoolChunk pc = {0, 0, 0};
+    SeedStats ss = {0, 0, 0, 0};
+    // persistent waves: (read, orientation) pairs in grid-stride order
+    for (uint32_t ro = blockIdx.x; ro < n_ro; ro += gridDim.x) {
+        const uint32_t r = ro >> 1, ori = ro & 1;
+        if (!((bd.ori_mask >> ori) & 1) || bd.pre[r] != 0) {
+            if (lane == 0) {
+                bd.cand_nf[ro] = 0;
+                bd.cand_nr[ro] = 0;
+                bd.cand_off[ro] = 0;
+            }
+            continue;
         }
-        bd.cand_nf[ro] = 0;
-        bd.cand_nr[ro] = 0;
-        bd.cand_off[ro] = 0;
+        wave_sync();
+        uint64_t E;
+        uint32_t bc;
+        const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc);
+        if (rc != 0 && lane == 0) {
+            const uint32_t h = atomicAdd(bd.heavy_n, 1u);
+            if (h < bd.heavy_cap) {
+                bd.heavy_ro[h] = ro;
+                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_bc[h] = bc;
+            } else {
+                atomicOr(bd.flags, FLAG_HEAVY_FULL);
+            }
+            bd.cand_nf[ro] = 0;
+            bd.cand_nr[ro] = 0;
+            bd.cand_off[ro] = 0;
+        }
+    }
+    if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
+    if (lane == 0 && ss.seeded) {
+        atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
+        atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
+        atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
+        atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
     }
 }
 
@@ -665,7 +784,6 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     if (hi >= hs.count) return;
     const uint32_t ro = bd.heavy_ro[hs.first + hi];
     uint8_t* base = hs.base + (size_t)hi * hs.stride;
-    SeedWork w;
     size_t o = 0;
     auto take = [&](size_t bytes) {
         uint8_t* p = base + o;
@@ -673,7 +791,8 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         return p;
     };
     const uint32_t ce = hs.cap_e;
-    w.klo = (uint64_t*)take(8 * FAME_DEV_MAX_READ);
+    SeedWork<TabT, uint32_t> w;
+    w.klo = (TabT*)take(8 * FAME_DEV_MAX_READ);
     w.klen = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
     w.koff = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
     w.ent = (uint32_t*)take(4ull * ce);
@@ -685,28 +804,31 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.hfirst = (uint32_t*)take(4ull * hs.hsize);
     w.hcnt = (uint32_t*)take(4ull * hs.hsize);
     w.cap_ins = ce;
-    w.ins_key[0] = (uint32_t*)take(4ull * ce);
-    w.ins_key[1] = (uint32_t*)take(4ull * ce);
-    w.ins_cnt[0] = (uint32_t*)take(4ull * ce);
-    w.ins_cnt[1] = (uint32_t*)take(4ull * ce);
+    w.ins_key = (uint32_t*)take(8ull * ce);
+    w.ins_cnt = (uint16_t*)take(4ull * ce);
     w.tb_cap = hs.tb_cap;
-    w.tb[0] = (uint32_t*)take(4ull * hs.tb_cap);
-    w.tb[1] = (uint32_t*)take(4ull * hs.tb_cap);
-    w.cap_list = ce;
-    w.tmp[0] = (uint32_t*)take(4ull * ce);
-    w.tmp[1] = (uint32_t*)take(4ull * ce);
-    w.over[0] = (uint32_t*)take(4ull * ce);
-    w.over[1] = (uint32_t*)take(4ull * ce);
+    w.tb = (uint32_t*)take(4ull * hs.tb_cap);
+    w.tmp = (uint32_t*)take(4ull * ce);
+    w.over = (uint32_t*)take(4ull * ce);
+    w.cand = (uint32_t*)take(8ull * ce);
+    seed_table_init(sh, threadIdx.x);
     for (uint32_t i = threadIdx.x; i < hs.hsize; i += 64) {
         w.hkey[i] = EMPTY;
         w.hfirst[i] = 0xFFFFFFFFu;
         w.hcnt[i] = 0;
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync();
     uint64_t E;
     uint32_t bc;
-    int rc = seed_one<TabT>(ix, bd, ro, sh, w, &E, &bc);
+    PoolChunk pc = {0, 0, 0};
+    SeedStats ss = {0, 0, 0, 0};
+    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc);
+    if (threadIdx.x == 0 && ss.seeded) {
+        atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
+        atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
+        atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
+        atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
+    }
     if (rc != 0 && threadIdx.x == 0) {
         atomicOr(bd.flags, FLAG_CAPACITY);
         bd.cand_nf[ro] = 0;
@@ -718,17 +840,44 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
 // k_verify: approximate matching of one read over one window per lane
 // ---------------------------------------------------------------------------
 
-// lmap ReadQueue.cpp:73-76 (A0 C1 G2 T3), 4 = N / other
-__device__ __forceinline__ uint32_t base_code(uint8_t c)
+// Myers / Hyyro bit-vector step for one text letter (pattern bit i = pattern
+// character i; the score is D[m][j], the edit distance of the whole pattern
+// to the best substring ending at this text position).  It equals the lowest
+// layer of ShiftAnd<6>'s automaton (ShiftAnd.h:276-341) holding the accepting
+// state, whenever that distance is <= 6.
+template <int W>
+__device__ __forceinline__ void myers_step(uint64_t (&Pv)[W], uint64_t (&Mv)[W], const uint64_t* __restrict__ peq,
+                                           int lastw, uint64_t lastbit, int& score)
 {
-    return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 3u : 4u;
+    uint64_t carry = 0, phc = 0, mhc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t eq = peq[w * 256];
+        const uint64_t pv = Pv[w], mv = Mv[w];
+        const uint64_t xv = eq | mv;
+        const uint64_t t = eq & pv;
+        const uint64_t s1 = t + pv;
+        const uint64_t c1 = s1 < t;
+        const uint64_t s2 = s1 + carry;
+        carry = c1 | (uint64_t)(s2 < s1);
+        const uint64_t xh = (s2 ^ pv) | eq;
+        const uint64_t ph = mv | ~(xh | pv);
+        const uint64_t mh = pv & xh;
+        if (w == lastw) score += (ph & lastbit) ? 1 : ((mh & lastbit) ? -1 : 0);
+        const uint64_t ph2 = (ph << 1) | phc;
+        const uint64_t mh2 = (mh << 1) | mhc;
+        phc = ph >> 63;
+        mhc = mh >> 63;
+        Pv[w] = mh2 | ~(xv | ph2);
+        Mv[w] = ph2 & xv;
+    }
 }
 
 // returns the number of window bytes scanned (0 when the lane has no job)
 template <int W>
 __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
                                                const uint32_t* __restrict__ only, uint32_t n_only,
-                                               uint64_t* peq)
+                                               uint64_t* __restrict__ s_peq)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t njobs = only ? n_only : bd.pool_n[pool];
@@ -736,22 +885,43 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     if (only && bd.ovf_pool[tid] != pool) return 0;
     const uint32_t j = only ? only[tid] : tid;
     const uint32_t meta = bd.job_meta[pool][j];
+    if (meta == JOB_GAP) return 0;
     const uint32_t ro = bd.job_ro[pool][j];
     const uint32_t r = ro >> 1, ori = ro & 1;
     const bool rev = (meta >> 31) != 0;
     const uint32_t id = meta & 0x7FFFFFFFu;
     const uint32_t m = bd.lens[r];
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-    // Peq masks of the pattern (ShiftAnd loadBitmasks ShiftAnd.h:345-432):
-    // pattern T also matches text C (bisulfite asymmetry)
+    // Peq masks of the pattern, laid out [letter*W + word][lane] so that the
+    // lanes of a wave read consecutive LDS words (ShiftAnd loadBitmasks
+    // ShiftAnd.h:345-432: pattern T also matches text C)
+    uint64_t* peq = s_peq + threadIdx.x;
+    {
+        uint64_t pm[4][W];
 #pragma unroll
-    for (int i = 0; i < 4 * W; ++i) peq[i] = 0;
-    for (uint32_t i = 0; i < m; ++i) {
-        const uint32_t c = base_code(pat[i]);
-        const uint64_t bit = 1ull << (i & 63);
-        const uint32_t wi = i >> 6;
-        if (c < 4) peq[c * W + wi] |= bit;
-        if (c == 3) peq[1 * W + wi] |= bit;
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int w = 0; w < W; ++w) pm[c][w] = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t a = 0, cc = 0, gg = 0, tt = 0;
+            for (uint32_t i = 64 * w; i < m && i < 64u * (w + 1); ++i) {
+                const uint8_t ch = pat[i];
+                const uint64_t bit = 1ull << (i & 63);
+                a |= ch == 'A' ? bit : 0;
+                cc |= (ch == 'C' || ch == 'T') ? bit : 0;
+                gg |= ch == 'G' ? bit : 0;
+                tt |= ch == 'T' ? bit : 0;
+            }
+            pm[0][w] = a;
+            pm[1][w] = cc;
+            pm[2][w] = gg;
+            pm[3][w] = tt;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int w = 0; w < W; ++w) peq[(c * W + w) * 256] = pm[c][w];
     }
     const uint4 win = ix.win[id];
     const uint32_t startPos = win.x, chrom = win.y;
@@ -787,72 +957,68 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     uint32_t nm = 0;
     const uint32_t cap = bd.match_cap;
     uint32_t* outm = bd.job_match[pool] + (size_t)j * cap;
-    uint32_t* ovf = nullptr;
-    if (only) ovf = bd.ovf_buf + bd.job_ovf_off[pool][j];
-    for (uint64_t q = 0; q < tlen; ++q) {
-        const uint64_t p = rev ? tlen - 1 - q : q;
-        uint32_t c = base_code(text[p]);
-        if (c == 4) {
+    uint32_t* ovf = only ? bd.ovf_buf + bd.job_ovf_off[pool][j] : nullptr;
+    // the text is read 16 bytes at a time from 16-byte aligned addresses
+    // (the sequence buffer is padded so that the last chunk stays inside it)
+    const uintptr_t ta = (uintptr_t)text;
+    const uintptr_t a0 = ta & ~(uintptr_t)15;
+    const int64_t skip = (int64_t)(ta - a0);
+    const int64_t nchunk = (skip + (int64_t)tlen + 15) >> 4;
+    for (int64_t ci = 0; ci < nchunk; ++ci) {
+        const int64_t chunk = rev ? nchunk - 1 - ci : ci;
+        const uint4 v = *(const uint4*)(a0 + 16 * chunk);
+        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                Pv[w] = ~0ull;
-                Mv[w] = 0;
+        for (int ii = 0; ii < 16; ++ii) {
+            const int i = rev ? 15 - ii : ii;
+            const int64_t p = 16 * chunk + i - skip; // position in the text
+            if (p < 0 || p >= (int64_t)tlen) continue;
+            const uint32_t b = (wd[i >> 2] >> (8 * (i & 3))) & 0xFF;
+            if (b == 'N') { // the automaton restarts (ShiftAnd.h:142-146, 212-214)
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    Pv[w] = ~0ull;
+                    Mv[w] = 0;
+                }
+                score = (int)m;
+                continue;
             }
-            score = (int)m;
-            continue;
-        }
-        if (rev) c = 3 - c; // complement (ShiftAnd.h:197-211)
-        uint64_t carry = 0, phc = 0, mhc = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const uint64_t eq = peq[c * W + w];
-            const uint64_t pv = Pv[w], mv = Mv[w];
-            const uint64_t xv = eq | mv;
-            const uint64_t t = eq & pv;
-            uint64_t s1 = t + pv;
-            const uint64_t c1 = s1 < t;
-            const uint64_t s2 = s1 + carry;
-            carry = c1 | (uint64_t)(s2 < s1);
-            const uint64_t xh = (s2 ^ pv) | eq;
-            const uint64_t ph = mv | ~(xh | pv);
-            const uint64_t mh = pv & xh;
-            if (w == lastw) score += (ph & lastbit) ? 1 : ((mh & lastbit) ? -1 : 0);
-            const uint64_t ph2 = (ph << 1) | phc;
-            const uint64_t mh2 = (mh << 1) | mhc;
-            phc = ph >> 63;
-            mhc = mh >> 63;
-            Pv[w] = mh2 | ~(xv | ph2);
-            Mv[w] = ph2 & xv;
-        }
-        ++ncomp;
-        if (ncomp < (uint64_t)m - NERR) continue;
-        if (score <= (int)NERR) {
-            const uint32_t e = (uint32_t)score;
-            // reported end: querySeq it-start+MISCOUNT; queryRevSeq it-end+pLen-1+MISCOUNT
-            const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
-            const uint32_t packed = (pos << 8) | e;
-            if (was) {
-                if (e <= prevErrs) {
-                    const uint32_t k = nm - 1;
-                    if (ovf) ovf[k] = packed;
-                    else if (k < cap) outm[k] = packed;
+            // A0 C1 G2 T3 from the ASCII code; complemented when scanning
+            // the reverse strand (ShiftAnd.h:197-211)
+            uint32_t c = ((b >> 1) ^ (b >> 2)) & 3;
+            if (rev) c = 3 - c;
+            myers_step<W>(Pv, Mv, peq + c * W * 256, lastw, lastbit, score);
+            ++ncomp;
+            if (ncomp < (uint64_t)m - NERR) continue;
+            if (score <= (int)NERR) {
+                const uint32_t e = (uint32_t)score;
+                // reported end: querySeq it-start+MISCOUNT,
+                // queryRevSeq it-end+pLen-1+MISCOUNT
+                const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
+                const uint32_t packed = (pos << 8) | e;
+                if (was) {
+                    if (e <= prevErrs) {
+                        const uint32_t k = nm - 1;
+                        if (ovf) ovf[k] = packed;
+                        else if (k < cap) outm[k] = packed;
+                        prevErrs = e;
+                    }
+                } else {
+                    if (ovf) ovf[nm] = packed;
+                    else if (nm < cap) outm[nm] = packed;
+                    ++nm;
+                    was = true;
                     prevErrs = e;
                 }
             } else {
-                if (ovf) ovf[nm] = packed;
-                else if (nm < cap) outm[nm] = packed;
-                ++nm;
-                was = true;
-                prevErrs = e;
+                was = false;
             }
-        } else {
-            was = false;
         }
     }
     if (!only) {
         bd.job_nm[pool][j] = nm;
         if (nm > cap) {
-            uint32_t o = atomicAdd(bd.ovf_n, 1u);
+            const uint32_t o = atomicAdd(bd.ovf_n, 1u);
             if (o < bd.ovf_list_cap) {
                 bd.ovf_pool[o] = pool;
                 bd.ovf_job[o] = j;
@@ -868,11 +1034,11 @@ template <int W>
 __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
                                                 const uint32_t* __restrict__ only, uint32_t n_only)
 {
-    __shared__ uint64_t s_peq[256][4 * W];
+    __shared__ uint64_t s_peq[4 * W * 256];
     __shared__ unsigned long long s_work[2];
     if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t t = verify_one<W>(ix, bd, pool, only, n_only, s_peq[threadIdx.x]);
+    const uint64_t t = verify_one<W>(ix, bd, pool, only, n_only, s_peq);
     if (t && !only) {
         atomicAdd(&s_work[0], 1ull);
         atomicAdd(&s_work[1], (unsigned long long)t);
@@ -966,6 +1132,7 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
     __shared__ uint32_t s_cnt[5];
     if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
     __syncthreads();
+    int want = 0;
     if (r < n) {
         uint8_t st = 3; // invalid, no counter
         uint64_t out = 0;
@@ -1025,6 +1192,28 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
         bd.use[r] = use;
         bd.mat[r] = out;
         bd.valid[r] = st == 0;
+        // computeMethLvl returns at once for windows without CpGs (:3287);
+        // the others are queued by branch (wave-aggregated appends)
+        if (st == 0 && ix.win[(uint32_t)(out >> 32)].z != CPGDUMMY) {
+            if (((out >> 16) & 0xFF) == 0)
+                want = 1;
+            else
+                want = 2;
+        }
+    }
+    {
+        const int lane = threadIdx.x & 63;
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2);
+        uint32_t basedir = 0, basedp = 0;
+        if (lane == 0) {
+            if (bdir) basedir = atomicAdd(bd.direct_n, (uint32_t)__popcll(bdir));
+            if (bdp) basedp = atomicAdd(bd.dp_n, (uint32_t)__popcll(bdp));
+        }
+        basedir = __shfl(basedir, 0, 64);
+        basedp = __shfl(basedp, 0, 64);
+        if (want == 1) bd.direct_list[basedir + __popcll(bdir & below)] = r;
+        if (want == 2) bd.dp_list[basedp + __popcll(bdp & below)] = r;
     }
     __syncthreads();
     if (threadIdx.x < 5 && s_cnt[threadIdx.x])
@@ -1032,7 +1221,10 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
 }
 
 // ---------------------------------------------------------------------------
-// k_meth: computeMethLvl
+// k_meth: computeMethLvl (ReadQueue.cpp:3278-3947), split by branch:
+// k_meth_direct serves reads matched without errors (direct CpG lookup),
+// k_meth_dp the others (banded DP + backtrace + CpG walk).  k_select queues
+// both lists.
 // ---------------------------------------------------------------------------
 
 // CompiFwd / CompiRev ReadQueue.h:295-338
@@ -1052,31 +1244,6 @@ __device__ __forceinline__ int cmp_rev(uint8_t r, uint8_t g)
     }
 }
 
-// BandedMatrix<uint16_t, 9>::operator() BandedMatrix.h:67-75, stored per
-// lane with a stride so that the lanes of a wave touch consecutive words
-struct DpMat {
-    uint16_t* base;
-    size_t stride;
-    long rows;
-    __device__ __forceinline__ long idx(long r, long c) const
-    {
-        const long bnd = DPBAND + 1;
-        return r * (2 * bnd + 1) + ((bnd - r > 0) ? bnd - r : 0) + (c - r + bnd);
-    }
-    __device__ __forceinline__ uint16_t get(long r, long c) const
-    {
-        long i = idx(r, c);
-        if (i < 0 || i >= rows * (2 * (DPBAND + 1) + 1)) return 0xFFFF;
-        return base[(size_t)i * stride];
-    }
-    __device__ __forceinline__ void put(long r, long c, uint16_t v) const
-    {
-        long i = idx(r, c);
-        if (i < 0 || i >= rows * (2 * (DPBAND + 1) + 1)) return;
-        base[(size_t)i * stride] = v;
-    }
-};
-
 // *(colPat - j + 1) with colPat = chromosome + ref_end; bytes outside the
 // chromosome read as 0
 __device__ __forceinline__ uint8_t col_at(const uint8_t* cs, uint64_t clen, int64_t ref_end, long j)
@@ -1085,61 +1252,104 @@ __device__ __forceinline__ uint8_t col_at(const uint8_t* cs, uint64_t clen, int6
     return (p < 0 || (uint64_t)p >= clen) ? 0 : cs[p];
 }
 
-__global__ __launch_bounds__(256) void k_meth(DevIndex ix, BatchDev bd, uint32_t n, uint16_t* dp_scratch,
-                                              size_t dp_stride, uint8_t* trace_scratch)
+__global__ __launch_bounds__(256) void k_meth_direct(DevIndex ix, BatchDev bd)
 {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t gsz = gridDim.x * blockDim.x;
-    const uint32_t RL = ix.readlen;
-    const uint32_t RL2 = RL - 2;
-    for (uint32_t r = gid; r < n; r += gsz) {
-        if (bd.status[r] != 0) continue;
+    const uint32_t n = *bd.direct_n;
+    const uint32_t RL = ix.readlen, RL2 = RL - 2;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t r = bd.direct_list[t];
         const uint64_t mat = bd.mat[r];
         const bool isFwd = (mat >> 24) & 1;
         const uint32_t metaID = (uint32_t)(mat >> 32);
         const uint32_t offset = (uint32_t)(mat & 0xFFFF);
-        const uint32_t errNum = (uint32_t)((mat >> 16) & 0xFF);
         const uint4 m = ix.win[metaID];
-        if (m.z == CPGDUMMY) continue;
         const uint32_t L = bd.lens[r];
         const uint8_t* seq = (bd.use[r] == 1 ? bd.seqs : bd.rev) + bd.offs[r];
         const uint32_t metaPos = m.x;
         const uint32_t minPos = (uint32_t)((uint64_t)metaPos + offset - ((uint64_t)L - 1));
         const uint32_t maxPos = metaPos + offset;
         uint32_t* meth = bd.meth;
-        if (errNum == 0) {
-            for (uint32_t id = m.z; id <= m.w; ++id) {
-                const uint32_t pos = ix.cpg_pos[id];
-                if (pos + RL2 < minPos) continue;
-                if (isFwd) {
-                    if (pos + RL2 > maxPos) break;
-                } else {
-                    if (pos + RL - 1 > maxPos) break;
-                }
-                const uint32_t trueCpGPos = pos - 1 + RL - 1;
-                const uint32_t readCpGPos =
-                    (uint32_t)((uint64_t)L - (uint64_t)(uint32_t)(metaPos + offset - trueCpGPos) - 1);
-                if (isFwd) {
-                    const uint32_t k = readCpGPos + MISCOUNT;
-                    if ((uint64_t)k >= L) continue;
-                    const uint8_t ch = seq[k];
-                    if (ch == 'T')
-                        atomicAdd(&meth[4ull * id + 1], 1u);
-                    else if (ch == 'C')
-                        atomicAdd(&meth[4ull * id + 0], 1u);
-                } else {
-                    if ((uint64_t)(uint32_t)(readCpGPos + 1 + MISCOUNT + 2) > L) continue;
-                    const uint8_t ch = seq[(uint64_t)L - readCpGPos - 1 - MISCOUNT - 2];
-                    if (ch == 'T')
-                        atomicAdd(&meth[4ull * id + 3], 1u);
-                    else if (ch == 'C')
-                        atomicAdd(&meth[4ull * id + 2], 1u);
-                }
+        // errNum == 0 branch :3355-3484
+        for (uint32_t id = m.z; id <= m.w; ++id) {
+            const uint32_t pos = ix.cpg_pos[id];
+            if (pos + RL2 < minPos) continue;
+            if (isFwd) {
+                if (pos + RL2 > maxPos) break;
+            } else {
+                if (pos + RL - 1 > maxPos) break;
             }
-            continue;
+            const uint32_t trueCpGPos = pos - 1 + RL - 1;
+            const uint32_t readCpGPos =
+                (uint32_t)((uint64_t)L - (uint64_t)(uint32_t)(metaPos + offset - trueCpGPos) - 1);
+            if (isFwd) {
+                const uint32_t k = readCpGPos + MISCOUNT;
+                if ((uint64_t)k >= L) continue;
+                const uint8_t ch = seq[k];
+                if (ch == 'T')
+                    atomicAdd(&meth[4ull * id + 1], 1u);
+                else if (ch == 'C')
+                    atomicAdd(&meth[4ull * id + 0], 1u);
+            } else {
+                if ((uint64_t)(uint32_t)(readCpGPos + 1 + MISCOUNT + 2) > L) continue;
+                const uint8_t ch = seq[(uint64_t)L - readCpGPos - 1 - MISCOUNT - 2];
+                if (ch == 'T')
+                    atomicAdd(&meth[4ull * id + 3], 1u);
+                else if (ch == 'C')
+                    atomicAdd(&meth[4ull * id + 2], 1u);
+            }
         }
-        // DP branch ReadQueue.cpp:3625-3912
-        atomicAdd(&bd.stats[5], 1ull);
+    }
+}
+
+// The DP matrix of LevenshtDP<uint16_t, 8> lives in a BandedMatrix<uint16_t, 9>
+// (BandedMatrix.h:67-75): row r, column c at flat index
+// r*19 + max(9-r, 0) + (c-r+9).  Rows 0..8 then occupy [17r+18, 18r+27] with
+// unused slots between them, and cell (8,17) shares its slot with (9,0).
+// Here each lane stores its cells by (row, offset c-r+9) and the backtrace
+// maps the reference's flat index back to the cell holding that slot last
+// (unused slots read as the matrix's fill value 0xFFFF).
+__device__ __forceinline__ uint16_t dp_get(const uint16_t* S, size_t stride, long L, long r, long c)
+{
+    const long idx = r * 19 + ((9 - r) > 0 ? 9 - r : 0) + (c - r + 9);
+    if (idx < 0 || idx >= (L + 1) * 19) return 0xFFFF;
+    long rr, kk;
+    if (idx >= 171) {
+        rr = idx / 19;
+        kk = idx % 19;
+    } else {
+        rr = -1;
+#pragma unroll
+        for (long q = 0; q <= 8; ++q)
+            if (idx >= 17 * q + 18 && idx <= 18 * q + 27) rr = q;
+        if (rr < 0) return 0xFFFF;
+        kk = (idx - 17 * rr - 18) - rr + 9;
+    }
+    return S[(size_t)(rr * 19 + kk) * stride];
+}
+
+__global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint16_t* dp_scratch, size_t stride,
+                                                 uint8_t* trace_scratch)
+{
+    const uint32_t n = *bd.dp_n;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gsz = gridDim.x * blockDim.x;
+    const uint32_t RL = ix.readlen, RL2 = RL - 2;
+    uint16_t* S = dp_scratch + gid;
+    uint8_t* tr = trace_scratch + gid;
+    if (gid == 0) atomicAdd(&bd.stats[5], (unsigned long long)n);
+    for (uint32_t t = gid; t < n; t += gsz) {
+        const uint32_t r = bd.dp_list[t];
+        const uint64_t mat = bd.mat[r];
+        const bool isFwd = (mat >> 24) & 1;
+        const bool rv = !isFwd;
+        const uint32_t metaID = (uint32_t)(mat >> 32);
+        const uint32_t offset = (uint32_t)(mat & 0xFFFF);
+        const uint4 m = ix.win[metaID];
+        const uint32_t L = bd.lens[r];
+        const uint8_t* seq = (bd.use[r] == 1 ? bd.seqs : bd.rev) + bd.offs[r];
+        const uint32_t metaPos = m.x;
+        const uint32_t minPos = (uint32_t)((uint64_t)metaPos + offset - ((uint64_t)L - 1));
+        const uint32_t maxPos = metaPos + offset;
         const uint32_t chrom = ix.cpg_chrom[m.z];
         const uint8_t* cs = ix.seq + ix.coff[chrom];
         const uint64_t clen = ix.clen[chrom];
@@ -1154,75 +1364,101 @@ __global__ __launch_bounds__(256) void k_meth(DevIndex ix, BatchDev bd, uint32_t
                 break;
             }
         }
-        DpMat M{dp_scratch + gid, dp_stride, (long)L + 1};
-        const long band = DPBAND;
-        const long ncell = M.rows * (2 * (band + 1) + 1);
-        for (long i = 0; i < ncell; ++i) M.base[(size_t)i * dp_stride] = 0xFFFF;
-        // border init in the reference's write order (LevenshtDP.h:212-233)
-        M.put(0, 0, 0);
-        for (long c = 1; c <= band; ++c) M.put(0, c, (uint16_t)c);
-        for (long rr = 1; rr <= band; ++rr) M.put(rr, 0, (uint16_t)rr);
-        for (long c = 0; c < (long)(L - band); ++c) M.put(c + band + 1, c, (uint16_t)(0xFFFF - c - band - 1));
-        for (long rr = 0; rr < (long)L; ++rr) M.put(rr, rr + band + 1, (uint16_t)(0xFFFF - rr - band - 1));
-        const bool rv = !isFwd;
-        for (long rr = 1; rr <= (long)L; ++rr)
-            for (long o = -band; o <= band; ++o) {
-                if (rr + o <= 0) continue;
-                const long j = rr + o;
-                const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
-                const uint8_t gc = col_at(cs, clen, ref_end, j);
-                const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
-                int v = (int)M.get(rr - 1, j) + 1;
-                const int v2 = (int)M.get(rr, j - 1) + 1;
-                const int v3 = (int)M.get(rr - 1, j - 1) + mm;
-                v = v2 < v ? v2 : v;
-                v = v3 < v ? v3 : v;
-                M.put(rr, j, (uint16_t)v);
+        // fill (runDPFill / runDPFillRev LevenshtDP.h:207-323): one row at a
+        // time in registers; values are computed in int and kept as u16
+        int prev[19], cur[19];
+        uint8_t g[17]; // reference bytes of columns r-8 .. r+8
+#pragma unroll
+        for (int k = 0; k < 19; ++k) prev[k] = 0xFFFF;
+        prev[9] = 0;
+#pragma unroll
+        for (int o = 1; o <= 8; ++o) prev[9 + o] = o;
+        prev[18] = 0xFFFF - 9; // right band of row 0
+#pragma unroll
+        for (int k = 9; k < 19; ++k) S[(size_t)k * stride] = (uint16_t)prev[k];
+#pragma unroll
+        for (int k = 0; k < 17; ++k) g[k] = (k >= 9) ? col_at(cs, clen, ref_end, k - 8) : 0; // row 0: col k-8
+        for (long rr = 1; rr <= (long)L; ++rr) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) g[k] = g[k + 1];
+            g[16] = col_at(cs, clen, ref_end, rr + 8);
+            const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
+            // left band (offset -9): the first column for rows 1..8 is not at
+            // offset -9; row 9's slot holds the right band of row 8 (65518)
+            cur[0] = rr >= 10 ? (int)(0xFFFF - rr) : (rr == 9 ? 0xFFFF - 8 - 9 : 0xFFFF);
+#pragma unroll
+            for (int k = 1; k < 18; ++k) {
+                const long o = k - 9, c = rr + o;
+                int v;
+                if (c < 0) {
+                    v = 0xFFFF;
+                } else if (c == 0) {
+                    v = (int)rr; // first column (rows 1..8)
+                } else {
+                    const uint8_t gc = g[o + 8];
+                    const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
+                    int a = prev[k + 1] + 1;
+                    const int b = cur[k - 1] + 1;
+                    const int d = prev[k] + mm;
+                    a = b < a ? b : a;
+                    a = d < a ? d : a;
+                    v = a & 0xFFFF;
+                }
+                cur[k] = v;
             }
+            cur[18] = rr < (long)L ? (int)(0xFFFF - rr - 9) : 0xFFFF;
+            const int k0 = rr >= 9 ? 0 : (int)(9 - rr);
+            uint16_t* row = S + (size_t)(rr * 19) * stride;
+#pragma unroll
+            for (int k = 0; k < 19; ++k) {
+                if (k >= k0) row[(size_t)k * stride] = (uint16_t)cur[k];
+                prev[k] = cur[k];
+            }
+        }
         // backtrace (backtrackDP / backtrackDPRev LevenshtDP.h:329-544)
         uint16_t minimum = 0xFFFF;
         unsigned long col = 0;
-        for (long o = -band; o <= band; ++o) {
-            const uint16_t v = M.get((long)L, (long)L + o);
+        for (long o = -DPBAND; o <= DPBAND; ++o) {
+            const uint16_t v = dp_get(S, stride, L, (long)L, (long)L + o);
             if (v < minimum) {
                 col = (unsigned long)((long)L + o);
                 minimum = v;
             }
         }
-        uint8_t* tr = trace_scratch + gid;
-        const size_t tstride = gsz;
         int tn = 0;
         for (long rr = (long)L; rr > 0;) {
             const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
             const uint8_t gc = col_at(cs, clen, ref_end, (long)col);
             const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
             const long c = (long)col;
-            uint8_t t;
-            const uint16_t up = M.get(rr - 1, c), left = M.get(rr, c - 1), dg = M.get(rr - 1, c - 1);
+            uint8_t tt;
+            const uint16_t up = dp_get(S, stride, L, rr - 1, c);
+            const uint16_t left = dp_get(S, stride, L, rr, c - 1);
+            const uint16_t dg = dp_get(S, stride, L, rr - 1, c - 1);
             if (up < left) {
                 if ((int)up + 1 < (int)dg + mm) {
-                    t = 2;
+                    tt = 2; // INSERTION
                     --rr;
                 } else {
-                    t = mm ? 1 : 0;
+                    tt = mm ? 1 : 0;
                     --rr;
                     --col;
                 }
             } else {
                 if ((int)left + 1 < (int)dg + mm) {
-                    t = 3;
+                    tt = 3; // DELETION
                     --col;
                 } else {
-                    t = mm ? 1 : 0;
+                    tt = mm ? 1 : 0;
                     --rr;
                     --col;
                 }
             }
-            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * tstride] = t;
+            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * gsz] = tt;
             ++tn;
         }
         while (col > 0) {
-            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * tstride] = 3;
+            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * gsz] = 3;
             ++tn;
             --col;
         }
@@ -1230,13 +1466,15 @@ __global__ __launch_bounds__(256) void k_meth(DevIndex ix, BatchDev bd, uint32_t
             atomicOr(bd.flags, FLAG_CAPACITY);
             continue;
         }
+        // CpG walk :3690-3912
+        uint32_t* meth = bd.meth;
         uint32_t refSeqPos = metaPos + offset;
         int32_t readSeqPos = (int32_t)L - 1;
         int32_t alignPos = tn - 1;
         for (int32_t id = maxIndex; id >= minIndex; --id) {
             const uint32_t pos = ix.cpg_pos[id];
             while (alignPos >= 0 && pos + RL2 < refSeqPos) {
-                switch (tr[(size_t)alignPos * tstride]) {
+                switch (tr[(size_t)alignPos * gsz]) {
                 case 0:
                 case 1:
                     --readSeqPos;
@@ -1273,6 +1511,7 @@ __global__ __launch_bounds__(256) void k_meth(DevIndex ix, BatchDev bd, uint32_t
     }
 }
 
+// ---------------------------------------------------------------------------
 // exclusive scan of the match counts of overflowed jobs (one thread; the
 // list is empty in practice and short when not)
 __global__ void k_ovf_offsets(BatchDev bd, uint32_t n_ovf, uint32_t* total)
@@ -1328,10 +1567,11 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)
 {
     if (!n_ro) return;
+    const uint32_t g = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
     if (ix.tab_is64)
-        hipLaunchKernelGGL(k_seed_light<uint64_t>, dim3(n_ro), dim3(64), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL(k_seed_light<uint64_t>, dim3(g), dim3(64), 0, st, ix, bd, n_ro);
     else
-        hipLaunchKernelGGL(k_seed_light<uint32_t>, dim3(n_ro), dim3(64), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL(k_seed_light<uint32_t>, dim3(g), dim3(64), 0, st, ix, bd, n_ro);
 }
 
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st)
@@ -1366,7 +1606,9 @@ void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* d
                  uint8_t* trace, uint32_t grid_threads, hipStream_t st)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_meth, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, n, dp, dp_stride, trace);
+    const uint32_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_meth_direct, dim3(g < 8192 ? g : 8192), dim3(256), 0, st, ix, bd);
+    hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dp_stride, trace);
 }
 
 void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st)

@@ -87,7 +87,7 @@ struct fame_ctx {
         b_job_ovf_off[FAME_NPOOL];
     DevBuf b_ovf_n, b_ovf_pool, b_ovf_job, b_ovf_buf, b_ovf_total;
     DevBuf b_heavy_n, b_heavy_ro, b_heavy_e, b_heavy_bc, b_heavy_scratch;
-    DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_trace;
+    DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_trace, b_direct, b_dplist, b_methn;
     uint32_t pool_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t match_cap = 4;
     uint32_t meth_threads = 65536;
@@ -144,6 +144,8 @@ void load_index(fame_ctx* c, const std::string& path)
     c->seedbits = f.get<uint32_t>();
     if (c->kmerlen != 32) throw FameError{FAME_ERR_FORMAT, "index k-mer length is not 32"};
     if (c->htab == 0) throw FameError{FAME_ERR_FORMAT, "index hash table size is 0"};
+    if (c->seedbits != FAME_SEEDBITS)
+        throw FameError{FAME_ERR_FORMAT, "index built with another spaced seed (SEEDBITS)"};
     if (c->readlen <= 32 || c->readlen > FAME_MAX_READ)
         throw FameError{FAME_ERR_FORMAT, "index READLEN outside (32, 255]"};
     hipStream_t st = c->stream;
@@ -360,7 +362,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.cand_nr = grow<uint32_t>(c->b_cand_nr, n_ro + 1);
     bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
     bd.match_cap = c->match_cap;
-    const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n);
+    // room for the slots persistent seed waves leave unused at chunk ends
+    const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n) + 64u * FAME_SEED_WAVES;
     for (int p = 0; p < FAME_NPOOL; ++p)
         if (c->pool_cap[p] < want_cap) c->pool_cap[p] = want_cap;
     bd.ovf_n = grow<uint32_t>(c->b_ovf_n, 1);
@@ -381,6 +384,11 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.flags = c->d_flags.as<uint32_t>();
     bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
     HIPC(hipMemsetAsync(bd.stats, 0, 8 * FAME_NSTATS, st));
+    bd.direct_list = grow<uint32_t>(c->b_direct, n + 1);
+    bd.dp_list = grow<uint32_t>(c->b_dplist, n + 1);
+    bd.direct_n = grow<uint32_t>(c->b_methn, 2);
+    bd.dp_n = bd.direct_n + 1;
+    HIPC(hipMemsetAsync(bd.direct_n, 0, 8, st));
 
     tm.mark(0);
     HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
@@ -443,9 +451,12 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             add(emax);
             add(4 * emax);
             for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
-            for (int i = 0; i < 4; ++i) add(4 * emax);
-            for (int i = 0; i < 2; ++i) add(4ull * hs.tb_cap);
-            for (int i = 0; i < 4; ++i) add(4 * emax);
+            add(8 * emax);
+            add(4 * emax);
+            add(4ull * hs.tb_cap);
+            add(4 * emax);
+            add(4 * emax);
+            add(8 * emax);
             hs.stride = stride;
             const size_t budget = 2ull << 30;
             uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(heavy_n, budget / stride));
@@ -507,7 +518,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     launch_select(c->ix, bd, n, get_stranded, st);
     tm.mark(4);
     {
-        const size_t cells = (size_t)(FAME_DEV_MAX_READ + 1) * (2 * (FAME_DPBAND + 1) + 1);
+        const size_t cells = (size_t)(FAME_DEV_MAX_READ + 1) * (2 * (FAME_DPBAND + 1) + 1) + 19;
         uint16_t* dp = grow<uint16_t>(c->b_dp, cells * c->meth_threads);
         uint8_t* tr = grow<uint8_t>(c->b_trace, (size_t)FAME_DEV_TRACE_CAP * c->meth_threads);
         launch_meth(c->ix, bd, n, dp, c->meth_threads, tr, c->meth_threads, st);

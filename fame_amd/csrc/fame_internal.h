@@ -10,6 +10,8 @@
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
 #define FAME_NSTATS 8
+#define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
+#define FAME_SEED_WAVES 16384      // persistent waves of the seed kernel
 #define FAME_DPBAND 8 // 2*MISCOUNT + ADDMIS (ReadQueue.cpp:3640)
 
 namespace fame {
@@ -84,6 +86,11 @@ struct BatchDev {
     uint8_t* valid;
     uint64_t* counters; // succ, nonunique, unsucc, r1fwd, r1rev (accumulated)
     uint32_t* meth;     // 4 x u32 per CpG
+    // matched reads whose window has CpGs: errNum == 0 / errNum > 0
+    uint32_t* direct_list;
+    uint32_t* direct_n;
+    uint32_t* dp_list;
+    uint32_t* dp_n;
     uint32_t* flags;
     // work counters: k-mers hashed, bucket entries gathered, (read,
     // orientation) pairs seeded, verify jobs, window bytes scanned, DP
