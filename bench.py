@@ -136,8 +136,10 @@ def main():
         n = len(lens)
         stage_ms = {}
         work = {}
+        launches = [0]
 
         def call(s, e, stranded, cnt):
+            launches[0] += 1
             ctx.match_device(d_seqs.data_ptr(), d_offs.data_ptr() + 8 * s, d_lens.data_ptr() + 4 * s,
                              e - s, stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s)
             for k, v in ctx.timings().items():
@@ -162,6 +164,7 @@ def main():
             step()
         stage_ms.clear()
         work.clear()
+        launches[0] = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -183,10 +186,13 @@ def main():
         seed_gbs = seed_bytes / (per_stage["seed"] / 1e3) / 1e9 if per_stage.get("seed") else 0.0
         dev_stages = {k: per_stage[k] for k in ("prep", "seed", "verify", "select", "meth")}
         dominant = max(dev_stages, key=dev_stages.get)
+        per_launch = launches[0] / K
         roof = {"kernel": "k_seed_light (seed + q-gram filter)", "bound": "hbm",
                 "achieved": round(seed_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_step": int(seed_bytes), "ms_per_step": round(per_stage["seed"], 3),
+                "launches_per_step": per_launch,
+                "bytes_per_launch": int(seed_bytes / per_launch),
+                "avg_launch_ms": round(per_stage["seed"] / per_launch, 3),
                 "dominant_stage": dominant}
         verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
         out = {

@@ -322,40 +322,48 @@ __device__ int hs_insert(Hs<IdxT>& h, uint32_t idx, int lane)
 // k_prep
 // ---------------------------------------------------------------------------
 
-__global__ void k_prep(const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs,
-                       const uint32_t* __restrict__ lens, uint32_t n, uint32_t readlen,
-                       uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
-                       uint32_t* __restrict__ err_flag)
+// one wavefront per read (persistent, grid-stride): coalesced loads of the
+// read, the reverse complement stored reversed, N found by a ballot
+__global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs,
+                                              const uint32_t* __restrict__ lens, uint32_t n, uint32_t readlen,
+                                              uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
+                                              uint32_t* __restrict__ err_flag)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t L = lens[r];
-    const uint8_t* s = seqs + offs[r];
-    uint8_t* q = rev + offs[r];
-    uint8_t st = 0;
-    // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned subtraction
-    // of the reference is kept
-    if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
-    if (L > FAME_DEV_MAX_READ) {
-        atomicOr(err_flag, 1u);
-        st = 3;
-    }
-    if (L < KMERLEN) st = 3;
-    if (st == 0) {
-        for (uint32_t p = 0; p < L; ++p) {
-            uint8_t c = s[p], o = 0;
-            switch (c) {
-            case 'A': o = 'T'; break;
-            case 'C': o = 'G'; break;
-            case 'G': o = 'C'; break;
-            case 'T': o = 'A'; break;
-            case 'N': st = 3; break;
-            default: break; // unknown characters stay '\0' in revSeq
-            }
-            q[L - 1 - p] = o;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t r = wave; r < n; r += nwave) {
+        const uint32_t L = lens[r];
+        const uint8_t* s = seqs + offs[r];
+        uint8_t* q = rev + offs[r];
+        uint8_t st = 0;
+        // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned
+        // subtraction of the reference is kept
+        if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
+        if (L > FAME_DEV_MAX_READ) {
+            if (lane == 0) atomicOr(err_flag, 1u);
+            st = 3;
         }
+        if (L < KMERLEN) st = 3;
+        if (st == 0) {
+            bool nflag = false;
+            for (uint32_t p = lane; p < L; p += 64) {
+                const uint8_t c = s[p];
+                uint8_t o = 0;
+                switch (c) {
+                case 'A': o = 'T'; break;
+                case 'C': o = 'G'; break;
+                case 'G': o = 'C'; break;
+                case 'T': o = 'A'; break;
+                case 'N': nflag = true; break;
+                default: break; // unknown characters stay '\0' in revSeq
+                }
+                q[L - 1 - p] = o;
+            }
+            if (__ballot(nflag)) st = 3;
+        }
+        if (lane == 0) pre[r] = st;
     }
-    pre[r] = st;
 }
 
 // ---------------------------------------------------------------------------
@@ -1560,8 +1568,9 @@ __global__ void k_meth_u16(const uint32_t* __restrict__ in, uint64_t n, uint16_t
 void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, st, bd.seqs, bd.offs, bd.lens, n,
-                       readlen, bd.rev, bd.pre, bd.flags);
+    const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
+    hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
+                       bd.lens, n, readlen, bd.rev, bd.pre, bd.flags);
 }
 
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)

@@ -323,7 +323,7 @@ T* grow(DevBuf& b, size_t count)
 }
 
 struct Timer {
-    hipEvent_t e[7];
+    hipEvent_t e[16];
     hipStream_t st;
     explicit Timer(hipStream_t s) : st(s)
     {
@@ -390,8 +390,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.dp_n = bd.direct_n + 1;
     HIPC(hipMemsetAsync(bd.direct_n, 0, 8, st));
 
-    tm.mark(0);
     HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    tm.mark(0);
     launch_prep(bd, n, c->readlen, st);
     tm.mark(1);
     // seed: retried with larger pools when a pool overflowed
@@ -405,7 +405,9 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         }
         HIPC(hipMemsetAsync(bd.pool_n, 0, 4 * FAME_NPOOL, st));
         HIPC(hipMemsetAsync(bd.heavy_n, 0, 4, st));
+        tm.mark(2);
         launch_seed_light(c->ix, bd, n_ro, st);
+        tm.mark(3);
         HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&heavy_n, bd.heavy_n, 4, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
@@ -462,11 +464,13 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(heavy_n, budget / stride));
             HIPC(c->b_heavy_scratch.reserve(stride * per));
             hs.base = c->b_heavy_scratch.as<uint8_t>();
+            tm.mark(12);
             for (uint32_t first = 0; first < heavy_n; first += per) {
                 hs.first = first;
                 hs.count = std::min(per, heavy_n - first);
                 launch_seed_heavy(c->ix, bd, hs, st);
             }
+            tm.mark(13);
             HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
             HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
@@ -485,7 +489,6 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
     }
     c->last_heavy = heavy_n;
-    tm.mark(2);
     // verify
     for (int p = 0; p < FAME_NPOOL; ++p) {
         const uint32_t cap = std::max<uint32_t>(pool_n[p], 1);
@@ -495,7 +498,9 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     }
     bd.ovf_buf = grow<uint32_t>(c->b_ovf_buf, 1);
     HIPC(hipMemsetAsync(bd.ovf_n, 0, 4, st));
+    tm.mark(4);
     for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, pool_n[p], nullptr, st);
+    tm.mark(5);
     uint32_t ovf_n = 0;
     HIPC(hipMemcpyAsync(&ovf_n, bd.ovf_n, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
@@ -512,18 +517,21 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         HIPC(hipMemcpyAsync(&total, d_total, 4, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         bd.ovf_buf = grow<uint32_t>(c->b_ovf_buf, total + 1);
+        tm.mark(14);
         for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, ovf_n, bd.ovf_job, st);
+        tm.mark(15);
     }
-    tm.mark(3);
+    tm.mark(6);
     launch_select(c->ix, bd, n, get_stranded, st);
-    tm.mark(4);
+    tm.mark(7);
     {
         const size_t cells = (size_t)(FAME_DEV_MAX_READ + 1) * (2 * (FAME_DPBAND + 1) + 1) + 19;
         uint16_t* dp = grow<uint16_t>(c->b_dp, cells * c->meth_threads);
         uint8_t* tr = grow<uint8_t>(c->b_trace, (size_t)FAME_DEV_TRACE_CAP * c->meth_threads);
+        tm.mark(8);
         launch_meth(c->ix, bd, n, dp, c->meth_threads, tr, c->meth_threads, st);
+        tm.mark(9);
     }
-    tm.mark(5);
     uint64_t cnt[5];
     HIPC(hipMemcpyAsync(cnt, bd.counters, 40, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(c->stats, bd.stats, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
@@ -539,11 +547,12 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     counters[2] += cnt[2];
     HIPC(hipMemsetAsync(bd.counters, 0, 24, st));
     HIPC(hipStreamSynchronize(st));
+    // kernel-only intervals (events bracket the launches, not the host waits)
     c->timings[0] = tm.ms(0, 1);
-    c->timings[1] = tm.ms(1, 2);
-    c->timings[2] = tm.ms(2, 3);
-    c->timings[3] = tm.ms(3, 4);
-    c->timings[4] = tm.ms(4, 5);
+    c->timings[1] = tm.ms(2, 3) + (heavy_n ? tm.ms(12, 13) : 0.0);
+    c->timings[2] = tm.ms(4, 5) + (c->last_ovf ? tm.ms(14, 15) : 0.0);
+    c->timings[3] = tm.ms(6, 7);
+    c->timings[4] = tm.ms(8, 9);
     c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FAME_OK;
 }
