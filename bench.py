@@ -91,7 +91,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by read "
               "chunks with a replicated index, which this build does not launch", file=sys.stderr)
@@ -221,6 +226,20 @@ def main():
         print(json.dumps(out), flush=True)
     finally:
         shutil.rmtree(wd, ignore_errors=True)
+
+
+def copy_probe(mib):
+    """A plain elementwise copy (torch.neg on int32: reads and writes `mib`
+    MiB once each), three launches; used to scale the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB read + {mib} MiB written")
 
 
 def cpu_leg(args, ctx, reads, idx_path, wd, nchrom):

@@ -594,7 +594,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     // insertion sequences: a key enters its map at its first occurrence if
     // that occurrence may insert (k-mer 0, or k-mer cIdx+1 with cIdx < maxQPos)
     const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
-    uint32_t nins[2] = {0, 0};
+    uint32_t nins[2] = {0, 0}, nq[2] = {0, 0};
     for (uint32_t j0 = 0; j0 < E; j0 += 64) {
         const uint32_t j = j0 + lane;
         bool take0 = false, take1 = false;
@@ -629,14 +629,27 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         }
         nins[0] += __popcll(b0);
         nins[1] += __popcll(b1);
+        // windows at or above QTHRESH; the first one of each map is kept
+        // directly, since a map holding at most one such window yields it
+        // whatever the iteration order
+        const bool q0 = take0 && cnt >= QTHRESH, q1 = take1 && cnt >= QTHRESH;
+        const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
+        if (q0 && nq[0] == 0 && (c0 & below) == 0) w.cand[0] = key;
+        if (q1 && nq[1] == 0 && (c1 & below) == 0) w.cand[w.cap_ins] = key;
+        nq[0] += __popcll(c0);
+        nq[1] += __popcll(c1);
     }
-    if (nins[0] > w.cap_ins || nins[1] > w.cap_ins) return 1;
+    if ((nins[0] > w.cap_ins && nq[0] > 1) || (nins[1] > w.cap_ins && nq[1] > 1)) return 1;
     wave_sync();
-    // hopscotch placement of each map, then its windows with count >= QTHRESH
-    // in iteration order (buckets, then the overflow list)
+    // maps with two or more windows at or above QTHRESH: hopscotch placement,
+    // then those windows in iteration order (buckets, then the overflow list)
     uint32_t ncand[2];
 #pragma unroll
     for (int mp = 0; mp < 2; ++mp) {
+        if (nq[mp] <= 1) {
+            ncand[mp] = nq[mp];
+            continue;
+        }
         Hs<IdxT> h;
         h.tb = w.tb;
         h.tb_cap = w.tb_cap;
