@@ -161,11 +161,14 @@ struct Hs {
 
 enum { HS_OK = 0, HS_FAIL = 1, HS_NEED_REHASH = 2 };
 
-// lanes see each other's LDS / global writes (one wavefront per block)
+// the lanes of one wavefront see each other's LDS / global writes: a
+// wavefront-scope release/acquire pair (no instruction on CDNA, an ordering
+// point for the compiler); the waves of a block run independently
 __device__ __forceinline__ void wave_sync()
 {
-    __threadfence_block();
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -382,7 +385,7 @@ struct SeedWork {
     uint32_t* hkey; // heavy path: hash table keyed by packed entry
     uint32_t* hfirst;
     uint32_t* hcnt;
-    uint32_t hsize; // power of two; 0 = pairwise comparison
+    uint32_t hsize; // power of two, >= cap_e
     uint32_t* ins_key; // [2][cap_ins] keys in first-insertion order per map
     uint16_t* ins_cnt; // [2][cap_ins] their u16 counts
     uint32_t cap_ins;
@@ -391,21 +394,23 @@ struct SeedWork {
     IdxT* tmp;         // rehash / output list
     IdxT* over;        // overflow list
     uint32_t* cand;    // [2][cap_ins] candidate windows per map
+    uint8_t* code;     // [L] reduced base codes of the read
 };
 
 struct SeedShared {
     // rot[i][c] = rol(seed_c, 31 - i): the term of position i of a spaced
     // k-mer (nthash.hpp:480-489); code 3 (N and others) contributes 0
     uint64_t rot[KMERLEN][4];
-    uint8_t code[FAME_DEV_MAX_READ + 1];
 };
 
-__device__ __forceinline__ void seed_table_init(SeedShared& sh, int lane)
+// filled by the whole block at kernel start
+__device__ __forceinline__ void seed_table_init(SeedShared& sh)
 {
-    for (int t = lane; t < (int)KMERLEN * 4; t += 64) {
+    for (int t = threadIdx.x; t < (int)KMERLEN * 4; t += blockDim.x) {
         const int i = t >> 2, c = t & 3;
         sh.rot[i][c] = rol64(red_seed((uint32_t)c), (int)(KMERLEN - 1 - i));
     }
+    __syncthreads();
 }
 
 // Returns 0 when done, 1 when the (read, orientation) needs the heavy path
@@ -479,28 +484,42 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
                         const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const uint32_t r = ro >> 1, ori = ro & 1;
     const uint32_t L = bd.lens[r];
     const uint8_t* s = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-    for (uint32_t i = lane; i < L; i += 64) sh.code[i] = (uint8_t)red_code(s[i]);
+    for (uint32_t i = lane; i < L; i += 64) w.code[i] = (uint8_t)red_code(s[i]);
     wave_sync();
     const uint32_t K = L - KMERLEN + 1;
     const TabT* tab = (const TabT*)ix.tab;
     // hashes + tabIndex gathers (NTPS64 closed form; key = h % HTABSIZE)
-    uint32_t bsum = 0;
-    for (uint32_t k = lane; k < K; k += 64) {
-        uint64_t h = 0;
+    // all gathers of a lane are issued before the first is consumed
+    constexpr int KPL = (FAME_DEV_MAX_READ - KMERLEN + 1 + 63) / 64;
+    TabT lo[KPL], hi[KPL];
 #pragma unroll
-        for (uint32_t i = 0; i < KMERLEN; ++i)
-            if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][sh.code[k + i]];
-        const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
-        const TabT lo = tab[key];
-        const TabT hi = tab[key + 1];
-        w.klo[k] = lo;
-        const uint64_t len = (uint64_t)(hi - lo);
-        w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
-        bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = lane + 64 * q;
+        lo[q] = hi[q] = 0;
+        if (k < K) {
+            uint64_t h = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < KMERLEN; ++i)
+                if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][w.code[k + i]];
+            const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
+            lo[q] = tab[key];
+            hi[q] = tab[key + 1];
+        }
+    }
+    uint32_t bsum = 0;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = lane + 64 * q;
+        if (k < K) {
+            w.klo[k] = lo[q];
+            const uint64_t len = (uint64_t)(hi[q] - lo[q]);
+            w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+            bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+        }
     }
     bsum = wave_sum_u32(bsum);
     const uint32_t bucketCount = bsum / 5;
@@ -551,27 +570,22 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         w.eaux[j] = skip ? 0u : 1u;
     }
     wave_sync();
-    // first occurrence and total count (u16) of every kept key
-    if (w.hsize == 0) {
-        for (uint32_t j = lane; j < E; j += 64) {
-            if (!(w.eaux[j] & 1)) continue;
-            const uint32_t key = w.ent[j];
-            bool first = true;
-            uint32_t cnt = 0;
-            for (uint32_t i = 0; i < E; ++i) {
-                if ((w.eaux[i] & 1) && w.ent[i] == key) {
-                    ++cnt;
-                    if (i < j) first = false;
-                }
-            }
-            w.eaux[j] = 1u | (first ? 2u : 0u) | ((cnt & 0xFFFFu) << 16);
-        }
-    } else {
+    // first occurrence and total count (u16) of every kept key: an open
+    // addressing table keyed by the packed entry (hsize, a power of two, is
+    // at least the number of entries, so every probe sequence ends)
+    {
         const uint32_t hm = w.hsize - 1;
+        const int hshift = __clz(w.hsize) + 1;
+        for (uint32_t i = lane; i < w.hsize; i += 64) {
+            w.hkey[i] = EMPTY;
+            w.hfirst[i] = 0xFFFFFFFFu;
+            w.hcnt[i] = 0;
+        }
+        wave_sync();
         for (uint32_t j = lane; j < E; j += 64) {
             if (!(w.eaux[j] & 1)) continue;
             const uint32_t key = w.ent[j];
-            uint32_t slot = (key * 2654435761u) & hm;
+            uint32_t slot = (key * 2654435761u) >> hshift;
             for (;;) {
                 const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
                 if (old == EMPTY || old == key) break;
@@ -711,16 +725,17 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
 constexpr uint32_t L_CAP_E = 256;   // index hits per (read, orientation) in LDS
 constexpr uint32_t L_CAP_INS = 128; // keys per map in LDS
 constexpr uint32_t L_TB = 256 + NBH - 1;
+constexpr uint32_t L_HASH = 256;    // counting table slots (>= L_CAP_E)
+static_assert(L_HASH >= L_CAP_E && (L_HASH & (L_HASH - 1)) == 0, "counting table");
 
+// per-wave LDS of the light seed kernel; the gather / count arrays and the
+// hopscotch arrays share storage
 template <typename TabT>
-__global__ __launch_bounds__(64) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
-{
-    __shared__ SeedShared sh;
-    __shared__ TabT s_klo[FAME_DEV_MAX_READ];
-    __shared__ uint32_t s_klen[FAME_DEV_MAX_READ];
-    __shared__ uint32_t s_koff[FAME_DEV_MAX_READ];
-    // gather / count arrays and hopscotch arrays share storage
-    __shared__ union {
+struct SeedLds {
+    TabT klo[FAME_DEV_MAX_READ];
+    uint32_t klen[FAME_DEV_MAX_READ];
+    uint32_t koff[FAME_DEV_MAX_READ];
+    union {
         struct {
             uint32_t ent[L_CAP_E];
             uint32_t eaux[L_CAP_E];
@@ -732,34 +747,59 @@ __global__ __launch_bounds__(64) void k_seed_light(DevIndex ix, BatchDev bd, uin
             uint16_t over[L_CAP_INS];
         } b;
     } u;
-    __shared__ uint32_t s_ins_key[2 * L_CAP_INS];
-    __shared__ uint16_t s_ins_cnt[2 * L_CAP_INS];
-    __shared__ uint32_t s_cand[2 * L_CAP_INS];
+    // the counting table is dead before the insertion lists are written
+    union {
+        struct {
+            uint32_t key[L_HASH];
+            uint32_t first[L_HASH];
+            uint32_t cnt[L_HASH];
+        } h;
+        struct {
+            uint32_t ins_key[2 * L_CAP_INS];
+            uint16_t ins_cnt[2 * L_CAP_INS];
+            uint32_t cand[2 * L_CAP_INS];
+        } c;
+    } v;
+    uint8_t code[FAME_DEV_MAX_READ + 1];
+};
 
+// FAME_SEED_WPB independent waves per block (one block slot of a CU then
+// holds several waves); each wave works through (read, orientation) pairs
+// in grid-stride order
+template <typename TabT>
+__global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
+{
+    __shared__ SeedShared sh;
+    __shared__ SeedLds<TabT> lds[FAME_SEED_WPB];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = threadIdx.x >> 6;
+    SeedLds<TabT>& m = lds[wv];
     SeedWork<TabT, uint16_t> w;
-    w.klo = s_klo;
-    w.klen = s_klen;
-    w.koff = s_koff;
-    w.ent = u.a.ent;
-    w.ekm = u.a.ekm;
-    w.eaux = u.a.eaux;
+    w.klo = m.klo;
+    w.klen = m.klen;
+    w.koff = m.koff;
+    w.ent = m.u.a.ent;
+    w.ekm = m.u.a.ekm;
+    w.eaux = m.u.a.eaux;
     w.cap_e = L_CAP_E;
-    w.hsize = 0;
-    w.hkey = w.hfirst = w.hcnt = nullptr;
-    w.ins_key = s_ins_key;
-    w.ins_cnt = s_ins_cnt;
+    w.hsize = L_HASH;
+    w.hkey = m.v.h.key;
+    w.hfirst = m.v.h.first;
+    w.hcnt = m.v.h.cnt;
+    w.ins_key = m.v.c.ins_key;
+    w.ins_cnt = m.v.c.ins_cnt;
     w.cap_ins = L_CAP_INS;
-    w.tb = u.b.tb;
+    w.tb = m.u.b.tb;
     w.tb_cap = L_TB;
-    w.tmp = u.b.tmp;
-    w.over = u.b.over;
-    w.cand = s_cand;
-    const int lane = threadIdx.x;
-    seed_table_init(sh, lane);
+    w.tmp = m.u.b.tmp;
+    w.over = m.u.b.over;
+    w.cand = m.v.c.cand;
+    w.code = m.code;
+    seed_table_init(sh);
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
-    // persistent waves: (read, orientation) pairs in grid-stride order
-    for (uint32_t ro = blockIdx.x; ro < n_ro; ro += gridDim.x) {
+    const uint32_t nw = gridDim.x * FAME_SEED_WPB;
+    for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; ro < n_ro; ro += nw) {
         const uint32_t r = ro >> 1, ori = ro & 1;
         if (!((bd.ori_mask >> ori) & 1) || bd.pre[r] != 0) {
             if (lane == 0) {
@@ -801,6 +841,7 @@ template <typename TabT>
 __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, HeavyScratch hs)
 {
     __shared__ SeedShared sh;
+    __shared__ uint8_t s_code[FAME_DEV_MAX_READ + 1];
     const uint32_t hi = blockIdx.x;
     if (hi >= hs.count) return;
     const uint32_t ro = bd.heavy_ro[hs.first + hi];
@@ -832,13 +873,8 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.tmp = (uint32_t*)take(4ull * ce);
     w.over = (uint32_t*)take(4ull * ce);
     w.cand = (uint32_t*)take(8ull * ce);
-    seed_table_init(sh, threadIdx.x);
-    for (uint32_t i = threadIdx.x; i < hs.hsize; i += 64) {
-        w.hkey[i] = EMPTY;
-        w.hfirst[i] = 0xFFFFFFFFu;
-        w.hcnt[i] = 0;
-    }
-    wave_sync();
+    w.code = s_code;
+    seed_table_init(sh);
     uint64_t E;
     uint32_t bc;
     PoolChunk pc = {0, 0, 0};
@@ -1589,11 +1625,12 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)
 {
     if (!n_ro) return;
-    const uint32_t g = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
+    const uint32_t waves = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
+    const uint32_t g = (waves + FAME_SEED_WPB - 1) / FAME_SEED_WPB;
     if (ix.tab_is64)
-        hipLaunchKernelGGL(k_seed_light<uint64_t>, dim3(g), dim3(64), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL(k_seed_light<uint64_t>, dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
     else
-        hipLaunchKernelGGL(k_seed_light<uint32_t>, dim3(g), dim3(64), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL(k_seed_light<uint32_t>, dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
 }
 
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st)

@@ -12,6 +12,9 @@
 #define FAME_NSTATS 8
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
 #define FAME_SEED_WAVES 16384      // persistent waves of the seed kernel
+#ifndef FAME_SEED_WPB
+#define FAME_SEED_WPB 2            // independent waves per block of the seed kernel
+#endif
 #define FAME_DPBAND 8 // 2*MISCOUNT + ADDMIS (ReadQueue.cpp:3640)
 
 namespace fame {
