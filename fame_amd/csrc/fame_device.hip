@@ -897,30 +897,32 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
 // k_verify: approximate matching of one read over one window per lane
 // ---------------------------------------------------------------------------
 
-// Myers / Hyyro bit-vector step for one text letter (pattern bit i = pattern
-// character i; the score is D[m][j], the edit distance of the whole pattern
-// to the best substring ending at this text position).  It equals the lowest
-// layer of ShiftAnd<6>'s automaton (ShiftAnd.h:276-341) holding the accepting
-// state, whenever that distance is <= 6.
+// Myers / Hyyro bit-vector step for one text letter.  The pattern occupies
+// bits [z, 64W) of the W words (z = 64W - m) and the z low bits never
+// match, which adds exactly z to every distance of the last row: the score
+// kept here is D[m][j] + z, the edit distance of the whole pattern to the
+// best substring ending at this text position, plus z.  The last pattern
+// bit is bit 63 of the top word.  D[m][j] equals the lowest layer of
+// ShiftAnd<6>'s automaton (ShiftAnd.h:276-341) holding the accepting state,
+// whenever that distance is <= 6.
 template <int W>
 __device__ __forceinline__ void myers_step(uint64_t (&Pv)[W], uint64_t (&Mv)[W], const uint64_t* __restrict__ peq,
-                                           int lastw, uint64_t lastbit, int& score)
+                                           int& score)
 {
-    uint64_t carry = 0, phc = 0, mhc = 0;
+    unsigned long long carry = 0;
+    uint64_t phc = 0, mhc = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const uint64_t eq = peq[w * 256];
         const uint64_t pv = Pv[w], mv = Mv[w];
         const uint64_t xv = eq | mv;
-        const uint64_t t = eq & pv;
-        const uint64_t s1 = t + pv;
-        const uint64_t c1 = s1 < t;
-        const uint64_t s2 = s1 + carry;
-        carry = c1 | (uint64_t)(s2 < s1);
-        const uint64_t xh = (s2 ^ pv) | eq;
+        unsigned long long co;
+        const uint64_t sum = __builtin_addcll(eq & pv, pv, carry, &co);
+        carry = co;
+        const uint64_t xh = (sum ^ pv) | eq;
         const uint64_t ph = mv | ~(xh | pv);
         const uint64_t mh = pv & xh;
-        if (w == lastw) score += (ph & lastbit) ? 1 : ((mh & lastbit) ? -1 : 0);
+        if (w == W - 1) score += (int)(ph >> 63) - (int)(mh >> 63);
         const uint64_t ph2 = (ph << 1) | phc;
         const uint64_t mh2 = (mh << 1) | mhc;
         phc = ph >> 63;
@@ -947,18 +949,15 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint32_t r = ro >> 1, ori = ro & 1;
     const bool rev = (meta >> 31) != 0;
     const uint32_t id = meta & 0x7FFFFFFFu;
-    const uint32_t m = bd.lens[r];
+    const uint32_t m = bd.lens[r]; // 64(W-1) < m <= 64W: the pool holds reads of W words
+    const int z = 64 * W - (int)m;
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
     // Peq masks of the pattern, laid out [letter*W + word][lane] so that the
     // lanes of a wave read consecutive LDS words (ShiftAnd loadBitmasks
-    // ShiftAnd.h:345-432: pattern T also matches text C)
+    // ShiftAnd.h:345-432: pattern T also matches text C), shifted up by z
     uint64_t* peq = s_peq + threadIdx.x;
     {
         uint64_t pm[4][W];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int w = 0; w < W; ++w) pm[c][w] = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             uint64_t a = 0, cc = 0, gg = 0, tt = 0;
@@ -978,7 +977,11 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int w = 0; w < W; ++w) peq[(c * W + w) * 256] = pm[c][w];
+            for (int w = W - 1; w >= 0; --w) {
+                uint64_t x = pm[c][w] << z;
+                if (w > 0 && z > 0) x |= pm[c][w - 1] >> (64 - z);
+                peq[(c * W + w) * 256] = x;
+            }
     }
     const uint4 win = ix.win[id];
     const uint32_t startPos = win.x, chrom = win.y;
@@ -1005,18 +1008,51 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         Pv[w] = ~0ull;
         Mv[w] = 0;
     }
-    int score = (int)m;
-    const int lastw = (int)((m - 1) >> 6);
-    const uint64_t lastbit = 1ull << ((m - 1) & 63);
+    const int score0 = 64 * W; // m + z
+    const int thr = (int)NERR + z;
+    int score = score0;
     bool was = false;
     uint32_t prevErrs = MISCOUNT + 1;
-    uint64_t ncomp = 0;
+    uint32_t ncomp = 0;
+    const uint32_t minc = m - NERR;
     uint32_t nm = 0;
     const uint32_t cap = bd.match_cap;
     uint32_t* outm = bd.job_match[pool] + (size_t)j * cap;
     uint32_t* ovf = only ? bd.ovf_buf + bd.job_ovf_off[pool][j] : nullptr;
+    // one text letter: the Myers step, then FAME's run logic (a run of
+    // consecutive accepting positions reports one match: its first position,
+    // replaced by later ones with no more errors)
+    auto step = [&](uint32_t c, int64_t p) {
+        myers_step<W>(Pv, Mv, peq + c * (W * 256), score);
+        ++ncomp;
+        if (ncomp < minc) return;
+        if (score <= thr) {
+            const uint32_t e = (uint32_t)(score - z);
+            // reported end: querySeq it-start+MISCOUNT,
+            // queryRevSeq it-end+pLen-1+MISCOUNT
+            const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
+            const uint32_t packed = (pos << 8) | e;
+            if (was) {
+                if (e <= prevErrs) {
+                    const uint32_t k = nm - 1;
+                    if (ovf) ovf[k] = packed;
+                    else if (k < cap) outm[k] = packed;
+                    prevErrs = e;
+                }
+            } else {
+                if (ovf) ovf[nm] = packed;
+                else if (nm < cap) outm[nm] = packed;
+                ++nm;
+                was = true;
+                prevErrs = e;
+            }
+        } else {
+            was = false;
+        }
+    };
     // the text is read 16 bytes at a time from 16-byte aligned addresses
-    // (the sequence buffer is padded so that the last chunk stays inside it)
+    // (the sequence buffer is padded so that the last chunk stays inside it);
+    // the reverse strand is scanned from its end
     const uintptr_t ta = (uintptr_t)text;
     const uintptr_t a0 = ta & ~(uintptr_t)15;
     const int64_t skip = (int64_t)(ta - a0);
@@ -1024,51 +1060,55 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     for (int64_t ci = 0; ci < nchunk; ++ci) {
         const int64_t chunk = rev ? nchunk - 1 - ci : ci;
         const uint4 v = *(const uint4*)(a0 + 16 * chunk);
-        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+        // the 16 bytes in scan order
+        uint32_t wd[4];
+        if (rev) {
+            wd[0] = __builtin_bswap32(v.w);
+            wd[1] = __builtin_bswap32(v.z);
+            wd[2] = __builtin_bswap32(v.y);
+            wd[3] = __builtin_bswap32(v.x);
+        } else {
+            wd[0] = v.x;
+            wd[1] = v.y;
+            wd[2] = v.z;
+            wd[3] = v.w;
+        }
+        const int64_t base = 16 * chunk - skip; // text position of the chunk's first memory byte
+        const int mlo = base < 0 ? (int)(-base) : 0;
+        const int64_t rem = (int64_t)tlen - base;
+        const int mhi = rem < 16 ? (int)rem : 16;
+        const int lo = rev ? 16 - mhi : mlo; // scan-order range of text bytes
+        const int hi = rev ? 16 - mlo : mhi;
+        // A0 C1 G2 T3 = ((b >> 1) ^ (b >> 2)) & 3 per byte, complemented on
+        // the reverse strand (ShiftAnd.h:197-211); 'N' bytes flagged
+        uint32_t cw[4];
+        uint32_t nflag = 0;
 #pragma unroll
-        for (int ii = 0; ii < 16; ++ii) {
-            const int i = rev ? 15 - ii : ii;
-            const int64_t p = 16 * chunk + i - skip; // position in the text
-            if (p < 0 || p >= (int64_t)tlen) continue;
-            const uint32_t b = (wd[i >> 2] >> (8 * (i & 3))) & 0xFF;
-            if (b == 'N') { // the automaton restarts (ShiftAnd.h:142-146, 212-214)
+        for (int k = 0; k < 4; ++k) {
+            cw[k] = (((wd[k] >> 1) ^ (wd[k] >> 2)) & 0x03030303u) ^ (rev ? 0x03030303u : 0u);
+            const uint32_t t = wd[k] ^ 0x4E4E4E4Eu;
+            nflag |= (t - 0x01010101u) & ~t & 0x80808080u;
+        }
+        const int64_t pb = rev ? base + 15 : base; // position of scan-order byte 0
+        const int64_t pd = rev ? -1 : 1;
+        if (lo == 0 && hi == 16 && nflag == 0) {
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    Pv[w] = ~0ull;
-                    Mv[w] = 0;
-                }
-                score = (int)m;
-                continue;
-            }
-            // A0 C1 G2 T3 from the ASCII code; complemented when scanning
-            // the reverse strand (ShiftAnd.h:197-211)
-            uint32_t c = ((b >> 1) ^ (b >> 2)) & 3;
-            if (rev) c = 3 - c;
-            myers_step<W>(Pv, Mv, peq + c * W * 256, lastw, lastbit, score);
-            ++ncomp;
-            if (ncomp < (uint64_t)m - NERR) continue;
-            if (score <= (int)NERR) {
-                const uint32_t e = (uint32_t)score;
-                // reported end: querySeq it-start+MISCOUNT,
-                // queryRevSeq it-end+pLen-1+MISCOUNT
-                const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
-                const uint32_t packed = (pos << 8) | e;
-                if (was) {
-                    if (e <= prevErrs) {
-                        const uint32_t k = nm - 1;
-                        if (ovf) ovf[k] = packed;
-                        else if (k < cap) outm[k] = packed;
-                        prevErrs = e;
+            for (int ii = 0; ii < 16; ++ii) step((cw[ii >> 2] >> (8 * (ii & 3))) & 3u, pb + pd * ii);
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) {
+                if (ii < lo || ii >= hi) continue;
+                const uint32_t b = (wd[ii >> 2] >> (8 * (ii & 3))) & 0xFFu;
+                if (b == 'N') { // the automaton restarts (ShiftAnd.h:142-146, 212-214)
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        Pv[w] = ~0ull;
+                        Mv[w] = 0;
                     }
-                } else {
-                    if (ovf) ovf[nm] = packed;
-                    else if (nm < cap) outm[nm] = packed;
-                    ++nm;
-                    was = true;
-                    prevErrs = e;
+                    score = score0;
+                    continue;
                 }
-            } else {
-                was = false;
+                step((cw[ii >> 2] >> (8 * (ii & 3))) & 3u, pb + pd * ii);
             }
         }
     }
