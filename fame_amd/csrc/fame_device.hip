@@ -1405,7 +1405,15 @@ __global__ __launch_bounds__(256) void k_meth_direct(DevIndex ix, BatchDev bd)
 // Here each lane stores its cells by (row, offset c-r+9) and the backtrace
 // maps the reference's flat index back to the cell holding that slot last
 // (unused slots read as the matrix's fill value 0xFFFF).
-__device__ __forceinline__ uint16_t dp_get(const uint16_t* S, size_t stride, long L, long r, long c)
+// Stored value of cell (r, c) of the banded matrix: the flat index of
+// BandedMatrix<uint16_t, 9> (r*19 + max(9-r, 0) + c-r+9, so that the slots of
+// neighbouring rows alias at the band edges) mapped to the rows kept in S
+// (rows 0..DP_SROWS-1, slot kk of row rr at S[(rr*19 + kk) * stride]).
+// Flat positions no row was written to read as 0xFFFF; *bad is set when a
+// position beyond the kept rows is asked for.
+constexpr int DP_SROWS = FAME_DP_SROWS;
+
+__device__ __forceinline__ uint16_t dp_get(const uint16_t* S, size_t stride, long L, long r, long c, bool* bad)
 {
     const long idx = r * 19 + ((9 - r) > 0 ? 9 - r : 0) + (c - r + 9);
     if (idx < 0 || idx >= (L + 1) * 19) return 0xFFFF;
@@ -1421,18 +1429,56 @@ __device__ __forceinline__ uint16_t dp_get(const uint16_t* S, size_t stride, lon
         if (rr < 0) return 0xFFFF;
         kk = (idx - 17 * rr - 18) - rr + 9;
     }
+    if (rr >= DP_SROWS) {
+        *bad = true;
+        return 0xFFFF;
+    }
     return S[(size_t)(rr * 19 + kk) * stride];
 }
 
-__global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint16_t* dp_scratch, size_t stride,
-                                                 uint8_t* trace_scratch)
+// a reference letter g matches the read letter when it equals t1 or t2
+// (cmp_fwd / cmp_rev above): forward t1 = r and t2 = C for r = T; reverse
+// the complement, and G for r = T; no target for other reverse letters
+__device__ __forceinline__ void dp_targets(uint32_t rc, bool rv, uint32_t& t1, uint32_t& t2)
+{
+    if (!rv) {
+        t1 = rc;
+        t2 = rc == 'T' ? 'C' : rc;
+    } else {
+        t1 = rc == 'T' ? 'A' : rc == 'G' ? 'C' : rc == 'A' ? 'T' : rc == 'C' ? 'G' : 0x100u;
+        t2 = rc == 'T' ? 'G' : t1;
+    }
+}
+
+// backtrackDP's choice at slot k of row rr (LevenshtDP.h:329-544) from the
+// stored values up = (rr-1, c), left = (rr, c-1), diagonal = (rr-1, c-1):
+// 2 insertion, 3 deletion, else the diagonal step (0 match, 1 mismatch)
+__device__ __forceinline__ uint32_t dp_choice(int up, int left, int dg, int mm)
+{
+    if (up < left) return (up + 1 < dg + mm) ? 2u : (uint32_t)mm;
+    return (left + 1 < dg + mm) ? 3u : (uint32_t)mm;
+}
+
+// One lane per read.  Rows 0..9 are filled by the general recurrence (the
+// first column and the cells left of it lie in the band); from row 10 on
+// every band slot but the two edges holds a computed distance, the edges
+// hold 0xFFFF - rr (left) and 0xFFFF - rr - 9 (right), so a traceback never
+// leaves slots 1..17 there.  Rows 0..DP_SROWS-1 are kept as values for the
+// last part of the traceback; for rows >= 11 only the traceback choice of
+// slots 1..17 is kept (2 bits each, one word per row), computed during the
+// fill from the same stored values the traceback would read.
+__global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint16_t* __restrict__ dp_scratch,
+                                                 uint64_t* __restrict__ dec_scratch,
+                                                 uint64_t* __restrict__ trace_scratch)
 {
     const uint32_t n = *bd.dp_n;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t gsz = gridDim.x * blockDim.x;
+    const size_t stride = gsz;
     const uint32_t RL = ix.readlen, RL2 = RL - 2;
     uint16_t* S = dp_scratch + gid;
-    uint8_t* tr = trace_scratch + gid;
+    uint64_t* D = dec_scratch + gid;
+    uint64_t* TR = trace_scratch + gid;
     if (gid == 0) atomicAdd(&bd.stats[5], (unsigned long long)n);
     for (uint32_t t = gid; t < n; t += gsz) {
         const uint32_t r = bd.dp_list[t];
@@ -1462,9 +1508,11 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             }
         }
         // fill (runDPFill / runDPFillRev LevenshtDP.h:207-323): one row at a
-        // time in registers; values are computed in int and kept as u16
+        // time in registers; values are computed in int and kept as u16.
+        // gw holds the reference bytes of columns r-8 .. r+8 (byte i = column
+        // r-8+i), shifted by one byte per row.
         int prev[19], cur[19];
-        uint8_t g[17]; // reference bytes of columns r-8 .. r+8
+        uint32_t gw[5];
 #pragma unroll
         for (int k = 0; k < 19; ++k) prev[k] = 0xFFFF;
         prev[9] = 0;
@@ -1474,15 +1522,24 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
 #pragma unroll
         for (int k = 9; k < 19; ++k) S[(size_t)k * stride] = (uint16_t)prev[k];
 #pragma unroll
-        for (int k = 0; k < 17; ++k) g[k] = (k >= 9) ? col_at(cs, clen, ref_end, k - 8) : 0; // row 0: col k-8
-        for (long rr = 1; rr <= (long)L; ++rr) {
+        for (int i = 0; i < 5; ++i) gw[i] = 0;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) g[k] = g[k + 1];
-            g[16] = col_at(cs, clen, ref_end, rr + 8);
-            const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
+        for (int k = 9; k < 17; ++k) gw[k >> 2] |= col_at(cs, clen, ref_end, k - 8) << (8 * (k & 3)); // row 0: col k-8
+        auto shift_in = [&](uint32_t b) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gw[i] = (gw[i] >> 8) | (gw[i + 1] << 24);
+            gw[4] = b;
+        };
+        auto gbyte = [&](int i) -> uint32_t { return (gw[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
+        const long Lg = (long)L < 9 ? (long)L : 9;
+#pragma unroll 1
+        for (long rr = 1; rr <= Lg; ++rr) {
+            shift_in(col_at(cs, clen, ref_end, rr + 8));
+            uint32_t t1, t2;
+            dp_targets(rv ? seq[rr - 1] : seq[L - rr], rv, t1, t2);
             // left band (offset -9): the first column for rows 1..8 is not at
             // offset -9; row 9's slot holds the right band of row 8 (65518)
-            cur[0] = rr >= 10 ? (int)(0xFFFF - rr) : (rr == 9 ? 0xFFFF - 8 - 9 : 0xFFFF);
+            cur[0] = rr == 9 ? 0xFFFF - 8 - 9 : 0xFFFF;
 #pragma unroll
             for (int k = 1; k < 18; ++k) {
                 const long o = k - 9, c = rr + o;
@@ -1492,8 +1549,8 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
                 } else if (c == 0) {
                     v = (int)rr; // first column (rows 1..8)
                 } else {
-                    const uint8_t gc = g[o + 8];
-                    const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
+                    const uint32_t gc = gbyte(k - 1);
+                    const int mm = (gc != t1 && gc != t2) ? 1 : 0;
                     int a = prev[k + 1] + 1;
                     const int b = cur[k - 1] + 1;
                     const int d = prev[k] + mm;
@@ -1512,54 +1569,135 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
                 prev[k] = cur[k];
             }
         }
-        // backtrace (backtrackDP / backtrackDPRev LevenshtDP.h:329-544)
-        uint16_t minimum = 0xFFFF;
-        unsigned long col = 0;
-        for (long o = -DPBAND; o <= DPBAND; ++o) {
-            const uint16_t v = dp_get(S, stride, L, (long)L, (long)L + o);
-            if (v < minimum) {
-                col = (unsigned long)((long)L + o);
-                minimum = v;
+        // rows 10..L: every slot 1..17 is a computed cell.  The letters of
+        // the next rows are loaded four rows ahead (b0..b3 / c0..c3).
+        {
+            auto ref_b = [&](long rq) -> uint32_t { return rq <= (long)L ? col_at(cs, clen, ref_end, rq + 8) : 0u; };
+            auto read_b = [&](long rq) -> uint32_t {
+                return rq <= (long)L ? (uint32_t)(rv ? seq[rq - 1] : seq[L - rq]) : 0u;
+            };
+            uint32_t b0 = ref_b(10), b1 = ref_b(11), b2 = ref_b(12), b3 = ref_b(13);
+            uint32_t c0 = read_b(10), c1 = read_b(11), c2 = read_b(12), c3 = read_b(13);
+#pragma unroll 1
+            for (long rr = 10; rr <= (long)L; ++rr) {
+                shift_in(b0);
+                uint32_t t1, t2;
+                dp_targets(c0, rv, t1, t2);
+                b0 = b1;
+                b1 = b2;
+                b2 = b3;
+                b3 = ref_b(rr + 4);
+                c0 = c1;
+                c1 = c2;
+                c2 = c3;
+                c3 = read_b(rr + 4);
+                uint64_t dw = 0;
+                cur[0] = (int)(0xFFFF - rr);
+#pragma unroll
+                for (int k = 1; k < 18; ++k) {
+                    const uint32_t gc = gbyte(k - 1);
+                    const int mm = (gc != t1 && gc != t2) ? 1 : 0;
+                    const int up = prev[k + 1], left = cur[k - 1], dg = prev[k];
+                    dw |= (uint64_t)dp_choice(up, left, dg, mm) << (2 * (k - 1));
+                    int a = up + 1;
+                    const int b = left + 1;
+                    const int d = dg + mm;
+                    a = b < a ? b : a;
+                    a = d < a ? d : a;
+                    cur[k] = a & 0xFFFF;
+                }
+                cur[18] = rr < (long)L ? (int)(0xFFFF - rr - 9) : 0xFFFF;
+                if (rr < DP_SROWS) {
+                    uint16_t* row = S + (size_t)(rr * 19) * stride;
+#pragma unroll
+                    for (int k = 0; k < 19; ++k) row[(size_t)k * stride] = (uint16_t)cur[k];
+                }
+                if (rr >= 11) D[(size_t)rr * stride] = dw;
+#pragma unroll
+                for (int k = 0; k < 19; ++k) prev[k] = cur[k];
             }
         }
+        // backtrace (backtrackDP / backtrackDPRev LevenshtDP.h:329-544): the
+        // end column is the first minimum of row L over offsets -8..8
+        bool bad = false;
+        uint16_t minimum = 0xFFFF;
+        unsigned long col = 0;
+        if (L >= 10) {
+#pragma unroll
+            for (int k = 1; k < 18; ++k)
+                if (prev[k] < (int)minimum) {
+                    col = (unsigned long)((long)L + k - 9);
+                    minimum = (uint16_t)prev[k];
+                }
+        } else {
+            for (long o = -DPBAND; o <= DPBAND; ++o) {
+                const uint16_t v = dp_get(S, stride, L, (long)L, (long)L + o, &bad);
+                if (v < minimum) {
+                    col = (unsigned long)((long)L + o);
+                    minimum = v;
+                }
+            }
+        }
+        // the trace: 2-bit steps, 32 to a word
         int tn = 0;
-        for (long rr = (long)L; rr > 0;) {
+        uint64_t tw = 0;
+        auto put = [&](uint32_t tt) {
+            if (tn < FAME_DEV_TRACE_CAP) {
+                tw |= (uint64_t)tt << (2 * (tn & 31));
+                if ((tn & 31) == 31) {
+                    TR[(size_t)(tn >> 5) * stride] = tw;
+                    tw = 0;
+                }
+            }
+            ++tn;
+        };
+        long rr = (long)L;
+        if (rr >= 11) { // rows >= 11 from the kept choices, read four rows ahead
+            long k = (long)col - rr + 9;
+            uint64_t q0 = D[(size_t)rr * stride];
+            uint64_t q1 = rr - 1 >= 11 ? D[(size_t)(rr - 1) * stride] : 0;
+            uint64_t q2 = rr - 2 >= 11 ? D[(size_t)(rr - 2) * stride] : 0;
+            uint64_t q3 = rr - 3 >= 11 ? D[(size_t)(rr - 3) * stride] : 0;
+            while (rr >= 11) {
+                if (k < 1 || k > 17) {
+                    bad = true;
+                    break;
+                }
+                const uint32_t tt = (uint32_t)(q0 >> (2 * (k - 1))) & 3u;
+                put(tt);
+                if (tt == 3) {
+                    --k; // deletion: same row, one column left
+                } else {
+                    if (tt == 2) ++k; // insertion: row above, same column
+                    --rr;
+                    q0 = q1;
+                    q1 = q2;
+                    q2 = q3;
+                    q3 = rr - 3 >= 11 ? D[(size_t)(rr - 3) * stride] : 0;
+                }
+            }
+            col = (unsigned long)(rr + k - 9);
+        }
+        while (rr > 0 && !bad && tn <= FAME_DEV_TRACE_CAP) { // rows <= 10 from the kept values
             const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
             const uint8_t gc = col_at(cs, clen, ref_end, (long)col);
             const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
             const long c = (long)col;
-            uint8_t tt;
-            const uint16_t up = dp_get(S, stride, L, rr - 1, c);
-            const uint16_t left = dp_get(S, stride, L, rr, c - 1);
-            const uint16_t dg = dp_get(S, stride, L, rr - 1, c - 1);
-            if (up < left) {
-                if ((int)up + 1 < (int)dg + mm) {
-                    tt = 2; // INSERTION
-                    --rr;
-                } else {
-                    tt = mm ? 1 : 0;
-                    --rr;
-                    --col;
-                }
-            } else {
-                if ((int)left + 1 < (int)dg + mm) {
-                    tt = 3; // DELETION
-                    --col;
-                } else {
-                    tt = mm ? 1 : 0;
-                    --rr;
-                    --col;
-                }
-            }
-            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * gsz] = tt;
-            ++tn;
+            const uint16_t up = dp_get(S, stride, L, rr - 1, c, &bad);
+            const uint16_t left = dp_get(S, stride, L, rr, c - 1, &bad);
+            const uint16_t dg = dp_get(S, stride, L, rr - 1, c - 1, &bad);
+            const uint32_t tt = dp_choice(up, left, dg, mm);
+            if (tt != 3) --rr;
+            if (tt != 2) --col;
+            put(tt);
         }
-        while (col > 0) {
-            if (tn < FAME_DEV_TRACE_CAP) tr[(size_t)tn * gsz] = 3;
-            ++tn;
+        while (col > 0 && tn <= FAME_DEV_TRACE_CAP) {
+            put(3);
             --col;
         }
-        if (tn > FAME_DEV_TRACE_CAP) {
+        if (rr > 0 || col > 0) bad = true;
+        if ((tn & 31) && tn <= FAME_DEV_TRACE_CAP) TR[(size_t)(tn >> 5) * stride] = tw;
+        if (tn > FAME_DEV_TRACE_CAP || bad) {
             atomicOr(bd.flags, FLAG_CAPACITY);
             continue;
         }
@@ -1568,10 +1706,16 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
         uint32_t refSeqPos = metaPos + offset;
         int32_t readSeqPos = (int32_t)L - 1;
         int32_t alignPos = tn - 1;
+        int32_t wcur = -1;
+        uint64_t wv = 0;
         for (int32_t id = maxIndex; id >= minIndex; --id) {
             const uint32_t pos = ix.cpg_pos[id];
             while (alignPos >= 0 && pos + RL2 < refSeqPos) {
-                switch (tr[(size_t)alignPos * gsz]) {
+                if ((alignPos >> 5) != wcur) {
+                    wcur = alignPos >> 5;
+                    wv = TR[(size_t)wcur * stride];
+                }
+                switch ((uint32_t)(wv >> (2 * (alignPos & 31))) & 3u) {
                 case 0:
                 case 1:
                     --readSeqPos;
@@ -1701,13 +1845,24 @@ void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_s
     hipLaunchKernelGGL(k_select, dim3((n + 255) / 256), dim3(256), 0, st, ix, bd, n, get_stranded);
 }
 
-void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, size_t dp_stride,
-                 uint8_t* trace, uint32_t grid_threads, hipStream_t st)
+// lanes of k_meth_dp that are resident at once on the current device
+uint32_t meth_dp_lanes()
+{
+    int nb = 0, dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_meth_dp, 256, 0) != hipSuccess || nb <= 0)
+        return 65536;
+    return (uint32_t)nb * (uint32_t)p.multiProcessorCount * 256u;
+}
+
+void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
+                 uint32_t grid_threads, hipStream_t st)
 {
     if (!n) return;
     const uint32_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_meth_direct, dim3(g < 8192 ? g : 8192), dim3(256), 0, st, ix, bd);
-    hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dp_stride, trace);
+    hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dec, trace);
 }
 
 void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st)

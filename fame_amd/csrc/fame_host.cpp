@@ -87,10 +87,10 @@ struct fame_ctx {
         b_job_ovf_off[FAME_NPOOL];
     DevBuf b_ovf_n, b_ovf_pool, b_ovf_job, b_ovf_buf, b_ovf_total;
     DevBuf b_heavy_n, b_heavy_ro, b_heavy_e, b_heavy_bc, b_heavy_scratch;
-    DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_trace, b_direct, b_dplist, b_methn;
+    DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_dec, b_trace, b_direct, b_dplist, b_methn;
     uint32_t pool_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t match_cap = 4;
-    uint32_t meth_threads = 65536;
+    uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
     double timings[6] = {0, 0, 0, 0, 0, 0};
     uint64_t last_heavy = 0, last_ovf = 0;
 };
@@ -525,11 +525,12 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     launch_select(c->ix, bd, n, get_stranded, st);
     tm.mark(7);
     {
-        const size_t cells = (size_t)(FAME_DEV_MAX_READ + 1) * (2 * (FAME_DPBAND + 1) + 1) + 19;
+        const size_t cells = (size_t)FAME_DP_SROWS * (2 * (FAME_DPBAND + 1) + 1);
         uint16_t* dp = grow<uint16_t>(c->b_dp, cells * c->meth_threads);
-        uint8_t* tr = grow<uint8_t>(c->b_trace, (size_t)FAME_DEV_TRACE_CAP * c->meth_threads);
+        uint64_t* dec = grow<uint64_t>(c->b_dec, (size_t)(FAME_DEV_MAX_READ + 1) * c->meth_threads);
+        uint64_t* tr = grow<uint64_t>(c->b_trace, (size_t)FAME_DP_TRACE_WORDS * c->meth_threads);
         tm.mark(8);
-        launch_meth(c->ix, bd, n, dp, c->meth_threads, tr, c->meth_threads, st);
+        launch_meth(c->ix, bd, n, dp, dec, tr, c->meth_threads, st);
         tm.mark(9);
     }
     uint64_t cnt[5];
@@ -591,6 +592,7 @@ int fame_open(const char* index_path, int both_strands, fame_ctx** out)
     int rc = guarded(c.get(), [&]() {
         HIPC(hipGetDevice(&c->device));
         HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->meth_threads = meth_dp_lanes();
         load_index(c.get(), index_path);
         return FAME_OK;
     });

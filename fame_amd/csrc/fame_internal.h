@@ -114,8 +114,13 @@ void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratc
 void launch_verify(const DevIndex& ix, const BatchDev& bd, uint32_t pool, uint32_t njobs,
                    const uint32_t* only, hipStream_t st);
 void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_stranded, hipStream_t st);
-void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, size_t dp_stride,
-                 uint8_t* trace, uint32_t grid_threads, hipStream_t st);
+// DP scratch per lane: FAME_DP_SROWS * 19 u16 values, FAME_DEV_MAX_READ + 1
+// u64 choice words, FAME_DP_TRACE_WORDS u64 trace words (lane-strided)
+#define FAME_DP_SROWS 12
+#define FAME_DP_TRACE_WORDS ((FAME_DEV_TRACE_CAP + 31) / 32)
+void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
+                 uint32_t grid_threads, hipStream_t st);
+uint32_t meth_dp_lanes();
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
