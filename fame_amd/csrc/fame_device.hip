@@ -483,8 +483,18 @@ struct SeedStats {
 // below it (an aligned word holding one byte of the read lies in that
 // byte's page); skip = the read's offset in the first word
 struct ReadWords {
-    uint32_t w0, w1, skip;
+    uint32_t w0, w1, skip, L;
 };
+
+// a wave-uniform value in a vector register: loads indexed by it are vector
+// loads, waited on by vmcnt, so they can be issued a turn ahead without
+// holding up the LDS waits (scalar loads share lgkmcnt with LDS)
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t x)
+{
+    uint32_t v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"(x));
+    return v;
+}
 
 template <typename TabT, typename IdxT>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
@@ -493,7 +503,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
 {
     const int lane = threadIdx.x & 63;
     const uint32_t r = ro >> 1, ori = ro & 1;
-    const uint32_t L = bd.lens[r];
+    const uint32_t L = rw ? __builtin_amdgcn_readfirstlane(rw->L) : bd.lens[r];
     if (rw) { // the read as prefetched aligned words: lane l holds bytes 4l .. 4l+3 (and 256 + 4l ..)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -819,17 +829,17 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     // software pipeline over this wave's (read, orientation) pairs: the
     // flags / length / offset two turns ahead and the bytes one turn ahead
     struct Pending {
-        uint32_t ro, skip, n;
+        uint32_t ro, skip, n, L;
         bool act;
         const uint8_t* base;
     };
     auto meta = [&](uint32_t ro, Pending& p) {
         p.ro = ro;
         p.act = false;
-        p.skip = p.n = 0;
+        p.skip = p.n = p.L = 0;
         p.base = nullptr;
         if (ro < n_ro) {
-            const uint32_t r = ro >> 1, ori = ro & 1;
+            const uint32_t r = in_vgpr(ro >> 1), ori = ro & 1;
             const uint8_t pre = bd.pre[r];
             const uint32_t L = bd.lens[r];
             const uint8_t* sp = (ori ? bd.rev : bd.seqs) + bd.offs[r];
@@ -837,11 +847,13 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             p.skip = (uint32_t)((uintptr_t)sp & 3);
             p.base = sp - p.skip;
             p.n = p.skip + L;
+            p.L = L;
         }
     };
     auto words = [&](const Pending& p, ReadWords& rw) {
         rw.w0 = rw.w1 = 0;
         rw.skip = p.skip;
+        rw.L = p.L;
         if (p.act) {
             if (4u * lane < p.n) rw.w0 = *(const uint32_t*)(p.base + 4 * lane);
             if (256u + 4u * lane < p.n) rw.w1 = *(const uint32_t*)(p.base + 256 + 4 * lane);
@@ -857,6 +869,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         Pending nn;
         meta(ro + 2 * nw, nn);
         words(nxt, nwd);
+        asm volatile("" ::: "memory"); // the loads above stay ahead of this turn's work
         if (!cur.act) {
             if (lane == 0) {
                 bd.cand_nf[ro] = 0;
@@ -991,6 +1004,34 @@ __device__ __forceinline__ void myers_step(uint64_t (&Pv)[W], uint64_t (&Mv)[W],
     }
 }
 
+// Piece filter of the window scan.  A match with at most NERR errors aligns
+// one of NERR+1 disjoint pieces of the pattern (length FP each, pattern
+// positions [i*FP, (i+1)*FP)) to the text without an error.  Letters are
+// compared in the three-letter alphabet A, G, {C,T}: a pattern letter
+// matching a text letter (Peq) implies equal reduced letters.  A piece found
+// at scan position q (its first letter) bounds the matches it can belong
+// to: the matched substring starts at >= q - i*FP - NERR and ends at
+// <= q - i*FP + m - 1 + NERR, at >= q - i*FP + m - 1 - NERR.  The bit-vector
+// scan restarted at the start of that region therefore reproduces every
+// end position with at most NERR errors there, with its error count, and
+// no position outside the regions is one.  Regions are kept as a mask of
+// 16-byte scan chunks; a run of consecutive marked chunks is scanned from a
+// fresh automaton.  Windows holding an N, patterns shorter than
+// (NERR+1)*FILT_MIN_P and windows wider than the mask are scanned whole.
+constexpr int FILT_MIN_P = 10;
+constexpr int FILT_MAX_P = 14;
+constexpr int FILT_MASKW = 4; // mask words: up to 256 chunks of 16 bytes
+
+__device__ __forceinline__ uint32_t spread16(uint32_t x)
+{
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
 // returns the number of window bytes scanned (0 when the lane has no job)
 template <int W>
 __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
@@ -1013,13 +1054,21 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
     // Peq masks of the pattern, laid out [letter*W + word][lane] so that the
     // lanes of a wave read consecutive LDS words (ShiftAnd loadBitmasks
-    // ShiftAnd.h:345-432: pattern T also matches text C), shifted up by z
+    // ShiftAnd.h:345-432: pattern T also matches text C), shifted up by z;
+    // the reduced piece codes of the filter (two bits per letter, A 0, G 2,
+    // C/T 3, the piece's first letter in the low bits)
     uint64_t* peq = s_peq + threadIdx.x;
+    constexpr int NP = (int)NERR + 1;
+    const int FP = (int)m / NP < FILT_MAX_P ? (int)m / NP : FILT_MAX_P;
+    uint32_t pc[NP];
+    bool filt = W >= 2 && FP >= FILT_MIN_P;
     {
         uint64_t pm[4][W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             uint64_t a = 0, cc = 0, gg = 0, tt = 0;
+            // unrolled so that groups of byte loads are in flight together
+#pragma unroll 16
             for (uint32_t i = 64 * w; i < m && i < 64u * (w + 1); ++i) {
                 const uint8_t ch = pat[i];
                 const uint64_t bit = 1ull << (i & 63);
@@ -1041,6 +1090,22 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 if (w > 0 && z > 0) x |= pm[c][w - 1] >> (64 - z);
                 peq[(c * W + w) * 256] = x;
             }
+        if (W >= 2) {
+            const uint32_t fm = (1u << FP) - 1;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                // pieces lie in pattern positions < NP * FILT_MAX_P <= 128
+                const uint32_t off = (uint32_t)(i * FP);
+                auto ext = [&](int c) -> uint32_t {
+                    const uint64_t lo = pm[c][0], hi = pm[c][W >= 2 ? 1 : 0];
+                    const uint64_t v = off == 0 ? lo : off < 64 ? ((lo >> off) | (hi << (64 - off))) : (hi >> (off - 64));
+                    return (uint32_t)v & fm;
+                };
+                const uint32_t ba = ext(0), bct = ext(1), bg = ext(2);
+                const uint32_t sct = spread16(bct), sg = spread16(bg);
+                pc[i] = ((ba | bct | bg) == fm) ? (sct | (sct << 1) | (sg << 1)) : 0xFFFFFFFFu;
+            }
+        }
     }
     const uint4 win = ix.win[id];
     const uint32_t startPos = win.x, chrom = win.y;
@@ -1078,86 +1143,102 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint32_t cap = bd.match_cap;
     uint32_t* outm = bd.job_match[pool] + (size_t)j * cap;
     uint32_t* ovf = only ? bd.ovf_buf + bd.job_ovf_off[pool][j] : nullptr;
-    // one text letter: the Myers step, then FAME's run logic (a run of
-    // consecutive accepting positions reports one match: its first position,
-    // replaced by later ones with no more errors)
+    // FAME's run logic: a run of consecutive accepting positions reports one
+    // match, its first position, replaced by later ones with no more errors.
+    // The run's current entry is kept in runbest and stored when the run
+    // ends (the stored values equal those of storing at every replacement).
+    uint32_t runbest = 0;
+    auto flush = [&]() {
+        const uint32_t k = nm - 1;
+        if (ovf) ovf[k] = runbest;
+        else if (k < cap) outm[k] = runbest;
+    };
     auto step = [&](uint32_t c, int64_t p) {
         myers_step<W>(Pv, Mv, peq + c * (W * 256), score);
         ++ncomp;
-        if (ncomp < minc) return;
-        if (score <= thr) {
-            const uint32_t e = (uint32_t)(score - z);
-            // reported end: querySeq it-start+MISCOUNT,
-            // queryRevSeq it-end+pLen-1+MISCOUNT
-            const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
-            const uint32_t packed = (pos << 8) | e;
-            if (was) {
-                if (e <= prevErrs) {
-                    const uint32_t k = nm - 1;
-                    if (ovf) ovf[k] = packed;
-                    else if (k < cap) outm[k] = packed;
-                    prevErrs = e;
-                }
-            } else {
-                if (ovf) ovf[nm] = packed;
-                else if (nm < cap) outm[nm] = packed;
-                ++nm;
-                was = true;
-                prevErrs = e;
-            }
-        } else {
-            was = false;
-        }
+        const bool valid = ncomp >= minc; // earlier positions leave the run state alone
+        const bool acc = valid && score <= thr;
+        const uint32_t e = (uint32_t)(score - z);
+        // reported end: querySeq it-start+MISCOUNT,
+        // queryRevSeq it-end+pLen-1+MISCOUNT
+        const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
+        const uint32_t packed = (pos << 8) | e;
+        const bool newrun = acc && !was;
+        const bool upd = acc && (newrun || e <= prevErrs);
+        if (valid && !acc && was) flush();
+        nm += newrun ? 1u : 0u;
+        runbest = upd ? packed : runbest;
+        prevErrs = upd ? e : prevErrs;
+        was = valid ? acc : was;
     };
     // the text is read 16 bytes at a time from 16-byte aligned addresses
     // (the sequence buffer is padded so that the last chunk stays inside it);
-    // the reverse strand is scanned from its end
+    // the reverse strand is scanned from its end.  Scan chunk ci is memory
+    // chunk ci (forward) or nchunk-1-ci (reverse).
     const uintptr_t ta = (uintptr_t)text;
-    const uintptr_t a0 = ta & ~(uintptr_t)15;
-    const int64_t skip = (int64_t)(ta - a0);
+    const int64_t skip = (int64_t)(ta & 15);
+    // pointer arithmetic from the index's sequence pointer keeps the loads
+    // in the global address space (a round trip through an integer would
+    // make them flat loads, which also count against LDS waits)
+    const uint8_t* a0 = text - skip;
     const int64_t nchunk = (skip + (int64_t)tlen + 15) >> 4;
-    for (int64_t ci = 0; ci < nchunk; ++ci) {
+    struct Chunk {
+        uint32_t wd[4], cw[4], nflag;
+        int lo, hi;     // scan-order range of text bytes in the chunk
+        int64_t pb, pd; // text position of scan-order byte 0, and its step
+    };
+    // the 16 bytes of scan chunk ci (zeros beyond the last chunk)
+    auto raw_chunk = [&](int64_t ci) -> uint4 {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (ci >= 0 && ci < nchunk) v = *(const uint4*)(a0 + 16 * (rev ? nchunk - 1 - ci : ci));
+        return v;
+    };
+    // the same, for 0 <= ci (indexes past the end read the last chunk)
+    auto raw_chunk_c = [&](int64_t ci) -> uint4 {
+        const int64_t c = ci < nchunk ? ci : nchunk - 1;
+        return *(const uint4*)(a0 + 16 * (rev ? nchunk - 1 - c : c));
+    };
+    auto load_chunk = [&](int64_t ci, const uint4 v, Chunk& k) {
         const int64_t chunk = rev ? nchunk - 1 - ci : ci;
-        const uint4 v = *(const uint4*)(a0 + 16 * chunk);
-        // the 16 bytes in scan order
-        uint32_t wd[4];
         if (rev) {
-            wd[0] = __builtin_bswap32(v.w);
-            wd[1] = __builtin_bswap32(v.z);
-            wd[2] = __builtin_bswap32(v.y);
-            wd[3] = __builtin_bswap32(v.x);
+            k.wd[0] = __builtin_bswap32(v.w);
+            k.wd[1] = __builtin_bswap32(v.z);
+            k.wd[2] = __builtin_bswap32(v.y);
+            k.wd[3] = __builtin_bswap32(v.x);
         } else {
-            wd[0] = v.x;
-            wd[1] = v.y;
-            wd[2] = v.z;
-            wd[3] = v.w;
+            k.wd[0] = v.x;
+            k.wd[1] = v.y;
+            k.wd[2] = v.z;
+            k.wd[3] = v.w;
         }
         const int64_t base = 16 * chunk - skip; // text position of the chunk's first memory byte
         const int mlo = base < 0 ? (int)(-base) : 0;
         const int64_t rem = (int64_t)tlen - base;
         const int mhi = rem < 16 ? (int)rem : 16;
-        const int lo = rev ? 16 - mhi : mlo; // scan-order range of text bytes
-        const int hi = rev ? 16 - mlo : mhi;
+        k.lo = rev ? 16 - mhi : mlo;
+        k.hi = rev ? 16 - mlo : mhi;
         // A0 C1 G2 T3 = ((b >> 1) ^ (b >> 2)) & 3 per byte, complemented on
         // the reverse strand (ShiftAnd.h:197-211); 'N' bytes flagged
-        uint32_t cw[4];
-        uint32_t nflag = 0;
+        k.nflag = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            cw[k] = (((wd[k] >> 1) ^ (wd[k] >> 2)) & 0x03030303u) ^ (rev ? 0x03030303u : 0u);
-            const uint32_t t = wd[k] ^ 0x4E4E4E4Eu;
-            nflag |= (t - 0x01010101u) & ~t & 0x80808080u;
+        for (int q = 0; q < 4; ++q) {
+            k.cw[q] = (((k.wd[q] >> 1) ^ (k.wd[q] >> 2)) & 0x03030303u) ^ (rev ? 0x03030303u : 0u);
+            const uint32_t t = k.wd[q] ^ 0x4E4E4E4Eu;
+            k.nflag |= (t - 0x01010101u) & ~t & 0x80808080u;
         }
-        const int64_t pb = rev ? base + 15 : base; // position of scan-order byte 0
-        const int64_t pd = rev ? -1 : 1;
-        if (lo == 0 && hi == 16 && nflag == 0) {
+        k.pb = rev ? base + 15 : base;
+        k.pd = rev ? -1 : 1;
+    };
+    // the bit-vector scan over one chunk
+    auto scan_chunk = [&](const Chunk& k) {
+        if (k.lo == 0 && k.hi == 16 && k.nflag == 0) {
 #pragma unroll
-            for (int ii = 0; ii < 16; ++ii) step((cw[ii >> 2] >> (8 * (ii & 3))) & 3u, pb + pd * ii);
+            for (int ii = 0; ii < 16; ++ii) step((k.cw[ii >> 2] >> (8 * (ii & 3))) & 3u, k.pb + k.pd * ii);
         } else {
 #pragma unroll
             for (int ii = 0; ii < 16; ++ii) {
-                if (ii < lo || ii >= hi) continue;
-                const uint32_t b = (wd[ii >> 2] >> (8 * (ii & 3))) & 0xFFu;
+                if (ii < k.lo || ii >= k.hi) continue;
+                const uint32_t b = (k.wd[ii >> 2] >> (8 * (ii & 3))) & 0xFFu;
                 if (b == 'N') { // the automaton restarts (ShiftAnd.h:142-146, 212-214)
 #pragma unroll
                     for (int w = 0; w < W; ++w) {
@@ -1167,10 +1248,144 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                     score = score0;
                     continue;
                 }
-                step((cw[ii >> 2] >> (8 * (ii & 3))) & 3u, pb + pd * ii);
+                step((k.cw[ii >> 2] >> (8 * (ii & 3))) & 3u, k.pb + k.pd * ii);
+            }
+        }
+    };
+    // filter pass: the mask of scan chunks to scan
+    uint64_t cmask[FILT_MASKW];
+#pragma unroll
+    for (int q = 0; q < FILT_MASKW; ++q) cmask[q] = 0;
+    if (filt && nchunk > 64 * FILT_MASKW) filt = false;
+    if (filt) {
+        const int sh = 2 * (FP - 1);
+        uint32_t code = 0;
+        int64_t x = 0; // scan index of the next text byte
+        // marks the scan chunks holding scan positions [s0, s1]
+        auto mark_range = [&](int64_t s0, int64_t s1) {
+            if (s0 < 0) s0 = 0;
+            if (s1 > (int64_t)tlen - 1) s1 = (int64_t)tlen - 1;
+            if (s0 > s1) return;
+            int64_t c0, c1;
+            if (!rev) {
+                c0 = (s0 + skip) >> 4;
+                c1 = (s1 + skip) >> 4;
+            } else {
+                c0 = nchunk - 1 - (((int64_t)tlen - 1 - s0 + skip) >> 4);
+                c1 = nchunk - 1 - (((int64_t)tlen - 1 - s1 + skip) >> 4);
+            }
+#pragma unroll
+            for (int q2 = 0; q2 < FILT_MASKW; ++q2) {
+                const int64_t lo = c0 > 64 * q2 ? c0 : 64 * q2;
+                const int64_t hi = c1 < 64 * q2 + 63 ? c1 : 64 * q2 + 63;
+                if (lo <= hi) cmask[q2] |= (~0ull >> (63 - (hi - lo))) << (lo - 64 * q2);
+            }
+        };
+        // zero when the code equals a piece (kept in vector registers: a
+        // compare per piece would hand each result to the scalar unit)
+        auto hitmin = [&]() -> uint32_t {
+            uint32_t mn = code ^ pc[0];
+#pragma unroll
+            for (int i = 1; i < NP; ++i) mn = min(mn, code ^ pc[i]);
+            return mn;
+        };
+        // the scan of a chunk is short next to a random load: chunks go in
+        // blocks of two, the next block loaded while this one is scanned
+        // (the empty asm keeps the loads where they are issued)
+        auto filter_chunk = [&](int64_t ci, const uint4 v) -> bool {
+            Chunk k;
+            load_chunk(ci, v, k);
+            if (k.nflag) return false; // an N restarts the automaton: scan the window whole
+            uint32_t rcw[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rcw[q] = k.cw[q] | ((k.cw[q] << 1) & 0x02020202u);
+            uint32_t acc = 0xFFFFFFFFu;
+            if (k.lo == 0 && k.hi == 16) {
+#pragma unroll
+                for (int ii = 0; ii < 16; ++ii) {
+                    code = (code >> 2) | (((rcw[ii >> 2] >> (8 * (ii & 3))) & 3u) << sh);
+                    acc = min(acc, hitmin());
+                }
+            } else {
+#pragma unroll
+                for (int ii = 0; ii < 16; ++ii) {
+                    if (ii < k.lo || ii >= k.hi) continue;
+                    code = (code >> 2) | (((rcw[ii >> 2] >> (8 * (ii & 3))) & 3u) << sh);
+                    acc = min(acc, hitmin());
+                }
+            }
+            if (acc == 0) // a piece ends in this chunk: the union of the regions of any piece there
+                mark_range(x - (int64_t)NP * FP + 1 - (int64_t)NERR,
+                           x + (k.hi - k.lo) - 1 - FP + (int64_t)m + (int64_t)NERR);
+            x += k.hi - k.lo;
+            return true;
+        };
+        if (nchunk > 0) {
+            uint4 c1 = raw_chunk_c(0), c2 = raw_chunk_c(1);
+            for (int64_t ci = 0; ci < nchunk; ci += 2) {
+                const uint4 n1 = raw_chunk_c(ci + 2), n2 = raw_chunk_c(ci + 3);
+                asm volatile("" ::: "memory");
+                if (!filter_chunk(ci, c1) || (ci + 1 < nchunk && !filter_chunk(ci + 1, c2))) {
+                    filt = false;
+                    break;
+                }
+                c1 = n1;
+                c2 = n2;
             }
         }
     }
+    if (filt) {
+        // the marked chunks in scan order, the next one loaded ahead; a fresh
+        // automaton at the start of every run of consecutive chunks
+        auto next_marked = [&]() -> int64_t {
+            int64_t ci = -1;
+#pragma unroll
+            for (int q = 0; q < FILT_MASKW; ++q)
+                if (ci < 0 && cmask[q]) {
+                    ci = 64 * q + __ffsll((long long)cmask[q]) - 1;
+                    cmask[q] &= cmask[q] - 1;
+                }
+            return ci;
+        };
+        int64_t prevci = -2;
+        int64_t ci = next_marked();
+        uint4 v1 = raw_chunk(ci);
+        while (ci >= 0) {
+            const int64_t cn = next_marked();
+            const uint4 v = v1;
+            v1 = raw_chunk(cn);
+            asm volatile("" ::: "memory");
+            if (ci != prevci + 1) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    Pv[w] = ~0ull;
+                    Mv[w] = 0;
+                }
+                score = score0;
+                if (was) flush();
+                was = false;
+            }
+            prevci = ci;
+            Chunk k;
+            load_chunk(ci, v, k);
+            // scan index of the chunk's first text byte (no N in the window)
+            const int64_t p0 = k.pb + k.pd * k.lo;
+            ncomp = (uint32_t)(rev ? (int64_t)tlen - 1 - p0 : p0);
+            scan_chunk(k);
+            ci = cn;
+        }
+    } else {
+        uint4 v1 = raw_chunk(0);
+        for (int64_t ci = 0; ci < nchunk; ++ci) {
+            const uint4 v = v1;
+            v1 = raw_chunk(ci + 1);
+            asm volatile("" ::: "memory");
+            Chunk k;
+            load_chunk(ci, v, k);
+            scan_chunk(k);
+        }
+    }
+    if (was) flush();
     if (!only) {
         bd.job_nm[pool][j] = nm;
         if (nm > cap) {
