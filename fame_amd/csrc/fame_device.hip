@@ -479,43 +479,16 @@ struct SeedStats {
     uint64_t kmers, entries, seeded, bytes;
 };
 
-// a read loaded ahead of its turn as 4-byte words from the aligned address
-// below it (an aligned word holding one byte of the read lies in that
-// byte's page); skip = the read's offset in the first word
-struct ReadWords {
-    uint32_t w0, w1, skip, L;
-};
-
-// a wave-uniform value in a vector register: loads indexed by it are vector
-// loads, waited on by vmcnt, so they can be issued a turn ahead without
-// holding up the LDS waits (scalar loads share lgkmcnt with LDS)
-__device__ __forceinline__ uint32_t in_vgpr(uint32_t x)
-{
-    uint32_t v;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"(x));
-    return v;
-}
-
 template <typename TabT, typename IdxT>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
-                        uint64_t* E_out, uint32_t* bc_out, const ReadWords* rw)
+                        uint64_t* E_out, uint32_t* bc_out)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t r = ro >> 1, ori = ro & 1;
-    const uint32_t L = rw ? __builtin_amdgcn_readfirstlane(rw->L) : bd.lens[r];
-    if (rw) { // the read as prefetched aligned words: lane l holds bytes 4l .. 4l+3 (and 256 + 4l ..)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int i0 = 4 * lane + b - (int)rw->skip;
-            if (i0 >= 0 && i0 < (int)L) w.code[i0] = (uint8_t)red_code((uint8_t)(rw->w0 >> (8 * b)));
-            const int i1 = 256 + 4 * lane + b - (int)rw->skip;
-            if (i1 < (int)L) w.code[i1] = (uint8_t)red_code((uint8_t)(rw->w1 >> (8 * b)));
-        }
-    } else {
-        const uint8_t* s = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-        for (uint32_t i = lane; i < L; i += 64) w.code[i] = (uint8_t)red_code(s[i]);
-    }
+    const uint32_t L = bd.lens[r];
+    const uint8_t* s = (ori ? bd.rev : bd.seqs) + bd.offs[r];
+    for (uint32_t i = lane; i < L; i += 64) w.code[i] = (uint8_t)red_code(s[i]);
     wave_sync();
     const uint32_t K = L - KMERLEN + 1;
     const TabT* tab = (const TabT*)ix.tab;
@@ -799,7 +772,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     __shared__ SeedShared sh;
     __shared__ SeedLds<TabT> lds[FAME_SEED_WPB];
     const int lane = threadIdx.x & 63;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+    const uint32_t wv = threadIdx.x >> 6;
     SeedLds<TabT>& m = lds[wv];
     SeedWork<TabT, uint16_t> w;
     w.klo = m.klo;
@@ -826,78 +799,33 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
     const uint32_t nw = gridDim.x * FAME_SEED_WPB;
-    // software pipeline over this wave's (read, orientation) pairs: the
-    // flags / length / offset two turns ahead and the bytes one turn ahead
-    struct Pending {
-        uint32_t ro, skip, n, L;
-        bool act;
-        const uint8_t* base;
-    };
-    auto meta = [&](uint32_t ro, Pending& p) {
-        p.ro = ro;
-        p.act = false;
-        p.skip = p.n = p.L = 0;
-        p.base = nullptr;
-        if (ro < n_ro) {
-            const uint32_t r = in_vgpr(ro >> 1), ori = ro & 1;
-            const uint8_t pre = bd.pre[r];
-            const uint32_t L = bd.lens[r];
-            const uint8_t* sp = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-            p.act = ((bd.ori_mask >> ori) & 1) && pre == 0;
-            p.skip = (uint32_t)((uintptr_t)sp & 3);
-            p.base = sp - p.skip;
-            p.n = p.skip + L;
-            p.L = L;
-        }
-    };
-    auto words = [&](const Pending& p, ReadWords& rw) {
-        rw.w0 = rw.w1 = 0;
-        rw.skip = p.skip;
-        rw.L = p.L;
-        if (p.act) {
-            if (4u * lane < p.n) rw.w0 = *(const uint32_t*)(p.base + 4 * lane);
-            if (256u + 4u * lane < p.n) rw.w1 = *(const uint32_t*)(p.base + 256 + 4 * lane);
-        }
-    };
-    const uint32_t first = blockIdx.x * FAME_SEED_WPB + wv;
-    Pending cur, nxt;
-    ReadWords cw, nwd;
-    meta(first, cur);
-    words(cur, cw);
-    meta(first + nw, nxt);
-    for (uint32_t ro = first; ro < n_ro; ro += nw) {
-        Pending nn;
-        meta(ro + 2 * nw, nn);
-        words(nxt, nwd);
-        asm volatile("" ::: "memory"); // the loads above stay ahead of this turn's work
-        if (!cur.act) {
+    for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; ro < n_ro; ro += nw) {
+        const uint32_t r = ro >> 1, ori = ro & 1;
+        if (!((bd.ori_mask >> ori) & 1) || bd.pre[r] != 0) {
             if (lane == 0) {
                 bd.cand_nf[ro] = 0;
                 bd.cand_nr[ro] = 0;
                 bd.cand_off[ro] = 0;
             }
-        } else {
-            wave_sync();
-            uint64_t E;
-            uint32_t bc;
-            const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, &cw);
-            if (rc != 0 && lane == 0) {
-                const uint32_t h = atomicAdd(bd.heavy_n, 1u);
-                if (h < bd.heavy_cap) {
-                    bd.heavy_ro[h] = ro;
-                    bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
-                    bd.heavy_bc[h] = bc;
-                } else {
-                    atomicOr(bd.flags, FLAG_HEAVY_FULL);
-                }
-                bd.cand_nf[ro] = 0;
-                bd.cand_nr[ro] = 0;
-                bd.cand_off[ro] = 0;
-            }
+            continue;
         }
-        cur = nxt;
-        cw = nwd;
-        nxt = nn;
+        wave_sync();
+        uint64_t E;
+        uint32_t bc;
+        const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc);
+        if (rc != 0 && lane == 0) {
+            const uint32_t h = atomicAdd(bd.heavy_n, 1u);
+            if (h < bd.heavy_cap) {
+                bd.heavy_ro[h] = ro;
+                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_bc[h] = bc;
+            } else {
+                atomicOr(bd.flags, FLAG_HEAVY_FULL);
+            }
+            bd.cand_nf[ro] = 0;
+            bd.cand_nr[ro] = 0;
+            bd.cand_off[ro] = 0;
+        }
     }
     if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
     if (lane == 0 && ss.seeded) {
@@ -951,7 +879,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     uint32_t bc;
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
-    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, nullptr);
+    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc);
     if (threadIdx.x == 0 && ss.seeded) {
         atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
@@ -1015,8 +943,10 @@ __device__ __forceinline__ void myers_step(uint64_t (&Pv)[W], uint64_t (&Mv)[W],
 // scan restarted at the start of that region therefore reproduces every
 // end position with at most NERR errors there, with its error count, and
 // no position outside the regions is one.  Regions are kept as a mask of
-// 16-byte scan chunks; a run of consecutive marked chunks is scanned from a
-// fresh automaton.  Windows holding an N, patterns shorter than
+// 16-byte scan chunks: a chunk in which some piece ends marks the union of
+// the regions of any piece ending anywhere in it (which piece it was is not
+// tracked), and a run of consecutive marked chunks is scanned from a fresh
+// automaton.  Windows holding an N, patterns shorter than
 // (NERR+1)*FILT_MIN_P and windows wider than the mask are scanned whole.
 constexpr int FILT_MIN_P = 10;
 constexpr int FILT_MAX_P = 14;
