@@ -327,9 +327,14 @@ __device__ int hs_insert(Hs<IdxT>& h, uint32_t idx, int lane)
 
 // one wavefront per read (persistent, grid-stride): coalesced loads of the
 // read, the reverse complement stored reversed, N found by a ballot
+// Also writes, per (read, orientation) to be seeded, a 256-byte row of the
+// reduced letters (red_code) with the length in byte 255 (0 for a pair the
+// seed stage skips), so that the seed stage reads one coalesced row per pair
+// instead of chasing offsets.
 __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, const uint64_t* __restrict__ offs,
                                               const uint32_t* __restrict__ lens, uint32_t n, uint32_t readlen,
                                               uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
+                                              uint8_t* __restrict__ codes, uint32_t ori_mask,
                                               uint32_t* __restrict__ err_flag)
 {
     const int lane = threadIdx.x & 63;
@@ -348,6 +353,8 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
             st = 3;
         }
         if (L < KMERLEN) st = 3;
+        uint8_t* c0 = codes + (size_t)(2 * r) * 256;
+        uint8_t* c1 = c0 + 256;
         if (st == 0) {
             bool nflag = false;
             for (uint32_t p = lane; p < L; p += 64) {
@@ -362,10 +369,16 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                 default: break; // unknown characters stay '\0' in revSeq
                 }
                 q[L - 1 - p] = o;
+                if (ori_mask & 1) c0[p] = (uint8_t)red_code(c);
+                if (ori_mask & 2) c1[L - 1 - p] = (uint8_t)red_code(o);
             }
             if (__ballot(nflag)) st = 3;
         }
-        if (lane == 0) pre[r] = st;
+        if (lane == 0) {
+            pre[r] = st;
+            c0[255] = (st == 0 && (ori_mask & 1)) ? (uint8_t)L : 0;
+            c1[255] = (st == 0 && (ori_mask & 2)) ? (uint8_t)L : 0;
+        }
     }
 }
 
@@ -482,13 +495,13 @@ struct SeedStats {
 template <typename TabT, typename IdxT>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
-                        uint64_t* E_out, uint32_t* bc_out)
+                        uint64_t* E_out, uint32_t* bc_out, uint32_t cw)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t r = ro >> 1, ori = ro & 1;
-    const uint32_t L = bd.lens[r];
-    const uint8_t* s = (ori ? bd.rev : bd.seqs) + bd.offs[r];
-    for (uint32_t i = lane; i < L; i += 64) w.code[i] = (uint8_t)red_code(s[i]);
+    // cw: this lane's word of the row of reduced letters (the length is in
+    // byte 255, the top byte of lane 63's word)
+    const uint32_t L = (uint32_t)__shfl(cw, 63, 64) >> 24;
+    ((uint32_t*)w.code)[lane] = cw;
     wave_sync();
     const uint32_t K = L - KMERLEN + 1;
     const TabT* tab = (const TabT*)ix.tab;
@@ -760,7 +773,7 @@ struct SeedLds {
             uint32_t cand[2 * L_CAP_INS];
         } c;
     } v;
-    uint8_t code[FAME_DEV_MAX_READ + 1];
+    alignas(4) uint8_t code[256];
 };
 
 // FAME_SEED_WPB independent waves per block (one block slot of a CU then
@@ -800,8 +813,8 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     SeedStats ss = {0, 0, 0, 0};
     const uint32_t nw = gridDim.x * FAME_SEED_WPB;
     for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; ro < n_ro; ro += nw) {
-        const uint32_t r = ro >> 1, ori = ro & 1;
-        if (!((bd.ori_mask >> ori) & 1) || bd.pre[r] != 0) {
+        const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
+        if ((__shfl(cw, 63, 64) >> 24) == 0) { // not seeded: invalid read or orientation not asked for
             if (lane == 0) {
                 bd.cand_nf[ro] = 0;
                 bd.cand_nr[ro] = 0;
@@ -812,7 +825,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         wave_sync();
         uint64_t E;
         uint32_t bc;
-        const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc);
+        const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, cw);
         if (rc != 0 && lane == 0) {
             const uint32_t h = atomicAdd(bd.heavy_n, 1u);
             if (h < bd.heavy_cap) {
@@ -841,7 +854,7 @@ template <typename TabT>
 __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, HeavyScratch hs)
 {
     __shared__ SeedShared sh;
-    __shared__ uint8_t s_code[FAME_DEV_MAX_READ + 1];
+    __shared__ alignas(4) uint8_t s_code[256];
     const uint32_t hi = blockIdx.x;
     if (hi >= hs.count) return;
     const uint32_t ro = bd.heavy_ro[hs.first + hi];
@@ -879,7 +892,8 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     uint32_t bc;
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
-    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc);
+    const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
+    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
     if (threadIdx.x == 0 && ss.seeded) {
         atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
@@ -2007,7 +2021,7 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
     if (!n) return;
     const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
-                       bd.lens, n, readlen, bd.rev, bd.pre, bd.flags);
+                       bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags);
 }
 
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)

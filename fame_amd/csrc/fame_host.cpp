@@ -82,7 +82,7 @@ struct fame_ctx {
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
     // batch buffers
-    DevBuf b_seqs, b_offs, b_lens, b_rev, b_pre, b_cand_off, b_cand_nf, b_cand_nr, b_pool_n;
+    DevBuf b_seqs, b_offs, b_lens, b_rev, b_pre, b_codes, b_cand_off, b_cand_nf, b_cand_nr, b_pool_n;
     DevBuf b_job_meta[FAME_NPOOL], b_job_ro[FAME_NPOOL], b_job_nm[FAME_NPOOL], b_job_match[FAME_NPOOL],
         b_job_ovf_off[FAME_NPOOL];
     DevBuf b_ovf_n, b_ovf_pool, b_ovf_job, b_ovf_buf, b_ovf_total;
@@ -357,6 +357,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
     bd.ori_mask = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
     const uint32_t n_ro = 2 * n;
+    bd.codes = grow<uint8_t>(c->b_codes, (size_t)n_ro * 256 + 256);
     bd.cand_off = grow<uint32_t>(c->b_cand_off, n_ro + 1);
     bd.cand_nf = grow<uint32_t>(c->b_cand_nf, n_ro + 1);
     bd.cand_nr = grow<uint32_t>(c->b_cand_nr, n_ro + 1);
