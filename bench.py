@@ -139,18 +139,12 @@ def main():
         d_mat = torch.zeros(len(lens), dtype=torch.int64, device=dev)
         torch.cuda.synchronize()
         n = len(lens)
-        stage_ms = {}
-        work = {}
-        launches = [0]
+        calls = []  # per call: reads, stage timings (device ms), work counts
 
         def call(s, e, stranded, cnt):
-            launches[0] += 1
             ctx.match_device(d_seqs.data_ptr(), d_offs.data_ptr() + 8 * s, d_lens.data_ptr() + 4 * s,
                              e - s, stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s)
-            for k, v in ctx.timings().items():
-                stage_ms[k] = stage_ms.get(k, 0.0) + v
-            for k, v in ctx.stats().items():
-                work[k] = v if k == "tab_bytes" else work.get(k, 0) + v
+            calls.append((e - s, ctx.timings(), ctx.stats()))
 
         def step():
             ctx.reset()
@@ -167,9 +161,7 @@ def main():
 
         for _ in range(args.warmup):
             step()
-        stage_ms.clear()
-        work.clear()
-        launches[0] = 0
+        calls.clear()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -181,24 +173,23 @@ def main():
         value = succ / (ms_step / 1e3)
         log(f"step {ms_step:.1f} ms; matched {succ} nonunique {int(cnt[1])} unmatched {int(cnt[2])} of {n}")
         K = max(1, args.steps)
-        per_stage = {k: v / K for k, v in stage_ms.items()}
-        per_work = {k: (v // K if k != "tab_bytes" else v) for k, v in work.items()}
+        per_stage, per_work = {}, {}
+        for _, tm, st in calls:
+            for k, v in tm.items():
+                per_stage[k] = per_stage.get(k, 0.0) + v / K
+            for k, v in st.items():
+                per_work[k] = v if k == "tab_bytes" else per_work.get(k, 0) + v
+        per_work = {k: (v // K if k != "tab_bytes" else v) for k, v in per_work.items()}
         log("stage ms per step:", {k: round(v, 2) for k, v in per_stage.items()})
         log("work per step:", per_work)
         # ---------------- roofline ----------------
-        seed_bytes = (2 * per_work["tab_bytes"] * per_work["kmers"] + 4 * per_work["entries"] +
-                      per_work["read_bytes"] + 8 * per_work["jobs"])
-        seed_gbs = seed_bytes / (per_stage["seed"] / 1e3) / 1e9 if per_stage.get("seed") else 0.0
+        # over the full-size calls (one launch of each kernel per call)
+        full = [c for c in calls if c[0] == args.batch] or calls
         dev_stages = {k: per_stage[k] for k in ("prep", "seed", "verify", "select", "meth")}
         dominant = max(dev_stages, key=dev_stages.get)
-        per_launch = launches[0] / K
-        roof = {"kernel": "k_seed_light (seed + q-gram filter)", "bound": "hbm",
-                "achieved": round(seed_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "launches_per_step": per_launch,
-                "bytes_per_launch": int(seed_bytes / per_launch),
-                "avg_launch_ms": round(per_stage["seed"] / per_launch, 3),
-                "dominant_stage": dominant}
+        roof = roofline("seed", full)
+        roof["dominant_stage"] = dominant
+        roof_verify = roofline("verify", full)
         verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
         out = {
             "metric": METRIC, "value": round(value, 1), "unit": "reads/s", "n_gpus": 1,
@@ -215,6 +206,7 @@ def main():
             "work": per_work,
             "verify_window_GBps": round(verify_rate / 1e9, 2),
             "roofline": roof,
+            "roofline_verify": roof_verify,
             "setup_s": {"index_build": round(t_build, 1), "reads": round(t_reads, 1),
                         "index_load": round(t_load, 1)},
         }
@@ -226,6 +218,46 @@ def main():
         print(json.dumps(out), flush=True)
     finally:
         shutil.rmtree(wd, ignore_errors=True)
+
+
+def algorithmic_bytes(stage, st):
+    """Bytes the stage's algorithm must move for one call, from the kernels'
+    own work counters: seed = two tabIndex reads per k-mer, one index entry
+    per bucket hit, the read's letters and the candidate records written;
+    verify = the window text of every candidate (WINLEN + MISCOUNT + ADDMIS - 1
+    letters, as FAME's ShiftAnd scans it)."""
+    if stage == "seed":
+        return 2 * st["tab_bytes"] * st["kmers"] + 4 * st["entries"] + st["read_bytes"] + 8 * st["jobs"]
+    return st["window_bytes"]
+
+
+TRAFFIC = {"seed": ("k_seed_light", "k_seed_light_genome300M_2Mreads_traffic.md"),
+           "verify": ("k_verify<2>", "k_verify2_genome300M_2Mreads_traffic.md")}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC[stage][1])
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1].strip())
+    return None
+
+
+def roofline(stage, calls):
+    n = len(calls)
+    ms = sum(c[1][stage] for c in calls) / n
+    nbytes = sum(algorithmic_bytes(stage, c[2]) for c in calls) / n
+    gbs = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    traffic = measured_traffic(stage)
+    return {"kernel": TRAFFIC[stage][0], "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_GBps": round(traffic / (ms / 1e3) / 1e9, 1) if traffic and ms > 0 else None,
+            "algorithmic_bytes_per_launch": int(nbytes), "avg_launch_ms": round(ms, 3),
+            "launches": n, "reads_per_launch": int(sum(c[0] for c in calls) / n)}
 
 
 def copy_probe(mib):
