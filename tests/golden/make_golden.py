@@ -43,6 +43,11 @@ CASES = {
                           n=3000, seed=14, unord=False, flip=True),
     "se_rl150": dict(chroms=[80000, 50000], repeats=8, htab=1 << 15, readlen=150, chunk=300000,
                      n=4000, seed=15, unord=False, flip=False, len_range=(90, 150)),
+    # reads of 120..255 letters against a READLEN 100 index: every verify pool
+    # width (two to four 64-bit words) and long DP tracebacks
+    "se_long": dict(chroms=[60000, 45000], repeats=0, htab=1 << 16, readlen=100, chunk=300000,
+                    n=3000, seed=16, unord=False, flip=False, len_range=(120, 255), gen_len=255,
+                    share="se_basic"),
 }
 
 
@@ -79,7 +84,7 @@ def make_genome(cfg):
 def make_reads(cfg, chroms):
     rng = np.random.default_rng(cfg["seed"] + 2000)
     RL = cfg["readlen"]
-    rs = synth.make_reads(chroms, cfg["n"], RL, seed=cfg["seed"] + 3,
+    rs = synth.make_reads(chroms, cfg["n"], cfg.get("gen_len", RL), seed=cfg["seed"] + 3,
                           len_range=cfg.get("len_range"))
     seqs = [bytes(rs.seqs[i, :rs.lens[i]]) for i in range(len(rs))]
     n = len(seqs)
