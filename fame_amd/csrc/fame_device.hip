@@ -335,9 +335,10 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                                               const uint32_t* __restrict__ lens, uint32_t n, uint32_t readlen,
                                               uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
                                               uint8_t* __restrict__ codes, uint32_t ori_mask,
-                                              uint32_t* __restrict__ err_flag)
+                                              uint32_t* __restrict__ err_flag, uint32_t* __restrict__ max_len)
 {
     const int lane = threadIdx.x & 63;
+    uint32_t lmax = 0; // longest read this wave prepared for seeding
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t r = wave; r < n; r += nwave) {
@@ -374,12 +375,14 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
             }
             if (__ballot(nflag)) st = 3;
         }
+        if (st == 0 && L > lmax) lmax = L;
         if (lane == 0) {
             pre[r] = st;
             c0[255] = (st == 0 && (ori_mask & 1)) ? (uint8_t)L : 0;
             c1[255] = (st == 0 && (ori_mask & 2)) ? (uint8_t)L : 0;
         }
     }
+    if (lane == 0 && lmax) atomicMax(max_len, lmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,7 +495,7 @@ struct SeedStats {
     uint64_t kmers, entries, seeded, bytes;
 };
 
-template <typename TabT, typename IdxT>
+template <typename TabT, typename IdxT, int KPL>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out, uint32_t cw)
@@ -507,7 +510,6 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     const TabT* tab = (const TabT*)ix.tab;
     // hashes + tabIndex gathers (NTPS64 closed form; key = h % HTABSIZE)
     // all gathers of a lane are issued before the first is consumed
-    constexpr int KPL = (FAME_DEV_MAX_READ - KMERLEN + 1 + 63) / 64;
     TabT lo[KPL], hi[KPL];
 #pragma unroll
     for (int q = 0; q < KPL; ++q) {
@@ -735,42 +737,51 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     return 0;
 }
 
-constexpr uint32_t L_CAP_E = 256;   // index hits per (read, orientation) in LDS
-constexpr uint32_t L_CAP_INS = 128; // keys per map in LDS
-constexpr uint32_t L_TB = 256 + NBH - 1;
-constexpr uint32_t L_HASH = 256;    // counting table slots (>= L_CAP_E)
-static_assert(L_HASH >= L_CAP_E && (L_HASH & (L_HASH - 1)) == 0, "counting table");
+// Capacities of the light seed kernel for reads of at most ML letters
+// (ML = 128 or 256): index hits ML, keys per map ML/2, counting slots ML.
+// Larger (read, orientation) pairs go to the heavy kernel.
+template <int ML>
+struct SeedCap {
+    static constexpr uint32_t E = ML;
+    static constexpr uint32_t INS = ML / 2;
+    static constexpr uint32_t TB = ML + NBH - 1;
+    static constexpr uint32_t HASH = ML;
+    static constexpr int KPL = (ML - (int)KMERLEN + 63) / 64; // k-mers per lane
+    static_assert((HASH & (HASH - 1)) == 0 && HASH >= E, "counting table");
+};
+constexpr int SEED_SHORT = 128; // read length bound of the short variant
 
 // per-wave LDS of the light seed kernel; the gather / count arrays and the
 // hopscotch arrays share storage
-template <typename TabT>
+template <typename TabT, int ML>
 struct SeedLds {
-    TabT klo[FAME_DEV_MAX_READ];
-    uint32_t klen[FAME_DEV_MAX_READ];
-    uint32_t koff[FAME_DEV_MAX_READ];
+    using C = SeedCap<ML>;
+    TabT klo[ML];
+    uint32_t klen[ML];
+    uint32_t koff[ML];
     union {
         struct {
-            uint32_t ent[L_CAP_E];
-            uint32_t eaux[L_CAP_E];
-            uint8_t ekm[L_CAP_E];
+            uint32_t ent[C::E];
+            uint32_t eaux[C::E];
+            uint8_t ekm[C::E];
         } a;
         struct {
-            uint16_t tb[L_TB];
-            uint16_t tmp[L_CAP_INS];
-            uint16_t over[L_CAP_INS];
+            uint16_t tb[C::TB];
+            uint16_t tmp[C::INS];
+            uint16_t over[C::INS];
         } b;
     } u;
     // the counting table is dead before the insertion lists are written
     union {
         struct {
-            uint32_t key[L_HASH];
-            uint32_t first[L_HASH];
-            uint32_t cnt[L_HASH];
+            uint32_t key[C::HASH];
+            uint32_t first[C::HASH];
+            uint32_t cnt[C::HASH];
         } h;
         struct {
-            uint32_t ins_key[2 * L_CAP_INS];
-            uint16_t ins_cnt[2 * L_CAP_INS];
-            uint32_t cand[2 * L_CAP_INS];
+            uint32_t ins_key[2 * C::INS];
+            uint16_t ins_cnt[2 * C::INS];
+            uint32_t cand[2 * C::INS];
         } c;
     } v;
     alignas(4) uint8_t code[256];
@@ -778,15 +789,19 @@ struct SeedLds {
 
 // FAME_SEED_WPB independent waves per block (one block slot of a CU then
 // holds several waves); each wave works through (read, orientation) pairs
-// in grid-stride order
-template <typename TabT>
+// in grid-stride order.  Both variants are launched; the one that does not
+// fit the batch's longest read (from k_prep) returns at once.
+template <typename TabT, int ML>
 __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
 {
+    using C = SeedCap<ML>;
+    const bool short_batch = *bd.max_len <= (uint32_t)SEED_SHORT;
+    if ((ML == SEED_SHORT) != short_batch) return;
     __shared__ SeedShared sh;
-    __shared__ SeedLds<TabT> lds[FAME_SEED_WPB];
+    __shared__ SeedLds<TabT, ML> lds[FAME_SEED_WPB];
     const int lane = threadIdx.x & 63;
     const uint32_t wv = threadIdx.x >> 6;
-    SeedLds<TabT>& m = lds[wv];
+    SeedLds<TabT, ML>& m = lds[wv];
     SeedWork<TabT, uint16_t> w;
     w.klo = m.klo;
     w.klen = m.klen;
@@ -794,16 +809,16 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     w.ent = m.u.a.ent;
     w.ekm = m.u.a.ekm;
     w.eaux = m.u.a.eaux;
-    w.cap_e = L_CAP_E;
-    w.hsize = L_HASH;
+    w.cap_e = C::E;
+    w.hsize = C::HASH;
     w.hkey = m.v.h.key;
     w.hfirst = m.v.h.first;
     w.hcnt = m.v.h.cnt;
     w.ins_key = m.v.c.ins_key;
     w.ins_cnt = m.v.c.ins_cnt;
-    w.cap_ins = L_CAP_INS;
+    w.cap_ins = C::INS;
     w.tb = m.u.b.tb;
-    w.tb_cap = L_TB;
+    w.tb_cap = C::TB;
     w.tmp = m.u.b.tmp;
     w.over = m.u.b.over;
     w.cand = m.v.c.cand;
@@ -825,7 +840,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         wave_sync();
         uint64_t E;
         uint32_t bc;
-        const int rc = seed_one<TabT, uint16_t>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, cw);
+        const int rc = seed_one<TabT, uint16_t, C::KPL>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, cw);
         if (rc != 0 && lane == 0) {
             const uint32_t h = atomicAdd(bd.heavy_n, 1u);
             if (h < bd.heavy_cap) {
@@ -893,7 +908,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
     const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
-    const int rc = seed_one<TabT, uint32_t>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
+    const int rc = seed_one<TabT, uint32_t, SeedCap<256>::KPL>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
     if (threadIdx.x == 0 && ss.seeded) {
         atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
@@ -2021,7 +2036,7 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
     if (!n) return;
     const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
-                       bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags);
+                       bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len);
 }
 
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)
@@ -2029,10 +2044,13 @@ void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hi
     if (!n_ro) return;
     const uint32_t waves = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
     const uint32_t g = (waves + FAME_SEED_WPB - 1) / FAME_SEED_WPB;
-    if (ix.tab_is64)
-        hipLaunchKernelGGL(k_seed_light<uint64_t>, dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-    else
-        hipLaunchKernelGGL(k_seed_light<uint32_t>, dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    if (ix.tab_is64) {
+        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, 256>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    } else {
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    }
 }
 
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st)

@@ -383,6 +383,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.counters = c->d_counters.as<uint64_t>();
     bd.meth = c->d_meth.as<uint32_t>();
     bd.flags = c->d_flags.as<uint32_t>();
+    bd.max_len = bd.flags + 1;
     bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
     HIPC(hipMemsetAsync(bd.stats, 0, 8 * FAME_NSTATS, st));
     bd.direct_list = grow<uint32_t>(c->b_direct, n + 1);
@@ -391,7 +392,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.dp_n = bd.direct_n + 1;
     HIPC(hipMemsetAsync(bd.direct_n, 0, 8, st));
 
-    HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    HIPC(hipMemsetAsync(bd.flags, 0, 8, st)); // flags and the longest read
     tm.mark(0);
     launch_prep(bd, n, c->readlen, st);
     tm.mark(1);

@@ -58,6 +58,7 @@ struct BatchDev {
     uint8_t* rev;
     uint8_t* pre;      // 0 = to match, 3 = invalid (short, N, too long)
     uint8_t* codes;    // [2n][256]: reduced letters of each (read, orientation), byte 255 = length, 0 = skip
+    uint32_t* max_len; // longest read to be seeded (k_prep)
     uint32_t ori_mask; // bit0: query ori 0, bit1: query ori 1
     // candidate windows per ro, in hopscotch iteration order (fwd map then rev map)
     uint32_t* cand_off;
