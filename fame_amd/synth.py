@@ -218,6 +218,54 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
     return ReadSet(seqs, lens, quals, {"chrom": t_chrom, "pos": t_pos, "ob": t_ob})
 
 
+def make_pairs(chroms, n, read_len, gap=(20, 400), seed=3, len_range=None, block=1 << 20, **kw):
+    """Sample `n` read pairs of a directional bisulfite library.
+
+    Each pair comes from one converted fragment of length L1 + g + L2
+    (g uniform in `gap`, mate lengths uniform in `len_range` or read_len):
+    mate 1 is the fragment's first L1 letters, mate 2 the reverse complement
+    of its last L2 letters.  The fragments are drawn by `make_reads` (same
+    bisulfite, error and strand model) in blocks, each truncated to its own
+    length, so the output depends only on the arguments.
+    Returns (ReadSet mate 1, ReadSet mate 2); truth["gap"] holds g.
+    """
+    W = 2 * read_len + gap[1]
+    out1 = np.zeros((n, read_len), np.uint8)
+    out2 = np.zeros((n, read_len), np.uint8)
+    l1 = np.zeros(n, np.uint32)
+    l2 = np.zeros(n, np.uint32)
+    truth = {"chrom": np.zeros(n, np.int32), "pos": np.zeros(n, np.int64),
+             "ob": np.zeros(n, bool), "gap": np.zeros(n, np.int64)}
+    j = np.arange(read_len)[None, :]
+    for b, s in enumerate(range(0, n, block)):
+        m = min(block, n - s)
+        fr = make_reads(chroms, m, W, seed=seed * 7919 + b, **kw)
+        rng = np.random.default_rng([seed, b, 17])
+        if len_range is None:
+            a1 = np.full(m, read_len, np.int64)
+            a2 = np.full(m, read_len, np.int64)
+        else:
+            a1 = rng.integers(len_range[0], len_range[1] + 1, size=m)
+            a2 = rng.integers(len_range[0], len_range[1] + 1, size=m)
+        g = rng.integers(gap[0], gap[1] + 1, size=m)
+        end = a1 + g + a2
+        m1 = fr.seqs[:, :read_len].copy()
+        m1[j >= a1[:, None]] = 0
+        # mate 2: letters [end-L2, end) of the fragment, reverse complemented,
+        # left-aligned in a read_len row
+        src = np.clip(end[:, None] - 1 - j, 0, W - 1)
+        m2 = _COMP[np.take_along_axis(fr.seqs, src, axis=1)]
+        m2[j >= a2[:, None]] = 0
+        out1[s:s + m] = m1
+        out2[s:s + m] = m2
+        l1[s:s + m] = a1
+        l2[s:s + m] = a2
+        for k in ("chrom", "pos", "ob"):
+            truth[k][s:s + m] = fr.truth[k]
+        truth["gap"][s:s + m] = g
+    return ReadSet(out1, l1, None, truth), ReadSet(out2, l2, None, dict(truth))
+
+
 def config_genome(name, scale=1.0, seed=1):
     """Chromosome lengths for the named workloads.
 

@@ -79,6 +79,8 @@ struct orc_ctx {
     int match_r1_fwd;
     int both;
     uint64_t ub;
+    uint32_t minpd, maxpd;  /* MINPDIST / MAXPDIST (CONST.h:41-42) */
+    uint64_t* filt_sorted;  /* filteredKmers, sorted (paired path) */
 };
 
 /* ------------------------------------------------------------------ */
@@ -307,7 +309,7 @@ void orc_ntps64_rolling(const uint8_t* seq, uint32_t len, uint32_t seedbits, uin
 }
 
 /* ------------------------------------------------------------------ */
-/* tsl::hopscotch_map<uint32_t, uint16_t, MetaHash> (ReadQueue.h:137-144, */
+/* tsl::hopscotch_map<uint32_t, V, MetaHash> (ReadQueue.h:137-144,       */
 /* hopscotch_map.h:76-79, hopscotch_hash.h) -- physical emulation        */
 /* ------------------------------------------------------------------ */
 
@@ -319,10 +321,11 @@ typedef struct {
     uint64_t arr;   /* m_buckets.size() */
     uint64_t* info; /* bit 0 occupied, bit 1 overflow, bits 2.. neighbours */
     uint32_t* key;
-    uint16_t* val;
+    uint32_t* val; /* the value: a u16 count (single-end) or a packed
+                      tuple<u8, u8, bool, bool> (paired: c0 | c1 << 8 | f2 << 16 | f3 << 17) */
     uint64_t nelem; /* m_nb_elements (includes the overflow list) */
     uint32_t* okey;
-    uint16_t* oval;
+    uint32_t* oval;
     uint64_t nover, ocap;
     uint64_t max_thr, min_thr;
 } hsmap;
@@ -346,7 +349,7 @@ static void hs_init(hsmap* m, uint64_t bucket_count)
         m->arr = bucket_count + NBH - 1;
         m->info = calloc(m->arr, 8);
         m->key = calloc(m->arr, 4);
-        m->val = calloc(m->arr, 2);
+        m->val = calloc(m->arr, 4);
     }
     const float ml = 0.9f;
     const float bc = (float)(m->arr ? m->arr - NBH + 1 : 0);
@@ -368,12 +371,12 @@ static inline uint64_t hs_next_count(const hsmap* m) { return (m->mask + 1) * 2;
 
 static inline void hs_toggle(hsmap* m, uint64_t b, uint64_t i) { m->info[b] ^= 1ULL << (i + 2); }
 
-static void hs_push_over(hsmap* m, uint32_t k, uint16_t v)
+static void hs_push_over(hsmap* m, uint32_t k, uint32_t v)
 {
     if (m->nover == m->ocap) {
         m->ocap = m->ocap ? 2 * m->ocap : 8;
         m->okey = realloc(m->okey, m->ocap * 4);
-        m->oval = realloc(m->oval, m->ocap * 2);
+        m->oval = realloc(m->oval, m->ocap * 4);
     }
     m->okey[m->nover] = k;
     m->oval[m->nover] = v;
@@ -381,7 +384,7 @@ static void hs_push_over(hsmap* m, uint32_t k, uint16_t v)
 }
 
 /* find_value_impl hopscotch_hash.h:1589-1605 */
-static uint16_t* hs_find(hsmap* m, uint32_t k)
+static uint32_t* hs_find(hsmap* m, uint32_t k)
 {
     if (!m->arr) return NULL; /* static empty bucket: no neighbours, no overflow */
     uint64_t ib = k & m->mask;
@@ -397,7 +400,7 @@ static uint16_t* hs_find(hsmap* m, uint32_t k)
     return NULL;
 }
 
-static void hs_insert_impl(hsmap* m, uint64_t ib, uint32_t k, uint16_t v);
+static void hs_insert_impl(hsmap* m, uint64_t ib, uint32_t k, uint32_t v);
 
 /* rehash_impl hopscotch_hash.h:1244-1290 (+ rehash :1179-1182) */
 static void hs_rehash(hsmap* m, uint64_t count)
@@ -459,7 +462,7 @@ static int hs_will_change(const hsmap* m, uint64_t ib)
 }
 
 /* insert_impl hopscotch_hash.h:1426-1458 */
-static void hs_insert_impl(hsmap* m, uint64_t ib, uint32_t k, uint16_t v)
+static void hs_insert_impl(hsmap* m, uint64_t ib, uint32_t k, uint32_t v)
 {
     if (m->nelem - m->nover >= m->max_thr) {
         hs_rehash(m, hs_next_count(m));
@@ -497,25 +500,27 @@ static void hs_insert_impl(hsmap* m, uint64_t ib, uint32_t k, uint16_t v)
     hs_insert_impl(m, k & m->mask, k, v);
 }
 
-/* operator[] followed by ++ (value-initialised u16 then incremented) */
+/* operator[]: the value of k, inserted value-initialised (0) when absent */
+static uint32_t* hs_get(hsmap* m, uint32_t k)
+{
+    uint32_t* p = hs_find(m, k);
+    if (p) return p;
+    /* a map without buckets (static empty bucket): insert_impl sees 0 >= 0
+     * and rehashes first */
+    hs_insert_impl(m, m->arr ? (k & m->mask) : 0, k, 0);
+    return hs_find(m, k);
+}
+
+/* operator[] followed by ++ on a u16 count */
 static void hs_bump_or_insert(hsmap* m, uint32_t k)
 {
-    uint16_t* p = hs_find(m, k);
-    if (p) {
-        ++*p;
-        return;
-    }
-    if (!m->arr) {
-        /* static empty bucket: insert_impl sees 0 >= 0 and rehashes first */
-        hs_insert_impl(m, 0, k, 1);
-        return;
-    }
-    hs_insert_impl(m, k & m->mask, k, 1);
+    uint32_t* p = hs_get(m, k);
+    *p = (uint16_t)(*p + 1);
 }
 
 /* iteration order: buckets in physical order, then the overflow list
  * (hopscotch_iterator hopscotch_hash.h:562-573, begin() :752-759) */
-static int hs_items(const hsmap* m, uint32_t* keys, uint16_t* vals, int cap)
+static int hs_items(const hsmap* m, uint32_t* keys, uint32_t* vals, int cap)
 {
     int n = 0;
     for (uint64_t b = 0; b < m->arr; ++b)
@@ -585,8 +590,8 @@ static void seed_refs(const orc_index* ix, const uint8_t* seq, uint32_t L, hsmap
             if (may_insert) {
                 hs_bump_or_insert(m, metaId);
             } else {
-                uint16_t* p = hs_find(m, metaId);
-                if (p) ++*p;
+                uint32_t* p = hs_find(m, metaId);
+                if (p) *p = (uint16_t)(*p + 1);
             }
         }
     }
@@ -599,8 +604,12 @@ int orc_seed_maps(orc_ctx* c, const uint8_t* seq, uint32_t len, uint32_t* fwd_ke
     if (len < KMERLEN) return -1;
     hsmap f, r;
     seed_refs(c->ix, seq, len, &f, &r, bucket_count);
-    *nfwd = hs_items(&f, fwd_keys, fwd_cnt, cap);
-    *nrev = hs_items(&r, rev_keys, rev_cnt, cap);
+    uint32_t* v = xmalloc((size_t)(cap > 0 ? cap : 1) * 4);
+    *nfwd = hs_items(&f, fwd_keys, v, cap);
+    for (int i = 0; i < *nfwd && i < cap; ++i) fwd_cnt[i] = (uint16_t)v[i];
+    *nrev = hs_items(&r, rev_keys, v, cap);
+    for (int i = 0; i < *nrev && i < cap; ++i) rev_cnt[i] = (uint16_t)v[i];
+    free(v);
     hs_free(&f);
     hs_free(&r);
     return 0;
@@ -854,7 +863,7 @@ static int sa_query_seed_set(const orc_index* ix, orc_sa* sa, hsmap* fwd, hsmap*
         }
         int nk = hs_items(m, NULL, NULL, 0);
         uint32_t* keys = xmalloc((size_t)nk * 4 + 4);
-        uint16_t* cnts = xmalloc((size_t)nk * 2 + 2);
+        uint32_t* cnts = xmalloc((size_t)nk * 4 + 4);
         hs_items(m, keys, cnts, nk);
         for (int ki = 0; ki < nk; ++ki) {
             if (cnts[ki] < QTHRESH) continue; /* q-gram lemma :1950 / :2044 */
@@ -1200,6 +1209,8 @@ orc_ctx* orc_ctx_new(orc_index* ix, int both_strands)
     c->meth = calloc(ix->ncpg * 4 + 4, 2);
     c->match_r1_fwd = 1;
     c->both = both_strands;
+    c->minpd = 50;
+    c->maxpd = 550;
     return c;
 }
 
@@ -1207,6 +1218,7 @@ void orc_ctx_free(orc_ctx* c)
 {
     if (!c) return;
     free(c->meth);
+    free(c->filt_sorted);
     free(c);
 }
 
@@ -1371,5 +1383,693 @@ int orc_write_tsv(const orc_ctx* c, const char* path)
                 lv[0], lv[1], lv[2], lv[3]);
     }
     fclose(f);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* paired-end: matchPairedReads ReadQueue.cpp:935-1655                  */
+/* ------------------------------------------------------------------ */
+
+/* map values of the paired maps: tuple<u8 c0, u8 c1, bool f2, bool f3> */
+#define P_C0(v) ((v) & 0xffu)
+#define P_C1(v) (((v) >> 8) & 0xffu)
+#define P_F2 (1u << 16)
+#define P_F3 (1u << 17)
+#define KMERDIST 10
+static inline uint32_t p_inc0(uint32_t v) { return (v & ~0xffu) | ((v + 1) & 0xffu); }
+static inline uint32_t p_inc1(uint32_t v) { return (v & ~0xff00u) | ((v + 0x100u) & 0xff00u); }
+
+void orc_set_pdist(orc_ctx* c, uint32_t minpd, uint32_t maxpd)
+{
+    c->minpd = minpd;
+    c->maxpd = maxpd;
+}
+
+static int cmp_u64(const void* a, const void* b)
+{
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* ref.filteredKmers.find(kSeq) with the u32 kSeq widened to u64 */
+static int filt_has(const orc_ctx* cx, uint32_t kseq)
+{
+    const orc_index* ix = cx->ix;
+    if (!ix->nfilt) return 0;
+    uint64_t k = kseq;
+    return bsearch(&k, cx->filt_sorted, ix->nfilt, 8, cmp_u64) != NULL;
+}
+
+/* the u32 kSeq (two bits per letter: C,T 3, G 2, others 0; the last 16
+ * letters survive) and the C mask of a k-mer, as getSeedRefsFirstRead /
+ * SecondRead build them (:2518-2550, :2606-2640) */
+static inline uint32_t kseq_code(uint8_t c) { return (c == 'C' || c == 'T') ? 3u : (c == 'G' ? 2u : 0u); }
+
+/* getSeedRefsFirstRead ReadQueue.cpp:2468-2794: counts into maps sized by
+ * the bucket count, then the windows with c0 >= QTHRESH copied in that
+ * map's iteration order into maps of 20 buckets (outF / outR) */
+static void seed_first(const orc_ctx* cx, const uint8_t* seq, uint32_t L, hsmap* outF, hsmap* outR)
+{
+    const orc_index* ix = cx->ix;
+    const uint32_t nk = L - KMERLEN;
+    uint32_t bucketCount = 0;
+    for (uint32_t c = 0; c <= nk; ++c) {
+        const uint64_t key = hkey(ix, seq + c);
+        bucketCount += (uint32_t)(ix->tab[key + 1] - ix->tab[key]);
+    }
+    bucketCount /= 5;
+    hsmap f, r;
+    hs_init(&f, bucketCount);
+    hs_init(&r, bucketCount);
+    uint32_t cMask = 0, kSeq = 0;
+    for (uint32_t i = 0; i < KMERLEN; ++i) {
+        cMask = (cMask << 1) | (seq[i] == 'C');
+        kSeq = (kSeq << 2) + kseq_code(seq[i]);
+    }
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    for (uint32_t c = 0; c <= nk; ++c) {
+        if (c > 0) {
+            const uint8_t nx = seq[KMERLEN + c - 1];
+            kSeq = (kSeq << 2) + kseq_code(nx);
+            cMask = (cMask << 1) | (nx == 'C'); /* & KMERMASK = 0xffffffff */
+            if (filt_has(cx, kSeq)) continue;   /* not checked for k-mer 0 */
+        }
+        const uint64_t key = hkey(ix, seq + c);
+        uint64_t lastId = 0xffffffffffffffffULL;
+        int wasFwd = 0, wasStart = 0;
+        for (uint64_t i = ix->tab[key]; i < ix->tab[key + 1]; ++i) {
+            if (cMask & ix->seedbits & ix->ktmask[i]) continue; /* read C where the reference has T */
+            const uint32_t metaId = ix->kmeta[i] & 0x7fffffffu;
+            const int isFwd = ix->strand[i];
+            const int isStart = (ix->kmeta[i] >> 31) & 1;
+            if (metaId == lastId && isFwd == wasFwd && isStart == wasStart) continue;
+            lastId = metaId;
+            wasFwd = isFwd;
+            wasStart = isStart;
+            hsmap* m = isFwd ? &f : &r;
+            if (c == 0 || c - 1 < maxQPos) {
+                uint32_t* p = hs_get(m, metaId);
+                *p = p_inc0(*p);
+            } else {
+                uint32_t* p = hs_find(m, metaId);
+                if (p) *p = p_inc0(*p);
+            }
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        hsmap* src = pass ? &r : &f;
+        hsmap* dst = pass ? outR : outF;
+        hs_init(dst, 20);
+        const int n = hs_items(src, NULL, NULL, 0);
+        uint32_t* k = xmalloc((size_t)n * 4 + 4);
+        uint32_t* v = xmalloc((size_t)n * 4 + 4);
+        hs_items(src, k, v, n);
+        for (int i = 0; i < n; ++i)
+            if (P_C0(v[i]) >= QTHRESH) *hs_get(dst, k[i]) = P_C0(v[i]);
+        free(k);
+        free(v);
+    }
+    hs_free(&f);
+    hs_free(&r);
+}
+
+/* getSeedRefsSecondRead ReadQueue.cpp:2801-3164; returns hasCpG */
+static int seed_second(const orc_ctx* cx, const uint8_t* seq, uint32_t L, hsmap* F, hsmap* R)
+{
+    const orc_index* ix = cx->ix;
+    const uint32_t nk = L - KMERLEN;
+    const int cw = (int)((double)cx->maxpd / ix->winlen) + 1; /* contextWLen */
+    uint32_t cMask = 0, kSeq = 0;
+    for (uint32_t i = 0; i < KMERLEN; ++i) {
+        cMask = (cMask << 1) | (seq[i] == 'C');
+        kSeq = (kSeq << 2) + kseq_code(seq[i]);
+    }
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    for (uint32_t c = 0; c <= nk; ++c) {
+        if (c > 0) {
+            const uint8_t nx = seq[KMERLEN + c - 1];
+            kSeq = (kSeq << 2) + kseq_code(nx);
+            cMask = (cMask << 1) | (nx == 'C');
+            if (filt_has(cx, kSeq)) continue;
+        }
+        const uint64_t key = hkey(ix, seq + c);
+        uint64_t lastId = 0xffffffffffffffffULL;
+        int wasFwd = 0;
+        for (uint64_t i = ix->tab[key]; i < ix->tab[key + 1]; ++i) {
+            if (cMask & ix->seedbits & ix->ktmask[i]) continue;
+            const uint32_t metaId = ix->kmeta[i] & 0x7fffffffu;
+            const int isFwd = ix->strand[i];
+            if (metaId == lastId && isFwd == wasFwd) continue; /* isStart not compared here */
+            lastId = metaId;
+            wasFwd = isFwd;
+            hsmap* m = isFwd ? F : R;
+            if (c == 0 || c - 1 < maxQPos) {
+                /* credit the window when it or a neighbour within contextWLen
+                 * holds a first-read window (operator[] may insert it) */
+                for (int o = -cw; o <= cw; ++o)
+                    if (hs_find(m, metaId + (uint32_t)o)) {
+                        uint32_t* p = hs_get(m, metaId);
+                        *p = p_inc1(*p);
+                        break;
+                    }
+            } else {
+                uint32_t* p = hs_find(m, metaId);
+                if (p) *p = p_inc1(*p);
+            }
+        }
+    }
+    int has = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        hsmap* m = pass ? R : F;
+        const int n = hs_items(m, NULL, NULL, 0);
+        uint32_t* k = xmalloc((size_t)n * 4 + 4);
+        uint32_t* v = xmalloc((size_t)n * 4 + 4);
+        hs_items(m, k, v, n);
+        for (int i = 0; i < n; ++i) {
+            if (P_C1(v[i]) < QTHRESH) continue;
+            *hs_find(m, k[i]) |= P_F2;
+            for (int u = 1; u <= cw; ++u) {
+                uint32_t* p = hs_find(m, k[i] + (uint32_t)u);
+                if (p) *p |= P_F2;
+                p = hs_find(m, k[i] - (uint32_t)u);
+                if (p) *p |= P_F2;
+            }
+        }
+        free(k);
+        free(v);
+    }
+    for (int pass = 0; pass < 2 && !has; ++pass) {
+        hsmap* m = pass ? R : F;
+        const int n = hs_items(m, NULL, NULL, 0);
+        uint32_t* k = xmalloc((size_t)n * 4 + 4);
+        uint32_t* v = xmalloc((size_t)n * 4 + 4);
+        hs_items(m, k, v, n);
+        for (int i = 0; i < n && !has; ++i)
+            if ((v[i] & P_F2) && ix->win[k[i]].startInd != CPGDUMMY) has = 1;
+        free(k);
+        free(v);
+    }
+    return has;
+}
+
+/* std::sort of libstdc++ (GCC 11, bits/stl_algo.h / stl_heap.h): introsort
+ * with median-of-three pivots, heap sort past depth 2*floor(log2 n), then
+ * a final insertion sort; comparator "count greater" on c0 (cmpMetaFirst)
+ * or c1 (cmpMetaSecond), ReadQueue.h:340-351.  The order of equal counts
+ * follows the algorithm, so it is reproduced step by step. */
+typedef struct {
+    uint32_t k, v;
+} pe_item;
+
+static inline int pe_less(pe_item a, pe_item b, int sec)
+{
+    return sec ? P_C1(a.v) > P_C1(b.v) : P_C0(a.v) > P_C0(b.v);
+}
+static inline void pe_swap(pe_item* a, long i, long j)
+{
+    pe_item t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+static void pe_insertion(pe_item* a, long lo, long hi, int sec)
+{
+    if (lo == hi) return;
+    for (long i = lo + 1; i < hi; ++i) {
+        pe_item val = a[i];
+        if (pe_less(val, a[lo], sec)) {
+            memmove(&a[lo + 1], &a[lo], (size_t)(i - lo) * sizeof *a);
+            a[lo] = val;
+        } else {
+            long j = i;
+            while (pe_less(val, a[j - 1], sec)) {
+                a[j] = a[j - 1];
+                --j;
+            }
+            a[j] = val;
+        }
+    }
+}
+static void pe_unguarded_insertion(pe_item* a, long lo, long hi, int sec)
+{
+    for (long i = lo; i < hi; ++i) {
+        pe_item val = a[i];
+        long j = i;
+        while (pe_less(val, a[j - 1], sec)) {
+            a[j] = a[j - 1];
+            --j;
+        }
+        a[j] = val;
+    }
+}
+static void pe_push_heap(pe_item* a, long first, long hole, long top, pe_item val, int sec)
+{
+    long parent = (hole - 1) / 2;
+    while (hole > top && pe_less(a[first + parent], val, sec)) {
+        a[first + hole] = a[first + parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    a[first + hole] = val;
+}
+static void pe_adjust_heap(pe_item* a, long first, long hole, long len, pe_item val, int sec)
+{
+    const long top = hole;
+    long child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (pe_less(a[first + child], a[first + child - 1], sec)) child--;
+        a[first + hole] = a[first + child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        a[first + hole] = a[first + child - 1];
+        hole = child - 1;
+    }
+    pe_push_heap(a, first, hole, top, val, sec);
+}
+static void pe_make_heap(pe_item* a, long first, long last, int sec)
+{
+    const long len = last - first;
+    if (len < 2) return;
+    long parent = (len - 2) / 2;
+    for (;;) {
+        pe_adjust_heap(a, first, parent, len, a[first + parent], sec);
+        if (parent == 0) return;
+        parent--;
+    }
+}
+static void pe_pop_heap(pe_item* a, long first, long last, long result, int sec)
+{
+    pe_item v = a[result];
+    a[result] = a[first];
+    pe_adjust_heap(a, first, 0, last - first, v, sec);
+}
+static void pe_heap_sort_range(pe_item* a, long first, long last, int sec)
+{
+    /* __partial_sort(first, last, last): heap_select (no element past the
+     * heap) then sort_heap */
+    pe_make_heap(a, first, last, sec);
+    while (last - first > 1) {
+        --last;
+        pe_pop_heap(a, first, last, last, sec);
+    }
+}
+static void pe_median_to_first(pe_item* a, long result, long x, long y, long z, int sec)
+{
+    if (pe_less(a[x], a[y], sec)) {
+        if (pe_less(a[y], a[z], sec))
+            pe_swap(a, result, y);
+        else if (pe_less(a[x], a[z], sec))
+            pe_swap(a, result, z);
+        else
+            pe_swap(a, result, x);
+    } else if (pe_less(a[x], a[z], sec))
+        pe_swap(a, result, x);
+    else if (pe_less(a[y], a[z], sec))
+        pe_swap(a, result, z);
+    else
+        pe_swap(a, result, y);
+}
+static long pe_partition(pe_item* a, long first, long last, long pivot, int sec)
+{
+    for (;;) {
+        while (pe_less(a[first], a[pivot], sec)) ++first;
+        --last;
+        while (pe_less(a[pivot], a[last], sec)) --last;
+        if (!(first < last)) return first;
+        pe_swap(a, first, last);
+        ++first;
+    }
+}
+static void pe_introsort(pe_item* a, long first, long last, long depth, int sec)
+{
+    while (last - first > 16) {
+        if (depth == 0) {
+            pe_heap_sort_range(a, first, last, sec);
+            return;
+        }
+        --depth;
+        const long mid = first + (last - first) / 2;
+        pe_median_to_first(a, first, first + 1, mid, last - 1, sec);
+        const long cut = pe_partition(a, first + 1, last, first, sec);
+        pe_introsort(a, cut, last, depth, sec);
+        last = cut;
+    }
+}
+static void pe_sort(pe_item* a, long n, int sec)
+{
+    if (n == 0) return;
+    long lg = 0;
+    while ((1L << (lg + 1)) <= n) ++lg;
+    pe_introsort(a, 0, n, 2 * lg, sec);
+    if (n > 16) {
+        pe_insertion(a, 0, 16, sec);
+        pe_unguarded_insertion(a, 16, n, sec);
+    } else {
+        pe_insertion(a, 0, n, sec);
+    }
+}
+
+void orc_std_sort_counts(uint32_t* keys, uint32_t* vals, uint32_t n, int second)
+{
+    pe_item* a = xmalloc((size_t)n * sizeof *a + sizeof *a);
+    for (uint32_t i = 0; i < n; ++i) {
+        a[i].k = keys[i];
+        a[i].v = vals[i];
+    }
+    pe_sort(a, n, second);
+    for (uint32_t i = 0; i < n; ++i) {
+        keys[i] = a[i].k;
+        vals[i] = a[i].v;
+    }
+    free(a);
+}
+
+typedef struct {
+    uint64_t* m;
+    int n, cap;
+} matlist;
+
+static void ml_push(matlist* l, uint64_t m)
+{
+    if (l->n == l->cap) {
+        l->cap = l->cap ? 2 * l->cap : 16;
+        l->m = realloc(l->m, (size_t)l->cap * 8);
+    }
+    l->m[l->n++] = m;
+}
+
+/* matchFwdFirst / matchRevFirst / matchFwdSecond / matchRevSecond
+ * ReadQueue.cpp:4538-4776: one candidate window; returns 0 to stop the
+ * walk.  `second` selects the count (c1) and flag (f3) of the second read;
+ * the first read also marks f3 on the window and its neighbours in `m`. */
+static int pe_match_window(const orc_ctx* cx, orc_sa* sa, pe_item meta, int isFwd, int second, hsmap* m,
+                           uint8_t* prevChr, uint64_t* prevOff, matlist* out, int32_t* bmCount, matchvec* v)
+{
+    const orc_index* ix = cx->ix;
+    const int cw = (int)((double)cx->maxpd / ix->winlen) + 1;
+    const int32_t cnt = (int32_t)(second ? P_C1(meta.v) : P_C0(meta.v));
+    const int32_t lim = *bmCount - KMERDIST > (int32_t)QTHRESH ? *bmCount - KMERDIST : (int32_t)QTHRESH;
+    if (cnt < lim) return 0;
+    if (!(meta.v & (second ? P_F3 : P_F2))) return 1;
+    const struct orc_win* w = &ix->win[meta.k];
+    const uint8_t* cs = ix->cseq[w->chrom];
+    const uint64_t clen = ix->clen[w->chrom];
+    const uint64_t span = (uint64_t)ix->winlen + MISCOUNT + ADDMIS - 1;
+    v->n = 0;
+    if (isFwd) {
+        uint64_t end = (uint64_t)w->startPos + span;
+        if (end > clen) end = clen;
+        sa_query_fwd(sa, cs + w->startPos, end - w->startPos, v);
+    } else {
+        /* matchRevFirst: endIt = begin + startPos - 1, or begin when startPos
+         * is 0 (:4612-4616); matchRevSecond has no such guard (:4736), where
+         * startPos 0 would put endIt one before the chromosome: the scan then
+         * also covers position 0 */
+        uint64_t s0 = (uint64_t)w->startPos + span;
+        if (s0 >= clen) s0 = clen - 1;
+        if (second && w->startPos == 0) {
+            sa_query_rev(sa, cs, s0 + 1, v);
+        } else {
+            const uint64_t e0 = w->startPos > 0 ? (uint64_t)w->startPos - 1 : 0;
+            sa_query_rev(sa, cs + e0 + 1, s0 > e0 ? s0 - e0 : 0, v);
+        }
+    }
+    int i = 0;
+    if (v->n > 0 && v->pos[0] + w->startPos == *prevOff && w->chrom == *prevChr) ++i;
+    for (; i < v->n; ++i) ml_push(out, mk_match((uint16_t)v->pos[i], v->err[i], isFwd ? 1 : 0, 0, meta.k));
+    if (v->n > 0) {
+        if (cnt > *bmCount) *bmCount = cnt;
+        if (!second) {
+            *hs_find(m, meta.k) |= P_F3;
+            for (int u = 1; u <= cw; ++u) {
+                uint32_t* p = hs_find(m, meta.k + (uint32_t)u);
+                if (p) *p |= P_F3;
+                p = hs_find(m, meta.k - (uint32_t)u);
+                if (p) *p |= P_F3;
+            }
+        }
+        *prevChr = w->chrom;
+        *prevOff = w->startPos + v->pos[v->n - 1];
+    } else {
+        *prevChr = 0;
+        *prevOff = 0xffffffffffffffffULL;
+    }
+    return 1;
+}
+
+/* saQuerySeedSetRefFirst :2167-2243 / saQuerySeedSetRefSecond :2249-2323 */
+static void pe_query(const orc_ctx* cx, orc_sa* sa, hsmap* F, hsmap* R, int second, matlist* out)
+{
+    pe_item* vec[2];
+    int nv[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        hsmap* m = pass ? R : F;
+        const int n = hs_items(m, NULL, NULL, 0);
+        uint32_t* k = xmalloc((size_t)n * 4 + 4);
+        uint32_t* v = xmalloc((size_t)n * 4 + 4);
+        hs_items(m, k, v, n);
+        vec[pass] = xmalloc((size_t)n * sizeof(pe_item) + sizeof(pe_item));
+        for (int i = 0; i < n; ++i) {
+            vec[pass][i].k = k[i];
+            vec[pass][i].v = v[i];
+        }
+        free(k);
+        free(v);
+        nv[pass] = n;
+        pe_sort(vec[pass], n, second);
+    }
+    matchvec mv = {0};
+    int32_t bmCount = 0;
+    uint8_t prevChr = 0;
+    uint64_t prevOff = 0xffffffffffffffffULL;
+    /* the strand whose first entry has the larger c0 goes first (ties: rev);
+     * both walks compare c0 even for the second read */
+    int order[2] = {0, 1}, nord = 0;
+    if (nv[0] > 0 && nv[1] > 0) {
+        if (P_C0(vec[0][0].v) > P_C0(vec[1][0].v)) {
+            order[0] = 0;
+            order[1] = 1;
+        } else {
+            order[0] = 1;
+            order[1] = 0;
+        }
+        nord = 2;
+    } else if (nv[0] > 0) {
+        order[0] = 0;
+        nord = 1;
+    } else if (nv[1] > 0) {
+        order[0] = 1;
+        nord = 1;
+    }
+    for (int o = 0; o < nord; ++o) {
+        const int pass = order[o];
+        if (o == 1) {
+            prevChr = 0;
+            prevOff = 0xffffffffffffffffULL;
+        }
+        for (int i = 0; i < nv[pass]; ++i)
+            if (!pe_match_window(cx, sa, vec[pass][i], pass == 0, second, pass ? R : F, &prevChr, &prevOff, out,
+                                 &bmCount, &mv))
+                break;
+    }
+    free(mv.pos);
+    free(mv.err);
+    free(vec[0]);
+    free(vec[1]);
+}
+
+/* extractPairedMatch ReadQueue.cpp:3255-3275 */
+static int pe_extract(const orc_ctx* cx, uint64_t m1, uint64_t m2)
+{
+    const orc_index* ix = cx->ix;
+    const struct orc_win* w1 = &ix->win[(uint32_t)(m1 >> 32)];
+    const struct orc_win* w2 = &ix->win[(uint32_t)(m2 >> 32)];
+    if (w1->chrom != w2->chrom) return -1;
+    const uint32_t p1 = w1->startPos + (uint32_t)(m1 & 0xffff);
+    const uint32_t p2 = w2->startPos + (uint32_t)(m2 & 0xffff);
+    const uint32_t d = p2 > p1 ? p2 - p1 : p1 - p2;
+    if (d <= cx->maxpd + ix->readlen) return (int)((m1 >> 16) & 0xff) + (int)((m2 >> 16) & 0xff);
+    return -1;
+}
+
+/* the reverse complement of a mate; returns 0 when the mate holds an N or an
+ * unknown letter (the paired path rejects both, :1016-1088) */
+static int pe_revcomp(const uint8_t* s, uint32_t L, uint8_t* out)
+{
+    int ok = 1;
+    for (uint32_t p = 0, q = L - 1; p < L; ++p, --q) {
+        switch (s[p]) {
+        case 'A': out[q] = 'T'; break;
+        case 'C': out[q] = 'G'; break;
+        case 'G': out[q] = 'C'; break;
+        case 'T': out[q] = 'A'; break;
+        default: ok = 0; break;
+        }
+    }
+    return ok;
+}
+
+/* one pair: counters[0..4] += succ, nonunique, unsucc, succPaired, tooShort */
+static void match_pair(orc_ctx* cx, const uint8_t* in1, uint32_t L1, const uint8_t* in2, uint32_t L2,
+                       int getStranded, uint64_t cnt[5], uint8_t* v1, uint64_t* mat1, uint8_t* v2,
+                       uint64_t* mat2)
+{
+    const orc_index* ix = cx->ix;
+    const uint32_t minlen = (uint32_t)ceilf((float)ix->readlen * 0.75f);
+    int inv1 = L1 < minlen, inv2 = L2 < minlen;
+    if (inv1 || inv2) cnt[4] += 1;
+    *v1 = *v2 = 1;
+    *mat1 = *mat2 = 0;
+    uint8_t* s1 = calloc(L1 + 1, 1);
+    uint8_t* s2 = calloc(L2 + 1, 1);
+    uint8_t* rs1 = calloc(L1 + 1, 1);
+    uint8_t* rs2 = calloc(L2 + 1, 1);
+    memcpy(s1, in1, L1);
+    memcpy(s2, in2, L2);
+    if (!pe_revcomp(s1, L1, rs1)) inv1 = 1;
+    if (!pe_revcomp(s2, L2, rs2)) inv2 = 1;
+    if (inv1 || inv2) {
+        cnt[4] += 2;
+        *v1 = !inv1;
+        *v2 = !inv2;
+        goto out;
+    }
+    /* mates shorter than a k-mer never reach here: 0.75 READLEN > 32 */
+    {
+        matlist m1F = {0}, m2R = {0}, m1R = {0}, m2F = {0};
+        orc_sa* sa = xmalloc(sizeof(orc_sa));
+        hsmap F, R;
+        const int c1 = cx->both || getStranded || cx->match_r1_fwd;
+        const int c2 = cx->both || getStranded || !cx->match_r1_fwd;
+        if (c1) {
+            seed_first(cx, s1, L1, &F, &R);
+            if (F.nelem || R.nelem) {
+                if (seed_second(cx, rs2, L2, &F, &R)) {
+                    sa_load(sa, s1, L1);
+                    pe_query(cx, sa, &F, &R, 0, &m1F);
+                    if (m1F.n) {
+                        sa_load(sa, rs2, L2);
+                        pe_query(cx, sa, &F, &R, 1, &m2R);
+                    }
+                }
+            }
+            hs_free(&F);
+            hs_free(&R);
+        }
+        if (c2) {
+            seed_first(cx, rs1, L1, &F, &R);
+            if (F.nelem || R.nelem) {
+                if (seed_second(cx, s2, L2, &F, &R)) {
+                    sa_load(sa, rs1, L1);
+                    pe_query(cx, sa, &F, &R, 0, &m1R);
+                    if (m1R.n) {
+                        sa_load(sa, s2, L2);
+                        pe_query(cx, sa, &F, &R, 1, &m2F);
+                    }
+                }
+            }
+            hs_free(&F);
+            hs_free(&R);
+        }
+        free(sa);
+        if ((m1F.n == 0 && m1R.n == 0) || (m2F.n == 0 && m2R.n == 0)) {
+            *v1 = *v2 = 0;
+            cnt[2] += 2;
+        } else {
+            const int worst = 2 * (int)NERR + 1;
+            int best = worst, nonUnique = 0, orig = 1;
+            uint64_t b1 = 0, b2 = 0;
+            for (int pass = 0; pass < 2; ++pass) {
+                if (pass == 0 ? !c1 : !c2) continue;
+                const matlist* A = pass == 0 ? &m1F : &m1R;
+                const matlist* B = pass == 0 ? &m2R : &m2F;
+                for (int a = 0; a < A->n; ++a)
+                    for (int b = 0; b < B->n; ++b) {
+                        const int e = pe_extract(cx, A->m[a], B->m[b]);
+                        if (e < 0) continue;
+                        if (e == best) {
+                            nonUnique = 1;
+                        } else if (e < best) {
+                            best = e;
+                            b1 = A->m[a];
+                            b2 = B->m[b];
+                            if (pass == 1) orig = 0;
+                            nonUnique = 0;
+                        }
+                    }
+            }
+            if (best == worst) {
+                *v1 = *v2 = 0;
+                cnt[2] += 2;
+            } else if (nonUnique) {
+                *v1 = *v2 = 0;
+                cnt[1] += 2;
+            } else {
+                *mat1 = b1;
+                *mat2 = b2;
+                if (orig) {
+                    if (getStranded) __atomic_fetch_add(&cx->r1fwd, 1, __ATOMIC_RELAXED);
+                    compute_meth(cx, b1, s1, L1);
+                    compute_meth(cx, b2, rs2, L2);
+                } else {
+                    if (getStranded) __atomic_fetch_add(&cx->r1rev, 1, __ATOMIC_RELAXED);
+                    compute_meth(cx, b1, rs1, L1);
+                    compute_meth(cx, b2, s2, L2);
+                }
+                const int d1 = ix->win[(uint32_t)(b1 >> 32)].startInd == CPGDUMMY;
+                const int d2 = ix->win[(uint32_t)(b2 >> 32)].startInd == CPGDUMMY;
+                if (d1 && d2) {
+                    cnt[4] += 2;
+                } else if (d1 || d2) {
+                    cnt[4] += 1;
+                    cnt[0] += 1;
+                } else {
+                    cnt[3] += 1;
+                    cnt[0] += 2;
+                }
+            }
+        }
+        free(m1F.m);
+        free(m2R.m);
+        free(m1R.m);
+        free(m2F.m);
+    }
+out:
+    free(s1);
+    free(s2);
+    free(rs1);
+    free(rs2);
+}
+
+int orc_match_pairs(orc_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, const uint32_t* lens1,
+                    const uint8_t* seqs2, const uint64_t* offs2, const uint32_t* lens2, uint32_t n,
+                    int get_stranded, uint64_t counters[5], uint8_t* valid1, uint64_t* mat1,
+                    uint8_t* valid2, uint64_t* mat2, int nthreads)
+{
+    if (c->ix->nfilt && !c->filt_sorted) {
+        c->filt_sorted = xmalloc(c->ix->nfilt * 8);
+        memcpy(c->filt_sorted, c->ix->filt, c->ix->nfilt * 8);
+        qsort(c->filt_sorted, c->ix->nfilt, 8, cmp_u64);
+    }
+    uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 32) reduction(+ : a0, a1, a2, a3, a4)
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t cnt[5] = {0, 0, 0, 0, 0};
+        match_pair(c, seqs1 + offs1[i], lens1[i], seqs2 + offs2[i], lens2[i], get_stranded, cnt, &valid1[i],
+                   &mat1[i], &valid2[i], &mat2[i]);
+        a0 += cnt[0];
+        a1 += cnt[1];
+        a2 += cnt[2];
+        a3 += cnt[3];
+        a4 += cnt[4];
+    }
+    counters[0] += a0;
+    counters[1] += a1;
+    counters[2] += a2;
+    counters[3] += a3;
+    counters[4] += a4;
     return 0;
 }

@@ -1,4 +1,4 @@
-/* CPU restatement of FAME's single-end read-matching path.
+/* CPU restatement of FAME's read-matching path (single-end and paired-end).
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
@@ -85,6 +85,21 @@ int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count,
 int orc_seed_maps(orc_ctx* c, const uint8_t* seq, uint32_t len, uint32_t* fwd_keys,
                   uint16_t* fwd_cnt, int* nfwd, uint32_t* rev_keys, uint16_t* rev_cnt, int* nrev,
                   int cap, uint32_t* bucket_count);
+
+/* ---- paired-end path ---- */
+
+/* MINPDIST / MAXPDIST (CONST.h:41-42; compile-time in FAME), default 50 / 550 */
+void orc_set_pdist(orc_ctx* c, uint32_t minpd, uint32_t maxpd);
+/* ReadQueue::matchPairedReads ReadQueue.cpp:935-1655 over n pairs.
+ * counters[0..4] += succ, nonunique, unsucc, succPaired, tooShort; valid =
+ * !Read::isInvalid of each mate, mat = Read::mat when the pair matched */
+int orc_match_pairs(orc_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, const uint32_t* lens1,
+                    const uint8_t* seqs2, const uint64_t* offs2, const uint32_t* lens2, uint32_t n,
+                    int get_stranded, uint64_t counters[5], uint8_t* valid1, uint64_t* mat1,
+                    uint8_t* valid2, uint64_t* mat2, int nthreads);
+/* std::sort (libstdc++ 11) of (key, packed value) pairs by count greater:
+ * c0 (cmpMetaFirst) or c1 (cmpMetaSecond) */
+void orc_std_sort_counts(uint32_t* keys, uint32_t* vals, uint32_t n, int second);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,10 @@ def lib():
         L.orc_dp_align.argtypes = [_u8p, C.c_uint32, _u8p, C.c_uint64, C.c_int64, C.c_int, _u8p,
                                    C.c_int, C.POINTER(C.c_int)]
         L.orc_hopscotch_order.argtypes = [_u32p, C.c_uint32, C.c_uint64, _u32p]
+        L.orc_set_pdist.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.orc_match_pairs.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, _u8p, _u64p, _u32p, C.c_uint32,
+                                      C.c_int, _u64p, _u8p, _u64p, _u8p, _u64p, C.c_int]
+        L.orc_std_sort_counts.argtypes = [_u32p, _u32p, C.c_uint32, C.c_int]
         L.orc_seed_maps.argtypes = [C.c_void_p, _u8p, C.c_uint32, _u32p, _u16p, C.POINTER(C.c_int),
                                     _u32p, _u16p, C.POINTER(C.c_int), C.c_int, _u32p]
         _lib = L
@@ -111,6 +115,30 @@ class Matcher:
                               _ptr(mat, _u64p), int(nthreads))
         return valid, mat
 
+    def set_pdist(self, minpd, maxpd):
+        lib().orc_set_pdist(self.h, int(minpd), int(maxpd))
+
+    def match_pairs(self, r1, r2, get_stranded, nthreads=0):
+        """matchPairedReads over pairs given as two (seqs, offs, lens) triples;
+        counters5 accumulates succ, nonunique, unsucc, succPaired, tooShort."""
+        n = int(len(r1[2]))
+        v1 = np.zeros(n, np.uint8)
+        v2 = np.zeros(n, np.uint8)
+        m1 = np.zeros(n, np.uint64)
+        m2 = np.zeros(n, np.uint64)
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in zip(r1, (np.uint8, np.uint64, np.uint32))]
+        b = [np.ascontiguousarray(x, dtype=t) for x, t in zip(r2, (np.uint8, np.uint64, np.uint32))]
+        for arr in (a, b):
+            if arr[0].size == 0:
+                arr[0] = np.zeros(1, np.uint8)
+        if not hasattr(self, "counters5"):
+            self.counters5 = np.zeros(5, dtype=np.uint64)
+        lib().orc_match_pairs(self.h, _ptr(a[0], _u8p), _ptr(a[1], _u64p), _ptr(a[2], _u32p),
+                              _ptr(b[0], _u8p), _ptr(b[1], _u64p), _ptr(b[2], _u32p), n, int(get_stranded),
+                              _ptr(self.counters5, _u64p), _ptr(v1, _u8p), _ptr(m1, _u64p), _ptr(v2, _u8p),
+                              _ptr(m2, _u64p), int(nthreads))
+        return v1, m1, v2, m2
+
     def decide_strand(self):
         f = C.c_uint64()
         r = C.c_uint64()
@@ -166,6 +194,43 @@ def match_stream(index, seqs, offs, lens, chunk=300000, both_strands=False, nthr
         valid[s:e], mat[s:e] = m.match(seqs, offs[s:e], lens[s:e], False, nthreads)
         s = e
     return m, valid, mat
+
+
+def match_pairs_stream(index, r1, r2, chunk=300000, both_strands=False, minpd=50, maxpd=550, nthreads=0):
+    """queryRoutinePaired main.cpp:400-441 over in-memory pairs; r1 / r2 are
+    (seqs, offs, lens) triples of the first and second mates."""
+    m = Matcher(index, both_strands)
+    m.set_pdist(minpd, maxpd)
+    n = len(r1[2])
+    v1 = np.zeros(n, np.uint8)
+    v2 = np.zeros(n, np.uint8)
+    m1 = np.zeros(n, np.uint64)
+    m2 = np.zeros(n, np.uint64)
+
+    def run(s, e, stranded):
+        sub1 = (r1[0], r1[1][s:e], r1[2][s:e])
+        sub2 = (r2[0], r2[1][s:e], r2[2][s:e])
+        v1[s:e], m1[s:e], v2[s:e], m2[s:e] = m.match_pairs(sub1, sub2, stranded, nthreads)
+
+    s = 0
+    if not both_strands:
+        e = min(n, chunk)
+        run(s, e, True)
+        m.decide_strand()
+        s = e
+    while s < n:
+        e = min(n, s + chunk)
+        run(s, e, False)
+        s = e
+    return m, v1, m1, v2, m2
+
+
+def std_sort_counts(keys, vals, second):
+    """std::sort (libstdc++) of (key, packed count) pairs, count greater."""
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().orc_std_sort_counts(_ptr(k, _u32p), _ptr(v, _u32p), k.size, int(second))
+    return k, v
 
 
 def ntps64(kmer, seedbits):

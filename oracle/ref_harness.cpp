@@ -14,6 +14,10 @@
 //         <out_prefix>_cpg.tsv      via ReadQueue::printMethylationLevels
 //         <out_prefix>_reads.bin    "FAMERES1", u64 n, u8 valid[n], u64 mat[n]
 //         <out_prefix>_stats.json   counters, strand tallies, per-chunk timings
+//   match_pe <index> <r1.fq> <r2.fq> <out_prefix> [--unord_reads] [--max_reads N]
+//       mirrors queryRoutinePaired main.cpp:400-441; writes the same TSV and
+//       stats, and <out_prefix>_pairs.bin: "FAMEPE01", u64 n, u8 valid1[n],
+//       u8 valid2[n], u64 mat1[n], u64 mat2[n] (mat as left in Read::mat)
 //
 // ReadQueue keeps readBuffer / strand tallies private; they are reached by
 // compiling this translation unit only with `private` defined as `public`.
@@ -172,6 +176,67 @@ static int do_match(int argc, char** argv)
     return 0;
 }
 
+static int do_match_pe(int argc, char** argv)
+{
+    if (argc < 6) return 2;
+    bool both = false;
+    uint64_t budget = 0;
+    for (int i = 6; i < argc; ++i) {
+        if (!strcmp(argv[i], "--unord_reads")) both = true;
+        if (!strcmp(argv[i], "--max_reads") && i + 1 < argc) budget = std::stoull(argv[++i]);
+    }
+    auto t0 = clk::now();
+    RefGenome ref{std::string(argv[2])};
+    double load_s = secs(t0, clk::now());
+    ReadQueue q(argv[3], argv[4], ref, false, both);
+    std::vector<uint8_t> v1, v2;
+    std::vector<uint64_t> m1, m2;
+    std::vector<ChunkTime> chunks;
+    uint64_t succ = 0, nonu = 0, unsucc = 0, succP = 0, tooShort = 0, seen = 0;
+    unsigned n = 0;
+    auto run = [&](bool stranded) {
+        // matchPairedReads writes mat only for matched pairs (ReadQueue.cpp:1593-1594)
+        // and parseChunk leaves it unset, so clear it first: unmatched pairs then
+        // read 0, as in the oracle and the HIP path.
+        for (unsigned i = 0; i < n; ++i) q.readBuffer[i].mat = q.readBuffer2[i].mat = 0;
+        auto a = clk::now();
+        q.matchPairedReads(n, succ, nonu, unsucc, succP, tooShort, stranded);
+        chunks.push_back({n, secs(a, clk::now()), stranded});
+        for (unsigned i = 0; i < n; ++i) {
+            v1.push_back(q.readBuffer[i].isInvalid ? 0 : 1);
+            v2.push_back(q.readBuffer2[i].isInvalid ? 0 : 1);
+            m1.push_back(q.readBuffer[i].mat);
+            m2.push_back(q.readBuffer2[i].mat);
+        }
+    };
+    // queryRoutinePaired, main.cpp:413-431
+    if (!both) {
+        parse_bounded(q, n, seen, budget);
+        run(true);
+        q.decideStrand();
+    }
+    while (parse_bounded(q, n, seen, budget)) run(false);
+    run(false);
+    std::string out = argv[5];
+    q.printMethylationLevels(out);
+    {
+        std::ofstream of(out + "_pairs.bin", std::ios::binary | std::ios::trunc);
+        const char magic[8] = {'F', 'A', 'M', 'E', 'P', 'E', '0', '1'};
+        of.write(magic, 8);
+        uint64_t nn = v1.size();
+        of.write(reinterpret_cast<const char*>(&nn), 8);
+        of.write(reinterpret_cast<const char*>(v1.data()), nn);
+        of.write(reinterpret_cast<const char*>(v2.data()), nn);
+        of.write(reinterpret_cast<const char*>(m1.data()), 8 * nn);
+        of.write(reinterpret_cast<const char*>(m2.data()), 8 * nn);
+    }
+    write_stats(out + "_stats.json", "pe", v1.size(), succ, nonu, unsucc, succP, tooShort, q, load_s,
+                chunks);
+    std::cout << "HARNESS pairs=" << v1.size() << " succ=" << succ << " nonunique=" << nonu
+              << " unsucc=" << unsucc << " paired=" << succP << " tooshort=" << tooShort << "\n";
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -182,6 +247,7 @@ int main(int argc, char** argv)
     std::string mode = argv[1];
     if (mode == "build") return do_build(argc, argv);
     if (mode == "match") return do_match(argc, argv);
+    if (mode == "match_pe") return do_match_pe(argc, argv);
     std::cerr << "unknown mode\n";
     return 2;
 }

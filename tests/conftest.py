@@ -12,7 +12,18 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-CASES = sorted(d for d in os.listdir(GOLDEN) if os.path.isfile(os.path.join(GOLDEN, d, "case.json")))
+
+def _cases(mode):
+    out = []
+    for d in sorted(os.listdir(GOLDEN)):
+        f = os.path.join(GOLDEN, d, "case.json")
+        if os.path.isfile(f) and json.load(open(f)).get("mode", "se") == mode:
+            out.append(d)
+    return out
+
+
+CASES = _cases("se")
+PAIR_CASES = _cases("pe")
 
 
 def pytest_configure(config):
@@ -43,3 +54,41 @@ class Case:
 @pytest.fixture(params=CASES)
 def golden_case(request, tmp_path):
     return Case(request.param, str(tmp_path))
+
+
+def _unpack_index(meta, tmpdir):
+    idir = os.path.join(GOLDEN, meta["index_dir"])
+    path = os.path.join(tmpdir, "index.bin")
+    for f in ("index.bin", "index.bin_strands"):
+        with gzip.open(os.path.join(idir, f + ".gz"), "rb") as a, open(os.path.join(tmpdir, f), "wb") as b:
+            shutil.copyfileobj(a, b)
+    return path
+
+
+class PairCase:
+    """A paired-end fixture: two mates per pair, five counters."""
+
+    def __init__(self, name, tmpdir):
+        d = os.path.join(GOLDEN, name)
+        self.name = name
+        self.meta = json.load(open(os.path.join(d, "case.json")))
+        self.index = _unpack_index(self.meta, tmpdir)
+        r = np.load(os.path.join(d, "reads.npz"))
+        self.r1 = (r["seqs1"], r["offs1"], r["lens1"])
+        self.r2 = (r["seqs2"], r["offs2"], r["lens2"])
+        g = np.load(os.path.join(d, "golden.npz"))
+        self.valid1, self.valid2 = g["valid1"], g["valid2"]
+        self.mat1, self.mat2, self.meth = g["mat1"], g["mat2"], g["meth"]
+        with gzip.open(os.path.join(d, "cpg.tsv.gz"), "rb") as f:
+            self.tsv = f.read()
+        self.chunk = self.meta["chunk"]
+        self.unord = self.meta["unord"]
+        self.minpd, self.maxpd = self.meta["minpd"], self.meta["maxpd"]
+        self.ref = self.meta["reference"]
+        self.counters = [self.ref[k] for k in ("succ", "nonunique", "unsucc", "succ_paired",
+                                               "too_short")]
+
+
+@pytest.fixture(params=PAIR_CASES)
+def golden_pair(request, tmp_path):
+    return PairCase(request.param, str(tmp_path))

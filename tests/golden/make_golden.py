@@ -3,13 +3,17 @@ CPU path (built from its sources into oracle/_ref by oracle/Makefile).
 
     python tests/golden/make_golden.py [case ...]
 
-Each case directory holds
+Each single-end case directory holds
     index.bin.gz, index.bin_strands.gz  the reference-built --load_index files
     reads.npz                           flat read bytes + offsets + lengths
     golden.npz                          per-read valid / MATCH from the reference,
                                         CpG counts parsed from its TSV
     cpg.tsv.gz                          the reference's <out>_cpg.tsv, verbatim
     case.json                           configuration, counters, strand tallies
+
+Paired-end cases (pe_*) hold reads.npz / golden.npz with per-mate arrays
+(seqs1/offs1/lens1, seqs2/..., valid1/valid2/mat1/mat2) and the five paired
+counters in case.json.
 
 Inputs are synthetic (fame_amd.synth) with fixed seeds, plus hand-placed
 special reads (N, short, unknown characters, random, chromosome edges,
@@ -51,12 +55,29 @@ CASES = {
 }
 
 
-def ref_binary(cfg):
+# Paired-end cases.  Each shares the genome and index of a single-end case
+# (MINPDIST / MAXPDIST do not enter the index); `gap` is the distance between
+# the mates' inner ends, `flip` swaps the mates (read 1 from the G->A strand).
+PAIRED = {
+    "pe_basic": dict(share="se_basic", n=3000, seed=21, chunk=300000, minpd=50, maxpd=550,
+                     gap=(20, 400), unord=False, flip=False),
+    "pe_repeats": dict(share="se_repeats", n=4000, seed=22, chunk=300000, minpd=50, maxpd=550,
+                       gap=(20, 450), unord=False, flip=False),
+    "pe_unord": dict(share="se_repeats", n=2000, seed=23, chunk=300000, minpd=50, maxpd=550,
+                     gap=(20, 400), unord=True, flip=False),
+    "pe_rl150": dict(share="se_rl150", n=3000, seed=24, chunk=300000, minpd=20, maxpd=400,
+                     gap=(20, 400), unord=False, flip=False, len_range=(100, 150)),
+    "pe_ga_chunked": dict(share="se_ga_chunked", n=2500, seed=25, chunk=700, minpd=50, maxpd=550,
+                          gap=(20, 400), unord=False, flip=True),
+}
+
+
+def ref_binary(cfg, minpd=50, maxpd=550):
     h = cfg["htab"]
-    name = f"h{h}_r{cfg['readlen']}_c8_n{len(cfg['chroms'])}_p50-550_k{cfg['chunk']}"
+    name = f"h{h}_r{cfg['readlen']}_c8_n{len(cfg['chroms'])}_p{minpd}-{maxpd}_k{cfg['chunk']}"
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"HTAB={h}",
                     f"READLEN={cfg['readlen']}", "CORENUM=8", f"CHROMNUM={len(cfg['chroms'])}",
-                    f"CHUNK={cfg['chunk']}"], check=True)
+                    f"CHUNK={cfg['chunk']}", f"MINPD={minpd}", f"MAXPD={maxpd}"], check=True)
     return os.path.join(ROOT, "oracle", "_ref", f"fame_ref_{name}")
 
 
@@ -196,7 +217,160 @@ def run_case(name):
     print(name, meta["reference"])
 
 
+def _ct(s):
+    s = s.copy()
+    s[s == ord("C")] = ord("T")
+    return s
+
+
+def make_pair_reads(pc, gcfg, chroms):
+    """Mates from synth.make_pairs plus hand-placed special pairs."""
+    rng = np.random.default_rng(pc["seed"] + 2000)
+    RL = gcfg["readlen"]
+    a, b = synth.make_pairs(chroms, pc["n"], RL, gap=pc["gap"], seed=pc["seed"] + 3,
+                            len_range=pc.get("len_range"))
+    s1 = [bytes(a.seqs[i, :a.lens[i]]) for i in range(len(a))]
+    s2 = [bytes(b.seqs[i, :b.lens[i]]) for i in range(len(b))]
+    n = len(s1)
+    pick = iter(rng.permutation(n).tolist())
+    k = max(4, n // 100)
+    short = int(np.ceil(RL * 0.75))
+
+    def take(m):
+        return [next(pick) for _ in range(m)]
+
+    def put_n(s):
+        s = bytearray(s)
+        s[rng.integers(0, len(s))] = ord("N")
+        return bytes(s)
+
+    def put_unknown(s):
+        s = bytearray(s)
+        s[rng.integers(0, len(s))] = ord(rng.choice([b"n", b"X", b"a"]))
+        return bytes(s)
+
+    def rand_read():
+        return bytes(synth._BASES[rng.integers(0, 4, size=RL)])
+
+    for i in take(k):                      # an N in mate 1
+        s1[i] = put_n(s1[i])
+    for i in take(k):                      # an N in mate 2
+        s2[i] = put_n(s2[i])
+    for i in take(k // 2):                 # N in both
+        s1[i], s2[i] = put_n(s1[i]), put_n(s2[i])
+    for i in take(k):                      # mate 1 too short (and just long enough)
+        s1[i] = s1[i][:int(rng.integers(30, short))]
+    for i in take(k // 2):
+        s1[i] = s1[i][:short]
+    for i in take(k // 2):                 # mate 2 too short
+        s2[i] = s2[i][:int(rng.integers(30, short))]
+    for i in take(k // 2):                 # both too short
+        s1[i], s2[i] = s1[i][:40], s2[i][:int(rng.integers(30, short))]
+    for i in take(k // 2):                 # too short plus an N
+        s1[i], s2[i] = s1[i][:50], put_n(s2[i])
+    for i in take(k // 2):                 # unknown characters
+        s1[i] = put_unknown(s1[i])
+    for i in take(k // 2):
+        s2[i] = put_unknown(s2[i])
+    for i in take(k):                      # mate 2 without origin
+        s2[i] = rand_read()
+    for i in take(k // 2):                 # mate 1 without origin
+        s1[i] = rand_read()
+    for i in take(k // 2):                 # neither has an origin
+        s1[i], s2[i] = rand_read(), rand_read()
+    for j, i in enumerate(take(k)):        # mates far apart or on different chromosomes
+        c = chroms[(j + 1) % len(chroms)]
+        p = int(rng.integers(20, c.size - RL - 20))
+        s2[i] = bytes(synth.revcomp(_ct(c[p:p + RL])))
+    for j, i in enumerate(take(k)):        # fragments at chromosome ends (>= 12 bp in)
+        c = chroms[j % len(chroms)]
+        g = int(rng.integers(pc["gap"][0], pc["gap"][1] + 1))
+        F = 2 * RL + g
+        p = int(rng.integers(12, 300)) if j % 2 else c.size - F - int(rng.integers(12, 300))
+        fr = _ct(c[p:p + F])
+        s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+    for i in take(k // 2):                 # mates longer than READLEN
+        c = chroms[0]
+        L1, L2 = RL + int(rng.integers(1, 40)), RL + int(rng.integers(1, 40))
+        g = int(rng.integers(pc["gap"][0], pc["gap"][1] + 1))
+        F = L1 + g + L2
+        p = int(rng.integers(100, c.size - F - 100))
+        fr = _ct(c[p:p + F])
+        s1[i], s2[i] = bytes(fr[:L1]), bytes(synth.revcomp(fr[F - L2:]))
+    for i in take(k // 2):                 # mate distance just inside / outside the limit
+        c = chroms[1 % len(chroms)]
+        g = pc["maxpd"] - int(rng.integers(-3, 4))
+        F = 2 * RL + g
+        p = int(rng.integers(100, c.size - F - 100))
+        fr = _ct(c[p:p + F])
+        s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+    if pc["flip"]:
+        s1, s2 = s2, s1
+    return s1, s2
+
+
+def _flat(seqs):
+    lens = np.array([len(s) for s in seqs], np.uint32)
+    offs = np.zeros(len(seqs), np.uint64)
+    offs[1:] = np.cumsum(lens.astype(np.uint64))[:-1]
+    return np.frombuffer(b"".join(seqs), np.uint8), offs, lens
+
+
+def gxx_version():
+    r = subprocess.run(["g++", "--version"], capture_output=True, text=True, check=True)
+    return r.stdout.splitlines()[0]
+
+
+def run_pair_case(name):
+    pc = PAIRED[name]
+    share = pc["share"]
+    gcfg = dict(CASES[share])
+    gcfg["chunk"] = pc["chunk"]
+    out = os.path.join(HERE, name)
+    os.makedirs(out, exist_ok=True)
+    binp = ref_binary(gcfg, pc["minpd"], pc["maxpd"])
+    with tempfile.TemporaryDirectory() as td:
+        sd = os.path.join(HERE, CASES[share].get("share") or share)
+        for f in ("index.bin", "index.bin_strands"):
+            with gzip.open(os.path.join(sd, f + ".gz"), "rb") as a, open(os.path.join(td, f), "wb") as b:
+                shutil.copyfileobj(a, b)
+        chroms = make_genome(CASES[CASES[share].get("share") or share])
+        s1, s2 = make_pair_reads(pc, gcfg, chroms)
+        write_fastq(os.path.join(td, "r1.fq"), s1)
+        write_fastq(os.path.join(td, "r2.fq"), s2)
+        args = [binp, "match_pe", "index.bin", "r1.fq", "r2.fq", "out"]
+        args += ["--unord_reads"] if pc["unord"] else []
+        subprocess.run(args, cwd=td, check=True, stdout=subprocess.DEVNULL)
+        raw = open(os.path.join(td, "out_pairs.bin"), "rb").read()
+        assert raw[:8] == b"FAMEPE01"
+        n = int(np.frombuffer(raw[8:16], np.uint64)[0])
+        v1 = np.frombuffer(raw[16:16 + n], np.uint8).copy()
+        v2 = np.frombuffer(raw[16 + n:16 + 2 * n], np.uint8).copy()
+        m1 = np.frombuffer(raw[16 + 2 * n:16 + 10 * n], np.uint64).copy()
+        m2 = np.frombuffer(raw[16 + 10 * n:16 + 18 * n], np.uint64).copy()
+        meth = parse_tsv(os.path.join(td, "out_cpg.tsv"))
+        stats = json.load(open(os.path.join(td, "out_stats.json")))
+        gz(os.path.join(td, "out_cpg.tsv"), os.path.join(out, "cpg.tsv.gz"))
+    f1, o1, l1 = _flat(s1)
+    f2, o2, l2 = _flat(s2)
+    np.savez_compressed(os.path.join(out, "reads.npz"), seqs1=f1, offs1=o1, lens1=l1,
+                        seqs2=f2, offs2=o2, lens2=l2)
+    np.savez_compressed(os.path.join(out, "golden.npz"), valid1=v1, valid2=v2, mat1=m1, mat2=m2,
+                        meth=meth)
+    meta = {k: v for k, v in pc.items()}
+    meta["mode"] = "pe"
+    meta["readlen"] = gcfg["readlen"]
+    meta["index_dir"] = CASES[share].get("share") or share
+    meta["reference"] = {k: stats[k] for k in ("reads", "succ", "nonunique", "unsucc", "succ_paired",
+                                                "too_short", "r1_fwd", "r1_rev", "match_r1_fwd",
+                                                "htabsize", "readlen", "chunksize")}
+    # std::sort's order of equal counts decides ties in this path
+    meta["libstdcxx"] = gxx_version()
+    json.dump(meta, open(os.path.join(out, "case.json"), "w"), indent=1)
+    print(name, meta["reference"])
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + list(PAIRED)
     for nm in names:
-        run_case(nm)
+        run_pair_case(nm) if nm in PAIRED else run_case(nm)

@@ -28,3 +28,22 @@ def test_oracle_matches_reference(golden_case, tmp_path):
     if not c.unord:
         fwd, f, r = m.decide_strand()
         assert (f, r, fwd) == (c.ref["r1_fwd"], c.ref["r1_rev"], c.ref["match_r1_fwd"])
+
+
+def test_oracle_pairs_match_reference(golden_pair, tmp_path):
+    c = golden_pair
+    ix = orc.Index(c.index)
+    m, v1, m1, v2, m2 = orc.match_pairs_stream(ix, c.r1, c.r2, chunk=c.chunk, both_strands=c.unord,
+                                               minpd=c.minpd, maxpd=c.maxpd)
+    assert np.array_equal(v1, c.valid1)
+    assert np.array_equal(v2, c.valid2)
+    assert np.array_equal(m1, c.mat1)
+    assert np.array_equal(m2, c.mat2)
+    assert m.counters5.tolist() == c.counters
+    assert np.array_equal(m.meth(), c.meth)
+    tsv = os.path.join(str(tmp_path), "o_cpg.tsv")
+    m.write_tsv(tsv)
+    assert open(tsv, "rb").read() == c.tsv
+    if not c.unord:
+        fwd, f, r = m.decide_strand()
+        assert (f, r, fwd) == (c.ref["r1_fwd"], c.ref["r1_rev"], c.ref["match_r1_fwd"])
