@@ -20,7 +20,7 @@ MAX_READ = 255
 
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
-           "fame_match_reads_device", "fame_decide_strand", "fame_get_meth", "fame_reset",
+           "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_decide_strand", "fame_get_meth", "fame_reset",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
@@ -52,6 +52,9 @@ def load():
         L.fame_match_reads_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_uint32, C.c_int, _u64p, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
+        L.fame_set_pair_dist.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.fame_match_pairs.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, _u8p, _u64p, _u32p, C.c_uint32,
+                                       C.c_int, _u64p, _u8p, _u64p, _u8p, _u64p]
         L.fame_decide_strand.argtypes = [C.c_void_p, _u64p, _u64p, C.POINTER(C.c_int)]
         L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
         L.fame_reset.argtypes = [C.c_void_p]
@@ -106,6 +109,32 @@ class Context:
         self._check(load().fame_match_reads_device(self.h, d_seqs, d_offs, d_lens, int(n),
                                                    int(get_stranded), ptr(counters, _u64p),
                                                    d_valid or None, d_mat or None, stream or None))
+
+    def set_pair_dist(self, minpd, maxpd):
+        self._check(load().fame_set_pair_dist(self.h, int(minpd), int(maxpd)))
+
+    def match_pairs(self, r1, r2, get_stranded, counters):
+        """matchPairedReads over pairs (mate 1 triple r1, mate 2 triple r2 of
+        (seqs, offs, lens)); counters (5 x u64) accumulate."""
+        n = int(len(r1[2]))
+        if len(r2[2]) != n:
+            raise FameError("mate files hold different numbers of reads")
+        v1 = np.zeros(max(n, 1), np.uint8)
+        v2 = np.zeros(max(n, 1), np.uint8)
+        m1 = np.zeros(max(n, 1), np.uint64)
+        m2 = np.zeros(max(n, 1), np.uint64)
+        a = []
+        for seqs, offs, lens in (r1, r2):
+            seqs = np.ascontiguousarray(seqs, np.uint8)
+            if seqs.size == 0:
+                seqs = np.zeros(1, np.uint8)
+            a.append((seqs, np.ascontiguousarray(offs, np.uint64), np.ascontiguousarray(lens, np.uint32)))
+        self._check(load().fame_match_pairs(self.h, ptr(a[0][0], _u8p), ptr(a[0][1], _u64p),
+                                            ptr(a[0][2], _u32p), ptr(a[1][0], _u8p), ptr(a[1][1], _u64p),
+                                            ptr(a[1][2], _u32p), n, int(get_stranded),
+                                            ptr(counters, _u64p), ptr(v1, _u8p), ptr(m1, _u64p),
+                                            ptr(v2, _u8p), ptr(m2, _u64p)))
+        return v1[:n], m1[:n], v2[:n], m2[:n]
 
     def decide_strand(self):
         f = C.c_uint64()

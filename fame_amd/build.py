@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfamehip.so")
 INDEX_LIB = os.path.join(HERE, "libfameindex.so")
-SOURCES = [os.path.join(HERE, "csrc", "fame_device.hip"), os.path.join(HERE, "csrc", "fame_host.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", "fame_device.hip"), os.path.join(HERE, "csrc", "fame_paired.hip"),
+           os.path.join(HERE, "csrc", "fame_host.cpp")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
          # packed FP32 instructions are kept out of all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -32,294 +32,9 @@
 #include <stdint.h>
 
 #include "fame_internal.h"
+#include "fame_dev_common.h"
 
 namespace fame {
-
-// CONST.h:80-97 and ReadQueue.cpp:532, 3640
-constexpr uint32_t KMERLEN = 32;
-constexpr uint32_t QTHRESH = 5;
-constexpr uint32_t MISCOUNT = 2;
-constexpr uint32_t ADDMIS = 4;
-constexpr uint32_t NERR = MISCOUNT + ADDMIS;
-constexpr int DPBAND = FAME_DPBAND;
-static_assert(DPBAND == 2 * MISCOUNT + ADDMIS, "DP band");
-constexpr uint32_t CPGDUMMY = 0xFFFFFFFFu;
-constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-constexpr uint32_t NBH = 62;             // hopscotch NeighborhoodSize
-constexpr uint32_t JOB_GAP = 0xFFFFFFFFu; // unused verify slot
-// spaced seed (CONST.h:57-58); RefGenome::load rejects other seeds
-// (RefGenome.cpp:275-280), and so does fame_open
-constexpr uint32_t SEEDBITS = FAME_SEEDBITS;
-constexpr uint32_t MAX_PROBES = 12 * NBH; // MAX_PROBES_FOR_EMPTY_BUCKET
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint64_t rol64(uint64_t v, int s)
-{
-    s &= 63;
-    return s ? (v << s) | (v >> (64 - s)) : v;
-}
-
-// code of a byte in ntHash's reduced (C->T) table msTab (nthash.hpp:88-121):
-// 0 = seedA, 1 = seedG, 2 = seedT, 3 = zero row
-__device__ __forceinline__ uint32_t red_code(uint8_t c)
-{
-    switch (c) {
-    case 'A': case 'a': case 4:
-        return 0;
-    case 'G': case 'g': case 3:
-        return 1;
-    case 'C': case 'c': case 'T': case 't': case 1: case 7:
-        return 2;
-    default:
-        return 3;
-    }
-}
-
-__device__ __forceinline__ uint64_t red_seed(uint32_t code)
-{
-    // nthash.hpp:24-27 (seedC is never used by the reduced table)
-    return code == 0 ? 0x3c8bfbb395c60474ULL
-         : code == 1 ? 0x20323ed082572324ULL
-         : code == 2 ? 0x295549f54be24456ULL
-                     : 0ULL;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// inclusive scan over the 64 lanes of a wave
-__device__ __forceinline__ uint32_t wave_iscan_u32(uint32_t v, int lane)
-{
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint64_t rup_pow2(uint64_t v)
-{
-    if (v && !(v & (v - 1))) return v;
-    if (!v) return 1;
-    --v;
-    v |= v >> 1;
-    v |= v >> 2;
-    v |= v >> 4;
-    v |= v >> 8;
-    v |= v >> 16;
-    v |= v >> 32;
-    return v + 1;
-}
-
-// size_type(float(x) * f) with f = 0.9f or 0.1f (hopscotch_hash.h:1175-1176)
-__device__ __forceinline__ uint32_t hs_thr(uint64_t b, float f)
-{
-    float x = __fmul_rn((float)b, f);
-    return (uint32_t)x;
-}
-
-// size_type(std::ceil(float(n) / 0.9f)) (hopscotch_hash.h:1180); the float
-// quotient is obtained exactly as the correctly rounded double quotient
-// rounded to float (checked exhaustively for n < 2^24 on the host)
-__device__ __forceinline__ uint64_t hs_ceil_div09(uint64_t n)
-{
-    double q = (double)n / (double)0.9f;
-    float f = __double2float_rn(q);
-    return (uint64_t)ceilf(f);
-}
-
-// ---------------------------------------------------------------------------
-// tsl::hopscotch_map<uint32_t, uint16_t, identity> placement emulation,
-// executed by a whole wavefront: the probes for an empty bucket, the
-// neighbourhood checks, the rehash collection and the final iteration are
-// ballots over 64 buckets at a time; the decisions (and so the control flow)
-// are wave-uniform.  Only the order of iteration is reproduced: keys are
-// inserted in the order in which operator[] first created them (count
-// updates never move an element).  tb[] holds, per physical bucket, the
-// index of the key stored there.
-// ---------------------------------------------------------------------------
-
-template <typename IdxT>
-struct Hs {
-    IdxT* tb;
-    uint32_t tb_cap;
-    IdxT* over;
-    uint32_t over_cap;
-    IdxT* tmp;
-    uint32_t tmp_cap;
-    const uint32_t* key;
-    uint64_t B, mask, arr;
-    uint32_t nelem, nover, max_thr, min_thr;
-    static constexpr IdxT NONE = (IdxT)~(IdxT)0;
-};
-
-enum { HS_OK = 0, HS_FAIL = 1, HS_NEED_REHASH = 2 };
-
-// the lanes of one wavefront see each other's LDS / global writes: a
-// wavefront-scope release/acquire pair (no instruction on CDNA, an ordering
-// point for the compiler); the waves of a block run independently
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
-
-// hopscotch_hash ctor (hopscotch_hash.h:599-628) on a fresh bucket array;
-// the overflow list is kept (rehash_impl moves it to the new table)
-template <typename IdxT>
-__device__ bool hs_setup(Hs<IdxT>& h, uint64_t bucket_count, int lane)
-{
-    const uint64_t B = bucket_count ? rup_pow2(bucket_count) : 0;
-    const uint64_t arr = B ? B + NBH - 1 : 0;
-    if (arr > h.tb_cap) return false;
-    h.B = B;
-    h.mask = B ? B - 1 : 0;
-    h.arr = arr;
-    for (uint64_t i = lane; i < arr; i += 64) h.tb[i] = Hs<IdxT>::NONE;
-    h.max_thr = hs_thr(B, 0.9f);
-    h.min_thr = hs_thr(B, 0.1f);
-    wave_sync();
-    return true;
-}
-
-// find_empty_bucket hopscotch_hash.h:1491-1500
-template <typename IdxT>
-__device__ uint64_t hs_find_empty(const Hs<IdxT>& h, uint64_t ib, int lane)
-{
-    const uint64_t lim = ib + MAX_PROBES < h.arr ? ib + MAX_PROBES : h.arr;
-    for (uint64_t b0 = ib; b0 < lim; b0 += 64) {
-        const uint64_t b = b0 + lane;
-        const uint64_t bal = __ballot(b < lim && h.tb[b] == Hs<IdxT>::NONE);
-        if (bal) return b0 + __ffsll((long long)bal) - 1;
-    }
-    return h.arr;
-}
-
-// swap_empty_bucket_closer hopscotch_hash.h:1545-1578 (rare: the
-// neighbourhood of a home bucket is full); computed by lane 0, where the
-// neighbourhood bitmap of bucket tc is "the buckets tc+i holding a key whose
-// home is tc"
-template <typename IdxT>
-__device__ bool hs_swap_closer(Hs<IdxT>& h, uint64_t& e, int lane)
-{
-    uint64_t ne = e;
-    int found = 0;
-    if (lane == 0) {
-        const uint64_t start = e - NBH + 1;
-        for (uint64_t tc = start; tc < e && !found; ++tc) {
-            for (uint64_t ts = tc; ts < e; ++ts) {
-                const IdxT i = h.tb[ts];
-                if (i != Hs<IdxT>::NONE && (h.key[i] & h.mask) == tc) {
-                    h.tb[e] = i;
-                    h.tb[ts] = Hs<IdxT>::NONE;
-                    ne = ts;
-                    found = 1;
-                    break;
-                }
-            }
-        }
-    }
-    wave_sync();
-    found = __shfl(found, 0, 64);
-    e = __shfl((unsigned long long)ne, 0, 64);
-    return found != 0;
-}
-
-// will_neighborhood_change_on_rehash hopscotch_hash.h:1466-1486
-template <typename IdxT>
-__device__ bool hs_will_change(const Hs<IdxT>& h, uint64_t ib, int lane)
-{
-    const uint64_t mask2 = rup_pow2((h.mask + 1) * 2) - 1;
-    const uint64_t b = ib + lane;
-    bool ch = false;
-    if (lane < (int)NBH && b < h.arr) {
-        const IdxT i = h.tb[b];
-        if (i != Hs<IdxT>::NONE) {
-            const uint64_t k = h.key[i];
-            ch = (k & h.mask) != (k & mask2);
-        }
-    }
-    return __ballot(ch) != 0;
-}
-
-// one pass of insert_impl hopscotch_hash.h:1426-1458 without its rehash
-// calls: HS_NEED_REHASH where the reference rehashes (load threshold reached
-// first, or no bucket in the neighbourhood and no overflow allowed)
-template <typename IdxT>
-__device__ int hs_place(Hs<IdxT>& h, uint32_t idx, int lane)
-{
-    if (h.nelem - h.nover >= h.max_thr) return HS_NEED_REHASH;
-    const uint64_t ib = h.key[idx] & h.mask;
-    uint64_t e = hs_find_empty(h, ib, lane);
-    if (e < h.arr) {
-        for (;;) {
-            if (e - ib < NBH) {
-                if (lane == 0) h.tb[e] = (IdxT)idx;
-                wave_sync();
-                h.nelem++;
-                return HS_OK;
-            }
-            if (!hs_swap_closer(h, e, lane)) break;
-        }
-    }
-    if (h.nelem < h.min_thr || !hs_will_change(h, ib, lane)) {
-        if (h.nover >= h.over_cap) return HS_FAIL;
-        if (lane == 0) h.over[h.nover] = (IdxT)idx;
-        wave_sync();
-        h.nover++;
-        h.nelem++;
-        return HS_OK;
-    }
-    return HS_NEED_REHASH;
-}
-
-// rehash_impl hopscotch_hash.h:1244-1290: a new table, the overflow list
-// carried over, then the old buckets re-inserted in physical order.  A
-// rehash needed inside this re-insertion is reported as a capacity failure.
-template <typename IdxT>
-__device__ int hs_rehash(Hs<IdxT>& h, uint64_t count, int lane)
-{
-    const uint64_t need = hs_ceil_div09(h.nelem);
-    if (need > count) count = need;
-    uint32_t n = 0;
-    for (uint64_t b0 = 0; b0 < h.arr; b0 += 64) {
-        const uint64_t b = b0 + lane;
-        const IdxT i = b < h.arr ? h.tb[b] : Hs<IdxT>::NONE;
-        const bool occ = i != Hs<IdxT>::NONE;
-        const uint64_t bal = __ballot(occ);
-        const uint32_t p = n + __popcll(bal & lanes_below(lane));
-        if (occ && p < h.tmp_cap) h.tmp[p] = i;
-        n += __popcll(bal);
-    }
-    if (n > h.tmp_cap) return HS_FAIL;
-    wave_sync();
-    if (!hs_setup(h, count, lane)) return HS_FAIL;
-    h.nelem = h.nover;
-    for (uint32_t t = 0; t < n; ++t)
-        if (hs_place(h, (uint32_t)h.tmp[t], lane) != HS_OK) return HS_FAIL;
-    return HS_OK;
-}
-
-// operator[] creating a key: insert_impl with its rehash-and-retry loop
-template <typename IdxT>
-__device__ int hs_insert(Hs<IdxT>& h, uint32_t idx, int lane)
-{
-    for (;;) {
-        const int r = hs_place(h, idx, lane);
-        if (r == HS_OK) return HS_OK;
-        if (r == HS_FAIL) return HS_FAIL;
-        if (hs_rehash(h, (h.mask + 1) * 2, lane) != HS_OK) return HS_FAIL;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // k_prep
@@ -335,7 +50,8 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                                               const uint32_t* __restrict__ lens, uint32_t n, uint32_t readlen,
                                               uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
                                               uint8_t* __restrict__ codes, uint32_t ori_mask,
-                                              uint32_t* __restrict__ err_flag, uint32_t* __restrict__ max_len)
+                                              uint32_t* __restrict__ err_flag, uint32_t* __restrict__ max_len,
+                                              uint32_t nsplit, uint32_t minlen, int paired)
 {
     const int lane = threadIdx.x & 63;
     uint32_t lmax = 0; // longest read this wave prepared for seeding
@@ -346,17 +62,23 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
         const uint8_t* s = seqs + offs[r];
         uint8_t* q = rev + offs[r];
         uint8_t st = 0;
-        // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned
-        // subtraction of the reference is kept
-        if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
+        // ori 0 / 1 of mate 2 (reads from nsplit on) pair with ori 1 / 0 of mate 1
+        const uint32_t om = r < nsplit ? ori_mask : (((ori_mask & 1) << 1) | ((ori_mask >> 1) & 1));
+        if (!paired) {
+            // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned
+            // subtraction of the reference is kept
+            if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
+            if (L < KMERLEN) st = 3;
+        } else if (L < minlen) {
+            st = 1; // readSize < ceil(0.75 READLEN) (ReadQueue.cpp:971-980); minlen >= KMERLEN
+        }
         if (L > FAME_DEV_MAX_READ) {
             if (lane == 0) atomicOr(err_flag, 1u);
             st = 3;
         }
-        if (L < KMERLEN) st = 3;
         uint8_t* c0 = codes + (size_t)(2 * r) * 256;
         uint8_t* c1 = c0 + 256;
-        if (st == 0) {
+        if (st == 0 || (paired && st == 1)) {
             bool nflag = false;
             for (uint32_t p = lane; p < L; p += 64) {
                 const uint8_t c = s[p];
@@ -367,19 +89,21 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                 case 'G': o = 'C'; break;
                 case 'T': o = 'A'; break;
                 case 'N': nflag = true; break;
-                default: break; // unknown characters stay '\0' in revSeq
+                default: nflag |= paired != 0; break; // unknown characters stay '\0' in revSeq
                 }
                 q[L - 1 - p] = o;
-                if (ori_mask & 1) c0[p] = (uint8_t)red_code(c);
-                if (ori_mask & 2) c1[L - 1 - p] = (uint8_t)red_code(o);
+                if (om & 1) c0[p] = (uint8_t)red_code(c);
+                if (om & 2) c1[L - 1 - p] = (uint8_t)red_code(o);
             }
-            if (__ballot(nflag)) st = 3;
+            // single-end: N invalidates (ReadQueue.cpp:456-460); paired: N and
+            // unknown letters invalidate the mate (:1016-1027)
+            if (__ballot(nflag)) st = paired ? (uint8_t)(st | 2) : (uint8_t)3;
         }
         if (st == 0 && L > lmax) lmax = L;
         if (lane == 0) {
             pre[r] = st;
-            c0[255] = (st == 0 && (ori_mask & 1)) ? (uint8_t)L : 0;
-            c1[255] = (st == 0 && (ori_mask & 2)) ? (uint8_t)L : 0;
+            c0[255] = (st == 0 && (om & 1)) ? (uint8_t)L : 0;
+            c1[255] = (st == 0 && (om & 2)) ? (uint8_t)L : 0;
         }
     }
     if (lane == 0 && lmax) atomicMax(max_len, lmax);
@@ -412,84 +136,6 @@ struct SeedWork {
     uint32_t* cand;    // [2][cap_ins] candidate windows per map
     uint8_t* code;     // [L] reduced base codes of the read
 };
-
-struct SeedShared {
-    // rot[i][c] = rol(seed_c, 31 - i): the term of position i of a spaced
-    // k-mer (nthash.hpp:480-489); code 3 (N and others) contributes 0
-    uint64_t rot[KMERLEN][4];
-};
-
-// filled by the whole block at kernel start
-__device__ __forceinline__ void seed_table_init(SeedShared& sh)
-{
-    for (int t = threadIdx.x; t < (int)KMERLEN * 4; t += blockDim.x) {
-        const int i = t >> 2, c = t & 3;
-        sh.rot[i][c] = rol64(red_seed((uint32_t)c), (int)(KMERLEN - 1 - i));
-    }
-    __syncthreads();
-}
-
-// Returns 0 when done, 1 when the (read, orientation) needs the heavy path
-// (capacity); E / bucketCount are reported through the out pointers.
-// Candidate slots of the verify pools.  Persistent seed waves take pool
-// space in chunks (one atomic per chunk instead of one per read); the slots
-// a wave leaves unused are marked JOB_GAP.  `exact` takes exactly what a
-// read needs (heavy path).
-struct PoolChunk {
-    uint32_t pool, base, left; // one open chunk at a time (a batch rarely mixes pools)
-};
-constexpr uint32_t POOL_CHUNK = 64;
-
-__device__ __forceinline__ uint32_t pc_get_cap(const BatchDev& bd, uint32_t p)
-{
-    uint32_t v = bd.pool_cap[0];
-#pragma unroll
-    for (uint32_t q = 1; q < FAME_NPOOL; ++q) v = p == q ? bd.pool_cap[q] : v;
-    return v;
-}
-__device__ __forceinline__ uint32_t* pc_meta(const BatchDev& bd, uint32_t p)
-{
-    uint32_t* v = bd.job_meta[0];
-#pragma unroll
-    for (uint32_t q = 1; q < FAME_NPOOL; ++q) v = p == q ? bd.job_meta[q] : v;
-    return v;
-}
-
-__device__ void pool_gap_fill(const BatchDev& bd, uint32_t p, uint32_t base, uint32_t n, int lane)
-{
-    if (base + n > pc_get_cap(bd, p)) return;
-    uint32_t* meta = pc_meta(bd, p);
-    for (uint32_t t = lane; t < n; t += 64) meta[base + t] = JOB_GAP;
-}
-
-// returns the first slot of `need` consecutive slots, or ~0u if the pool is full
-__device__ uint32_t pool_take(const BatchDev& bd, PoolChunk& pc, uint32_t p, uint32_t need, bool exact, int lane)
-{
-    if (exact) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&bd.pool_n[p], need);
-        b = __shfl(b, 0, 64);
-        return b + need <= bd.pool_cap[p] ? b : ~0u;
-    }
-    if (pc.pool != p || pc.left < need) {
-        if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
-        const uint32_t sz = need > POOL_CHUNK ? need : POOL_CHUNK;
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&bd.pool_n[p], sz);
-        b = __shfl(b, 0, 64);
-        pc.pool = p;
-        pc.base = b;
-        pc.left = sz;
-        if (b + sz > pc_get_cap(bd, p)) {
-            pc.left = 0;
-            return ~0u;
-        }
-    }
-    const uint32_t b = pc.base;
-    pc.base += need;
-    pc.left -= need;
-    return b;
-}
 
 struct SeedStats {
     uint64_t kmers, entries, seeded, bytes;
@@ -575,7 +221,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         }
         uint32_t k = lo;
         while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
-        w.ent[j] = ix.kent[(uint64_t)w.klo[k] + (j - w.koff[k])];
+        w.ent[j] = ix.kent[(uint64_t)w.klo[k] + (j - w.koff[k])] & 0x7FFFFFFFu; // without isStart
         w.ekm[j] = (uint8_t)k;
     }
     wave_sync();
@@ -1007,7 +653,10 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint32_t ro = bd.job_ro[pool][j];
     const uint32_t r = ro >> 1, ori = ro & 1;
     const bool rev = (meta >> 31) != 0;
-    const uint32_t id = meta & 0x7FFFFFFFu;
+    const uint32_t id = meta & 0x3FFFFFFFu;
+    // bit 30: matchRevSecond's scan, which has no guard for a window at
+    // the chromosome start (ReadQueue.cpp:4736) and then also reads letter 0
+    const bool unguarded = ((meta >> 30) & 1) != 0;
     const uint32_t m = bd.lens[r]; // 64(W-1) < m <= 64W: the pool holds reads of W words
     const int z = 64 * W - (int)m;
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
@@ -1084,6 +733,10 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         if (s0 >= clen) s0 = clen - 1;
         text = cs + e0 + 1;
         tlen = s0 > e0 ? s0 - e0 : 0;
+        if (unguarded && startPos == 0) {
+            text = cs;
+            tlen = s0 + 1;
+        }
     }
     uint64_t Pv[W], Mv[W];
 #pragma unroll
@@ -1381,19 +1034,6 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
 // k_select: saQuerySeedSetRef uniqueness logic + decision block, per read
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint64_t mk_match(uint32_t off, uint32_t err, uint64_t fwd, uint64_t meta)
-{
-    // MATCH::constructMatch structs.h:185-188 (isStart is always 0)
-    return (uint64_t)(off & 0xFFFFu) | ((uint64_t)(err & 0xFFu) << 16) | (fwd << 24) | (meta << 32);
-}
-
-__device__ __forceinline__ uint32_t job_match_at(const BatchDev& bd, uint32_t pool, uint32_t j,
-                                                 uint32_t k)
-{
-    if (bd.job_nm[pool][j] > bd.match_cap) return bd.ovf_buf[bd.job_ovf_off[pool][j] + k];
-    return bd.job_match[pool][(size_t)j * bd.match_cap + k];
-}
-
 __device__ int query_seed_set(const DevIndex& ix, const BatchDev& bd, uint32_t ro, uint32_t pool,
                               uint64_t* mat)
 {
@@ -1414,7 +1054,7 @@ __device__ int query_seed_set(const DevIndex& ix, const BatchDev& bd, uint32_t r
         }
         const bool isfwd = c < nf;
         const uint32_t j = off + c;
-        const uint32_t id = bd.job_meta[pool][j] & 0x7FFFFFFFu;
+        const uint32_t id = bd.job_meta[pool][j] & 0x3FFFFFFFu;
         const uint4 w = ix.win[id];
         const uint32_t nm = bd.job_nm[pool][j];
         uint32_t i = 0;
@@ -2010,9 +1650,17 @@ __global__ void k_pack_kmers(const uint32_t* __restrict__ raw, const uint8_t* __
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t g = first + i;
-    const uint32_t meta = raw[2 * i] & 0x7FFFFFFFu; // KMER_S::getMetaCpG
+    const uint32_t m = raw[2 * i];
+    const uint32_t meta = m & 0x3FFFFFFFu; // KMER_S::getMetaCpG (metaId < 2^30 is checked on load)
     const uint32_t fwd = (strand_bits[g >> 3] >> (g & 7)) & 1;
-    out[g] = (meta << 1) | fwd;
+    out[g] = (m & 0x80000000u) | (meta << 1) | fwd; // isStartCpG in bit 31
+}
+
+// KMER_S::tmask of every entry (second word of kmerTableSmall)
+__global__ void k_pack_tmask(const uint32_t* __restrict__ raw, uint64_t n, uint32_t* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = raw[2 * i + 1];
 }
 
 __global__ void k_narrow_tab(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out)
@@ -2036,7 +1684,25 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
     if (!n) return;
     const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
-                       bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len);
+                       bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
+                       0xFFFFFFFFu, 0u, 0);
+}
+
+// paired batches: 2n mate reads, mate 2 from nsplit on; pre[] = 1 short | 2 N
+// or unknown letter
+void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, uint32_t minlen, hipStream_t st)
+{
+    if (!n_reads) return;
+    const uint32_t blocks = (n_reads + 3) / 4;
+    hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
+                       bd.lens, n_reads, 0u, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
+                       nsplit, minlen, 1);
+}
+
+void launch_pack_tmask(const uint32_t* raw, uint64_t n, uint32_t* out, hipStream_t st)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_pack_tmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, n, out);
 }
 
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -77,7 +78,7 @@ struct fame_ctx {
     std::vector<uint8_t> cpg_chrom;
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
-    DevBuf d_tab, d_kent, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
+    DevBuf d_tab, d_kent, d_ktm, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
@@ -90,6 +91,16 @@ struct fame_ctx {
     DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_dec, b_trace, b_direct, b_dplist, b_methn;
     uint32_t pool_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t match_cap = 4;
+    // paired-end: MINPDIST / MAXPDIST (CONST.h:41-42 defaults) and buffers
+    uint32_t minpd = 50, maxpd = 550;
+    DevBuf d_pe_counters;
+    DevBuf b2_pool_n, b2_job_meta[FAME_NPOOL], b2_job_ro[FAME_NPOOL], b2_job_nm[FAME_NPOOL],
+        b2_job_match[FAME_NPOOL], b2_job_ovf_off[FAME_NPOOL];
+    DevBuf b2_ovf_n, b2_ovf_pool, b2_ovf_job, b2_ovf_buf, b2_ovf_total;
+    DevBuf b_acc1[FAME_NPOOL], b_acc2[FAME_NPOOL];
+    DevBuf b_pe_off, b_pe_nf, b_pe_nr, b_pe_m1, b_pe_mapn, b_pe_map, b_pe_srt, b_pe_scratch, b_pe_big;
+    uint32_t pool2_cap[FAME_NPOOL] = {0, 0, 0, 0};
+    uint32_t pe_map_cap = 0;
     uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
     double timings[6] = {0, 0, 0, 0, 0, 0};
     uint64_t last_heavy = 0, last_ovf = 0;
@@ -245,6 +256,7 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(hipMemcpy(sbits.p, bits.data(), bits.size(), hipMemcpyHostToDevice));
     }
     HIPC(c->d_kent.reserve(4 * c->nkmer + 4));
+    HIPC(c->d_ktm.reserve(4 * c->nkmer + 4));
     {
         DevBuf stage;
         HIPC(stage.reserve(kStage));
@@ -256,6 +268,7 @@ void load_index(fame_ctx* c, const std::string& path)
             f.read(buf.data(), 8 * n);
             HIPC(hipMemcpy(stage.p, buf.data(), 8 * n, hipMemcpyHostToDevice));
             launch_pack_kmers(stage.as<uint32_t>(), sbits.as<uint8_t>(), first, n, c->d_kent.as<uint32_t>(), st);
+            launch_pack_tmask(stage.as<uint32_t>(), n, c->d_ktm.as<uint32_t>() + first, st);
             HIPC(hipStreamSynchronize(st));
         }
     }
@@ -263,6 +276,7 @@ void load_index(fame_ctx* c, const std::string& path)
     f.skip(8 * f.get<uint64_t>()); // metaStartCpGs
     // windows {u32 startPos; u8 chrom; 3 pad; u32 startInd; u32 endInd}
     c->nwin = f.get<uint64_t>();
+    if (c->nwin >= (1ull << 30)) throw FameError{FAME_ERR_FORMAT, "more than 2^30 windows"};
     {
         std::vector<uint32_t> raw(4 * c->nwin);
         f.read(raw.data(), 16 * c->nwin);
@@ -274,7 +288,20 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(hipMemcpy(c->d_win.p, raw.data(), 16 * c->nwin, hipMemcpyHostToDevice));
     }
     c->nfilt = f.get<uint64_t>();
-    f.skip(8 * c->nfilt); // filteredKmers (paired-end filter only)
+    {
+        // filteredKmers: the paired path looks up a u32 k-mer code, so only
+        // the values below 2^32 can ever be found (ReadQueue.cpp:2527, 2643)
+        std::vector<uint64_t> fk(c->nfilt);
+        f.read(fk.data(), 8 * c->nfilt);
+        std::vector<uint32_t> f32;
+        for (uint64_t v : fk)
+            if (v < (1ull << 32)) f32.push_back((uint32_t)v);
+        std::sort(f32.begin(), f32.end());
+        f32.erase(std::unique(f32.begin(), f32.end()), f32.end());
+        c->ix.nfilt32 = (uint32_t)f32.size();
+        HIPC(c->d_filt32.reserve(4 * f32.size() + 4));
+        if (!f32.empty()) HIPC(hipMemcpy(c->d_filt32.p, f32.data(), 4 * f32.size(), hipMemcpyHostToDevice));
+    }
     // chrMap
     c->chr_name.assign(256, std::string());
     uint64_t nmap = 0;
@@ -300,6 +327,8 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.hpow2 = (c->htab & (c->htab - 1)) == 0;
     ix.hmask = c->htab - 1;
     ix.kent = c->d_kent.as<uint32_t>();
+    ix.ktm = c->d_ktm.as<uint32_t>();
+    ix.filt32 = c->d_filt32.as<uint32_t>();
     ix.win = c->d_win.as<uint4>();
     ix.seq = c->d_seq.as<uint8_t>();
     ix.coff = c->d_coff.as<uint64_t>();
@@ -560,6 +589,297 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     return FAME_OK;
 }
 
+// verify jobs of one phase: the match arrays, then the jobs whose matches
+// overflowed match_cap once more into an exactly sized buffer
+int verify_phase(fame_ctx* c, BatchDev& bd, const uint32_t pool_n[FAME_NPOOL], DevBuf (&nm)[FAME_NPOOL],
+                 DevBuf (&match)[FAME_NPOOL], DevBuf (&ovf_off)[FAME_NPOOL], DevBuf& ovf_buf, DevBuf& ovf_total,
+                 hipStream_t st)
+{
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        const uint32_t cap = std::max<uint32_t>(pool_n[p], 1);
+        bd.job_nm[p] = grow<uint32_t>(nm[p], cap);
+        bd.job_match[p] = grow<uint32_t>(match[p], (size_t)cap * c->match_cap);
+        bd.job_ovf_off[p] = grow<uint32_t>(ovf_off[p], cap);
+    }
+    bd.ovf_buf = grow<uint32_t>(ovf_buf, 1);
+    HIPC(hipMemsetAsync(bd.ovf_n, 0, 4, st));
+    for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, pool_n[p], nullptr, st);
+    uint32_t ovf_n = 0, flags = 0;
+    HIPC(hipMemcpyAsync(&ovf_n, bd.ovf_n, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_OVF_FULL) {
+        set_err(c, "match overflow list full");
+        return FAME_ERR_CAPACITY;
+    }
+    c->last_ovf += ovf_n;
+    if (ovf_n) {
+        uint32_t* d_total = grow<uint32_t>(ovf_total, 2);
+        launch_ovf_offsets(bd, ovf_n, d_total, st);
+        uint32_t total = 0;
+        HIPC(hipMemcpyAsync(&total, d_total, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        bd.ovf_buf = grow<uint32_t>(ovf_buf, total + 1);
+        for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, ovf_n, bd.ovf_job, st);
+    }
+    return FAME_OK;
+}
+
+// scratch slice of the paired seed stage for reads of at most cap_e entries
+void pe_scratch_dims(PairDev& pd, uint32_t cap_e)
+{
+    auto rup = [](uint64_t v) {
+        uint64_t p = 1;
+        while (p < v) p <<= 1;
+        return p;
+    };
+    pd.cap_e = cap_e;
+    pd.hsize = (uint32_t)rup(2ull * cap_e);
+    pd.tb_cap = (uint32_t)(2 * rup(2ull * cap_e) + 64);
+    size_t stride = 0;
+    auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+    add(8 * 256);
+    add(4 * 256);
+    add(4 * 256);
+    add(4 * 256);
+    add(256);
+    add(4ull * cap_e);
+    add(cap_e);
+    add(4ull * cap_e);
+    for (int i = 0; i < 3; ++i) add(4ull * pd.hsize);
+    add(8ull * cap_e);
+    add(4ull * cap_e);
+    add(4ull * pd.tb_cap);
+    add(4ull * cap_e);
+    add(4ull * cap_e);
+    pd.stride = stride;
+}
+
+// ReadQueue::matchPairedReads over n pairs held as 2n device reads (mate 1 of
+// pair i is read i, mate 2 read n + i)
+int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+              uint64_t total_bytes, int get_stranded, uint64_t counters[5], hipStream_t st)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Timer tm(st);
+    const uint32_t nr = 2 * n;
+    const uint32_t nitems = 2 * n;
+    c->last_ovf = 0;
+    c->last_heavy = 0;
+    BatchDev bd{};
+    bd.seqs = d_seqs;
+    bd.offs = d_offs;
+    bd.lens = d_lens;
+    bd.rev = grow<uint8_t>(c->b_rev, total_bytes + 64);
+    bd.pre = grow<uint8_t>(c->b_pre, nr + 1);
+    const bool o0 = c->both || get_stranded || c->match_r1_fwd;
+    const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
+    bd.ori_mask = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
+    bd.codes = grow<uint8_t>(c->b_codes, (size_t)2 * nr * 256 + 256);
+    bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
+    bd.match_cap = c->match_cap;
+    const uint32_t seed_grid = std::min<uint32_t>(std::max<uint32_t>(nitems, 1), 4096);
+    const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n) + 64u * seed_grid;
+    for (int p = 0; p < FAME_NPOOL; ++p) c->pool_cap[p] = std::max(c->pool_cap[p], want_cap);
+    bd.ovf_n = grow<uint32_t>(c->b_ovf_n, 1);
+    bd.ovf_list_cap = std::max<uint32_t>(1024, n / 4);
+    bd.ovf_pool = grow<uint32_t>(c->b_ovf_pool, bd.ovf_list_cap);
+    bd.ovf_job = grow<uint32_t>(c->b_ovf_job, bd.ovf_list_cap);
+    bd.status = grow<uint8_t>(c->b_status, nr + 1);
+    bd.use = grow<uint8_t>(c->b_use, nr + 1);
+    bd.mat = grow<uint64_t>(c->b_mat, nr + 1);
+    bd.valid = grow<uint8_t>(c->b_valid, nr + 1);
+    bd.counters = c->d_counters.as<uint64_t>();
+    bd.meth = c->d_meth.as<uint32_t>();
+    bd.flags = c->d_flags.as<uint32_t>();
+    bd.max_len = bd.flags + 1;
+    bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
+    HIPC(hipMemsetAsync(bd.stats, 0, 8 * FAME_NSTATS, st));
+    bd.direct_list = grow<uint32_t>(c->b_direct, nr + 1);
+    bd.dp_list = grow<uint32_t>(c->b_dplist, nr + 1);
+    bd.direct_n = grow<uint32_t>(c->b_methn, 2);
+    bd.dp_n = bd.direct_n + 1;
+    HIPC(hipMemsetAsync(bd.direct_n, 0, 8, st));
+    HIPC(hipMemsetAsync(bd.flags, 0, 8, st));
+
+    PairDev pd{};
+    pd.n = n;
+    pd.combos = bd.ori_mask;
+    pd.cw = c->maxpd / c->winlen + 1;
+    pd.maxpd = c->maxpd;
+    pd.map_off = grow<uint32_t>(c->b_pe_off, nitems + 1);
+    pd.map_nf = grow<uint32_t>(c->b_pe_nf, nitems + 1);
+    pd.map_nr = grow<uint32_t>(c->b_pe_nr, nitems + 1);
+    pd.m1_n = grow<uint32_t>(c->b_pe_m1, nitems + 1);
+    pd.map_n = grow<uint32_t>(c->b_pe_mapn, 4);
+    pd.big_n = pd.map_n + 1;
+    c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, std::max<uint32_t>(1u << 16, 8 * n));
+    HIPC(c->d_pe_counters.reserve(64));
+    pd.counters = c->d_pe_counters.as<uint64_t>();
+    HIPC(hipMemsetAsync(pd.counters, 0, 40, st));
+    pd.big_cap = std::max<uint32_t>(1024, nitems);
+    pd.big_item = grow<uint32_t>(c->b_pe_big, 2ull * pd.big_cap);
+    pd.big_e = pd.big_item + pd.big_cap;
+    pe_scratch_dims(pd, 2048);
+    pd.nslices = seed_grid;
+    const uint32_t minlen = (uint32_t)std::ceil((float)c->readlen * 0.75);
+
+    tm.mark(0);
+    launch_prep_pairs(bd, nr, n, minlen, st);
+    tm.mark(1);
+    // seed: retried with larger pools when a pool overflowed
+    uint32_t pool_n[FAME_NPOOL];
+    uint32_t flags = 0;
+    for (int attempt = 0;; ++attempt) {
+        for (int p = 0; p < FAME_NPOOL; ++p) {
+            bd.pool_cap[p] = c->pool_cap[p];
+            bd.job_meta[p] = grow<uint32_t>(c->b_job_meta[p], c->pool_cap[p]);
+            bd.job_ro[p] = grow<uint32_t>(c->b_job_ro[p], c->pool_cap[p]);
+        }
+        pd.map_cap = c->pe_map_cap;
+        pd.map = grow<PeItem>(c->b_pe_map, pd.map_cap);
+        pd.srt = grow<PeItem>(c->b_pe_srt, pd.map_cap);
+        pe_scratch_dims(pd, 2048);
+        pd.scratch = grow<uint8_t>(c->b_pe_scratch, pd.stride * pd.nslices);
+        pd.list = nullptr;
+        pd.list_n = 0;
+        HIPC(hipMemsetAsync(bd.pool_n, 0, 4 * FAME_NPOOL, st));
+        HIPC(hipMemsetAsync(pd.map_n, 0, 8, st));
+        tm.mark(2);
+        launch_pe_seed(c->ix, bd, pd, seed_grid, st);
+        tm.mark(3);
+        uint32_t cnt2[2] = {0, 0};
+        HIPC(hipMemcpyAsync(cnt2, pd.map_n, 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (flags & FLAG_TOO_LONG) {
+            set_err(c, "read longer than FAME_MAX_READ");
+            return FAME_ERR_ARG;
+        }
+        if (flags & FLAG_HEAVY_FULL) {
+            set_err(c, "paired seed overflow list full");
+            return FAME_ERR_CAPACITY;
+        }
+        const uint32_t big_n = cnt2[1];
+        c->last_heavy = big_n;
+        if (big_n && !(flags & FLAG_POOL_FULL)) {
+            std::vector<uint32_t> be(big_n);
+            HIPC(hipMemcpy(be.data(), pd.big_e, 4 * big_n, hipMemcpyDeviceToHost));
+            uint64_t emax = 1;
+            for (uint32_t x = 0; x < big_n; ++x) emax = std::max<uint64_t>(emax, be[x]);
+            if (emax > (1ull << 24)) {
+                set_err(c, "a mate has more than 2^24 index hits");
+                return FAME_ERR_CAPACITY;
+            }
+            PairDev pb = pd;
+            pe_scratch_dims(pb, (uint32_t)emax);
+            const size_t budget = 2ull << 30;
+            pb.nslices = (uint32_t)std::max<size_t>(1, std::min<size_t>(big_n, budget / pb.stride));
+            pb.scratch = grow<uint8_t>(c->b_heavy_scratch, pb.stride * pb.nslices);
+            pb.list = pd.big_item;
+            pb.list_n = big_n;
+            tm.mark(12);
+            launch_pe_seed(c->ix, bd, pb, pb.nslices, st);
+            tm.mark(13);
+            HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+        }
+        if (flags & FLAG_CAPACITY) {
+            set_err(c, "a pair's candidate maps exceeded the device emulation limits");
+            return FAME_ERR_CAPACITY;
+        }
+        HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(cnt2, pd.map_n, 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (!(flags & FLAG_POOL_FULL)) break;
+        if (attempt > 3) {
+            set_err(c, "paired candidate pools keep overflowing");
+            return FAME_ERR_CAPACITY;
+        }
+        for (int p = 0; p < FAME_NPOOL; ++p)
+            c->pool_cap[p] = std::max<uint32_t>(c->pool_cap[p], pool_n[p] + (pool_n[p] >> 2) + 1024);
+        c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, cnt2[0] + (cnt2[0] >> 2) + 1024);
+        HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    }
+    // first reads over their windows
+    tm.mark(4);
+    int rc = verify_phase(c, bd, pool_n, c->b_job_nm, c->b_job_match, c->b_job_ovf_off, c->b_ovf_buf,
+                          c->b_ovf_total, st);
+    if (rc != FAME_OK) return rc;
+    tm.mark(5);
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        pd.acc1[p] = grow<uint8_t>(c->b_acc1[p], std::max<uint32_t>(pool_n[p], 1));
+        if (pool_n[p]) HIPC(hipMemsetAsync(pd.acc1[p], 0, pool_n[p], st));
+    }
+    // first walks, then the second reads' jobs (retried with larger pools)
+    BatchDev b2 = bd;
+    b2.pool_n = grow<uint32_t>(c->b2_pool_n, FAME_NPOOL);
+    b2.ovf_n = grow<uint32_t>(c->b2_ovf_n, 1);
+    b2.ovf_pool = grow<uint32_t>(c->b2_ovf_pool, bd.ovf_list_cap);
+    b2.ovf_job = grow<uint32_t>(c->b2_ovf_job, bd.ovf_list_cap);
+    uint32_t pool2_n[FAME_NPOOL];
+    for (int p = 0; p < FAME_NPOOL; ++p) c->pool2_cap[p] = std::max(c->pool2_cap[p], std::max<uint32_t>(1u << 16, 2 * n));
+    for (int attempt = 0;; ++attempt) {
+        for (int p = 0; p < FAME_NPOOL; ++p) {
+            b2.pool_cap[p] = c->pool2_cap[p];
+            b2.job_meta[p] = grow<uint32_t>(c->b2_job_meta[p], c->pool2_cap[p]);
+            b2.job_ro[p] = grow<uint32_t>(c->b2_job_ro[p], c->pool2_cap[p]);
+        }
+        HIPC(hipMemsetAsync(b2.pool_n, 0, 4 * FAME_NPOOL, st));
+        tm.mark(6);
+        launch_pe_walk1(c->ix, bd, b2, pd, st);
+        tm.mark(7);
+        HIPC(hipMemcpyAsync(pool2_n, b2.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (!(flags & FLAG_POOL_FULL)) break;
+        if (attempt > 3) {
+            set_err(c, "second-read pools keep overflowing");
+            return FAME_ERR_CAPACITY;
+        }
+        for (int p = 0; p < FAME_NPOOL; ++p)
+            c->pool2_cap[p] = std::max<uint32_t>(c->pool2_cap[p], pool2_n[p] + (pool2_n[p] >> 2) + 1024);
+        HIPC(hipMemsetAsync(bd.flags, 0, 4, st));
+    }
+    tm.mark(8);
+    rc = verify_phase(c, b2, pool2_n, c->b2_job_nm, c->b2_job_match, c->b2_job_ovf_off, c->b2_ovf_buf,
+                      c->b2_ovf_total, st);
+    if (rc != FAME_OK) return rc;
+    tm.mark(9);
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        pd.acc2[p] = grow<uint8_t>(c->b_acc2[p], std::max<uint32_t>(pool2_n[p], 1));
+        if (pool2_n[p]) HIPC(hipMemsetAsync(pd.acc2[p], 0, pool2_n[p], st));
+    }
+    tm.mark(10);
+    launch_pe_final(c->ix, bd, b2, pd, get_stranded, st);
+    tm.mark(11);
+    {
+        const size_t cells = (size_t)FAME_DP_SROWS * (2 * (FAME_DPBAND + 1) + 1);
+        uint16_t* dp = grow<uint16_t>(c->b_dp, cells * c->meth_threads);
+        uint64_t* dec = grow<uint64_t>(c->b_dec, (size_t)(FAME_DEV_MAX_READ + 1) * c->meth_threads);
+        uint64_t* tr = grow<uint64_t>(c->b_trace, (size_t)FAME_DP_TRACE_WORDS * c->meth_threads);
+        tm.mark(14);
+        launch_meth(c->ix, bd, nr, dp, dec, tr, c->meth_threads, st);
+        tm.mark(15);
+    }
+    uint64_t cnt[5];
+    HIPC(hipMemcpyAsync(cnt, pd.counters, 40, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_CAPACITY) {
+        set_err(c, "alignment trace exceeded its capacity");
+        return FAME_ERR_CAPACITY;
+    }
+    for (int k = 0; k < 5; ++k) counters[k] += cnt[k];
+    c->timings[0] = tm.ms(0, 1);
+    c->timings[1] = tm.ms(2, 3) + (c->last_heavy ? tm.ms(12, 13) : 0.0);
+    c->timings[2] = tm.ms(4, 5) + tm.ms(8, 9);
+    c->timings[3] = tm.ms(6, 7) + tm.ms(10, 11);
+    c->timings[4] = tm.ms(14, 15);
+    c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FAME_OK;
+}
+
 template <typename F>
 int guarded(fame_ctx* c, F&& f)
 {
@@ -687,6 +1007,71 @@ int fame_match_reads_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* 
         if (rc != FAME_OK) return rc;
         if (n && d_valid) HIPC(hipMemcpyAsync(d_valid, c->b_valid.p, n, hipMemcpyDeviceToDevice, st));
         if (n && d_mat) HIPC(hipMemcpyAsync(d_mat, c->b_mat.p, 8ull * n, hipMemcpyDeviceToDevice, st));
+        HIPC(hipStreamSynchronize(st));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_set_pair_dist(fame_ctx* c, uint32_t minpd, uint32_t maxpd)
+{
+    if (!c) return FAME_ERR_ARG;
+    if (c->winlen == 0 || maxpd / c->winlen + 1 > 31 || minpd > maxpd) {
+        set_err(c, "MAXPDIST must be below 30 windows and not below MINPDIST");
+        return FAME_ERR_ARG;
+    }
+    c->minpd = minpd;
+    c->maxpd = maxpd;
+    return FAME_OK;
+}
+
+int fame_match_pairs(fame_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, const uint32_t* lens1,
+                     const uint8_t* seqs2, const uint64_t* offs2, const uint32_t* lens2, uint32_t n,
+                     int get_stranded, uint64_t counters[5], uint8_t* valid1, uint64_t* mat1, uint8_t* valid2,
+                     uint64_t* mat2)
+{
+    if (!c || !counters || (n && (!seqs1 || !offs1 || !lens1 || !seqs2 || !offs2 || !lens2))) return FAME_ERR_ARG;
+    if (!c->d_tab.p) {
+        set_err(c, "context has no index");
+        return FAME_ERR_ARG;
+    }
+    if ((uint32_t)std::ceil((float)c->readlen * 0.75) < 32) {
+        set_err(c, "paired matching needs ceil(0.75 READLEN) >= 32");
+        return FAME_ERR_FORMAT;
+    }
+    if (n > 0x7FFFFFFFu) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        uint64_t t1 = 0, t2 = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (lens1[i] > FAME_MAX_READ || lens2[i] > FAME_MAX_READ) {
+                set_err(c, "mate of pair " + std::to_string(i) + " is longer than FAME_MAX_READ");
+                return (int)FAME_ERR_ARG;
+            }
+            t1 = std::max<uint64_t>(t1, offs1[i] + lens1[i]);
+            t2 = std::max<uint64_t>(t2, offs2[i] + lens2[i]);
+        }
+        hipStream_t st = c->stream;
+        const uint64_t total = t1 + t2;
+        uint8_t* d_s = grow<uint8_t>(c->b_seqs, total + 64);
+        uint64_t* d_o = grow<uint64_t>(c->b_offs, 2ull * n + 1);
+        uint32_t* d_l = grow<uint32_t>(c->b_lens, 2ull * n + 1);
+        std::vector<uint64_t> o(2ull * n);
+        for (uint32_t i = 0; i < n; ++i) {
+            o[i] = offs1[i];
+            o[n + i] = offs2[i] + t1;
+        }
+        if (t1) HIPC(hipMemcpyAsync(d_s, seqs1, t1, hipMemcpyHostToDevice, st));
+        if (t2) HIPC(hipMemcpyAsync(d_s + t1, seqs2, t2, hipMemcpyHostToDevice, st));
+        if (n) {
+            HIPC(hipMemcpyAsync(d_o, o.data(), 16ull * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_l, lens1, 4ull * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_l + n, lens2, 4ull * n, hipMemcpyHostToDevice, st));
+        }
+        int rc = run_pairs(c, d_s, d_o, d_l, n, total, get_stranded, counters, st);
+        if (rc != FAME_OK) return rc;
+        if (n && valid1) HIPC(hipMemcpyAsync(valid1, c->b_valid.p, n, hipMemcpyDeviceToHost, st));
+        if (n && valid2) HIPC(hipMemcpyAsync(valid2, c->b_valid.as<uint8_t>() + n, n, hipMemcpyDeviceToHost, st));
+        if (n && mat1) HIPC(hipMemcpyAsync(mat1, c->b_mat.p, 8ull * n, hipMemcpyDeviceToHost, st));
+        if (n && mat2) HIPC(hipMemcpyAsync(mat2, c->b_mat.as<uint64_t>() + n, 8ull * n, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });

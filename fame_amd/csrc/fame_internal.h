@@ -29,8 +29,12 @@ enum : uint32_t {
 
 // The index as resident in HBM.
 //   tab      tabIndex, u32 when every prefix fits (nkmer < 2^32), else u64
-//   kent     one u32 per k-mer table entry: metaId << 1 | forward-strand bit
-//            (KMER_S::getMetaCpG + strandTable, structs.h:118-151)
+//   kent     one u32 per k-mer table entry: isStart << 31 | metaId << 1 |
+//            forward-strand bit (KMER_S::getMetaCpG / isStartCpG +
+//            strandTable, structs.h:118-151); metaId < 2^30
+//   ktm      KMER_S::tmask of every entry (read by the paired-end path)
+//   filt32   the filteredKmers values below 2^32, sorted (the paired path
+//            looks up a 32-bit k-mer code there, ReadQueue.cpp:2527)
 //   win      metaWindow as {startPos, chrom, startInd, endInd}
 //   seq      all chromosomes back to back, coff / clen per chromosome
 //   cpg_pos  cpgTable[i].pos, cpg_chrom cpgTable[i].chrom
@@ -40,6 +44,9 @@ struct DevIndex {
     uint64_t htab, hmask;
     int hpow2;
     const uint32_t* kent;
+    const uint32_t* ktm;
+    const uint32_t* filt32;
+    uint32_t nfilt32;
     const uint4* win;
     const uint8_t* seq;
     const uint64_t* coff;
@@ -103,6 +110,47 @@ struct BatchDev {
     unsigned long long* stats;
 };
 
+// Paired-end batch (matchPairedReads ReadQueue.cpp:935-1655).  The batch's
+// 2n mates are laid out as 2n reads of a BatchDev: read i is mate 1 of pair
+// i, read n + i its mate 2.  An item t = 2i + c is combination c of pair i:
+// c = 0 matches (mate 1, revcomp mate 2), c = 1 (revcomp mate 1, mate 2);
+// the first of the two is "the first read" of FAME's paired seeding.
+struct PeItem {
+    uint32_t k, v, j; // window, value (c0 | c1 << 8 | flags), verify slot
+};
+
+struct PairDev {
+    uint32_t n;      // pairs
+    uint32_t combos; // bit c: combination c is queried
+    uint32_t cw;     // contextWLen: neighbouring windows that pair a mate
+    uint32_t maxpd;  // MAXPDIST
+    // per item: the item's slice of the map pool (both strand maps, each in
+    // hopscotch iteration order) and its first-read match count
+    uint32_t* map_off;
+    uint32_t* map_nf;
+    uint32_t* map_nr;
+    uint32_t* m1_n;
+    uint32_t* map_n; // pool fill (device counter)
+    uint32_t map_cap;
+    PeItem* map;     // [map_cap] maps as built by the seed stage
+    PeItem* srt;     // [map_cap] the same slices sorted by a count
+    uint8_t* acc1[FAME_NPOOL]; // per first-read verify slot: 1 walked | 2 first match dropped
+    uint8_t* acc2[FAME_NPOOL]; // per second-read verify slot
+    uint64_t* counters;        // succ, nonunique, unsucc, succPaired, tooShort
+    // seed scratch: one slice per block; items that outgrow it are listed
+    // and re-run with larger slices
+    uint8_t* scratch;
+    size_t stride;
+    uint32_t nslices;
+    uint32_t cap_e, hsize, tb_cap;
+    const uint32_t* list; // NULL: every item; else the items to run
+    uint32_t list_n;
+    uint32_t* big_n;
+    uint32_t* big_item;
+    uint32_t* big_e;
+    uint32_t big_cap;
+};
+
 struct HeavyScratch {
     uint8_t* base;
     size_t stride;
@@ -123,6 +171,12 @@ void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_s
 void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
                  uint32_t grid_threads, hipStream_t st);
 uint32_t meth_dp_lanes();
+void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, uint32_t minlen, hipStream_t st);
+void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, uint32_t grid, hipStream_t st);
+void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st);
+void launch_pe_final(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd,
+                     int get_stranded, hipStream_t st);
+void launch_pack_tmask(const uint32_t* raw, uint64_t n, uint32_t* out, hipStream_t st);
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);

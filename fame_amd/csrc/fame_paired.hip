@@ -1,0 +1,1131 @@
+// HIP kernels of the paired-end matching path (gfx950):
+// ReadQueue::matchPairedReads ReadQueue.cpp:935-1655.
+//
+// A batch of n pairs is 2n reads of a BatchDev (mate 1 of pair i is read i,
+// mate 2 is read n + i; k_prep in paired mode validates them and writes the
+// reverse complements and reduced-letter rows).  Combination c of pair i is
+// item t = 2i + c: c = 0 seeds (mate 1, revcomp mate 2), c = 1 (revcomp
+// mate 1, mate 2) (ReadQueue.cpp:1092-1300).  Per batch:
+//   k_pe_seed    one wavefront per item: getSeedRefsFirstRead (:2468-2794)
+//                on the first read (tmask / filteredKmers filter, u8 counts,
+//                the count-sized map's iteration order), getSeedRefsSecondRead
+//                (:2801-3164) on the second read against the first read's
+//                windows (neighbour rule, inserts, flag2, hasCpG), then the
+//                iteration order of the two 20-bucket maps; queues the first
+//                read's verify jobs
+//   k_verify     (fame_device.hip) the first reads over their windows
+//   k_pe_walk1   one lane per item: saQuerySeedSetRefFirst (:2167-2243) -
+//                std::sort by c0, the walk with the KMERDIST cut, flag3 on
+//                matched windows and their neighbours - then the c1 order of
+//                saQuerySeedSetRefSecond (:2249-2323) and the second reads'
+//                verify jobs
+//   k_verify     the second reads
+//   k_pe_final   one lane per pair: the second walk, every (mate 1, mate 2)
+//                match combination through extractPairedMatch (:3255-3275),
+//                the best pair, counters, and the methylation queues
+//   k_meth_*     (fame_device.hip) computeMethLvl for both mates
+//
+// std::sort is libstdc++'s introsort; the order it leaves among equal counts
+// decides which windows a walk reaches, so pe_sort repeats its steps.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fame_internal.h"
+#include "fame_dev_common.h"
+
+namespace fame {
+
+constexpr uint32_t KMERDIST = 10;      // CONST.h:88
+constexpr uint32_t PF2 = 1u << 16;     // map value tuple<u8 c0, u8 c1, bool, bool>: the two flags
+constexpr uint32_t PF3 = 1u << 17;
+constexpr uint32_t PE_MAPCAP = 256;    // windows per strand map of one item (LDS)
+constexpr uint32_t PE_MH = 1024;       // lookup slots for both maps
+constexpr uint32_t PE_NONE = 0xFFFFFFFFu;
+constexpr int PE_KPL = (FAME_DEV_MAX_READ - (int)KMERLEN + 1 + 63) / 64; // k-mers per lane
+
+__device__ __forceinline__ uint32_t pe_c0(uint32_t v) { return v & 0xFFu; }
+__device__ __forceinline__ uint32_t pe_c1(uint32_t v) { return (v >> 8) & 0xFFu; }
+
+// per-block LDS: the item's two small maps (keys, values, a lookup table)
+// and the letters of the read being seeded
+struct PeLds {
+    uint32_t mkey[2][PE_MAPCAP];
+    uint32_t mval[2][PE_MAPCAP];
+    uint32_t hk[PE_MH]; // key << 1 | strand, EMPTY when free
+    uint32_t hv[PE_MH]; // index in that strand's arrays
+    uint32_t nmap[2];
+    uint32_t cbits[9];  // bit p: letter p of the read is C
+    uint32_t kbits[17]; // 2-bit letter codes (C, T 3; G 2; others 0), 16 per word
+    alignas(8) uint8_t code[256];
+};
+
+// per-block global scratch of the seed stage
+template <typename TabT>
+struct PeWork {
+    TabT* klo;
+    uint32_t* klen;
+    uint32_t* koff;
+    uint32_t* kcm;  // C mask of each k-mer, first letter in the top bit
+    uint8_t* kfl;   // k-mer skipped by the filteredKmers lookup
+    uint32_t* ent;  // [cap_e] kent of each bucket entry
+    uint8_t* ekm;   // [cap_e] its k-mer
+    uint32_t* eaux; // [cap_e] bit 0 kept; first / count after counting
+    uint32_t cap_e;
+    uint32_t* hkey;
+    uint32_t* hfirst;
+    uint32_t* hcnt;
+    uint32_t hsize;
+    uint32_t* ins_key; // [2][cap_e]
+    uint16_t* ins_cnt; // [2][cap_e]
+    uint32_t* tb;
+    uint32_t tb_cap;
+    uint32_t* tmp;
+    uint32_t* over;
+};
+
+__device__ __forceinline__ uint32_t pe_slot(uint32_t k2) { return (k2 * 2654435761u) >> 22; } // 10 bits
+
+// index of (key, strand) in the LDS maps, or PE_NONE
+__device__ __forceinline__ uint32_t pe_find(const PeLds& m, uint32_t key, uint32_t s)
+{
+    if (key >= 0x40000000u) return PE_NONE; // metaId - u below 0 wraps
+    const uint32_t k2 = (key << 1) | s;
+    uint32_t h = pe_slot(k2);
+    for (uint32_t probe = 0; probe < PE_MH; ++probe) {
+        const uint32_t x = m.hk[h];
+        if (x == k2) return m.hv[h];
+        if (x == EMPTY) return PE_NONE;
+        h = (h + 1) & (PE_MH - 1);
+    }
+    return PE_NONE;
+}
+
+// one lane inserts a key known to be absent
+__device__ __forceinline__ void pe_put(PeLds& m, uint32_t key, uint32_t s, uint32_t idx)
+{
+    const uint32_t k2 = (key << 1) | s;
+    uint32_t h = pe_slot(k2);
+    while (m.hk[h] != EMPTY) h = (h + 1) & (PE_MH - 1);
+    m.hk[h] = k2;
+    m.hv[h] = idx;
+}
+
+__device__ __forceinline__ bool pe_filtered(const DevIndex& ix, uint32_t code)
+{
+    uint32_t lo = 0, hi = ix.nfilt32;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t v = ix.filt32[mid];
+        if (v == code) return true;
+        if (v < code) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
+
+// The bucket entries of one (read, orientation) with the paired filters
+// (ReadQueue.cpp:2556-2600 first read, :2646-2700 second read): entries of
+// k-mers found in filteredKmers (k-mer 0 excepted) and entries whose tmask
+// meets a read C under SEEDBITS are dropped; of consecutive entries of one
+// bucket that remain, those repeating the previous one's (metaId, strand)
+// (and isStart, for the first read) are dropped.  eaux[j] bit 0 = kept.
+// Returns 1 when the entries do not fit the scratch (E_out set).
+template <typename TabT>
+__device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, const SeedShared& sh, PeLds& m,
+                          const PeWork<TabT>& w, bool first, uint32_t& E_out, uint32_t& bc_out, uint32_t& L_out)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t cwd = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
+    const uint32_t L = (uint32_t)__shfl(cwd, 63, 64) >> 24;
+    L_out = L;
+    ((uint32_t*)m.code)[lane] = cwd;
+    if (lane < 9) m.cbits[lane] = 0;
+    if (lane < 17) m.kbits[lane] = 0;
+    wave_sync();
+    {
+        const uint32_t r = ro >> 1;
+        const uint8_t* pat = ((ro & 1) ? bd.rev : bd.seqs) + bd.offs[r];
+        for (uint32_t p = lane; p < L; p += 64) {
+            const uint8_t c = pat[p];
+            if (c == 'C') atomicOr(&m.cbits[p >> 5], 1u << (p & 31));
+            const uint32_t k2 = (c == 'C' || c == 'T') ? 3u : (c == 'G' ? 2u : 0u);
+            if (k2) atomicOr(&m.kbits[p >> 4], k2 << (2 * (p & 15)));
+        }
+    }
+    wave_sync();
+    const uint32_t K = L - KMERLEN + 1;
+    const TabT* tab = (const TabT*)ix.tab;
+    TabT lo[PE_KPL], hi[PE_KPL];
+#pragma unroll
+    for (int q = 0; q < PE_KPL; ++q) {
+        const uint32_t k = lane + 64 * q;
+        lo[q] = hi[q] = 0;
+        if (k < K) {
+            uint64_t h = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < KMERLEN; ++i)
+                if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][m.code[k + i]];
+            const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
+            lo[q] = tab[key];
+            hi[q] = tab[key + 1];
+        }
+    }
+    uint32_t bsum = 0;
+#pragma unroll
+    for (int q = 0; q < PE_KPL; ++q) {
+        const uint32_t k = lane + 64 * q;
+        if (k < K) {
+            w.klo[k] = lo[q];
+            const uint64_t len = (uint64_t)(hi[q] - lo[q]);
+            w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+            bsum += (uint32_t)len;
+            // cMask: the k-mer's C letters, its first letter in the top bit
+            const uint32_t wi = k >> 5, sb = k & 31;
+            uint32_t win = m.cbits[wi] >> sb;
+            if (sb) win |= m.cbits[wi + 1] << (32 - sb);
+            w.kcm[k] = __brev(win);
+            // kSeq: the u32 two-bit code of the k-mer's last 16 letters, the
+            // earliest in the top bits (:2520-2530)
+            bool f = false;
+            if (k > 0 && ix.nfilt32) {
+                const uint32_t s = 2 * (k + 16), wj = s >> 5, sc = s & 31;
+                uint32_t x = m.kbits[wj] >> sc;
+                if (sc) x |= m.kbits[wj + 1] << (32 - sc);
+                uint32_t y = __brev(x);
+                y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
+                f = pe_filtered(ix, y);
+            }
+            w.kfl[k] = f ? 1 : 0;
+        }
+    }
+    bsum = wave_sum_u32(bsum);
+    bc_out = bsum / 5;
+    uint64_t carry = 0;
+    bool big = false;
+    for (uint32_t k0 = 0; k0 < K; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint32_t v = k < K ? w.klen[k] : 0;
+        if (v > 0xFFFFFFu) {
+            v = 0xFFFFFFu;
+            big = true;
+        }
+        const uint32_t inc = wave_iscan_u32(v, lane);
+        const uint32_t tot = __shfl(inc, 63, 64);
+        if (k < K) w.koff[k] = (uint32_t)(carry + inc - v);
+        carry += tot;
+        if (carry > 0x7FFFFFFFull) big = true;
+    }
+    big = __ballot(big) != 0;
+    E_out = big ? 0xFFFFFFFFu : (uint32_t)carry;
+    if (big || carry > w.cap_e) return 1;
+    const uint32_t E = (uint32_t)carry;
+    wave_sync();
+    for (uint32_t j = lane; j < E; j += 64) {
+        uint32_t a = 0, b = K;
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (w.koff[mid] <= j) a = mid;
+            else b = mid;
+        }
+        uint32_t k = a;
+        while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k;
+        const uint64_t g = (uint64_t)w.klo[k] + (j - w.koff[k]);
+        const uint32_t e = ix.kent[g];
+        const bool filt = w.kfl[k] || (w.kcm[k] & SEEDBITS & ix.ktm[g]) != 0;
+        w.ent[j] = first ? e : (e & 0x7FFFFFFFu);
+        w.ekm[j] = (uint8_t)k;
+        w.eaux[j] = filt ? 0u : 1u;
+    }
+    wave_sync();
+    // previous unfiltered entry (exclusive max-scan); dropped when in the
+    // same bucket with the same packed (isStart,) metaId, strand
+    int pc = -1;
+    for (uint32_t j0 = 0; j0 < E; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool un = j < E && (w.eaux[j] & 1);
+        int inc = un ? (int)j : -1;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc = max(inc, t);
+        }
+        int ex = __shfl_up(inc, 1, 64);
+        if (lane == 0) ex = -1;
+        ex = max(ex, pc);
+        pc = max(pc, __shfl(inc, 63, 64));
+        if (un && ex >= 0) {
+            const uint32_t k = w.ekm[j];
+            if ((uint32_t)ex >= w.koff[k] && w.ent[ex] == w.ent[j]) w.eaux[j] = 0;
+        }
+    }
+    wave_sync();
+    return 0;
+}
+
+// getSeedRefsFirstRead's counting (:2556-2640) and the copy of the windows
+// with c0 >= QTHRESH, in the iteration order of the map sized by the bucket
+// count, into the LDS maps (:2760-2790).  Returns 1 on a capacity overflow
+// (retry with larger scratch), 2 when the LDS maps are too small.
+template <typename TabT>
+__device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L, uint32_t bucketCount)
+{
+    const int lane = threadIdx.x & 63;
+    {
+        const uint32_t hm = w.hsize - 1;
+        const int hshift = __clz(w.hsize) + 1;
+        for (uint32_t i = lane; i < w.hsize; i += 64) {
+            w.hkey[i] = EMPTY;
+            w.hfirst[i] = 0xFFFFFFFFu;
+            w.hcnt[i] = 0;
+        }
+        wave_sync();
+        for (uint32_t j = lane; j < E; j += 64) {
+            if (!(w.eaux[j] & 1)) continue;
+            const uint32_t key = w.ent[j] & 0x7FFFFFFFu; // metaId << 1 | strand
+            uint32_t slot = (key * 2654435761u) >> hshift;
+            for (;;) {
+                const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
+                if (old == EMPTY || old == key) break;
+                slot = (slot + 1) & hm;
+            }
+            atomicMin(&w.hfirst[slot], j);
+            atomicAdd(&w.hcnt[slot], 1u);
+            w.eaux[j] = 1u | (slot << 8);
+        }
+        wave_sync();
+        for (uint32_t j = lane; j < E; j += 64) {
+            const uint32_t a = w.eaux[j];
+            if (!(a & 1)) continue;
+            const uint32_t slot = a >> 8;
+            const bool fst = w.hfirst[slot] == j;
+            // the tuple's c0 is a u8: counts wrap modulo 256
+            w.eaux[j] = 1u | (fst ? 2u : 0u) | ((w.hcnt[slot] & 0xFFu) << 16);
+        }
+    }
+    wave_sync();
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    uint32_t nins[2] = {0, 0}, nq[2] = {0, 0};
+    const uint64_t below = lanes_below(lane);
+    for (uint32_t j0 = 0; j0 < E; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        bool t0 = false, t1 = false;
+        uint32_t key = 0, cnt = 0;
+        if (j < E) {
+            const uint32_t a = w.eaux[j];
+            const uint32_t km = w.ekm[j];
+            const bool ins = km == 0 || (km - 1) < maxQPos;
+            if ((a & 2) && ins) {
+                const uint32_t e = w.ent[j];
+                key = (e >> 1) & 0x3FFFFFFFu;
+                cnt = a >> 16;
+                t0 = (e & 1) != 0;
+                t1 = !t0;
+            }
+        }
+        const uint64_t b0 = __ballot(t0), b1 = __ballot(t1);
+        if (t0) {
+            const uint32_t p = nins[0] + __popcll(b0 & below);
+            if (p < w.cap_e) {
+                w.ins_key[p] = key;
+                w.ins_cnt[p] = (uint16_t)cnt;
+            }
+        }
+        if (t1) {
+            const uint32_t p = nins[1] + __popcll(b1 & below);
+            if (p < w.cap_e) {
+                w.ins_key[w.cap_e + p] = key;
+                w.ins_cnt[w.cap_e + p] = (uint16_t)cnt;
+            }
+        }
+        nins[0] += __popcll(b0);
+        nins[1] += __popcll(b1);
+        const bool q0 = t0 && cnt >= QTHRESH, q1 = t1 && cnt >= QTHRESH;
+        const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
+        if (q0 && nq[0] == 0 && (c0 & below) == 0) {
+            m.mkey[0][0] = key;
+            m.mval[0][0] = cnt;
+        }
+        if (q1 && nq[1] == 0 && (c1 & below) == 0) {
+            m.mkey[1][0] = key;
+            m.mval[1][0] = cnt;
+        }
+        nq[0] += __popcll(c0);
+        nq[1] += __popcll(c1);
+    }
+    if (nins[0] > w.cap_e || nins[1] > w.cap_e) return 1;
+    wave_sync();
+#pragma unroll
+    for (int mp = 0; mp < 2; ++mp) {
+        if (nq[mp] > PE_MAPCAP) return 2;
+        if (nq[mp] <= 1) {
+            if (lane == 0) m.nmap[mp] = nq[mp];
+            continue;
+        }
+        Hs<uint32_t> h;
+        h.tb = w.tb;
+        h.tb_cap = w.tb_cap;
+        h.over = w.over;
+        h.over_cap = w.cap_e;
+        h.tmp = w.tmp;
+        h.tmp_cap = w.cap_e;
+        h.key = w.ins_key + mp * w.cap_e;
+        h.nelem = h.nover = 0;
+        if (!hs_setup(h, bucketCount, lane)) return 1;
+        for (uint32_t i = 0; i < nins[mp]; ++i)
+            if (hs_insert(h, i, lane) != HS_OK) return 1;
+        const uint16_t* cnt = w.ins_cnt + mp * w.cap_e;
+        uint32_t nc = 0;
+        for (uint64_t b0 = 0; b0 < h.arr + h.nover; b0 += 64) {
+            const uint64_t b = b0 + lane;
+            uint32_t i = Hs<uint32_t>::NONE;
+            if (b < h.arr) i = h.tb[b];
+            else if (b < h.arr + h.nover) i = h.over[b - h.arr];
+            const bool keep = i != Hs<uint32_t>::NONE && cnt[i] >= QTHRESH;
+            const uint64_t bal = __ballot(keep);
+            if (keep) {
+                const uint32_t p = nc + __popcll(bal & below);
+                m.mkey[mp][p] = h.key[i];
+                m.mval[mp][p] = cnt[i];
+            }
+            nc += __popcll(bal);
+        }
+        if (lane == 0) m.nmap[mp] = nc;
+        wave_sync();
+    }
+    wave_sync();
+    return 0;
+}
+
+// getSeedRefsSecondRead's counting (:2646-3120): kept entries in index
+// order; a window is credited when it or a window within contextWLen is in
+// its strand's map, which inserts it (operator[]), so later entries see it.
+// Entries from k-mers past maxQPos only credit windows already present.
+// The order matters (inserts enable later credits), so the wave walks the
+// entries one at a time; each step's neighbour lookups run on 2cw+1 lanes.
+// Returns 2 when a map outgrows the LDS arrays.
+template <typename TabT>
+__device__ int pe_second(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L, int cw)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    const int span = 2 * cw + 1;
+    for (uint32_t j0 = 0; j0 < E; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        uint32_t e = 0, km = 0;
+        bool kept = false;
+        if (j < E) {
+            kept = (w.eaux[j] & 1) != 0;
+            e = w.ent[j];
+            km = w.ekm[j];
+        }
+        uint64_t kb = __ballot(kept);
+        while (kb) {
+            const int src = __ffsll((long long)kb) - 1;
+            kb &= kb - 1;
+            const uint32_t ee = __shfl(e, src, 64), kk = __shfl(km, src, 64);
+            const uint32_t key = (ee >> 1) & 0x3FFFFFFFu, s = (ee & 1) ? 0u : 1u;
+            const bool elig = kk == 0 || (kk - 1) < maxQPos;
+            uint32_t idx = PE_NONE;
+            bool found = false;
+            if (lane < span) {
+                idx = pe_find(m, key + (uint32_t)(lane - cw), s);
+                found = idx != PE_NONE;
+            }
+            const uint64_t fb = __ballot(found);
+            uint32_t self = __shfl(idx, cw, 64);
+            if (elig && fb != 0 && self == PE_NONE) {
+                const uint32_t n0 = m.nmap[s];
+                if (n0 >= PE_MAPCAP) return 2;
+                if (lane == 0) {
+                    m.mkey[s][n0] = key;
+                    m.mval[s][n0] = 0;
+                    pe_put(m, key, s, n0);
+                    m.nmap[s] = n0 + 1;
+                }
+                self = n0;
+            }
+            if (self != PE_NONE && lane == 0) {
+                const uint32_t v = m.mval[s][self];
+                m.mval[s][self] = (v & ~0xFF00u) | ((v + 0x100u) & 0xFF00u); // u8 c1
+            }
+            wave_sync();
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_pe_seed
+// ---------------------------------------------------------------------------
+
+template <typename TabT>
+__global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDev pd)
+{
+    __shared__ SeedShared sh;
+    __shared__ PeLds m;
+    const int lane = threadIdx.x;
+    seed_table_init(sh);
+    uint8_t* base = pd.scratch + (size_t)blockIdx.x * pd.stride;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t* p = base + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    PeWork<TabT> w;
+    w.klo = (TabT*)take(8 * 256);
+    w.klen = (uint32_t*)take(4 * 256);
+    w.koff = (uint32_t*)take(4 * 256);
+    w.kcm = (uint32_t*)take(4 * 256);
+    w.kfl = (uint8_t*)take(256);
+    w.cap_e = pd.cap_e;
+    w.ent = (uint32_t*)take(4ull * pd.cap_e);
+    w.ekm = (uint8_t*)take(pd.cap_e);
+    w.eaux = (uint32_t*)take(4ull * pd.cap_e);
+    w.hsize = pd.hsize;
+    w.hkey = (uint32_t*)take(4ull * pd.hsize);
+    w.hfirst = (uint32_t*)take(4ull * pd.hsize);
+    w.hcnt = (uint32_t*)take(4ull * pd.hsize);
+    w.ins_key = (uint32_t*)take(8ull * pd.cap_e);
+    w.ins_cnt = (uint16_t*)take(4ull * pd.cap_e);
+    w.tb_cap = pd.tb_cap;
+    w.tb = (uint32_t*)take(4ull * pd.tb_cap);
+    w.tmp = (uint32_t*)take(4ull * pd.cap_e);
+    w.over = (uint32_t*)take(4ull * pd.cap_e);
+    PoolChunk pcA = {0, 0, 0};
+    const uint64_t below = lanes_below(lane);
+    const uint32_t nitems = pd.list ? pd.list_n : 2 * pd.n;
+    for (uint32_t ti = blockIdx.x; ti < nitems; ti += gridDim.x) {
+        const uint32_t t = pd.list ? pd.list[ti] : ti;
+        const uint32_t i = t >> 1, c = t & 1;
+        uint32_t nf = 0, nr = 0, off = 0;
+        // pairs with an invalid mate are counted by k_pe_final only
+        const bool on = ((pd.combos >> c) & 1) && bd.pre[i] == 0 && bd.pre[pd.n + i] == 0;
+        int rc = 0;
+        uint32_t EA = 0, EB = 0;
+        if (on) {
+            const uint32_t roA = 2 * i + c;
+            const uint32_t roB = 2 * (pd.n + i) + (1 - c);
+            uint32_t bc, LA, LB, bcB;
+            rc = pe_entries<TabT>(ix, bd, roA, sh, m, w, true, EA, bc, LA);
+            if (rc == 0) rc = pe_first<TabT>(w, m, EA, LA, bc);
+            if (rc == 0 && (m.nmap[0] | m.nmap[1])) {
+                // the lookup table over the first read's windows
+                for (uint32_t x = lane; x < PE_MH; x += 64) m.hk[x] = EMPTY;
+                wave_sync();
+                if (lane == 0)
+                    for (uint32_t s = 0; s < 2; ++s)
+                        for (uint32_t x = 0; x < m.nmap[s]; ++x) pe_put(m, m.mkey[s][x], s, x);
+                wave_sync();
+                rc = pe_entries<TabT>(ix, bd, roB, sh, m, w, false, EB, bcB, LB);
+                if (rc == 0) rc = pe_second<TabT>(w, m, EB, LB, (int)pd.cw);
+                if (rc == 0) {
+                    // flag2: windows with c1 >= QTHRESH and their present
+                    // neighbours within contextWLen (:3125-3150); hasCpG
+                    const uint32_t n0 = m.nmap[0], ntot = n0 + m.nmap[1];
+                    for (uint32_t x = lane; x < ntot; x += 64) {
+                        const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
+                        if (pe_c1(m.mval[s][y]) < QTHRESH) continue;
+                        const uint32_t key = m.mkey[s][y];
+                        atomicOr(&m.mval[s][y], PF2);
+                        for (int u = 1; u <= (int)pd.cw; ++u) {
+                            uint32_t q = pe_find(m, key + (uint32_t)u, s);
+                            if (q != PE_NONE) atomicOr(&m.mval[s][q], PF2);
+                            q = pe_find(m, key - (uint32_t)u, s);
+                            if (q != PE_NONE) atomicOr(&m.mval[s][q], PF2);
+                        }
+                    }
+                    wave_sync();
+                    bool cpg = false;
+                    uint32_t nj = 0;
+                    for (uint32_t x = lane; x < ntot; x += 64) {
+                        const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
+                        const uint32_t v = m.mval[s][y];
+                        if ((v & PF2) && ix.win[m.mkey[s][y]].z != CPGDUMMY) cpg = true;
+                        if ((v & PF2) && pe_c0(v) >= QTHRESH) ++nj;
+                    }
+                    const bool hasCpG = __ballot(cpg) != 0;
+                    nj = wave_sum_u32(nj);
+                    if (hasCpG) {
+                        nf = m.nmap[0];
+                        nr = m.nmap[1];
+                        if (lane == 0) off = atomicAdd(pd.map_n, nf + nr);
+                        off = __shfl(off, 0, 64);
+                        if (off + nf + nr > pd.map_cap) {
+                            if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
+                            nf = nr = 0;
+                        }
+                    }
+                    if (nf + nr) {
+                        const uint32_t pool = (LA + 63) / 64 - 1;
+                        uint32_t jb = nj ? pool_take(bd, pcA, pool, nj, false, lane) : 0;
+                        if (jb == ~0u) {
+                            if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
+                            nf = nr = 0;
+                        } else {
+                            uint32_t* jm = pc_meta(bd, pool);
+                            uint32_t* jr = bd.job_ro[0];
+#pragma unroll
+                            for (uint32_t q = 1; q < FAME_NPOOL; ++q) jr = pool == q ? bd.job_ro[q] : jr;
+                            // iteration order of the two 20-bucket maps (:2760-2790
+                            // build them with 20 buckets, :2801-3164 insert into them)
+                            uint32_t pos = off;
+                            for (uint32_t s = 0; s < 2; ++s) {
+                                Hs<uint32_t> h;
+                                h.tb = w.tb;
+                                h.tb_cap = w.tb_cap;
+                                h.over = w.over;
+                                h.over_cap = w.cap_e;
+                                h.tmp = w.tmp;
+                                h.tmp_cap = w.cap_e;
+                                h.key = m.mkey[s];
+                                h.nelem = h.nover = 0;
+                                const uint32_t ns = m.nmap[s];
+                                if (!hs_setup(h, 20, lane)) {
+                                    rc = 2;
+                                    break;
+                                }
+                                for (uint32_t x = 0; x < ns && rc == 0; ++x)
+                                    if (hs_insert(h, x, lane) != HS_OK) rc = 2;
+                                if (rc) break;
+                                for (uint64_t b0 = 0; b0 < h.arr + h.nover; b0 += 64) {
+                                    const uint64_t b = b0 + lane;
+                                    uint32_t x = Hs<uint32_t>::NONE;
+                                    if (b < h.arr) x = h.tb[b];
+                                    else if (b < h.arr + h.nover) x = h.over[b - h.arr];
+                                    const bool occ = x != Hs<uint32_t>::NONE;
+                                    const uint32_t v = occ ? m.mval[s][x] : 0u;
+                                    const bool job = occ && (v & PF2) && pe_c0(v) >= QTHRESH;
+                                    const uint64_t bo = __ballot(occ), bj = __ballot(job);
+                                    if (occ) {
+                                        PeItem it;
+                                        it.k = m.mkey[s][x];
+                                        it.v = v;
+                                        it.j = PE_NONE;
+                                        if (job) {
+                                            it.j = jb + __popcll(bj & below);
+                                            jm[it.j] = it.k | (s ? 0x80000000u : 0u);
+                                            jr[it.j] = roA;
+                                        }
+                                        pd.map[pos + __popcll(bo & below)] = it;
+                                    }
+                                    pos += __popcll(bo);
+                                    jb += __popcll(bj);
+                                }
+                                wave_sync();
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (rc == 1) {
+            // entries beyond this scratch: listed for the larger run
+            if (lane == 0) {
+                if (pd.list) {
+                    atomicOr(bd.flags, FLAG_CAPACITY);
+                } else {
+                    const uint32_t b = atomicAdd(pd.big_n, 1u);
+                    if (b < pd.big_cap) {
+                        pd.big_item[b] = t;
+                        pd.big_e[b] = max(EA, EB);
+                    } else {
+                        atomicOr(bd.flags, FLAG_HEAVY_FULL);
+                    }
+                }
+            }
+            nf = nr = 0;
+        } else if (rc == 2) {
+            if (lane == 0) atomicOr(bd.flags, FLAG_CAPACITY);
+            nf = nr = 0;
+        }
+        if (lane == 0) {
+            pd.map_off[t] = off;
+            pd.map_nf[t] = nf;
+            pd.map_nr[t] = nr;
+        }
+        wave_sync();
+    }
+    if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
+}
+
+// ---------------------------------------------------------------------------
+// std::sort (libstdc++ bits/stl_algo.h, stl_heap.h): introsort with
+// median-of-three pivots and unguarded partitions, heap sort past a depth of
+// 2*floor(log2 n), then insertion sort (the first 16 guarded); comparator
+// "count greater" (ReadQueue.h:339-351).  Sub-ranges are independent, so
+// the recursion on the right part runs from an explicit stack.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool pe_less(const PeItem& a, const PeItem& b, int sec)
+{
+    return sec ? pe_c1(a.v) > pe_c1(b.v) : pe_c0(a.v) > pe_c0(b.v);
+}
+
+__device__ __forceinline__ void pe_swap(PeItem* a, int i, int j)
+{
+    const PeItem t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+
+__device__ void pe_push_heap(PeItem* a, int first, int hole, int top, PeItem val, int sec)
+{
+    int parent = (hole - 1) / 2;
+    while (hole > top && pe_less(a[first + parent], val, sec)) {
+        a[first + hole] = a[first + parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    a[first + hole] = val;
+}
+
+__device__ void pe_adjust_heap(PeItem* a, int first, int hole, int len, PeItem val, int sec)
+{
+    const int top = hole;
+    int child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (pe_less(a[first + child], a[first + child - 1], sec)) child--;
+        a[first + hole] = a[first + child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        a[first + hole] = a[first + child - 1];
+        hole = child - 1;
+    }
+    pe_push_heap(a, first, hole, top, val, sec);
+}
+
+__device__ void pe_heap_sort(PeItem* a, int first, int last, int sec)
+{
+    const int len = last - first;
+    if (len >= 2) {
+        for (int parent = (len - 2) / 2;; --parent) {
+            pe_adjust_heap(a, first, parent, len, a[first + parent], sec);
+            if (parent == 0) break;
+        }
+    }
+    while (last - first > 1) {
+        --last;
+        const PeItem v = a[last];
+        a[last] = a[first];
+        pe_adjust_heap(a, first, 0, last - first, v, sec);
+    }
+}
+
+__device__ void pe_median_to_first(PeItem* a, int result, int x, int y, int z, int sec)
+{
+    if (pe_less(a[x], a[y], sec)) {
+        if (pe_less(a[y], a[z], sec)) pe_swap(a, result, y);
+        else if (pe_less(a[x], a[z], sec)) pe_swap(a, result, z);
+        else pe_swap(a, result, x);
+    } else if (pe_less(a[x], a[z], sec)) {
+        pe_swap(a, result, x);
+    } else if (pe_less(a[y], a[z], sec)) {
+        pe_swap(a, result, z);
+    } else {
+        pe_swap(a, result, y);
+    }
+}
+
+__device__ void pe_sort(PeItem* a, int n, int sec)
+{
+    if (n <= 1) return;
+    int lg = 0;
+    while ((1 << (lg + 1)) <= n) ++lg;
+    int sf[48], sl[48], sd[48];
+    int sp = 0;
+    sf[0] = 0;
+    sl[0] = n;
+    sd[0] = 2 * lg;
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int first = sf[sp];
+        int last = sl[sp], depth = sd[sp];
+        while (last - first > 16) {
+            if (depth == 0) {
+                pe_heap_sort(a, first, last, sec);
+                break;
+            }
+            --depth;
+            const int mid = first + (last - first) / 2;
+            pe_median_to_first(a, first, first + 1, mid, last - 1, sec);
+            // __unguarded_partition(first + 1, last, first)
+            // (the bounds tests never fire: the median of three stops both
+            // scans, as in libstdc++; they only keep every access in range)
+            int lo = first + 1, hi = last;
+            for (;;) {
+                while (lo < last - 1 && pe_less(a[lo], a[first], sec)) ++lo;
+                --hi;
+                while (hi > first && pe_less(a[first], a[hi], sec)) --hi;
+                if (!(lo < hi)) break;
+                pe_swap(a, lo, hi);
+                ++lo;
+            }
+            const int cut = lo;
+            if (sp < 48) {
+                sf[sp] = cut;
+                sl[sp] = last;
+                sd[sp] = depth;
+                ++sp;
+            }
+            last = cut;
+        }
+    }
+    // __final_insertion_sort
+    const int g = n > 16 ? 16 : n;
+    for (int i = 1; i < g; ++i) {
+        const PeItem val = a[i];
+        if (pe_less(val, a[0], sec)) {
+            for (int k = i; k > 0; --k) a[k] = a[k - 1];
+            a[0] = val;
+        } else {
+            int k = i;
+            while (k > 0 && pe_less(val, a[k - 1], sec)) {
+                a[k] = a[k - 1];
+                --k;
+            }
+            a[k] = val;
+        }
+    }
+    for (int i = g; i < n; ++i) {
+        const PeItem val = a[i];
+        int k = i;
+        while (k > 0 && pe_less(val, a[k - 1], sec)) {
+            a[k] = a[k - 1];
+            --k;
+        }
+        a[k] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the walks of saQuerySeedSetRefFirst / Second over verified windows
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pool_of(uint32_t L) { return (L + 63) / 64 - 1; }
+
+__device__ __forceinline__ uint32_t jm_at(const BatchDev& bd, uint32_t pool, uint32_t j, uint32_t k)
+{
+    return job_match_at(bd, pool, j, k);
+}
+
+__device__ __forceinline__ uint8_t* acc_of(uint8_t* const (&acc)[FAME_NPOOL], uint32_t p)
+{
+    uint8_t* v = acc[0];
+#pragma unroll
+    for (uint32_t q = 1; q < FAME_NPOOL; ++q) v = p == q ? acc[q] : v;
+    return v;
+}
+
+// One strand's walk (matchFwdFirst / matchRevFirst :4538-4640, matchFwdSecond
+// / matchRevSecond :4650-4776): windows in sorted order until one's count
+// falls below max(bmCount - KMERDIST, QTHRESH); windows without the flag are
+// passed; a window's first match is dropped when it repeats the previous
+// window's last one.  Marks each walked slot in acc (1, | 2 when the first
+// match was dropped); `mark` (first read) receives flag3 for the matched
+// windows and their neighbours.  Returns the matches kept.
+__device__ uint32_t pe_walk(const DevIndex& ix, const BatchDev& vb, uint8_t* const (&acc)[FAME_NPOOL],
+                            uint32_t pool, const PeItem* list, uint32_t n, int sec, uint32_t flag, int32_t& bm,
+                            PeItem* mark, uint32_t nmark, int cw)
+{
+    uint32_t prevChr = 0;
+    uint64_t prevOff = ~0ull;
+    uint32_t kept = 0;
+    uint8_t* ac = acc_of(acc, pool);
+    for (uint32_t x = 0; x < n; ++x) {
+        const PeItem it = list[x];
+        const int32_t cnt = (int32_t)(sec ? pe_c1(it.v) : pe_c0(it.v));
+        const int32_t lim = bm - (int32_t)KMERDIST > (int32_t)QTHRESH ? bm - (int32_t)KMERDIST : (int32_t)QTHRESH;
+        if (cnt < lim) break;
+        if (!(it.v & flag)) continue;
+        const uint32_t j = it.j;
+        if (j == PE_NONE) continue; // not reached: flagged windows at or above QTHRESH carry a slot
+        const uint32_t nm = vb.job_nm[pool][j];
+        const uint4 w = ix.win[it.k];
+        uint32_t skip = 0;
+        if (nm > 0) {
+            const uint32_t m0 = jm_at(vb, pool, j, 0);
+            if ((uint64_t)(m0 >> 8) + w.x == prevOff && w.y == prevChr) skip = 1;
+        }
+        ac[j] = (uint8_t)(1u | (skip << 1));
+        kept += nm - skip;
+        if (nm > 0) {
+            if (cnt > bm) bm = cnt;
+            if (mark) {
+                for (uint32_t y = 0; y < nmark; ++y) {
+                    const int32_t d = (int32_t)(mark[y].k - it.k);
+                    if (d >= -cw && d <= cw) mark[y].v |= PF3;
+                }
+            }
+            prevChr = w.y;
+            prevOff = (uint64_t)w.x + (jm_at(vb, pool, j, nm - 1) >> 8);
+        } else {
+            prevChr = 0;
+            prevOff = ~0ull;
+        }
+    }
+    return kept;
+}
+
+// both strands: the one whose first (sorted) window has the larger c0 goes
+// first, the reverse map on ties (:2185-2240, :2265-2320)
+__device__ uint32_t pe_query(const DevIndex& ix, const BatchDev& vb, uint8_t* const (&acc)[FAME_NPOOL],
+                             uint32_t pool, const PeItem* lf, uint32_t nf, const PeItem* lr, uint32_t nr, int sec,
+                             uint32_t flag, PeItem* mf, PeItem* mr, int cw)
+{
+    int32_t bm = 0;
+    bool fwd_first = nf > 0 && (nr == 0 || pe_c0(lf[0].v) > pe_c0(lr[0].v));
+    uint32_t kept = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool fw = pass == 0 ? fwd_first : !fwd_first;
+        if (fw) kept += pe_walk(ix, vb, acc, pool, lf, nf, sec, flag, bm, mf, mf ? nf : 0, cw);
+        else kept += pe_walk(ix, vb, acc, pool, lr, nr, sec, flag, bm, mr, mr ? nr : 0, cw);
+    }
+    return kept;
+}
+
+__global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * pd.n) return;
+    const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t];
+    if (nf + nr == 0) {
+        pd.m1_n[t] = 0;
+        return;
+    }
+    const uint32_t i = t >> 1, c = t & 1;
+    const uint32_t off = pd.map_off[t];
+    PeItem* mp = pd.map + off;
+    PeItem* sp = pd.srt + off;
+    for (uint32_t x = 0; x < nf + nr; ++x) sp[x] = mp[x];
+    pe_sort(sp, (int)nf, 0);
+    pe_sort(sp + nf, (int)nr, 0);
+    const uint32_t poolA = pool_of(b1.lens[i]); // mate 1 is the first read in both combinations
+    const uint32_t m1 = pe_query(ix, b1, pd.acc1, poolA, sp, nf, sp + nf, nr, 0, PF2, mp, mp + nf, (int)pd.cw);
+    pd.m1_n[t] = m1;
+    if (m1 == 0) return;
+    // saQuerySeedSetRefSecond's order: the maps (with flag3) sorted by c1
+    for (uint32_t x = 0; x < nf + nr; ++x) sp[x] = mp[x];
+    pe_sort(sp, (int)nf, 1);
+    pe_sort(sp + nf, (int)nr, 1);
+    const uint32_t rB = pd.n + i;
+    const uint32_t roB = 2 * rB + (1 - c);
+    const uint32_t poolB = pool_of(b1.lens[rB]);
+    uint32_t nj = 0;
+    for (uint32_t x = 0; x < nf + nr; ++x)
+        if ((sp[x].v & PF3) && pe_c1(sp[x].v) >= QTHRESH) ++nj;
+    uint32_t base = 0;
+    bool ok = true;
+    if (nj) {
+        base = atomicAdd(&b2.pool_n[poolB], nj);
+        if (base + nj > pc_get_cap(b2, poolB)) {
+            atomicOr(b1.flags, FLAG_POOL_FULL);
+            ok = false;
+        }
+    }
+    uint32_t* jm = pc_meta(b2, poolB);
+    uint32_t* jr = b2.job_ro[0];
+#pragma unroll
+    for (uint32_t q = 1; q < FAME_NPOOL; ++q) jr = poolB == q ? b2.job_ro[q] : jr;
+    uint32_t k = 0;
+    for (uint32_t x = 0; x < nf + nr; ++x) {
+        sp[x].j = PE_NONE;
+        if (ok && (sp[x].v & PF3) && pe_c1(sp[x].v) >= QTHRESH) {
+            const uint32_t j = base + k++;
+            const bool rev = x >= nf;
+            // matchRevSecond scans from the chromosome start itself when the
+            // window starts there (bit 30)
+            const bool ung = rev && ix.win[sp[x].k].x == 0;
+            jm[j] = sp[x].k | (rev ? 0x80000000u : 0u) | (ung ? 0x40000000u : 0u);
+            jr[j] = roB;
+            sp[x].j = j;
+        }
+    }
+}
+
+// extractPairedMatch :3255-3275 (MINPDIST is not used)
+__device__ __forceinline__ int pe_extract(const DevIndex& ix, uint64_t m1, uint64_t m2, uint32_t maxd)
+{
+    const uint4 w1 = ix.win[(uint32_t)(m1 >> 32)];
+    const uint4 w2 = ix.win[(uint32_t)(m2 >> 32)];
+    if (w1.y != w2.y) return -1;
+    const uint32_t p1 = w1.x + (uint32_t)(m1 & 0xFFFF);
+    const uint32_t p2 = w2.x + (uint32_t)(m2 & 0xFFFF);
+    const uint32_t d = p2 > p1 ? p2 - p1 : p1 - p2;
+    if (d <= maxd) return (int)((m1 >> 16) & 0xFF) + (int)((m2 >> 16) & 0xFF);
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_pe_final(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd,
+                                                  int get_stranded)
+{
+    __shared__ uint32_t s_cnt[7];
+    if (threadIdx.x < 7) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int want1 = 0, want2 = 0;
+    if (i < pd.n) {
+        const uint32_t n = pd.n;
+        const uint8_t p1 = b1.pre[i], p2 = b1.pre[n + i];
+        uint64_t mat1 = 0, mat2 = 0;
+        uint8_t v1 = 1, v2 = 1, use1 = 0, use2 = 0;
+        if ((p1 & 1) || (p2 & 1)) atomicAdd(&s_cnt[4], 1u);
+        if (p1 || p2) {
+            atomicAdd(&s_cnt[4], 2u);
+            v1 = p1 == 0;
+            v2 = p2 == 0;
+        } else {
+            uint32_t m1c[2] = {0, 0}, m2c[2] = {0, 0};
+            for (uint32_t c = 0; c < 2; ++c) {
+                const uint32_t t = 2 * i + c;
+                if (!((pd.combos >> c) & 1)) continue;
+                m1c[c] = pd.m1_n[t];
+                if (!m1c[c]) continue;
+                const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t];
+                const PeItem* sp = pd.srt + pd.map_off[t];
+                const uint32_t poolB = pool_of(b1.lens[n + i]);
+                m2c[c] = pe_query(ix, b2, pd.acc2, poolB, sp, nf, sp + nf, nr, 1, PF3, nullptr, nullptr,
+                                  (int)pd.cw);
+            }
+            if ((m1c[0] == 0 && m1c[1] == 0) || (m2c[0] == 0 && m2c[1] == 0)) {
+                atomicAdd(&s_cnt[2], 2u);
+                v1 = v2 = 0;
+            } else {
+                // every (mate 1, mate 2) combination of both passes: the
+                // lowest summed error count, unique or not (:1309-1373)
+                const uint32_t maxd = pd.maxpd + ix.readlen;
+                int best = 2 * (int)NERR + 1, nbest = 0;
+                uint64_t b1m = 0, b2m = 0;
+                uint32_t orig = 1;
+                for (uint32_t c = 0; c < 2; ++c) {
+                    if (!m1c[c] || !m2c[c]) continue;
+                    const uint32_t t = 2 * i + c;
+                    const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t], off = pd.map_off[t];
+                    const uint32_t poolA = pool_of(b1.lens[i]);
+                    const uint32_t poolB = pool_of(b1.lens[n + i]);
+                    const uint8_t* aA = acc_of(pd.acc1, poolA);
+                    const uint8_t* aB = acc_of(pd.acc2, poolB);
+                    for (uint32_t xa = 0; xa < nf + nr; ++xa) {
+                        const PeItem A = pd.map[off + xa];
+                        if (A.j == PE_NONE || !aA[A.j]) continue;
+                        const uint32_t nmA = b1.job_nm[poolA][A.j];
+                        const uint64_t fA = xa < nf ? 1 : 0;
+                        for (uint32_t ka = (aA[A.j] >> 1) & 1; ka < nmA; ++ka) {
+                            const uint32_t ma = jm_at(b1, poolA, A.j, ka);
+                            const uint64_t MA = mk_match(ma >> 8, ma & 0xFF, fA, A.k);
+                            for (uint32_t xb = 0; xb < nf + nr; ++xb) {
+                                const PeItem B = pd.srt[off + xb];
+                                if (B.j == PE_NONE || !aB[B.j]) continue;
+                                const uint32_t nmB = b2.job_nm[poolB][B.j];
+                                const uint64_t fB = xb < nf ? 1 : 0;
+                                for (uint32_t kb = (aB[B.j] >> 1) & 1; kb < nmB; ++kb) {
+                                    const uint32_t mb = jm_at(b2, poolB, B.j, kb);
+                                    const uint64_t MB = mk_match(mb >> 8, mb & 0xFF, fB, B.k);
+                                    const int e = pe_extract(ix, MA, MB, maxd);
+                                    if (e < 0) continue;
+                                    if (e < best) {
+                                        best = e;
+                                        nbest = 1;
+                                        b1m = MA;
+                                        b2m = MB;
+                                        orig = c == 0;
+                                    } else if (e == best) {
+                                        ++nbest;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                if (nbest == 0) {
+                    atomicAdd(&s_cnt[2], 2u);
+                    v1 = v2 = 0;
+                } else if (nbest > 1) {
+                    atomicAdd(&s_cnt[1], 2u);
+                    v1 = v2 = 0;
+                } else {
+                    mat1 = b1m;
+                    mat2 = b2m;
+                    // computeMethLvl(r1.mat, r1.seq / revSeq1) and (r2.mat,
+                    // revSeq2 / r2.seq) (:1603-1618)
+                    use1 = orig ? 1 : 2;
+                    use2 = orig ? 2 : 1;
+                    if (get_stranded) atomicAdd(&s_cnt[orig ? 5 : 6], 1u);
+                    const bool d1 = ix.win[(uint32_t)(b1m >> 32)].z == CPGDUMMY;
+                    const bool d2 = ix.win[(uint32_t)(b2m >> 32)].z == CPGDUMMY;
+                    if (d1 && d2) {
+                        atomicAdd(&s_cnt[4], 2u);
+                    } else if (d1 || d2) {
+                        atomicAdd(&s_cnt[4], 1u);
+                        atomicAdd(&s_cnt[0], 1u);
+                    } else {
+                        atomicAdd(&s_cnt[3], 1u);
+                        atomicAdd(&s_cnt[0], 2u);
+                    }
+                    if (!d1) want1 = ((b1m >> 16) & 0xFF) == 0 ? 1 : 2;
+                    if (!d2) want2 = ((b2m >> 16) & 0xFF) == 0 ? 1 : 2;
+                }
+            }
+        }
+        b1.valid[i] = v1;
+        b1.valid[n + i] = v2;
+        b1.mat[i] = mat1;
+        b1.mat[n + i] = mat2;
+        b1.use[i] = use1;
+        b1.use[n + i] = use2;
+    }
+    {
+        const int lane = threadIdx.x & 63;
+        const uint64_t below = lanes_below(lane);
+        for (int mate = 0; mate < 2; ++mate) {
+            const int want = mate ? want2 : want1;
+            const uint32_t r = mate ? pd.n + i : i;
+            const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2);
+            uint32_t basedir = 0, basedp = 0;
+            if (lane == 0) {
+                if (bdir) basedir = atomicAdd(b1.direct_n, (uint32_t)__popcll(bdir));
+                if (bdp) basedp = atomicAdd(b1.dp_n, (uint32_t)__popcll(bdp));
+            }
+            basedir = __shfl(basedir, 0, 64);
+            basedp = __shfl(basedp, 0, 64);
+            if (want == 1) b1.direct_list[basedir + __popcll(bdir & below)] = r;
+            if (want == 2) b1.dp_list[basedp + __popcll(bdp & below)] = r;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 5 && s_cnt[threadIdx.x])
+        atomicAdd((unsigned long long*)&pd.counters[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    if (threadIdx.x >= 5 && threadIdx.x < 7 && s_cnt[threadIdx.x])
+        atomicAdd((unsigned long long*)&b1.counters[threadIdx.x - 2], (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, uint32_t grid, hipStream_t st)
+{
+    if (!grid) return;
+    if (ix.tab_is64) hipLaunchKernelGGL(k_pe_seed<uint64_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+    else hipLaunchKernelGGL(k_pe_seed<uint32_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+}
+
+void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st)
+{
+    const uint32_t items = 2 * pd.n;
+    if (!items) return;
+    hipLaunchKernelGGL(k_pe_walk1, dim3((items + 255) / 256), dim3(256), 0, st, ix, bd1, bd2, pd);
+}
+
+void launch_pe_final(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd,
+                     int get_stranded, hipStream_t st)
+{
+    if (!pd.n) return;
+    hipLaunchKernelGGL(k_pe_final, dim3((pd.n + 255) / 256), dim3(256), 0, st, ix, bd1, bd2, pd, get_stranded);
+}
+
+} // namespace fame

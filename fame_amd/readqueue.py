@@ -23,36 +23,51 @@ class RefGenome:
         self.path = file_path
 
 
-class ReadQueue:
-    """Single-end ReadQueue (ReadQueue.cpp:25-78)."""
+def _parse(f, limit):
+    seqs = []
+    while len(seqs) < limit:
+        head = f.readline()
+        if not head:
+            break
+        seq = f.readline().rstrip(b"\n")
+        f.readline()
+        f.readline()
+        seqs.append(seq)
+    lens = np.array([len(s) for s in seqs], np.uint32)
+    offs = np.zeros(len(seqs), np.uint64)
+    if len(seqs) > 1:
+        offs[1:] = np.cumsum(lens.astype(np.uint64))[:-1]
+    return (np.frombuffer(b"".join(seqs), np.uint8), offs, lens)
 
-    def __init__(self, file_path, ref, is_gz=False, bs_flag=False, chunk_size=CHUNKSIZE):
+
+class ReadQueue:
+    """ReadQueue: single-end (ReadQueue.cpp:25-78) or, with file_path2, the
+    paired-end constructor (:79-180)."""
+
+    def __init__(self, file_path, ref, is_gz=False, bs_flag=False, chunk_size=CHUNKSIZE, file_path2=None,
+                 minpd=50, maxpd=550):
         self.ctx = _lib.Context(ref.path, both_strands=bs_flag)
         self.bs_flag = bs_flag
         self.chunk_size = chunk_size
         opener = gzip.open if is_gz else open
         self._f = opener(file_path, "rb") if file_path else None
-        self.buf = None  # (seqs, offs, lens) of the parsed chunk
+        self._f2 = opener(file_path2, "rb") if file_path2 else None
+        self.is_paired = file_path2 is not None
+        if self.is_paired:
+            self.ctx.set_pair_dist(minpd, maxpd)
+        self.buf = None   # (seqs, offs, lens) of the parsed chunk
+        self.buf2 = None  # the same for the mates of a paired run
         self.results = None
 
     # parseChunk ReadQueue.cpp:218-296
     def parseChunk(self):
-        seqs = []
-        n = 0
-        while n < self.chunk_size:
-            head = self._f.readline()
-            if not head:
-                break
-            seq = self._f.readline().rstrip(b"\n")
-            self._f.readline()
-            self._f.readline()
-            seqs.append(seq)
-            n += 1
-        lens = np.array([len(s) for s in seqs], np.uint32)
-        offs = np.zeros(len(seqs), np.uint64)
-        if len(seqs) > 1:
-            offs[1:] = np.cumsum(lens.astype(np.uint64))[:-1]
-        self.buf = (np.frombuffer(b"".join(seqs), np.uint8), offs, lens)
+        self.buf = _parse(self._f, self.chunk_size)
+        n = len(self.buf[2])
+        if self.is_paired:
+            self.buf2 = _parse(self._f2, self.chunk_size)
+            if len(self.buf2[2]) != n:
+                # the reference prints a message and exits (:280-286)
+                raise _lib.FameError("Not the same number of reads available in the paired read files")
         return n >= self.chunk_size, n
 
     def set_chunk(self, seqs, offs, lens):
@@ -68,6 +83,20 @@ class ReadQueue:
         self.results = (valid, mat)
         return True
 
+    def set_pairs(self, r1, r2):
+        """Places an in-memory chunk of pairs in the read buffers."""
+        self.buf, self.buf2 = r1, r2
+
+    # matchPairedReads ReadQueue.cpp:935-1655
+    def matchPairedReads(self, procReads, counters, getStranded):
+        """counters: [succ, nonunique, unsucc, succPaired, tooShort], accumulated."""
+        r1 = (self.buf[0], self.buf[1][:procReads], self.buf[2][:procReads])
+        r2 = (self.buf2[0], self.buf2[1][:procReads], self.buf2[2][:procReads])
+        c = np.array(counters[:5], np.uint64)
+        self.results = self.ctx.match_pairs(r1, r2, getStranded, c)
+        counters[:5] = [int(x) for x in c]
+        return True
+
     # decideStrand ReadQueue.cpp:376-400
     def decideStrand(self):
         fwd, f, r = self.ctx.decide_strand()
@@ -78,8 +107,9 @@ class ReadQueue:
         self.ctx.print_methylation_levels(filename)
 
     def close(self):
-        if self._f:
-            self._f.close()
+        for f in (self._f, self._f2):
+            if f:
+                f.close()
         self.ctx.close()
 
 
@@ -100,6 +130,29 @@ def query_routine(rque, both_strands, on_chunk=None):
         if on_chunk:
             on_chunk(rque.results)
     rque.matchReads(n, counters, False)
+    if on_chunk:
+        on_chunk(rque.results)
+    return counters
+
+
+def query_routine_paired(rque, both_strands, on_chunk=None):
+    """queryRoutinePaired main.cpp:400-441; returns
+    [succ, nonunique, unsucc, succPaired, tooShort]."""
+    counters = [0, 0, 0, 0, 0]
+    if not both_strands:
+        _, n = rque.parseChunk()
+        rque.matchPairedReads(n, counters, True)
+        if on_chunk:
+            on_chunk(rque.results)
+        rque.decideStrand()
+    while True:
+        more, n = rque.parseChunk()
+        if not more:
+            break
+        rque.matchPairedReads(n, counters, False)
+        if on_chunk:
+            on_chunk(rque.results)
+    rque.matchPairedReads(n, counters, False)
     if on_chunk:
         on_chunk(rque.results)
     return counters

@@ -76,6 +76,25 @@ int fame_match_reads_device(fame_ctx* ctx, const uint8_t* d_seqs, const uint64_t
                             uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat,
                             void* stream);
 
+/* MINPDIST / MAXPDIST (CONST.h:41-42, compile-time constants in the
+ * reference; defaults 50 / 550 here).  Only MAXPDIST enters the matching:
+ * contextWLen = MAXPDIST / WINLEN + 1 (ReadQueue.cpp:2854, 4540, 4607) and the
+ * mate distance bound MAXPDIST + READLEN of extractPairedMatch (:3270).
+ * FAME_ERR_ARG when MAXPDIST spans 30 windows or more. */
+int fame_set_pair_dist(fame_ctx* ctx, uint32_t minpd, uint32_t maxpd);
+
+/* ReadQueue::matchPairedReads ReadQueue.cpp:935-1655.  Pair i is mate 1
+ * seqs1[offs1[i] .. +lens1[i]) and mate 2 seqs2[offs2[i] .. +lens2[i]).
+ * counters[0..4] += succMatch, nonUniqueMatch, unSuccMatch, succPairedMatch,
+ * tooShortCount (main.cpp:405-409).  valid1/2[i] = !isInvalid of the mates and
+ * mat1/2[i] = their Read::mat, set for matched pairs and 0 otherwise (the
+ * reference leaves mat unset there).  Needs ceil(0.75 READLEN) >= 32
+ * (FAME_ERR_FORMAT otherwise).  The output pointers may be NULL. */
+int fame_match_pairs(fame_ctx* ctx, const uint8_t* seqs1, const uint64_t* offs1, const uint32_t* lens1,
+                     const uint8_t* seqs2, const uint64_t* offs2, const uint32_t* lens2, uint32_t n,
+                     int get_stranded, uint64_t counters[5], uint8_t* valid1, uint64_t* mat1,
+                     uint8_t* valid2, uint64_t* mat2);
+
 /* ReadQueue::decideStrand ReadQueue.cpp:376-400 (without the console
  * report): matchR1Fwd = r1FwdMatches > r1RevMatches.  Any pointer may be
  * NULL. */
