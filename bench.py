@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X read-matching path on the genome-scale workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome_se|uneven_se|paired]
 
 Workload (the reference's GPU drop-in case): a synthetic 300 Mbp reference
 of 24 chromosomes (i.i.d. bases, GC 41 %, CpG depleted to ~1 %), indexed
@@ -16,6 +16,15 @@ both orientations, the strand decision, then the remaining reads in one
 orientation.  Reads are resident in HBM before the timed region; index
 construction and loading are outside it.  value = aligned reads/s
 (succMatch / time).
+
+Other workloads (--workload): uneven_se = the same genome with 3 000
+segments copied 5-50 times (1 % divergence), 10 M single-end reads of
+50-150 bp (30 % from the copied segments), READLEN 150; paired = 300 Mbp
+with 20 % of the sequence in 300 bp repeat families (10 % divergence),
+indexed without KMERCUTOFF (--no_loss), 5 M pairs of 2 x 150 bp with inner
+gap uniform in [20, 400], indel rate 0.2 %, MINPDIST 20 / MAXPDIST 400,
+READLEN 150 (value = matched reads / s, both mates counted, as FAME's
+succMatch counts them).
 
 Also reported: per-stage device time, the roofline of the dominant kernel,
 and FAME's own OpenMP CPU path (built from its sources into oracle/_ref)
@@ -58,10 +67,47 @@ def workdir_for(need_bytes, requested):
     raise SystemExit(f"no folder with {need_bytes / 1e9:.1f} GB free for the index")
 
 
-def ref_binary(htab, readlen, cores, nchrom):
-    name = f"fame_ref_h{htab}_r{readlen}_c{cores}_n{nchrom}_p50-550_k{CHUNKSIZE}"
+def ref_binary(htab, readlen, cores, nchrom, minpd=50, maxpd=550):
+    name = f"fame_ref_h{htab}_r{readlen}_c{cores}_n{nchrom}_p{minpd}-{maxpd}_k{CHUNKSIZE}"
     p = os.path.join(ROOT, "oracle", "_ref", name)
     return p if os.path.exists(p) else None
+
+
+# workloads: genome, index constants, reads
+WORKLOADS = {
+    "genome_se": dict(readlen=100, n=10_000_000, paired=False, noloss=False, minpd=50, maxpd=550,
+                      desc="genome-scale single-end: 300 Mbp / 24 chromosomes, 10M x 100 bp bisulfite "
+                           "reads, HTABSIZE 2^30, plain one-kernel-per-stage form"),
+    "uneven_se": dict(readlen=150, n=10_000_000, paired=False, noloss=False, minpd=50, maxpd=550,
+                      len_range=(50, 150), near_copies=3000, copied_frac=0.3,
+                      desc="uneven single-end: 300 Mbp with 3000 segments copied 5-50 times, 10M reads of "
+                           "50-150 bp (30% from the copies), READLEN 150, plain form"),
+    "paired": dict(readlen=150, n=5_000_000, paired=True, noloss=True, minpd=20, maxpd=400,
+                   repeat_frac=0.2, gap=(20, 400), indel=0.002,
+                   desc="paired-end repeat-heavy: 300 Mbp with 20% in 300 bp repeat families (10% "
+                        "divergence), --no_loss index, 5M pairs of 2 x 150 bp, gap 20-400, indel 0.2%, "
+                        "plain form"),
+}
+
+
+def make_workload(wl, args, chroms):
+    """Reads (single-end ReadSet) or pairs ((ReadSet, ReadSet)) of a workload."""
+    from fame_amd import synth
+    n = args.reads or wl["n"]
+    if wl["paired"]:
+        return synth.make_pairs(chroms, n, wl["readlen"], gap=wl["gap"], seed=args.seed + 2,
+                                indel_rate=wl["indel"])
+    if "near_copies" not in wl:
+        return synth.make_reads(chroms, n, wl["readlen"], seed=args.seed + 2, len_range=wl.get("len_range"))
+    regions = wl["regions"]
+    nc = int(n * wl["copied_frac"])
+    a = synth.make_reads(chroms, n - nc, wl["readlen"], seed=args.seed + 2, len_range=wl["len_range"])
+    b = synth.make_reads(chroms, nc, wl["readlen"], seed=args.seed + 3, len_range=wl["len_range"],
+                         regions=regions)
+    order = np.random.default_rng(args.seed + 4).permutation(n)
+    seqs = np.concatenate([a.seqs, b.seqs])[order]
+    lens = np.concatenate([a.lens, b.lens])[order]
+    return synth.ReadSet(seqs, lens)
 
 
 def write_fastq(path, seqs, lens):
@@ -82,11 +128,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--reads", type=int, default=10_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="genome_se")
+    ap.add_argument("--reads", type=int, default=0, help="reads (pairs) per step; 0 = the workload's")
     ap.add_argument("--genome-scale", type=float, default=1.0, help="1.0 = 300 Mbp")
     ap.add_argument("--htab", type=int, default=1 << 30)
-    ap.add_argument("--batch", type=int, default=2_000_000, help="reads per call after the first chunk")
-    ap.add_argument("--cpu-sample", type=int, default=600_000)
+    ap.add_argument("--batch", type=int, default=2_000_000, help="reads (pairs) per call after the first chunk")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="reads (pairs) of the CPU leg; 0 = default")
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
@@ -101,6 +148,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by read "
               "chunks with a replicated index, which this build does not launch", file=sys.stderr)
         raise SystemExit(2)
+    wl = dict(WORKLOADS[args.workload])
+    if not args.cpu_sample:
+        args.cpu_sample = 300_000 if wl["paired"] else 600_000
 
     import torch  # device memory and synchronisation only
 
@@ -114,22 +164,42 @@ def main():
     lens_c = synth.config_genome("genome", scale=args.genome_scale, seed=args.seed)
     log(f"genome: {len(lens_c)} chromosomes, {sum(lens_c) / 1e6:.1f} Mbp")
     chroms = synth.make_genome(lens_c, seed=args.seed)
+    if wl.get("repeat_frac"):
+        k = synth.add_repeat_families(chroms, frac=wl["repeat_frac"], seed=args.seed + 5)
+        log(f"{k} repeat-family copies placed")
+    if wl.get("near_copies"):
+        wl["regions"] = synth.add_near_copies(chroms, segments=wl["near_copies"], seed=args.seed + 6)
+        log(f"{len(wl['regions'][0])} near-copy regions")
     est = args.htab * 8 + sum(lens_c) * 10 + (1 << 30)
     wd = workdir_for(est, args.workdir)
     idx_path = os.path.join(wd, "genome.idx")
+    RL = wl["readlen"]
     try:
         t0 = time.time()
-        index.build_index(chroms, idx_path, readlen=100, htab=args.htab, kmercutoff=1500)
+        index.build_index(chroms, idx_path, readlen=RL, htab=args.htab, kmercutoff=1500, noloss=wl["noloss"])
         t_build = time.time() - t0
         log(f"index built in {t_build:.1f}s ({os.path.getsize(idx_path) / 1e9:.2f} GB) in {wd}")
         t0 = time.time()
-        reads = synth.make_reads(chroms, args.reads, 100, seed=args.seed + 2)
+        reads = make_workload(wl, args, chroms)
         t_reads = time.time() - t0
-        log(f"{args.reads} reads generated in {t_reads:.1f}s")
-        seqs, offs, lens = reads.flat()
+        paired = wl["paired"]
+        if paired:
+            r1, r2 = reads
+            s1, o1, l1 = r1.flat()
+            s2, o2, l2 = r2.flat()
+            seqs = np.concatenate([s1, s2])
+            offs = np.concatenate([o1, o2 + np.uint64(s1.size)])
+            lens = np.concatenate([l1, l2])
+            n = len(l1)
+        else:
+            seqs, offs, lens = reads.flat()
+            n = len(lens)
+        log(f"{n} {'pairs' if paired else 'reads'} generated in {t_reads:.1f}s")
         # ---------------- device ----------------
         t0 = time.time()
         ctx = _lib.Context(idx_path)
+        if paired:
+            ctx.set_pair_dist(wl["minpd"], wl["maxpd"])
         t_load = time.time() - t0
         log(f"index resident in HBM after {t_load:.1f}s: {ctx.info}")
         d_seqs = torch.from_numpy(seqs).to(dev)
@@ -138,17 +208,23 @@ def main():
         d_valid = torch.zeros(len(lens), dtype=torch.uint8, device=dev)
         d_mat = torch.zeros(len(lens), dtype=torch.int64, device=dev)
         torch.cuda.synchronize()
-        n = len(lens)
         calls = []  # per call: reads, stage timings (device ms), work counts
+        ncnt = 5 if paired else 3
 
         def call(s, e, stranded, cnt):
-            ctx.match_device(d_seqs.data_ptr(), d_offs.data_ptr() + 8 * s, d_lens.data_ptr() + 4 * s,
-                             e - s, stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s)
+            if paired:
+                ctx.match_pairs_device(d_seqs.data_ptr(), d_offs.data_ptr() + 8 * s, d_lens.data_ptr() + 4 * s,
+                                       d_offs.data_ptr() + 8 * (n + s), d_lens.data_ptr() + 4 * (n + s), e - s,
+                                       stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s,
+                                       d_valid.data_ptr() + n + s, d_mat.data_ptr() + 8 * (n + s))
+            else:
+                ctx.match_device(d_seqs.data_ptr(), d_offs.data_ptr() + 8 * s, d_lens.data_ptr() + 4 * s,
+                                 e - s, stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s)
             calls.append((e - s, ctx.timings(), ctx.stats()))
 
         def step():
             ctx.reset()
-            cnt = np.zeros(3, np.uint64)
+            cnt = np.zeros(ncnt, np.uint64)
             e = min(n, CHUNKSIZE)
             call(0, e, True, cnt)
             ctx.decide_strand()
@@ -171,7 +247,7 @@ def main():
         ms_step = 1e3 * t_run / max(1, args.steps)
         succ = int(cnt[0])
         value = succ / (ms_step / 1e3)
-        log(f"step {ms_step:.1f} ms; matched {succ} nonunique {int(cnt[1])} unmatched {int(cnt[2])} of {n}")
+        log(f"step {ms_step:.1f} ms; counters {cnt.tolist()} over {n} {'pairs' if paired else 'reads'}")
         K = max(1, args.steps)
         per_stage, per_work = {}, {}
         for _, tm, st in calls:
@@ -182,37 +258,39 @@ def main():
         per_work = {k: (v // K if k != "tab_bytes" else v) for k, v in per_work.items()}
         log("stage ms per step:", {k: round(v, 2) for k, v in per_stage.items()})
         log("work per step:", per_work)
-        # ---------------- roofline ----------------
-        # over the full-size calls (one launch of each kernel per call)
-        full = [c for c in calls if c[0] == args.batch] or calls
-        dev_stages = {k: per_stage[k] for k in ("prep", "seed", "verify", "select", "meth")}
-        dominant = max(dev_stages, key=dev_stages.get)
-        roof = roofline("seed", full)
-        roof["dominant_stage"] = dominant
-        roof_verify = roofline("verify", full)
-        verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
+        names = ["succ", "nonunique", "unsucc"] + (["succ_paired", "too_short"] if paired else [])
         out = {
             "metric": METRIC, "value": round(value, 1), "unit": "reads/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
             "data": "synthetic",
-            "config": {"workload": "genome-scale single-end: 300 Mbp / 24 chromosomes, 10M x 100 bp "
-                                   "bisulfite reads, HTABSIZE 2^30, plain one-kernel-per-stage form",
-                       "genome_bp": int(sum(lens_c)), "reads": n, "read_len": 100,
-                       "batch": args.batch, "first_chunk": CHUNKSIZE, "form": "plain"},
-            "input_reads_per_s": round(n / (ms_step / 1e3), 1),
-            "counters": {"succ": succ, "nonunique": int(cnt[1]), "unsucc": int(cnt[2])},
+            "config": {"workload": wl["desc"], "genome_bp": int(sum(lens_c)), ("pairs" if paired else "reads"): n,
+                       "read_len": RL, "batch": args.batch, "first_chunk": CHUNKSIZE, "form": "plain"},
+            "input_reads_per_s": round((2 if paired else 1) * n / (ms_step / 1e3), 1),
+            "counters": dict(zip(names, (int(x) for x in cnt))),
             "stage_ms": {k: round(v, 3) for k, v in per_stage.items()},
             "work": per_work,
-            "verify_window_GBps": round(verify_rate / 1e9, 2),
-            "roofline": roof,
-            "roofline_verify": roof_verify,
             "setup_s": {"index_build": round(t_build, 1), "reads": round(t_reads, 1),
                         "index_load": round(t_load, 1)},
         }
+        if not paired:
+            # ---------------- roofline ----------------
+            # over the full-size calls (one launch of each kernel per call)
+            full = [c for c in calls if c[0] == args.batch] or calls
+            dev_stages = {k: per_stage[k] for k in ("prep", "seed", "verify", "select", "meth")}
+            roof = roofline("seed", full)
+            roof["dominant_stage"] = max(dev_stages, key=dev_stages.get)
+            verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
+            out["verify_window_GBps"] = round(verify_rate / 1e9, 2)
+            out["roofline"] = roof
+            out["roofline_verify"] = roofline("verify", full)
         # ---------------- CPU baseline + parity on a sample ----------------
         if not args.no_cpu:
-            out["cpu_baseline"], out["parity"] = cpu_leg(args, ctx, reads, idx_path, wd, len(lens_c))
+            if paired:
+                out["cpu_baseline"], out["parity"] = cpu_leg_pairs(args, wl, ctx, reads, idx_path, wd,
+                                                                   len(lens_c))
+            else:
+                out["cpu_baseline"], out["parity"] = cpu_leg(args, wl, ctx, reads, idx_path, wd, len(lens_c))
         ctx.close()
         out["wall_s"] = round(time.time() - t_all, 1)
         print(json.dumps(out), flush=True)
@@ -274,11 +352,11 @@ def copy_probe(mib):
     log(f"copy probe: 3 x {mib} MiB read + {mib} MiB written")
 
 
-def cpu_leg(args, ctx, reads, idx_path, wd, nchrom):
+def cpu_leg(args, wl, ctx, reads, idx_path, wd, nchrom):
     """FAME's own CPU path on the first `cpu_sample` reads of the workload, and
     the HIP path on the same sample for a per-read / report comparison."""
     S = min(args.cpu_sample, len(reads))
-    binp = ref_binary(args.htab, 100, args.cpu_cores, nchrom)
+    binp = ref_binary(args.htab, wl["readlen"], args.cpu_cores, nchrom)
     sample = f"first {S} reads of the workload (chunk 1 of {CHUNKSIZE} in both orientations, the rest stranded)"
     if binp is None:
         log("reference CPU binary not built; cpu_baseline skipped")
@@ -323,6 +401,74 @@ def cpu_leg(args, ctx, reads, idx_path, wd, nchrom):
     mism = int(np.count_nonzero((valid != rvalid) | (mat != rmat)))
     parity = {"reads": S, "per_read_mismatches": mism, "cpg_report_identical": bool(same_tsv),
               "counters_equal": cnt.tolist() == [st["succ"], st["nonunique"], st["unsucc"]]}
+    log("parity on the sample:", parity)
+    return cpu, parity
+
+
+def read_pairs_bin(path):
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"FAMEPE01"
+    n = int(np.frombuffer(raw[8:16], np.uint64)[0])
+    v1 = np.frombuffer(raw[16:16 + n], np.uint8)
+    v2 = np.frombuffer(raw[16 + n:16 + 2 * n], np.uint8)
+    m1 = np.frombuffer(raw[16 + 2 * n:16 + 10 * n], np.uint64)
+    m2 = np.frombuffer(raw[16 + 10 * n:16 + 18 * n], np.uint64)
+    return v1, m1, v2, m2
+
+
+def cpu_leg_pairs(args, wl, ctx, pairs, idx_path, wd, nchrom):
+    """FAME's paired CPU path (queryRoutinePaired) on the first `cpu_sample`
+    pairs, and the HIP path on the same sample for a per-pair comparison."""
+    r1, r2 = pairs
+    S = min(args.cpu_sample, len(r1))
+    binp = ref_binary(args.htab, wl["readlen"], args.cpu_cores, nchrom, wl["minpd"], wl["maxpd"])
+    sample = (f"first {S} pairs of the workload (chunk 1 of {CHUNKSIZE} in both combinations, the rest "
+              "stranded)")
+    if binp is None:
+        log("reference CPU binary not built; cpu_baseline skipped")
+        return None, None
+    fq1, fq2 = os.path.join(wd, "s1.fq"), os.path.join(wd, "s2.fq")
+    write_fastq(fq1, r1.seqs[:S], r1.lens[:S])
+    write_fastq(fq2, r2.seqs[:S], r2.lens[:S])
+    log(f"CPU reference on {S} pairs with {args.cpu_cores} threads ...")
+    t0 = time.time()
+    r = subprocess.run([binp, "match_pe", idx_path, fq1, fq2, os.path.join(wd, "cpu")], cwd=wd,
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        log("reference run failed:", r.stderr[-500:])
+        return None, None
+    st = json.load(open(os.path.join(wd, "cpu_stats.json")))
+    log(f"CPU reference: {st['succ']} matched reads in {st['match_s']:.2f}s (load {st['load_s']:.1f}s, "
+        f"wall {time.time() - t0:.1f}s)")
+    cpu = {"value": round(st["succ"] / st["match_s"], 1), "unit": "reads/s", "cores": args.cpu_cores,
+           "kind": "reference", "sample": sample, "match_s": round(st["match_s"], 3),
+           "pairs": int(st["reads"]), "succ": int(st["succ"])}
+    rv1, rm1, rv2, rm2 = read_pairs_bin(os.path.join(wd, "cpu_pairs.bin"))
+    ctx.reset()
+    a = r1.subset(np.arange(S)).flat()
+    b = r2.subset(np.arange(S)).flat()
+    cnt = np.zeros(5, np.uint64)
+    v1 = np.zeros(S, np.uint8)
+    v2 = np.zeros(S, np.uint8)
+    m1 = np.zeros(S, np.uint64)
+    m2 = np.zeros(S, np.uint64)
+    s = 0
+    stranded = True
+    while s < S:
+        e = min(S, s + CHUNKSIZE)
+        v1[s:e], m1[s:e], v2[s:e], m2[s:e] = ctx.match_pairs((a[0], a[1][s:e], a[2][s:e]),
+                                                             (b[0], b[1][s:e], b[2][s:e]), stranded, cnt)
+        if stranded:
+            ctx.decide_strand()
+            stranded = False
+        s = e
+    base = os.path.join(wd, "hip")
+    ctx.print_methylation_levels(base)
+    same_tsv = open(base + "_cpg.tsv", "rb").read() == open(os.path.join(wd, "cpu_cpg.tsv"), "rb").read()
+    mism = int(np.count_nonzero((v1 != rv1) | (v2 != rv2) | (m1 != rm1) | (m2 != rm2)))
+    parity = {"pairs": S, "per_pair_mismatches": mism, "cpg_report_identical": bool(same_tsv),
+              "counters_equal": cnt.tolist() == [st["succ"], st["nonunique"], st["unsucc"], st["succ_paired"],
+                                                 st["too_short"]]}
     log("parity on the sample:", parity)
     return cpu, parity
 

@@ -20,7 +20,8 @@ MAX_READ = 255
 
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
-           "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_decide_strand", "fame_get_meth", "fame_reset",
+           "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
+           "fame_decide_strand", "fame_get_meth", "fame_reset",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
@@ -55,6 +56,8 @@ def load():
         L.fame_set_pair_dist.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.fame_match_pairs.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, _u8p, _u64p, _u32p, C.c_uint32,
                                        C.c_int, _u64p, _u8p, _u64p, _u8p, _u64p]
+        L.fame_match_pairs_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5 + [C.c_uint32, C.c_int, _u64p] + \
+            [C.c_void_p] * 5
         L.fame_decide_strand.argtypes = [C.c_void_p, _u64p, _u64p, C.POINTER(C.c_int)]
         L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
         L.fame_reset.argtypes = [C.c_void_p]
@@ -135,6 +138,13 @@ class Context:
                                             ptr(counters, _u64p), ptr(v1, _u8p), ptr(m1, _u64p),
                                             ptr(v2, _u8p), ptr(m2, _u64p)))
         return v1[:n], m1[:n], v2[:n], m2[:n]
+
+    def match_pairs_device(self, d_seqs, d_offs1, d_lens1, d_offs2, d_lens2, n, get_stranded, counters,
+                           d_valid1=0, d_mat1=0, d_valid2=0, d_mat2=0, stream=0):
+        self._check(load().fame_match_pairs_device(self.h, d_seqs, d_offs1, d_lens1, d_offs2, d_lens2, int(n),
+                                                   int(get_stranded), ptr(counters, _u64p), d_valid1 or None,
+                                                   d_mat1 or None, d_valid2 or None, d_mat2 or None,
+                                                   stream or None))
 
     def decide_strand(self):
         f = C.c_uint64()

@@ -864,6 +864,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     }
     uint64_t cnt[5];
     HIPC(hipMemcpyAsync(cnt, pd.counters, 40, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(c->stats, bd.stats, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (flags & FLAG_CAPACITY) {
@@ -1072,6 +1073,50 @@ int fame_match_pairs(fame_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, c
         if (n && valid2) HIPC(hipMemcpyAsync(valid2, c->b_valid.as<uint8_t>() + n, n, hipMemcpyDeviceToHost, st));
         if (n && mat1) HIPC(hipMemcpyAsync(mat1, c->b_mat.p, 8ull * n, hipMemcpyDeviceToHost, st));
         if (n && mat2) HIPC(hipMemcpyAsync(mat2, c->b_mat.as<uint64_t>() + n, 8ull * n, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_match_pairs_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs1, const uint32_t* d_lens1,
+                            const uint64_t* d_offs2, const uint32_t* d_lens2, uint32_t n, int get_stranded,
+                            uint64_t counters[5], uint8_t* d_valid1, uint64_t* d_mat1, uint8_t* d_valid2,
+                            uint64_t* d_mat2, void* stream)
+{
+    if (!c || !counters || (n && (!d_seqs || !d_offs1 || !d_lens1 || !d_offs2 || !d_lens2))) return FAME_ERR_ARG;
+    if ((uint32_t)std::ceil((float)c->readlen * 0.75) < 32) {
+        set_err(c, "paired matching needs ceil(0.75 READLEN) >= 32");
+        return FAME_ERR_FORMAT;
+    }
+    if (n > 0x7FFFFFFFu) return FAME_ERR_ARG;
+    return guarded(c, [&]() {
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        // the 2n mates as one read list: offsets and lengths of mate 1, then
+        // of mate 2 (device-to-device copies); both index the same d_seqs
+        uint64_t* d_o = grow<uint64_t>(c->b_offs, 2ull * n + 1);
+        uint32_t* d_l = grow<uint32_t>(c->b_lens, 2ull * n + 1);
+        uint64_t total = 0;
+        if (n) {
+            HIPC(hipMemcpyAsync(d_o, d_offs1, 8ull * n, hipMemcpyDeviceToDevice, st));
+            HIPC(hipMemcpyAsync(d_o + n, d_offs2, 8ull * n, hipMemcpyDeviceToDevice, st));
+            HIPC(hipMemcpyAsync(d_l, d_lens1, 4ull * n, hipMemcpyDeviceToDevice, st));
+            HIPC(hipMemcpyAsync(d_l + n, d_lens2, 4ull * n, hipMemcpyDeviceToDevice, st));
+            unsigned long long* d_end = grow<unsigned long long>(c->b_ovf_total, 1);
+            HIPC(hipMemsetAsync(d_end, 0, 8, st));
+            launch_max_end(d_o, d_l, 2 * n, d_end, st);
+            unsigned long long e = 0;
+            HIPC(hipMemcpyAsync(&e, d_end, 8, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            total = e;
+        }
+        int rc = run_pairs(c, d_seqs, d_o, d_l, n, total, get_stranded, counters, st);
+        if (rc != FAME_OK) return rc;
+        if (n && d_valid1) HIPC(hipMemcpyAsync(d_valid1, c->b_valid.p, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_valid2)
+            HIPC(hipMemcpyAsync(d_valid2, c->b_valid.as<uint8_t>() + n, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat1) HIPC(hipMemcpyAsync(d_mat1, c->b_mat.p, 8ull * n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat2)
+            HIPC(hipMemcpyAsync(d_mat2, c->b_mat.as<uint64_t>() + n, 8ull * n, hipMemcpyDeviceToDevice, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });

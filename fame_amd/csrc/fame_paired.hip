@@ -271,9 +271,12 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
 {
     const int lane = threadIdx.x & 63;
     {
-        const uint32_t hm = w.hsize - 1;
-        const int hshift = __clz(w.hsize) + 1;
-        for (uint32_t i = lane; i < w.hsize; i += 64) {
+        // a table of at least twice the entries (a power of two, >= 64)
+        uint32_t hs = 64;
+        while (hs < 2 * E && hs < w.hsize) hs <<= 1;
+        const uint32_t hm = hs - 1;
+        const int hshift = __clz(hs) + 1;
+        for (uint32_t i = lane; i < hs; i += 64) {
             w.hkey[i] = EMPTY;
             w.hfirst[i] = 0xFFFFFFFFu;
             w.hcnt[i] = 0;
@@ -493,6 +496,7 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
     w.tmp = (uint32_t*)take(4ull * pd.cap_e);
     w.over = (uint32_t*)take(4ull * pd.cap_e);
     PoolChunk pcA = {0, 0, 0};
+    uint64_t st_k = 0, st_e = 0, st_n = 0, st_b = 0;
     const uint64_t below = lanes_below(lane);
     const uint32_t nitems = pd.list ? pd.list_n : 2 * pd.n;
     for (uint32_t ti = blockIdx.x; ti < nitems; ti += gridDim.x) {
@@ -508,7 +512,13 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
             const uint32_t roB = 2 * (pd.n + i) + (1 - c);
             uint32_t bc, LA, LB, bcB;
             rc = pe_entries<TabT>(ix, bd, roA, sh, m, w, true, EA, bc, LA);
-            if (rc == 0) rc = pe_first<TabT>(w, m, EA, LA, bc);
+            if (rc == 0) {
+                st_k += LA - KMERLEN + 1;
+                st_e += EA;
+                st_b += LA;
+                ++st_n;
+                rc = pe_first<TabT>(w, m, EA, LA, bc);
+            }
             if (rc == 0 && (m.nmap[0] | m.nmap[1])) {
                 // the lookup table over the first read's windows
                 for (uint32_t x = lane; x < PE_MH; x += 64) m.hk[x] = EMPTY;
@@ -518,7 +528,12 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
                         for (uint32_t x = 0; x < m.nmap[s]; ++x) pe_put(m, m.mkey[s][x], s, x);
                 wave_sync();
                 rc = pe_entries<TabT>(ix, bd, roB, sh, m, w, false, EB, bcB, LB);
-                if (rc == 0) rc = pe_second<TabT>(w, m, EB, LB, (int)pd.cw);
+                if (rc == 0) {
+                    st_k += LB - KMERLEN + 1;
+                    st_e += EB;
+                    st_b += LB;
+                    rc = pe_second<TabT>(w, m, EB, LB, (int)pd.cw);
+                }
                 if (rc == 0) {
                     // flag2: windows with c1 >= QTHRESH and their present
                     // neighbours within contextWLen (:3125-3150); hasCpG
@@ -647,6 +662,12 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
         wave_sync();
     }
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
+    if (lane == 0 && st_n) {
+        atomicAdd(&bd.stats[0], (unsigned long long)st_k);
+        atomicAdd(&bd.stats[1], (unsigned long long)st_e);
+        atomicAdd(&bd.stats[2], (unsigned long long)st_n);
+        atomicAdd(&bd.stats[6], (unsigned long long)st_b);
+    }
 }
 
 // ---------------------------------------------------------------------------

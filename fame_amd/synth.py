@@ -48,6 +48,71 @@ def make_genome(chrom_lengths, seed=1, cpg_target=0.01):
     return chroms
 
 
+def add_repeat_families(chroms, frac=0.2, unit=300, divergence=0.10, copies_per_family=200, seed=5):
+    """Overwrite a fraction `frac` of the genome with copies of `unit`-bp
+    repeat families (in place).  Family consensi are segments of the genome
+    itself (same base and CpG model); every copy carries substitutions at
+    `divergence` of its positions relative to its consensus.  Copies occupy
+    whole `unit`-bp slots, so they never overlap.  Returns the number of
+    copies placed."""
+    rng = np.random.default_rng(seed)
+    slots = [c.size // unit for c in chroms]
+    total_slots = sum(slots)
+    ncopy = int(frac * total_slots)
+    nfam = max(1, ncopy // copies_per_family)
+    # consensi from random genome positions
+    cons = np.empty((nfam, unit), np.uint8)
+    for f in range(nfam):
+        c = chroms[int(rng.integers(0, len(chroms)))]
+        p = int(rng.integers(0, c.size - unit))
+        cons[f] = c[p:p + unit]
+    chosen = rng.choice(total_slots, size=ncopy, replace=False)
+    fam = rng.integers(0, nfam, size=ncopy)
+    first = np.zeros(len(chroms) + 1, np.int64)
+    first[1:] = np.cumsum(slots)
+    placed = 0
+    for ci, c in enumerate(chroms):
+        sel = (chosen >= first[ci]) & (chosen < first[ci + 1])
+        idx = chosen[sel] - first[ci]
+        if idx.size == 0:
+            continue
+        cp = cons[fam[sel]].copy()
+        mut = rng.random(cp.shape) < divergence
+        cp[mut] = _BASES[(_CODE[cp[mut]] + rng.integers(1, 4, size=int(mut.sum()))) % 4]
+        view = c[:slots[ci] * unit].reshape(slots[ci], unit)
+        view[idx] = cp
+        placed += idx.size
+    return placed
+
+
+def add_near_copies(chroms, segments=3000, seg_len=(1500, 4000), copies=(5, 50), divergence=0.01, seed=6):
+    """Copy `segments` genome segments 5-50 times each (in place) with
+    substitutions at `divergence`, so that reads from them have many
+    candidate windows.  Returns (chrom, start, end) arrays of every copy and
+    of the source segments, for sampling reads there (make_reads regions)."""
+    rng = np.random.default_rng(seed)
+    rc, rs, re_ = [], [], []
+    for _ in range(segments):
+        L = int(rng.integers(seg_len[0], seg_len[1] + 1))
+        src = int(rng.integers(0, len(chroms)))
+        p = int(rng.integers(0, chroms[src].size - L))
+        seg = chroms[src][p:p + L].copy()
+        rc.append(src)
+        rs.append(p)
+        re_.append(p + L)
+        for _k in range(int(rng.integers(copies[0], copies[1] + 1))):
+            dst = int(rng.integers(0, len(chroms)))
+            d = int(rng.integers(0, chroms[dst].size - L))
+            cp = seg.copy()
+            mut = rng.random(L) < divergence
+            cp[mut] = _BASES[(_CODE[cp[mut]] + rng.integers(1, 4, size=int(mut.sum()))) % 4]
+            chroms[dst][d:d + L] = cp
+            rc.append(dst)
+            rs.append(d)
+            re_.append(d + L)
+    return np.array(rc, np.int64), np.array(rs, np.int64), np.array(re_, np.int64)
+
+
 def write_fasta(path, chroms, names=None, width=80):
     with open(path, "wb") as f:
         for i, seq in enumerate(chroms):
@@ -127,13 +192,17 @@ for _i, _b in enumerate(b"ACGT"):
 
 def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_rate=0.0005,
                len_range=None, ob_fraction=0.5, batch=1 << 18, regions=None,
-               non_cpg_conv=0.995, threads=None):
+               non_cpg_conv=0.995, threads=None, origins=None, lengths=None):
     """Sample `n` bisulfite reads.
 
     read_len   nominal (maximum) length; with `len_range=(lo, hi)` each read's
                length is uniform in [lo, hi] (hi <= read_len).
     regions    optional (chrom_idx, start, end) arrays restricting the origin
                of every read (used for repeat-heavy mixes).
+    origins    optional (chrom_idx, pos, ob) arrays fixing every read's origin:
+               an OT read covers [pos, pos + L), an OB read is the reverse
+               complement of [pos + read_len + 8 - L, pos + read_len + 8);
+               lengths then gives each read's L.
     Batches are generated on a thread pool, each from its own seeded stream,
     so the output depends only on the arguments.
     """
@@ -157,7 +226,10 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
     def one(s):
         rng = np.random.default_rng([seed, s])
         m = min(batch, n - s)
-        if regions is None:
+        if origins is not None:
+            ci = np.asarray(origins[0][s:s + m], np.int64)
+            pos = np.asarray(origins[1][s:s + m], np.int64)
+        elif regions is None:
             ci = rng.choice(lens_c.size, size=m, p=lens_c / lens_c.sum())
             pos = (rng.random(m) * (lens_c[ci] - pad - 2)).astype(np.int64) + 1
         else:
@@ -167,7 +239,11 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
             span = np.maximum(re_[k] - rs[k] - pad - 2, 1)
             pos = rs[k] + (rng.random(m) * span).astype(np.int64) + 1
         ob = rng.random(m) < ob_fraction
-        if len_range is None:
+        if origins is not None:
+            ob = np.asarray(origins[2][s:s + m], bool)
+        if lengths is not None:
+            L = np.asarray(lengths[s:s + m], np.int64)
+        elif len_range is None:
             L = np.full(m, read_len, dtype=np.int64)
         else:
             L = rng.integers(len_range[0], len_range[1] + 1, size=m)
@@ -203,7 +279,7 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
         out = np.take_along_axis(read, np.minimum(src, pad - 1), axis=1)
         rows = np.flatnonzero(has & ~is_del[:, 0])
         out[rows, p_ev[rows, 0]] = _BASES[rng.integers(0, 4, size=rows.size)]
-        if len_range is not None:
+        if len_range is not None or lengths is not None:
             out[j >= L[:, None]] = 0
         seqs[s:s + m] = out
         lens[s:s + m] = L
@@ -218,52 +294,49 @@ def make_reads(chroms, n, read_len, seed=3, levels=None, sub_rate=0.005, indel_r
     return ReadSet(seqs, lens, quals, {"chrom": t_chrom, "pos": t_pos, "ob": t_ob})
 
 
-def make_pairs(chroms, n, read_len, gap=(20, 400), seed=3, len_range=None, block=1 << 20, **kw):
+def make_pairs(chroms, n, read_len, gap=(20, 400), seed=3, len_range=None, ob_fraction=0.5, **kw):
     """Sample `n` read pairs of a directional bisulfite library.
 
-    Each pair comes from one converted fragment of length L1 + g + L2
-    (g uniform in `gap`, mate lengths uniform in `len_range` or read_len):
-    mate 1 is the fragment's first L1 letters, mate 2 the reverse complement
-    of its last L2 letters.  The fragments are drawn by `make_reads` (same
-    bisulfite, error and strand model) in blocks, each truncated to its own
-    length, so the output depends only on the arguments.
-    Returns (ReadSet mate 1, ReadSet mate 2); truth["gap"] holds g.
+    A pair comes from a fragment of length L1 + g + L2 (g uniform in `gap`,
+    mate lengths uniform in `len_range`, else read_len) on the converted
+    original top (OT) or bottom (OB) strand: mate 1 is the fragment's first
+    L1 letters, mate 2 the reverse complement of its last L2 letters.  Both
+    mates are drawn by `make_reads` at fixed origins (same bisulfite, error
+    and CpG level model; the two ends of a fragment do not overlap, so their
+    conversions are independent).  Returns (ReadSet mate 1, ReadSet mate 2);
+    truth holds the fragment's chromosome, top-strand start, strand and g.
     """
-    W = 2 * read_len + gap[1]
-    out1 = np.zeros((n, read_len), np.uint8)
-    out2 = np.zeros((n, read_len), np.uint8)
-    l1 = np.zeros(n, np.uint32)
-    l2 = np.zeros(n, np.uint32)
-    truth = {"chrom": np.zeros(n, np.int32), "pos": np.zeros(n, np.int64),
-             "ob": np.zeros(n, bool), "gap": np.zeros(n, np.int64)}
+    rng = np.random.default_rng([seed, 17])
+    pad = read_len + 8
+    lens_c = np.array([c.size for c in chroms], dtype=np.int64)
+    if len_range is None:
+        l1 = np.full(n, read_len, np.int64)
+        l2 = np.full(n, read_len, np.int64)
+    else:
+        l1 = rng.integers(len_range[0], len_range[1] + 1, size=n)
+        l2 = rng.integers(len_range[0], len_range[1] + 1, size=n)
+    g = rng.integers(gap[0], gap[1] + 1, size=n)
+    F = l1 + g + l2
+    ci = rng.choice(lens_c.size, size=n, p=lens_c / lens_c.sum())
+    span = 2 * read_len + gap[1] + 2 * pad + 8
+    p = (rng.random(n) * (lens_c[ci] - span)).astype(np.int64) + pad + 2
+    ob = rng.random(n) < ob_fraction
+    e = p + F
+    # mate 1: OT -> [p, p + L1) top; OB -> revcomp of [e - L1, e)
+    pos1 = np.where(ob, e - pad, p)
+    # mate 2 = revcomp of X: OT X = converted top [e - L2, e); OB X = the OB
+    # read of [p, p + L2)
+    pos2 = np.where(ob, p + l2 - pad, e - l2)
+    a = make_reads(chroms, n, read_len, seed=seed * 7919 + 1, origins=(ci, pos1, ob), lengths=l1, **kw)
+    x = make_reads(chroms, n, read_len, seed=seed * 7919 + 2, origins=(ci, pos2, ob), lengths=l2, **kw)
+    m2 = np.zeros_like(x.seqs)
     j = np.arange(read_len)[None, :]
-    for b, s in enumerate(range(0, n, block)):
-        m = min(block, n - s)
-        fr = make_reads(chroms, m, W, seed=seed * 7919 + b, **kw)
-        rng = np.random.default_rng([seed, b, 17])
-        if len_range is None:
-            a1 = np.full(m, read_len, np.int64)
-            a2 = np.full(m, read_len, np.int64)
-        else:
-            a1 = rng.integers(len_range[0], len_range[1] + 1, size=m)
-            a2 = rng.integers(len_range[0], len_range[1] + 1, size=m)
-        g = rng.integers(gap[0], gap[1] + 1, size=m)
-        end = a1 + g + a2
-        m1 = fr.seqs[:, :read_len].copy()
-        m1[j >= a1[:, None]] = 0
-        # mate 2: letters [end-L2, end) of the fragment, reverse complemented,
-        # left-aligned in a read_len row
-        src = np.clip(end[:, None] - 1 - j, 0, W - 1)
-        m2 = _COMP[np.take_along_axis(fr.seqs, src, axis=1)]
-        m2[j >= a2[:, None]] = 0
-        out1[s:s + m] = m1
-        out2[s:s + m] = m2
-        l1[s:s + m] = a1
-        l2[s:s + m] = a2
-        for k in ("chrom", "pos", "ob"):
-            truth[k][s:s + m] = fr.truth[k]
-        truth["gap"][s:s + m] = g
-    return ReadSet(out1, l1, None, truth), ReadSet(out2, l2, None, dict(truth))
+    src = np.clip(l2[:, None] - 1 - j, 0, read_len - 1)
+    m2[:] = _COMP[np.take_along_axis(x.seqs, src, axis=1)]
+    m2[j >= l2[:, None]] = 0
+    truth = {"chrom": ci.astype(np.int32), "pos": p, "ob": ob, "gap": g}
+    return (ReadSet(a.seqs, l1.astype(np.uint32), None, truth),
+            ReadSet(m2, l2.astype(np.uint32), None, dict(truth)))
 
 
 def config_genome(name, scale=1.0, seed=1):

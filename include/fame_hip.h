@@ -95,6 +95,16 @@ int fame_match_pairs(fame_ctx* ctx, const uint8_t* seqs1, const uint64_t* offs1,
                      int get_stranded, uint64_t counters[5], uint8_t* valid1, uint64_t* mat1,
                      uint8_t* valid2, uint64_t* mat2);
 
+/* The same with every buffer in device memory: both mates index d_seqs
+ * (d_seqs[d_offs1[i] ..] is mate 1 of pair i, d_seqs[d_offs2[i] ..] its mate
+ * 2); results go to the device arrays (any may be NULL).  `stream` as for
+ * fame_match_reads_device.  No counterpart in the reference: the
+ * resident-input form of matchPairedReads used by benchmarks. */
+int fame_match_pairs_device(fame_ctx* ctx, const uint8_t* d_seqs, const uint64_t* d_offs1,
+                            const uint32_t* d_lens1, const uint64_t* d_offs2, const uint32_t* d_lens2,
+                            uint32_t n, int get_stranded, uint64_t counters[5], uint8_t* d_valid1,
+                            uint64_t* d_mat1, uint8_t* d_valid2, uint64_t* d_mat2, void* stream);
+
 /* ReadQueue::decideStrand ReadQueue.cpp:376-400 (without the console
  * report): matchR1Fwd = r1FwdMatches > r1RevMatches.  Any pointer may be
  * NULL. */
