@@ -1656,11 +1656,13 @@ __global__ void k_pack_kmers(const uint32_t* __restrict__ raw, const uint8_t* __
     out[g] = (m & 0x80000000u) | (meta << 1) | fwd; // isStartCpG in bit 31
 }
 
-// KMER_S::tmask of every entry (second word of kmerTableSmall)
-__global__ void k_pack_tmask(const uint32_t* __restrict__ raw, uint64_t n, uint32_t* __restrict__ out)
+// the paired path's entries: the packed kent word and KMER_S::tmask (second
+// word of kmerTableSmall) side by side
+__global__ void k_pack_tmask(const uint32_t* __restrict__ raw, const uint32_t* __restrict__ kent, uint64_t n,
+                             uint64_t* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = raw[2 * i + 1];
+    if (i < n) out[i] = (uint64_t)kent[i] | ((uint64_t)raw[2 * i + 1] << 32);
 }
 
 __global__ void k_narrow_tab(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out)
@@ -1699,10 +1701,10 @@ void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, ui
                        nsplit, minlen, 1);
 }
 
-void launch_pack_tmask(const uint32_t* raw, uint64_t n, uint32_t* out, hipStream_t st)
+void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_pack_tmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, n, out);
+    hipLaunchKernelGGL(k_pack_tmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, kent, n, out);
 }
 
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)

@@ -78,7 +78,7 @@ struct fame_ctx {
     std::vector<uint8_t> cpg_chrom;
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
-    DevBuf d_tab, d_kent, d_ktm, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
+    DevBuf d_tab, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
@@ -98,12 +98,13 @@ struct fame_ctx {
         b2_job_match[FAME_NPOOL], b2_job_ovf_off[FAME_NPOOL];
     DevBuf b2_ovf_n, b2_ovf_pool, b2_ovf_job, b2_ovf_buf, b2_ovf_total;
     DevBuf b_acc1[FAME_NPOOL], b_acc2[FAME_NPOOL];
-    DevBuf b_pe_off, b_pe_nf, b_pe_nr, b_pe_m1, b_pe_mapn, b_pe_map, b_pe_srt, b_pe_scratch, b_pe_big;
+    DevBuf b_pe_off, b_pe_nf, b_pe_nr, b_pe_m1, b_pe_mapn, b_pe_map, b_pe_srt, b_pe_big, b_pe_list;
     uint32_t pool2_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t pe_map_cap = 0;
     uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
     double timings[6] = {0, 0, 0, 0, 0, 0};
     uint64_t last_heavy = 0, last_ovf = 0;
+    double big_ms = 0; // paired seed runs with global scratch, last call
 };
 
 namespace {
@@ -256,7 +257,7 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(hipMemcpy(sbits.p, bits.data(), bits.size(), hipMemcpyHostToDevice));
     }
     HIPC(c->d_kent.reserve(4 * c->nkmer + 4));
-    HIPC(c->d_ktm.reserve(4 * c->nkmer + 4));
+    HIPC(c->d_kpe.reserve(8 * c->nkmer + 8));
     {
         DevBuf stage;
         HIPC(stage.reserve(kStage));
@@ -268,7 +269,8 @@ void load_index(fame_ctx* c, const std::string& path)
             f.read(buf.data(), 8 * n);
             HIPC(hipMemcpy(stage.p, buf.data(), 8 * n, hipMemcpyHostToDevice));
             launch_pack_kmers(stage.as<uint32_t>(), sbits.as<uint8_t>(), first, n, c->d_kent.as<uint32_t>(), st);
-            launch_pack_tmask(stage.as<uint32_t>(), n, c->d_ktm.as<uint32_t>() + first, st);
+            launch_pack_tmask(stage.as<uint32_t>(), c->d_kent.as<uint32_t>() + first, n,
+                              c->d_kpe.as<uint64_t>() + first, st);
             HIPC(hipStreamSynchronize(st));
         }
     }
@@ -327,7 +329,7 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.hpow2 = (c->htab & (c->htab - 1)) == 0;
     ix.hmask = c->htab - 1;
     ix.kent = c->d_kent.as<uint32_t>();
-    ix.ktm = c->d_ktm.as<uint32_t>();
+    ix.kpe = c->d_kpe.as<uint64_t>();
     ix.filt32 = c->d_filt32.as<uint32_t>();
     ix.win = c->d_win.as<uint4>();
     ix.seq = c->d_seq.as<uint8_t>();
@@ -625,7 +627,8 @@ int verify_phase(fame_ctx* c, BatchDev& bd, const uint32_t pool_n[FAME_NPOOL], D
     return FAME_OK;
 }
 
-// scratch slice of the paired seed stage for reads of at most cap_e entries
+// global scratch slice of the paired seed stage for reads of at most cap_e
+// index hits (the order of k_pe_seed_big's carving)
 void pe_scratch_dims(PairDev& pd, uint32_t cap_e)
 {
     auto rup = [](uint64_t v) {
@@ -635,7 +638,12 @@ void pe_scratch_dims(PairDev& pd, uint32_t cap_e)
     };
     pd.cap_e = cap_e;
     pd.hsize = (uint32_t)rup(2ull * cap_e);
-    pd.tb_cap = (uint32_t)(2 * rup(2ull * cap_e) + 64);
+    // a strand map holds at most the first read's windows (<= E / QTHRESH)
+    // and the windows the second read's entries create (<= E)
+    pd.mcap = cap_e + cap_e / 5 + 64;
+    pd.cap_t = pd.mcap;
+    pd.tb_cap = (uint32_t)(2 * rup(2ull * pd.mcap) + 64);
+    pd.mhsz = (uint32_t)rup(2ull * (2ull * pd.mcap + cap_e));
     size_t stride = 0;
     auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
     add(8 * 256);
@@ -650,8 +658,11 @@ void pe_scratch_dims(PairDev& pd, uint32_t cap_e)
     add(8ull * cap_e);
     add(4ull * cap_e);
     add(4ull * pd.tb_cap);
-    add(4ull * cap_e);
-    add(4ull * cap_e);
+    add(4ull * pd.cap_t);
+    add(4ull * pd.cap_t);
+    add(8ull * pd.mcap);
+    add(8ull * pd.mcap);
+    for (int i = 0; i < 4; ++i) add(4ull * pd.mhsz);
     pd.stride = stride;
 }
 
@@ -666,6 +677,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     const uint32_t nitems = 2 * n;
     c->last_ovf = 0;
     c->last_heavy = 0;
+    c->big_ms = 0;
     BatchDev bd{};
     bd.seqs = d_seqs;
     bd.offs = d_offs;
@@ -678,7 +690,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.codes = grow<uint8_t>(c->b_codes, (size_t)2 * nr * 256 + 256);
     bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
     bd.match_cap = c->match_cap;
-    const uint32_t seed_grid = std::min<uint32_t>(std::max<uint32_t>(nitems, 1), 4096);
+    const uint32_t seed_grid = std::min<uint32_t>(std::max<uint32_t>(nitems, 1), 8192);
     const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n) + 64u * seed_grid;
     for (int p = 0; p < FAME_NPOOL; ++p) c->pool_cap[p] = std::max(c->pool_cap[p], want_cap);
     bd.ovf_n = grow<uint32_t>(c->b_ovf_n, 1);
@@ -720,8 +732,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     pd.big_cap = std::max<uint32_t>(1024, nitems);
     pd.big_item = grow<uint32_t>(c->b_pe_big, 2ull * pd.big_cap);
     pd.big_e = pd.big_item + pd.big_cap;
-    pe_scratch_dims(pd, 2048);
-    pd.nslices = seed_grid;
+    pd.nslices = 0;
     const uint32_t minlen = (uint32_t)std::ceil((float)c->readlen * 0.75);
 
     tm.mark(0);
@@ -739,8 +750,6 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         pd.map_cap = c->pe_map_cap;
         pd.map = grow<PeItem>(c->b_pe_map, pd.map_cap);
         pd.srt = grow<PeItem>(c->b_pe_srt, pd.map_cap);
-        pe_scratch_dims(pd, 2048);
-        pd.scratch = grow<uint8_t>(c->b_pe_scratch, pd.stride * pd.nslices);
         pd.list = nullptr;
         pd.list_n = 0;
         HIPC(hipMemsetAsync(bd.pool_n, 0, 4 * FAME_NPOOL, st));
@@ -760,29 +769,48 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             set_err(c, "paired seed overflow list full");
             return FAME_ERR_CAPACITY;
         }
-        const uint32_t big_n = cnt2[1];
+        // items beyond the LDS kernel: global-scratch runs, each sized for
+        // the largest listed read, whose own overflows are listed again
+        // with twice the capacity
+        uint32_t big_n = cnt2[1];
         c->last_heavy = big_n;
-        if (big_n && !(flags & FLAG_POOL_FULL)) {
-            std::vector<uint32_t> be(big_n);
+        uint64_t cap_prev = 0;
+        for (int round = 0; big_n && !(flags & FLAG_POOL_FULL); ++round) {
+            if (round >= 16) {
+                flags |= FLAG_CAPACITY;
+                break;
+            }
+            std::vector<uint32_t> be(big_n), bi(big_n);
             HIPC(hipMemcpy(be.data(), pd.big_e, 4 * big_n, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(bi.data(), pd.big_item, 4 * big_n, hipMemcpyDeviceToHost));
             uint64_t emax = 1;
             for (uint32_t x = 0; x < big_n; ++x) emax = std::max<uint64_t>(emax, be[x]);
+            // the first run is sized for the bulk (<= 1024 hits, many
+            // slices); what still overflows goes to larger runs
+            emax = std::min<uint64_t>(emax, std::max<uint64_t>(1024, 2 * cap_prev));
+            emax = std::max<uint64_t>(emax, std::max<uint64_t>(2 * cap_prev, 256));
             if (emax > (1ull << 24)) {
                 set_err(c, "a mate has more than 2^24 index hits");
                 return FAME_ERR_CAPACITY;
             }
+            cap_prev = emax;
             PairDev pb = pd;
             pe_scratch_dims(pb, (uint32_t)emax);
             const size_t budget = 2ull << 30;
             pb.nslices = (uint32_t)std::max<size_t>(1, std::min<size_t>(big_n, budget / pb.stride));
             pb.scratch = grow<uint8_t>(c->b_heavy_scratch, pb.stride * pb.nslices);
-            pb.list = pd.big_item;
+            uint32_t* lst = grow<uint32_t>(c->b_pe_list, big_n + 1);
+            HIPC(hipMemcpyAsync(lst, bi.data(), 4 * big_n, hipMemcpyHostToDevice, st));
+            pb.list = lst;
             pb.list_n = big_n;
+            HIPC(hipMemsetAsync(pd.big_n, 0, 4, st));
             tm.mark(12);
             launch_pe_seed(c->ix, bd, pb, pb.nslices, st);
             tm.mark(13);
             HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(&big_n, pd.big_n, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
+            c->big_ms += tm.ms(12, 13);
         }
         if (flags & FLAG_CAPACITY) {
             set_err(c, "a pair's candidate maps exceeded the device emulation limits");
@@ -873,7 +901,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     }
     for (int k = 0; k < 5; ++k) counters[k] += cnt[k];
     c->timings[0] = tm.ms(0, 1);
-    c->timings[1] = tm.ms(2, 3) + (c->last_heavy ? tm.ms(12, 13) : 0.0);
+    c->timings[1] = tm.ms(2, 3) + c->big_ms;
     c->timings[2] = tm.ms(4, 5) + tm.ms(8, 9);
     c->timings[3] = tm.ms(6, 7) + tm.ms(10, 11);
     c->timings[4] = tm.ms(14, 15);

@@ -32,7 +32,8 @@ enum : uint32_t {
 //   kent     one u32 per k-mer table entry: isStart << 31 | metaId << 1 |
 //            forward-strand bit (KMER_S::getMetaCpG / isStartCpG +
 //            strandTable, structs.h:118-151); metaId < 2^30
-//   ktm      KMER_S::tmask of every entry (read by the paired-end path)
+//   kpe      the paired-end path's copy: kent | KMER_S::tmask << 32 (one
+//            gather per entry)
 //   filt32   the filteredKmers values below 2^32, sorted (the paired path
 //            looks up a 32-bit k-mer code there, ReadQueue.cpp:2527)
 //   win      metaWindow as {startPos, chrom, startInd, endInd}
@@ -44,7 +45,7 @@ struct DevIndex {
     uint64_t htab, hmask;
     int hpow2;
     const uint32_t* kent;
-    const uint32_t* ktm;
+    const uint64_t* kpe;
     const uint32_t* filt32;
     uint32_t nfilt32;
     const uint4* win;
@@ -137,12 +138,12 @@ struct PairDev {
     uint8_t* acc1[FAME_NPOOL]; // per first-read verify slot: 1 walked | 2 first match dropped
     uint8_t* acc2[FAME_NPOOL]; // per second-read verify slot
     uint64_t* counters;        // succ, nonunique, unsucc, succPaired, tooShort
-    // seed scratch: one slice per block; items that outgrow it are listed
-    // and re-run with larger slices
+    // items that outgrow the LDS seed kernel are listed and re-run with one
+    // global scratch slice per block
     uint8_t* scratch;
     size_t stride;
     uint32_t nslices;
-    uint32_t cap_e, hsize, tb_cap;
+    uint32_t cap_e, hsize, tb_cap, cap_t, mcap, mhsz;
     const uint32_t* list; // NULL: every item; else the items to run
     uint32_t list_n;
     uint32_t* big_n;
@@ -176,7 +177,7 @@ void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, u
 void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st);
 void launch_pe_final(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd,
                      int get_stranded, hipStream_t st);
-void launch_pack_tmask(const uint32_t* raw, uint64_t n, uint32_t* out, hipStream_t st);
+void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st);
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);

@@ -39,28 +39,23 @@ namespace fame {
 constexpr uint32_t KMERDIST = 10;      // CONST.h:88
 constexpr uint32_t PF2 = 1u << 16;     // map value tuple<u8 c0, u8 c1, bool, bool>: the two flags
 constexpr uint32_t PF3 = 1u << 17;
-constexpr uint32_t PE_MAPCAP = 256;    // windows per strand map of one item (LDS)
-constexpr uint32_t PE_MH = 1024;       // lookup slots for both maps
 constexpr uint32_t PE_NONE = 0xFFFFFFFFu;
 constexpr int PE_KPL = (FAME_DEV_MAX_READ - (int)KMERLEN + 1 + 63) / 64; // k-mers per lane
 
 __device__ __forceinline__ uint32_t pe_c0(uint32_t v) { return v & 0xFFu; }
 __device__ __forceinline__ uint32_t pe_c1(uint32_t v) { return (v >> 8) & 0xFFu; }
 
-// per-block LDS: the item's two small maps (keys, values, a lookup table)
-// and the letters of the read being seeded
+// per-block LDS that every variant keeps there: the read's reduced letters,
+// its C letters and 2-bit codes, the map sizes
 struct PeLds {
-    uint32_t mkey[2][PE_MAPCAP];
-    uint32_t mval[2][PE_MAPCAP];
-    uint32_t hk[PE_MH]; // key << 1 | strand, EMPTY when free
-    uint32_t hv[PE_MH]; // index in that strand's arrays
     uint32_t nmap[2];
     uint32_t cbits[9];  // bit p: letter p of the read is C
     uint32_t kbits[17]; // 2-bit letter codes (C, T 3; G 2; others 0), 16 per word
     alignas(8) uint8_t code[256];
 };
 
-// per-block global scratch of the seed stage
+// working arrays of the seed stage: in LDS for reads of at most PE_LE index
+// hits (k_pe_seed light), in per-block global scratch otherwise
 template <typename TabT>
 struct PeWork {
     TabT* klo;
@@ -70,9 +65,9 @@ struct PeWork {
     uint8_t* kfl;   // k-mer skipped by the filteredKmers lookup
     uint32_t* ent;  // [cap_e] kent of each bucket entry
     uint8_t* ekm;   // [cap_e] its k-mer
-    uint32_t* eaux; // [cap_e] bit 0 kept; first / count after counting
+    uint32_t* eaux; // [cap_e] bit 0 kept; first / count, or lookup slot
     uint32_t cap_e;
-    uint32_t* hkey;
+    uint32_t* hkey; // counting table (first read)
     uint32_t* hfirst;
     uint32_t* hcnt;
     uint32_t hsize;
@@ -80,35 +75,71 @@ struct PeWork {
     uint16_t* ins_cnt; // [2][cap_e]
     uint32_t* tb;
     uint32_t tb_cap;
-    uint32_t* tmp;
-    uint32_t* over;
+    uint32_t* tmp;  // [cap_t]
+    uint32_t* over; // [cap_t]
+    uint32_t cap_t;
+    // the item's two strand maps, keys in insertion order
+    uint32_t* mkey; // [2][mcap]
+    uint32_t* mval; // [2][mcap]
+    uint32_t mcap;
+    // lookup over every window the item touches: key << 1 | strand, the
+    // insertion time (0: first-read window, j + 1: inserted at second-read
+    // entry j, PE_NEVER), the second read's count, the map index
+    uint32_t* hk;
+    uint32_t* ht;
+    uint32_t* hc;
+    uint32_t* hi;
+    uint32_t mhsz; // power of two
 };
 
-__device__ __forceinline__ uint32_t pe_slot(uint32_t k2) { return (k2 * 2654435761u) >> 22; } // 10 bits
+constexpr uint32_t PE_NEVER = 0xFFFFFFFFu;
+constexpr uint32_t PE_LE = 256; // index hits per read, light variant
+constexpr uint32_t PE_LM = 64;  // windows per strand map, light variant
+constexpr uint32_t PE_LH = 512; // lookup slots, light variant
+constexpr uint32_t PE_LTB = 640;
 
-// index of (key, strand) in the LDS maps, or PE_NONE
-__device__ __forceinline__ uint32_t pe_find(const PeLds& m, uint32_t key, uint32_t s)
+template <typename TabT>
+__device__ __forceinline__ uint32_t pe_slot(const PeWork<TabT>& w, uint32_t k2)
+{
+    return (k2 * 2654435761u) >> (__clz(w.mhsz) + 1);
+}
+
+// lookup slot of (key, strand), or PE_NONE
+template <typename TabT>
+__device__ __forceinline__ uint32_t pe_lookup(const PeWork<TabT>& w, uint32_t key, uint32_t s)
 {
     if (key >= 0x40000000u) return PE_NONE; // metaId - u below 0 wraps
     const uint32_t k2 = (key << 1) | s;
-    uint32_t h = pe_slot(k2);
-    for (uint32_t probe = 0; probe < PE_MH; ++probe) {
-        const uint32_t x = m.hk[h];
-        if (x == k2) return m.hv[h];
+    uint32_t h = pe_slot(w, k2);
+    for (uint32_t probe = 0; probe < w.mhsz; ++probe) {
+        const uint32_t x = w.hk[h];
+        if (x == k2) return h;
         if (x == EMPTY) return PE_NONE;
-        h = (h + 1) & (PE_MH - 1);
+        h = (h + 1) & (w.mhsz - 1);
     }
     return PE_NONE;
 }
 
-// one lane inserts a key known to be absent
-__device__ __forceinline__ void pe_put(PeLds& m, uint32_t key, uint32_t s, uint32_t idx)
+// index of (key, strand) in its strand map, or PE_NONE
+template <typename TabT>
+__device__ __forceinline__ uint32_t pe_find(const PeWork<TabT>& w, uint32_t key, uint32_t s)
+{
+    const uint32_t h = pe_lookup(w, key, s);
+    return h == PE_NONE ? PE_NONE : w.hi[h];
+}
+
+// the slot of (key, strand), created (time PE_NEVER, not in a map) when
+// absent; any lane, concurrently
+template <typename TabT>
+__device__ __forceinline__ uint32_t pe_claim(const PeWork<TabT>& w, uint32_t key, uint32_t s)
 {
     const uint32_t k2 = (key << 1) | s;
-    uint32_t h = pe_slot(k2);
-    while (m.hk[h] != EMPTY) h = (h + 1) & (PE_MH - 1);
-    m.hk[h] = k2;
-    m.hv[h] = idx;
+    uint32_t h = pe_slot(w, k2);
+    for (;;) {
+        const uint32_t old = atomicCAS(&w.hk[h], EMPTY, k2);
+        if (old == EMPTY || old == k2) return h;
+        h = (h + 1) & (w.mhsz - 1);
+    }
 }
 
 __device__ __forceinline__ bool pe_filtered(const DevIndex& ix, uint32_t code)
@@ -231,8 +262,9 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
         uint32_t k = a;
         while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k;
         const uint64_t g = (uint64_t)w.klo[k] + (j - w.koff[k]);
-        const uint32_t e = ix.kent[g];
-        const bool filt = w.kfl[k] || (w.kcm[k] & SEEDBITS & ix.ktm[g]) != 0;
+        const uint64_t kp = ix.kpe[g]; // kent | tmask << 32
+        const uint32_t e = (uint32_t)kp;
+        const bool filt = w.kfl[k] || (w.kcm[k] & SEEDBITS & (uint32_t)(kp >> 32)) != 0;
         w.ent[j] = first ? e : (e & 0x7FFFFFFFu);
         w.ekm[j] = (uint8_t)k;
         w.eaux[j] = filt ? 0u : 1u;
@@ -264,8 +296,8 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
 
 // getSeedRefsFirstRead's counting (:2556-2640) and the copy of the windows
 // with c0 >= QTHRESH, in the iteration order of the map sized by the bucket
-// count, into the LDS maps (:2760-2790).  Returns 1 on a capacity overflow
-// (retry with larger scratch), 2 when the LDS maps are too small.
+// count, into the item's maps (:2760-2790).  Returns 1 on a capacity
+// overflow (retry with larger scratch).
 template <typename TabT>
 __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L, uint32_t bucketCount)
 {
@@ -345,12 +377,12 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
         const bool q0 = t0 && cnt >= QTHRESH, q1 = t1 && cnt >= QTHRESH;
         const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
         if (q0 && nq[0] == 0 && (c0 & below) == 0) {
-            m.mkey[0][0] = key;
-            m.mval[0][0] = cnt;
+            w.mkey[0] = key;
+            w.mval[0] = cnt;
         }
         if (q1 && nq[1] == 0 && (c1 & below) == 0) {
-            m.mkey[1][0] = key;
-            m.mval[1][0] = cnt;
+            w.mkey[w.mcap] = key;
+            w.mval[w.mcap] = cnt;
         }
         nq[0] += __popcll(c0);
         nq[1] += __popcll(c1);
@@ -359,7 +391,7 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
     wave_sync();
 #pragma unroll
     for (int mp = 0; mp < 2; ++mp) {
-        if (nq[mp] > PE_MAPCAP) return 2;
+        if (nq[mp] > w.mcap) return 1;
         if (nq[mp] <= 1) {
             if (lane == 0) m.nmap[mp] = nq[mp];
             continue;
@@ -387,8 +419,8 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
             const uint64_t bal = __ballot(keep);
             if (keep) {
                 const uint32_t p = nc + __popcll(bal & below);
-                m.mkey[mp][p] = h.key[i];
-                m.mval[mp][p] = cnt[i];
+                w.mkey[mp * w.mcap + p] = h.key[i];
+                w.mval[mp * w.mcap + p] = cnt[i];
             }
             nc += __popcll(bal);
         }
@@ -399,61 +431,109 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
     return 0;
 }
 
-// getSeedRefsSecondRead's counting (:2646-3120): kept entries in index
-// order; a window is credited when it or a window within contextWLen is in
-// its strand's map, which inserts it (operator[]), so later entries see it.
-// Entries from k-mers past maxQPos only credit windows already present.
-// The order matters (inserts enable later credits), so the wave walks the
-// entries one at a time; each step's neighbour lookups run on 2cw+1 lanes.
-// Returns 2 when a map outgrows the LDS arrays.
+// getSeedRefsSecondRead's counting (:2646-3120).  In the reference the kept
+// entries are visited in index order: an entry from a k-mer that may insert
+// (k-mer 0, or cIdx < maxQPos) credits its window when the window or one
+// within contextWLen is in its strand's map, creating the window through
+// operator[] if needed; other entries credit only windows already present.
+// Let T(w) be the entry at which window w enters the map (0 for the first
+// read's windows).  Then T(w) = the first inserting entry j of w with a
+// neighbour v, T(v) < j + 1, and w is credited by exactly the entries j with
+// T(w) <= j + 1.  T is found as a fixpoint: every round lowers T(w) for
+// entries whose neighbour is present before them; starting from "never",
+// the values stay at or above the sequential ones and a round without a
+// change leaves exactly them (a first difference would have a witness
+// neighbour with an earlier difference).  Rounds = the longest chain of
+// windows entering through each other, usually one.  Returns 1 on overflow.
 template <typename TabT>
 __device__ int pe_second(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L, int cw)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
-    const int span = 2 * cw + 1;
+    // every kept entry's window gets a slot
+    for (uint32_t j = lane; j < E; j += 64) {
+        if (!(w.eaux[j] & 1)) continue;
+        const uint32_t e = w.ent[j];
+        const uint32_t h = pe_claim(w, (e >> 1) & 0x3FFFFFFFu, (e & 1) ? 0u : 1u);
+        w.eaux[j] = 1u | (h << 8);
+    }
+    wave_sync();
+    for (int round = 0;; ++round) {
+        bool changed = false;
+        for (uint32_t j = lane; j < E; j += 64) {
+            const uint32_t a = w.eaux[j];
+            if (!(a & 1)) continue;
+            const uint32_t km = w.ekm[j];
+            if (!(km == 0 || (km - 1) < maxQPos)) continue;
+            const uint32_t h = a >> 8;
+            if (w.ht[h] <= j + 1) continue;
+            const uint32_t e = w.ent[j];
+            const uint32_t key = (e >> 1) & 0x3FFFFFFFu, s = (e & 1) ? 0u : 1u;
+            bool nb = false;
+            for (int o = -cw; o <= cw && !nb; ++o) {
+                if (o == 0) continue;
+                const uint32_t q = pe_lookup(w, key + (uint32_t)o, s);
+                nb = q != PE_NONE && w.ht[q] < j + 1;
+            }
+            if (nb && atomicMin(&w.ht[h], j + 1) > j + 1) changed = true;
+        }
+        wave_sync();
+        if (!__ballot(changed)) break;
+    }
+    // credits
+    for (uint32_t j = lane; j < E; j += 64) {
+        const uint32_t a = w.eaux[j];
+        if (!(a & 1)) continue;
+        const uint32_t h = a >> 8;
+        if (w.ht[h] <= j + 1) atomicAdd(&w.hc[h], 1u);
+    }
+    wave_sync();
+    // windows entering the maps, in order of entry (each entry creates at
+    // most one window)
+    uint32_t n0 = m.nmap[0], n1 = m.nmap[1];
+    const uint64_t below = lanes_below(lane);
     for (uint32_t j0 = 0; j0 < E; j0 += 64) {
         const uint32_t j = j0 + lane;
-        uint32_t e = 0, km = 0;
-        bool kept = false;
+        bool i0 = false, i1 = false;
+        uint32_t h = 0, key = 0;
         if (j < E) {
-            kept = (w.eaux[j] & 1) != 0;
-            e = w.ent[j];
-            km = w.ekm[j];
-        }
-        uint64_t kb = __ballot(kept);
-        while (kb) {
-            const int src = __ffsll((long long)kb) - 1;
-            kb &= kb - 1;
-            const uint32_t ee = __shfl(e, src, 64), kk = __shfl(km, src, 64);
-            const uint32_t key = (ee >> 1) & 0x3FFFFFFFu, s = (ee & 1) ? 0u : 1u;
-            const bool elig = kk == 0 || (kk - 1) < maxQPos;
-            uint32_t idx = PE_NONE;
-            bool found = false;
-            if (lane < span) {
-                idx = pe_find(m, key + (uint32_t)(lane - cw), s);
-                found = idx != PE_NONE;
-            }
-            const uint64_t fb = __ballot(found);
-            uint32_t self = __shfl(idx, cw, 64);
-            if (elig && fb != 0 && self == PE_NONE) {
-                const uint32_t n0 = m.nmap[s];
-                if (n0 >= PE_MAPCAP) return 2;
-                if (lane == 0) {
-                    m.mkey[s][n0] = key;
-                    m.mval[s][n0] = 0;
-                    pe_put(m, key, s, n0);
-                    m.nmap[s] = n0 + 1;
+            const uint32_t a = w.eaux[j];
+            if (a & 1) {
+                h = a >> 8;
+                if (w.ht[h] == j + 1) {
+                    const uint32_t e = w.ent[j];
+                    key = (e >> 1) & 0x3FFFFFFFu;
+                    i0 = (e & 1) != 0;
+                    i1 = !i0;
                 }
-                self = n0;
             }
-            if (self != PE_NONE && lane == 0) {
-                const uint32_t v = m.mval[s][self];
-                m.mval[s][self] = (v & ~0xFF00u) | ((v + 0x100u) & 0xFF00u); // u8 c1
-            }
-            wave_sync();
         }
+        const uint64_t b0 = __ballot(i0), b1 = __ballot(i1);
+        if (n0 + __popcll(b0) > w.mcap || n1 + __popcll(b1) > w.mcap) return 1;
+        if (i0 || i1) {
+            const uint32_t x = i0 ? n0 + __popcll(b0 & below) : n1 + __popcll(b1 & below);
+            const uint32_t sm = i0 ? 0u : w.mcap;
+            w.mkey[sm + x] = key;
+            w.mval[sm + x] = 0;
+            w.hi[h] = x;
+        }
+        n0 += __popcll(b0);
+        n1 += __popcll(b1);
     }
+    wave_sync();
+    if (lane == 0) {
+        m.nmap[0] = n0;
+        m.nmap[1] = n1;
+    }
+    // c1 into the map values (u8)
+    for (uint32_t h = lane; h < w.mhsz; h += 64) {
+        const uint32_t k2 = w.hk[h];
+        if (k2 == EMPTY) continue;
+        const uint32_t x = w.hi[h];
+        if (x == PE_NONE) continue;
+        w.mval[(k2 & 1) * w.mcap + x] |= (w.hc[h] & 0xFFu) << 8;
+    }
+    wave_sync();
     return 0;
 }
 
@@ -461,8 +541,271 @@ __device__ int pe_second(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L
 // k_pe_seed
 // ---------------------------------------------------------------------------
 
+// one item (pair combination) on one wavefront
+template <typename TabT>
+__device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, const SeedShared& sh, PeLds& m,
+                        const PeWork<TabT>& w, uint32_t t, PoolChunk& pcA, uint64_t (&st)[4])
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lanes_below(lane);
+    const uint32_t i = t >> 1, c = t & 1;
+    uint32_t nf = 0, nr = 0, off = 0;
+    // pairs with an invalid mate are counted by k_pe_final only
+    const bool on = ((pd.combos >> c) & 1) && bd.pre[i] == 0 && bd.pre[pd.n + i] == 0;
+    int rc = 0;
+    uint32_t EA = 0, EB = 0;
+    if (on) {
+        const uint32_t roA = 2 * i + c;
+        const uint32_t roB = 2 * (pd.n + i) + (1 - c);
+        uint32_t bc, LA, LB, bcB;
+        if (lane < 2) m.nmap[lane] = 0;
+        rc = pe_entries<TabT>(ix, bd, roA, sh, m, w, true, EA, bc, LA);
+        if (rc == 0) {
+            st[0] += LA - KMERLEN + 1;
+            st[1] += EA;
+            st[3] += LA;
+            ++st[2];
+            rc = pe_first<TabT>(w, m, EA, LA, bc);
+        }
+        if (rc == 0 && (m.nmap[0] | m.nmap[1])) {
+            // the lookup table, holding the first read's windows (time 0)
+            for (uint32_t x = lane; x < w.mhsz; x += 64) {
+                w.hk[x] = EMPTY;
+                w.ht[x] = PE_NEVER;
+                w.hc[x] = 0;
+                w.hi[x] = PE_NONE;
+            }
+            wave_sync();
+            for (uint32_t s = 0; s < 2; ++s)
+                for (uint32_t x = lane; x < m.nmap[s]; x += 64) {
+                    const uint32_t h = pe_claim(w, w.mkey[s * w.mcap + x], s);
+                    w.ht[h] = 0;
+                    w.hi[h] = x;
+                }
+            wave_sync();
+            rc = pe_entries<TabT>(ix, bd, roB, sh, m, w, false, EB, bcB, LB);
+            if (rc == 0) {
+                st[0] += LB - KMERLEN + 1;
+                st[1] += EB;
+                st[3] += LB;
+                rc = pe_second<TabT>(w, m, EB, LB, (int)pd.cw);
+            }
+            if (rc == 0) {
+                // flag2: windows with c1 >= QTHRESH and their present
+                // neighbours within contextWLen (:3125-3150); hasCpG
+                const uint32_t n0 = m.nmap[0], ntot = n0 + m.nmap[1];
+                for (uint32_t x = lane; x < ntot; x += 64) {
+                    const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
+                    uint32_t* mv = w.mval + s * w.mcap;
+                    if (pe_c1(mv[y]) < QTHRESH) continue;
+                    const uint32_t key = w.mkey[s * w.mcap + y];
+                    atomicOr(&mv[y], PF2);
+                    for (int u = 1; u <= (int)pd.cw; ++u) {
+                        uint32_t q = pe_find(w, key + (uint32_t)u, s);
+                        if (q != PE_NONE) atomicOr(&mv[q], PF2);
+                        q = pe_find(w, key - (uint32_t)u, s);
+                        if (q != PE_NONE) atomicOr(&mv[q], PF2);
+                    }
+                }
+                wave_sync();
+                bool cpg = false;
+                uint32_t nj = 0;
+                for (uint32_t x = lane; x < ntot; x += 64) {
+                    const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
+                    const uint32_t v = w.mval[s * w.mcap + y];
+                    if ((v & PF2) && ix.win[w.mkey[s * w.mcap + y]].z != CPGDUMMY) cpg = true;
+                    if ((v & PF2) && pe_c0(v) >= QTHRESH) ++nj;
+                }
+                const bool hasCpG = __ballot(cpg) != 0;
+                nj = wave_sum_u32(nj);
+                if (hasCpG) {
+                    nf = m.nmap[0];
+                    nr = m.nmap[1];
+                    if (lane == 0) off = atomicAdd(pd.map_n, nf + nr);
+                    off = __shfl(off, 0, 64);
+                    if (off + nf + nr > pd.map_cap) {
+                        if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
+                        nf = nr = 0;
+                    }
+                }
+                if (nf + nr) {
+                    const uint32_t pool = (LA + 63) / 64 - 1;
+                    uint32_t jb = nj ? pool_take(bd, pcA, pool, nj, false, lane) : 0;
+                    if (jb == ~0u) {
+                        if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
+                        nf = nr = 0;
+                    } else {
+                        uint32_t* jm = pc_meta(bd, pool);
+                        uint32_t* jr = bd.job_ro[0];
+#pragma unroll
+                        for (uint32_t q = 1; q < FAME_NPOOL; ++q) jr = pool == q ? bd.job_ro[q] : jr;
+                        // iteration order of the two 20-bucket maps (:2760-2790
+                        // build them with 20 buckets, :2801-3164 insert into them)
+                        uint32_t pos = off;
+                        for (uint32_t s = 0; s < 2 && rc == 0; ++s) {
+                            Hs<uint32_t> h;
+                            h.tb = w.tb;
+                            h.tb_cap = w.tb_cap;
+                            h.over = w.over;
+                            h.over_cap = w.cap_t;
+                            h.tmp = w.tmp;
+                            h.tmp_cap = w.cap_t;
+                            h.key = w.mkey + s * w.mcap;
+                            h.nelem = h.nover = 0;
+                            const uint32_t ns = m.nmap[s];
+                            if (!hs_setup(h, 20, lane)) {
+                                rc = 1;
+                                break;
+                            }
+                            for (uint32_t x = 0; x < ns && rc == 0; ++x)
+                                if (hs_insert(h, x, lane) != HS_OK) rc = 1;
+                            if (rc) break;
+                            const uint32_t* mv = w.mval + s * w.mcap;
+                            for (uint64_t b0 = 0; b0 < h.arr + h.nover; b0 += 64) {
+                                const uint64_t b = b0 + lane;
+                                uint32_t x = Hs<uint32_t>::NONE;
+                                if (b < h.arr) x = h.tb[b];
+                                else if (b < h.arr + h.nover) x = h.over[b - h.arr];
+                                const bool occ = x != Hs<uint32_t>::NONE;
+                                const uint32_t v = occ ? mv[x] : 0u;
+                                const bool job = occ && (v & PF2) && pe_c0(v) >= QTHRESH;
+                                const uint64_t bo = __ballot(occ), bj = __ballot(job);
+                                if (occ) {
+                                    PeItem it;
+                                    it.k = h.key[x];
+                                    it.v = v;
+                                    it.j = PE_NONE;
+                                    if (job) {
+                                        it.j = jb + __popcll(bj & below);
+                                        jm[it.j] = it.k | (s ? 0x80000000u : 0u);
+                                        jr[it.j] = roA;
+                                    }
+                                    pd.map[pos + __popcll(bo & below)] = it;
+                                }
+                                pos += __popcll(bo);
+                                jb += __popcll(bj);
+                            }
+                            wave_sync();
+                        }
+                        // a map outgrew the scratch after its slots were taken:
+                        // the slots' jobs are complete, the item is re-run
+                        if (rc) nf = nr = 0;
+                    }
+                }
+            }
+        }
+    }
+    if (rc == 1) {
+        // beyond this variant's capacity: listed for a global-scratch run
+        // (the host grows the scratch from run to run)
+        if (lane == 0) {
+            const uint32_t b = atomicAdd(pd.big_n, 1u);
+            if (b < pd.big_cap) {
+                pd.big_item[b] = t;
+                pd.big_e[b] = max(EA, EB);
+            } else {
+                atomicOr(bd.flags, FLAG_HEAVY_FULL);
+            }
+        }
+        nf = nr = 0;
+    }
+    if (lane == 0) {
+        pd.map_off[t] = off;
+        pd.map_nf[t] = nf;
+        pd.map_nr[t] = nr;
+    }
+    wave_sync();
+}
+
+template <typename TabT>
+struct PeLight {
+    TabT klo[256];
+    uint32_t klen[256];
+    uint32_t koff[256];
+    uint32_t kcm[256];
+    uint8_t kfl[256];
+    uint32_t ent[PE_LE];
+    uint32_t eaux[PE_LE];
+    uint8_t ekm[PE_LE];
+    // phases in turn: the first read's counting table, its insertion lists
+    // and hopscotch arrays, the lookup over the item's windows (second read,
+    // flag2), the 20-bucket maps' hopscotch arrays
+    union {
+        struct {
+            uint32_t key[2 * PE_LE];
+            uint32_t first[2 * PE_LE];
+            uint32_t cnt[2 * PE_LE];
+        } h;
+        struct {
+            uint32_t ins_key[2 * PE_LE];
+            uint16_t ins_cnt[2 * PE_LE];
+            uint32_t tb[PE_LTB];
+            uint32_t tmp[PE_LE];
+            uint32_t over[PE_LE];
+        } c;
+        struct {
+            uint32_t hk[PE_LH];
+            uint32_t ht[PE_LH];
+            uint32_t hc[PE_LH];
+            uint32_t hi[PE_LH];
+        } m;
+    } u;
+    uint32_t mkey[2 * PE_LM];
+    uint32_t mval[2 * PE_LM];
+};
+
 template <typename TabT>
 __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDev pd)
+{
+    __shared__ SeedShared sh;
+    __shared__ PeLds m;
+    __shared__ PeLight<TabT> L;
+    const int lane = threadIdx.x;
+    seed_table_init(sh);
+    PeWork<TabT> w;
+    w.klo = L.klo;
+    w.klen = L.klen;
+    w.koff = L.koff;
+    w.kcm = L.kcm;
+    w.kfl = L.kfl;
+    w.cap_e = PE_LE;
+    w.ent = L.ent;
+    w.ekm = L.ekm;
+    w.eaux = L.eaux;
+    w.hsize = 2 * PE_LE;
+    w.hkey = L.u.h.key;
+    w.hfirst = L.u.h.first;
+    w.hcnt = L.u.h.cnt;
+    w.ins_key = L.u.c.ins_key;
+    w.ins_cnt = L.u.c.ins_cnt;
+    w.tb_cap = PE_LTB;
+    w.tb = L.u.c.tb;
+    w.tmp = L.u.c.tmp;
+    w.over = L.u.c.over;
+    w.cap_t = PE_LE;
+    w.mkey = L.mkey;
+    w.mval = L.mval;
+    w.mcap = PE_LM;
+    w.hk = L.u.m.hk;
+    w.ht = L.u.m.ht;
+    w.hc = L.u.m.hc;
+    w.hi = L.u.m.hi;
+    w.mhsz = PE_LH;
+    PoolChunk pcA = {0, 0, 0};
+    uint64_t st[4] = {0, 0, 0, 0};
+    for (uint32_t t = blockIdx.x; t < 2 * pd.n; t += gridDim.x) pe_item<TabT>(ix, bd, pd, sh, m, w, t, pcA, st);
+    if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
+    if (lane == 0 && st[2]) {
+        atomicAdd(&bd.stats[0], (unsigned long long)st[0]);
+        atomicAdd(&bd.stats[1], (unsigned long long)st[1]);
+        atomicAdd(&bd.stats[2], (unsigned long long)st[2]);
+        atomicAdd(&bd.stats[6], (unsigned long long)st[3]);
+    }
+}
+
+// the listed items with per-block global scratch sized for the largest
+template <typename TabT>
+__global__ __launch_bounds__(64) void k_pe_seed_big(DevIndex ix, BatchDev bd, PairDev pd)
 {
     __shared__ SeedShared sh;
     __shared__ PeLds m;
@@ -493,180 +836,27 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
     w.ins_cnt = (uint16_t*)take(4ull * pd.cap_e);
     w.tb_cap = pd.tb_cap;
     w.tb = (uint32_t*)take(4ull * pd.tb_cap);
-    w.tmp = (uint32_t*)take(4ull * pd.cap_e);
-    w.over = (uint32_t*)take(4ull * pd.cap_e);
+    w.cap_t = pd.cap_t;
+    w.tmp = (uint32_t*)take(4ull * pd.cap_t);
+    w.over = (uint32_t*)take(4ull * pd.cap_t);
+    w.mcap = pd.mcap;
+    w.mkey = (uint32_t*)take(8ull * pd.mcap);
+    w.mval = (uint32_t*)take(8ull * pd.mcap);
+    w.mhsz = pd.mhsz;
+    w.hk = (uint32_t*)take(4ull * pd.mhsz);
+    w.ht = (uint32_t*)take(4ull * pd.mhsz);
+    w.hc = (uint32_t*)take(4ull * pd.mhsz);
+    w.hi = (uint32_t*)take(4ull * pd.mhsz);
     PoolChunk pcA = {0, 0, 0};
-    uint64_t st_k = 0, st_e = 0, st_n = 0, st_b = 0;
-    const uint64_t below = lanes_below(lane);
-    const uint32_t nitems = pd.list ? pd.list_n : 2 * pd.n;
-    for (uint32_t ti = blockIdx.x; ti < nitems; ti += gridDim.x) {
-        const uint32_t t = pd.list ? pd.list[ti] : ti;
-        const uint32_t i = t >> 1, c = t & 1;
-        uint32_t nf = 0, nr = 0, off = 0;
-        // pairs with an invalid mate are counted by k_pe_final only
-        const bool on = ((pd.combos >> c) & 1) && bd.pre[i] == 0 && bd.pre[pd.n + i] == 0;
-        int rc = 0;
-        uint32_t EA = 0, EB = 0;
-        if (on) {
-            const uint32_t roA = 2 * i + c;
-            const uint32_t roB = 2 * (pd.n + i) + (1 - c);
-            uint32_t bc, LA, LB, bcB;
-            rc = pe_entries<TabT>(ix, bd, roA, sh, m, w, true, EA, bc, LA);
-            if (rc == 0) {
-                st_k += LA - KMERLEN + 1;
-                st_e += EA;
-                st_b += LA;
-                ++st_n;
-                rc = pe_first<TabT>(w, m, EA, LA, bc);
-            }
-            if (rc == 0 && (m.nmap[0] | m.nmap[1])) {
-                // the lookup table over the first read's windows
-                for (uint32_t x = lane; x < PE_MH; x += 64) m.hk[x] = EMPTY;
-                wave_sync();
-                if (lane == 0)
-                    for (uint32_t s = 0; s < 2; ++s)
-                        for (uint32_t x = 0; x < m.nmap[s]; ++x) pe_put(m, m.mkey[s][x], s, x);
-                wave_sync();
-                rc = pe_entries<TabT>(ix, bd, roB, sh, m, w, false, EB, bcB, LB);
-                if (rc == 0) {
-                    st_k += LB - KMERLEN + 1;
-                    st_e += EB;
-                    st_b += LB;
-                    rc = pe_second<TabT>(w, m, EB, LB, (int)pd.cw);
-                }
-                if (rc == 0) {
-                    // flag2: windows with c1 >= QTHRESH and their present
-                    // neighbours within contextWLen (:3125-3150); hasCpG
-                    const uint32_t n0 = m.nmap[0], ntot = n0 + m.nmap[1];
-                    for (uint32_t x = lane; x < ntot; x += 64) {
-                        const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
-                        if (pe_c1(m.mval[s][y]) < QTHRESH) continue;
-                        const uint32_t key = m.mkey[s][y];
-                        atomicOr(&m.mval[s][y], PF2);
-                        for (int u = 1; u <= (int)pd.cw; ++u) {
-                            uint32_t q = pe_find(m, key + (uint32_t)u, s);
-                            if (q != PE_NONE) atomicOr(&m.mval[s][q], PF2);
-                            q = pe_find(m, key - (uint32_t)u, s);
-                            if (q != PE_NONE) atomicOr(&m.mval[s][q], PF2);
-                        }
-                    }
-                    wave_sync();
-                    bool cpg = false;
-                    uint32_t nj = 0;
-                    for (uint32_t x = lane; x < ntot; x += 64) {
-                        const uint32_t s = x < n0 ? 0u : 1u, y = x < n0 ? x : x - n0;
-                        const uint32_t v = m.mval[s][y];
-                        if ((v & PF2) && ix.win[m.mkey[s][y]].z != CPGDUMMY) cpg = true;
-                        if ((v & PF2) && pe_c0(v) >= QTHRESH) ++nj;
-                    }
-                    const bool hasCpG = __ballot(cpg) != 0;
-                    nj = wave_sum_u32(nj);
-                    if (hasCpG) {
-                        nf = m.nmap[0];
-                        nr = m.nmap[1];
-                        if (lane == 0) off = atomicAdd(pd.map_n, nf + nr);
-                        off = __shfl(off, 0, 64);
-                        if (off + nf + nr > pd.map_cap) {
-                            if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
-                            nf = nr = 0;
-                        }
-                    }
-                    if (nf + nr) {
-                        const uint32_t pool = (LA + 63) / 64 - 1;
-                        uint32_t jb = nj ? pool_take(bd, pcA, pool, nj, false, lane) : 0;
-                        if (jb == ~0u) {
-                            if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
-                            nf = nr = 0;
-                        } else {
-                            uint32_t* jm = pc_meta(bd, pool);
-                            uint32_t* jr = bd.job_ro[0];
-#pragma unroll
-                            for (uint32_t q = 1; q < FAME_NPOOL; ++q) jr = pool == q ? bd.job_ro[q] : jr;
-                            // iteration order of the two 20-bucket maps (:2760-2790
-                            // build them with 20 buckets, :2801-3164 insert into them)
-                            uint32_t pos = off;
-                            for (uint32_t s = 0; s < 2; ++s) {
-                                Hs<uint32_t> h;
-                                h.tb = w.tb;
-                                h.tb_cap = w.tb_cap;
-                                h.over = w.over;
-                                h.over_cap = w.cap_e;
-                                h.tmp = w.tmp;
-                                h.tmp_cap = w.cap_e;
-                                h.key = m.mkey[s];
-                                h.nelem = h.nover = 0;
-                                const uint32_t ns = m.nmap[s];
-                                if (!hs_setup(h, 20, lane)) {
-                                    rc = 2;
-                                    break;
-                                }
-                                for (uint32_t x = 0; x < ns && rc == 0; ++x)
-                                    if (hs_insert(h, x, lane) != HS_OK) rc = 2;
-                                if (rc) break;
-                                for (uint64_t b0 = 0; b0 < h.arr + h.nover; b0 += 64) {
-                                    const uint64_t b = b0 + lane;
-                                    uint32_t x = Hs<uint32_t>::NONE;
-                                    if (b < h.arr) x = h.tb[b];
-                                    else if (b < h.arr + h.nover) x = h.over[b - h.arr];
-                                    const bool occ = x != Hs<uint32_t>::NONE;
-                                    const uint32_t v = occ ? m.mval[s][x] : 0u;
-                                    const bool job = occ && (v & PF2) && pe_c0(v) >= QTHRESH;
-                                    const uint64_t bo = __ballot(occ), bj = __ballot(job);
-                                    if (occ) {
-                                        PeItem it;
-                                        it.k = m.mkey[s][x];
-                                        it.v = v;
-                                        it.j = PE_NONE;
-                                        if (job) {
-                                            it.j = jb + __popcll(bj & below);
-                                            jm[it.j] = it.k | (s ? 0x80000000u : 0u);
-                                            jr[it.j] = roA;
-                                        }
-                                        pd.map[pos + __popcll(bo & below)] = it;
-                                    }
-                                    pos += __popcll(bo);
-                                    jb += __popcll(bj);
-                                }
-                                wave_sync();
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        if (rc == 1) {
-            // entries beyond this scratch: listed for the larger run
-            if (lane == 0) {
-                if (pd.list) {
-                    atomicOr(bd.flags, FLAG_CAPACITY);
-                } else {
-                    const uint32_t b = atomicAdd(pd.big_n, 1u);
-                    if (b < pd.big_cap) {
-                        pd.big_item[b] = t;
-                        pd.big_e[b] = max(EA, EB);
-                    } else {
-                        atomicOr(bd.flags, FLAG_HEAVY_FULL);
-                    }
-                }
-            }
-            nf = nr = 0;
-        } else if (rc == 2) {
-            if (lane == 0) atomicOr(bd.flags, FLAG_CAPACITY);
-            nf = nr = 0;
-        }
-        if (lane == 0) {
-            pd.map_off[t] = off;
-            pd.map_nf[t] = nf;
-            pd.map_nr[t] = nr;
-        }
-        wave_sync();
-    }
+    uint64_t st[4] = {0, 0, 0, 0};
+    for (uint32_t ti = blockIdx.x; ti < pd.list_n; ti += gridDim.x)
+        pe_item<TabT>(ix, bd, pd, sh, m, w, pd.list[ti], pcA, st);
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
-    if (lane == 0 && st_n) {
-        atomicAdd(&bd.stats[0], (unsigned long long)st_k);
-        atomicAdd(&bd.stats[1], (unsigned long long)st_e);
-        atomicAdd(&bd.stats[2], (unsigned long long)st_n);
-        atomicAdd(&bd.stats[6], (unsigned long long)st_b);
+    if (lane == 0 && st[2]) {
+        atomicAdd(&bd.stats[0], (unsigned long long)st[0]);
+        atomicAdd(&bd.stats[1], (unsigned long long)st[1]);
+        atomicAdd(&bd.stats[2], (unsigned long long)st[2]);
+        atomicAdd(&bd.stats[6], (unsigned long long)st[3]);
     }
 }
 
@@ -1131,8 +1321,13 @@ __global__ __launch_bounds__(256) void k_pe_final(DevIndex ix, BatchDev b1, Batc
 void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, uint32_t grid, hipStream_t st)
 {
     if (!grid) return;
-    if (ix.tab_is64) hipLaunchKernelGGL(k_pe_seed<uint64_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
-    else hipLaunchKernelGGL(k_pe_seed<uint32_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+    if (pd.list) {
+        if (ix.tab_is64) hipLaunchKernelGGL(k_pe_seed_big<uint64_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+        else hipLaunchKernelGGL(k_pe_seed_big<uint32_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+    } else {
+        if (ix.tab_is64) hipLaunchKernelGGL(k_pe_seed<uint64_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+        else hipLaunchKernelGGL(k_pe_seed<uint32_t>, dim3(grid), dim3(64), 0, st, ix, bd, pd);
+    }
 }
 
 void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st)
