@@ -26,6 +26,10 @@ gap uniform in [20, 400], indel rate 0.2 %, MINPDIST 20 / MAXPDIST 400,
 READLEN 150 (value = matched reads / s, both mates counted, as FAME's
 succMatch counts them).
 
+serve = the genome_se reference and read model served as a request stream
+(value = aligned reads/s at the full offered load; p50 / p99 chunk latency
+per load, beside the plain form's at the same load).
+
 Also reported: per-stage device time, the roofline of the dominant kernel,
 and FAME's own OpenMP CPU path (built from its sources into oracle/_ref)
 timed on this host on a bounded sample of the same workload, whose per-read
@@ -82,6 +86,10 @@ WORKLOADS = {
                       len_range=(50, 150), near_copies=3000, copied_frac=0.3,
                       desc="uneven single-end: 300 Mbp with 3000 segments copied 5-50 times, 10M reads of "
                            "50-150 bp (30% from the copies), READLEN 150, plain form"),
+    "serve": dict(readlen=100, n=4_000_000, paired=False, noloss=False, minpd=50, maxpd=550, serve=True,
+                  desc="serving loop: 300 Mbp index resident, chunks of single-end 100 bp reads arriving as a "
+                       "Poisson stream, log-normal sizes (median 20k, sigma 1, clipped to [1k, 300k]), offered "
+                       "load swept 20-100% of the batched peak, continuous batching vs the plain form"),
     "paired": dict(readlen=150, n=5_000_000, paired=True, noloss=True, minpd=20, maxpd=400,
                    repeat_frac=0.2, gap=(20, 400), indel=0.002,
                    desc="paired-end repeat-heavy: 300 Mbp with 20% in 300 bp repeat families (10% "
@@ -138,6 +146,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per load point")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -195,6 +204,11 @@ def main():
             seqs, offs, lens = reads.flat()
             n = len(lens)
         log(f"{n} {'pairs' if paired else 'reads'} generated in {t_reads:.1f}s")
+        if wl.get("serve"):
+            out = serve_bench(args, wl, idx_path, reads, lens_c, t_build, t_reads)
+            out["wall_s"] = round(time.time() - t_all, 1)
+            print(json.dumps(out), flush=True)
+            return
         # ---------------- device ----------------
         t0 = time.time()
         ctx = _lib.Context(idx_path)
@@ -403,6 +417,138 @@ def cpu_leg(args, wl, ctx, reads, idx_path, wd, nchrom):
               "counters_equal": cnt.tolist() == [st["succ"], st["nonunique"], st["unsucc"]]}
     log("parity on the sample:", parity)
     return cpu, parity
+
+
+def chunk_sizes(rng, k):
+    """log-normal chunk sizes, median 20k reads, sigma 1, clipped to [1k, 300k]"""
+    return np.clip(np.round(rng.lognormal(np.log(20000), 1.0, size=k)), 1000, 300000).astype(np.int64)
+
+
+def run_stream(submit, wait, arrivals, sizes, starts):
+    """Submits chunk i at arrivals[i] (s from now) from this thread and waits
+    for completions in another; returns per-chunk latency (ms, from the
+    scheduled arrival) and results."""
+    import threading
+    import queue
+    q = queue.Queue()
+    res = [None] * len(sizes)
+
+    def collector():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            i, tk, t_arr = item
+            res[i] = wait(tk, t_arr)
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    for i, (a, k, s0) in enumerate(zip(arrivals, sizes, starts)):
+        now = time.perf_counter() - t0
+        if a > now:
+            time.sleep(a - now)
+        q.put((i, submit(int(s0), int(k)), t0 + a))
+    q.put(None)
+    th.join()
+    return res, time.perf_counter() - t0
+
+
+def serve_bench(args, wl, idx_path, reads, lens_c, t_build, t_reads):
+    """Continuous-batching server vs the plain form on a Poisson stream."""
+    import torch
+    from fame_amd import _lib
+    seqs, offs, lens = reads.flat()
+    # the read pool in pinned host memory: chunks are copied to the device
+    # straight from it
+    pin = torch.empty(seqs.size, dtype=torch.uint8, pin_memory=True)
+    pin.numpy()[:] = seqs
+    seqs = pin.numpy()
+    n = len(lens)
+    rng = np.random.default_rng(args.seed + 9)
+
+    def new_ctx():
+        ctx = _lib.Context(idx_path)
+        cnt = np.zeros(3, np.uint64)
+        ctx.match(seqs, offs[:CHUNKSIZE], lens[:CHUNKSIZE], True, cnt)  # the strand decision chunk
+        ctx.decide_strand()
+        return ctx
+
+    def plan(k_req, rate_chunks):
+        sz = chunk_sizes(rng, k_req)
+        st = rng.integers(CHUNKSIZE, n - sz, size=k_req)
+        arr = np.cumsum(rng.exponential(1.0 / rate_chunks, size=k_req)) if rate_chunks else np.zeros(k_req)
+        return arr, sz, st
+
+    mean_chunk = float(np.mean(chunk_sizes(np.random.default_rng(1), 100000)))
+    # batched peak: every chunk queued at once
+    ctx = new_ctx()
+    srv = _lib.Server(ctx)
+    tick = {}
+
+    def s_submit(s0, k):
+        return srv.submit(seqs, offs[s0:s0 + k], lens[s0:s0 + k])
+
+    def s_wait(tk, t_arr):
+        v, m, cnt, tm = srv.wait(tk)
+        return (time.perf_counter() - t_arr) * 1e3, int(cnt[0]), tm["batch_reads"]
+
+    arr, sz, st = plan(300, 0)
+    srv.wait(s_submit(int(st[0]), int(sz[0])))  # warm-up
+    res, wall = run_stream(s_submit, s_wait, arr, sz, st)
+    peak_reads = float(sz.sum()) / wall
+    peak_aligned = sum(r[1] for r in res) / wall
+    log(f"batched peak: {peak_reads / 1e6:.1f} M reads/s ({peak_aligned / 1e6:.1f} M aligned), "
+        f"{srv.stats()}")
+    # plain form: one call per chunk, in arrival order
+    pctx = new_ctx()
+
+    def p_submit(s0, k):
+        return (s0, k)
+
+    def p_wait(tk, t_arr):
+        s0, k = tk
+        cnt = np.zeros(3, np.uint64)
+        pctx.match(seqs, offs[s0:s0 + k], lens[s0:s0 + k], False, cnt)
+        return (time.perf_counter() - t_arr) * 1e3, int(cnt[0]), k
+
+    arr, sz, st = plan(100, 0)
+    res, wall = run_stream(p_submit, p_wait, arr, sz, st)
+    plain_peak = float(sz.sum()) / wall
+    log(f"plain peak: {plain_peak / 1e6:.1f} M reads/s")
+    sweep = []
+    for frac in (0.2, 0.4, 0.6, 0.8, 1.0):
+        rate_reads = frac * peak_reads
+        rate_chunks = rate_reads / mean_chunk
+        k_req = int(min(3000, max(60, rate_chunks * args.serve_seconds)))
+        arr, sz, st = plan(k_req, rate_chunks)
+        row = {"load": frac, "offered_reads_per_s": round(rate_reads, 1), "chunks": k_req}
+        for form, sub, wt in (("batched", s_submit, s_wait), ("plain", p_submit, p_wait)):
+            res, wall = run_stream(sub, wt, arr, sz, st)
+            lat = np.array([r[0] for r in res])
+            row[form] = {"served_reads_per_s": round(float(sz.sum()) / wall, 1),
+                         "aligned_reads_per_s": round(sum(r[1] for r in res) / wall, 1),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                         "mean_batch_reads": round(float(np.mean([r[2] for r in res])), 1)}
+        log(f"load {frac:.1f}: batched p50 {row['batched']['p50_ms']} p99 {row['batched']['p99_ms']} ms, "
+            f"plain p50 {row['plain']['p50_ms']} p99 {row['plain']['p99_ms']} ms")
+        sweep.append(row)
+    srv.close()
+    ctx.close()
+    pctx.close()
+    top = sweep[-1]["batched"]
+    return {
+        "metric": METRIC, "value": top["aligned_reads_per_s"], "unit": "reads/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": wl["desc"], "genome_bp": int(sum(lens_c)), "read_pool": n, "read_len": 100,
+                   "form": "serving (continuous batching) vs plain"},
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
+        "batched_peak_reads_per_s": round(peak_reads, 1), "plain_peak_reads_per_s": round(plain_peak, 1),
+        "sweep": sweep,
+        "setup_s": {"index_build": round(t_build, 1), "reads": round(t_reads, 1)},
+    }
 
 
 def read_pairs_bin(path):

@@ -21,7 +21,8 @@ MAX_READ = 255
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
            "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
-           "fame_decide_strand", "fame_get_meth", "fame_reset",
+           "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_serve_open", "fame_serve_submit",
+           "fame_serve_wait", "fame_serve_stats", "fame_serve_close",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
@@ -59,6 +60,12 @@ def load():
         L.fame_match_pairs_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5 + [C.c_uint32, C.c_int, _u64p] + \
             [C.c_void_p] * 5
         L.fame_decide_strand.argtypes = [C.c_void_p, _u64p, _u64p, C.POINTER(C.c_int)]
+        L.fame_serve_open.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]
+        L.fame_serve_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
+        L.fame_serve_wait.argtypes = [C.c_void_p, C.c_uint64, _u64p, _dbl]
+        L.fame_serve_stats.argtypes = [C.c_void_p, _u64p, _dbl]
+        L.fame_serve_close.argtypes = [C.c_void_p]
         L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
         L.fame_reset.argtypes = [C.c_void_p]
         L.fame_print_methylation_levels.argtypes = [C.c_void_p, C.c_char_p]
@@ -179,6 +186,63 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             load().fame_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over a Context (fame_serve_*): submit chunks of reads as
+    they arrive, wait for each one's results.  The chunk arrays passed to
+    submit are kept alive here until the chunk completes."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch_reads=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(load().fame_serve_open(ctx.h, int(hbm_budget), int(max_batch_reads), C.byref(h)))
+        self.h = h.value
+        self._live = {}
+
+    def submit(self, seqs, offs, lens):
+        n = int(len(lens))
+        seqs = np.ascontiguousarray(seqs, np.uint8)
+        if seqs.size == 0:
+            seqs = np.zeros(1, np.uint8)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.uint32)
+        valid = np.zeros(max(n, 1), np.uint8)
+        mat = np.zeros(max(n, 1), np.uint64)
+        t = C.c_uint64()
+        rc = load().fame_serve_submit(self.h, seqs.ctypes.data, offs.ctypes.data, lens.ctypes.data, n,
+                                      valid.ctypes.data, mat.ctypes.data, C.byref(t))
+        self.ctx._check(rc)
+        self._live[t.value] = (seqs, offs, lens, valid, mat, n)
+        return t.value
+
+    def wait(self, ticket):
+        """(valid, mat, counters[3], times_ms {queued, gpu, latency, batch_reads})"""
+        cnt = np.zeros(3, np.uint64)
+        tm = np.zeros(4, np.float64)
+        rc = load().fame_serve_wait(self.h, ticket, ptr(cnt, _u64p), ptr(tm, _dbl))
+        keep = self._live.pop(ticket)
+        self.ctx._check(rc)
+        n = keep[5]
+        return keep[3][:n], keep[4][:n], cnt, dict(zip(("queued", "gpu", "latency", "batch_reads"), tm.tolist()))
+
+    def stats(self):
+        out = np.zeros(4, np.uint64)
+        busy = C.c_double()
+        load().fame_serve_stats(self.h, ptr(out, _u64p), C.byref(busy))
+        return dict(zip(("batches", "requests", "reads", "max_batch_reads"), (int(x) for x in out)),
+                    busy_ms=busy.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fame_serve_close(self.h)
             self.h = None
 
     def __del__(self):

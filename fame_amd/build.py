@@ -11,8 +11,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfamehip.so")
 INDEX_LIB = os.path.join(HERE, "libfameindex.so")
 SOURCES = [os.path.join(HERE, "csrc", "fame_device.hip"), os.path.join(HERE, "csrc", "fame_paired.hip"),
-           os.path.join(HERE, "csrc", "fame_host.cpp")]
-FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+           os.path.join(HERE, "csrc", "fame_host.cpp"), os.path.join(HERE, "csrc", "fame_serve.cpp")]
+FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
          # packed FP32 instructions are kept out of all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

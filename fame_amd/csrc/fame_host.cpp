@@ -928,6 +928,31 @@ int guarded(fame_ctx* c, F&& f)
 
 } // namespace
 
+// hooks of the serving layer (fame_serve.cpp)
+namespace fame {
+
+int serve_run(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+              uint64_t total_bytes, hipStream_t st, const uint8_t** d_status, const uint8_t** d_valid,
+              const uint64_t** d_mat)
+{
+    return guarded(c, [&]() {
+        uint64_t cnt[3] = {0, 0, 0};
+        int rc = run_batch(c, d_seqs, d_offs, d_lens, n, total_bytes, 0, cnt, st);
+        if (rc != FAME_OK) return rc;
+        *d_status = c->b_status.as<uint8_t>();
+        *d_valid = c->b_valid.as<uint8_t>();
+        *d_mat = c->b_mat.as<uint64_t>();
+        return (int)FAME_OK;
+    });
+}
+
+hipStream_t serve_stream(fame_ctx* c) { return c->stream; }
+int serve_device(fame_ctx* c) { return c->device; }
+uint32_t serve_readlen(fame_ctx* c) { return c->readlen; }
+void serve_set_err(fame_ctx* c, const std::string& m) { set_err(c, m); }
+
+} // namespace fame
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------

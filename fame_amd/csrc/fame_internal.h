@@ -6,6 +6,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
+struct fame_ctx;
+
 #define FAME_DEV_MAX_READ 255
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
@@ -177,6 +181,15 @@ void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, u
 void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st);
 void launch_pe_final(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd,
                      int get_stranded, hipStream_t st);
+// serving layer hooks (fame_host.cpp): one batch on stream st with the
+// strand already decided; the per-read outputs stay in the context's buffers
+int serve_run(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+              uint64_t total_bytes, hipStream_t st, const uint8_t** d_status, const uint8_t** d_valid,
+              const uint64_t** d_mat);
+hipStream_t serve_stream(fame_ctx* c);
+int serve_device(fame_ctx* c);
+uint32_t serve_readlen(fame_ctx* c);
+void serve_set_err(fame_ctx* c, const std::string& m);
 void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st);
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);

@@ -121,6 +121,36 @@ int fame_reset(fame_ctx* ctx);
  * <basename>_cpg.tsv, byte-identical to the reference's report */
 int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
 
+/* ---- serving loop (no counterpart in the reference) ----------------------
+ * A request loop over the context's resident index for callers that submit
+ * chunks of single-end reads as they arrive (the strand decided beforehand,
+ * i.e. matchReads with getStranded = false; with --unord_reads both).
+ * Waiting requests are merged into one device batch while it fits the HBM
+ * budget; the next batch is copied to the device while the current one
+ * computes.  Every read gets the result fame_match_reads gives it, whatever
+ * it is batched with; CpG counts accumulate in the context.  The context
+ * must not be used for anything else while a server is open on it. */
+typedef struct fame_server fame_server;
+
+/* hbm_budget: device bytes the two in-flight batches may use (0 = half the
+ * free HBM); max_batch_reads: reads per batch (0 = 2M) */
+int fame_serve_open(fame_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_reads, fame_server** out);
+
+/* queues a chunk; the read buffers must stay valid, and valid / mat (n
+ * entries each, may be NULL) are written, before fame_serve_wait returns */
+int fame_serve_submit(fame_server* srv, const uint8_t* seqs, const uint64_t* offs, const uint32_t* lens,
+                      uint32_t n, uint8_t* valid, uint64_t* mat, uint64_t* ticket);
+
+/* waits for a chunk: counters[0..2] += succ, nonunique, unsucc of its reads;
+ * times_ms = queued, on the GPU, arrival to completion, reads of its batch */
+int fame_serve_wait(fame_server* srv, uint64_t ticket, uint64_t counters[3], double times_ms[4]);
+
+/* batches, requests, reads served, reads per batch at most; busy_ms = time
+ * the device spent on batches */
+int fame_serve_stats(fame_server* srv, uint64_t out[4], double* busy_ms);
+
+int fame_serve_close(fame_server* srv);
+
 /* per-stage device time of the last fame_match_reads* call, milliseconds:
  * prep, seed, verify, select, meth, and the total including host waits */
 int fame_last_timings(const fame_ctx* ctx, double out[6]);
