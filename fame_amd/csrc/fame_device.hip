@@ -232,12 +232,16 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     }
     wave_sync();
     // first occurrence and total count (u16) of every kept key: an open
-    // addressing table keyed by the packed entry (hsize, a power of two, is
-    // at least the number of entries, so every probe sequence ends)
+    // addressing table keyed by the packed entry (a power of two of at least
+    // twice the entries, at most hsize >= cap_e: every probe sequence ends;
+    // large scratch tables are only cleared as far as the read needs)
     {
-        const uint32_t hm = w.hsize - 1;
-        const int hshift = __clz(w.hsize) + 1;
-        for (uint32_t i = lane; i < w.hsize; i += 64) {
+        uint32_t hs = 64;
+        while (hs < 2 * (uint32_t)E && hs < w.hsize) hs <<= 1;
+        if (hs > w.hsize) hs = w.hsize;
+        const uint32_t hm = hs - 1;
+        const int hshift = __clz(hs) + 1;
+        for (uint32_t i = lane; i < hs; i += 64) {
             w.hkey[i] = EMPTY;
             w.hfirst[i] = 0xFFFFFFFFu;
             w.hcnt[i] = 0;

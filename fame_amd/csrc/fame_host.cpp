@@ -455,53 +455,88 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         }
         // heavy (read, orientation) pairs: global-memory scratch, in slices
         if (heavy_n) {
-            std::vector<uint32_t> he(heavy_n), hb(heavy_n);
+            std::vector<uint32_t> he(heavy_n), hb(heavy_n), hr(heavy_n);
             HIPC(hipMemcpy(he.data(), bd.heavy_e, 4 * heavy_n, hipMemcpyDeviceToHost));
             HIPC(hipMemcpy(hb.data(), bd.heavy_bc, 4 * heavy_n, hipMemcpyDeviceToHost));
-            uint64_t emax = 1, bcmax = 1;
-            for (uint32_t i = 0; i < heavy_n; ++i) {
-                emax = std::max<uint64_t>(emax, he[i]);
-                bcmax = std::max<uint64_t>(bcmax, hb[i]);
-            }
-            if (emax > (1ull << 26)) {
+            HIPC(hipMemcpy(hr.data(), bd.heavy_ro, 4 * heavy_n, hipMemcpyDeviceToHost));
+            uint64_t emax_all = 1;
+            for (uint32_t i = 0; i < heavy_n; ++i) emax_all = std::max<uint64_t>(emax_all, he[i]);
+            if (emax_all > (1ull << 26)) {
                 set_err(c, "a read has more than 2^26 index hits");
                 return FAME_ERR_CAPACITY;
+            }
+            // size classes (caps 1024 * 4^k): each run's scratch slices are
+            // sized for its own largest read, so one outlier does not shrink
+            // the number of slices of the others
+            std::vector<uint32_t> ord(heavy_n);
+            for (uint32_t i = 0; i < heavy_n; ++i) ord[i] = i;
+            std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return he[a] < he[b]; });
+            {
+                std::vector<uint32_t> r2(heavy_n);
+                for (uint32_t i = 0; i < heavy_n; ++i) r2[i] = hr[ord[i]];
+                HIPC(hipMemcpyAsync(bd.heavy_ro, r2.data(), 4 * heavy_n, hipMemcpyHostToDevice, st));
+                HIPC(hipStreamSynchronize(st));
             }
             auto rup = [](uint64_t v) {
                 uint64_t p = 1;
                 while (p < v) p <<= 1;
                 return p;
             };
-            HeavyScratch hs{};
-            hs.cap_e = (uint32_t)emax;
-            hs.hsize = (uint32_t)rup(2 * emax);
-            uint64_t tbb = std::max(rup(bcmax), rup(2 * emax));
-            hs.tb_cap = (uint32_t)(2 * tbb + 64);
-            size_t stride = 0;
-            auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
-            add(8 * FAME_DEV_MAX_READ);
-            add(4 * FAME_DEV_MAX_READ);
-            add(4 * FAME_DEV_MAX_READ);
-            add(4 * emax);
-            add(emax);
-            add(4 * emax);
-            for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
-            add(8 * emax);
-            add(4 * emax);
-            add(4ull * hs.tb_cap);
-            add(4 * emax);
-            add(4 * emax);
-            add(8 * emax);
-            hs.stride = stride;
-            const size_t budget = 2ull << 30;
-            uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(heavy_n, budget / stride));
-            HIPC(c->b_heavy_scratch.reserve(stride * per));
-            hs.base = c->b_heavy_scratch.as<uint8_t>();
+            // the classes and their scratch first (the buffer is sized once,
+            // before any launch uses it)
+            struct Cls {
+                uint32_t c0, c1, per;
+                HeavyScratch hs;
+            };
+            std::vector<Cls> classes;
+            size_t need = 0;
+            for (uint32_t c0 = 0; c0 < heavy_n;) {
+                uint64_t cap = 1024;
+                while (cap < he[ord[c0]]) cap *= 4;
+                uint32_t c1 = c0;
+                uint64_t emax = 1, bcmax = 1;
+                while (c1 < heavy_n && he[ord[c1]] <= cap) {
+                    emax = std::max<uint64_t>(emax, he[ord[c1]]);
+                    bcmax = std::max<uint64_t>(bcmax, hb[ord[c1]]);
+                    ++c1;
+                }
+                HeavyScratch hs{};
+                hs.cap_e = (uint32_t)emax;
+                hs.hsize = (uint32_t)rup(2 * emax);
+                uint64_t tbb = std::max(rup(bcmax), rup(2 * emax));
+                hs.tb_cap = (uint32_t)(2 * tbb + 64);
+                size_t stride = 0;
+                auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+                add(8 * FAME_DEV_MAX_READ);
+                add(4 * FAME_DEV_MAX_READ);
+                add(4 * FAME_DEV_MAX_READ);
+                add(4 * emax);
+                add(emax);
+                add(4 * emax);
+                for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
+                add(8 * emax);
+                add(4 * emax);
+                add(4ull * hs.tb_cap);
+                add(4 * emax);
+                add(4 * emax);
+                add(8 * emax);
+                hs.stride = stride;
+                const size_t budget = 2ull << 30;
+                const uint32_t cnt = c1 - c0;
+                const uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(cnt, budget / stride));
+                need = std::max(need, stride * per);
+                classes.push_back({c0, c1, per, hs});
+                c0 = c1;
+            }
+            HIPC(c->b_heavy_scratch.reserve(need));
             tm.mark(12);
-            for (uint32_t first = 0; first < heavy_n; first += per) {
-                hs.first = first;
-                hs.count = std::min(per, heavy_n - first);
-                launch_seed_heavy(c->ix, bd, hs, st);
+            for (Cls& k : classes) {
+                k.hs.base = c->b_heavy_scratch.as<uint8_t>();
+                for (uint32_t first = k.c0; first < k.c1; first += k.per) {
+                    k.hs.first = first;
+                    k.hs.count = std::min(k.per, k.c1 - first);
+                    launch_seed_heavy(c->ix, bd, k.hs, st);
+                }
             }
             tm.mark(13);
             HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
