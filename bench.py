@@ -81,11 +81,11 @@ def ref_binary(htab, readlen, cores, nchrom, minpd=50, maxpd=550):
 WORKLOADS = {
     "genome_se": dict(readlen=100, n=10_000_000, paired=False, noloss=False, minpd=50, maxpd=550,
                       desc="genome-scale single-end: 300 Mbp / 24 chromosomes, 10M x 100 bp bisulfite "
-                           "reads, HTABSIZE 2^30, plain one-kernel-per-stage form"),
+                           "reads, HTABSIZE 2^30"),
     "uneven_se": dict(readlen=150, n=10_000_000, paired=False, noloss=False, minpd=50, maxpd=550,
                       len_range=(50, 150), near_copies=3000, copied_frac=0.3,
                       desc="uneven single-end: 300 Mbp with 3000 segments copied 5-50 times, 10M reads of "
-                           "50-150 bp (30% from the copies), READLEN 150, plain form"),
+                           "50-150 bp (30% from the copies), READLEN 150"),
     "serve": dict(readlen=100, n=4_000_000, paired=False, noloss=False, minpd=50, maxpd=550, serve=True,
                   desc="serving loop: 300 Mbp index resident, chunks of single-end 100 bp reads arriving as a "
                        "Poisson stream, log-normal sizes (median 20k, sigma 1, clipped to [1k, 300k]), offered "
@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per load point")
+    ap.add_argument("--serve-form", default="batched", help="pipeline form of the server's batches")
+    ap.add_argument("--forms", default="plain,batched",
+                    help="single-end forms to time (plain: one kernel per stage; batched: one graph per batch)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -249,19 +252,32 @@ def main():
                 s = e
             return cnt
 
-        for _ in range(args.warmup):
-            step()
-        calls.clear()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            cnt = step()
-        torch.cuda.synchronize()
-        t_run = time.perf_counter() - t0
-        ms_step = 1e3 * t_run / max(1, args.steps)
+        forms = ["plain"] if paired else [f for f in args.forms.split(",") if f]
+        results = {}
+        for form in forms:
+            if not paired:
+                ctx.set_form(form)
+            for _ in range(args.warmup):
+                step()
+            calls.clear()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                cnt = step()
+            torch.cuda.synchronize()
+            t_run = time.perf_counter() - t0
+            ms_f = 1e3 * t_run / max(1, args.steps)
+            results[form] = (ms_f, cnt.copy(), list(calls))
+            log(f"[{form}] step {ms_f:.1f} ms; counters {cnt.tolist()} over {n} {'pairs' if paired else 'reads'}"
+                + (f"; {ctx.form_info()}" if form == "batched" else ""))
+        # headline: the fastest form; the per-stage figures come from the plain form
+        best = min(results, key=lambda f: results[f][0])
+        ms_step, cnt, _ = results[best]
+        calls = results[forms[0]][2] if "plain" not in results else results["plain"][2]
         succ = int(cnt[0])
         value = succ / (ms_step / 1e3)
-        log(f"step {ms_step:.1f} ms; counters {cnt.tolist()} over {n} {'pairs' if paired else 'reads'}")
+        for f in results:
+            assert results[f][1].tolist() == results[best][1].tolist(), "forms disagree"
         K = max(1, args.steps)
         per_stage, per_work = {}, {}
         for _, tm, st in calls:
@@ -279,7 +295,10 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
             "data": "synthetic",
             "config": {"workload": wl["desc"], "genome_bp": int(sum(lens_c)), ("pairs" if paired else "reads"): n,
-                       "read_len": RL, "batch": args.batch, "first_chunk": CHUNKSIZE, "form": "plain"},
+                       "read_len": RL, "batch": args.batch, "first_chunk": CHUNKSIZE, "form": best},
+            "forms": {f: {"ms_per_step": round(results[f][0], 3),
+                          "aligned_reads_per_s": round(int(results[f][1][0]) / (results[f][0] / 1e3), 1)}
+                      for f in results},
             "input_reads_per_s": round((2 if paired else 1) * n / (ms_step / 1e3), 1),
             "counters": dict(zip(names, (int(x) for x in cnt))),
             "stage_ms": {k: round(v, 3) for k, v in per_stage.items()},
@@ -287,7 +306,9 @@ def main():
             "setup_s": {"index_build": round(t_build, 1), "reads": round(t_reads, 1),
                         "index_load": round(t_load, 1)},
         }
-        if not paired:
+        if "plain" in results and "batched" in results:
+            out["batched_vs_plain"] = round(results["plain"][0] / results["batched"][0], 3)
+        if not paired and "plain" in results:
             # ---------------- roofline ----------------
             # over the full-size calls (one launch of each kernel per call)
             full = [c for c in calls if c[0] == args.batch] or calls
@@ -483,6 +504,7 @@ def serve_bench(args, wl, idx_path, reads, lens_c, t_build, t_reads):
     mean_chunk = float(np.mean(chunk_sizes(np.random.default_rng(1), 100000)))
     # batched peak: every chunk queued at once
     ctx = new_ctx()
+    ctx.set_form(args.serve_form)
     srv = _lib.Server(ctx)
     tick = {}
 
@@ -543,7 +565,7 @@ def serve_bench(args, wl, idx_path, reads, lens_c, t_build, t_reads):
         "steps": 1, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": wl["desc"], "genome_bp": int(sum(lens_c)), "read_pool": n, "read_len": 100,
-                   "form": "serving (continuous batching) vs plain"},
+                   "form": f"serving (continuous batching, {args.serve_form} batches) vs plain"},
         "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
         "batched_peak_reads_per_s": round(peak_reads, 1), "plain_peak_reads_per_s": round(plain_peak, 1),
         "sweep": sweep,

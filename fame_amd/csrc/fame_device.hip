@@ -51,9 +51,11 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                                               uint8_t* __restrict__ rev, uint8_t* __restrict__ pre,
                                               uint8_t* __restrict__ codes, uint32_t ori_mask,
                                               uint32_t* __restrict__ err_flag, uint32_t* __restrict__ max_len,
-                                              uint32_t nsplit, uint32_t minlen, int paired)
+                                              uint32_t nsplit, uint32_t minlen, int paired,
+                                              const uint32_t* __restrict__ n_dev)
 {
     const int lane = threadIdx.x & 63;
+    if (n_dev) n = *n_dev;
     uint32_t lmax = 0; // longest read this wave prepared for seeding
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
@@ -445,6 +447,7 @@ template <typename TabT, int ML>
 __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
 {
     using C = SeedCap<ML>;
+    if (bd.n_dev) n_ro = 2 * *bd.n_dev;
     const bool short_batch = *bd.max_len <= (uint32_t)SEED_SHORT;
     if ((ML == SEED_SHORT) != short_batch) return;
     __shared__ SeedShared sh;
@@ -520,10 +523,15 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
 {
     __shared__ SeedShared sh;
     __shared__ alignas(4) uint8_t s_code[256];
-    const uint32_t hi = blockIdx.x;
-    if (hi >= hs.count) return;
-    const uint32_t ro = bd.heavy_ro[hs.first + hi];
-    uint8_t* base = hs.base + (size_t)hi * hs.stride;
+    // plain form: block hi serves item first + hi; batched form (count =
+    // FAME_JOBS_ON_DEVICE): persistent blocks over the device list
+    uint32_t count = hs.count;
+    if (count == FAME_JOBS_ON_DEVICE) {
+        if (*bd.flags & FLAG_ABORT) return;
+        count = min(*bd.heavy_n, bd.heavy_cap);
+    }
+    if (blockIdx.x >= count) return;
+    uint8_t* base = hs.base + (size_t)blockIdx.x * hs.stride;
     size_t o = 0;
     auto take = [&](size_t bytes) {
         uint8_t* p = base + o;
@@ -553,22 +561,25 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.cand = (uint32_t*)take(8ull * ce);
     w.code = s_code;
     seed_table_init(sh);
-    uint64_t E;
-    uint32_t bc;
-    PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
-    const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
-    const int rc = seed_one<TabT, uint32_t, SeedCap<256>::KPL>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
+    for (uint32_t hi = blockIdx.x; hi < count; hi += gridDim.x) {
+        const uint32_t ro = bd.heavy_ro[hs.first + hi];
+        uint64_t E;
+        uint32_t bc;
+        PoolChunk pc = {0, 0, 0};
+        const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
+        const int rc = seed_one<TabT, uint32_t, SeedCap<256>::KPL>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
+        if (rc != 0 && threadIdx.x == 0) {
+            atomicOr(bd.flags, FLAG_CAPACITY);
+            bd.cand_nf[ro] = 0;
+            bd.cand_nr[ro] = 0;
+        }
+    }
     if (threadIdx.x == 0 && ss.seeded) {
         atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
-    }
-    if (rc != 0 && threadIdx.x == 0) {
-        atomicOr(bd.flags, FLAG_CAPACITY);
-        bd.cand_nf[ro] = 0;
-        bd.cand_nr[ro] = 0;
     }
 }
 
@@ -644,12 +655,9 @@ __device__ __forceinline__ uint32_t spread16(uint32_t x)
 // returns the number of window bytes scanned (0 when the lane has no job)
 template <int W>
 __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
-                                               const uint32_t* __restrict__ only, uint32_t n_only,
+                                               const uint32_t* __restrict__ only, uint32_t tid,
                                                uint64_t* __restrict__ s_peq)
 {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t njobs = only ? n_only : bd.pool_n[pool];
-    if (tid >= njobs) return 0;
     if (only && bd.ovf_pool[tid] != pool) return 0;
     const uint32_t j = only ? only[tid] : tid;
     const uint32_t meta = bd.job_meta[pool][j];
@@ -1025,10 +1033,21 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
     __shared__ unsigned long long s_work[2];
     if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t t = verify_one<W>(ix, bd, pool, only, n_only, s_peq);
-    if (t && !only) {
-        atomicAdd(&s_work[0], 1ull);
-        atomicAdd(&s_work[1], (unsigned long long)t);
+    // plain form: one thread per job (the host sizes the grid); batched form:
+    // persistent blocks over the device's job count, nothing after an abort
+    uint32_t njobs;
+    if (only) njobs = n_only != FAME_JOBS_ON_DEVICE ? n_only : min(*bd.ovf_n, bd.ovf_list_cap);
+    else njobs = min(bd.pool_n[pool], pc_get_cap(bd, pool));
+    if (bd.n_dev && (*bd.flags & FLAG_ABORT)) njobs = 0;
+    uint64_t jobs = 0, bytes = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < njobs; t += gridDim.x * blockDim.x) {
+        const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq);
+        jobs += b ? 1 : 0;
+        bytes += b;
+    }
+    if (jobs && !only) {
+        atomicAdd(&s_work[0], (unsigned long long)jobs);
+        atomicAdd(&s_work[1], (unsigned long long)bytes);
     }
     __syncthreads();
     if (threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&bd.stats[3 + threadIdx.x], s_work[threadIdx.x]);
@@ -1102,6 +1121,10 @@ __device__ int query_seed_set(const DevIndex& ix, const BatchDev& bd, uint32_t r
 __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32_t n,
                                                 int get_stranded)
 {
+    if (bd.n_dev) {
+        if (*bd.flags & FLAG_ABORT) return; // the host re-runs the batch on the plain path
+        n = *bd.n_dev;
+    }
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ uint32_t s_cnt[5];
     if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
@@ -1635,6 +1658,10 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
 __global__ void k_ovf_offsets(BatchDev bd, uint32_t n_ovf, uint32_t* total)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (n_ovf == FAME_JOBS_ON_DEVICE) {
+        if (*bd.flags & FLAG_ABORT) return;
+        n_ovf = min(*bd.ovf_n, bd.ovf_list_cap);
+    }
     uint32_t o = 0;
     for (uint32_t i = 0; i < n_ovf; ++i) {
         const uint32_t p = bd.ovf_pool[i], j = bd.ovf_job[i];
@@ -1642,6 +1669,25 @@ __global__ void k_ovf_offsets(BatchDev bd, uint32_t n_ovf, uint32_t* total)
         o += bd.job_nm[p][j];
     }
     *total = o;
+    if (bd.ovf_buf_cap && o > bd.ovf_buf_cap) atomicOr(bd.flags, FLAG_OVF_FULL);
+}
+
+// batched form: one batch's counters, lists and flags cleared (a kernel, so
+// that the whole batch can be replayed from a graph)
+__global__ void k_batch_reset(BatchDev bd)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < FAME_NPOOL) bd.pool_n[t] = 0;
+    if (t == 0) {
+        *bd.heavy_n = 0;
+        *bd.ovf_n = 0;
+        *bd.direct_n = 0;
+        *bd.dp_n = 0;
+        bd.flags[0] = 0;
+        *bd.max_len = 0;
+    }
+    if (t < FAME_NSTATS) bd.stats[t] = 0;
+    if (t < 3) bd.counters[t] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1691,7 +1737,7 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
     const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
-                       0xFFFFFFFFu, 0u, 0);
+                       0xFFFFFFFFu, 0u, 0, bd.n_dev);
 }
 
 // paired batches: 2n mate reads, mate 2 from nsplit on; pre[] = 1 short | 2 N
@@ -1702,7 +1748,7 @@ void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, ui
     const uint32_t blocks = (n_reads + 3) / 4;
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n_reads, 0u, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
-                       nsplit, minlen, 1);
+                       nsplit, minlen, 1, (const uint32_t*)nullptr);
 }
 
 void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st)
@@ -1734,11 +1780,23 @@ void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratc
         hipLaunchKernelGGL(k_seed_heavy<uint32_t>, dim3(hs.count), dim3(64), 0, st, ix, bd, hs);
 }
 
+void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
+                           hipStream_t st)
+{
+    HeavyScratch h = hs;
+    h.first = 0;
+    h.count = FAME_JOBS_ON_DEVICE;
+    if (ix.tab_is64)
+        hipLaunchKernelGGL(k_seed_heavy<uint64_t>, dim3(blocks), dim3(64), 0, st, ix, bd, h);
+    else
+        hipLaunchKernelGGL(k_seed_heavy<uint32_t>, dim3(blocks), dim3(64), 0, st, ix, bd, h);
+}
+
 void launch_verify(const DevIndex& ix, const BatchDev& bd, uint32_t pool, uint32_t njobs,
-                   const uint32_t* only, hipStream_t st)
+                   const uint32_t* only, hipStream_t st, uint32_t blocks)
 {
     if (!njobs) return;
-    const dim3 g((njobs + 255) / 256), b(256);
+    const dim3 g(njobs == FAME_JOBS_ON_DEVICE ? blocks : (njobs + 255) / 256), b(256);
     switch (pool) {
     case 0: hipLaunchKernelGGL(k_verify<1>, g, b, 0, st, ix, bd, 0u, only, only ? njobs : 0u); break;
     case 1: hipLaunchKernelGGL(k_verify<2>, g, b, 0, st, ix, bd, 1u, only, only ? njobs : 0u); break;
@@ -1776,6 +1834,11 @@ void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* d
 void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st)
 {
     hipLaunchKernelGGL(k_ovf_offsets, dim3(1), dim3(1), 0, st, bd, n_ovf, total);
+}
+
+void launch_batch_reset(const BatchDev& bd, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, bd);
 }
 
 __global__ void k_max_end(const uint64_t* __restrict__ offs, const uint32_t* __restrict__ lens, uint32_t n,

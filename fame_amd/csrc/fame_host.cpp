@@ -64,6 +64,8 @@ struct FameError {
 
 } // namespace
 
+struct GraphPlan;
+
 struct fame_ctx {
     std::string err;
     int device = 0;
@@ -104,6 +106,10 @@ struct fame_ctx {
     uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
     double timings[6] = {0, 0, 0, 0, 0, 0};
     uint64_t last_heavy = 0, last_ovf = 0;
+    // batched form: plans per capacity class, their graphs, and counts
+    int form = 0; // 0 plain, 1 batched
+    std::vector<std::unique_ptr<GraphPlan>> plans;
+    uint64_t graph_calls = 0, graph_fallbacks = 0, graph_captures = 0;
     double big_ms = 0; // paired seed runs with global scratch, last call
 };
 
@@ -626,6 +632,312 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     return FAME_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batched form: the whole batch as one graph launch
+// ---------------------------------------------------------------------------
+
+} // namespace
+
+// candidate persistent grids of the verify kernels, tried on the first
+// batches of a plan (online: every trial batch is real work), then fixed
+constexpr uint32_t kVerifyBlocks[] = {1024, 2048, 4096, 8192};
+constexpr int kNCand = 4;
+
+struct GraphKey {
+    const void* s;
+    const void* o;
+    const void* l;
+    int stranded;
+    uint32_t ori;
+    int cand;
+    bool operator==(const GraphKey& k) const
+    {
+        return s == k.s && o == k.o && l == k.l && stranded == k.stranded && ori == k.ori && cand == k.cand;
+    }
+};
+
+struct GraphPlan {
+    uint32_t cap = 0; // reads per batch
+    uint64_t rev_bytes = 0;
+    DevBuf rev, pre, codes, cand_off, cand_nf, cand_nr, pool_n;
+    DevBuf job_meta[FAME_NPOOL], job_ro[FAME_NPOOL], job_nm[FAME_NPOOL], job_match[FAME_NPOOL],
+        job_ovf_off[FAME_NPOOL];
+    DevBuf ovf_n, ovf_pool, ovf_job, ovf_buf, ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy_scratch;
+    DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace;
+    uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 256;
+    HeavyScratch hs{};
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+    uint32_t* h_io = nullptr; // pinned: [0] n, [1] flags, [2..7] counters (3 x u64)
+    // autotuner: ms per read of each candidate, the choice (-1 while trying)
+    double cand_ms[kNCand] = {0, 0, 0, 0};
+    int cand_reads[kNCand] = {0, 0, 0, 0};
+    int chosen = -1;
+    int next_trial = 0;
+    void drop_graphs()
+    {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
+    ~GraphPlan()
+    {
+        drop_graphs();
+        if (h_io) (void)hipHostFree(h_io);
+    }
+};
+
+namespace {
+
+GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes)
+{
+    uint32_t cap = 1u << 16;
+    while (cap < n) cap <<= 1;
+    GraphPlan* pl = nullptr;
+    for (auto& p : c->plans)
+        if (p->cap == cap) pl = p.get();
+    if (!pl) {
+        c->plans.emplace_back(new GraphPlan());
+        pl = c->plans.back().get();
+        pl->cap = cap;
+        const uint64_t n_ro = 2ull * cap;
+        HIPC(pl->pre.reserve(cap + 1));
+        HIPC(pl->codes.reserve(n_ro * 256 + 256));
+        HIPC(pl->cand_off.reserve(4 * n_ro + 4));
+        HIPC(pl->cand_nf.reserve(4 * n_ro + 4));
+        HIPC(pl->cand_nr.reserve(4 * n_ro + 4));
+        HIPC(pl->pool_n.reserve(4 * FAME_NPOOL));
+        pl->pool_cap = std::max<uint32_t>(1u << 16, 4 * cap) + 64u * FAME_SEED_WAVES;
+        for (int p = 0; p < FAME_NPOOL; ++p) {
+            HIPC(pl->job_meta[p].reserve(4ull * pl->pool_cap));
+            HIPC(pl->job_ro[p].reserve(4ull * pl->pool_cap));
+            HIPC(pl->job_nm[p].reserve(4ull * pl->pool_cap));
+            HIPC(pl->job_match[p].reserve(4ull * pl->pool_cap * c->match_cap));
+            HIPC(pl->job_ovf_off[p].reserve(4ull * pl->pool_cap));
+        }
+        pl->ovf_list_cap = std::max<uint32_t>(1024, cap / 4);
+        pl->ovf_buf_cap = 1u << 24;
+        HIPC(pl->ovf_n.reserve(8));
+        HIPC(pl->ovf_pool.reserve(4ull * pl->ovf_list_cap));
+        HIPC(pl->ovf_job.reserve(4ull * pl->ovf_list_cap));
+        HIPC(pl->ovf_buf.reserve(4ull * pl->ovf_buf_cap));
+        HIPC(pl->ovf_total.reserve(8));
+        pl->heavy_cap = (uint32_t)n_ro;
+        HIPC(pl->heavy_n.reserve(8));
+        HIPC(pl->heavy_ro.reserve(4 * n_ro));
+        HIPC(pl->heavy_e.reserve(4 * n_ro));
+        HIPC(pl->heavy_bc.reserve(4 * n_ro));
+        // heavy (read, orientation) pairs of up to 16384 index hits in
+        // graph; larger ones make the batch fall back to the plain path
+        const uint64_t emax = 16384;
+        HeavyScratch& hs = pl->hs;
+        hs.cap_e = (uint32_t)emax;
+        hs.hsize = 2 * emax;
+        hs.tb_cap = (uint32_t)(2 * (2 * emax) + 64);
+        size_t stride = 0;
+        auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+        add(8 * FAME_DEV_MAX_READ);
+        add(4 * FAME_DEV_MAX_READ);
+        add(4 * FAME_DEV_MAX_READ);
+        add(4 * emax);
+        add(emax);
+        add(4 * emax);
+        for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
+        add(8 * emax);
+        add(4 * emax);
+        add(4ull * hs.tb_cap);
+        add(4 * emax);
+        add(4 * emax);
+        add(8 * emax);
+        hs.stride = stride;
+        HIPC(pl->heavy_scratch.reserve(stride * pl->heavy_blocks));
+        hs.base = pl->heavy_scratch.as<uint8_t>();
+        HIPC(pl->status.reserve(cap + 1));
+        HIPC(pl->use.reserve(cap + 1));
+        HIPC(pl->mat.reserve(8ull * cap + 8));
+        HIPC(pl->valid.reserve(cap + 1));
+        HIPC(pl->stats.reserve(8 * FAME_NSTATS));
+        HIPC(pl->direct.reserve(4ull * cap + 4));
+        HIPC(pl->dplist.reserve(4ull * cap + 4));
+        HIPC(pl->methn.reserve(8));
+        HIPC(pl->flags.reserve(16));
+        HIPC(pl->ndev.reserve(8));
+        const size_t cells = (size_t)FAME_DP_SROWS * (2 * (FAME_DPBAND + 1) + 1);
+        HIPC(pl->dp.reserve(2 * cells * c->meth_threads));
+        HIPC(pl->dec.reserve(8ull * (FAME_DEV_MAX_READ + 1) * c->meth_threads));
+        HIPC(pl->trace.reserve(8ull * FAME_DP_TRACE_WORDS * c->meth_threads));
+        HIPC(hipHostMalloc((void**)&pl->h_io, 64, 0));
+    }
+    if (total_bytes + 64 > pl->rev_bytes) {
+        // the graphs hold the old pointer
+        pl->drop_graphs();
+        HIPC(pl->rev.reserve(total_bytes + 64));
+        pl->rev_bytes = total_bytes + 64;
+    }
+    return pl;
+}
+
+BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uint64_t* d_offs,
+                    const uint32_t* d_lens, uint32_t ori_mask)
+{
+    BatchDev bd{};
+    bd.n_dev = pl->ndev.as<uint32_t>();
+    bd.seqs = d_seqs;
+    bd.offs = d_offs;
+    bd.lens = d_lens;
+    bd.rev = pl->rev.as<uint8_t>();
+    bd.pre = pl->pre.as<uint8_t>();
+    bd.codes = pl->codes.as<uint8_t>();
+    bd.ori_mask = ori_mask;
+    bd.cand_off = pl->cand_off.as<uint32_t>();
+    bd.cand_nf = pl->cand_nf.as<uint32_t>();
+    bd.cand_nr = pl->cand_nr.as<uint32_t>();
+    bd.pool_n = pl->pool_n.as<uint32_t>();
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        bd.pool_cap[p] = pl->pool_cap;
+        bd.job_meta[p] = pl->job_meta[p].as<uint32_t>();
+        bd.job_ro[p] = pl->job_ro[p].as<uint32_t>();
+        bd.job_nm[p] = pl->job_nm[p].as<uint32_t>();
+        bd.job_match[p] = pl->job_match[p].as<uint32_t>();
+        bd.job_ovf_off[p] = pl->job_ovf_off[p].as<uint32_t>();
+    }
+    bd.match_cap = c->match_cap;
+    bd.ovf_n = pl->ovf_n.as<uint32_t>();
+    bd.ovf_pool = pl->ovf_pool.as<uint32_t>();
+    bd.ovf_job = pl->ovf_job.as<uint32_t>();
+    bd.ovf_list_cap = pl->ovf_list_cap;
+    bd.ovf_buf = pl->ovf_buf.as<uint32_t>();
+    bd.ovf_buf_cap = pl->ovf_buf_cap;
+    bd.heavy_n = pl->heavy_n.as<uint32_t>();
+    bd.heavy_ro = pl->heavy_ro.as<uint32_t>();
+    bd.heavy_e = pl->heavy_e.as<uint32_t>();
+    bd.heavy_bc = pl->heavy_bc.as<uint32_t>();
+    bd.heavy_cap = pl->heavy_cap;
+    bd.status = pl->status.as<uint8_t>();
+    bd.use = pl->use.as<uint8_t>();
+    bd.mat = pl->mat.as<uint64_t>();
+    bd.valid = pl->valid.as<uint8_t>();
+    bd.counters = c->d_counters.as<uint64_t>();
+    bd.meth = c->d_meth.as<uint32_t>();
+    bd.direct_list = pl->direct.as<uint32_t>();
+    bd.dp_list = pl->dplist.as<uint32_t>();
+    bd.direct_n = pl->methn.as<uint32_t>();
+    bd.dp_n = bd.direct_n + 1;
+    bd.flags = pl->flags.as<uint32_t>();
+    bd.max_len = bd.flags + 1;
+    bd.stats = pl->stats.as<unsigned long long>();
+    return bd;
+}
+
+// one batch through the plan's graph; 1 = the batch outgrew the graph's
+// fixed capacities and must run on the plain path (nothing was changed)
+int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+                    uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st,
+                    const uint8_t** out_valid, const uint64_t** out_mat, const uint8_t** out_status)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    GraphPlan* pl = plan_for(c, n, total_bytes);
+    const bool o0 = c->both || get_stranded || c->match_r1_fwd;
+    const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
+    const uint32_t ori = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
+    if (pl->chosen < 0)
+        for (auto& q : c->plans)
+            if (q->chosen >= 0) pl->chosen = q->chosen; // the grid suits the GPU, not the class
+    const int cand = pl->chosen >= 0 ? pl->chosen : pl->next_trial;
+    GraphKey key{d_seqs, d_offs, d_lens, get_stranded, ori, cand};
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : pl->graphs)
+        if (g.first == key) exec = g.second;
+    if (!exec) {
+        BatchDev bd = plan_batch(c, pl, d_seqs, d_offs, d_lens, ori);
+        const uint32_t cap = pl->cap;
+        hipGraph_t graph = nullptr;
+        HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch_batch_reset(bd, st);
+        launch_prep(bd, cap, c->readlen, st);
+        launch_seed_light(c->ix, bd, 2 * cap, st);
+        launch_seed_heavy_dev(c->ix, bd, pl->hs, pl->heavy_blocks, st);
+        for (int p = 0; p < FAME_NPOOL; ++p)
+            launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, kVerifyBlocks[cand]);
+        launch_ovf_offsets(bd, FAME_JOBS_ON_DEVICE, pl->ovf_total.as<uint32_t>(), st);
+        for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, bd.ovf_job, st, 64);
+        launch_select(c->ix, bd, cap, get_stranded, st);
+        launch_meth(c->ix, bd, cap, pl->dp.as<uint16_t>(), pl->dec.as<uint64_t>(), pl->trace.as<uint64_t>(),
+                    c->meth_threads, st);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            HIPC(e);
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPC(e);
+        pl->graphs.push_back({key, exec});
+        c->graph_captures += 1;
+    }
+    Timer tm(st);
+    pl->h_io[0] = n;
+    HIPC(hipMemcpyAsync(pl->ndev.p, pl->h_io, 4, hipMemcpyHostToDevice, st));
+    tm.mark(0);
+    HIPC(hipGraphLaunch(exec, st));
+    tm.mark(1);
+    HIPC(hipMemcpyAsync(pl->h_io + 1, pl->flags.p, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(pl->h_io + 2, c->d_counters.p, 24, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(c->stats, pl->stats.p, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    c->graph_calls += 1;
+    const uint32_t flags = pl->h_io[1];
+    // the batch's counters stay on the device until read: clear them for the
+    // next call of either form
+    HIPC(hipMemsetAsync(c->d_counters.p, 0, 24, st));
+    HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_ABORT) {
+        c->graph_fallbacks += 1;
+        return 1;
+    }
+    const double ms = tm.ms(0, 1);
+    if (pl->chosen < 0 && n >= pl->cap / 4) {
+        pl->cand_ms[cand] += ms;
+        pl->cand_reads[cand] += (int)std::min<uint32_t>(n, 0x7FFFFFFF);
+        pl->next_trial += 1;
+        if (pl->next_trial >= kNCand) {
+            int best = 0;
+            for (int k = 1; k < kNCand; ++k)
+                if (pl->cand_ms[k] / std::max(1, pl->cand_reads[k]) < pl->cand_ms[best] / std::max(1, pl->cand_reads[best]))
+                    best = k;
+            pl->chosen = best;
+        }
+    }
+    uint64_t cnt[3];
+    memcpy(cnt, pl->h_io + 2, 24);
+    counters[0] += cnt[0];
+    counters[1] += cnt[1];
+    counters[2] += cnt[2];
+    *out_valid = pl->valid.as<uint8_t>();
+    *out_mat = pl->mat.as<uint64_t>();
+    if (out_status) *out_status = pl->status.as<uint8_t>();
+    c->timings[0] = c->timings[1] = c->timings[2] = c->timings[3] = c->timings[4] = 0;
+    c->timings[1] = ms; // the whole graph
+    c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FAME_OK;
+}
+
+// one single-end batch in the context's form; the per-read outputs stay in
+// device buffers whose addresses are returned
+int match_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+                uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st,
+                const uint8_t** out_valid, const uint64_t** out_mat, const uint8_t** out_status)
+{
+    if (c->form == 1 && n) {
+        const int rc = run_batch_graph(c, d_seqs, d_offs, d_lens, n, total_bytes, get_stranded, counters, st,
+                                       out_valid, out_mat, out_status);
+        if (rc != 1) return rc;
+    }
+    const int rc = run_batch(c, d_seqs, d_offs, d_lens, n, total_bytes, get_stranded, counters, st);
+    *out_valid = c->b_valid.as<uint8_t>();
+    *out_mat = c->b_mat.as<uint64_t>();
+    if (out_status) *out_status = c->b_status.as<uint8_t>();
+    return rc;
+}
+
 // verify jobs of one phase: the match arrays, then the jobs whose matches
 // overflowed match_cap once more into an exactly sized buffer
 int verify_phase(fame_ctx* c, BatchDev& bd, const uint32_t pool_n[FAME_NPOOL], DevBuf (&nm)[FAME_NPOOL],
@@ -972,12 +1284,7 @@ int serve_run(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
 {
     return guarded(c, [&]() {
         uint64_t cnt[3] = {0, 0, 0};
-        int rc = run_batch(c, d_seqs, d_offs, d_lens, n, total_bytes, 0, cnt, st);
-        if (rc != FAME_OK) return rc;
-        *d_status = c->b_status.as<uint8_t>();
-        *d_valid = c->b_valid.as<uint8_t>();
-        *d_mat = c->b_mat.as<uint64_t>();
-        return (int)FAME_OK;
+        return match_batch(c, d_seqs, d_offs, d_lens, n, total_bytes, 0, cnt, st, d_valid, d_mat, d_status);
     });
 }
 
@@ -1064,10 +1371,12 @@ int fame_match_reads(fame_ctx* c, const uint8_t* seqs, const uint64_t* offs, con
             HIPC(hipMemcpyAsync(d_o, offs, 8ull * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_l, lens, 4ull * n, hipMemcpyHostToDevice, st));
         }
-        int rc = run_batch(c, d_s, d_o, d_l, n, total, get_stranded, counters, st);
+        const uint8_t* ov = nullptr;
+        const uint64_t* om = nullptr;
+        int rc = match_batch(c, d_s, d_o, d_l, n, total, get_stranded, counters, st, &ov, &om, nullptr);
         if (rc != FAME_OK) return rc;
-        if (n && valid) HIPC(hipMemcpyAsync(valid, c->b_valid.p, n, hipMemcpyDeviceToHost, st));
-        if (n && mat) HIPC(hipMemcpyAsync(mat, c->b_mat.p, 8ull * n, hipMemcpyDeviceToHost, st));
+        if (n && valid) HIPC(hipMemcpyAsync(valid, ov, n, hipMemcpyDeviceToHost, st));
+        if (n && mat) HIPC(hipMemcpyAsync(mat, om, 8ull * n, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });
@@ -1092,10 +1401,12 @@ int fame_match_reads_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* 
             HIPC(hipStreamSynchronize(st));
             total = e;
         }
-        int rc = run_batch(c, d_seqs, d_offs, d_lens, n, total, get_stranded, counters, st);
+        const uint8_t* ov = nullptr;
+        const uint64_t* om = nullptr;
+        int rc = match_batch(c, d_seqs, d_offs, d_lens, n, total, get_stranded, counters, st, &ov, &om, nullptr);
         if (rc != FAME_OK) return rc;
-        if (n && d_valid) HIPC(hipMemcpyAsync(d_valid, c->b_valid.p, n, hipMemcpyDeviceToDevice, st));
-        if (n && d_mat) HIPC(hipMemcpyAsync(d_mat, c->b_mat.p, 8ull * n, hipMemcpyDeviceToDevice, st));
+        if (n && d_valid) HIPC(hipMemcpyAsync(d_valid, ov, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat) HIPC(hipMemcpyAsync(d_mat, om, 8ull * n, hipMemcpyDeviceToDevice, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });
@@ -1208,6 +1519,31 @@ int fame_match_pairs_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* 
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });
+}
+
+int fame_set_form(fame_ctx* c, int form)
+{
+    if (!c || (form != FAME_FORM_PLAIN && form != FAME_FORM_BATCHED)) return FAME_ERR_ARG;
+    c->form = form;
+    return FAME_OK;
+}
+
+int fame_form_info(const fame_ctx* c, uint64_t out[FAME_FORM_FIELDS])
+{
+    if (!c || !out) return FAME_ERR_ARG;
+    out[0] = (uint64_t)c->form;
+    out[1] = c->graph_calls;
+    out[2] = c->graph_fallbacks;
+    out[3] = c->graph_captures;
+    out[4] = c->plans.size();
+    out[5] = 0;
+    uint32_t big = 0;
+    for (auto& p : c->plans)
+        if (p->cap >= big && p->chosen >= 0) {
+            big = p->cap;
+            out[5] = kVerifyBlocks[p->chosen];
+        }
+    return FAME_OK;
 }
 
 int fame_decide_strand(fame_ctx* c, uint64_t* r1_fwd, uint64_t* r1_rev, int* match_r1_fwd)

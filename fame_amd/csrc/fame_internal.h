@@ -29,6 +29,9 @@ enum : uint32_t {
     FLAG_HEAVY_FULL = 4u,
     FLAG_CAPACITY = 8u,
     FLAG_OVF_FULL = 16u,
+    // any of these makes the batched form's later stages skip the batch,
+    // which the host then runs on the plain path
+    FLAG_ABORT = 1u | 2u | 4u | 8u | 16u,
 };
 
 // The index as resident in HBM.
@@ -64,6 +67,7 @@ struct DevIndex {
 // Per-batch device buffers.  A (read, orientation) pair is ro = 2*read + ori
 // (ori 0 = the read as given, 1 = its reverse complement).
 struct BatchDev {
+    const uint32_t* n_dev; // batched form: the batch size in device memory (NULL: plain form)
     const uint8_t* seqs;
     const uint64_t* offs;
     const uint32_t* lens;
@@ -90,6 +94,7 @@ struct BatchDev {
     uint32_t* ovf_job;
     uint32_t ovf_list_cap;
     uint32_t* ovf_buf;
+    uint32_t ovf_buf_cap; // batched form: entries of ovf_buf (0: sized by the host)
     // heavy (read, orientation) pairs deferred from the LDS seed kernel
     uint32_t* heavy_n;
     uint32_t* heavy_ro;
@@ -164,10 +169,19 @@ struct HeavyScratch {
 };
 
 void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st);
+// batched form: counters and lists of one batch cleared on the device
+void launch_batch_reset(const BatchDev& bd, hipStream_t st);
+// batched form: the deferred (read, orientation) pairs from the device list,
+// `blocks` scratch slices of hs
+void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
+                           hipStream_t st);
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st);
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st);
+// njobs = FAME_JOBS_ON_DEVICE: as many as the device counter holds, on
+// `blocks` persistent blocks (batched form)
+#define FAME_JOBS_ON_DEVICE 0xFFFFFFFFu
 void launch_verify(const DevIndex& ix, const BatchDev& bd, uint32_t pool, uint32_t njobs,
-                   const uint32_t* only, hipStream_t st);
+                   const uint32_t* only, hipStream_t st, uint32_t blocks = 0);
 void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_stranded, hipStream_t st);
 // DP scratch per lane: FAME_DP_SROWS * 19 u16 values, FAME_DEV_MAX_READ + 1
 // u64 choice words, FAME_DP_TRACE_WORDS u64 trace words (lane-strided)

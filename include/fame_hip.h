@@ -121,6 +121,28 @@ int fame_reset(fame_ctx* ctx);
  * <basename>_cpg.tsv, byte-identical to the reference's report */
 int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
 
+/* ---- batched form (no counterpart in the reference) ----------------------
+ * FAME_FORM_PLAIN (default): one kernel per stage, the host reads the job
+ * counts between stages to size the next launches.  FAME_FORM_BATCHED: the
+ * whole single-end batch is one hipGraph launch (captured per input buffers
+ * and capacity class of batch sizes, replayed afterwards): the counts stay on
+ * the device, the verify / heavy-seed / overflow kernels are persistent over
+ * them, and the only host wait is at the end.  A batch that outgrows the
+ * graph's fixed capacities (a pool, 16384 index hits for one read, the match
+ * overflow buffer) is detected on the device before anything is counted and
+ * re-run on the plain path, so results are the same in either form.  The
+ * persistent verify grid is picked per capacity class by timing candidate
+ * grids on the first batches (online autotuning on real work).  Paired
+ * matching always uses the plain form. */
+#define FAME_FORM_PLAIN 0
+#define FAME_FORM_BATCHED 1
+int fame_set_form(fame_ctx* ctx, int form);
+
+/* form, batched calls, of which re-run plain, graphs captured, capacity
+ * classes, the autotuned verify grid (blocks) of the largest tuned class */
+#define FAME_FORM_FIELDS 6
+int fame_form_info(const fame_ctx* ctx, uint64_t out[FAME_FORM_FIELDS]);
+
 /* ---- serving loop (no counterpart in the reference) ----------------------
  * A request loop over the context's resident index for callers that submit
  * chunks of single-end reads as they arrive (the strand decided beforehand,

@@ -1,0 +1,76 @@
+"""Batched form (one hipGraph launch per batch, counts kept on the device):
+bit-exact against FAME's outputs on the golden fixtures, and every replay of
+a captured graph equals the plain form's result for the same chunk, with
+eager work of the other form between replays."""
+import os
+
+import numpy as np
+import pytest
+
+from fame_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def stream(ctx, c, chunk):
+    n = len(c.lens)
+    valid = np.zeros(n, np.uint8)
+    mat = np.zeros(n, np.uint64)
+    cnt = np.zeros(3, np.uint64)
+    s = 0
+    if not c.unord:
+        e = min(n, chunk)
+        valid[s:e], mat[s:e] = ctx.match(c.seqs, c.offs[s:e], c.lens[s:e], True, cnt)
+        ctx.decide_strand()
+        s = e
+    while s < n:
+        e = min(n, s + chunk)
+        valid[s:e], mat[s:e] = ctx.match(c.seqs, c.offs[s:e], c.lens[s:e], False, cnt)
+        s = e
+    return valid, mat, cnt
+
+
+def test_batched_matches_reference(golden_case, tmp_path):
+    c = golden_case
+    print(f"case {c.name}", flush=True)
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    ctx.set_form("batched")
+    valid, mat, cnt = stream(ctx, c, max(len(c.lens) // 3, 1))
+    info = ctx.form_info()
+    assert info["batched_calls"] >= 1
+    assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    base = os.path.join(str(tmp_path), "b")
+    ctx.print_methylation_levels(base)
+    assert open(base + "_cpg.tsv", "rb").read() == c.tsv
+    ctx.close()
+
+
+def test_graph_replays_equal_plain(golden_case):
+    """The same chunk six times through its graph (captured once per
+    autotuner candidate, then replayed), a plain call of another chunk
+    between replays; every replay equals the plain result."""
+    c = golden_case
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    n = len(c.lens)
+    half = n // 2
+    cnt = np.zeros(3, np.uint64)
+    if not c.unord:
+        ctx.match(c.seqs, c.offs[:half], c.lens[:half], True, cnt)
+        ctx.decide_strand()
+    a = (c.seqs, c.offs[half:], c.lens[half:])
+    b = (c.seqs, c.offs[:half], c.lens[:half])
+    ctx.set_form("plain")
+    pv, pm = ctx.match(*a, False, cnt)
+    pv, pm = pv.copy(), pm.copy()
+    for k in range(6):
+        ctx.set_form("batched")
+        cb = np.zeros(3, np.uint64)
+        v, m = ctx.match(*a, False, cb)
+        assert np.array_equal(v, pv) and np.array_equal(m, pm), f"replay {k}"
+        ctx.set_form("plain")
+        ctx.match(*b, False, np.zeros(3, np.uint64))
+    info = ctx.form_info()
+    assert info["batched_calls"] == 6 and info["fallbacks"] == 0
+    ctx.close()
