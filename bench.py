@@ -148,6 +148,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per load point")
     ap.add_argument("--serve-form", default="batched", help="pipeline form of the server's batches")
+    ap.add_argument("--sub-batch", type=int, default=0, help="batched form: reads per pipelined sub-batch (0 = library default)")
     ap.add_argument("--forms", default="plain,batched",
                     help="single-end forms to time (plain: one kernel per stage; batched: one graph per batch)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -239,15 +240,21 @@ def main():
                                  e - s, stranded, cnt, d_valid.data_ptr() + s, d_mat.data_ptr() + 8 * s)
             calls.append((e - s, ctx.timings(), ctx.stats()))
 
+        cur = {"form": "plain"}
+
         def step():
+            # plain: FAME's loop with batches of --batch reads; batched: the
+            # rest in one call, which the library splits into pipelined
+            # sub-batches on two lanes
             ctx.reset()
             cnt = np.zeros(ncnt, np.uint64)
             e = min(n, CHUNKSIZE)
             call(0, e, True, cnt)
             ctx.decide_strand()
             s = e
+            B = n if cur["form"] == "batched" else args.batch
             while s < n:
-                e = min(n, s + args.batch)
+                e = min(n, s + B)
                 call(s, e, False, cnt)
                 s = e
             return cnt
@@ -255,8 +262,11 @@ def main():
         forms = ["plain"] if paired else [f for f in args.forms.split(",") if f]
         results = {}
         for form in forms:
+            cur["form"] = form
             if not paired:
                 ctx.set_form(form)
+                if args.sub_batch:
+                    ctx.set_pipeline(args.sub_batch)
             for _ in range(args.warmup):
                 step()
             calls.clear()

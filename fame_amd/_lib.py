@@ -21,7 +21,7 @@ MAX_READ = 255
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
            "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
-           "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_set_form", "fame_form_info", "fame_serve_open", "fame_serve_submit",
+           "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_set_form", "fame_set_pipeline", "fame_form_info", "fame_serve_open", "fame_serve_submit",
            "fame_serve_wait", "fame_serve_stats", "fame_serve_close",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
@@ -62,6 +62,7 @@ def load():
         L.fame_decide_strand.argtypes = [C.c_void_p, _u64p, _u64p, C.POINTER(C.c_int)]
         L.fame_set_form.argtypes = [C.c_void_p, C.c_int]
         L.fame_form_info.argtypes = [C.c_void_p, _u64p]
+        L.fame_set_pipeline.argtypes = [C.c_void_p, C.c_uint32]
         L.fame_serve_open.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]
         L.fame_serve_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                         C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
@@ -159,11 +160,14 @@ class Context:
         """"plain" (one kernel per stage) or "batched" (one graph launch per batch)"""
         self._check(load().fame_set_form(self.h, {"plain": 0, "batched": 1}[form]))
 
+    def set_pipeline(self, sub_batch_reads):
+        self._check(load().fame_set_pipeline(self.h, int(sub_batch_reads)))
+
     def form_info(self):
-        t = np.zeros(6, np.uint64)
+        t = np.zeros(8, np.uint64)
         self._check(load().fame_form_info(self.h, ptr(t, _u64p)))
-        return dict(zip(("form", "batched_calls", "fallbacks", "graphs", "classes", "verify_blocks"),
-                        (int(x) for x in t)))
+        return dict(zip(("form", "graph_launches", "fallbacks", "graphs", "classes", "verify_blocks",
+                         "pipelined_calls", "sub_batch"), (int(x) for x in t)))
 
     def decide_strand(self):
         f = C.c_uint64()

@@ -1687,7 +1687,6 @@ __global__ void k_batch_reset(BatchDev bd)
         *bd.max_len = 0;
     }
     if (t < FAME_NSTATS) bd.stats[t] = 0;
-    if (t < 3) bd.counters[t] = 0;
 }
 
 // ---------------------------------------------------------------------------

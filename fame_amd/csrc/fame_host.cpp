@@ -70,6 +70,7 @@ struct fame_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr; // second lane of the batched form
     bool both = false;
     bool match_r1_fwd = true;
     // header
@@ -110,6 +111,9 @@ struct fame_ctx {
     int form = 0; // 0 plain, 1 batched
     std::vector<std::unique_ptr<GraphPlan>> plans;
     uint64_t graph_calls = 0, graph_fallbacks = 0, graph_captures = 0;
+    uint32_t pipe_batch = 1u << 21; // sub-batch of a large batched call (two lanes)
+    DevBuf b_pvalid, b_pmat, b_pstatus;  // outputs of a pipelined call
+    uint64_t pipe_calls = 0;
     double big_ms = 0; // paired seed runs with global scratch, last call
 };
 
@@ -658,16 +662,19 @@ struct GraphKey {
 
 struct GraphPlan {
     uint32_t cap = 0; // reads per batch
+    int lane = 0;     // 0: single calls; 1, 2: the two lanes of a pipelined call
     uint64_t rev_bytes = 0;
     DevBuf rev, pre, codes, cand_off, cand_nf, cand_nr, pool_n;
     DevBuf job_meta[FAME_NPOOL], job_ro[FAME_NPOOL], job_nm[FAME_NPOOL], job_match[FAME_NPOOL],
         job_ovf_off[FAME_NPOOL];
     DevBuf ovf_n, ovf_pool, ovf_job, ovf_buf, ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy_scratch;
-    DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace;
+    DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace, counters;
     uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 256;
     HeavyScratch hs{};
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
-    uint32_t* h_io = nullptr; // pinned: [0] n, [1] flags, [2..7] counters (3 x u64)
+    // pinned: [0] n, [1] flags, [2..7] counters (3 x u64); pipelined calls:
+    // [16 + k] n and [32 + k] flags of the lane's k-th sub-batch
+    uint32_t* h_io = nullptr;
     // autotuner: ms per read of each candidate, the choice (-1 while trying)
     double cand_ms[kNCand] = {0, 0, 0, 0};
     int cand_reads[kNCand] = {0, 0, 0, 0};
@@ -687,17 +694,19 @@ struct GraphPlan {
 
 namespace {
 
-GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes)
+GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
 {
     uint32_t cap = 1u << 16;
     while (cap < n) cap <<= 1;
     GraphPlan* pl = nullptr;
     for (auto& p : c->plans)
-        if (p->cap == cap) pl = p.get();
+        if (p->cap == cap && p->lane == lane) pl = p.get();
     if (!pl) {
         c->plans.emplace_back(new GraphPlan());
         pl = c->plans.back().get();
         pl->cap = cap;
+        pl->lane = lane;
+        HIPC(pl->counters.reserve(64));
         const uint64_t n_ro = 2ull * cap;
         HIPC(pl->pre.reserve(cap + 1));
         HIPC(pl->codes.reserve(n_ro * 256 + 256));
@@ -764,7 +773,7 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes)
         HIPC(pl->dp.reserve(2 * cells * c->meth_threads));
         HIPC(pl->dec.reserve(8ull * (FAME_DEV_MAX_READ + 1) * c->meth_threads));
         HIPC(pl->trace.reserve(8ull * FAME_DP_TRACE_WORDS * c->meth_threads));
-        HIPC(hipHostMalloc((void**)&pl->h_io, 64, 0));
+        HIPC(hipHostMalloc((void**)&pl->h_io, 256, 0));
     }
     if (total_bytes + 64 > pl->rev_bytes) {
         // the graphs hold the old pointer
@@ -815,7 +824,9 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
     bd.use = pl->use.as<uint8_t>();
     bd.mat = pl->mat.as<uint64_t>();
     bd.valid = pl->valid.as<uint8_t>();
-    bd.counters = c->d_counters.as<uint64_t>();
+    // single calls count into the context (strand tallies included); the
+    // lanes of a pipelined call count into their own arrays
+    bd.counters = pl->lane ? pl->counters.as<uint64_t>() : c->d_counters.as<uint64_t>();
     bd.meth = c->d_meth.as<uint32_t>();
     bd.direct_list = pl->direct.as<uint32_t>();
     bd.dp_list = pl->dplist.as<uint32_t>();
@@ -825,6 +836,40 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
     bd.max_len = bd.flags + 1;
     bd.stats = pl->stats.as<unsigned long long>();
     return bd;
+}
+
+// the batch pipeline of plan pl over the given inputs, captured into a graph
+hipGraphExec_t capture_graph(fame_ctx* c, GraphPlan* pl, const GraphKey& key, const uint8_t* d_seqs,
+                             const uint64_t* d_offs, const uint32_t* d_lens, uint32_t ori, int get_stranded, int cand,
+                             hipStream_t st)
+{
+    BatchDev bd = plan_batch(c, pl, d_seqs, d_offs, d_lens, ori);
+    const uint32_t cap = pl->cap;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_batch_reset(bd, st);
+    launch_prep(bd, cap, c->readlen, st);
+    launch_seed_light(c->ix, bd, 2 * cap, st);
+    launch_seed_heavy_dev(c->ix, bd, pl->hs, pl->heavy_blocks, st);
+    for (int p = 0; p < FAME_NPOOL; ++p)
+        launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, kVerifyBlocks[cand]);
+    launch_ovf_offsets(bd, FAME_JOBS_ON_DEVICE, pl->ovf_total.as<uint32_t>(), st);
+    for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, bd.ovf_job, st, 64);
+    launch_select(c->ix, bd, cap, get_stranded, st);
+    launch_meth(c->ix, bd, cap, pl->dp.as<uint16_t>(), pl->dec.as<uint64_t>(), pl->trace.as<uint64_t>(),
+                c->meth_threads, st);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        HIPC(e);
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPC(e);
+    pl->graphs.push_back({key, exec});
+    c->graph_captures += 1;
+    return exec;
 }
 
 // one batch through the plan's graph; 1 = the batch outgrew the graph's
@@ -846,35 +891,10 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     hipGraphExec_t exec = nullptr;
     for (auto& g : pl->graphs)
         if (g.first == key) exec = g.second;
-    if (!exec) {
-        BatchDev bd = plan_batch(c, pl, d_seqs, d_offs, d_lens, ori);
-        const uint32_t cap = pl->cap;
-        hipGraph_t graph = nullptr;
-        HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch_reset(bd, st);
-        launch_prep(bd, cap, c->readlen, st);
-        launch_seed_light(c->ix, bd, 2 * cap, st);
-        launch_seed_heavy_dev(c->ix, bd, pl->hs, pl->heavy_blocks, st);
-        for (int p = 0; p < FAME_NPOOL; ++p)
-            launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, kVerifyBlocks[cand]);
-        launch_ovf_offsets(bd, FAME_JOBS_ON_DEVICE, pl->ovf_total.as<uint32_t>(), st);
-        for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, bd.ovf_job, st, 64);
-        launch_select(c->ix, bd, cap, get_stranded, st);
-        launch_meth(c->ix, bd, cap, pl->dp.as<uint16_t>(), pl->dec.as<uint64_t>(), pl->trace.as<uint64_t>(),
-                    c->meth_threads, st);
-        hipError_t e = hipStreamEndCapture(st, &graph);
-        if (e != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            HIPC(e);
-        }
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPC(e);
-        pl->graphs.push_back({key, exec});
-        c->graph_captures += 1;
-    }
+    if (!exec) exec = capture_graph(c, pl, key, d_seqs, d_offs, d_lens, ori, get_stranded, cand, st);
     Timer tm(st);
     pl->h_io[0] = n;
+    HIPC(hipMemsetAsync(c->d_counters.p, 0, 24, st));
     HIPC(hipMemcpyAsync(pl->ndev.p, pl->h_io, 4, hipMemcpyHostToDevice, st));
     tm.mark(0);
     HIPC(hipGraphLaunch(exec, st));
@@ -920,12 +940,108 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     return FAME_OK;
 }
 
+// a large batched call: sub-batches of pipe_batch reads alternate between two
+// lanes (own graph buffers, own stream), so that one lane's memory-bound
+// seeding overlaps the other's compute-bound verification and DP.  Every
+// read is independent and the shared CpG counts are atomic sums, so the
+// split does not change any result.  Sub-batches that outgrew the graphs are
+// re-run on the plain path at the end.
+int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
+                  uint64_t total_bytes, uint64_t counters[3], const uint8_t** out_valid, const uint64_t** out_mat,
+                  const uint8_t** out_status)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t B = c->pipe_batch;
+    const bool o0 = c->both || c->match_r1_fwd;
+    const bool o1 = c->both || !c->match_r1_fwd;
+    const uint32_t ori = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
+    hipStream_t ls[2] = {c->stream, c->stream2};
+    GraphPlan* pl[2] = {plan_for(c, B, total_bytes, 1), plan_for(c, B, total_bytes, 2)};
+    int cand = 2;
+    for (auto& q : c->plans)
+        if (q->chosen >= 0) cand = q->chosen;
+    uint8_t* ov = grow<uint8_t>(c->b_pvalid, n + 1);
+    uint64_t* om = grow<uint64_t>(c->b_pmat, n + 1);
+    uint8_t* os = grow<uint8_t>(c->b_pstatus, n + 1);
+    const uint32_t nb = (n + B - 1) / B;
+    struct Sub {
+        uint32_t s, n, lane, slot;
+    };
+    std::vector<Sub> subs;
+    for (int l = 0; l < 2; ++l) HIPC(hipMemsetAsync(pl[l]->counters.p, 0, 64, ls[l]));
+    // rounds of up to 16 sub-batches per lane (the pinned ring of each lane)
+    for (uint32_t k0 = 0; k0 < nb; k0 += 32) {
+        const uint32_t k1 = std::min(nb, k0 + 32);
+        // graphs first: nothing is captured while a lane has work queued
+        std::vector<hipGraphExec_t> ex(k1 - k0, nullptr);
+        std::vector<Sub> round;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const int l = (int)(k & 1);
+            const uint32_t s0 = k * B;
+            GraphKey key{d_seqs, d_offs + s0, d_lens + s0, 0, ori, cand};
+            for (auto& g : pl[l]->graphs)
+                if (g.first == key) ex[k - k0] = g.second;
+            if (!ex[k - k0]) {
+                HIPC(hipStreamSynchronize(ls[l]));
+                ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, ori, 0, cand, ls[l]);
+            }
+        }
+        for (uint32_t k = k0; k < k1; ++k) {
+            const int l = (int)(k & 1);
+            const uint32_t s0 = k * B, m = std::min(n - s0, B);
+            const uint32_t slot = (k - k0) >> 1;
+            GraphPlan* p = pl[l];
+            p->h_io[16 + slot] = m;
+            HIPC(hipMemcpyAsync(p->ndev.p, p->h_io + 16 + slot, 4, hipMemcpyHostToDevice, ls[l]));
+            HIPC(hipGraphLaunch(ex[k - k0], ls[l]));
+            HIPC(hipMemcpyAsync(p->h_io + 32 + slot, p->flags.p, 4, hipMemcpyDeviceToHost, ls[l]));
+            HIPC(hipMemcpyAsync(ov + s0, p->valid.p, m, hipMemcpyDeviceToDevice, ls[l]));
+            HIPC(hipMemcpyAsync(om + s0, p->mat.p, 8ull * m, hipMemcpyDeviceToDevice, ls[l]));
+            HIPC(hipMemcpyAsync(os + s0, p->status.p, m, hipMemcpyDeviceToDevice, ls[l]));
+            round.push_back({s0, m, (uint32_t)l, slot});
+        }
+        HIPC(hipStreamSynchronize(ls[0]));
+        HIPC(hipStreamSynchronize(ls[1]));
+        // the round's flags, read before the ring is reused; aborted
+        // sub-batches are carried to the end
+        for (const Sub& x : round)
+            if (pl[x.lane]->h_io[32 + x.slot] & FLAG_ABORT) subs.push_back(x);
+        round.clear();
+    }
+    c->graph_calls += nb;
+    c->pipe_calls += 1;
+    uint64_t cnt[2][8];
+    HIPC(hipMemcpy(cnt[0], pl[0]->counters.p, 64, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(cnt[1], pl[1]->counters.p, 64, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) counters[k] += cnt[0][k] + cnt[1][k];
+    // the sub-batches that outgrew the graphs, on the plain path
+    for (Sub& x : subs) {
+        c->graph_fallbacks += 1;
+        int rc = run_batch(c, d_seqs, d_offs + x.s, d_lens + x.s, x.n, total_bytes, 0, counters, c->stream);
+        if (rc != FAME_OK) return rc;
+        HIPC(hipMemcpyAsync(ov + x.s, c->b_valid.p, x.n, hipMemcpyDeviceToDevice, c->stream));
+        HIPC(hipMemcpyAsync(om + x.s, c->b_mat.p, 8ull * x.n, hipMemcpyDeviceToDevice, c->stream));
+        HIPC(hipMemcpyAsync(os + x.s, c->b_status.p, x.n, hipMemcpyDeviceToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    *out_valid = ov;
+    *out_mat = om;
+    if (out_status) *out_status = os;
+    memset(c->stats, 0, sizeof c->stats);
+    for (int k = 0; k < 5; ++k) c->timings[k] = 0;
+    c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->timings[1] = c->timings[5];
+    return FAME_OK;
+}
+
 // one single-end batch in the context's form; the per-read outputs stay in
 // device buffers whose addresses are returned
 int match_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
                 uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st,
                 const uint8_t** out_valid, const uint64_t** out_mat, const uint8_t** out_status)
 {
+    if (c->form == 1 && !get_stranded && n > c->pipe_batch)
+        return run_pipelined(c, d_seqs, d_offs, d_lens, n, total_bytes, counters, out_valid, out_mat, out_status);
     if (c->form == 1 && n) {
         const int rc = run_batch_graph(c, d_seqs, d_offs, d_lens, n, total_bytes, get_stranded, counters, st,
                                        out_valid, out_mat, out_status);
@@ -1310,6 +1426,7 @@ int fame_open(const char* index_path, int both_strands, fame_ctx** out)
     int rc = guarded(c.get(), [&]() {
         HIPC(hipGetDevice(&c->device));
         HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         c->meth_threads = meth_dp_lanes();
         load_index(c.get(), index_path);
         return FAME_OK;
@@ -1326,7 +1443,9 @@ int fame_open(const char* index_path, int both_strands, fame_ctx** out)
 int fame_close(fame_ctx* c)
 {
     if (!c) return FAME_OK;
+    c->plans.clear();
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     return FAME_OK;
 }
@@ -1528,6 +1647,13 @@ int fame_set_form(fame_ctx* c, int form)
     return FAME_OK;
 }
 
+int fame_set_pipeline(fame_ctx* c, uint32_t sub_batch_reads)
+{
+    if (!c || sub_batch_reads < 256) return FAME_ERR_ARG;
+    c->pipe_batch = sub_batch_reads;
+    return FAME_OK;
+}
+
 int fame_form_info(const fame_ctx* c, uint64_t out[FAME_FORM_FIELDS])
 {
     if (!c || !out) return FAME_ERR_ARG;
@@ -1537,6 +1663,8 @@ int fame_form_info(const fame_ctx* c, uint64_t out[FAME_FORM_FIELDS])
     out[3] = c->graph_captures;
     out[4] = c->plans.size();
     out[5] = 0;
+    out[6] = c->pipe_calls;
+    out[7] = c->pipe_batch;
     uint32_t big = 0;
     for (auto& p : c->plans)
         if (p->cap >= big && p->chosen >= 0) {

@@ -138,9 +138,17 @@ int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
 #define FAME_FORM_BATCHED 1
 int fame_set_form(fame_ctx* ctx, int form);
 
-/* form, batched calls, of which re-run plain, graphs captured, capacity
- * classes, the autotuned verify grid (blocks) of the largest tuned class */
-#define FAME_FORM_FIELDS 6
+/* In the batched form a call of more than sub_batch_reads reads (default
+ * 2^21) that is not the stranded first chunk is split into sub-batches that
+ * alternate between two lanes (graph buffers and HIP streams of their own),
+ * so that one lane's seeding overlaps the other's verification and DP.
+ * FAME_ERR_ARG below 256. */
+int fame_set_pipeline(fame_ctx* ctx, uint32_t sub_batch_reads);
+
+/* form, batched graph launches, of which re-run plain, graphs captured,
+ * capacity classes, the autotuned verify grid (blocks) of the largest tuned
+ * class, pipelined calls, sub-batch size */
+#define FAME_FORM_FIELDS 8
 int fame_form_info(const fame_ctx* ctx, uint64_t out[FAME_FORM_FIELDS]);
 
 /* ---- serving loop (no counterpart in the reference) ----------------------

@@ -37,7 +37,7 @@ def test_batched_matches_reference(golden_case, tmp_path):
     ctx.set_form("batched")
     valid, mat, cnt = stream(ctx, c, max(len(c.lens) // 3, 1))
     info = ctx.form_info()
-    assert info["batched_calls"] >= 1
+    assert info["graph_launches"] >= 1
     assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
     assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
     assert np.array_equal(ctx.meth(), c.meth)
@@ -72,5 +72,21 @@ def test_graph_replays_equal_plain(golden_case):
         ctx.set_form("plain")
         ctx.match(*b, False, np.zeros(3, np.uint64))
     info = ctx.form_info()
-    assert info["batched_calls"] == 6 and info["fallbacks"] == 0
+    assert info["graph_launches"] == 6 and info["fallbacks"] == 0
+    ctx.close()
+
+
+def test_pipelined_lanes_match_reference(golden_case, tmp_path):
+    """Sub-batches of 700 reads alternating between the two lanes (own
+    buffers and streams, work in flight together): still bit-exact."""
+    c = golden_case
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    ctx.set_form("batched")
+    ctx.set_pipeline(700)
+    valid, mat, cnt = stream(ctx, c, max(len(c.lens) // 2, 1))
+    info = ctx.form_info()
+    assert info["pipelined_calls"] >= 1
+    assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
