@@ -531,6 +531,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         count = min(*bd.heavy_n, bd.heavy_cap);
     }
     if (blockIdx.x >= count) return;
+    // (a tier's blocks skip the items of the other tier)
     uint8_t* base = hs.base + (size_t)blockIdx.x * hs.stride;
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -562,7 +563,12 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.code = s_code;
     seed_table_init(sh);
     SeedStats ss = {0, 0, 0, 0};
+    const bool tiered = hs.count == FAME_JOBS_ON_DEVICE;
     for (uint32_t hi = blockIdx.x; hi < count; hi += gridDim.x) {
+        if (tiered) {
+            const uint32_t e = bd.heavy_e[hi];
+            if (e < hs.e_lo || e > hs.e_hi) continue;
+        }
         const uint32_t ro = bd.heavy_ro[hs.first + hi];
         uint64_t E;
         uint32_t bc;
@@ -1756,18 +1762,41 @@ void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, ui
     hipLaunchKernelGGL(k_pack_tmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, kent, n, out);
 }
 
-void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st)
+void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint)
 {
     if (!n_ro) return;
     const uint32_t waves = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
     const uint32_t g = (waves + FAME_SEED_WPB - 1) / FAME_SEED_WPB;
-    if (ix.tab_is64) {
-        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint64_t, 256>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-    } else {
-        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint32_t, 256>), dim3(g), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    // the variant that does not fit the batch returns at once; with a hint of
+    // the longest read it gets a small grid (still complete if the hint is
+    // wrong: the kernels are persistent)
+    uint32_t gs = g, gl = g;
+    if (lmax_hint) {
+        if (lmax_hint <= (uint32_t)SEED_SHORT) gl = 16;
+        else gs = 16;
     }
+    if (ix.tab_is64) {
+        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, 256>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    } else {
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+    }
+}
+
+__global__ void k_len_range(const uint32_t* __restrict__ lens, uint32_t n, uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        atomicMin(&out[0], lens[i]);
+        atomicMax(&out[1], lens[i]);
+    }
+}
+
+void launch_len_range(const uint32_t* lens, uint32_t n, uint32_t* out, hipStream_t st)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_len_range, dim3((n + 255) / 256), dim3(256), 0, st, lens, n, out);
 }
 
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st)

@@ -667,13 +667,15 @@ struct GraphPlan {
     DevBuf rev, pre, codes, cand_off, cand_nf, cand_nr, pool_n;
     DevBuf job_meta[FAME_NPOOL], job_ro[FAME_NPOOL], job_nm[FAME_NPOOL], job_match[FAME_NPOOL],
         job_ovf_off[FAME_NPOOL];
-    DevBuf ovf_n, ovf_pool, ovf_job, ovf_buf, ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy_scratch;
+    DevBuf ovf_n, ovf_pool, ovf_job, ovf_buf, ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy_scratch,
+        heavy_scratch2;
     DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace, counters;
-    uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 256;
-    HeavyScratch hs{};
+    uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 1024, heavy_blocks2 = 16;
+    HeavyScratch hs{}, hs2{};
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
-    // pinned: [0] n, [1] flags, [2..7] counters (3 x u64); pipelined calls:
-    // [16 + k] n and [32 + k] flags of the lane's k-th sub-batch
+    // pinned: [0] n, [1] flags, [2..7] counters (3 x u64), [8..11] pool
+    // counts; pipelined calls: [16 + k] n, [32 + k] flags and [64 + 4k ..]
+    // pool counts of the lane's k-th sub-batch
     uint32_t* h_io = nullptr;
     // autotuner: ms per read of each candidate, the choice (-1 while trying)
     double cand_ms[kNCand] = {0, 0, 0, 0};
@@ -693,6 +695,31 @@ struct GraphPlan {
 };
 
 namespace {
+
+// verify pools of a plan (the same capacity for each read-length class)
+void plan_pools(fame_ctx* c, GraphPlan* pl, uint32_t cap)
+{
+    pl->pool_cap = cap;
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        HIPC(pl->job_meta[p].reserve(4ull * cap));
+        HIPC(pl->job_ro[p].reserve(4ull * cap));
+        HIPC(pl->job_nm[p].reserve(4ull * cap));
+        HIPC(pl->job_match[p].reserve(4ull * cap * c->match_cap));
+        HIPC(pl->job_ovf_off[p].reserve(4ull * cap));
+    }
+}
+
+// after a batch overflowed a pool: pools for the demand it showed (the
+// device counters keep counting past the capacity), graphs re-captured
+void plan_grow_pools(fame_ctx* c, GraphPlan* pl, const uint32_t* demand)
+{
+    uint64_t need = 0;
+    for (int p = 0; p < FAME_NPOOL; ++p) need = std::max<uint64_t>(need, demand[p]);
+    need = 2 * need + 65536;
+    if (need <= pl->pool_cap || need > 0xFFFFFFF0ull) return;
+    pl->drop_graphs();
+    plan_pools(c, pl, (uint32_t)need);
+}
 
 GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
 {
@@ -714,14 +741,7 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
         HIPC(pl->cand_nf.reserve(4 * n_ro + 4));
         HIPC(pl->cand_nr.reserve(4 * n_ro + 4));
         HIPC(pl->pool_n.reserve(4 * FAME_NPOOL));
-        pl->pool_cap = std::max<uint32_t>(1u << 16, 4 * cap) + 64u * FAME_SEED_WAVES;
-        for (int p = 0; p < FAME_NPOOL; ++p) {
-            HIPC(pl->job_meta[p].reserve(4ull * pl->pool_cap));
-            HIPC(pl->job_ro[p].reserve(4ull * pl->pool_cap));
-            HIPC(pl->job_nm[p].reserve(4ull * pl->pool_cap));
-            HIPC(pl->job_match[p].reserve(4ull * pl->pool_cap * c->match_cap));
-            HIPC(pl->job_ovf_off[p].reserve(4ull * pl->pool_cap));
-        }
+        plan_pools(c, pl, std::max<uint32_t>(1u << 16, 4 * cap) + 64u * FAME_SEED_WAVES);
         pl->ovf_list_cap = std::max<uint32_t>(1024, cap / 4);
         pl->ovf_buf_cap = 1u << 24;
         HIPC(pl->ovf_n.reserve(8));
@@ -734,31 +754,40 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
         HIPC(pl->heavy_ro.reserve(4 * n_ro));
         HIPC(pl->heavy_e.reserve(4 * n_ro));
         HIPC(pl->heavy_bc.reserve(4 * n_ro));
-        // heavy (read, orientation) pairs of up to 16384 index hits in
-        // graph; larger ones make the batch fall back to the plain path
-        const uint64_t emax = 16384;
-        HeavyScratch& hs = pl->hs;
-        hs.cap_e = (uint32_t)emax;
-        hs.hsize = 2 * emax;
-        hs.tb_cap = (uint32_t)(2 * (2 * emax) + 64);
-        size_t stride = 0;
-        auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
-        add(8 * FAME_DEV_MAX_READ);
-        add(4 * FAME_DEV_MAX_READ);
-        add(4 * FAME_DEV_MAX_READ);
-        add(4 * emax);
-        add(emax);
-        add(4 * emax);
-        for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
-        add(8 * emax);
-        add(4 * emax);
-        add(4ull * hs.tb_cap);
-        add(4 * emax);
-        add(4 * emax);
-        add(8 * emax);
-        hs.stride = stride;
-        HIPC(pl->heavy_scratch.reserve(stride * pl->heavy_blocks));
-        hs.base = pl->heavy_scratch.as<uint8_t>();
+        // heavy (read, orientation) pairs in graph, two tiers: up to 16384
+        // index hits on 1024 scratch slices, up to 2^20 on 16; larger ones make
+        // the batch fall back to the plain path
+        auto dims = [](HeavyScratch& hs, uint64_t emax) {
+            hs.cap_e = (uint32_t)emax;
+            hs.hsize = (uint32_t)(2 * emax);
+            hs.tb_cap = (uint32_t)(2 * (2 * emax) + 64);
+            size_t stride = 0;
+            auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+            add(8 * FAME_DEV_MAX_READ);
+            add(4 * FAME_DEV_MAX_READ);
+            add(4 * FAME_DEV_MAX_READ);
+            add(4 * emax);
+            add(emax);
+            add(4 * emax);
+            for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
+            add(8 * emax);
+            add(4 * emax);
+            add(4ull * hs.tb_cap);
+            add(4 * emax);
+            add(4 * emax);
+            add(8 * emax);
+            hs.stride = stride;
+        };
+        dims(pl->hs, 16384);
+        pl->hs.e_lo = 0;
+        pl->hs.e_hi = 16384;
+        dims(pl->hs2, 1u << 20);
+        pl->hs2.e_lo = 16385;
+        pl->hs2.e_hi = 0xFFFFFFFFu;
+        HIPC(pl->heavy_scratch.reserve(pl->hs.stride * pl->heavy_blocks));
+        HIPC(pl->heavy_scratch2.reserve(pl->hs2.stride * pl->heavy_blocks2));
+        pl->hs.base = pl->heavy_scratch.as<uint8_t>();
+        pl->hs2.base = pl->heavy_scratch2.as<uint8_t>();
         HIPC(pl->status.reserve(cap + 1));
         HIPC(pl->use.reserve(cap + 1));
         HIPC(pl->mat.reserve(8ull * cap + 8));
@@ -773,7 +802,7 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
         HIPC(pl->dp.reserve(2 * cells * c->meth_threads));
         HIPC(pl->dec.reserve(8ull * (FAME_DEV_MAX_READ + 1) * c->meth_threads));
         HIPC(pl->trace.reserve(8ull * FAME_DP_TRACE_WORDS * c->meth_threads));
-        HIPC(hipHostMalloc((void**)&pl->h_io, 256, 0));
+        HIPC(hipHostMalloc((void**)&pl->h_io, 1024, 0));
     }
     if (total_bytes + 64 > pl->rev_bytes) {
         // the graphs hold the old pointer
@@ -840,20 +869,35 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
 
 // the batch pipeline of plan pl over the given inputs, captured into a graph
 hipGraphExec_t capture_graph(fame_ctx* c, GraphPlan* pl, const GraphKey& key, const uint8_t* d_seqs,
-                             const uint64_t* d_offs, const uint32_t* d_lens, uint32_t ori, int get_stranded, int cand,
-                             hipStream_t st)
+                             const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n, uint32_t ori,
+                             int get_stranded, int cand, hipStream_t st)
 {
     BatchDev bd = plan_batch(c, pl, d_seqs, d_offs, d_lens, ori);
     const uint32_t cap = pl->cap;
+    // grids for the read lengths of the batch being captured: verify pools
+    // and the seed variant its lengths need get full persistent grids, the
+    // others small ones (complete whatever data later replays bring)
+    uint32_t lr[2] = {0xFFFFFFFFu, 0};
+    {
+        uint32_t* d_lr = reinterpret_cast<uint32_t*>(pl->ovf_total.as<uint8_t>());
+        HIPC(hipMemcpyAsync(d_lr, lr, 8, hipMemcpyHostToDevice, st));
+        launch_len_range(d_lens, n, d_lr, st);
+        HIPC(hipMemcpyAsync(lr, d_lr, 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+    }
+    const uint32_t wmin = lr[0] ? (lr[0] + 63) / 64 : 1, wmax = lr[1] ? (lr[1] + 63) / 64 : 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     launch_batch_reset(bd, st);
     launch_prep(bd, cap, c->readlen, st);
-    launch_seed_light(c->ix, bd, 2 * cap, st);
+    launch_seed_light(c->ix, bd, 2 * cap, st, std::max<uint32_t>(lr[1], 1));
     launch_seed_heavy_dev(c->ix, bd, pl->hs, pl->heavy_blocks, st);
-    for (int p = 0; p < FAME_NPOOL; ++p)
-        launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, kVerifyBlocks[cand]);
+    launch_seed_heavy_dev(c->ix, bd, pl->hs2, pl->heavy_blocks2, st);
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        const bool used = (uint32_t)p + 1 >= wmin && (uint32_t)p + 1 <= wmax;
+        launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, used ? kVerifyBlocks[cand] : 16);
+    }
     launch_ovf_offsets(bd, FAME_JOBS_ON_DEVICE, pl->ovf_total.as<uint32_t>(), st);
     for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, bd.ovf_job, st, 64);
     launch_select(c->ix, bd, cap, get_stranded, st);
@@ -891,7 +935,7 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     hipGraphExec_t exec = nullptr;
     for (auto& g : pl->graphs)
         if (g.first == key) exec = g.second;
-    if (!exec) exec = capture_graph(c, pl, key, d_seqs, d_offs, d_lens, ori, get_stranded, cand, st);
+    if (!exec) exec = capture_graph(c, pl, key, d_seqs, d_offs, d_lens, n, ori, get_stranded, cand, st);
     Timer tm(st);
     pl->h_io[0] = n;
     HIPC(hipMemsetAsync(c->d_counters.p, 0, 24, st));
@@ -901,6 +945,7 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     tm.mark(1);
     HIPC(hipMemcpyAsync(pl->h_io + 1, pl->flags.p, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(pl->h_io + 2, c->d_counters.p, 24, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(pl->h_io + 8, pl->pool_n.p, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(c->stats, pl->stats.p, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     c->graph_calls += 1;
@@ -911,6 +956,7 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     HIPC(hipStreamSynchronize(st));
     if (flags & FLAG_ABORT) {
         c->graph_fallbacks += 1;
+        if (flags & FLAG_POOL_FULL) plan_grow_pools(c, pl, pl->h_io + 8);
         return 1;
     }
     const double ms = tm.ms(0, 1);
@@ -983,7 +1029,8 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
                 if (g.first == key) ex[k - k0] = g.second;
             if (!ex[k - k0]) {
                 HIPC(hipStreamSynchronize(ls[l]));
-                ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, ori, 0, cand, ls[l]);
+                ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, std::min(n - s0, B), ori, 0,
+                                           cand, ls[l]);
             }
         }
         for (uint32_t k = k0; k < k1; ++k) {
@@ -995,6 +1042,7 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
             HIPC(hipMemcpyAsync(p->ndev.p, p->h_io + 16 + slot, 4, hipMemcpyHostToDevice, ls[l]));
             HIPC(hipGraphLaunch(ex[k - k0], ls[l]));
             HIPC(hipMemcpyAsync(p->h_io + 32 + slot, p->flags.p, 4, hipMemcpyDeviceToHost, ls[l]));
+            HIPC(hipMemcpyAsync(p->h_io + 64 + 4 * slot, p->pool_n.p, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, ls[l]));
             HIPC(hipMemcpyAsync(ov + s0, p->valid.p, m, hipMemcpyDeviceToDevice, ls[l]));
             HIPC(hipMemcpyAsync(om + s0, p->mat.p, 8ull * m, hipMemcpyDeviceToDevice, ls[l]));
             HIPC(hipMemcpyAsync(os + s0, p->status.p, m, hipMemcpyDeviceToDevice, ls[l]));
@@ -1004,8 +1052,12 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
         HIPC(hipStreamSynchronize(ls[1]));
         // the round's flags, read before the ring is reused; aborted
         // sub-batches are carried to the end
-        for (const Sub& x : round)
-            if (pl[x.lane]->h_io[32 + x.slot] & FLAG_ABORT) subs.push_back(x);
+        for (const Sub& x : round) {
+            const uint32_t f = pl[x.lane]->h_io[32 + x.slot];
+            if (!(f & FLAG_ABORT)) continue;
+            subs.push_back(x);
+            if (f & FLAG_POOL_FULL) plan_grow_pools(c, pl[x.lane], pl[x.lane]->h_io + 64 + 4 * x.slot);
+        }
         round.clear();
     }
     c->graph_calls += nb;

@@ -166,6 +166,7 @@ struct HeavyScratch {
     size_t stride;
     uint32_t first, count;
     uint32_t cap_e, hsize, tb_cap;
+    uint32_t e_lo, e_hi; // device-list mode: only the items with e_lo <= E <= e_hi
 };
 
 void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st);
@@ -175,7 +176,9 @@ void launch_batch_reset(const BatchDev& bd, hipStream_t st);
 // `blocks` scratch slices of hs
 void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
                            hipStream_t st);
-void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st);
+void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint = 0);
+// shortest and longest of n lengths (out[0] must start at ~0, out[1] at 0)
+void launch_len_range(const uint32_t* lens, uint32_t n, uint32_t* out, hipStream_t st);
 void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st);
 // njobs = FAME_JOBS_ON_DEVICE: as many as the device counter holds, on
 // `blocks` persistent blocks (batched form)
