@@ -22,7 +22,8 @@ MAX_READ = 255
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
            "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
            "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_set_form", "fame_set_pipeline", "fame_form_info", "fame_serve_open", "fame_serve_submit",
-           "fame_serve_wait", "fame_serve_stats", "fame_serve_close",
+           "fame_serve_wait", "fame_serve_stats", "fame_serve_close", "fame_fastq_open", "fame_fastq_next",
+           "fame_fastq_close",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
@@ -69,6 +70,10 @@ def load():
         L.fame_serve_wait.argtypes = [C.c_void_p, C.c_uint64, _u64p, _dbl]
         L.fame_serve_stats.argtypes = [C.c_void_p, _u64p, _dbl]
         L.fame_serve_close.argtypes = [C.c_void_p]
+        L.fame_fastq_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.fame_fastq_next.argtypes = [C.c_void_p, C.c_uint32, _u32p, C.POINTER(_u8p), C.POINTER(_u64p),
+                                      C.POINTER(_u32p)]
+        L.fame_fastq_close.argtypes = [C.c_void_p]
         L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
         L.fame_reset.argtypes = [C.c_void_p]
         L.fame_print_methylation_levels.argtypes = [C.c_void_p, C.c_char_p]
@@ -259,6 +264,44 @@ class Server:
     def close(self):
         if getattr(self, "h", None):
             load().fame_serve_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FastqReader:
+    """Native FASTQ parser (fame_fastq_*): next(k) -> (seqs, offs, lens) of up
+    to k records as numpy copies, empty arrays at the end of the file."""
+
+    def __init__(self, path, gz=False):
+        h = C.c_void_p()
+        rc = load().fame_fastq_open(path.encode(), int(gz), C.byref(h))
+        if rc != FAME_OK:
+            raise FameError(f"{ERRORS.get(rc, rc)}: cannot open {path}")
+        self.h = h.value
+
+    def next(self, max_reads):
+        n = C.c_uint32()
+        s, o, l = _u8p(), _u64p(), _u32p()
+        rc = load().fame_fastq_next(self.h, int(max_reads), C.byref(n), C.byref(s), C.byref(o), C.byref(l))
+        if rc != FAME_OK:
+            raise FameError(f"{ERRORS.get(rc, rc)}: FASTQ parse failed")
+        k = n.value
+        if k == 0:
+            return np.zeros(0, np.uint8), np.zeros(0, np.uint64), np.zeros(0, np.uint32)
+        offs = np.ctypeslib.as_array(o, shape=(k,)).copy()
+        lens = np.ctypeslib.as_array(l, shape=(k,)).copy()
+        total = int(offs[-1] + lens[-1])
+        seqs = np.ctypeslib.as_array(s, shape=(max(total, 1),))[:total].copy()
+        return seqs, offs, lens
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fame_fastq_close(self.h)
             self.h = None
 
     def __del__(self):

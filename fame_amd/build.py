@@ -11,7 +11,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfamehip.so")
 INDEX_LIB = os.path.join(HERE, "libfameindex.so")
 SOURCES = [os.path.join(HERE, "csrc", "fame_device.hip"), os.path.join(HERE, "csrc", "fame_paired.hip"),
-           os.path.join(HERE, "csrc", "fame_host.cpp"), os.path.join(HERE, "csrc", "fame_serve.cpp")]
+           os.path.join(HERE, "csrc", "fame_host.cpp"), os.path.join(HERE, "csrc", "fame_serve.cpp"),
+           os.path.join(HERE, "csrc", "fame_fastq.cpp")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
          # packed FP32 instructions are kept out of all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -20,7 +21,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-pth
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + FLAGS + ["-o", tmp] + SOURCES
+    cmd = [hipcc] + FLAGS + ["-o", tmp] + SOURCES + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

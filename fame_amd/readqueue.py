@@ -6,8 +6,6 @@ counters are accumulated into the caller's list, the first chunk of a
 stranded run decides the orientation of read 1, and the report is written to
 ``<filename>_cpg.tsv``.
 """
-import gzip
-
 import numpy as np
 
 from . import _lib
@@ -23,23 +21,6 @@ class RefGenome:
         self.path = file_path
 
 
-def _parse(f, limit):
-    seqs = []
-    while len(seqs) < limit:
-        head = f.readline()
-        if not head:
-            break
-        seq = f.readline().rstrip(b"\n")
-        f.readline()
-        f.readline()
-        seqs.append(seq)
-    lens = np.array([len(s) for s in seqs], np.uint32)
-    offs = np.zeros(len(seqs), np.uint64)
-    if len(seqs) > 1:
-        offs[1:] = np.cumsum(lens.astype(np.uint64))[:-1]
-    return (np.frombuffer(b"".join(seqs), np.uint8), offs, lens)
-
-
 class ReadQueue:
     """ReadQueue: single-end (ReadQueue.cpp:25-78) or, with file_path2, the
     paired-end constructor (:79-180)."""
@@ -49,9 +30,11 @@ class ReadQueue:
         self.ctx = _lib.Context(ref.path, both_strands=bs_flag)
         self.bs_flag = bs_flag
         self.chunk_size = chunk_size
-        opener = gzip.open if is_gz else open
-        self._f = opener(file_path, "rb") if file_path else None
-        self._f2 = opener(file_path2, "rb") if file_path2 else None
+        # the native parser (fame_fastq_*), which reads as parseChunk /
+        # parseChunkGZ do: a record every four lines, its second line the
+        # sequence
+        self._f = _lib.FastqReader(file_path, is_gz) if file_path else None
+        self._f2 = _lib.FastqReader(file_path2, is_gz) if file_path2 else None
         self.is_paired = file_path2 is not None
         if self.is_paired:
             self.ctx.set_pair_dist(minpd, maxpd)
@@ -61,10 +44,10 @@ class ReadQueue:
 
     # parseChunk ReadQueue.cpp:218-296
     def parseChunk(self):
-        self.buf = _parse(self._f, self.chunk_size)
+        self.buf = self._f.next(self.chunk_size)
         n = len(self.buf[2])
         if self.is_paired:
-            self.buf2 = _parse(self._f2, self.chunk_size)
+            self.buf2 = self._f2.next(self.chunk_size)
             if len(self.buf2[2]) != n:
                 # the reference prints a message and exits (:280-286)
                 raise _lib.FameError("Not the same number of reads available in the paired read files")

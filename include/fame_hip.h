@@ -181,6 +181,20 @@ int fame_serve_stats(fame_server* srv, uint64_t out[4], double* busy_ms);
 
 int fame_serve_close(fame_server* srv);
 
+/* ---- FASTQ ingestion ------------------------------------------------------
+ * ReadQueue::parseChunk / parseChunkGZ (ReadQueue.cpp:218-373): records are
+ * every fourth line (the reference's std::getline loop: '\n' ends a line, a
+ * last unterminated line counts, a record's sequence is its second line or
+ * empty at end of input).  fame_fastq_next parses up to max_reads records
+ * with the host's threads into pinned buffers (reads back to back, u64
+ * offsets, u32 lengths), valid until the next call on the handle; *n = 0 at
+ * the end of the file.  gz selects gzip input (parseChunkGZ). */
+typedef struct fame_fastq fame_fastq;
+int fame_fastq_open(const char* path, int gz, fame_fastq** out);
+int fame_fastq_next(fame_fastq* fq, uint32_t max_reads, uint32_t* n, const uint8_t** seqs,
+                    const uint64_t** offs, const uint32_t** lens);
+int fame_fastq_close(fame_fastq* fq);
+
 /* per-stage device time of the last fame_match_reads* call, milliseconds:
  * prep, seed, verify, select, meth, and the total including host waits */
 int fame_last_timings(const fame_ctx* ctx, double out[6]);

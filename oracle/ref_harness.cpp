@@ -18,6 +18,11 @@
 //       mirrors queryRoutinePaired main.cpp:400-441; writes the same TSV and
 //       stats, and <out_prefix>_pairs.bin: "FAMEPE01", u64 n, u8 valid1[n],
 //       u8 valid2[n], u64 mat1[n], u64 mat2[n] (mat as left in Read::mat)
+//   parse <index> <reads.fq[.gz]> <out> [--gz]
+//       ReadQueue::parseChunk / parseChunkGZ (ReadQueue.cpp:218-373) until it
+//       reports the end; writes <out>: "FAMEFQ01", u64 n, u32 len[n], then the
+//       n sequences back to back, and per parse call (u32 procReads, u8 more)
+//       after them, preceded by a u64 count of calls
 //
 // ReadQueue keeps readBuffer / strand tallies private; they are reached by
 // compiling this translation unit only with `private` defined as `public`.
@@ -237,6 +242,43 @@ static int do_match_pe(int argc, char** argv)
     return 0;
 }
 
+static int do_parse(int argc, char** argv)
+{
+    if (argc < 5) return 2;
+    bool gz = false;
+    for (int i = 5; i < argc; ++i)
+        if (!strcmp(argv[i], "--gz")) gz = true;
+    RefGenome ref{std::string(argv[2])};
+    ReadQueue q(argv[3], ref, gz, false);
+    std::vector<uint32_t> lens;
+    std::string seqs;
+    std::vector<std::pair<uint32_t, uint8_t>> calls;
+    for (;;) {
+        unsigned n = 0;
+        const bool more = gz ? q.parseChunkGZ(n) : q.parseChunk(n);
+        calls.push_back({n, more ? 1 : 0});
+        for (unsigned i = 0; i < n; ++i) {
+            lens.push_back(static_cast<uint32_t>(q.readBuffer[i].seq.size()));
+            seqs += q.readBuffer[i].seq;
+        }
+        if (!more) break;
+    }
+    std::ofstream of(argv[4], std::ios::binary | std::ios::trunc);
+    const char magic[8] = {'F', 'A', 'M', 'E', 'F', 'Q', '0', '1'};
+    of.write(magic, 8);
+    uint64_t n = lens.size(), nc = calls.size();
+    of.write(reinterpret_cast<const char*>(&n), 8);
+    of.write(reinterpret_cast<const char*>(lens.data()), 4 * n);
+    of.write(seqs.data(), static_cast<std::streamsize>(seqs.size()));
+    of.write(reinterpret_cast<const char*>(&nc), 8);
+    for (auto& c : calls) {
+        of.write(reinterpret_cast<const char*>(&c.first), 4);
+        of.write(reinterpret_cast<const char*>(&c.second), 1);
+    }
+    std::cout << "HARNESS parsed=" << n << " calls=" << nc << "\n";
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -248,6 +290,7 @@ int main(int argc, char** argv)
     if (mode == "build") return do_build(argc, argv);
     if (mode == "match") return do_match(argc, argv);
     if (mode == "match_pe") return do_match_pe(argc, argv);
+    if (mode == "parse") return do_parse(argc, argv);
     std::cerr << "unknown mode\n";
     return 2;
 }

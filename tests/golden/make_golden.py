@@ -15,6 +15,12 @@ Paired-end cases (pe_*) hold reads.npz / golden.npz with per-mate arrays
 (seqs1/offs1/lens1, seqs2/..., valid1/valid2/mat1/mat2) and the five paired
 counters in case.json.
 
+The fastq/ directory holds FASTQ inputs with unusual layouts (CRLF, no
+final newline, truncated records, blank lines, gzip, chunk-size multiples)
+and, per input, what FAME's parseChunk / parseChunkGZ (CHUNKSIZE 700) read
+from it: <name>.expect.npz with lens, seqs and the (procReads, more) of each
+call.
+
 Inputs are synthetic (fame_amd.synth) with fixed seeds, plus hand-placed
 special reads (N, short, unknown characters, random, chromosome edges,
 longer than READLEN).  The reference is run here, in the container that holds
@@ -370,7 +376,76 @@ def run_pair_case(name):
     print(name, meta["reference"])
 
 
+def _fastq_inputs():
+    """name -> (bytes, gz): FASTQ layouts the parser must read as FAME does."""
+    r = np.load(os.path.join(HERE, "se_basic", "reads.npz"))
+    flat, offs, lens = r["seqs"].tobytes(), r["offs"], r["lens"]
+    seqs = [flat[int(o):int(o) + int(n)] for o, n in zip(offs, lens)]
+
+    def fq(ss, nl=b"\n", tail=True):
+        recs = [b"@r%d%s%s%s+%s%s" % (i, nl, x, nl, nl, b"I" * len(x)) for i, x in enumerate(ss)]
+        return nl.join(recs) + (nl if tail else b"")
+
+    out = {
+        "lf": (fq(seqs[:2000]), False),
+        "crlf": (fq(seqs[:300], b"\r\n"), False),
+        "notrail": (fq(seqs[:50], tail=False), False),
+        "exact1400": (fq(seqs[:1400]), False),
+        "trunc_seq": (fq(seqs[:20]) + b"@cut\n", False),
+        "trunc_nonl": (fq(seqs[:20]) + b"@cut", False),
+        "trunc_plus": (fq(seqs[:20]) + b"@cut\nACGT\n+\n", False),
+        "blank": (fq(seqs[:10]) + b"\n\n" + fq(seqs[10:30]) + b"\n", False),
+        "empty": (b"", False),
+        "gz_lf": (fq(seqs[:2000]), True),
+        "gz_notrail": (fq(seqs[:707], tail=False), True),
+    }
+    return out
+
+
+def run_fastq_cases():
+    cfg = CASES["se_ga_chunked"]
+    binp = ref_binary(cfg)
+    out = os.path.join(HERE, "fastq")
+    os.makedirs(out, exist_ok=True)
+    with tempfile.TemporaryDirectory() as td:
+        for f in ("index.bin", "index.bin_strands"):
+            with gzip.open(os.path.join(HERE, "se_ga_chunked", f + ".gz"), "rb") as a, \
+                    open(os.path.join(td, f), "wb") as b:
+                shutil.copyfileobj(a, b)
+        for name, (data, is_gz) in _fastq_inputs().items():
+            src = os.path.join(td, name + ".fq")
+            with open(src, "wb") as f:
+                f.write(data)
+            # the input travels gzipped either way; gz cases are read as such
+            stored = os.path.join(out, name + ".fq.gz")
+            with gzip.open(stored, "wb", compresslevel=9) as f:
+                f.write(data)
+            inp = stored if is_gz else src
+            subprocess.run([binp, "parse", "index.bin", inp, "out.bin"] + (["--gz"] if is_gz else []), cwd=td,
+                           check=True, stdout=subprocess.DEVNULL)
+            raw = open(os.path.join(td, "out.bin"), "rb").read()
+            assert raw[:8] == b"FAMEFQ01"
+            n = int(np.frombuffer(raw[8:16], np.uint64)[0])
+            lens = np.frombuffer(raw[16:16 + 4 * n], np.uint32).copy()
+            p = 16 + 4 * n
+            tot = int(lens.sum(dtype=np.uint64))
+            seqs = np.frombuffer(raw[p:p + tot], np.uint8).copy()
+            p += tot
+            nc = int(np.frombuffer(raw[p:p + 8], np.uint64)[0])
+            p += 8
+            calls = np.array([(int(np.frombuffer(raw[p + 5 * i:p + 5 * i + 4], np.uint32)[0]), raw[p + 5 * i + 4])
+                              for i in range(nc)], np.uint32).reshape(-1, 2)
+            np.savez_compressed(os.path.join(out, name + ".expect.npz"), lens=lens, seqs=seqs, calls=calls)
+            print("fastq", name, n, "reads", calls.tolist())
+        json.dump({"mode": "fastq", "chunksize": cfg["chunk"],
+                   "gz": [k for k, v in _fastq_inputs().items() if v[1]]},
+                  open(os.path.join(out, "case.json"), "w"), indent=1)
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + list(PAIRED)
+    names = sys.argv[1:] or list(CASES) + list(PAIRED) + ["fastq"]
     for nm in names:
-        run_pair_case(nm) if nm in PAIRED else run_case(nm)
+        if nm == "fastq":
+            run_fastq_cases()
+        else:
+            run_pair_case(nm) if nm in PAIRED else run_case(nm)
