@@ -7,16 +7,20 @@
  * positions that wrap near chromosome starts, size_t / long in the DP
  * indexing), because the reference's results depend on them.
  *
- * Pinning (see tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle_golden.py, test_oracle_units.py,
+ * test_oracle_sort.py):
  *   - whole path: per-read (valid, MATCH) and CpG TSV against the reference
  *     binary built from its own sources (oracle/_ref) on committed fixtures;
- *   - orc_ntps64: closed form == rolling form, and against the reference's
- *     seed table values;
- *   - orc_shiftand: ShiftAnd_unittest.cpp known answers (+MISCOUNT) and brute
- *     force bisulfite edit distance on small inputs;
- *   - orc_dp_align: LevenshtDP_unittest.cpp known answers;
- *   - orc_hopscotch_order: against tsl::hopscotch_map itself (compiled from
- *     the reference's vendored header into oracle/_ref).
+ *   - orc_ntps64 / orc_ntps64_rolling: ntHash::NTPS64 run by the reference
+ *     harness (direct and rolling), and direct == rolling;
+ *   - orc_shiftand_e: ShiftAnd<E> run by the harness (E = 0, 1, 2, 6, both
+ *     directions, N letters), ShiftAnd_unittest.cpp known answers (+MISCOUNT),
+ *     brute-force bisulfite approximate matching on small inputs;
+ *   - orc_dp_align_e: LevenshtDP run by the harness (band 8 with CompiFwd /
+ *     CompiRev, bands 0-2), LevenshtDP_unittest.cpp known answers, a textbook
+ *     banded Levenshtein for the distance;
+ *   - orc_hopscotch_order: tsl::hopscotch_map itself, run by the harness;
+ *   - orc_std_sort_counts: the host's libstdc++ std::sort.
  */
 #include "fame_oracle.h"
 
@@ -541,12 +545,15 @@ static int hs_items(const hsmap* m, uint32_t* keys, uint32_t* vals, int cap)
     return n;
 }
 
-int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count, uint32_t* out)
+int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count, uint32_t* out, uint32_t* vals,
+                        uint64_t* final_bucket_count)
 {
     hsmap m;
     hs_init(&m, bucket_count);
     for (uint32_t i = 0; i < n; ++i) hs_bump_or_insert(&m, keys[i]);
-    int r = hs_items(&m, out, NULL, (int)n);
+    int r = hs_items(&m, out, vals, (int)n);
+    /* bucket_count() hopscotch_hash.h:1139-1150 */
+    if (final_bucket_count) *final_bucket_count = m.arr ? m.arr - NBH + 1 : 0;
     hs_free(&m);
     return r;
 }
@@ -619,8 +626,11 @@ int orc_seed_maps(orc_ctx* c, const uint8_t* seq, uint32_t len, uint32_t* fwd_ke
 /* ShiftAnd<6> ShiftAnd.h:121-432                                       */
 /* ------------------------------------------------------------------ */
 
+/* E is the error count of ShiftAnd<E>: NERR on the matching path; other
+ * values serve the known-answer tests of ShiftAnd_unittest.cpp */
 typedef struct {
     int W;
+    unsigned E;
     uint64_t plen;
     uint64_t masks[4][MAXW];
     uint64_t acc[MAXW];
@@ -639,8 +649,9 @@ static inline int lmap(uint8_t c)
 }
 
 /* loadBitmasks ShiftAnd.h:345-432 */
-static void sa_load(orc_sa* s, const uint8_t* seq, uint32_t L)
+static void sa_load(orc_sa* s, const uint8_t* seq, uint32_t L, unsigned E)
 {
+    s->E = E;
     s->plen = L;
     s->W = (int)(L / 64 + 1);
     for (int i = 0; i < s->W; ++i) s->acc[i] = 0;
@@ -676,7 +687,7 @@ static void sa_load(orc_sa* s, const uint8_t* seq, uint32_t L)
 /* reset ShiftAnd.h:259-272 */
 static inline void sa_reset(orc_sa* s)
 {
-    for (unsigned i = 0; i <= NERR; ++i) {
+    for (unsigned i = 0; i <= s->E; ++i) {
         for (int j = s->W - 1; j > 0; --j) s->act[i][j] = 0;
         s->act[i][0] = (1ULL << (i + 1)) - 1;
     }
@@ -687,7 +698,7 @@ static inline void sa_letter(orc_sa* s, uint8_t c)
 {
     const uint64_t* mask = s->masks[lmap(c)];
     const int W = s->W;
-    for (unsigned i = NERR; i > 0; --i) {
+    for (unsigned i = s->E; i > 0; --i) {
         for (int j = W - 1; j > 0; --j)
             s->act[i][j] = ((s->act[i][j] << 1 | s->act[i][j - 1] >> 63) & mask[j]) | s->act[i - 1][j] |
                            (s->act[i - 1][j] << 1 | s->act[i - 1][j - 1] >> 63);
@@ -695,7 +706,7 @@ static inline void sa_letter(orc_sa* s, uint8_t c)
     }
     for (int j = W - 1; j > 0; --j) s->act[0][j] = (s->act[0][j] << 1 | s->act[0][j - 1] >> 63) & mask[j];
     s->act[0][0] = (s->act[0][0] << 1 | 1) & mask[0];
-    for (unsigned i = 1; i <= NERR; ++i) {
+    for (unsigned i = 1; i <= s->E; ++i) {
         for (int j = W - 1; j > 0; --j) s->act[i][j] |= s->act[i - 1][j] << 1 | s->act[i - 1][j - 1] >> 63;
         s->act[i][0] |= s->act[i - 1][0] << 1;
     }
@@ -704,7 +715,7 @@ static inline void sa_letter(orc_sa* s, uint8_t c)
 /* isMatch ShiftAnd.h:323-341 */
 static inline int sa_is_match(const orc_sa* s, uint8_t* err)
 {
-    for (unsigned i = 0; i <= NERR; ++i)
+    for (unsigned i = 0; i <= s->E; ++i)
         for (int j = 0; j < s->W; ++j)
             if (s->act[i][j] & s->acc[j]) {
                 *err = (uint8_t)i;
@@ -747,7 +758,7 @@ static void sa_query_fwd(orc_sa* s, const uint8_t* text, uint64_t tlen, matchvec
         }
         sa_letter(s, c);
         ++ncomp;
-        if (ncomp < s->plen - NERR) continue;
+        if (ncomp < s->plen - s->E) continue;
         uint8_t e;
         if (sa_is_match(s, &e)) {
             if (was) {
@@ -793,7 +804,7 @@ static void sa_query_rev(orc_sa* s, const uint8_t* text, uint64_t tlen, matchvec
         }
         sa_letter(s, c);
         ++ncomp;
-        if (ncomp < s->plen - NERR) continue;
+        if (ncomp < s->plen - s->E) continue;
         uint8_t e;
         if (sa_is_match(s, &e)) {
             const uint64_t pos = (p + 1) + s->plen - 1 + MISCOUNT;
@@ -814,12 +825,12 @@ static void sa_query_rev(orc_sa* s, const uint8_t* text, uint64_t tlen, matchvec
     }
 }
 
-int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
-                 uint64_t* pos, uint8_t* err, int cap, int* total)
+int orc_shiftand_e(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
+                   unsigned E, uint64_t* pos, uint8_t* err, int cap, int* total)
 {
-    if (plen == 0 || plen / 64 + 1 > MAXW) return -1;
+    if (plen == 0 || plen / 64 + 1 > MAXW || E > NERR) return -1;
     orc_sa s;
-    sa_load(&s, pat, plen);
+    sa_load(&s, pat, plen, E);
     matchvec v = {0};
     if (rev)
         sa_query_rev(&s, text, tlen, &v);
@@ -834,6 +845,27 @@ int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_
     free(v.pos);
     free(v.err);
     return n;
+}
+
+int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
+                 uint64_t* pos, uint8_t* err, int cap, int* total)
+{
+    return orc_shiftand_e(pat, plen, text, tlen, rev, NERR, pos, err, cap, total);
+}
+
+int orc_shiftand_state(const uint8_t* pat, uint32_t plen, unsigned E, uint64_t* masks, uint64_t* acc,
+                       uint64_t* act)
+{
+    if (plen == 0 || plen / 64 + 1 > MAXW || E > NERR) return -1;
+    orc_sa s;
+    sa_load(&s, pat, plen, E);
+    sa_reset(&s);
+    for (int m = 0; m < 4; ++m)
+        for (int j = 0; j < s.W; ++j) masks[m * s.W + j] = s.masks[m][j];
+    for (int j = 0; j < s.W; ++j) acc[j] = s.acc[j];
+    for (unsigned e = 0; e <= E; ++e)
+        for (int j = 0; j < s.W; ++j) act[e * s.W + j] = s.act[e][j];
+    return s.W;
 }
 
 /* ------------------------------------------------------------------ */
@@ -942,15 +974,15 @@ done:
 typedef struct {
     uint16_t* m;
     long rows; /* L + 1 */
+    long bmb;  /* BandedMatrix band: LevenshtDP band + 1 */
     int ub;
 } bmat;
 
-#define BMB ((long)DPBAND + 1) /* BandedMatrix band */
-
 /* BandedMatrix::operator() BandedMatrix.h:67-75 (including its overlap of
- * cell (8,17) with (9,0)) */
+ * cell (8,17) with (9,0) at band 8) */
 static inline uint16_t* bm_at(bmat* b, long r, long c)
 {
+    const long BMB = b->bmb;
     long off = (BMB - r > 0 ? BMB - r : 0) + (c - r + BMB);
     long idx = r * (2 * BMB + 1) + off;
     if (idx < 0 || idx >= b->rows * (2 * BMB + 1)) {
@@ -1003,12 +1035,38 @@ static inline int imin3(int a, int b, int c)
     return m < c ? m : c;
 }
 
-int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len,
-                 int64_t ref_end, int rev, uint8_t* trace, int cap, int* ub)
+/* the comparators of LevenshtDP_unittest.cpp: equality, and Watson-Crick
+ * complement (its reverse-query test) */
+static inline uint16_t cmp_eq(uint8_t r, uint8_t g) { return r == g ? 0 : 1; }
+static inline uint16_t cmp_wc(uint8_t r, uint8_t g)
 {
-    const long band = DPBAND;
+    switch (r) {
+    case 'A': return g == 'T' ? 0 : 1;
+    case 'C': return g == 'G' ? 0 : 1;
+    case 'G': return g == 'C' ? 0 : 1;
+    case 'T': return g == 'A' ? 0 : 1;
+    }
+    return 1;
+}
+
+static inline int dp_cmp(int mode, uint8_t r, uint8_t g)
+{
+    switch (mode) {
+    case ORC_CMP_FWD: return cmp_fwd(r, g);
+    case ORC_CMP_REV: return cmp_rev(r, g);
+    case ORC_CMP_EQ: return cmp_eq(r, g);
+    default: return cmp_wc(r, g);
+    }
+}
+
+int orc_dp_align_e(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len, int64_t ref_end,
+                   int rev, unsigned band_, int cmp, uint8_t* trace, int cap, int* ub, int* dist)
+{
+    const long band = (long)band_;
+    const long BMB = band + 1;
     bmat b;
     b.rows = (long)L + 1;
+    b.bmb = BMB;
     b.ub = 0;
     b.m = xmalloc((size_t)b.rows * (2 * BMB + 1) * 2);
     for (long i = 0; i < b.rows * (2 * BMB + 1); ++i) b.m[i] = 0xffff;
@@ -1026,7 +1084,7 @@ int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t re
             const long j = r + o;
             const uint8_t rc = rev ? row[r - 1] : row[L - r];
             const uint8_t gc = col_at(&cp, j);
-            const int mm = rev ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
+            const int mm = dp_cmp(cmp, rc, gc);
             const int v = imin3((int)*bm_at(&b, r - 1, j) + 1, (int)*bm_at(&b, r, j - 1) + 1,
                                 (int)*bm_at(&b, r - 1, j - 1) + mm);
             *bm_at(&b, r, j) = (uint16_t)v;
@@ -1045,7 +1103,7 @@ int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t re
     for (long r = (long)L; r > 0;) {
         const uint8_t rc = rev ? row[r - 1] : row[L - r];
         const uint8_t gc = col_at(&cp, (long)col);
-        const int mm = rev ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
+        const int mm = dp_cmp(cmp, rc, gc);
         const long c = (long)col;
         int t;
         if (*bm_at(&b, r - 1, c) < *bm_at(&b, r, c - 1)) {
@@ -1077,7 +1135,15 @@ int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t re
     }
     free(b.m);
     *ub = b.ub | cp.ub;
+    if (dist) *dist = minimum;
     return n;
+}
+
+int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len,
+                 int64_t ref_end, int rev, uint8_t* trace, int cap, int* ub)
+{
+    return orc_dp_align_e(row, L, ref, ref_len, ref_end, rev, DPBAND, rev ? ORC_CMP_REV : ORC_CMP_FWD, trace,
+                          cap, ub, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1256,14 +1322,14 @@ static int match_one(orc_ctx* cx, const uint8_t* in, uint32_t L, int getStranded
     orc_sa* sa = xmalloc(sizeof(orc_sa));
     if (cx->both || getStranded || cx->match_r1_fwd) {
         seed_refs(ix, seq, L, &f, &r, NULL);
-        sa_load(sa, seq, L);
+        sa_load(sa, seq, L, NERR);
         sF = sa_query_seed_set(ix, sa, &f, &r, &matchFwd);
         hs_free(&f);
         hs_free(&r);
     }
     if (cx->both || getStranded || !cx->match_r1_fwd) {
         seed_refs(ix, revSeq, L, &f, &r, NULL);
-        sa_load(sa, revSeq, L);
+        sa_load(sa, revSeq, L, NERR);
         sR = sa_query_seed_set(ix, sa, &f, &r, &matchRev);
         hs_free(&f);
         hs_free(&r);
@@ -1947,10 +2013,10 @@ static void match_pair(orc_ctx* cx, const uint8_t* in1, uint32_t L1, const uint8
             seed_first(cx, s1, L1, &F, &R);
             if (F.nelem || R.nelem) {
                 if (seed_second(cx, rs2, L2, &F, &R)) {
-                    sa_load(sa, s1, L1);
+                    sa_load(sa, s1, L1, NERR);
                     pe_query(cx, sa, &F, &R, 0, &m1F);
                     if (m1F.n) {
-                        sa_load(sa, rs2, L2);
+                        sa_load(sa, rs2, L2, NERR);
                         pe_query(cx, sa, &F, &R, 1, &m2R);
                     }
                 }
@@ -1962,10 +2028,10 @@ static void match_pair(orc_ctx* cx, const uint8_t* in1, uint32_t L1, const uint8
             seed_first(cx, rs1, L1, &F, &R);
             if (F.nelem || R.nelem) {
                 if (seed_second(cx, s2, L2, &F, &R)) {
-                    sa_load(sa, rs1, L1);
+                    sa_load(sa, rs1, L1, NERR);
                     pe_query(cx, sa, &F, &R, 0, &m1R);
                     if (m1R.n) {
-                        sa_load(sa, s2, L2);
+                        sa_load(sa, s2, L2, NERR);
                         pe_query(cx, sa, &F, &R, 1, &m2F);
                     }
                 }

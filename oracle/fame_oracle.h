@@ -65,6 +65,13 @@ void orc_ntps64_rolling(const uint8_t* seq, uint32_t len, uint32_t seedbits, uin
  * number of matches written (<= cap); *total receives the full count. */
 int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
                  uint64_t* pos, uint8_t* err, int cap, int* total);
+/* the same for ShiftAnd<E>, E <= 6 (the known-answer tests use E = 0..2) */
+int orc_shiftand_e(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_t tlen, int rev,
+                   unsigned E, uint64_t* pos, uint8_t* err, int cap, int* total);
+/* ShiftAnd<E>'s state after construction and reset(): masks[4][W] (A, C, G,
+ * T), accepted[W], active[E+1][W]; returns W = plen / 64 + 1 */
+int orc_shiftand_state(const uint8_t* pat, uint32_t plen, unsigned E, uint64_t* masks, uint64_t* acc,
+                       uint64_t* act);
 
 /* LevenshtDP<uint16_t,8> fill + backtrace (LevenshtDP.h:207-544) of read
  * `row` (len L) against reference bytes ending at ref[ref_end] read
@@ -75,10 +82,20 @@ int orc_shiftand(const uint8_t* pat, uint32_t plen, const uint8_t* text, uint32_
 int orc_dp_align(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len,
                  int64_t ref_end, int rev, uint8_t* trace, int cap, int* ub);
 
-/* tsl::hopscotch_map<uint32_t,uint16_t,identity> emulation: inserts keys in
- * the given order into a map constructed with `bucket_count`, and writes the
- * keys in iteration order.  Returns the number of keys written. */
-int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count, uint32_t* out);
+/* LevenshtDP<uint16_t, band> with a chosen comparator: CompiFwd / CompiRev
+ * (ReadQueue.h:295-338) as on the matching path, or the equality and
+ * complement comparators of LevenshtDP_unittest.cpp.  rev selects
+ * runDPFillRev / backtrackDPRev.  *dist receives getEditDist(). */
+enum { ORC_CMP_FWD = 0, ORC_CMP_REV = 1, ORC_CMP_EQ = 2, ORC_CMP_WC = 3 };
+int orc_dp_align_e(const uint8_t* row, uint32_t L, const uint8_t* ref, uint64_t ref_len, int64_t ref_end,
+                   int rev, unsigned band, int cmp, uint8_t* trace, int cap, int* ub, int* dist);
+
+/* tsl::hopscotch_map<uint32_t,uint16_t,identity> emulation: ++map[key] for
+ * the keys in the given order on a map constructed with `bucket_count`;
+ * writes the keys in iteration order, their counts (vals, may be NULL) and
+ * the final bucket_count() (may be NULL).  Returns the number of keys. */
+int orc_hopscotch_order(const uint32_t* keys, uint32_t n, uint64_t bucket_count, uint32_t* out, uint32_t* vals,
+                        uint64_t* final_bucket_count);
 
 /* getSeedRefs ReadQueue.cpp:2329-2461 for one sequence: writes the fwd and
  * rev maps in iteration order as (metaId, count) pairs. */

@@ -51,7 +51,12 @@ def lib():
                                    C.POINTER(C.c_int)]
         L.orc_dp_align.argtypes = [_u8p, C.c_uint32, _u8p, C.c_uint64, C.c_int64, C.c_int, _u8p,
                                    C.c_int, C.POINTER(C.c_int)]
-        L.orc_hopscotch_order.argtypes = [_u32p, C.c_uint32, C.c_uint64, _u32p]
+        L.orc_hopscotch_order.argtypes = [_u32p, C.c_uint32, C.c_uint64, _u32p, _u32p, _u64p]
+        L.orc_shiftand_e.argtypes = [_u8p, C.c_uint32, _u8p, C.c_uint32, C.c_int, C.c_uint, _u64p, _u8p,
+                                     C.c_int, C.POINTER(C.c_int)]
+        L.orc_shiftand_state.argtypes = [_u8p, C.c_uint32, C.c_uint, _u64p, _u64p, _u64p]
+        L.orc_dp_align_e.argtypes = [_u8p, C.c_uint32, _u8p, C.c_uint64, C.c_int64, C.c_int, C.c_uint, C.c_int,
+                                     _u8p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_set_pdist.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.orc_match_pairs.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, _u8p, _u64p, _u32p, C.c_uint32,
                                       C.c_int, _u64p, _u8p, _u64p, _u8p, _u64p, C.c_int]
@@ -245,7 +250,8 @@ def ntps64_rolling(seq, seedbits):
     return out
 
 
-def shiftand(pattern, text, rev=False, cap=4096):
+def shiftand(pattern, text, rev=False, cap=4096, E=6):
+    """ShiftAnd<E> query: [(position, errors)] as querySeq / queryRevSeq report them."""
     p = _bytes(pattern)
     t = _bytes(text)
     if t.size == 0:
@@ -256,25 +262,51 @@ def shiftand(pattern, text, rev=False, cap=4096):
     pos = np.zeros(cap, np.uint64)
     err = np.zeros(cap, np.uint8)
     tot = C.c_int()
-    n = lib().orc_shiftand(_ptr(p, _u8p), p.size, _ptr(t, _u8p), tl, int(rev), _ptr(pos, _u64p),
-                           _ptr(err, _u8p), cap, C.byref(tot))
+    n = lib().orc_shiftand_e(_ptr(p, _u8p), p.size, _ptr(t, _u8p), tl, int(rev), int(E), _ptr(pos, _u64p),
+                             _ptr(err, _u8p), cap, C.byref(tot))
     if n < 0:
-        raise ValueError("pattern too long")
+        raise ValueError("pattern too long or E > 6")
     return [(int(a), int(b)) for a, b in zip(pos[:n], err[:n])]
 
 
-def dp_align(row, ref, ref_end, rev=False, cap=4096):
+def shiftand_state(pattern, E):
+    """(masks[4][W], accepted[W], active[E+1][W]) after construction and reset()."""
+    p = _bytes(pattern)
+    W = p.size // 64 + 1
+    masks = np.zeros(4 * W, np.uint64)
+    acc = np.zeros(W, np.uint64)
+    act = np.zeros((E + 1) * W, np.uint64)
+    w = lib().orc_shiftand_state(_ptr(p, _u8p), p.size, int(E), _ptr(masks, _u64p), _ptr(acc, _u64p),
+                                 _ptr(act, _u64p))
+    if w < 0:
+        raise ValueError("pattern too long or E > 6")
+    return masks.reshape(4, W), acc, act.reshape(E + 1, W)
+
+
+CMP_FWD, CMP_REV, CMP_EQ, CMP_WC = 0, 1, 2, 3
+
+
+def dp_align(row, ref, ref_end, rev=False, cap=4096, band=8, cmp=None):
+    """LevenshtDP fill + backtrack: (trace, undefined-read flag, edit distance).
+    cmp defaults to CompiRev for rev, CompiFwd otherwise (the matching path)."""
     r = _bytes(row)
     g = _bytes(ref)
     tr = np.zeros(cap, np.uint8)
     ub = C.c_int()
-    n = lib().orc_dp_align(_ptr(r, _u8p), r.size, _ptr(g, _u8p), g.size, int(ref_end), int(rev),
-                           _ptr(tr, _u8p), cap, C.byref(ub))
-    return tr[:n].tolist(), bool(ub.value)
+    dist = C.c_int()
+    if cmp is None:
+        cmp = CMP_REV if rev else CMP_FWD
+    n = lib().orc_dp_align_e(_ptr(r, _u8p), r.size, _ptr(g, _u8p), g.size, int(ref_end), int(rev), int(band),
+                             int(cmp), _ptr(tr, _u8p), cap, C.byref(ub), C.byref(dist))
+    return tr[:n].tolist(), bool(ub.value), int(dist.value)
 
 
 def hopscotch_order(keys, bucket_count):
+    """(keys in iteration order, their counts, final bucket_count())."""
     k = np.ascontiguousarray(keys, dtype=np.uint32)
     out = np.zeros(max(1, k.size), np.uint32)
-    n = lib().orc_hopscotch_order(_ptr(k, _u32p), k.size, int(bucket_count), _ptr(out, _u32p))
-    return out[:n]
+    vals = np.zeros(max(1, k.size), np.uint32)
+    bc = np.zeros(1, np.uint64)
+    n = lib().orc_hopscotch_order(_ptr(k, _u32p), k.size, int(bucket_count), _ptr(out, _u32p),
+                                  _ptr(vals, _u32p), _ptr(bc, _u64p))
+    return out[:n], vals[:n], int(bc[0])

@@ -23,6 +23,12 @@
 //       reports the end; writes <out>: "FAMEFQ01", u64 n, u32 len[n], then the
 //       n sequences back to back, and per parse call (u32 procReads, u8 more)
 //       after them, preceded by a u64 count of calls
+//   units ntps64|shiftand|dp|hopscotch <in.bin> <out.bin>
+//       runs one of FAME's building blocks on the cases of <in.bin> (formats
+//       in do_units below): ntHash::NTPS64 direct and rolling
+//       (nthash.hpp:480-498), ShiftAnd<E>::querySeq / queryRevSeq, the
+//       LevenshtDP fill + backtrack used by computeMethLvl, and the
+//       iteration order of tsl::hopscotch_map<uint32_t, uint16_t, MetaHash>
 //
 // ReadQueue keeps readBuffer / strand tallies private; they are reached by
 // compiling this translation unit only with `private` defined as `public`.
@@ -279,6 +285,187 @@ static int do_parse(int argc, char** argv)
     return 0;
 }
 
+// ---- units ---------------------------------------------------------------
+// Input and output are little-endian binary streams; every case is
+// self-describing.
+
+struct In {
+    std::ifstream f;
+    template <typename T> T get()
+    {
+        T v{};
+        f.read(reinterpret_cast<char*>(&v), sizeof v);
+        return v;
+    }
+    std::string bytes(size_t n)
+    {
+        std::string s(n, '\0');
+        f.read(&s[0], static_cast<std::streamsize>(n));
+        return s;
+    }
+};
+
+struct Out {
+    std::ofstream f;
+    template <typename T> void put(T v) { f.write(reinterpret_cast<const char*>(&v), sizeof v); }
+};
+
+// ShiftAnd<E>: case = u8 E, u8 rev, u32 plen, pattern, u32 tlen, text.
+// rev scans text[tlen-1] down to text[0] (queryRevSeq with `end` one before
+// text[0], which a leading pad letter keeps inside the vector); out = u32 n,
+// then n x (u64 position, u8 errors)
+template <size_t E>
+static void sa_case(std::string& pat, const std::string& text, bool rev, std::array<uint8_t, 16>& lmap, Out& o)
+{
+    ShiftAnd<E> sa(pat, lmap);
+    std::vector<char> t;
+    t.push_back('A');
+    t.insert(t.end(), text.begin(), text.end());
+    std::vector<uint64_t> pos;
+    std::vector<uint8_t> err;
+    if (rev) sa.queryRevSeq(t.end() - 1, t.begin(), pos, err);
+    else sa.querySeq(t.begin() + 1, t.end(), pos, err);
+    o.put<uint32_t>(static_cast<uint32_t>(pos.size()));
+    for (size_t i = 0; i < pos.size(); ++i) {
+        o.put<uint64_t>(pos[i]);
+        o.put<uint8_t>(err[i]);
+    }
+}
+
+// LevenshtDP<uint16_t, B>: case = u8 band, u8 rev, u8 cmp (0 CompiFwd,
+// 1 CompiRev, 2 equality, 3 complement), u32 L, row, u32 rlen, ref,
+// u64 ref_end (colPat = ref + ref_end); out = u16 getEditDist(), u32 n,
+// n x u8 ERROR_T
+struct CmpEq {
+    uint8_t operator()(const char a, const char b) { return a == b ? 0 : 1; }
+};
+struct CmpWc {
+    uint8_t operator()(const char a, const char b)
+    {
+        switch (a) {
+        case 'A': return b == 'T' ? 0 : 1;
+        case 'C': return b == 'G' ? 0 : 1;
+        case 'G': return b == 'C' ? 0 : 1;
+        case 'T': return b == 'A' ? 0 : 1;
+        }
+        return 1;
+    }
+};
+
+template <size_t B, typename C>
+static void dp_run(const std::string& row, const char* col, bool rev, Out& o)
+{
+    C comp;
+    LevenshtDP<uint16_t, B> lev(row, col);
+    std::vector<ERROR_T> al;
+    if (rev) {
+        lev.template runDPFillRev<C>(comp);
+        lev.template backtrackDPRev<C>(comp, al);
+    } else {
+        lev.template runDPFill<C>(comp);
+        lev.template backtrackDP<C>(comp, al);
+    }
+    o.put<uint16_t>(lev.getEditDist());
+    o.put<uint32_t>(static_cast<uint32_t>(al.size()));
+    for (auto e : al) o.put<uint8_t>(static_cast<uint8_t>(e));
+}
+
+template <size_t B>
+static void dp_case(const std::string& row, const char* col, bool rev, int cmp, Out& o)
+{
+    switch (cmp) {
+    case 0: dp_run<B, ReadQueue::CompiFwd>(row, col, rev, o); break;
+    case 1: dp_run<B, ReadQueue::CompiRev>(row, col, rev, o); break;
+    case 2: dp_run<B, CmpEq>(row, col, rev, o); break;
+    default: dp_run<B, CmpWc>(row, col, rev, o); break;
+    }
+}
+
+static int do_units(int argc, char** argv)
+{
+    if (argc < 5) return 2;
+    const std::string kind = argv[2];
+    In in;
+    in.f.open(argv[3], std::ios::binary);
+    Out o;
+    o.f.open(argv[4], std::ios::binary | std::ios::trunc);
+    if (!in.f || !o.f) return 1;
+    const uint32_t ncase = in.get<uint32_t>();
+    if (kind == "ntps64") {
+        // case = u32 len, bases; out = (len-31) direct hashes, then the same
+        // k-mers by the rolling update as getSeedRefs runs it (ReadQueue.cpp:2363-2404)
+        for (uint32_t c = 0; c < ncase; ++c) {
+            const uint32_t len = in.get<uint32_t>();
+            std::string seq = in.bytes(len);
+            const unsigned K = MyConst::KMERLEN;
+            uint64_t h = 0;
+            for (uint32_t k = 0; k + K <= len; ++k) o.put<uint64_t>(ntHash::NTPS64(seq.data() + k, MyConst::SEED, K, h));
+            uint64_t fh = 0;
+            o.put<uint64_t>(ntHash::NTPS64(seq.data(), MyConst::SEED, K, fh));
+            for (uint32_t k = 0; k + K < len; ++k)
+                o.put<uint64_t>(ntHash::NTPS64(seq.data() + k + 1, MyConst::SEED, seq[k], seq[k + K], K, fh));
+        }
+    } else if (kind == "shiftand") {
+        std::array<uint8_t, 16> lmap{};
+        lmap['A' % 16] = 0;
+        lmap['C' % 16] = 1;
+        lmap['G' % 16] = 2;
+        lmap['T' % 16] = 3;
+        for (uint32_t c = 0; c < ncase; ++c) {
+            const uint8_t E = in.get<uint8_t>();
+            const bool rev = in.get<uint8_t>() != 0;
+            std::string pat = in.bytes(in.get<uint32_t>());
+            const std::string text = in.bytes(in.get<uint32_t>());
+            switch (E) {
+            case 0: sa_case<0>(pat, text, rev, lmap, o); break;
+            case 1: sa_case<1>(pat, text, rev, lmap, o); break;
+            case 2: sa_case<2>(pat, text, rev, lmap, o); break;
+            case 3: sa_case<3>(pat, text, rev, lmap, o); break;
+            case 4: sa_case<4>(pat, text, rev, lmap, o); break;
+            case 5: sa_case<5>(pat, text, rev, lmap, o); break;
+            case 6: sa_case<6>(pat, text, rev, lmap, o); break;
+            default: return 1;
+            }
+        }
+    } else if (kind == "dp") {
+        for (uint32_t c = 0; c < ncase; ++c) {
+            const uint8_t band = in.get<uint8_t>();
+            const bool rev = in.get<uint8_t>() != 0;
+            const int cmp = in.get<uint8_t>();
+            const std::string row = in.bytes(in.get<uint32_t>());
+            const std::string ref = in.bytes(in.get<uint32_t>());
+            const uint64_t end = in.get<uint64_t>();
+            const char* col = ref.data() + end;
+            switch (band) {
+            case 0: dp_case<0>(row, col, rev, cmp, o); break;
+            case 1: dp_case<1>(row, col, rev, cmp, o); break;
+            case 2: dp_case<2>(row, col, rev, cmp, o); break;
+            case 8: dp_case<8>(row, col, rev, cmp, o); break;
+            default: return 1;
+            }
+        }
+    } else if (kind == "hopscotch") {
+        // case = u64 bucket_count, u32 n, n x u32 keys (repeats bump the
+        // count, as ++map[metaId] in getSeedRefs); out = u32 m, m x (u32 key,
+        // u32 count) in iteration order, u64 bucket_count() at the end
+        for (uint32_t c = 0; c < ncase; ++c) {
+            const uint64_t bc = in.get<uint64_t>();
+            const uint32_t n = in.get<uint32_t>();
+            tsl::hopscotch_map<uint32_t, uint16_t, ReadQueue::MetaHash> m(bc);
+            for (uint32_t i = 0; i < n; ++i) ++m[in.get<uint32_t>()];
+            o.put<uint32_t>(static_cast<uint32_t>(m.size()));
+            for (auto it = m.begin(); it != m.end(); ++it) {
+                o.put<uint32_t>(it->first);
+                o.put<uint32_t>(it->second);
+            }
+            o.put<uint64_t>(m.bucket_count());
+        }
+    } else {
+        return 2;
+    }
+    return in.f ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -291,6 +478,7 @@ int main(int argc, char** argv)
     if (mode == "match") return do_match(argc, argv);
     if (mode == "match_pe") return do_match_pe(argc, argv);
     if (mode == "parse") return do_parse(argc, argv);
+    if (mode == "units") return do_units(argc, argv);
     std::cerr << "unknown mode\n";
     return 2;
 }
