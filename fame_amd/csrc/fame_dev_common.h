@@ -389,4 +389,14 @@ __device__ __forceinline__ uint32_t job_match_at(const BatchDev& bd, uint32_t po
 }
 
 
+// slot table entries (k_build_slots): bucket length, first entry or kent
+// offset, second entry
+__device__ inline uint32_t slot_len(uint64_t s)
+{
+    const uint32_t t = (uint32_t)(s >> 62);
+    return t < 3 ? t : ((uint32_t)(s >> 32) & 0x3FFFFFFFu);
+}
+__device__ inline uint32_t slot_lo(uint64_t s) { return (s >> 62) == 3 ? (uint32_t)s : ((uint32_t)s & 0x7FFFFFFFu); }
+__device__ inline uint32_t slot_e1(uint64_t s) { return (uint32_t)(s >> 31) & 0x7FFFFFFFu; }
+
 } // namespace fame

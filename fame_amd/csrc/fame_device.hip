@@ -120,6 +120,7 @@ struct SeedWork {
     TabT* klo;      // [K] bucket start
     uint32_t* klen; // [K] bucket length
     uint32_t* koff; // [K] exclusive prefix of klen
+    uint32_t* k2;   // [K] second entry of a two-entry bucket read from ix.slot (NULL: no slots)
     uint32_t* ent;  // [cap_e] packed metaId<<1 | fwd
     uint8_t* ekm;   // [cap_e] k-mer index of each entry
     uint32_t* eaux; // [cap_e] kept / first / count (or hash slot, heavy)
@@ -143,34 +144,70 @@ struct SeedStats {
     uint64_t kmers, entries, seeded, bytes;
 };
 
+// NTPS64 of k-mer k of the reduced-letter row `code` (closed form), reduced
+// to a hash key (h % HTABSIZE)
+__device__ __forceinline__ uint64_t seed_key(const DevIndex& ix, const SeedShared& sh, const uint8_t* code,
+                                             uint32_t k)
+{
+    uint64_t h = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < KMERLEN; ++i)
+        if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][code[k + i]];
+    return ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
+}
+
+// slot-table mode: the hashes of one (read, orientation) and the loads of
+// its K slots, issued without waiting for them (lanes past K load slot 0,
+// which the consumer ignores); seed_one takes the values with staged = true
+template <int KPL>
+__device__ __forceinline__ void seed_stage(const DevIndex& ix, const SeedShared& sh, const uint8_t* code,
+                                           uint32_t K, uint64_t (&sl)[KPL])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = lane + 64 * q;
+        const uint64_t key = k < K ? seed_key(ix, sh, code, k) : 0;
+        sl[q] = ix.slot[key];
+    }
+}
+
 template <typename TabT, typename IdxT, int KPL>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
-                        uint64_t* E_out, uint32_t* bc_out, uint32_t cw)
+                        uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
+                        const uint64_t* pre = nullptr)
 {
     const int lane = threadIdx.x & 63;
     // cw: this lane's word of the row of reduced letters (the length is in
     // byte 255, the top byte of lane 63's word)
     const uint32_t L = (uint32_t)__shfl(cw, 63, 64) >> 24;
-    ((uint32_t*)w.code)[lane] = cw;
-    wave_sync();
+    if (!staged) {
+        ((uint32_t*)w.code)[lane] = cw;
+        wave_sync();
+    }
     const uint32_t K = L - KMERLEN + 1;
     const TabT* tab = (const TabT*)ix.tab;
-    // hashes + tabIndex gathers (NTPS64 closed form; key = h % HTABSIZE)
-    // all gathers of a lane are issued before the first is consumed
+    const bool inl = ix.slot != nullptr && w.k2 != nullptr;
+    // hashes + bucket gathers (NTPS64 closed form; key = h % HTABSIZE): one
+    // slot per k-mer, or tabIndex[key], tabIndex[key + 1]; all gathers of a
+    // lane are issued before the first is consumed.  staged: the slots were
+    // loaded by seed_stage (pre)
     TabT lo[KPL], hi[KPL];
+    uint64_t sl[KPL];
 #pragma unroll
     for (int q = 0; q < KPL; ++q) {
         const uint32_t k = lane + 64 * q;
         lo[q] = hi[q] = 0;
-        if (k < K) {
-            uint64_t h = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < KMERLEN; ++i)
-                if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][w.code[k + i]];
-            const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
-            lo[q] = tab[key];
-            hi[q] = tab[key + 1];
+        sl[q] = staged ? pre[q] : 0;
+        if (k < K && !staged) {
+            const uint64_t key = seed_key(ix, sh, w.code, k);
+            if (inl) {
+                sl[q] = ix.slot[key];
+            } else {
+                lo[q] = tab[key];
+                hi[q] = tab[key + 1];
+            }
         }
     }
     uint32_t bsum = 0;
@@ -178,10 +215,18 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     for (int q = 0; q < KPL; ++q) {
         const uint32_t k = lane + 64 * q;
         if (k < K) {
-            w.klo[k] = lo[q];
-            const uint64_t len = (uint64_t)(hi[q] - lo[q]);
-            w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
-            bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+            if (inl) {
+                const uint32_t len = slot_len(sl[q]);
+                w.klo[k] = (TabT)slot_lo(sl[q]);
+                w.k2[k] = slot_e1(sl[q]);
+                w.klen[k] = len;
+                bsum += len;
+            } else {
+                w.klo[k] = lo[q];
+                const uint64_t len = (uint64_t)(hi[q] - lo[q]);
+                w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+                bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+            }
         }
     }
     bsum = wave_sum_u32(bsum);
@@ -223,7 +268,11 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         }
         uint32_t k = lo;
         while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
-        w.ent[j] = ix.kent[(uint64_t)w.klo[k] + (j - w.koff[k])] & 0x7FFFFFFFu; // without isStart
+        const uint32_t d = j - w.koff[k];
+        if (inl && w.klen[k] <= 2) // the bucket is in its slot
+            w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
+        else
+            w.ent[j] = ix.kent[(uint64_t)w.klo[k] + d] & 0x7FFFFFFFu; // without isStart
         w.ekm[j] = (uint8_t)k;
     }
     wave_sync();
@@ -411,6 +460,7 @@ struct SeedLds {
     TabT klo[ML];
     uint32_t klen[ML];
     uint32_t koff[ML];
+    uint32_t k2[ML];
     union {
         struct {
             uint32_t ent[C::E];
@@ -459,6 +509,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     w.klo = m.klo;
     w.klen = m.klen;
     w.koff = m.koff;
+    w.k2 = m.k2;
     w.ent = m.u.a.ent;
     w.ekm = m.u.a.ekm;
     w.eaux = m.u.a.eaux;
@@ -480,7 +531,70 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
     const uint32_t nw = gridDim.x * FAME_SEED_WPB;
-    for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; ro < n_ro; ro += nw) {
+    auto defer = [&](uint32_t ro, uint64_t E, uint32_t bc) {
+        if (lane == 0) {
+            const uint32_t h = atomicAdd(bd.heavy_n, 1u);
+            if (h < bd.heavy_cap) {
+                bd.heavy_ro[h] = ro;
+                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_bc[h] = bc;
+            } else {
+                atomicOr(bd.flags, FLAG_HEAVY_FULL);
+            }
+            bd.cand_nf[ro] = 0;
+            bd.cand_nr[ro] = 0;
+            bd.cand_off[ro] = 0;
+        }
+    };
+    if (ix.slot) {
+        // slot-table mode, software-pipelined over this wave's pairs: while
+        // pair j is worked on, the slot loads of pair j+1 and the code row of
+        // pair j+2 are in flight (rows past the batch read row 0, unused)
+        auto row = [&](uint32_t r) {
+            return ((const uint32_t*)(bd.codes + (size_t)(r < n_ro ? r : 0u) * 256))[lane];
+        };
+        uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv;
+        uint32_t cwc = row(ro), cwn = row(ro + nw);
+        uint32_t Lc = ro < n_ro ? ((uint32_t)__shfl(cwc, 63, 64) >> 24) : 0;
+        uint64_t slc[C::KPL], sln[C::KPL];
+#pragma unroll
+        for (int q = 0; q < C::KPL; ++q) slc[q] = sln[q] = 0;
+        if (Lc) {
+            ((uint32_t*)w.code)[lane] = cwc;
+            wave_sync();
+            seed_stage<C::KPL>(ix, sh, w.code, Lc - KMERLEN + 1, slc);
+        }
+        for (; ro < n_ro; ro += nw) {
+            const uint32_t rn = ro + nw;
+            const uint32_t cwnn = row(rn + nw);
+            const uint32_t Ln = rn < n_ro ? ((uint32_t)__shfl(cwn, 63, 64) >> 24) : 0;
+            if (Ln) {
+                wave_sync(); // the previous pair's hashing has read w.code
+                ((uint32_t*)w.code)[lane] = cwn;
+                wave_sync();
+                seed_stage<C::KPL>(ix, sh, w.code, Ln - KMERLEN + 1, sln);
+            }
+            if (Lc == 0) { // not seeded: invalid read or orientation not asked for
+                if (lane == 0) {
+                    bd.cand_nf[ro] = 0;
+                    bd.cand_nr[ro] = 0;
+                    bd.cand_off[ro] = 0;
+                }
+            } else {
+                uint64_t E;
+                uint32_t bc;
+                const int rc = seed_one<TabT, uint16_t, C::KPL>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, cwc,
+                                                                  true, slc);
+                if (rc != 0) defer(ro, E, bc);
+            }
+#pragma unroll
+            for (int q = 0; q < C::KPL; ++q) slc[q] = sln[q];
+            cwc = cwn;
+            cwn = cwnn;
+            Lc = Ln;
+        }
+    }
+    for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; !ix.slot && ro < n_ro; ro += nw) {
         const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
         if ((__shfl(cw, 63, 64) >> 24) == 0) { // not seeded: invalid read or orientation not asked for
             if (lane == 0) {
@@ -494,19 +608,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         uint64_t E;
         uint32_t bc;
         const int rc = seed_one<TabT, uint16_t, C::KPL>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc, cw);
-        if (rc != 0 && lane == 0) {
-            const uint32_t h = atomicAdd(bd.heavy_n, 1u);
-            if (h < bd.heavy_cap) {
-                bd.heavy_ro[h] = ro;
-                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
-                bd.heavy_bc[h] = bc;
-            } else {
-                atomicOr(bd.flags, FLAG_HEAVY_FULL);
-            }
-            bd.cand_nf[ro] = 0;
-            bd.cand_nr[ro] = 0;
-            bd.cand_off[ro] = 0;
-        }
+        if (rc != 0) defer(ro, E, bc);
     }
     if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
     if (lane == 0 && ss.seeded) {
@@ -544,6 +646,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.klo = (TabT*)take(8 * FAME_DEV_MAX_READ);
     w.klen = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
     w.koff = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
+    w.k2 = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
     w.ent = (uint32_t*)take(4ull * ce);
     w.ekm = (uint8_t*)take(ce);
     w.eaux = (uint32_t*)take(4ull * ce);
@@ -659,10 +762,23 @@ __device__ __forceinline__ uint32_t spread16(uint32_t x)
 }
 
 // returns the number of window bytes scanned (0 when the lane has no job)
+// multipliers of the piece hash (odd, 24 bits), tried in turn per pattern
+__constant__ uint32_t c_piece_mul[16] = {0x9E3779u, 0x7F4A7Du, 0x85EBCBu, 0xC2B2AFu, 0x27D4EBu, 0x165667u,
+                                         0xD6E8FDu, 0xA0761Du, 0xE70371u, 0x8EBC6Bu, 0x5851F5u, 0x2545F5u,
+                                         0xB5297Bu, 0x68E31Du, 0xF1357Bu, 0x1B8735u};
+
+// the low 32 bits of the product of the low 24 bits of a and b (HIP's
+// __umul24 returns them as a signed int, which a right shift would extend)
+__device__ __forceinline__ uint32_t mul24u(uint32_t a, uint32_t b) { return (uint32_t)__umul24(a, b); }
+
+// bucket of a top-aligned 16-letter code: the top 3 bits of the 24-bit
+// product of its last 10 letters (bits 12-31) with the multiplier
+__device__ __forceinline__ uint32_t piece_bucket(uint32_t x, uint32_t mul) { return mul24u(x >> 12, mul) >> 29; }
+
 template <int W>
 __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
                                                const uint32_t* __restrict__ only, uint32_t tid,
-                                               uint64_t* __restrict__ s_peq)
+                                               uint64_t* __restrict__ s_peq, uint2* __restrict__ s_ptab)
 {
     if (only && bd.ovf_pool[tid] != pool) return 0;
     const uint32_t j = only ? only[tid] : tid;
@@ -690,23 +806,63 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     bool filt = W >= 2 && FP >= FILT_MIN_P;
     {
         uint64_t pm[4][W];
+        {
+            // The pattern arrives in aligned 16-byte chunks, each holding at
+            // least one pattern byte (so no load leaves the buffer's pages).
+            // The letters of each class (A, C, G, T, exact byte compares:
+            // other bytes match nothing) become one bit per byte at raw
+            // position 16 * chunk + byte; the raw masks are then shifted down
+            // by the pattern's offset in its first chunk and cut at m.
+            const uint32_t psk = (uint32_t)((uintptr_t)pat & 15);
+            const uint8_t* p0 = pat - psk;
+            const uint32_t nch = (psk + m + 15) >> 4; // chunks holding pattern bytes
+            constexpr int RW = W + 1;                 // psk + m <= 15 + 64W bits
+            uint64_t raw[4][RW];
+            // bit 7 of each byte of x equal to ch (exact)
+            auto eqb = [](uint32_t x, uint32_t ch) -> uint32_t {
+                const uint32_t t = x ^ (ch * 0x01010101u);
+                return ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
+            };
+            // those four bits as a nibble (byte 0 -> bit 0)
+            auto nib = [](uint32_t z) -> uint64_t { return (uint64_t)((((z >> 7) * 0x204081u) >> 21) & 15u); };
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint64_t a = 0, cc = 0, gg = 0, tt = 0;
-            // unrolled so that groups of byte loads are in flight together
-#pragma unroll 16
-            for (uint32_t i = 64 * w; i < m && i < 64u * (w + 1); ++i) {
-                const uint8_t ch = pat[i];
-                const uint64_t bit = 1ull << (i & 63);
-                a |= ch == 'A' ? bit : 0;
-                cc |= (ch == 'C' || ch == 'T') ? bit : 0;
-                gg |= ch == 'G' ? bit : 0;
-                tt |= ch == 'T' ? bit : 0;
+            for (int rw = 0; rw < RW; ++rw) {
+                uint4 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t c = (uint32_t)(4 * rw + q);
+                    v[q] = *(const uint4*)(p0 + 16 * (c < nch ? c : nch - 1));
+                }
+                uint64_t wa = 0, wc = 0, wg = 0, wt = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool live = (uint32_t)(4 * rw + q) < nch;
+                    const uint32_t d4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+                    for (int dd = 0; dd < 4; ++dd) {
+                        const int sh = 16 * q + 4 * dd;
+                        const uint32_t x = live ? d4[dd] : 0u;
+                        wa |= nib(eqb(x, 'A')) << sh;
+                        wc |= nib(eqb(x, 'C')) << sh;
+                        wg |= nib(eqb(x, 'G')) << sh;
+                        wt |= nib(eqb(x, 'T')) << sh;
+                    }
+                }
+                raw[0][rw] = wa;
+                raw[1][rw] = wc | wt; // pattern T also matches text C
+                raw[2][rw] = wg;
+                raw[3][rw] = wt;
             }
-            pm[0][w] = a;
-            pm[1][w] = cc;
-            pm[2][w] = gg;
-            pm[3][w] = tt;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint64_t x = raw[c][w] >> psk;
+                    if (psk) x |= raw[c][w + 1] << (64 - psk);
+                    const int lim = (int)m - 64 * w; // pattern bits in this word
+                    if (lim < 64) x &= lim > 0 ? (~0ull >> (64 - lim)) : 0ull;
+                    pm[c][w] = x;
+                }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
@@ -730,6 +886,64 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 const uint32_t ba = ext(0), bct = ext(1), bg = ext(2);
                 const uint32_t sct = spread16(bct), sg = spread16(bg);
                 pc[i] = ((ba | bct | bg) == fm) ? (sct | (sct << 1) | (sg << 1)) : 0xFFFFFFFFu;
+            }
+        }
+    }
+    // The pieces in a per-lane hash table in LDS (8 buckets of two entries):
+    // each piece top-aligned in 32 bits (its last letter in bits 30-31, the
+    // bits below 32 - 2*FP masked off in compares).  The multiplier is the
+    // first of c_piece_mul under which no bucket holds more than two
+    // distinct pieces; free entries repeat a piece (never a spurious hit).
+    // Pieces with a letter other than A, C, G, T never match and are left
+    // out.  No multiplier found (or no piece): the window is scanned whole.
+    uint32_t hmul = 0, tmask = 0;
+    if (filt) {
+        const uint32_t tsh = 32u - 2u * (uint32_t)FP;
+        tmask = ~0u << tsh;
+        uint32_t pt[NP];
+        bool pv[NP];
+        uint32_t fill = 0;
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            pv[i] = pc[i] != 0xFFFFFFFFu;
+            pt[i] = pv[i] ? pc[i] << tsh : 0u;
+#pragma unroll
+            for (int j = 0; j < i; ++j) pv[i] = pv[i] && !(pv[j] && pt[j] == pt[i]);
+            fill = (!any && pv[i]) ? pt[i] : fill;
+            any = any || pv[i];
+        }
+        bool ok = false;
+        for (int t = 0; t < 16 && any && !ok; ++t) {
+            const uint32_t mu = c_piece_mul[t];
+            uint32_t occ = 0;
+            bool good = true;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                if (!pv[i]) continue;
+                const uint32_t b = piece_bucket(pt[i], mu);
+                good = good && ((occ >> (2 * b)) & 3u) < 2u;
+                occ += 1u << (2 * b);
+            }
+            if (good) {
+                ok = true;
+                hmul = mu;
+            }
+        }
+        if (!ok) {
+            filt = false;
+        } else {
+#pragma unroll
+            for (uint32_t b = 0; b < 8; ++b) {
+                uint32_t e0 = fill, e1 = fill, n = 0;
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const bool in = pv[i] && piece_bucket(pt[i], hmul) == b;
+                    e0 = (in && n == 0) ? pt[i] : e0;
+                    e1 = (in && n == 1) ? pt[i] : e1;
+                    n += in ? 1u : 0u;
+                }
+                s_ptab[b * 256 + threadIdx.x] = make_uint2(e0, e1);
             }
         }
     }
@@ -888,8 +1102,6 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     for (int q = 0; q < FILT_MASKW; ++q) cmask[q] = 0;
     if (filt && nchunk > 64 * FILT_MASKW) filt = false;
     if (filt) {
-        const int sh = 2 * (FP - 1);
-        uint32_t code = 0;
         int64_t x = 0; // scan index of the next text byte
         // marks the scan chunks holding scan positions [s0, s1]
         auto mark_range = [&](int64_t s0, int64_t s1) {
@@ -911,39 +1123,45 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 if (lo <= hi) cmask[q2] |= (~0ull >> (63 - (hi - lo))) << (lo - 64 * q2);
             }
         };
-        // zero when the code equals a piece (kept in vector registers: a
-        // compare per piece would hand each result to the scalar unit)
-        auto hitmin = [&]() -> uint32_t {
-            uint32_t mn = code ^ pc[0];
-#pragma unroll
-            for (int i = 1; i < NP; ++i) mn = min(mn, code ^ pc[i]);
-            return mn;
-        };
-        // the scan of a chunk is short next to a random load: chunks go in
+        // Per text letter q: the 16 letters ending at q, top-aligned, from
+        // this chunk's and the previous chunk's packed letters (2 bits each,
+        // A 0, G 2, C/T 3, scan order from bit 0), looked up in the piece
+        // table.  Letters outside the window only ever add spurious hits.
+        // The scan of a chunk is short next to a random load: chunks go in
         // blocks of two, the next block loaded while this one is scanned
-        // (the empty asm keeps the loads where they are issued)
+        // (the empty asm keeps the loads where they are issued).
+        uint32_t pprev = 0;
+        const uint32_t tbase = threadIdx.x * 8u;
         auto filter_chunk = [&](int64_t ci, const uint4 v) -> bool {
             Chunk k;
             load_chunk(ci, v, k);
             if (k.nflag) return false; // an N restarts the automaton: scan the window whole
-            uint32_t rcw[4];
+            uint32_t P = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) rcw[q] = k.cw[q] | ((k.cw[q] << 1) & 0x02020202u);
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t r = k.cw[q] | ((k.cw[q] << 1) & 0x02020202u);
+                P |= ((r | (r >> 6) | (r >> 12) | (r >> 18)) & 0xFFu) << (8 * q);
+            }
+            // byte offset of letter ii's bucket in the table: bucket * 2048
+            // (= bits 29-31 of the product moved to bits 11-13) | lane * 8
+            auto probe = [&](int ii) -> uint32_t {
+                const uint32_t xc = ii < 15 ? __builtin_amdgcn_alignbit(P, pprev, 2 * (ii + 1)) : P;
+                const uint32_t off = ((mul24u(xc >> 12, hmul) >> 18) & 0x3800u) | tbase;
+                const uint2 e = *(const uint2*)((const char*)s_ptab + off);
+                return min((xc ^ e.x) & tmask, (xc ^ e.y) & tmask);
+            };
             uint32_t acc = 0xFFFFFFFFu;
             if (k.lo == 0 && k.hi == 16) {
 #pragma unroll
-                for (int ii = 0; ii < 16; ++ii) {
-                    code = (code >> 2) | (((rcw[ii >> 2] >> (8 * (ii & 3))) & 3u) << sh);
-                    acc = min(acc, hitmin());
-                }
+                for (int ii = 0; ii < 16; ++ii) acc = min(acc, probe(ii));
             } else {
 #pragma unroll
                 for (int ii = 0; ii < 16; ++ii) {
-                    if (ii < k.lo || ii >= k.hi) continue;
-                    code = (code >> 2) | (((rcw[ii >> 2] >> (8 * (ii & 3))) & 3u) << sh);
-                    acc = min(acc, hitmin());
+                    const uint32_t r = probe(ii);
+                    acc = (ii >= k.lo && ii < k.hi) ? min(acc, r) : acc;
                 }
             }
+            pprev = P;
             if (acc == 0) // a piece ends in this chunk: the union of the regions of any piece there
                 mark_range(x - (int64_t)NP * FP + 1 - (int64_t)NERR,
                            x + (k.hi - k.lo) - 1 - FP + (int64_t)m + (int64_t)NERR);
@@ -1036,6 +1254,7 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
                                                 const uint32_t* __restrict__ only, uint32_t n_only)
 {
     __shared__ uint64_t s_peq[4 * W * 256];
+    __shared__ uint2 s_ptab[W >= 2 ? 8 * 256 : 1];
     __shared__ unsigned long long s_work[2];
     if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
     __syncthreads();
@@ -1047,7 +1266,7 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
     if (bd.n_dev && (*bd.flags & FLAG_ABORT)) njobs = 0;
     uint64_t jobs = 0, bytes = 0;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < njobs; t += gridDim.x * blockDim.x) {
-        const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq);
+        const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab);
         jobs += b ? 1 : 0;
         bytes += b;
     }
@@ -1726,6 +1945,36 @@ __global__ void k_narrow_tab(const uint64_t* __restrict__ in, uint64_t n, uint32
     if (i < n) out[i] = (uint32_t)in[i];
 }
 
+// The slot table of the single-end seed stage: per hash key one u64,
+//   0                                 empty bucket
+//   1 << 62 | e0                      one entry
+//   2 << 62 | e1 << 31 | e0           two entries, in table order
+//   3 << 62 | len << 32 | lo          three or more: kent[lo, lo + len)
+// with e = kent entry without isStart (31 bits, metaId < 2^30).  *bad is set
+// for a bucket of 2^30 entries or more (or a tabIndex that decreases): the
+// table is then not used.
+__global__ void k_build_slots(const uint32_t* __restrict__ tab, const uint32_t* __restrict__ kent, uint64_t htab,
+                              uint64_t* __restrict__ slot, uint32_t* __restrict__ bad)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t key = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; key < htab; key += step) {
+        const uint32_t lo = tab[key], hi = tab[key + 1];
+        const uint32_t len = hi - lo;
+        uint64_t v;
+        if (len == 0) {
+            v = 0;
+        } else if (len == 1) {
+            v = (1ull << 62) | (kent[lo] & 0x7FFFFFFFu);
+        } else if (len == 2) {
+            v = (2ull << 62) | ((uint64_t)(kent[lo + 1] & 0x7FFFFFFFu) << 31) | (kent[lo] & 0x7FFFFFFFu);
+        } else {
+            if (len >= (1u << 30)) atomicOr(bad, 1u);
+            v = (3ull << 62) | ((uint64_t)(len & 0x3FFFFFFFu) << 32) | lo;
+        }
+        slot[key] = v;
+    }
+}
+
 __global__ void k_meth_u16(const uint32_t* __restrict__ in, uint64_t n, uint16_t* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1894,6 +2143,13 @@ void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_
 {
     if (!n) return;
     hipLaunchKernelGGL(k_narrow_tab, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, out);
+}
+
+void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab, uint64_t* slot, uint32_t* bad,
+                        hipStream_t st)
+{
+    if (!htab) return;
+    hipLaunchKernelGGL(k_build_slots, dim3(16384), dim3(256), 0, st, tab, kent, htab, slot, bad);
 }
 
 void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st)

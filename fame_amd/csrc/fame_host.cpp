@@ -81,7 +81,7 @@ struct fame_ctx {
     std::vector<uint8_t> cpg_chrom;
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
-    DevBuf d_tab, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
+    DevBuf d_tab, d_slot, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
@@ -284,6 +284,19 @@ void load_index(fame_ctx* c, const std::string& path)
             HIPC(hipStreamSynchronize(st));
         }
     }
+    // slot table of the single-end seed stage (u32 tabIndex only)
+    if (!c->ix.tab_is64) {
+        HIPC(c->d_slot.reserve(8 * c->htab));
+        DevBuf bad;
+        HIPC(bad.reserve(4));
+        HIPC(hipMemsetAsync(bad.p, 0, 4, st));
+        launch_build_slots(c->d_tab.as<uint32_t>(), c->d_kent.as<uint32_t>(), c->htab, c->d_slot.as<uint64_t>(),
+                           bad.as<uint32_t>(), st);
+        uint32_t b = 0;
+        HIPC(hipMemcpyAsync(&b, bad.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (b) c->d_slot.release();
+    }
     f.skip(8 * f.get<uint64_t>()); // metaCpGs (empty in current builds)
     f.skip(8 * f.get<uint64_t>()); // metaStartCpGs
     // windows {u32 startPos; u8 chrom; 3 pad; u32 startInd; u32 endInd}
@@ -339,6 +352,7 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.hpow2 = (c->htab & (c->htab - 1)) == 0;
     ix.hmask = c->htab - 1;
     ix.kent = c->d_kent.as<uint32_t>();
+    ix.slot = c->d_slot.p ? c->d_slot.as<uint64_t>() : nullptr;
     ix.kpe = c->d_kpe.as<uint64_t>();
     ix.filt32 = c->d_filt32.as<uint32_t>();
     ix.win = c->d_win.as<uint4>();
@@ -515,21 +529,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
                 hs.hsize = (uint32_t)rup(2 * emax);
                 uint64_t tbb = std::max(rup(bcmax), rup(2 * emax));
                 hs.tb_cap = (uint32_t)(2 * tbb + 64);
-                size_t stride = 0;
-                auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
-                add(8 * FAME_DEV_MAX_READ);
-                add(4 * FAME_DEV_MAX_READ);
-                add(4 * FAME_DEV_MAX_READ);
-                add(4 * emax);
-                add(emax);
-                add(4 * emax);
-                for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
-                add(8 * emax);
-                add(4 * emax);
-                add(4ull * hs.tb_cap);
-                add(4 * emax);
-                add(4 * emax);
-                add(8 * emax);
+                const size_t stride = heavy_scratch_stride(hs);
                 hs.stride = stride;
                 const size_t budget = 2ull << 30;
                 const uint32_t cnt = c1 - c0;
@@ -761,22 +761,7 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
             hs.cap_e = (uint32_t)emax;
             hs.hsize = (uint32_t)(2 * emax);
             hs.tb_cap = (uint32_t)(2 * (2 * emax) + 64);
-            size_t stride = 0;
-            auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
-            add(8 * FAME_DEV_MAX_READ);
-            add(4 * FAME_DEV_MAX_READ);
-            add(4 * FAME_DEV_MAX_READ);
-            add(4 * emax);
-            add(emax);
-            add(4 * emax);
-            for (int i = 0; i < 3; ++i) add(4ull * hs.hsize);
-            add(8 * emax);
-            add(4 * emax);
-            add(4ull * hs.tb_cap);
-            add(4 * emax);
-            add(4 * emax);
-            add(8 * emax);
-            hs.stride = stride;
+            hs.stride = heavy_scratch_stride(hs);
         };
         dims(pl->hs, 16384);
         pl->hs.e_lo = 0;

@@ -52,6 +52,11 @@ struct DevIndex {
     uint64_t htab, hmask;
     int hpow2;
     const uint32_t* kent;
+    // per hash key, one u64 that holds a bucket of one or two entries itself
+    // (so that the single-end seed stage reads one line per k-mer), or its
+    // offset and length in kent; NULL when some bucket does not fit
+    // (see k_build_slots)
+    const uint64_t* slot;
     const uint64_t* kpe;
     const uint32_t* filt32;
     uint32_t nfilt32;
@@ -169,6 +174,30 @@ struct HeavyScratch {
     uint32_t e_lo, e_hi; // device-list mode: only the items with e_lo <= E <= e_hi
 };
 
+// bytes of one block's slice of heavy scratch: the arrays k_seed_heavy takes
+// from its slice, in the same order, each rounded up to 256 bytes
+inline size_t heavy_scratch_stride(const HeavyScratch& hs)
+{
+    size_t stride = 0;
+    auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
+    const size_t e = hs.cap_e;
+    add(8 * FAME_DEV_MAX_READ); // klo
+    add(4 * FAME_DEV_MAX_READ); // klen
+    add(4 * FAME_DEV_MAX_READ); // koff
+    add(4 * FAME_DEV_MAX_READ); // k2
+    add(4 * e);                 // ent
+    add(e);                     // ekm
+    add(4 * e);                 // eaux
+    for (int i = 0; i < 3; ++i) add(4ull * hs.hsize); // hkey, hfirst, hcnt
+    add(8 * e);                 // ins_key
+    add(4 * e);                 // ins_cnt
+    add(4ull * hs.tb_cap);      // tb
+    add(4 * e);                 // tmp
+    add(4 * e);                 // over
+    add(8 * e);                 // cand
+    return stride;
+}
+
 void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st);
 // batched form: counters and lists of one batch cleared on the device
 void launch_batch_reset(const BatchDev& bd, hipStream_t st);
@@ -211,6 +240,8 @@ void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, ui
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
+void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab, uint64_t* slot, uint32_t* bad,
+                        hipStream_t st);
 void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st);
 void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st);
 void launch_max_end(const uint64_t* offs, const uint32_t* lens, uint32_t n, unsigned long long* out, hipStream_t st);
