@@ -40,8 +40,9 @@ namespace fame {
 // k_prep
 // ---------------------------------------------------------------------------
 
-// one wavefront per read (persistent, grid-stride): coalesced loads of the
-// read, the reverse complement stored reversed, N found by a ballot
+// one wavefront per read (persistent, grid-stride over groups of 64 reads):
+// coalesced loads of the read, the reverse complement stored reversed, N
+// found by a ballot
 // Also writes, per (read, orientation) to be seeded, a 256-byte row of the
 // reduced letters (red_code) with the length in byte 255 (0 for a pair the
 // seed stage skips), so that the seed stage reads one coalesced row per pair
@@ -54,58 +55,103 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                                               uint32_t nsplit, uint32_t minlen, int paired,
                                               const uint32_t* __restrict__ n_dev)
 {
+    // per byte value c: complement (bits 0-7), red_code(c) (8-9),
+    // red_code(complement) (10-11), c == 'N' (12), c in A C G T N (13)
+    __shared__ uint16_t s_lt[256];
+    for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) {
+        uint32_t o = 0;
+        switch (v) {
+        case 'A': o = 'T'; break;
+        case 'C': o = 'G'; break;
+        case 'G': o = 'C'; break;
+        case 'T': o = 'A'; break;
+        default: break;
+        }
+        const bool known = v == 'A' || v == 'C' || v == 'G' || v == 'T' || v == 'N';
+        s_lt[v] = (uint16_t)(o | (red_code((uint8_t)v) << 8) | (red_code((uint8_t)o) << 10) | ((v == 'N') << 12) |
+                             (known << 13));
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     if (n_dev) n = *n_dev;
     uint32_t lmax = 0; // longest read this wave prepared for seeding
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t r = wave; r < n; r += nwave) {
-        const uint32_t L = lens[r];
-        const uint8_t* s = seqs + offs[r];
-        uint8_t* q = rev + offs[r];
-        uint8_t st = 0;
-        // ori 0 / 1 of mate 2 (reads from nsplit on) pair with ori 1 / 0 of mate 1
-        const uint32_t om = r < nsplit ? ori_mask : (((ori_mask & 1) << 1) | ((ori_mask >> 1) & 1));
-        if (!paired) {
-            // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned
-            // subtraction of the reference is kept
-            if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
-            if (L < KMERLEN) st = 3;
-        } else if (L < minlen) {
-            st = 1; // readSize < ceil(0.75 READLEN) (ReadQueue.cpp:971-980); minlen >= KMERLEN
-        }
-        if (L > FAME_DEV_MAX_READ) {
-            if (lane == 0) atomicOr(err_flag, 1u);
-            st = 3;
-        }
-        uint8_t* c0 = codes + (size_t)(2 * r) * 256;
-        uint8_t* c1 = c0 + 256;
-        if (st == 0 || (paired && st == 1)) {
-            bool nflag = false;
-            for (uint32_t p = lane; p < L; p += 64) {
-                const uint8_t c = s[p];
-                uint8_t o = 0;
-                switch (c) {
-                case 'A': o = 'T'; break;
-                case 'C': o = 'G'; break;
-                case 'G': o = 'C'; break;
-                case 'T': o = 'A'; break;
-                case 'N': nflag = true; break;
-                default: nflag |= paired != 0; break; // unknown characters stay '\0' in revSeq
-                }
-                q[L - 1 - p] = o;
-                if (om & 1) c0[p] = (uint8_t)red_code(c);
-                if (om & 2) c1[L - 1 - p] = (uint8_t)red_code(o);
+    // groups of 64 reads per wave: one coalesced load of their lengths and
+    // offsets, then read by read, the next read's bytes loaded while this
+    // one is worked on (lane p holds letters p, p + 64, p + 128, p + 192)
+    for (uint32_t r0 = wave * 64; r0 < n; r0 += nwave * 64) {
+        const uint32_t cnt = n - r0 < 64 ? n - r0 : 64;
+        const uint32_t myL = (uint32_t)lane < cnt ? lens[r0 + lane] : 0u;
+        const uint64_t myO = (uint32_t)lane < cnt ? offs[r0 + lane] : 0ull;
+        auto len_of = [&](uint32_t i) { return (uint32_t)__builtin_amdgcn_readlane((int)myL, (int)i); };
+        auto off_of = [&](uint32_t i) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myO, (int)i);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(myO >> 32), (int)i);
+            return ((uint64_t)hi << 32) | lo;
+        };
+        uint8_t nb[4];
+        auto fetch = [&](uint32_t i) {
+            const uint32_t L = len_of(i);
+            const uint32_t Lm = L <= FAME_DEV_MAX_READ ? L : 0u; // longer reads are flagged, not read
+            const uint8_t* sp = seqs + off_of(i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t p = (uint32_t)lane + 64u * q;
+                nb[q] = p < Lm ? sp[p] : (uint8_t)0;
             }
-            // single-end: N invalidates (ReadQueue.cpp:456-460); paired: N and
-            // unknown letters invalidate the mate (:1016-1027)
-            if (__ballot(nflag)) st = paired ? (uint8_t)(st | 2) : (uint8_t)3;
-        }
-        if (st == 0 && L > lmax) lmax = L;
-        if (lane == 0) {
-            pre[r] = st;
-            c0[255] = (st == 0 && (om & 1)) ? (uint8_t)L : 0;
-            c1[255] = (st == 0 && (om & 2)) ? (uint8_t)L : 0;
+        };
+        fetch(0);
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t r = r0 + i;
+            uint8_t cb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cb[q] = nb[q];
+            if (i + 1 < cnt) fetch(i + 1);
+            const uint32_t L = len_of(i);
+            uint8_t* qrev = rev + off_of(i);
+            uint8_t st = 0;
+            // ori 0 / 1 of mate 2 (reads from nsplit on) pair with ori 1 / 0 of mate 1
+            const uint32_t om = r < nsplit ? ori_mask : (((ori_mask & 1) << 1) | ((ori_mask >> 1) & 1));
+            if (!paired) {
+                // readSize < READLEN - 20 (ReadQueue.cpp:430): the unsigned
+                // subtraction of the reference is kept
+                if ((uint64_t)L < (uint64_t)(uint32_t)(readlen - 20u)) st = 3;
+                if (L < KMERLEN) st = 3;
+            } else if (L < minlen) {
+                st = 1; // readSize < ceil(0.75 READLEN) (ReadQueue.cpp:971-980); minlen >= KMERLEN
+            }
+            if (L > FAME_DEV_MAX_READ) {
+                if (lane == 0) atomicOr(err_flag, 1u);
+                st = 3;
+            }
+            uint8_t* c0 = codes + (size_t)(2 * r) * 256;
+            uint8_t* c1 = c0 + 256;
+            if (st == 0 || (paired && st == 1)) {
+                bool nflag = false;
+                // unknown characters stay '\0' in revSeq (branch-free per letter)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t p = (uint32_t)lane + 64u * q;
+                    const uint32_t t = s_lt[cb[q]];
+                    const bool in = p < L;
+                    nflag = nflag || (in && (((t >> 12) & 1u) || (paired && !((t >> 13) & 1u))));
+                    if (in) {
+                        qrev[L - 1 - p] = (uint8_t)t;
+                        if (om & 1) c0[p] = (uint8_t)((t >> 8) & 3u);
+                        if (om & 2) c1[L - 1 - p] = (uint8_t)((t >> 10) & 3u);
+                    }
+                }
+                // single-end: N invalidates (ReadQueue.cpp:456-460); paired: N and
+                // unknown letters invalidate the mate (:1016-1027)
+                if (__ballot(nflag)) st = paired ? (uint8_t)(st | 2) : (uint8_t)3;
+            }
+            if (st == 0 && L > lmax) lmax = L;
+            if (lane == 0) {
+                pre[r] = st;
+                c0[255] = (st == 0 && (om & 1)) ? (uint8_t)L : 0;
+                c1[255] = (st == 0 && (om & 2)) ? (uint8_t)L : 0;
+            }
         }
     }
     if (lane == 0 && lmax) atomicMax(max_len, lmax);
@@ -1988,7 +2034,7 @@ __global__ void k_meth_u16(const uint32_t* __restrict__ in, uint64_t n, uint16_t
 void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t st)
 {
     if (!n) return;
-    const uint32_t blocks = (n + 3) / 4; // 4 waves of 256 threads per block, one read per wave
+    const uint32_t blocks = (n + 255) / 256; // 4 waves per block, 64 reads per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
                        0xFFFFFFFFu, 0u, 0, bd.n_dev);
@@ -1999,7 +2045,7 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
 void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, uint32_t minlen, hipStream_t st)
 {
     if (!n_reads) return;
-    const uint32_t blocks = (n_reads + 3) / 4;
+    const uint32_t blocks = (n_reads + 255) / 256;
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n_reads, 0u, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
                        nsplit, minlen, 1, (const uint32_t*)nullptr);
@@ -2118,17 +2164,27 @@ void launch_batch_reset(const BatchDev& bd, hipStream_t st)
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, bd);
 }
 
+// furthest read end, one atomic per wave
 __global__ void k_max_end(const uint64_t* __restrict__ offs, const uint32_t* __restrict__ lens, uint32_t n,
                           unsigned long long* out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicMax(out, (unsigned long long)(offs[i] + lens[i]));
+    unsigned long long v = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long e = offs[i] + lens[i];
+        v = e > v ? e : v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
 }
 
 void launch_max_end(const uint64_t* offs, const uint32_t* lens, uint32_t n, unsigned long long* out, hipStream_t st)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_max_end, dim3((n + 255) / 256), dim3(256), 0, st, offs, lens, n, out);
+    const uint32_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_max_end, dim3(g < 1024 ? g : 1024), dim3(256), 0, st, offs, lens, n, out);
 }
 
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
