@@ -23,7 +23,7 @@ EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fam
            "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
            "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_set_form", "fame_set_pipeline", "fame_form_info", "fame_serve_open", "fame_serve_submit",
            "fame_serve_wait", "fame_serve_stats", "fame_serve_close", "fame_fastq_open", "fame_fastq_next",
-           "fame_fastq_close",
+           "fame_fastq_close", "fame_sc_open", "fame_sc_write_cell", "fame_sc_close",
            "fame_print_methylation_levels", "fame_last_timings", "fame_last_stats")
 
 _u8p = C.POINTER(C.c_uint8)
@@ -74,6 +74,9 @@ def load():
         L.fame_fastq_next.argtypes = [C.c_void_p, C.c_uint32, _u32p, C.POINTER(_u8p), C.POINTER(_u64p),
                                       C.POINTER(_u32p)]
         L.fame_fastq_close.argtypes = [C.c_void_p]
+        L.fame_sc_open.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.fame_sc_write_cell.argtypes = [C.c_void_p, C.c_char_p]
+        L.fame_sc_close.argtypes = [C.c_void_p]
         L.fame_get_meth.argtypes = [C.c_void_p, _u16p, C.c_uint64]
         L.fame_reset.argtypes = [C.c_void_p]
         L.fame_print_methylation_levels.argtypes = [C.c_void_p, C.c_char_p]
@@ -309,3 +312,22 @@ class FastqReader:
             self.close()
         except Exception:
             pass
+
+
+class ScMatrix:
+    """Single-cell count matrix (fame_sc_*) written from a Context's counts."""
+
+    def __init__(self, ctx, path):
+        h = C.c_void_p()
+        ctx._check(load().fame_sc_open(ctx.h, path.encode(), C.byref(h)))
+        self.ctx, self.h = ctx, h.value
+
+    def write_cell(self, sc_id):
+        self.ctx._check(load().fame_sc_write_cell(self.h, sc_id.encode()))
+
+    def close(self):
+        if self.h:
+            rc = load().fame_sc_close(self.h)
+            self.h = None
+            if rc != FAME_OK:
+                raise FameError(f"{ERRORS.get(rc, rc)}: single-cell output")

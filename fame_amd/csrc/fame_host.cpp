@@ -1772,6 +1772,69 @@ int fame_print_methylation_levels(fame_ctx* c, const char* basename)
     return FAME_OK;
 }
 
+// single-cell count matrix (ReadQueue's single-cell constructor
+// ReadQueue.cpp:181-216 and printSCMethylationLevels :1894-1921)
+struct fame_sc {
+    fame_ctx* ctx;
+    FILE* f;
+    std::vector<char> buf;
+};
+
+int fame_sc_open(fame_ctx* c, const char* path, fame_sc** out)
+{
+    if (!c || !path || !out) return FAME_ERR_ARG;
+    *out = nullptr;
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        set_err(c, std::string("cannot write ") + path);
+        return FAME_ERR_IO;
+    }
+    fame_sc* sc = new fame_sc{c, f, std::vector<char>(1 << 20)};
+    setvbuf(f, sc->buf.data(), _IOFBF, sc->buf.size());
+    // two header rows: the chromosome and the position of every CpG
+    fputs("\nSC_ID\tCount_Type\t", f);
+    for (uint64_t i = 0; i < c->ncpg; ++i) {
+        fputs(c->chr_name[c->cpg_chrom[i]].c_str(), f);
+        fputc('\t', f);
+    }
+    fputs("\nSC_ID\tCount_Type\t", f);
+    for (uint64_t i = 0; i < c->ncpg; ++i) fprintf(f, "%u\t", (unsigned)(c->cpg_pos[i] + c->readlen - 2));
+    *out = sc;
+    return FAME_OK;
+}
+
+int fame_sc_write_cell(fame_sc* sc, const char* sc_id)
+{
+    if (!sc || !sc_id) return FAME_ERR_ARG;
+    fame_ctx* c = sc->ctx;
+    std::vector<uint16_t> m(4 * c->ncpg + 4);
+    int rc = fame_get_meth(c, m.data(), c->ncpg);
+    if (rc != FAME_OK) return rc;
+    // the fourth row holds unmethRev under the label the reference prints
+    static const char* const label[4] = {"methFwd", "unmethFwd", "methRev", "unmethFwd"};
+    for (int k = 0; k < 4; ++k) {
+        fprintf(sc->f, "\n%s\t%s\t", sc_id, label[k]);
+        for (uint64_t i = 0; i < c->ncpg; ++i) fprintf(sc->f, "%u\t", (unsigned)m[4 * i + k]);
+    }
+    if (ferror(sc->f)) {
+        set_err(c, "single-cell output write failed");
+        return FAME_ERR_IO;
+    }
+    // the counts start again from zero for the next cell
+    return guarded(c, [&]() {
+        HIPC(hipMemset(c->d_meth.p, 0, 16 * c->ncpg + 16));
+        return (int)FAME_OK;
+    });
+}
+
+int fame_sc_close(fame_sc* sc)
+{
+    if (!sc) return FAME_OK;
+    const int bad = fclose(sc->f) != 0;
+    delete sc;
+    return bad ? FAME_ERR_IO : FAME_OK;
+}
+
 int fame_last_stats(const fame_ctx* c, uint64_t out[FAME_STATS_FIELDS])
 {
     if (!c || !out) return FAME_ERR_ARG;

@@ -57,6 +57,49 @@ class ReadQueue:
         """Places an in-memory chunk in the read buffer (instead of parsing)."""
         self.buf = (seqs, offs, lens)
 
+    @classmethod
+    def single_cell(cls, sc_output_path, ref, is_gz=False, bs_flag=False, is_paired=False, chunk_size=CHUNKSIZE,
+                    minpd=50, maxpd=550):
+        """The single-cell constructor (ReadQueue.cpp:181-216): no read file
+        yet, the count matrix at sc_output_path with its header rows."""
+        rq = cls(None, ref, is_gz, bs_flag, chunk_size, None, minpd, maxpd)
+        rq.is_paired = is_paired
+        if is_paired:
+            rq.ctx.set_pair_dist(minpd, maxpd)
+        rq.sc = _lib.ScMatrix(rq.ctx, sc_output_path)
+        return rq
+
+    # matchSCBatch ReadQueue.cpp:1660-1731
+    def matchSCBatch(self, scFile, scId, isGZ):
+        """One cell: strand tallies and counts from zero, the chunk loop of
+        queryRoutine over scFile, then the cell's rows of the matrix.
+        Returns [succ, nonunique, unsucc]."""
+        self.ctx.reset()
+        self._f = _lib.FastqReader(scFile, isGZ)
+        try:
+            counters = query_routine(self, self.bs_flag)
+        finally:
+            self._f.close()
+            self._f = None
+        self.sc.write_cell(scId)
+        return counters
+
+    # matchSCBatchPaired ReadQueue.cpp:1732-1854
+    def matchSCBatchPaired(self, scFile1, scFile2, scId, isGZ):
+        """The same for one cell's paired files; returns the five paired
+        counters."""
+        self.ctx.reset()
+        self._f = _lib.FastqReader(scFile1, isGZ)
+        self._f2 = _lib.FastqReader(scFile2, isGZ)
+        try:
+            counters = query_routine_paired(self, self.bs_flag)
+        finally:
+            self._f.close()
+            self._f2.close()
+            self._f = self._f2 = None
+        self.sc.write_cell(scId)
+        return counters
+
     # matchReads ReadQueue.cpp:404-933
     def matchReads(self, procReads, counters, getStranded):
         seqs, offs, lens = self.buf
@@ -93,6 +136,9 @@ class ReadQueue:
         for f in (self._f, self._f2):
             if f:
                 f.close()
+        if getattr(self, "sc", None):
+            self.sc.close()
+            self.sc = None
         self.ctx.close()
 
 
@@ -139,3 +185,26 @@ def query_routine_paired(rque, both_strands, on_chunk=None):
     if on_chunk:
         on_chunk(rque.results)
     return counters
+
+
+def _sc_cells(sc_meta_file, nfiles):
+    """Lines "<id> <file> [<file2>]" of the cell list, split at single spaces
+    as main.cpp:451-457 and :481-491 split them."""
+    with open(sc_meta_file) as f:
+        for line in f.read().split("\n"):
+            parts = line.split(" ")
+            if len(parts) >= 1 + nfiles and parts[0]:
+                yield parts[0], parts[1:1 + nfiles]
+
+
+def query_routine_sc(rque, is_gz, sc_meta_file):
+    """queryRoutineSC main.cpp:443-468: matchSCBatch per listed cell (file
+    paths as listed, relative ones from the working directory)."""
+    for sc_id, (path,) in _sc_cells(sc_meta_file, 1):
+        rque.matchSCBatch(path, sc_id, is_gz)
+
+
+def query_routine_sc_paired(rque, is_gz, sc_meta_file):
+    """queryRoutineSCPaired main.cpp:469-499: matchSCBatchPaired per cell."""
+    for sc_id, (p1, p2) in _sc_cells(sc_meta_file, 2):
+        rque.matchSCBatchPaired(p1, p2, sc_id, is_gz)

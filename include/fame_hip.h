@@ -121,6 +121,20 @@ int fame_reset(fame_ctx* ctx);
  * <basename>_cpg.tsv, byte-identical to the reference's report */
 int fame_print_methylation_levels(fame_ctx* ctx, const char* basename);
 
+/* ---- single cell ----------------------------------------------------------
+ * The count matrix of FAME's single-cell driver: fame_sc_open writes the
+ * header of the single-cell ReadQueue (ReadQueue.cpp:181-216: per CpG its
+ * chromosome, then its position + READLEN - 2); fame_sc_write_cell is
+ * printSCMethylationLevels (:1894-1921): four rows for the cell from the
+ * context's counts (the fourth, unmethRev, carries the label "unmethFwd" as
+ * in the reference), after which the counts are zeroed.  A cell's reads are
+ * matched on the same context between fame_reset and fame_sc_write_cell
+ * (matchSCBatch / matchSCBatchPaired :1660-1854). */
+typedef struct fame_sc fame_sc;
+int fame_sc_open(fame_ctx* ctx, const char* path, fame_sc** out);
+int fame_sc_write_cell(fame_sc* sc, const char* sc_id);
+int fame_sc_close(fame_sc* sc);
+
 /* ---- batched form (no counterpart in the reference) ----------------------
  * FAME_FORM_PLAIN (default): one kernel per stage, the host reads the job
  * counts between stages to size the next launches.  FAME_FORM_BATCHED: the

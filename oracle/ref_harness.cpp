@@ -23,6 +23,11 @@
 //       reports the end; writes <out>: "FAMEFQ01", u64 n, u32 len[n], then the
 //       n sequences back to back, and per parse call (u32 procReads, u8 more)
 //       after them, preceded by a u64 count of calls
+//   sc <index> <meta> <out> [--unord_reads] [--gz] [--pe]
+//       the single-cell driver queryRoutineSC / queryRoutineSCPaired
+//       (main.cpp:443-499) on FAME's single-cell ReadQueue (ReadQueue.cpp:
+//       181-216): per line "<id> <fastq> [<fastq2>]" of <meta>,
+//       matchSCBatch / matchSCBatchPaired; <out> is the count matrix
 //   units ntps64|shiftand|dp|hopscotch <in.bin> <out.bin>
 //       runs one of FAME's building blocks on the cases of <in.bin> (formats
 //       in do_units below): ntHash::NTPS64 direct and rolling
@@ -285,6 +290,38 @@ static int do_parse(int argc, char** argv)
     return 0;
 }
 
+static int do_sc(int argc, char** argv)
+{
+    if (argc < 5) return 2;
+    bool both = false, gz = false, pe = false;
+    for (int i = 5; i < argc; ++i) {
+        if (!strcmp(argv[i], "--unord_reads")) both = true;
+        if (!strcmp(argv[i], "--gz")) gz = true;
+        if (!strcmp(argv[i], "--pe")) pe = true;
+    }
+    RefGenome ref{std::string(argv[2])};
+    ReadQueue q(argv[4], ref, gz, both, pe);
+    std::ifstream meta(argv[3]);
+    std::string line;
+    uint64_t cells = 0;
+    while (std::getline(meta, line)) {
+        // fields separated by single spaces, as main.cpp:451-457 splits them
+        std::vector<std::string> f;
+        size_t old = 0, pos;
+        while ((pos = line.find_first_of(' ', old)) != std::string::npos) {
+            f.push_back(line.substr(old, pos - old));
+            old = pos + 1;
+        }
+        f.push_back(line.substr(old));
+        if (f.size() < (pe ? 3u : 2u)) continue;
+        if (pe) q.matchSCBatchPaired(f[1].c_str(), f[2].c_str(), f[0], gz);
+        else q.matchSCBatch(f[1].c_str(), f[0], gz);
+        ++cells;
+    }
+    std::cerr << "HARNESS cells=" << cells << "\n";
+    return 0;
+}
+
 // ---- units ---------------------------------------------------------------
 // Input and output are little-endian binary streams; every case is
 // self-describing.
@@ -479,6 +516,7 @@ int main(int argc, char** argv)
     if (mode == "match_pe") return do_match_pe(argc, argv);
     if (mode == "parse") return do_parse(argc, argv);
     if (mode == "units") return do_units(argc, argv);
+    if (mode == "sc") return do_sc(argc, argv);
     std::cerr << "unknown mode\n";
     return 2;
 }

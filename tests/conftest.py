@@ -92,3 +92,38 @@ class PairCase:
 @pytest.fixture(params=PAIR_CASES)
 def golden_pair(request, tmp_path):
     return PairCase(request.param, str(tmp_path))
+
+
+SC_CASES = _cases("sc")
+
+
+class ScCase:
+    """A single-cell fixture: per-cell FASTQs, the cell list, FAME's matrix."""
+
+    def __init__(self, name, tmpdir):
+        self.dir = os.path.join(GOLDEN, name)
+        self.name = name
+        self.meta = json.load(open(os.path.join(self.dir, "case.json")))
+        self.index = _unpack_index(self.meta, tmpdir)
+        self.expected = gzip.open(os.path.join(self.dir, "sc_out.txt.gz")).read()
+        self.cells = []  # (id, [fastq paths])
+        for line in open(os.path.join(self.dir, "cells.txt")).read().split("\n"):
+            parts = line.split(" ")
+            if len(parts) > 1:
+                self.cells.append((parts[0], [os.path.join(self.dir, p) for p in parts[1:]]))
+
+
+@pytest.fixture(params=SC_CASES)
+def golden_sc(request, tmp_path):
+    return ScCase(request.param, str(tmp_path))
+
+
+def read_fastq_gz(path):
+    """(seqs, offs, lens) of a gzipped FASTQ, every fourth line from the second."""
+    lines = gzip.open(path).read().split(b"\n")
+    seqs = [lines[i] for i in range(1, len(lines), 4)]
+    lens = np.array([len(x) for x in seqs], np.uint32)
+    offs = np.zeros(len(seqs), np.uint64)
+    if len(seqs) > 1:
+        offs[1:] = np.cumsum(lens[:-1].astype(np.uint64))
+    return np.frombuffer(b"".join(seqs), np.uint8), offs, lens

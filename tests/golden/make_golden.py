@@ -15,6 +15,10 @@ Paired-end cases (pe_*) hold reads.npz / golden.npz with per-mate arrays
 (seqs1/offs1/lens1, seqs2/..., valid1/valid2/mat1/mat2) and the five paired
 counters in case.json.
 
+Single-cell cases (sc_*) hold per-cell FASTQs cut from another fixture's
+reads, the cells.txt list FAME's single-cell driver reads, and the count
+matrix it wrote (sc_out.txt.gz).
+
 The fastq/ directory holds FASTQ inputs with unusual layouts (CRLF, no
 final newline, truncated records, blank lines, gzip, chunk-size multiples)
 and, per input, what FAME's parseChunk / parseChunkGZ (CHUNKSIZE 700) read
@@ -376,6 +380,61 @@ def run_pair_case(name):
     print(name, meta["reference"])
 
 
+SC_CASES = {
+    # single-cell runs: cells cut from an existing fixture's reads, matched
+    # against its index by FAME's single-cell driver
+    "sc_se": dict(reads="se_ga_chunked", cells=[("cell_A", 0, 1000), ("cell_B", 1000, 1900),
+                                                ("cell_C", 1900, 3000)], pe=False, unord=False),
+    "sc_pe": dict(reads="pe_basic", cells=[("p1", 0, 1000), ("p2", 1000, 2200), ("p3", 2200, 3000)], pe=True,
+                  unord=False),
+}
+
+
+def run_sc_case(name):
+    sc = SC_CASES[name]
+    src = json.load(open(os.path.join(HERE, sc["reads"], "case.json")))
+    idx_dir = src["index_dir"]
+    if sc["pe"]:
+        gcfg = dict(CASES[src["share"]])
+        gcfg["chunk"] = src["chunk"]
+        binp = ref_binary(gcfg, src["minpd"], src["maxpd"])
+    else:
+        binp = ref_binary(CASES[sc["reads"]])
+    out = os.path.join(HERE, name)
+    os.makedirs(out, exist_ok=True)
+    r = np.load(os.path.join(HERE, sc["reads"], "reads.npz"))
+    mates = (("seqs1", "offs1", "lens1"), ("seqs2", "offs2", "lens2")) if sc["pe"] else (("seqs", "offs", "lens"),)
+    with tempfile.TemporaryDirectory() as td:
+        for f in ("index.bin", "index.bin_strands"):
+            with gzip.open(os.path.join(HERE, idx_dir, f + ".gz"), "rb") as a, open(os.path.join(td, f), "wb") as b:
+                shutil.copyfileobj(a, b)
+        lines = []
+        for cid, c0, c1 in sc["cells"]:
+            paths = []
+            for k, (sn, on, ln) in enumerate(mates):
+                flat, offs, lens = r[sn].tobytes(), r[on], r[ln]
+                seqs = [flat[int(o):int(o) + int(n)] for o, n in zip(offs[c0:c1], lens[c0:c1])]
+                fn = f"{cid}_{k + 1}.fq.gz"
+                write_fastq(os.path.join(td, "tmp.fq"), seqs)
+                gz(os.path.join(td, "tmp.fq"), os.path.join(out, fn))
+                shutil.copy(os.path.join(out, fn), os.path.join(td, fn))
+                paths.append(fn)
+            lines.append(" ".join([cid] + paths))
+        with open(os.path.join(out, "cells.txt"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        shutil.copy(os.path.join(out, "cells.txt"), os.path.join(td, "cells.txt"))
+        args = [binp, "sc", "index.bin", "cells.txt", "sc_out.txt", "--gz"]
+        args += (["--pe"] if sc["pe"] else []) + (["--unord_reads"] if sc["unord"] else [])
+        subprocess.run(args, cwd=td, check=True, stdout=subprocess.DEVNULL)
+        gz(os.path.join(td, "sc_out.txt"), os.path.join(out, "sc_out.txt.gz"))
+    meta = {"mode": "sc", "pe": sc["pe"], "unord": sc["unord"], "index_dir": idx_dir, "reads": sc["reads"],
+            "chunk": src["chunk"], "cells": [c[0] for c in sc["cells"]]}
+    if sc["pe"]:
+        meta["minpd"], meta["maxpd"] = src["minpd"], src["maxpd"]
+    json.dump(meta, open(os.path.join(out, "case.json"), "w"), indent=1)
+    print(name, meta)
+
+
 def _fastq_inputs():
     """name -> (bytes, gz): FASTQ layouts the parser must read as FAME does."""
     r = np.load(os.path.join(HERE, "se_basic", "reads.npz"))
@@ -443,9 +502,11 @@ def run_fastq_cases():
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + list(PAIRED) + ["fastq"]
+    names = sys.argv[1:] or list(CASES) + list(PAIRED) + ["fastq"] + list(SC_CASES)
     for nm in names:
         if nm == "fastq":
             run_fastq_cases()
+        elif nm in SC_CASES:
+            run_sc_case(nm)
         else:
             run_pair_case(nm) if nm in PAIRED else run_case(nm)
