@@ -323,12 +323,15 @@ def main():
             # over the full-size calls (one launch of each kernel per call)
             full = [c for c in calls if c[0] == args.batch] or calls
             dev_stages = {k: per_stage[k] for k in ("prep", "seed", "verify", "select", "meth")}
-            roof = roofline("seed", full)
+            # the counter summaries under profiles/ were taken on the default
+            # workload's 2 M-read launches
+            same = args.workload == "genome_se" and args.batch == 2_000_000 and args.genome_scale == 1.0
+            roof = roofline("seed", full, same)
             roof["dominant_stage"] = max(dev_stages, key=dev_stages.get)
             verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
             out["verify_window_GBps"] = round(verify_rate / 1e9, 2)
             out["roofline"] = roof
-            out["roofline_verify"] = roofline("verify", full)
+            out["roofline_verify"] = roofline("verify", full, same)
         # ---------------- CPU baseline + parity on a sample ----------------
         if not args.no_cpu:
             if paired:
@@ -345,12 +348,14 @@ def main():
 
 def algorithmic_bytes(stage, st):
     """Bytes the stage's algorithm must move for one call, from the kernels'
-    own work counters: seed = two tabIndex reads per k-mer, one index entry
-    per bucket hit, the read's letters and the candidate records written;
-    verify = the window text of every candidate (WINLEN + MISCOUNT + ADDMIS - 1
-    letters, as FAME's ShiftAnd scans it)."""
+    own work counters: seed = the bucket lookup of every k-mer (one 8-byte
+    slot, which holds buckets of one or two entries), the index entries read
+    from the k-mer table (buckets of three or more), the read's letters and
+    the candidate records written; verify = the window text of every
+    candidate (WINLEN + MISCOUNT + ADDMIS - 1 letters, as FAME's ShiftAnd
+    scans it)."""
     if stage == "seed":
-        return 2 * st["tab_bytes"] * st["kmers"] + 4 * st["entries"] + st["read_bytes"] + 8 * st["jobs"]
+        return st["tab_bytes"] * st["kmers"] + 4 * st["entries"] + st["read_bytes"] + 8 * st["jobs"]
     return st["window_bytes"]
 
 
@@ -370,12 +375,14 @@ def measured_traffic(stage):
     return None
 
 
-def roofline(stage, calls):
+def roofline(stage, calls, traffic_known=True):
+    """traffic_known: the committed counter summary was taken on this
+    workload (the default one); otherwise traffic is null."""
     n = len(calls)
     ms = sum(c[1][stage] for c in calls) / n
     nbytes = sum(algorithmic_bytes(stage, c[2]) for c in calls) / n
     gbs = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-    traffic = measured_traffic(stage)
+    traffic = measured_traffic(stage) if traffic_known else None
     return {"kernel": TRAFFIC[stage][0], "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_GBps": round(traffic / (ms / 1e3) / 1e9, 1) if traffic and ms > 0 else None,

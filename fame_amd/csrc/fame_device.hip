@@ -280,6 +280,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     // exclusive prefix over k-mers in index order
     uint64_t carry = 0;
     bool big = false;
+    uint32_t gathered = 0; // entries read from the k-mer table (slot mode: buckets of three or more)
     for (uint32_t k0 = 0; k0 < K; k0 += 64) {
         const uint32_t k = k0 + lane;
         uint32_t v = k < K ? w.klen[k] : 0;
@@ -287,6 +288,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
             v = 0xFFFFFFu;
             big = true;
         }
+        gathered += (!inl || v >= 3) ? v : 0u;
         const uint32_t inc = wave_iscan_u32(v, lane);
         const uint32_t tot = __shfl(inc, 63, 64);
         if (k < K) w.koff[k] = (uint32_t)(carry + inc - v);
@@ -299,7 +301,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     *bc_out = bucketCount;
     if (big || E > w.cap_e) return 1;
     ss.kmers += K;
-    ss.entries += E;
+    ss.entries += wave_sum_u32(gathered);
     ss.seeded += 1;
     ss.bytes += L;
     wave_sync();

@@ -1840,7 +1840,7 @@ int fame_last_stats(const fame_ctx* c, uint64_t out[FAME_STATS_FIELDS])
     if (!c || !out) return FAME_ERR_ARG;
     static_assert(FAME_STATS_FIELDS == FAME_NSTATS, "stats width");
     memcpy(out, c->stats, sizeof c->stats);
-    out[7] = c->ix.tab_is64 ? 8 : 4;
+    out[7] = c->ix.slot ? 8 : (c->ix.tab_is64 ? 16 : 8); // bytes read per k-mer lookup
     return FAME_OK;
 }
 

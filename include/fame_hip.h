@@ -214,9 +214,10 @@ int fame_fastq_close(fame_fastq* fq);
 int fame_last_timings(const fame_ctx* ctx, double out[6]);
 
 /* work done by the last fame_match_reads* call, for roofline accounting:
- * k-mers hashed, index bucket entries gathered, (read, orientation) pairs
+ * k-mers hashed, index entries read from the k-mer table (with the slot
+ * table: those of buckets of three or more), (read, orientation) pairs
  * seeded, window verifications, window bytes scanned, banded alignments,
- * read bytes seeded, and the byte width of one tabIndex entry on the device.
+ * read bytes seeded, and the bytes read per k-mer bucket lookup.
  * No counterpart in the reference (instrumentation). */
 #define FAME_STATS_FIELDS 8
 int fame_last_stats(const fame_ctx* ctx, uint64_t out[FAME_STATS_FIELDS]);
