@@ -63,20 +63,45 @@ __device__ __forceinline__ uint64_t red_seed(uint32_t code)
                      : 0ULL;
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+// inclusive scan over the 64 lanes of a wave with DPP moves: shifts by 1,
+// 2, 4, 8 inside each row of 16 lanes, then the row totals are broadcast
+// into the rows above (row_bcast:15 into rows 1 and 3, row_bcast:31 into
+// rows 2 and 3).  Lanes without a source take 0.
+__device__ __forceinline__ uint32_t wave_iscan_u32(uint32_t v, int lane)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    (void)lane;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false); // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false); // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false); // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false); // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
     return v;
 }
 
-// inclusive scan over the 64 lanes of a wave
-__device__ __forceinline__ uint32_t wave_iscan_u32(uint32_t v, int lane)
+// inclusive max-scan over the 64 lanes of a wave (values >= -1; lanes
+// without a source take -1), the same DPP pattern as wave_iscan_u32
+__device__ __forceinline__ int wave_imaxscan_i32(int v)
 {
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xC, 0xF, false));
     return v;
+}
+
+// each lane receives the value of the lane below it (lane 0: -1), wave_shr:1
+__device__ __forceinline__ int wave_shr1_i32(int v) { return __builtin_amdgcn_update_dpp(-1, v, 0x138, 0xF, 0xF, false); }
+
+// lane i's value of a 32-bit register, to every lane (a scalar read)
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); }
+
+// sum over the 64 lanes of a wave (every lane receives it)
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_iscan_u32(v, 0), 63);
 }
 
 __device__ __forceinline__ uint64_t rup_pow2(uint64_t v)
@@ -352,7 +377,7 @@ __device__ inline uint32_t pool_take(const BatchDev& bd, PoolChunk& pc, uint32_t
     if (exact) {
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(&bd.pool_n[p], need);
-        b = __shfl(b, 0, 64);
+        b = lane_u32(b, 0);
         return b + need <= bd.pool_cap[p] ? b : ~0u;
     }
     if (pc.pool != p || pc.left < need) {
@@ -360,7 +385,7 @@ __device__ inline uint32_t pool_take(const BatchDev& bd, PoolChunk& pc, uint32_t
         const uint32_t sz = need > POOL_CHUNK ? need : POOL_CHUNK;
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(&bd.pool_n[p], sz);
-        b = __shfl(b, 0, 64);
+        b = lane_u32(b, 0);
         pc.pool = p;
         pc.base = b;
         pc.left = sz;

@@ -227,7 +227,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     const int lane = threadIdx.x & 63;
     // cw: this lane's word of the row of reduced letters (the length is in
     // byte 255, the top byte of lane 63's word)
-    const uint32_t L = (uint32_t)__shfl(cw, 63, 64) >> 24;
+    const uint32_t L = lane_u32(cw, 63) >> 24;
     if (!staged) {
         ((uint32_t*)w.code)[lane] = cw;
         wave_sync();
@@ -290,7 +290,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         }
         gathered += (!inl || v >= 3) ? v : 0u;
         const uint32_t inc = wave_iscan_u32(v, lane);
-        const uint32_t tot = __shfl(inc, 63, 64);
+        const uint32_t tot = lane_u32(inc, 63);
         if (k < K) w.koff[k] = (uint32_t)(carry + inc - v);
         carry += tot;
         if (carry > 0x7FFFFFFFull) big = true;
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         };
         uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv;
         uint32_t cwc = row(ro), cwn = row(ro + nw);
-        uint32_t Lc = ro < n_ro ? ((uint32_t)__shfl(cwc, 63, 64) >> 24) : 0;
+        uint32_t Lc = ro < n_ro ? (lane_u32(cwc, 63) >> 24) : 0;
         uint64_t slc[C::KPL], sln[C::KPL];
 #pragma unroll
         for (int q = 0; q < C::KPL; ++q) slc[q] = sln[q] = 0;
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         for (; ro < n_ro; ro += nw) {
             const uint32_t rn = ro + nw;
             const uint32_t cwnn = row(rn + nw);
-            const uint32_t Ln = rn < n_ro ? ((uint32_t)__shfl(cwn, 63, 64) >> 24) : 0;
+            const uint32_t Ln = rn < n_ro ? (lane_u32(cwn, 63) >> 24) : 0;
             if (Ln) {
                 wave_sync(); // the previous pair's hashing has read w.code
                 ((uint32_t*)w.code)[lane] = cwn;
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     }
     for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; !ix.slot && ro < n_ro; ro += nw) {
         const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
-        if ((__shfl(cw, 63, 64) >> 24) == 0) { // not seeded: invalid read or orientation not asked for
+        if ((lane_u32(cw, 63) >> 24) == 0) { // not seeded: invalid read or orientation not asked for
             if (lane == 0) {
                 bd.cand_nf[ro] = 0;
                 bd.cand_nr[ro] = 0;
@@ -1480,8 +1480,8 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
             if (bdir) basedir = atomicAdd(bd.direct_n, (uint32_t)__popcll(bdir));
             if (bdp) basedp = atomicAdd(bd.dp_n, (uint32_t)__popcll(bdp));
         }
-        basedir = __shfl(basedir, 0, 64);
-        basedp = __shfl(basedp, 0, 64);
+        basedir = lane_u32(basedir, 0);
+        basedp = lane_u32(basedp, 0);
         if (want == 1) bd.direct_list[basedir + __popcll(bdir & below)] = r;
         if (want == 2) bd.dp_list[basedp + __popcll(bdp & below)] = r;
     }

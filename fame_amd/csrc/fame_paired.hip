@@ -168,7 +168,7 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
 {
     const int lane = threadIdx.x & 63;
     const uint32_t cwd = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
-    const uint32_t L = (uint32_t)__shfl(cwd, 63, 64) >> 24;
+    const uint32_t L = lane_u32(cwd, 63) >> 24;
     L_out = L;
     ((uint32_t*)m.code)[lane] = cwd;
     if (lane < 9) m.cbits[lane] = 0;
@@ -242,7 +242,7 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
             big = true;
         }
         const uint32_t inc = wave_iscan_u32(v, lane);
-        const uint32_t tot = __shfl(inc, 63, 64);
+        const uint32_t tot = lane_u32(inc, 63);
         if (k < K) w.koff[k] = (uint32_t)(carry + inc - v);
         carry += tot;
         if (carry > 0x7FFFFFFFull) big = true;
@@ -276,15 +276,10 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
     for (uint32_t j0 = 0; j0 < E; j0 += 64) {
         const uint32_t j = j0 + lane;
         const bool un = j < E && (w.eaux[j] & 1);
-        int inc = un ? (int)j : -1;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(inc, o, 64);
-            if (lane >= o) inc = max(inc, t);
-        }
-        int ex = __shfl_up(inc, 1, 64);
-        if (lane == 0) ex = -1;
+        const int inc = wave_imaxscan_i32(un ? (int)j : -1);
+        int ex = wave_shr1_i32(inc);
         ex = max(ex, pc);
-        pc = max(pc, __shfl(inc, 63, 64));
+        pc = max(pc, (int)lane_u32((uint32_t)inc, 63));
         if (un && ex >= 0) {
             const uint32_t k = w.ekm[j];
             if ((uint32_t)ex >= w.koff[k] && w.ent[ex] == w.ent[j]) w.eaux[j] = 0;
@@ -622,7 +617,7 @@ __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& p
                     nf = m.nmap[0];
                     nr = m.nmap[1];
                     if (lane == 0) off = atomicAdd(pd.map_n, nf + nr);
-                    off = __shfl(off, 0, 64);
+                    off = lane_u32(off, 0);
                     if (off + nf + nr > pd.map_cap) {
                         if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
                         nf = nr = 0;
@@ -1301,8 +1296,8 @@ __global__ __launch_bounds__(256) void k_pe_final(DevIndex ix, BatchDev b1, Batc
                 if (bdir) basedir = atomicAdd(b1.direct_n, (uint32_t)__popcll(bdir));
                 if (bdp) basedp = atomicAdd(b1.dp_n, (uint32_t)__popcll(bdp));
             }
-            basedir = __shfl(basedir, 0, 64);
-            basedp = __shfl(basedp, 0, 64);
+            basedir = lane_u32(basedir, 0);
+            basedp = lane_u32(basedp, 0);
             if (want == 1) b1.direct_list[basedir + __popcll(bdir & below)] = r;
             if (want == 2) b1.dp_list[basedp + __popcll(bdp & below)] = r;
         }
