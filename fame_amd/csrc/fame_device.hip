@@ -306,22 +306,36 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     ss.bytes += L;
     wave_sync();
     // entry gather, one entry per lane: entry j belongs to the k-mer whose
-    // [koff, koff + klen) contains j (binary search over koff)
-    for (uint32_t j = lane; j < E; j += 64) {
-        uint32_t lo = 0, hi = K; // last k with koff[k] <= j and klen[k] > 0
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (w.koff[mid] <= j) lo = mid;
-            else hi = mid;
+    // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
+    // its index over its entries first; beyond, a binary search over koff.
+    const bool scatter = E <= 256;
+    if (scatter) {
+        for (uint32_t k = lane; k < K; k += 64) {
+            const uint32_t o = w.koff[k], n = w.klen[k];
+            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (uint8_t)k;
         }
-        uint32_t k = lo;
-        while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
+        wave_sync();
+    }
+    for (uint32_t j = lane; j < E; j += 64) {
+        uint32_t k;
+        if (scatter) {
+            k = w.ekm[j];
+        } else {
+            uint32_t lo = 0, hi = K; // last k with koff[k] <= j and klen[k] > 0
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (w.koff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            k = lo;
+            while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
+        }
         const uint32_t d = j - w.koff[k];
         if (inl && w.klen[k] <= 2) // the bucket is in its slot
             w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
         else
             w.ent[j] = ix.kent[(uint64_t)w.klo[k] + d] & 0x7FFFFFFFu; // without isStart
-        w.ekm[j] = (uint8_t)k;
+        if (!scatter) w.ekm[j] = (uint8_t)k;
     }
     wave_sync();
     // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
