@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <chrono>
 #include <cstdio>
@@ -284,8 +285,10 @@ void load_index(fame_ctx* c, const std::string& path)
             HIPC(hipStreamSynchronize(st));
         }
     }
-    // slot table of the single-end seed stage (u32 tabIndex only)
-    if (!c->ix.tab_is64) {
+    // slot table of the single-end seed stage (u32 tabIndex only; the
+    // environment variable FAME_NO_SLOT_TABLE=1 keeps the tabIndex path)
+    const char* no_slot = getenv("FAME_NO_SLOT_TABLE");
+    if (!c->ix.tab_is64 && !(no_slot && no_slot[0] == '1')) {
         HIPC(c->d_slot.reserve(8 * c->htab));
         DevBuf bad;
         HIPC(bad.reserve(4));

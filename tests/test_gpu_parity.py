@@ -61,3 +61,17 @@ def test_hip_matches_oracle_single_batch(golden_case):
     assert np.array_equal(valid, ov) and np.array_equal(mat, om)
     assert np.array_equal(ctx.meth(), m.meth())
     ctx.close()
+
+
+def test_hip_tabindex_path_matches_reference(golden_case, monkeypatch):
+    """The seed stage without the slot table (tabIndex + k-mer table reads,
+    as for indexes whose tabIndex needs 64 bits): the same results."""
+    monkeypatch.setenv("FAME_NO_SLOT_TABLE", "1")
+    c = golden_case
+    print(f"case {c.name}: tabIndex path", flush=True)
+    ctx, valid, mat, cnt, _ = run_hip(c)
+    bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
+    assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()
