@@ -329,6 +329,15 @@ def main():
             # `roofline` is the kernel of the larger of the two dominant stages
             # (seed, verify); both are also given under their own names
             r_seed, r_verify = roofline("seed", full, same), roofline("verify", full, same)
+            # the seed kernel's bucket lookups set against a plain random gather
+            # of as many 8-byte words from a table of the slot table's size
+            lookups = int(sum(c[2]["kmers"] for c in full) / len(full))
+            seed_rate = lookups / (r_seed["avg_launch_ms"] / 1e3)
+            probe_rate = gather_probe(lookups, args.htab)
+            r_seed["lookups_per_launch"] = lookups
+            r_seed["lookups_per_s"] = round(seed_rate, 1)
+            r_seed["gather_probe_lookups_per_s"] = round(probe_rate, 1)
+            r_seed["frac_of_gather_probe"] = round(seed_rate / probe_rate, 3)
             dominant = max(dev_stages, key=dev_stages.get)
             roof = dict(r_verify if dev_stages["verify"] > dev_stages["seed"] else r_seed)
             roof["dominant_stage"] = dominant
@@ -407,6 +416,31 @@ def copy_probe(mib):
         torch.neg(a, out=b)
     torch.cuda.synchronize()
     log(f"copy probe: 3 x {mib} MiB read + {mib} MiB written")
+
+
+def gather_probe(lookups, words, reps=3):
+    """torch.index_select of `lookups` 8-byte words at uniform random positions
+    of a `words`-word device table: the rate (lookups/s, best of `reps` after
+    one warm-up) at which a plain gather kernel reads independent random words
+    of a table that size, for comparison with the seed stage's bucket lookups."""
+    import torch
+    tab = torch.zeros(words, dtype=torch.int64, device="cuda:0")
+    idx = torch.randint(0, words, (lookups,), dtype=torch.int64, device="cuda:0")
+    out = torch.empty(lookups, dtype=torch.int64, device="cuda:0")
+    best = None
+    for i in range(reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.index_select(tab, 0, idx, out=out)
+        e1.record()
+        e1.synchronize()
+        if i:
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+    del tab, idx, out
+    torch.cuda.empty_cache()
+    log(f"gather probe: {lookups} random 8 B reads of a {words}-word table in {best:.3f} ms")
+    return lookups / (best / 1e3)
 
 
 def cpu_leg(args, wl, ctx, reads, idx_path, wd, nchrom):
