@@ -102,7 +102,7 @@ struct fame_ctx {
         b2_job_match[FAME_NPOOL], b2_job_ovf_off[FAME_NPOOL];
     DevBuf b2_ovf_n, b2_ovf_pool, b2_ovf_job, b2_ovf_buf, b2_ovf_total;
     DevBuf b_acc1[FAME_NPOOL], b_acc2[FAME_NPOOL];
-    DevBuf b_pe_off, b_pe_nf, b_pe_nr, b_pe_m1, b_pe_mapn, b_pe_map, b_pe_srt, b_pe_big, b_pe_list;
+    DevBuf b_pe_off, b_pe_nf, b_pe_nr, b_pe_m1, b_pe_mapn, b_pe_map, b_pe_srt, b_pe_big, b_pe_list, b_pe_wlist;
     uint32_t pool2_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t pe_map_cap = 0;
     uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
@@ -1226,8 +1226,12 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     pd.map_nf = grow<uint32_t>(c->b_pe_nf, nitems + 1);
     pd.map_nr = grow<uint32_t>(c->b_pe_nr, nitems + 1);
     pd.m1_n = grow<uint32_t>(c->b_pe_m1, nitems + 1);
-    pd.map_n = grow<uint32_t>(c->b_pe_mapn, 4);
+    pd.map_n = grow<uint32_t>(c->b_pe_mapn, 8);
     pd.big_n = pd.map_n + 1;
+    pd.next = pd.map_n + 2;
+    pd.wlist_n = pd.map_n + 3;
+    pd.wnext = pd.map_n + 4;
+    pd.wlist = grow<uint32_t>(c->b_pe_wlist, nitems + 1);
     c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, std::max<uint32_t>(1u << 16, 8 * n));
     HIPC(c->d_pe_counters.reserve(64));
     pd.counters = c->d_pe_counters.as<uint64_t>();
@@ -1256,7 +1260,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         pd.list = nullptr;
         pd.list_n = 0;
         HIPC(hipMemsetAsync(bd.pool_n, 0, 4 * FAME_NPOOL, st));
-        HIPC(hipMemsetAsync(pd.map_n, 0, 8, st));
+        HIPC(hipMemsetAsync(pd.map_n, 0, 12, st));
         tm.mark(2);
         launch_pe_seed(c->ix, bd, pd, seed_grid, st);
         tm.mark(3);
@@ -1357,6 +1361,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             b2.job_ro[p] = grow<uint32_t>(c->b2_job_ro[p], c->pool2_cap[p]);
         }
         HIPC(hipMemsetAsync(b2.pool_n, 0, 4 * FAME_NPOOL, st));
+        HIPC(hipMemsetAsync(pd.wlist_n, 0, 8, st));
         tm.mark(6);
         launch_pe_walk1(c->ix, bd, b2, pd, st);
         tm.mark(7);

@@ -146,6 +146,12 @@ struct PairDev {
     uint32_t* map_nr;
     uint32_t* m1_n;
     uint32_t* map_n; // pool fill (device counter)
+    uint32_t* next;  // next item to take (device counter, LDS seed kernel)
+    // items of many windows, walked one wave each (device list, its length
+    // and the next entry to take)
+    uint32_t* wlist;
+    uint32_t* wlist_n;
+    uint32_t* wnext;
     uint32_t map_cap;
     PeItem* map;     // [map_cap] maps as built by the seed stage
     PeItem* srt;     // [map_cap] the same slices sorted by a count

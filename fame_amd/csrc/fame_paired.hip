@@ -788,7 +788,20 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
     w.mhsz = PE_LH;
     PoolChunk pcA = {0, 0, 0};
     uint64_t st[4] = {0, 0, 0, 0};
-    for (uint32_t t = blockIdx.x; t < 2 * pd.n; t += gridDim.x) pe_item<TabT>(ix, bd, pd, sh, m, w, t, pcA, st);
+    // items of the queried combinations only, taken from a device counter
+    // (their costs vary by orders of magnitude), the next one's number
+    // fetched while the current one runs
+    const uint32_t both = pd.combos == 3u;
+    const uint32_t nq = both ? 2 * pd.n : (pd.combos ? pd.n : 0u);
+    const uint32_t c1 = pd.combos == 2u ? 1u : 0u;
+    uint32_t nt = 0;
+    if (lane == 0) nt = atomicAdd(pd.next, 1u);
+    uint32_t q = lane_u32(nt, 0);
+    while (q < nq) {
+        if (lane == 0) nt = atomicAdd(pd.next, 1u);
+        pe_item<TabT>(ix, bd, pd, sh, m, w, both ? q : 2 * q + c1, pcA, st);
+        q = lane_u32(nt, 0);
+    }
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
     if (lane == 0 && st[2]) {
         atomicAdd(&bd.stats[0], (unsigned long long)st[0]);
@@ -936,52 +949,11 @@ __device__ void pe_median_to_first(PeItem* a, int result, int x, int y, int z, i
     }
 }
 
-__device__ void pe_sort(PeItem* a, int n, int sec)
+// __final_insertion_sort: the first 16 by a guarded insertion sort, the rest
+// unguarded (a smaller element than a[0] exists among the first 16 after the
+// introsort loop); for n <= 16 this is all std::sort does
+__device__ __forceinline__ void pe_final_insertion(PeItem* a, int n, int sec)
 {
-    if (n <= 1) return;
-    int lg = 0;
-    while ((1 << (lg + 1)) <= n) ++lg;
-    int sf[48], sl[48], sd[48];
-    int sp = 0;
-    sf[0] = 0;
-    sl[0] = n;
-    sd[0] = 2 * lg;
-    sp = 1;
-    while (sp > 0) {
-        --sp;
-        const int first = sf[sp];
-        int last = sl[sp], depth = sd[sp];
-        while (last - first > 16) {
-            if (depth == 0) {
-                pe_heap_sort(a, first, last, sec);
-                break;
-            }
-            --depth;
-            const int mid = first + (last - first) / 2;
-            pe_median_to_first(a, first, first + 1, mid, last - 1, sec);
-            // __unguarded_partition(first + 1, last, first)
-            // (the bounds tests never fire: the median of three stops both
-            // scans, as in libstdc++; they only keep every access in range)
-            int lo = first + 1, hi = last;
-            for (;;) {
-                while (lo < last - 1 && pe_less(a[lo], a[first], sec)) ++lo;
-                --hi;
-                while (hi > first && pe_less(a[first], a[hi], sec)) --hi;
-                if (!(lo < hi)) break;
-                pe_swap(a, lo, hi);
-                ++lo;
-            }
-            const int cut = lo;
-            if (sp < 48) {
-                sf[sp] = cut;
-                sl[sp] = last;
-                sd[sp] = depth;
-                ++sp;
-            }
-            last = cut;
-        }
-    }
-    // __final_insertion_sort
     const int g = n > 16 ? 16 : n;
     for (int i = 1; i < g; ++i) {
         const PeItem val = a[i];
@@ -1006,6 +978,61 @@ __device__ void pe_sort(PeItem* a, int n, int sec)
         }
         a[k] = val;
     }
+}
+
+constexpr int PE_SORT_STACK = 48; // pending right parts (depth limit 2 log2 n < 48)
+
+// `stk`: room for 3 x PE_SORT_STACK ints (LDS or global memory), used for n > 16
+__device__ void pe_sort(PeItem* a, int n, int sec, int* stk)
+{
+    if (n <= 1) return;
+    if (n > 16) {
+        int* sf = stk;
+        int* sl = stk + PE_SORT_STACK;
+        int* sd = stk + 2 * PE_SORT_STACK;
+        int lg = 0;
+        while ((1 << (lg + 1)) <= n) ++lg;
+        sf[0] = 0;
+        sl[0] = n;
+        sd[0] = 2 * lg;
+        int sp = 1;
+        while (sp > 0) {
+            --sp;
+            const int first = sf[sp];
+            int last = sl[sp], depth = sd[sp];
+            while (last - first > 16) {
+                if (depth == 0) {
+                    pe_heap_sort(a, first, last, sec);
+                    break;
+                }
+                --depth;
+                const int mid = first + (last - first) / 2;
+                pe_median_to_first(a, first, first + 1, mid, last - 1, sec);
+                // __unguarded_partition(first + 1, last, first)
+                // (the bounds tests never fire: the median of three stops
+                // both scans, as in libstdc++; they only keep every access in
+                // range)
+                int lo = first + 1, hi = last;
+                for (;;) {
+                    while (lo < last - 1 && pe_less(a[lo], a[first], sec)) ++lo;
+                    --hi;
+                    while (hi > first && pe_less(a[first], a[hi], sec)) --hi;
+                    if (!(lo < hi)) break;
+                    pe_swap(a, lo, hi);
+                    ++lo;
+                }
+                const int cut = lo;
+                if (sp < PE_SORT_STACK) {
+                    sf[sp] = cut;
+                    sl[sp] = last;
+                    sd[sp] = depth;
+                    ++sp;
+                }
+                last = cut;
+            }
+        }
+    }
+    pe_final_insertion(a, n, sec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1034,9 +1061,11 @@ __device__ __forceinline__ uint8_t* acc_of(uint8_t* const (&acc)[FAME_NPOOL], ui
 // window's last one.  Marks each walked slot in acc (1, | 2 when the first
 // match was dropped); `mark` (first read) receives flag3 for the matched
 // windows and their neighbours.  Returns the matches kept.
+// All `step` threads of a group run the walk alike (uniform values); the
+// marking is split among them (`lane` = 0..step-1), lane 0 writes acc.
 __device__ uint32_t pe_walk(const DevIndex& ix, const BatchDev& vb, uint8_t* const (&acc)[FAME_NPOOL],
                             uint32_t pool, const PeItem* list, uint32_t n, int sec, uint32_t flag, int32_t& bm,
-                            PeItem* mark, uint32_t nmark, int cw)
+                            PeItem* mark, uint32_t nmark, int cw, uint32_t lane, uint32_t step)
 {
     uint32_t prevChr = 0;
     uint64_t prevOff = ~0ull;
@@ -1057,12 +1086,12 @@ __device__ uint32_t pe_walk(const DevIndex& ix, const BatchDev& vb, uint8_t* con
             const uint32_t m0 = jm_at(vb, pool, j, 0);
             if ((uint64_t)(m0 >> 8) + w.x == prevOff && w.y == prevChr) skip = 1;
         }
-        ac[j] = (uint8_t)(1u | (skip << 1));
+        if (lane == 0) ac[j] = (uint8_t)(1u | (skip << 1));
         kept += nm - skip;
         if (nm > 0) {
             if (cnt > bm) bm = cnt;
             if (mark) {
-                for (uint32_t y = 0; y < nmark; ++y) {
+                for (uint32_t y = lane; y < nmark; y += step) {
                     const int32_t d = (int32_t)(mark[y].k - it.k);
                     if (d >= -cw && d <= cw) mark[y].v |= PF3;
                 }
@@ -1081,58 +1110,78 @@ __device__ uint32_t pe_walk(const DevIndex& ix, const BatchDev& vb, uint8_t* con
 // first, the reverse map on ties (:2185-2240, :2265-2320)
 __device__ uint32_t pe_query(const DevIndex& ix, const BatchDev& vb, uint8_t* const (&acc)[FAME_NPOOL],
                              uint32_t pool, const PeItem* lf, uint32_t nf, const PeItem* lr, uint32_t nr, int sec,
-                             uint32_t flag, PeItem* mf, PeItem* mr, int cw)
+                             uint32_t flag, PeItem* mf, PeItem* mr, int cw, uint32_t lane = 0, uint32_t step = 1)
 {
     int32_t bm = 0;
     bool fwd_first = nf > 0 && (nr == 0 || pe_c0(lf[0].v) > pe_c0(lr[0].v));
     uint32_t kept = 0;
     for (int pass = 0; pass < 2; ++pass) {
         const bool fw = pass == 0 ? fwd_first : !fwd_first;
-        if (fw) kept += pe_walk(ix, vb, acc, pool, lf, nf, sec, flag, bm, mf, mf ? nf : 0, cw);
-        else kept += pe_walk(ix, vb, acc, pool, lr, nr, sec, flag, bm, mr, mr ? nr : 0, cw);
+        if (fw) kept += pe_walk(ix, vb, acc, pool, lf, nf, sec, flag, bm, mf, mf ? nf : 0, cw, lane, step);
+        else kept += pe_walk(ix, vb, acc, pool, lr, nr, sec, flag, bm, mr, mr ? nr : 0, cw, lane, step);
     }
     return kept;
 }
 
-__global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
+// windows of an item that k_pe_walk1 sorts and walks in LDS (more: in the
+// global map pool); a thread's two copies take PE_WSTRIDE words each, an odd
+// count, so that the threads of a wave fall on distinct banks
+constexpr uint32_t PE_WCAP = 8;
+constexpr uint32_t PE_WSTRIDE = 3 * PE_WCAP + 1;
+static_assert(sizeof(PeItem) == 12, "PeItem is three words");
+static_assert(PE_WCAP <= 16, "k_pe_walk1 sorts by insertion only");
+
+// saQuerySeedSetRefFirst for one item (the maps sorted by c0, both walks with
+// flag2, flag3 marked in `mp`), then saQuerySeedSetRefSecond's order (sorted
+// by c1) in `sp`.  Run by one thread (lane 0, step 1) or by a whole wave
+// (lane, step 64), which shares the copies and the marking and leaves the
+// sorts to lane 0.  Returns PE_NONE (every lane) when the first read matched
+// nothing, else (lane 0) the number of second-read jobs: the windows with
+// flag3 and c1 >= QTHRESH.
+__device__ __forceinline__ uint32_t pe_walk1_order(const DevIndex& ix, const BatchDev& b1, const PairDev& pd,
+                                                   uint32_t t, uint32_t nf, uint32_t nr, PeItem* mp, PeItem* sp,
+                                                   uint32_t lane, uint32_t step, int* stk)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * pd.n) return;
-    const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t];
-    if (nf + nr == 0) {
-        pd.m1_n[t] = 0;
-        return;
+    // stk == nullptr: at most 16 windows per map, insertion sort only
+    auto sort = [&](PeItem* a, uint32_t na, int sec) {
+        if (stk) pe_sort(a, (int)na, sec, stk);
+        else pe_final_insertion(a, (int)na, sec);
+    };
+    const uint32_t i = t >> 1;
+    const uint32_t n = nf + nr;
+    for (uint32_t x = lane; x < n; x += step) sp[x] = mp[x];
+    wave_sync();
+    if (lane == 0) {
+        sort(sp, nf, 0);
+        sort(sp + nf, nr, 0);
     }
-    const uint32_t i = t >> 1, c = t & 1;
-    const uint32_t off = pd.map_off[t];
-    PeItem* mp = pd.map + off;
-    PeItem* sp = pd.srt + off;
-    for (uint32_t x = 0; x < nf + nr; ++x) sp[x] = mp[x];
-    pe_sort(sp, (int)nf, 0);
-    pe_sort(sp + nf, (int)nr, 0);
+    wave_sync();
     const uint32_t poolA = pool_of(b1.lens[i]); // mate 1 is the first read in both combinations
-    const uint32_t m1 = pe_query(ix, b1, pd.acc1, poolA, sp, nf, sp + nf, nr, 0, PF2, mp, mp + nf, (int)pd.cw);
-    pd.m1_n[t] = m1;
-    if (m1 == 0) return;
-    // saQuerySeedSetRefSecond's order: the maps (with flag3) sorted by c1
-    for (uint32_t x = 0; x < nf + nr; ++x) sp[x] = mp[x];
-    pe_sort(sp, (int)nf, 1);
-    pe_sort(sp + nf, (int)nr, 1);
-    const uint32_t rB = pd.n + i;
-    const uint32_t roB = 2 * rB + (1 - c);
-    const uint32_t poolB = pool_of(b1.lens[rB]);
+    const uint32_t m1 =
+        pe_query(ix, b1, pd.acc1, poolA, sp, nf, sp + nf, nr, 0, PF2, mp, mp + nf, (int)pd.cw, lane, step);
+    wave_sync();
+    if (lane == 0) pd.m1_n[t] = m1;
+    if (m1 == 0) return PE_NONE;
+    for (uint32_t x = lane; x < n; x += step) sp[x] = mp[x];
+    wave_sync();
     uint32_t nj = 0;
-    for (uint32_t x = 0; x < nf + nr; ++x)
-        if ((sp[x].v & PF3) && pe_c1(sp[x].v) >= QTHRESH) ++nj;
-    uint32_t base = 0;
-    bool ok = true;
-    if (nj) {
-        base = atomicAdd(&b2.pool_n[poolB], nj);
-        if (base + nj > pc_get_cap(b2, poolB)) {
-            atomicOr(b1.flags, FLAG_POOL_FULL);
-            ok = false;
-        }
+    if (lane == 0) {
+        sort(sp, nf, 1);
+        sort(sp + nf, nr, 1);
+        for (uint32_t x = 0; x < n; ++x)
+            if ((sp[x].v & PF3) && pe_c1(sp[x].v) >= QTHRESH) ++nj;
     }
+    wave_sync();
+    return nj;
+}
+
+// the second reads' verify slots base, base + 1, ... for the counted windows
+// of `sp` (one thread); ok = false: the pool overflowed, slots are left out
+__device__ __forceinline__ void pe_walk1_assign(const DevIndex& ix, const BatchDev& b2, const PairDev& pd,
+                                                uint32_t t, uint32_t nf, uint32_t nr, PeItem* sp, uint32_t poolB,
+                                                uint32_t base, bool ok)
+{
+    const uint32_t roB = 2 * (pd.n + (t >> 1)) + (1 - (t & 1));
     uint32_t* jm = pc_meta(b2, poolB);
     uint32_t* jr = b2.job_ro[0];
 #pragma unroll
@@ -1150,6 +1199,115 @@ __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, Batc
             jr[j] = roB;
             sp[x].j = j;
         }
+    }
+}
+
+// one thread per item; items of more than PE_WCAP windows are listed for
+// k_pe_walk1_big.  The second reads' slots are taken with one atomic per
+// pool and wave.
+__global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
+{
+    __shared__ uint32_t s_map[256 * PE_WSTRIDE];
+    __shared__ uint32_t s_srt[256 * PE_WSTRIDE];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    PeItem* lm = (PeItem*)(s_map + threadIdx.x * PE_WSTRIDE);
+    PeItem* ls = (PeItem*)(s_srt + threadIdx.x * PE_WSTRIDE);
+    uint32_t nf = 0, nr = 0, off = 0, nj = PE_NONE, poolB = 0;
+    if (t < 2 * pd.n) {
+        nf = pd.map_nf[t];
+        nr = pd.map_nr[t];
+        // items of a combination not queried were not seeded: their entries are stale
+        if (!((pd.combos >> (t & 1)) & 1) || nf + nr == 0) {
+            pd.m1_n[t] = 0;
+        } else if (nf + nr > PE_WCAP) {
+            pd.wlist[atomicAdd(pd.wlist_n, 1u)] = t;
+        } else {
+            off = pd.map_off[t];
+            for (uint32_t x = 0; x < nf + nr; ++x) lm[x] = pd.map[off + x];
+            nj = pe_walk1_order(ix, b1, pd, t, nf, nr, lm, ls, 0, 1, nullptr);
+            poolB = pool_of(b1.lens[pd.n + (t >> 1)]);
+        }
+    }
+    // slots per pool: a wave scan of nj over the lanes of that pool
+    const uint32_t want = (nj != PE_NONE && nj) ? poolB : PE_NONE;
+    uint32_t base = 0;
+    bool ok = true;
+    uint64_t pending = __ballot(want != PE_NONE);
+    while (pending) {
+        const uint32_t lead = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+        const uint32_t p = __builtin_amdgcn_readlane(want, lead);
+        const bool mine = want == p;
+        const uint32_t v = mine ? nj : 0u;
+        const uint32_t inc = wave_iscan_u32(v, (int)lane);
+        const uint32_t tot = lane_u32(inc, 63);
+        uint32_t b = 0;
+        if (lane == lead) b = atomicAdd(&b2.pool_n[p], tot);
+        b = __builtin_amdgcn_readlane(b, lead);
+        if (mine) {
+            base = b + inc - v;
+            ok = b + tot <= pc_get_cap(b2, p);
+        }
+        if (lane == lead && b + tot > pc_get_cap(b2, p)) atomicOr(b1.flags, FLAG_POOL_FULL);
+        pending &= ~__ballot(mine);
+    }
+    if (nj != PE_NONE) {
+        pe_walk1_assign(ix, b2, pd, t, nf, nr, ls, poolB, base, ok);
+        for (uint32_t x = 0; x < nf + nr; ++x) pd.srt[off + x] = ls[x];
+    }
+}
+
+// windows of a listed item walked in LDS by k_pe_walk1_big (more: in the
+// global map pool)
+constexpr uint32_t PE_WBIG = 1360;
+
+// the listed items, one wave each, taken from a device counter
+__global__ __launch_bounds__(64) void k_pe_walk1_big(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
+{
+    __shared__ PeItem s_map[PE_WBIG];
+    __shared__ PeItem s_srt[PE_WBIG];
+    __shared__ int s_stk[3 * PE_SORT_STACK];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cnt = *pd.wlist_n;
+    uint32_t nt = 0;
+    if (lane == 0) nt = atomicAdd(pd.wnext, 1u);
+    uint32_t q = lane_u32(nt, 0);
+    while (q < cnt) {
+        if (lane == 0) nt = atomicAdd(pd.wnext, 1u);
+        const uint32_t t = pd.wlist[q];
+        const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t], off = pd.map_off[t];
+        PeItem* mp = pd.map + off;
+        PeItem* sp = pd.srt + off;
+        const bool in_lds = nf + nr <= PE_WBIG;
+        PeItem* wm = mp;
+        PeItem* ws = sp;
+        if (in_lds) {
+            for (uint32_t x = lane; x < nf + nr; x += 64) s_map[x] = mp[x];
+            wave_sync();
+            wm = s_map;
+            ws = s_srt;
+        }
+        const uint32_t nj = pe_walk1_order(ix, b1, pd, t, nf, nr, wm, ws, lane, 64, s_stk);
+        if (nj != PE_NONE) {
+            if (lane == 0) {
+                const uint32_t poolB = pool_of(b1.lens[pd.n + (t >> 1)]);
+                uint32_t base = 0;
+                bool ok = true;
+                if (nj) {
+                    base = atomicAdd(&b2.pool_n[poolB], nj);
+                    if (base + nj > pc_get_cap(b2, poolB)) {
+                        atomicOr(b1.flags, FLAG_POOL_FULL);
+                        ok = false;
+                    }
+                }
+                pe_walk1_assign(ix, b2, pd, t, nf, nr, ws, poolB, base, ok);
+            }
+            wave_sync();
+            if (in_lds)
+                for (uint32_t x = lane; x < nf + nr; x += 64) sp[x] = s_srt[x];
+        }
+        wave_sync();
+        q = lane_u32(nt, 0);
     }
 }
 
@@ -1330,6 +1488,8 @@ void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd
     const uint32_t items = 2 * pd.n;
     if (!items) return;
     hipLaunchKernelGGL(k_pe_walk1, dim3((items + 255) / 256), dim3(256), 0, st, ix, bd1, bd2, pd);
+    // the listed items: as many waves as fit at once (their count is on the device)
+    hipLaunchKernelGGL(k_pe_walk1_big, dim3(std::min<uint32_t>(items, 1024)), dim3(64), 0, st, ix, bd1, bd2, pd);
 }
 
 void launch_pe_final(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd,
