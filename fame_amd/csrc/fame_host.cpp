@@ -1232,7 +1232,8 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     pd.wlist_n = pd.map_n + 3;
     pd.wnext = pd.map_n + 4;
     pd.wlist = grow<uint32_t>(c->b_pe_wlist, nitems + 1);
-    c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, std::max<uint32_t>(1u << 16, 8 * n));
+    c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, std::max<uint32_t>(1u << 16, 8 * n) +
+                                                          FAME_PE_MAP_CHUNK * seed_grid);
     HIPC(c->d_pe_counters.reserve(64));
     pd.counters = c->d_pe_counters.as<uint64_t>();
     HIPC(hipMemsetAsync(pd.counters, 0, 40, st));

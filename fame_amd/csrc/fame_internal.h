@@ -12,6 +12,7 @@ struct fame_ctx;
 
 #define FAME_DEV_MAX_READ 255
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
+#define FAME_PE_MAP_CHUNK 256 // map pool entries a paired seed wave reserves at a time
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
 #define FAME_NSTATS 8
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)

@@ -93,6 +93,10 @@ struct PeWork {
 };
 
 constexpr uint32_t PE_NEVER = 0xFFFFFFFFu;
+
+struct MapChunk {
+    uint32_t base, left; // the wave's open chunk of the map pool
+};
 constexpr uint32_t PE_LE = 256; // index hits per read, light variant
 constexpr uint32_t PE_LM = 64;  // windows per strand map, light variant
 constexpr uint32_t PE_LH = 512; // lookup slots, light variant
@@ -539,7 +543,7 @@ __device__ int pe_second(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L
 // one item (pair combination) on one wavefront
 template <typename TabT>
 __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, const SeedShared& sh, PeLds& m,
-                        const PeWork<TabT>& w, uint32_t t, PoolChunk& pcA, uint64_t (&st)[4])
+                        const PeWork<TabT>& w, uint32_t t, PoolChunk& pcA, MapChunk& mc, uint64_t (&st)[4])
 {
     const int lane = threadIdx.x & 63;
     const uint64_t below = lanes_below(lane);
@@ -616,8 +620,17 @@ __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& p
                 if (hasCpG) {
                     nf = m.nmap[0];
                     nr = m.nmap[1];
-                    if (lane == 0) off = atomicAdd(pd.map_n, nf + nr);
-                    off = lane_u32(off, 0);
+                    // the wave's open chunk of the map pool (gaps are never read)
+                    if (nf + nr > mc.left) {
+                        const uint32_t take = max(nf + nr, (uint32_t)FAME_PE_MAP_CHUNK);
+                        uint32_t b = 0;
+                        if (lane == 0) b = atomicAdd(pd.map_n, take);
+                        mc.base = lane_u32(b, 0);
+                        mc.left = take;
+                    }
+                    off = mc.base;
+                    mc.base += nf + nr;
+                    mc.left -= nf + nr;
                     if (off + nf + nr > pd.map_cap) {
                         if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
                         nf = nr = 0;
@@ -794,13 +807,16 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
     const uint32_t both = pd.combos == 3u;
     const uint32_t nq = both ? 2 * pd.n : (pd.combos ? pd.n : 0u);
     const uint32_t c1 = pd.combos == 2u ? 1u : 0u;
+    MapChunk mc = {0, 0};
+    constexpr uint32_t TAKE = 4; // items per take
     uint32_t nt = 0;
-    if (lane == 0) nt = atomicAdd(pd.next, 1u);
-    uint32_t q = lane_u32(nt, 0);
-    while (q < nq) {
-        if (lane == 0) nt = atomicAdd(pd.next, 1u);
-        pe_item<TabT>(ix, bd, pd, sh, m, w, both ? q : 2 * q + c1, pcA, st);
-        q = lane_u32(nt, 0);
+    if (lane == 0) nt = atomicAdd(pd.next, TAKE);
+    uint32_t q0 = lane_u32(nt, 0);
+    while (q0 < nq) {
+        if (lane == 0) nt = atomicAdd(pd.next, TAKE);
+        const uint32_t qe = min(q0 + TAKE, nq);
+        for (uint32_t q = q0; q < qe; ++q) pe_item<TabT>(ix, bd, pd, sh, m, w, both ? q : 2 * q + c1, pcA, mc, st);
+        q0 = lane_u32(nt, 0);
     }
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
     if (lane == 0 && st[2]) {
@@ -857,8 +873,9 @@ __global__ __launch_bounds__(64) void k_pe_seed_big(DevIndex ix, BatchDev bd, Pa
     w.hi = (uint32_t*)take(4ull * pd.mhsz);
     PoolChunk pcA = {0, 0, 0};
     uint64_t st[4] = {0, 0, 0, 0};
+    MapChunk mc = {0, 0};
     for (uint32_t ti = blockIdx.x; ti < pd.list_n; ti += gridDim.x)
-        pe_item<TabT>(ix, bd, pd, sh, m, w, pd.list[ti], pcA, st);
+        pe_item<TabT>(ix, bd, pd, sh, m, w, pd.list[ti], pcA, mc, st);
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
     if (lane == 0 && st[2]) {
         atomicAdd(&bd.stats[0], (unsigned long long)st[0]);
