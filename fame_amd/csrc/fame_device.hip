@@ -2095,12 +2095,18 @@ void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hi
     }
 }
 
+// shortest and longest read, one atomic pair per wave
 __global__ void k_len_range(const uint32_t* __restrict__ lens, uint32_t n, uint32_t* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        atomicMin(&out[0], lens[i]);
-        atomicMax(&out[1], lens[i]);
+    uint32_t lo = i < n ? lens[i] : 0xFFFFFFFFu, hi = i < n ? lens[i] : 0u;
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, 64));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+        atomicMin(&out[0], lo);
+        atomicMax(&out[1], hi);
     }
 }
 

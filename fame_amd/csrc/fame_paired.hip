@@ -1231,6 +1231,7 @@ __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, Batc
     PeItem* lm = (PeItem*)(s_map + threadIdx.x * PE_WSTRIDE);
     PeItem* ls = (PeItem*)(s_srt + threadIdx.x * PE_WSTRIDE);
     uint32_t nf = 0, nr = 0, off = 0, nj = PE_NONE, poolB = 0;
+    bool heavy = false;
     if (t < 2 * pd.n) {
         nf = pd.map_nf[t];
         nr = pd.map_nr[t];
@@ -1238,13 +1239,22 @@ __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, Batc
         if (!((pd.combos >> (t & 1)) & 1) || nf + nr == 0) {
             pd.m1_n[t] = 0;
         } else if (nf + nr > PE_WCAP) {
-            pd.wlist[atomicAdd(pd.wlist_n, 1u)] = t;
+            heavy = true;
         } else {
             off = pd.map_off[t];
             for (uint32_t x = 0; x < nf + nr; ++x) lm[x] = pd.map[off + x];
             nj = pe_walk1_order(ix, b1, pd, t, nf, nr, lm, ls, 0, 1, nullptr);
             poolB = pool_of(b1.lens[pd.n + (t >> 1)]);
         }
+    }
+    // the heavy items onto the list, one atomic per wave
+    const uint64_t hb = __ballot(heavy);
+    if (hb) {
+        const uint32_t lead = (uint32_t)__ffsll((unsigned long long)hb) - 1;
+        uint32_t b = 0;
+        if (lane == lead) b = atomicAdd(pd.wlist_n, (uint32_t)__popcll(hb));
+        b = __builtin_amdgcn_readlane(b, lead);
+        if (heavy) pd.wlist[b + __popcll(hb & lanes_below((int)lane))] = t;
     }
     // slots per pool: a wave scan of nj over the lanes of that pool
     const uint32_t want = (nj != PE_NONE && nj) ? poolB : PE_NONE;
