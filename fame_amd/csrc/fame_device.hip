@@ -2116,13 +2116,26 @@ void launch_len_range(const uint32_t* lens, uint32_t n, uint32_t* out, hipStream
     hipLaunchKernelGGL(k_len_range, dim3((n + 255) / 256), dim3(256), 0, st, lens, n, out);
 }
 
-void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st)
+void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
+                       hipStream_t st)
 {
-    if (!hs.count) return;
+    if (!hs.count || !blocks) return;
     if (ix.tab_is64)
-        hipLaunchKernelGGL(k_seed_heavy<uint64_t>, dim3(hs.count), dim3(64), 0, st, ix, bd, hs);
+        hipLaunchKernelGGL(k_seed_heavy<uint64_t>, dim3(blocks), dim3(64), 0, st, ix, bd, hs);
     else
-        hipLaunchKernelGGL(k_seed_heavy<uint32_t>, dim3(hs.count), dim3(64), 0, st, ix, bd, hs);
+        hipLaunchKernelGGL(k_seed_heavy<uint32_t>, dim3(blocks), dim3(64), 0, st, ix, bd, hs);
+}
+
+// blocks of k_seed_heavy that are resident at once on the current device
+uint32_t seed_heavy_blocks(bool tab64)
+{
+    int nb = 0, dev = 0;
+    hipDeviceProp_t p;
+    const void* k = tab64 ? (const void*)k_seed_heavy<uint64_t> : (const void*)k_seed_heavy<uint32_t>;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 64, 0) != hipSuccess || nb <= 0)
+        return 4096;
+    return (uint32_t)nb * (uint32_t)p.multiProcessorCount;
 }
 
 void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,

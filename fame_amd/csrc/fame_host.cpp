@@ -517,6 +517,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             };
             std::vector<Cls> classes;
             size_t need = 0;
+            const uint32_t resident = seed_heavy_blocks(c->ix.tab_is64);
             for (uint32_t c0 = 0; c0 < heavy_n;) {
                 uint64_t cap = 1024;
                 while (cap < he[ord[c0]]) cap *= 4;
@@ -534,9 +535,12 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
                 hs.tb_cap = (uint32_t)(2 * tbb + 64);
                 const size_t stride = heavy_scratch_stride(hs);
                 hs.stride = stride;
+                // one launch per class: as many blocks (slices) as are resident
+                // at once, within the scratch budget, striding over the class
                 const size_t budget = 2ull << 30;
                 const uint32_t cnt = c1 - c0;
-                const uint32_t per = (uint32_t)std::max<size_t>(1, std::min<size_t>(cnt, budget / stride));
+                const uint32_t per = (uint32_t)std::max<size_t>(
+                    1, std::min<size_t>(std::min<size_t>(cnt, budget / stride), resident));
                 need = std::max(need, stride * per);
                 classes.push_back({c0, c1, per, hs});
                 c0 = c1;
@@ -545,11 +549,9 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             tm.mark(12);
             for (Cls& k : classes) {
                 k.hs.base = c->b_heavy_scratch.as<uint8_t>();
-                for (uint32_t first = k.c0; first < k.c1; first += k.per) {
-                    k.hs.first = first;
-                    k.hs.count = std::min(k.per, k.c1 - first);
-                    launch_seed_heavy(c->ix, bd, k.hs, st);
-                }
+                k.hs.first = k.c0;
+                k.hs.count = k.c1 - k.c0;
+                launch_seed_heavy(c->ix, bd, k.hs, k.per, st);
             }
             tm.mark(13);
             HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
@@ -758,8 +760,8 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
         HIPC(pl->heavy_e.reserve(4 * n_ro));
         HIPC(pl->heavy_bc.reserve(4 * n_ro));
         // heavy (read, orientation) pairs in graph, two tiers: up to 16384
-        // index hits on 1024 scratch slices, up to 2^20 on 16; larger ones make
-        // the batch fall back to the plain path
+        // index hits on 1024 scratch slices (or more, below), up to 2^20 on
+        // 16; larger ones make the batch fall back to the plain path
         auto dims = [](HeavyScratch& hs, uint64_t emax) {
             hs.cap_e = (uint32_t)emax;
             hs.hsize = (uint32_t)(2 * emax);
@@ -767,6 +769,9 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
             hs.stride = heavy_scratch_stride(hs);
         };
         dims(pl->hs, 16384);
+        // batches of a million reads or more: the first tier on as many
+        // slices as are resident at once (1.25 MB each)
+        if (cap >= (1u << 20)) pl->heavy_blocks = std::max<uint32_t>(1024, seed_heavy_blocks(c->ix.tab_is64));
         pl->hs.e_lo = 0;
         pl->hs.e_hi = 16384;
         dims(pl->hs2, 1u << 20);

@@ -215,7 +215,10 @@ void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavySc
 void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint = 0);
 // shortest and longest of n lengths (out[0] must start at ~0, out[1] at 0)
 void launch_len_range(const uint32_t* lens, uint32_t n, uint32_t* out, hipStream_t st);
-void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, hipStream_t st);
+// plain form: blocks (at most hs.count) stride over items first .. first + count - 1
+void launch_seed_heavy(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
+                       hipStream_t st);
+uint32_t seed_heavy_blocks(bool tab64);
 // njobs = FAME_JOBS_ON_DEVICE: as many as the device counter holds, on
 // `blocks` persistent blocks (batched form)
 #define FAME_JOBS_ON_DEVICE 0xFFFFFFFFu
