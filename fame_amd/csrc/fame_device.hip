@@ -593,6 +593,15 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     PoolChunk pc = {0, 0, 0};
     SeedStats ss = {0, 0, 0, 0};
     const uint32_t nw = gridDim.x * FAME_SEED_WPB;
+    // the (read, orientation) pairs of the queried orientations only: pair q
+    // is row 2q + o when one orientation o is queried (k_select never reads
+    // the other's rows), row q when both are; an even wave stride over all
+    // rows would leave every other wave without work
+    const uint32_t om = bd.ori_mask;
+    const bool both = om == 3u;
+    const uint32_t nq = both ? n_ro : (om ? n_ro >> 1 : 0u);
+    const uint32_t o1 = om == 2u ? 1u : 0u;
+    auto ro_of = [&](uint32_t q) { return both ? q : 2 * q + o1; };
     auto defer = [&](uint32_t ro, uint64_t E, uint32_t bc) {
         if (lane == 0) {
             const uint32_t h = atomicAdd(bd.heavy_n, 1u);
@@ -612,12 +621,12 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         // slot-table mode, software-pipelined over this wave's pairs: while
         // pair j is worked on, the slot loads of pair j+1 and the code row of
         // pair j+2 are in flight (rows past the batch read row 0, unused)
-        auto row = [&](uint32_t r) {
-            return ((const uint32_t*)(bd.codes + (size_t)(r < n_ro ? r : 0u) * 256))[lane];
+        auto row = [&](uint32_t q) {
+            return ((const uint32_t*)(bd.codes + (size_t)(q < nq ? ro_of(q) : 0u) * 256))[lane];
         };
-        uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv;
-        uint32_t cwc = row(ro), cwn = row(ro + nw);
-        uint32_t Lc = ro < n_ro ? (lane_u32(cwc, 63) >> 24) : 0;
+        uint32_t q = blockIdx.x * FAME_SEED_WPB + wv;
+        uint32_t cwc = row(q), cwn = row(q + nw);
+        uint32_t Lc = q < nq ? (lane_u32(cwc, 63) >> 24) : 0;
         uint64_t slc[C::KPL], sln[C::KPL];
 #pragma unroll
         for (int q = 0; q < C::KPL; ++q) slc[q] = sln[q] = 0;
@@ -626,17 +635,18 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             wave_sync();
             seed_stage<C::KPL>(ix, sh, w.code, Lc - KMERLEN + 1, slc);
         }
-        for (; ro < n_ro; ro += nw) {
-            const uint32_t rn = ro + nw;
-            const uint32_t cwnn = row(rn + nw);
-            const uint32_t Ln = rn < n_ro ? (lane_u32(cwn, 63) >> 24) : 0;
+        for (; q < nq; q += nw) {
+            const uint32_t ro = ro_of(q);
+            const uint32_t qn = q + nw;
+            const uint32_t cwnn = row(qn + nw);
+            const uint32_t Ln = qn < nq ? (lane_u32(cwn, 63) >> 24) : 0;
             if (Ln) {
                 wave_sync(); // the previous pair's hashing has read w.code
                 ((uint32_t*)w.code)[lane] = cwn;
                 wave_sync();
                 seed_stage<C::KPL>(ix, sh, w.code, Ln - KMERLEN + 1, sln);
             }
-            if (Lc == 0) { // not seeded: invalid read or orientation not asked for
+            if (Lc == 0) { // not seeded: invalid read
                 if (lane == 0) {
                     bd.cand_nf[ro] = 0;
                     bd.cand_nr[ro] = 0;
@@ -656,9 +666,10 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             Lc = Ln;
         }
     }
-    for (uint32_t ro = blockIdx.x * FAME_SEED_WPB + wv; !ix.slot && ro < n_ro; ro += nw) {
+    for (uint32_t q = blockIdx.x * FAME_SEED_WPB + wv; !ix.slot && q < nq; q += nw) {
+        const uint32_t ro = ro_of(q);
         const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
-        if ((lane_u32(cw, 63) >> 24) == 0) { // not seeded: invalid read or orientation not asked for
+        if ((lane_u32(cw, 63) >> 24) == 0) { // not seeded: invalid read
             if (lane == 0) {
                 bd.cand_nf[ro] = 0;
                 bd.cand_nr[ro] = 0;
