@@ -1237,6 +1237,16 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     pd.wlist_n = pd.map_n + 3;
     pd.wnext = pd.map_n + 4;
     pd.wlist = grow<uint32_t>(c->b_pe_wlist, nitems + 1);
+    // FAME_PE_WALK_LDS / FAME_PE_WALK_BIG_LDS lower the walk kernels' LDS
+    // limits (0: every item through k_pe_walk1_big, every such item's arrays
+    // in the map pool), so that tests reach those paths on small inputs
+    auto env_cap = [](const char* name, uint32_t dflt) {
+        const char* v = getenv(name);
+        if (!v || !*v) return dflt;
+        return (uint32_t)std::min<unsigned long>(strtoul(v, nullptr, 10), dflt);
+    };
+    pd.wcap = env_cap("FAME_PE_WALK_LDS", FAME_PE_WCAP);
+    pd.wbig = env_cap("FAME_PE_WALK_BIG_LDS", FAME_PE_WBIG);
     c->pe_map_cap = std::max<uint32_t>(c->pe_map_cap, std::max<uint32_t>(1u << 16, 8 * n) +
                                                           FAME_PE_MAP_CHUNK * seed_grid);
     HIPC(c->d_pe_counters.reserve(64));

@@ -13,6 +13,8 @@ struct fame_ctx;
 #define FAME_DEV_MAX_READ 255
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
 #define FAME_PE_MAP_CHUNK 256 // map pool entries a paired seed wave reserves at a time
+#define FAME_PE_WCAP 8         // windows of an item k_pe_walk1 walks in LDS, one lane per item
+#define FAME_PE_WBIG 1360      // windows of an item k_pe_walk1_big walks in LDS, one wave per item
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
 #define FAME_NSTATS 8
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
@@ -153,6 +155,8 @@ struct PairDev {
     uint32_t* wlist;
     uint32_t* wlist_n;
     uint32_t* wnext;
+    uint32_t wcap; // items of more windows go to k_pe_walk1_big (<= FAME_PE_WCAP)
+    uint32_t wbig; // which keeps at most this many in LDS (<= FAME_PE_WBIG)
     uint32_t map_cap;
     PeItem* map;     // [map_cap] maps as built by the seed stage
     PeItem* srt;     // [map_cap] the same slices sorted by a count

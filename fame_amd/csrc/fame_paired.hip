@@ -1143,7 +1143,7 @@ __device__ uint32_t pe_query(const DevIndex& ix, const BatchDev& vb, uint8_t* co
 // windows of an item that k_pe_walk1 sorts and walks in LDS (more: in the
 // global map pool); a thread's two copies take PE_WSTRIDE words each, an odd
 // count, so that the threads of a wave fall on distinct banks
-constexpr uint32_t PE_WCAP = 8;
+constexpr uint32_t PE_WCAP = FAME_PE_WCAP;
 constexpr uint32_t PE_WSTRIDE = 3 * PE_WCAP + 1;
 static_assert(sizeof(PeItem) == 12, "PeItem is three words");
 static_assert(PE_WCAP <= 16, "k_pe_walk1 sorts by insertion only");
@@ -1219,8 +1219,8 @@ __device__ __forceinline__ void pe_walk1_assign(const DevIndex& ix, const BatchD
     }
 }
 
-// one thread per item; items of more than PE_WCAP windows are listed for
-// k_pe_walk1_big.  The second reads' slots are taken with one atomic per
+// one thread per item; items of more than pd.wcap (<= PE_WCAP) windows are
+// listed for k_pe_walk1_big.  The second reads' slots are taken with one atomic per
 // pool and wave.
 __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
 {
@@ -1238,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, Batc
         // items of a combination not queried were not seeded: their entries are stale
         if (!((pd.combos >> (t & 1)) & 1) || nf + nr == 0) {
             pd.m1_n[t] = 0;
-        } else if (nf + nr > PE_WCAP) {
+        } else if (nf + nr > pd.wcap) {
             heavy = true;
         } else {
             off = pd.map_off[t];
@@ -1286,7 +1286,7 @@ __global__ __launch_bounds__(256) void k_pe_walk1(DevIndex ix, BatchDev b1, Batc
 
 // windows of a listed item walked in LDS by k_pe_walk1_big (more: in the
 // global map pool)
-constexpr uint32_t PE_WBIG = 1360;
+constexpr uint32_t PE_WBIG = FAME_PE_WBIG;
 
 // the listed items, one wave each, taken from a device counter
 __global__ __launch_bounds__(64) void k_pe_walk1_big(DevIndex ix, BatchDev b1, BatchDev b2, PairDev pd)
@@ -1305,7 +1305,7 @@ __global__ __launch_bounds__(64) void k_pe_walk1_big(DevIndex ix, BatchDev b1, B
         const uint32_t nf = pd.map_nf[t], nr = pd.map_nr[t], off = pd.map_off[t];
         PeItem* mp = pd.map + off;
         PeItem* sp = pd.srt + off;
-        const bool in_lds = nf + nr <= PE_WBIG;
+        const bool in_lds = nf + nr <= pd.wbig;
         PeItem* wm = mp;
         PeItem* ws = sp;
         if (in_lds) {

@@ -74,3 +74,22 @@ def test_hip_pairs_match_oracle_other_chunking(golden_pair):
     assert cnt.tolist() == m.counters5.tolist()
     assert np.array_equal(ctx.meth(), m.meth())
     ctx.close()
+
+
+@pytest.mark.parametrize("walk_env", [{"FAME_PE_WALK_LDS": "0"},
+                                      {"FAME_PE_WALK_LDS": "0", "FAME_PE_WALK_BIG_LDS": "0"}],
+                         ids=["wave-walk-lds", "wave-walk-map-pool"])
+def test_hip_pairs_wave_walks_match_reference(golden_pair, monkeypatch, walk_env):
+    """Every item through the one-wave walk kernel (k_pe_walk1_big), with its
+    windows in LDS or in the map pool: the same per-pair results, counters
+    and CpG counts as FAME."""
+    for k, v in walk_env.items():
+        monkeypatch.setenv(k, v)
+    c = golden_pair
+    print(f"case {c.name}: {len(c.r1[2])} pairs, {walk_env}", flush=True)
+    ctx, v1, m1, v2, m2, cnt, _ = run_hip_pairs(c)
+    bad = np.nonzero((v1 != c.valid1) | (v2 != c.valid2) | (m1 != c.mat1) | (m2 != c.mat2))[0]
+    assert bad.size == 0, f"{bad.size} pairs differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == c.counters
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()
