@@ -815,10 +815,10 @@ __device__ __forceinline__ void myers_step(uint64_t (&Pv)[W], uint64_t (&Mv)[W],
 // scan restarted at the start of that region therefore reproduces every
 // end position with at most NERR errors there, with its error count, and
 // no position outside the regions is one.  Regions are kept as a mask of
-// 16-byte scan chunks: a chunk in which some piece ends marks the union of
-// the regions of any piece ending anywhere in it (which piece it was is not
-// tracked), and a run of consecutive marked chunks is scanned from a fresh
-// automaton.  Windows holding an N, patterns shorter than
+// 16-byte scan chunks: a chunk in which pieces ilo..ihi end (lowest and
+// highest index among its hits) marks the union of the regions of those
+// pieces ending anywhere in it, and a run of consecutive marked chunks is
+// scanned from a fresh automaton.  Windows holding an N, patterns shorter than
 // (NERR+1)*FILT_MIN_P and windows wider than the mask are scanned whole.
 constexpr int FILT_MIN_P = 10;
 constexpr int FILT_MAX_P = 14;
@@ -969,7 +969,12 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     // distinct pieces; free entries repeat a piece (never a spurious hit).
     // Pieces with a letter other than A, C, G, T never match and are left
     // out.  No multiplier found (or no piece): the window is scanned whole.
+    // An entry's bits below 32 - 2*FP (at least 4) hold its piece's index,
+    // so that a hit tells which piece ended there; pieces whose code repeats
+    // an earlier piece's are left out of the table, and then a hit may be
+    // any of them (pdup: the chunk's regions are those of every piece).
     uint32_t hmul = 0, tmask = 0;
+    bool pdup = false;
     if (filt) {
         const uint32_t tsh = 32u - 2u * (uint32_t)FP;
         tmask = ~0u << tsh;
@@ -979,11 +984,13 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         bool any = false;
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            pv[i] = pc[i] != 0xFFFFFFFFu;
+            const bool valid = pc[i] != 0xFFFFFFFFu;
+            pv[i] = valid;
             pt[i] = pv[i] ? pc[i] << tsh : 0u;
 #pragma unroll
             for (int j = 0; j < i; ++j) pv[i] = pv[i] && !(pv[j] && pt[j] == pt[i]);
-            fill = (!any && pv[i]) ? pt[i] : fill;
+            pdup = pdup || (valid && !pv[i]);
+            fill = (!any && pv[i]) ? (pt[i] | (uint32_t)i) : fill;
             any = any || pv[i];
         }
         bool ok = false;
@@ -1012,8 +1019,8 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
                     const bool in = pv[i] && piece_bucket(pt[i], hmul) == b;
-                    e0 = (in && n == 0) ? pt[i] : e0;
-                    e1 = (in && n == 1) ? pt[i] : e1;
+                    e0 = (in && n == 0) ? (pt[i] | (uint32_t)i) : e0;
+                    e1 = (in && n == 1) ? (pt[i] | (uint32_t)i) : e1;
                     n += in ? 1u : 0u;
                 }
                 s_ptab[b * 256 + threadIdx.x] = make_uint2(e0, e1);
@@ -1216,28 +1223,45 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 P |= ((r | (r >> 6) | (r >> 12) | (r >> 18)) & 0xFFu) << (8 * q);
             }
             // byte offset of letter ii's bucket in the table: bucket * 2048
-            // (= bits 29-31 of the product moved to bits 11-13) | lane * 8
+            // (= bits 29-31 of the product moved to bits 11-13) | lane * 8.
+            // The masked code XOR an entry is its piece index (< 16) on a
+            // hit, at least 16 otherwise.
             auto probe = [&](int ii) -> uint32_t {
                 const uint32_t xc = ii < 15 ? __builtin_amdgcn_alignbit(P, pprev, 2 * (ii + 1)) : P;
                 const uint32_t off = ((mul24u(xc >> 12, hmul) >> 18) & 0x3800u) | tbase;
                 const uint2 e = *(const uint2*)((const char*)s_ptab + off);
-                return min((xc ^ e.x) & tmask, (xc ^ e.y) & tmask);
+                const uint32_t xm = xc & tmask;
+                return min(xm ^ e.x, xm ^ e.y);
             };
-            uint32_t acc = 0xFFFFFFFFu;
+            // lowest piece index among the chunk's hits, and 15 - the highest
+            uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
             if (k.lo == 0 && k.hi == 16) {
 #pragma unroll
-                for (int ii = 0; ii < 16; ++ii) acc = min(acc, probe(ii));
+                for (int ii = 0; ii < 16; ++ii) {
+                    const uint32_t r = probe(ii);
+                    alo = min(alo, r);
+                    ahi = min(ahi, r ^ 15u);
+                }
             } else {
 #pragma unroll
                 for (int ii = 0; ii < 16; ++ii) {
                     const uint32_t r = probe(ii);
-                    acc = (ii >= k.lo && ii < k.hi) ? min(acc, r) : acc;
+                    const bool in = ii >= k.lo && ii < k.hi;
+                    alo = in ? min(alo, r) : alo;
+                    ahi = in ? min(ahi, r ^ 15u) : ahi;
                 }
             }
             pprev = P;
-            if (acc == 0) // a piece ends in this chunk: the union of the regions of any piece there
-                mark_range(x - (int64_t)NP * FP + 1 - (int64_t)NERR,
-                           x + (k.hi - k.lo) - 1 - FP + (int64_t)m + (int64_t)NERR);
+            if (alo < 16u) {
+                // pieces ilo..ihi end in this chunk (every piece when codes
+                // repeat): the union of their regions.  Piece i ending at
+                // scan position e starts the match at >= e - (i+1)*FP + 1 -
+                // NERR and ends it at <= e - (i+1)*FP + m + NERR.
+                const int64_t ilo = pdup ? 0 : (int64_t)alo;
+                const int64_t ihi = pdup ? (int64_t)NP - 1 : 15 - (int64_t)ahi;
+                mark_range(x - (ihi + 1) * FP + 1 - (int64_t)NERR,
+                           x + (k.hi - k.lo) - 1 - (ilo + 1) * FP + (int64_t)m + (int64_t)NERR);
+            }
             x += k.hi - k.lo;
             return true;
         };
