@@ -62,6 +62,18 @@ CASES = {
     "se_long": dict(chroms=[60000, 45000], repeats=0, htab=1 << 16, readlen=100, chunk=300000,
                     n=3000, seed=16, unord=False, flip=False, len_range=(120, 255), gen_len=255,
                     share="se_basic"),
+    # an index with a blacklist: KMERCUTOFF 24 and planted poly-A / poly-T runs
+    # and motif families (16 A then a 16-letter tail), so that filterHashTable
+    # (RefGenome.cpp:2267-2352) removes k-mers and filteredKmers holds codes
+    # below 2^32, the only ones the paired path's u32 kSeq can find
+    # (ReadQueue.cpp:2527-2551, 2643, 2974); 20 % of the reads cover a motif
+    "se_filt": dict(chroms=[70000, 50000], repeats=4, htab=1 << 14, readlen=100, chunk=300000,
+                    n=4000, seed=17, unord=False, flip=False, cut=24, motifs=True),
+    # the same genome indexed with --no_loss (no blacklist: the motif buckets
+    # keep every entry)
+    "se_noloss": dict(chroms=[70000, 50000], repeats=4, htab=1 << 14, readlen=100, chunk=300000,
+                      n=2000, seed=18, unord=False, flip=False, cut=24, motifs=True, noloss=True,
+                      genome_of="se_filt"),
 }
 
 
@@ -79,19 +91,56 @@ PAIRED = {
                      gap=(20, 400), unord=False, flip=False, len_range=(100, 150)),
     "pe_ga_chunked": dict(share="se_ga_chunked", n=2500, seed=25, chunk=700, minpd=50, maxpd=550,
                           gap=(20, 400), unord=False, flip=True),
+    # pairs over the blacklisted index: the filteredKmers lookup of both
+    # mates' k-mers; a fifth of the fragments cover a planted motif
+    "pe_filt": dict(share="se_filt", n=3000, seed=26, chunk=300000, minpd=50, maxpd=550,
+                    gap=(20, 400), unord=False, flip=False),
+    "pe_noloss": dict(share="se_noloss", n=1500, seed=27, chunk=300000, minpd=50, maxpd=550,
+                      gap=(20, 400), unord=False, flip=False),
 }
 
 
 def ref_binary(cfg, minpd=50, maxpd=550):
     h = cfg["htab"]
+    cut = cfg.get("cut", 1500)
     name = f"h{h}_r{cfg['readlen']}_c8_n{len(cfg['chroms'])}_p{minpd}-{maxpd}_k{cfg['chunk']}"
+    if cut != 1500:
+        name += f"_x{cut}"
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"HTAB={h}",
                     f"READLEN={cfg['readlen']}", "CORENUM=8", f"CHROMNUM={len(cfg['chroms'])}",
-                    f"CHUNK={cfg['chunk']}", f"MINPD={minpd}", f"MAXPD={maxpd}"], check=True)
+                    f"CHUNK={cfg['chunk']}", f"MINPD={minpd}", f"MAXPD={maxpd}", f"CUT={cut}"], check=True)
     return os.path.join(ROOT, "oracle", "_ref", f"fame_ref_{name}")
 
 
-def make_genome(cfg):
+# motif families planted by plant_motifs: 16 A then these tails.  The paired
+# path's kSeq is the unspaced code of a k-mer's last 16 letters, a filtered
+# code the spaced code of all 32 (SEED position 30 is a gap), so a tail
+# whose letter 14 is A can be found (and blacklists every read k-mer ending
+# in it), one whose letter 14 is G never
+MOTIF_TAILS = (b"GTTCAGTCGATCAAAG", b"CATGGTTACGTCTTGA", b"TTGACCAGTAGCAGAT")
+
+
+def plant_motifs(cfg, chroms):
+    """Plants poly-A and poly-T runs and copies of the motif families in
+    place; returns the sites as (chromosome, start, length)."""
+    rng = np.random.default_rng(cfg["seed"] + 3000)
+    sites = []
+    # (AG)^8 is a tail too: its code is blacklisted through the copies of
+    # A^16 (AG)^8, and runs of (AG)^24 elsewhere give reads whose k-mers
+    # straddling a run's start end in it every second letter
+    runs = ([b"A" * 48] * 30 + [b"T" * 48] * 30 + [b"A" * 16 + t for t in MOTIF_TAILS for _ in range(80)]
+            + [b"A" * 16 + b"AG" * 8] * 80 + [b"A" * 16 + b"GA" * 8] * 80 + [b"AG" * 24] * 30)
+    for r in runs:
+        c = int(rng.integers(0, len(chroms)))
+        p = int(rng.integers(200, chroms[c].size - len(r) - 200))
+        chroms[c][p:p + len(r)] = np.frombuffer(r, np.uint8)
+        sites.append((c, p, len(r)))
+    return sites
+
+
+def make_genome(cfg, sites=None):
+    if cfg.get("genome_of"):
+        return make_genome(CASES[cfg["genome_of"]], sites)
     rng = np.random.default_rng(cfg["seed"] + 1000)
     chroms = synth.make_genome(cfg["chroms"], seed=cfg["seed"])
     # near-copies of a few segments (5-50 bp differences) so that reads have
@@ -109,7 +158,21 @@ def make_genome(cfg):
             idx = rng.integers(0, L, size=k)
             cp[idx] = synth._BASES[rng.integers(0, 4, size=k)]
             chroms[dst][d:d + L] = cp
+    if cfg.get("motifs"):
+        st = plant_motifs(cfg, chroms)
+        if sites is not None:
+            sites.extend(st)
     return chroms
+
+
+def motif_read(rng, chroms, sites, L):
+    """A read of L letters over a planted site, C->T converted at 90 %."""
+    c, p, n = sites[int(rng.integers(0, len(sites)))]
+    s0 = p + n - int(rng.integers(n // 2, L))
+    s0 = min(max(s0, 0), chroms[c].size - L)
+    s = chroms[c][s0:s0 + L].copy()
+    s[s == ord("C")] = np.where(rng.random((s == ord("C")).sum()) < 0.9, ord("T"), ord("C"))
+    return s
 
 
 def make_reads(cfg, chroms):
@@ -155,6 +218,12 @@ def make_reads(cfg, chroms):
         s = c[p:p + RL + 10].copy()
         s[s == ord("C")] = ord("T")
         seqs[i] = bytes(s)
+    if cfg.get("motifs"):
+        sites = []
+        make_genome(cfg, sites)
+        for i in pick[5 * k:5 * k + n // 5]:
+            s = motif_read(rng, chroms, sites, RL)
+            seqs[i] = bytes(synth.revcomp(s)) if rng.random() < 0.5 else bytes(s)
     if cfg["flip"]:
         # G->A libraries: read 1 is the reverse complement of the converted strand
         comp = bytes.maketrans(b"ACGTN", b"TGCAN")
@@ -194,8 +263,8 @@ def run_case(name):
         else:
             chroms = make_genome(cfg)
             synth.write_fasta(os.path.join(td, "g.fa"), chroms)
-            subprocess.run([binp, "build", "g.fa", "index.bin"], cwd=td, check=True,
-                           stdout=subprocess.DEVNULL)
+            subprocess.run([binp, "build", "g.fa", "index.bin"] + (["--no_loss"] if cfg.get("noloss") else []),
+                           cwd=td, check=True, stdout=subprocess.DEVNULL)
         seqs = make_reads(cfg, chroms)
         write_fastq(os.path.join(td, "r.fq"), seqs)
         args = [binp, "match", "index.bin", "r.fq", "out"] + (["--unord_reads"] if cfg["unord"] else [])
@@ -314,6 +383,38 @@ def make_pair_reads(pc, gcfg, chroms):
         p = int(rng.integers(100, c.size - F - 100))
         fr = _ct(c[p:p + F])
         s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+    if gcfg.get("motifs"):
+        # fragments over a planted site (either mate may hold it)
+        sites = []
+        make_genome(gcfg, sites)
+        for i in take(n // 5):
+            c, p, w = sites[int(rng.integers(0, len(sites)))]
+            g = int(rng.integers(pc["gap"][0], pc["gap"][1] + 1))
+            F = 2 * RL + g
+            s0 = p + w - int(rng.integers(w // 2, RL)) - (F - RL if rng.random() < 0.5 else 0)
+            s0 = min(max(s0, 0), chroms[c].size - F)
+            fr = chroms[c][s0:s0 + F].copy()
+            fr[fr == ord("C")] = np.where(rng.random((fr == ord("C")).sum()) < 0.9, ord("T"), ord("C"))
+            if rng.random() < 0.5:
+                s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+            else:
+                fr = synth.revcomp(fr)
+                s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+        # mate 1 starting 16 letters before an (AG)^24 run: its first k-mer
+        # ends in (AG)^8, whose code is blacklisted, and k-mer 0 is never
+        # looked up (ReadQueue.cpp:2527-2555)
+        ag = [x for x in sites if x[2] == 48 and bytes(chroms[x[0]][x[1]:x[1] + 2]) == b"AG"]
+        for i in take(n // 20):
+            c, p, w = ag[int(rng.integers(0, len(ag)))]
+            g = int(rng.integers(pc["gap"][0], pc["gap"][1] + 1))
+            L1 = int(rng.integers(short, RL + 1))
+            F = L1 + g + RL
+            s0 = p - 16
+            if s0 + F > chroms[c].size:
+                continue
+            fr = chroms[c][s0:s0 + F].copy()
+            fr[fr == ord("C")] = ord("T")
+            s1[i], s2[i] = bytes(fr[:L1]), bytes(synth.revcomp(fr[F - RL:]))
     if pc["flip"]:
         s1, s2 = s2, s1
     return s1, s2
