@@ -1920,7 +1920,7 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
         if (rr > 0 || col > 0) bad = true;
         if ((tn & 31) && tn <= FAME_DEV_TRACE_CAP) TR[(size_t)(tn >> 5) * stride] = tw;
         if (tn > FAME_DEV_TRACE_CAP || bad) {
-            atomicOr(bd.flags, FLAG_CAPACITY);
+            atomicOr(bd.flags, FLAG_DP_BROKEN);
             continue;
         }
         // CpG walk :3690-3912

@@ -621,8 +621,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     HIPC(hipMemcpyAsync(c->stats, bd.stats, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
-    if (flags & FLAG_CAPACITY) {
-        set_err(c, "alignment trace exceeded its capacity");
+    if (flags & FLAG_DP_BROKEN) {
+        set_err(c, "a banded alignment's traceback left the band (internal error)");
         return FAME_ERR_CAPACITY;
     }
     // per-call counters: the device accumulates; report the increment
@@ -806,6 +806,18 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
     return pl;
 }
 
+// FAME_POOL_CAP_LIMIT (tests only): verify pools of the fixed-capacity forms
+// (batched graphs, the allocation-free entry point) hold at most this many
+// slots, so that small inputs reach their overflow handling; 0 or unset: no
+// limit
+uint32_t pool_cap_limit(uint32_t cap)
+{
+    const char* v = getenv("FAME_POOL_CAP_LIMIT");
+    if (!v || !*v) return cap;
+    const unsigned long k = strtoul(v, nullptr, 10);
+    return k && k < cap ? (uint32_t)k : cap;
+}
+
 BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uint64_t* d_offs,
                     const uint32_t* d_lens, uint32_t ori_mask)
 {
@@ -823,7 +835,7 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
     bd.cand_nr = pl->cand_nr.as<uint32_t>();
     bd.pool_n = pl->pool_n.as<uint32_t>();
     for (int p = 0; p < FAME_NPOOL; ++p) {
-        bd.pool_cap[p] = pl->pool_cap;
+        bd.pool_cap[p] = pool_cap_limit(pl->pool_cap);
         bd.job_meta[p] = pl->job_meta[p].as<uint32_t>();
         bd.job_ro[p] = pl->job_ro[p].as<uint32_t>();
         bd.job_nm[p] = pl->job_nm[p].as<uint32_t>();
@@ -947,6 +959,11 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     // next call of either form
     HIPC(hipMemsetAsync(c->d_counters.p, 0, 24, st));
     HIPC(hipStreamSynchronize(st));
+    if (flags & FLAG_DP_BROKEN) {
+        // raised after reads were counted: an error, never a re-run
+        set_err(c, "a banded alignment's traceback left the band (internal error)");
+        return FAME_ERR_CAPACITY;
+    }
     if (flags & FLAG_ABORT) {
         c->graph_fallbacks += 1;
         if (flags & FLAG_POOL_FULL) plan_grow_pools(c, pl, pl->h_io + 8);
@@ -1047,6 +1064,10 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
         // sub-batches are carried to the end
         for (const Sub& x : round) {
             const uint32_t f = pl[x.lane]->h_io[32 + x.slot];
+            if (f & FLAG_DP_BROKEN) {
+                set_err(c, "a banded alignment's traceback left the band (internal error)");
+                return FAME_ERR_CAPACITY;
+            }
             if (!(f & FLAG_ABORT)) continue;
             subs.push_back(x);
             if (f & FLAG_POOL_FULL) plan_grow_pools(c, pl[x.lane], pl[x.lane]->h_io + 64 + 4 * x.slot);
@@ -1419,8 +1440,8 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     HIPC(hipMemcpyAsync(c->stats, bd.stats, 8 * FAME_NSTATS, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
-    if (flags & FLAG_CAPACITY) {
-        set_err(c, "alignment trace exceeded its capacity");
+    if (flags & FLAG_DP_BROKEN) {
+        set_err(c, "a banded alignment's traceback left the band (internal error)");
         return FAME_ERR_CAPACITY;
     }
     for (int k = 0; k < 5; ++k) counters[k] += cnt[k];

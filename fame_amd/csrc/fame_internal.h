@@ -35,6 +35,10 @@ enum : uint32_t {
     // any of these makes the batched form's later stages skip the batch,
     // which the host then runs on the plain path
     FLAG_ABORT = 1u | 2u | 4u | 8u | 16u,
+    // a banded alignment whose traceback left the band or its trace buffer:
+    // raised by the last stage, after other reads were counted, so it is a
+    // hard error of the call and never a reason to run the batch again
+    FLAG_DP_BROKEN = 32u,
 };
 
 // The index as resident in HBM.

@@ -166,11 +166,15 @@ def make_genome(cfg, sites=None):
 
 
 def motif_read(rng, chroms, sites, L):
-    """A read of L letters over a planted site, C->T converted at 90 %."""
+    """A read of L letters over a planted site from either strand (the
+    reverse complement taken before the conversion: a directional library),
+    C->T converted at 90 %."""
     c, p, n = sites[int(rng.integers(0, len(sites)))]
     s0 = p + n - int(rng.integers(n // 2, L))
     s0 = min(max(s0, 0), chroms[c].size - L)
     s = chroms[c][s0:s0 + L].copy()
+    if rng.random() < 0.5:
+        s = synth.revcomp(s)
     s[s == ord("C")] = np.where(rng.random((s == ord("C")).sum()) < 0.9, ord("T"), ord("C"))
     return s
 
@@ -222,8 +226,7 @@ def make_reads(cfg, chroms):
         sites = []
         make_genome(cfg, sites)
         for i in pick[5 * k:5 * k + n // 5]:
-            s = motif_read(rng, chroms, sites, RL)
-            seqs[i] = bytes(synth.revcomp(s)) if rng.random() < 0.5 else bytes(s)
+            seqs[i] = bytes(motif_read(rng, chroms, sites, RL))
     if cfg["flip"]:
         # G->A libraries: read 1 is the reverse complement of the converted strand
         comp = bytes.maketrans(b"ACGTN", b"TGCAN")
@@ -394,12 +397,10 @@ def make_pair_reads(pc, gcfg, chroms):
             s0 = p + w - int(rng.integers(w // 2, RL)) - (F - RL if rng.random() < 0.5 else 0)
             s0 = min(max(s0, 0), chroms[c].size - F)
             fr = chroms[c][s0:s0 + F].copy()
-            fr[fr == ord("C")] = np.where(rng.random((fr == ord("C")).sum()) < 0.9, ord("T"), ord("C"))
-            if rng.random() < 0.5:
-                s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
-            else:
+            if rng.random() < 0.5:  # the bottom strand, converted after the complement
                 fr = synth.revcomp(fr)
-                s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
+            fr[fr == ord("C")] = np.where(rng.random((fr == ord("C")).sum()) < 0.9, ord("T"), ord("C"))
+            s1[i], s2[i] = bytes(fr[:RL]), bytes(synth.revcomp(fr[F - RL:]))
         # mate 1 starting 16 letters before an (AG)^24 run: its first k-mer
         # ends in (AG)^8, whose code is blacklisted, and k-mer 0 is never
         # looked up (ReadQueue.cpp:2527-2555)

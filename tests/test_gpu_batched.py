@@ -90,3 +90,26 @@ def test_pipelined_lanes_match_reference(golden_case, tmp_path):
     assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
     assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
+
+
+def test_batched_overflow_reruns_plain(golden_case, monkeypatch):
+    """Verify pools capped at 256 slots (FAME_POOL_CAP_LIMIT, a test-only
+    limit): batches outgrow the graphs' fixed capacities, are detected on
+    the device before anything is counted and run again on the plain path;
+    per-read results, counters and CpG counts stay FAME's (nothing counted
+    twice), in single calls and in the pipelined lanes."""
+    monkeypatch.setenv("FAME_POOL_CAP_LIMIT", "256")
+    c = golden_case
+    print(f"case {c.name}: pools capped", flush=True)
+    for pipe in (0, 700):
+        ctx = _lib.Context(c.index, both_strands=c.unord)
+        ctx.set_form("batched")
+        if pipe:
+            ctx.set_pipeline(pipe)
+        valid, mat, cnt = stream(ctx, c, max(len(c.lens) // 2, 1))
+        info = ctx.form_info()
+        assert info["fallbacks"] > 0
+        assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
+        assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+        assert np.array_equal(ctx.meth(), c.meth)
+        ctx.close()
