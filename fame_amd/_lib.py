@@ -20,7 +20,7 @@ MAX_READ = 255
 
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",
-           "fame_match_reads_device", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
+           "fame_match_reads_device", "fame_match_reads_noalloc", "fame_set_pair_dist", "fame_match_pairs", "fame_match_pairs_device",
            "fame_decide_strand", "fame_get_meth", "fame_reset", "fame_set_form", "fame_set_pipeline", "fame_form_info", "fame_serve_open", "fame_serve_submit",
            "fame_serve_wait", "fame_serve_stats", "fame_serve_close", "fame_fastq_open", "fame_fastq_next",
            "fame_fastq_close", "fame_sc_open", "fame_sc_write_cell", "fame_sc_close",
@@ -55,6 +55,9 @@ def load():
         L.fame_match_reads_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_uint32, C.c_int, _u64p, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
+        L.fame_match_reads_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_uint32, C.c_int, _u64p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]
         L.fame_set_pair_dist.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.fame_match_pairs.argtypes = [C.c_void_p, _u8p, _u64p, _u32p, _u8p, _u64p, _u32p, C.c_uint32,
                                        C.c_int, _u64p, _u8p, _u64p, _u8p, _u64p]
@@ -130,6 +133,13 @@ class Context:
         self._check(load().fame_match_reads_device(self.h, d_seqs, d_offs, d_lens, int(n),
                                                    int(get_stranded), ptr(counters, _u64p),
                                                    d_valid or None, d_mat or None, stream or None))
+
+    def match_noalloc(self, d_seqs, d_offs, d_lens, n, get_stranded, counters, d_valid=0, d_mat=0, stream=0):
+        """fame_match_reads_noalloc: match_device without any device allocation
+        (at most 32768 reads per call, read ends within 8 MiB of d_seqs)"""
+        self._check(load().fame_match_reads_noalloc(self.h, d_seqs, d_offs, d_lens, int(n), int(get_stranded),
+                                                    ptr(counters, _u64p), d_valid or None, d_mat or None,
+                                                    stream or None))
 
     def set_pair_dist(self, minpd, maxpd):
         self._check(load().fame_set_pair_dist(self.h, int(minpd), int(maxpd)))

@@ -2108,10 +2108,12 @@ void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, ui
     hipLaunchKernelGGL(k_pack_tmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, kent, n, out);
 }
 
-void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint)
+void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint,
+                       uint32_t max_waves)
 {
     if (!n_ro) return;
-    const uint32_t waves = n_ro < FAME_SEED_WAVES ? n_ro : FAME_SEED_WAVES;
+    const uint32_t wmax = max_waves && max_waves < FAME_SEED_WAVES ? max_waves : FAME_SEED_WAVES;
+    const uint32_t waves = n_ro < wmax ? n_ro : wmax;
     const uint32_t g = (waves + FAME_SEED_WPB - 1) / FAME_SEED_WPB;
     // the variant that does not fit the batch returns at once; with a hint of
     // the longest read it gets a small grid (still complete if the hint is
@@ -2232,6 +2234,28 @@ void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hip
 void launch_batch_reset(const BatchDev& bd, hipStream_t st)
 {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, bd);
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, p, v);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st)
+{
+    if (!n) return;
+    const uint32_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero_u32, dim3(g < 1024 ? g : 1024), dim3(256), 0, st, p, n);
 }
 
 // furthest read end, one atomic per wave

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1473,6 +1474,257 @@ int guarded(fame_ctx* c, F&& f)
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// allocation-free single-end matching (fame_match_reads_noalloc)
+// ---------------------------------------------------------------------------
+
+// The per-batch buffers of the allocation-free entry point, carved from one
+// file-scope device array.  Every GPU holds its own copy until the process
+// ends.  Host code never passes this name to a kernel or a copy (on the host
+// it stands for a host-side shadow, not device memory): it uses the address
+// that hipGetSymbolAddress returned for the current device (na_base).
+static __device__ __align__(256) uint8_t scratch[FAME_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+constexpr uint32_t kNaSeedWaves = 1024;      // persistent seed waves (pool slack: 64 per wave and pool)
+constexpr uint32_t kNaPoolCap = 4 * FAME_NOALLOC_MAX_READS + 64 * kNaSeedWaves;
+constexpr uint32_t kNaOvfList = 8192;        // jobs with more than match_cap matches
+constexpr uint32_t kNaOvfBuf = 1u << 20;     // their matches
+constexpr uint32_t kNaHeavy1 = 16384, kNaHeavy1Slices = 8; // index hits of a deferred (read, orientation)
+constexpr uint32_t kNaHeavy2 = 1u << 18;     // one slice for the rare larger ones
+constexpr uint32_t kNaDpLanes = 8192;
+constexpr uint32_t kNaVerifyBlocks = 512;
+
+std::mutex g_na_mu;
+uint8_t* g_na_base[kNaMaxDevices];
+int g_na_state[kNaMaxDevices]; // 0 not looked up, 1 ready, -1 failed
+hipError_t g_na_err[kNaMaxDevices];
+
+// at fame_open, once per device, with that device current
+void noalloc_setup(int dev)
+{
+    if (dev < 0 || dev >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (g_na_state[dev] != 0) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+        g_na_base[dev] = static_cast<uint8_t*>(p);
+        g_na_state[dev] = 1;
+    } else {
+        g_na_err[dev] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+        g_na_state[dev] = -1;
+    }
+}
+
+// byte offsets of the buffers in the array (256-byte aligned), for batches of
+// up to FAME_NOALLOC_MAX_READS reads ending within FAME_NOALLOC_MAX_BYTES
+struct NaLayout {
+    size_t rev, pre, codes, cand_off, cand_nf, cand_nr, pool_n, job_meta[FAME_NPOOL], job_ro[FAME_NPOOL],
+        job_nm[FAME_NPOOL], job_match[FAME_NPOOL], job_ovf_off[FAME_NPOOL], ovf_n, ovf_pool, ovf_job, ovf_buf,
+        ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy1, heavy2, status, use, mat, valid, stats, direct,
+        dplist, methn, flags, ndev, end_word, dp, dec, trace, end;
+    HeavyScratch hs1, hs2;
+};
+
+NaLayout na_layout(uint32_t match_cap)
+{
+    NaLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t N = FAME_NOALLOC_MAX_READS, nro = 2 * N;
+    L.rev = take(FAME_NOALLOC_MAX_BYTES + 64);
+    L.pre = take(N + 1);
+    L.codes = take(nro * 256 + 256);
+    L.cand_off = take(4 * nro + 4);
+    L.cand_nf = take(4 * nro + 4);
+    L.cand_nr = take(4 * nro + 4);
+    L.pool_n = take(4 * FAME_NPOOL);
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        L.job_meta[p] = take(4ull * kNaPoolCap);
+        L.job_ro[p] = take(4ull * kNaPoolCap);
+        L.job_nm[p] = take(4ull * kNaPoolCap);
+        L.job_match[p] = take(4ull * kNaPoolCap * match_cap);
+        L.job_ovf_off[p] = take(4ull * kNaPoolCap);
+    }
+    L.ovf_n = take(8);
+    L.ovf_pool = take(4ull * kNaOvfList);
+    L.ovf_job = take(4ull * kNaOvfList);
+    L.ovf_buf = take(4ull * kNaOvfBuf);
+    L.ovf_total = take(8);
+    L.heavy_n = take(8);
+    L.heavy_ro = take(4 * nro);
+    L.heavy_e = take(4 * nro);
+    L.heavy_bc = take(4 * nro);
+    auto dims = [](HeavyScratch& hs, uint32_t emax, uint32_t lo, uint32_t hi) {
+        hs.cap_e = emax;
+        hs.hsize = 2 * emax;
+        hs.tb_cap = 2 * (2 * emax) + 64;
+        hs.stride = heavy_scratch_stride(hs);
+        hs.e_lo = lo;
+        hs.e_hi = hi;
+    };
+    dims(L.hs1, kNaHeavy1, 0, kNaHeavy1);
+    // the second tier also takes the items beyond its capacity, which fail
+    // there (FLAG_CAPACITY) and abort the batch before anything is counted
+    dims(L.hs2, kNaHeavy2, kNaHeavy1 + 1, 0xFFFFFFFFu);
+    L.heavy1 = take(L.hs1.stride * kNaHeavy1Slices);
+    L.heavy2 = take(L.hs2.stride);
+    L.status = take(N + 1);
+    L.use = take(N + 1);
+    L.mat = take(8 * N + 8);
+    L.valid = take(N + 1);
+    L.stats = take(8 * FAME_NSTATS);
+    L.direct = take(4 * N + 4);
+    L.dplist = take(4 * N + 4);
+    L.methn = take(8);
+    L.flags = take(16);
+    L.ndev = take(8);
+    L.end_word = take(8);
+    const size_t cells = (size_t)FAME_DP_SROWS * (2 * (FAME_DPBAND + 1) + 1);
+    L.dp = take(2 * cells * kNaDpLanes);
+    L.dec = take(8ull * (FAME_DEV_MAX_READ + 1) * kNaDpLanes);
+    L.trace = take(8ull * FAME_DP_TRACE_WORDS * kNaDpLanes);
+    L.end = o;
+    return L;
+}
+
+// One batch of at most FAME_NOALLOC_MAX_READS reads through the batched
+// form's kernels (counts on the device, fixed capacities, launched directly).
+// 1 = the batch outgrew a capacity; it was detected before anything was
+// counted (select and the CpG stage skip an aborted batch).
+int na_run(fame_ctx* c, uint8_t* base, const NaLayout& L, const uint8_t* d_seqs, const uint64_t* d_offs,
+           const uint32_t* d_lens, uint32_t n, int get_stranded, uint64_t counters[3], uint8_t* d_valid,
+           uint64_t* d_mat, hipStream_t st)
+{
+    auto at = [&](size_t off) { return base + off; };
+    const bool o0 = c->both || get_stranded || c->match_r1_fwd;
+    const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
+    BatchDev bd{};
+    uint32_t* ndev = (uint32_t*)at(L.ndev);
+    bd.n_dev = ndev;
+    bd.seqs = d_seqs;
+    bd.offs = d_offs;
+    bd.lens = d_lens;
+    bd.rev = at(L.rev);
+    bd.pre = at(L.pre);
+    bd.codes = at(L.codes);
+    bd.ori_mask = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
+    bd.cand_off = (uint32_t*)at(L.cand_off);
+    bd.cand_nf = (uint32_t*)at(L.cand_nf);
+    bd.cand_nr = (uint32_t*)at(L.cand_nr);
+    bd.pool_n = (uint32_t*)at(L.pool_n);
+    for (int p = 0; p < FAME_NPOOL; ++p) {
+        bd.pool_cap[p] = pool_cap_limit(kNaPoolCap);
+        bd.job_meta[p] = (uint32_t*)at(L.job_meta[p]);
+        bd.job_ro[p] = (uint32_t*)at(L.job_ro[p]);
+        bd.job_nm[p] = (uint32_t*)at(L.job_nm[p]);
+        bd.job_match[p] = (uint32_t*)at(L.job_match[p]);
+        bd.job_ovf_off[p] = (uint32_t*)at(L.job_ovf_off[p]);
+    }
+    bd.match_cap = c->match_cap;
+    bd.ovf_n = (uint32_t*)at(L.ovf_n);
+    bd.ovf_pool = (uint32_t*)at(L.ovf_pool);
+    bd.ovf_job = (uint32_t*)at(L.ovf_job);
+    bd.ovf_list_cap = kNaOvfList;
+    bd.ovf_buf = (uint32_t*)at(L.ovf_buf);
+    bd.ovf_buf_cap = kNaOvfBuf;
+    bd.heavy_n = (uint32_t*)at(L.heavy_n);
+    bd.heavy_ro = (uint32_t*)at(L.heavy_ro);
+    bd.heavy_e = (uint32_t*)at(L.heavy_e);
+    bd.heavy_bc = (uint32_t*)at(L.heavy_bc);
+    bd.heavy_cap = 2 * FAME_NOALLOC_MAX_READS;
+    bd.status = at(L.status);
+    bd.use = at(L.use);
+    bd.mat = (uint64_t*)at(L.mat);
+    bd.valid = at(L.valid);
+    bd.counters = c->d_counters.as<uint64_t>();
+    bd.meth = c->d_meth.as<uint32_t>();
+    bd.direct_list = (uint32_t*)at(L.direct);
+    bd.dp_list = (uint32_t*)at(L.dplist);
+    bd.direct_n = (uint32_t*)at(L.methn);
+    bd.dp_n = bd.direct_n + 1;
+    bd.flags = (uint32_t*)at(L.flags);
+    bd.max_len = bd.flags + 1;
+    bd.stats = (unsigned long long*)at(L.stats);
+    HeavyScratch h1 = L.hs1, h2 = L.hs2;
+    h1.base = at(L.heavy1);
+    h2.base = at(L.heavy2);
+    // the array keeps the last call's contents: the counters, lists and flags
+    // that must start at zero are cleared by kernels (k_batch_reset, and the
+    // context's per-call counters below)
+    launch_set_u32(ndev, n, st);
+    launch_zero_u32((uint32_t*)c->d_counters.p, 6, st);
+    launch_batch_reset(bd, st);
+    launch_prep(bd, n, c->readlen, st);
+    launch_seed_light(c->ix, bd, 2 * n, st, 0, kNaSeedWaves);
+    launch_seed_heavy_dev(c->ix, bd, h1, kNaHeavy1Slices, st);
+    launch_seed_heavy_dev(c->ix, bd, h2, 1, st);
+    for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, nullptr, st, kNaVerifyBlocks);
+    launch_ovf_offsets(bd, FAME_JOBS_ON_DEVICE, (uint32_t*)at(L.ovf_total), st);
+    for (int p = 0; p < FAME_NPOOL; ++p) launch_verify(c->ix, bd, p, FAME_JOBS_ON_DEVICE, bd.ovf_job, st, 64);
+    launch_select(c->ix, bd, n, get_stranded, st);
+    launch_meth(c->ix, bd, n, (uint16_t*)at(L.dp), (uint64_t*)at(L.dec), (uint64_t*)at(L.trace), kNaDpLanes, st);
+    uint32_t flags = 0;
+    uint64_t cnt[3] = {0, 0, 0};
+    HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(cnt, c->d_counters.p, 24, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    launch_zero_u32((uint32_t*)c->d_counters.p, 6, st);
+    if (flags & FLAG_DP_BROKEN) {
+        HIPC(hipStreamSynchronize(st));
+        set_err(c, "a banded alignment's traceback left the band (internal error)");
+        return FAME_ERR_CAPACITY;
+    }
+    if (flags & FLAG_TOO_LONG) {
+        HIPC(hipStreamSynchronize(st));
+        set_err(c, "a read is longer than FAME_MAX_READ");
+        return FAME_ERR_ARG;
+    }
+    if (flags & FLAG_ABORT) {
+        HIPC(hipStreamSynchronize(st));
+        return 1;
+    }
+    counters[0] += cnt[0];
+    counters[1] += cnt[1];
+    counters[2] += cnt[2];
+    if (d_valid) HIPC(hipMemcpyAsync(d_valid, bd.valid, n, hipMemcpyDeviceToDevice, st));
+    if (d_mat) HIPC(hipMemcpyAsync(d_mat, bd.mat, 8ull * n, hipMemcpyDeviceToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    return FAME_OK;
+}
+
+// a batch that outgrew a capacity is matched again in halves (nothing of it
+// was counted); a single read that still does not fit is an error
+int na_match(fame_ctx* c, uint8_t* base, const NaLayout& L, const uint8_t* d_seqs, const uint64_t* d_offs,
+             const uint32_t* d_lens, uint32_t first, uint32_t n, int get_stranded, uint64_t counters[3],
+             uint8_t* d_valid, uint64_t* d_mat, hipStream_t st)
+{
+    const int rc = na_run(c, base, L, d_seqs, d_offs + first, d_lens + first, n, get_stranded, counters,
+                          d_valid ? d_valid + first : nullptr, d_mat ? d_mat + first : nullptr, st);
+    if (rc != 1) return rc;
+    c->graph_fallbacks += 1;
+    if (n == 1) {
+        set_err(c, "read " + std::to_string(first) +
+                       " exceeds the allocation-free limits (index hits or candidate windows); the reads before it "
+                       "were matched and counted");
+        return FAME_ERR_CAPACITY;
+    }
+    const uint32_t h = n / 2;
+    const int r1 = na_match(c, base, L, d_seqs, d_offs, d_lens, first, h, get_stranded, counters, d_valid, d_mat, st);
+    if (r1 != FAME_OK) return r1;
+    return na_match(c, base, L, d_seqs, d_offs, d_lens, first + h, n - h, get_stranded, counters, d_valid, d_mat,
+                    st);
+}
+
+} // namespace
+
 // hooks of the serving layer (fame_serve.cpp)
 namespace fame {
 
@@ -1510,6 +1762,7 @@ int fame_open(const char* index_path, int both_strands, fame_ctx** out)
         HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         c->meth_threads = meth_dp_lanes();
+        noalloc_setup(c->device);
         load_index(c.get(), index_path);
         return FAME_OK;
     });
@@ -1610,6 +1863,59 @@ int fame_match_reads_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* 
         if (n && d_mat) HIPC(hipMemcpyAsync(d_mat, om, 8ull * n, hipMemcpyDeviceToDevice, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
+    });
+}
+
+int fame_match_reads_noalloc(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens,
+                             uint32_t n, int get_stranded, uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat,
+                             void* stream)
+{
+    if (!c || !counters || (n && (!d_seqs || !d_offs || !d_lens))) return FAME_ERR_ARG;
+    if (!c->d_tab.p) {
+        set_err(c, "context has no index");
+        return FAME_ERR_ARG;
+    }
+    if (n > FAME_NOALLOC_MAX_READS) {
+        set_err(c, "fame_match_reads_noalloc takes at most FAME_NOALLOC_MAX_READS (" +
+                       std::to_string(FAME_NOALLOC_MAX_READS) + ") reads, got " + std::to_string(n));
+        return FAME_ERR_ARG;
+    }
+    const int dev = c->device;
+    if (dev < 0 || dev >= kNaMaxDevices || g_na_state[dev] != 1) {
+        set_err(c, std::string("allocation-free scratch unavailable on this device: ") +
+                       (dev >= 0 && dev < kNaMaxDevices && g_na_state[dev] < 0 ? hipGetErrorString(g_na_err[dev])
+                                                                               : "not set up"));
+        return FAME_ERR_HIP;
+    }
+    if (!n) return FAME_OK;
+    return guarded(c, [&]() {
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        int cur = -1;
+        HIPC(hipGetDevice(&cur));
+        if (cur != dev) {
+            set_err(c, "the context's device is not the current device");
+            return (int)FAME_ERR_ARG;
+        }
+        const NaLayout L = na_layout(c->match_cap);
+        if (L.end > FAME_NOALLOC_SCRATCH_BYTES) {
+            set_err(c, "allocation-free layout exceeds its scratch array");
+            return (int)FAME_ERR_CAPACITY;
+        }
+        uint8_t* base = g_na_base[dev];
+        // the reverse complements sit at the reads' own offsets: the furthest
+        // read end must fit the array's copy buffer
+        uint32_t* ew = (uint32_t*)(base + L.end_word);
+        launch_zero_u32(ew, 2, st);
+        launch_max_end(d_offs, d_lens, n, (unsigned long long*)ew, st);
+        unsigned long long e = 0;
+        HIPC(hipMemcpyAsync(&e, ew, 8, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (e > FAME_NOALLOC_MAX_BYTES) {
+            set_err(c, "reads end " + std::to_string(e) + " bytes into d_seqs; fame_match_reads_noalloc takes at most "
+                       "FAME_NOALLOC_MAX_BYTES (" + std::to_string(FAME_NOALLOC_MAX_BYTES) + ")");
+            return (int)FAME_ERR_ARG;
+        }
+        return na_match(c, base, L, d_seqs, d_offs, d_lens, 0, n, get_stranded, counters, d_valid, d_mat, st);
     });
 }
 

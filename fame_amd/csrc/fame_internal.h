@@ -220,7 +220,14 @@ void launch_batch_reset(const BatchDev& bd, hipStream_t st);
 // `blocks` scratch slices of hs
 void launch_seed_heavy_dev(const DevIndex& ix, const BatchDev& bd, const HeavyScratch& hs, uint32_t blocks,
                            hipStream_t st);
-void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint = 0);
+// max_waves: persistent waves at most (0 = FAME_SEED_WAVES); each wave may
+// leave up to 63 verify slots unused per pool it writes
+void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hipStream_t st, uint32_t lmax_hint = 0,
+                       uint32_t max_waves = 0);
+// *p = v on the stream (a kernel: no host staging)
+void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st);
+// p[0 .. n) = 0 on the stream (a kernel)
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st);
 // shortest and longest of n lengths (out[0] must start at ~0, out[1] at 0)
 void launch_len_range(const uint32_t* lens, uint32_t n, uint32_t* out, hipStream_t st);
 // plain form: blocks (at most hs.count) stride over items first .. first + count - 1

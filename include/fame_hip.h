@@ -76,6 +76,29 @@ int fame_match_reads_device(fame_ctx* ctx, const uint8_t* d_seqs, const uint64_t
                             uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat,
                             void* stream);
 
+/* No counterpart in the reference: fame_match_reads_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or other allocating routine: its per-batch buffers are carved
+ * from one file-scope device array of FAME_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  Limits: at most FAME_NOALLOC_MAX_READS reads
+ * per call, every read ending within FAME_NOALLOC_MAX_BYTES of d_seqs
+ * (FAME_ERR_ARG otherwise, nothing done).  A batch that outgrows one of the
+ * array's capacities (verify pools, deferred reads, match overflow) is
+ * detected before anything is counted and matched again in halves; a
+ * single read that still does not fit (more than 2^18 index hits in one
+ * orientation) returns FAME_ERR_CAPACITY, with the reads before it
+ * matched and counted.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other contexts or threads) race and must
+ * be serialised by the caller.  The context's device must be current.
+ * Results are those of fame_match_reads_device. */
+#define FAME_NOALLOC_MAX_READS 32768u
+#define FAME_NOALLOC_MAX_BYTES (8ull << 20)
+#define FAME_NOALLOC_SCRATCH_BYTES (112ull << 20)
+int fame_match_reads_noalloc(fame_ctx* ctx, const uint8_t* d_seqs, const uint64_t* d_offs,
+                             const uint32_t* d_lens, uint32_t n, int get_stranded,
+                             uint64_t counters[3], uint8_t* d_valid, uint64_t* d_mat, void* stream);
+
 /* MINPDIST / MAXPDIST (CONST.h:41-42, compile-time constants in the
  * reference; defaults 50 / 550 here).  Only MAXPDIST enters the matching:
  * contextWLen = MAXPDIST / WINLEN + 1 (ReadQueue.cpp:2854, 4540, 4607) and the
