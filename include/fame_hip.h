@@ -2,8 +2,9 @@
  *
  * The reference's matching boundary is a C++ member API (FischerJo/FAME
  * ReadQueue.h:58-129, RefGenome.h:55).  Each entry point below replaces one
- * of its members; the C++ mirror in fame_amd/csrc/readqueue.hpp and the
- * Python mirror in fame_amd/readqueue.py call only these functions.
+ * of its members; the C++ drop-in (fame_amd/dropin/ReadQueue.h and
+ * RefGenome.h, linked with FAME's own main.cpp) and the Python mirror in
+ * fame_amd/readqueue.py call only these functions.
  *
  * Conventions: integer status codes (FAME_OK = 0, negative on error, message
  * in fame_last_error); caller-owned host buffers; device memory owned by the

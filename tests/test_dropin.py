@@ -1,0 +1,139 @@
+"""FAME's own, unmodified main.cpp linked against libfamehip.so and
+libfameindex.so through the RefGenome.h / ReadQueue.h of fame_amd/dropin:
+the drop-in as a FAME maintainer builds it (fame_amd/dropin/Makefile).
+
+CPU: its --store_index mode (FAME's FASTA reader, our index builder) writes
+the index FAME itself wrote for the fixture genome.  GPU: its --load_index
+modes (single-end, paired, single cell, --unord_reads) print FAME's counters
+and write FAME's CpG report / single-cell matrix byte for byte."""
+import gzip
+import json
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, read_fastq_gz
+from fame_amd.dropin import configs
+from test_index_builder import assert_same_index, parse
+
+BIN = os.path.join(ROOT, "fame_amd", "dropin", "_bin")
+
+
+def binary(htab, readlen, nchrom, chunk, cut, minpd=50, maxpd=550):
+    p = os.path.join(BIN, configs.name(htab, readlen, nchrom, minpd, maxpd, chunk, cut))
+    if not os.path.exists(p):
+        pytest.skip(f"drop-in binary {os.path.basename(p)} not built (needs FAME's sources at build time)")
+    return p
+
+
+def unpack_index(index_dir, td):
+    for f in ("index.bin", "index.bin_strands"):
+        with gzip.open(os.path.join(GOLDEN, index_dir, f + ".gz"), "rb") as a, open(os.path.join(td, f), "wb") as b:
+            shutil.copyfileobj(a, b)
+    return os.path.join(td, "index.bin")
+
+
+def header(path):
+    d = parse(path)
+    htab, readlen, winlen, kmerlen, cutoff, seedbits = d["hdr"]
+    return htab, readlen, len(d["chroms"]), cutoff
+
+
+def write_fastq(path, seqs, offs, lens):
+    flat = seqs.tobytes()
+    with open(path, "wb") as f:
+        for i in range(len(lens)):
+            s = flat[int(offs[i]):int(offs[i]) + int(lens[i])]
+            f.write(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)))
+
+
+def run(args, cwd):
+    r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("name", ["se_basic", "se_filt", "se_noloss", "se_repeats"])
+def test_dropin_store_index_equals_fame(name, tmp_path):
+    td = str(tmp_path)
+    meta = json.load(open(os.path.join(GOLDEN, name, "case.json")))
+    ref = unpack_index(meta["index_dir"], td)
+    a = parse(ref)
+    htab, readlen, nchrom, cut = header(ref)
+    exe = binary(htab, readlen, nchrom, meta["chunk"], cut)
+    with open(os.path.join(td, "g.fa"), "wb") as f:
+        for i, c in enumerate(a["chroms"]):
+            f.write(b">" + a["chrmap"][i].encode() + b"\n")
+            for k in range(0, len(c), 80):
+                f.write(c[k:k + 80] + b"\n")
+    noloss = bool(meta.get("noloss"))
+    run([exe, "--genome", "g.fa", "--store_index", "mine.idx"] + (["--no_loss"] if noloss else []), td)
+    assert_same_index(a, parse(os.path.join(td, "mine.idx")), noloss)
+
+
+def counters(out, paired):
+    if not paired:
+        m = re.search(r"Successfully matched: (\d+) / Unsuccessfully matched: (\d+) / Nonunique matches: (\d+)", out)
+        return [int(m.group(1)), int(m.group(3)), int(m.group(2))]
+    g = lambda k: int(re.search(k + r": (?:\(2\*\))?(\d+)", out).group(1))  # noqa: E731
+    return [g("Overall successfully matched"), g("Nonunique matches"), g("Unsuccessfully matched"),
+            g("Fully matched pairs"), g("Reads discarded as too short")]
+
+
+SE = ["se_basic", "se_ga_chunked", "se_filt", "se_noloss", "se_repeats", "se_unord", "se_long", "se_rl150"]
+PE = ["pe_basic", "pe_ga_chunked", "pe_filt", "pe_noloss", "pe_repeats", "pe_unord", "pe_rl150"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", SE + PE)
+def test_dropin_load_index_matches_fame(name, tmp_path):
+    td = str(tmp_path)
+    meta = json.load(open(os.path.join(GOLDEN, name, "case.json")))
+    idx = unpack_index(meta["index_dir"], td)
+    htab, readlen, nchrom, cut = header(idx)
+    paired = meta.get("mode") == "pe"
+    r = np.load(os.path.join(GOLDEN, name, "reads.npz"))
+    args = ["--load_index", "index.bin", "-o", "out"] + (["--unord_reads"] if meta["unord"] else [])
+    if paired:
+        exe = binary(htab, readlen, nchrom, meta["chunk"], cut, meta["minpd"], meta["maxpd"])
+        write_fastq(os.path.join(td, "r1.fq"), r["seqs1"], r["offs1"], r["lens1"])
+        write_fastq(os.path.join(td, "r2.fq"), r["seqs2"], r["offs2"], r["lens2"])
+        args += ["-r1", "r1.fq", "-r2", "r2.fq"]
+        want = [meta["reference"][k] for k in ("succ", "nonunique", "unsucc", "succ_paired", "too_short")]
+    else:
+        exe = binary(htab, readlen, nchrom, meta["chunk"], cut)
+        write_fastq(os.path.join(td, "r.fq"), r["seqs"], r["offs"], r["lens"])
+        args += ["-r", "r.fq"]
+        want = [meta["reference"][k] for k in ("succ", "nonunique", "unsucc")]
+    print(f"case {name}: {os.path.basename(exe)} {' '.join(args)}", flush=True)
+    out = run([exe] + args, td)
+    assert counters(out, paired) == want
+    got = open(os.path.join(td, "out_cpg.tsv"), "rb").read()
+    assert got == gzip.open(os.path.join(GOLDEN, name, "cpg.tsv.gz")).read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sc_se", "sc_pe"])
+def test_dropin_single_cell_matches_fame(name, tmp_path):
+    td = str(tmp_path)
+    d = os.path.join(GOLDEN, name)
+    meta = json.load(open(os.path.join(d, "case.json")))
+    idx = unpack_index(meta["index_dir"], td)
+    htab, readlen, nchrom, cut = header(idx)
+    if meta["pe"]:
+        exe = binary(htab, readlen, nchrom, meta["chunk"], cut, meta["minpd"], meta["maxpd"])
+    else:
+        exe = binary(htab, readlen, nchrom, meta["chunk"], cut)
+    for f in os.listdir(d):
+        if f.endswith(".fq.gz") or f == "cells.txt":
+            shutil.copy(os.path.join(d, f), os.path.join(td, f))
+    args = [exe, "--load_index", "index.bin", "--sc_input", "cells.txt", "--sc_output", "sc_out.txt",
+            "--gzip_reads", "-o", "out"] + (["--paired"] if meta["pe"] else []) + \
+        (["--unord_reads"] if meta["unord"] else [])
+    print(f"case {name}: {' '.join(args[1:])}", flush=True)
+    run(args, td)
+    assert open(os.path.join(td, "sc_out.txt"), "rb").read() == gzip.open(os.path.join(d, "sc_out.txt.gz")).read()
