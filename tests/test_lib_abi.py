@@ -31,3 +31,14 @@ def test_missing_index_is_an_error_not_a_crash():
     assert rc != 0
     if h.value:
         lib.fame_close(h.value)
+
+
+def test_index_library_exports_every_symbol():
+    """include/fame_index.h is the C ABI of libfameindex.so (the builder)."""
+    from fame_amd import index as fidx
+    src = open(os.path.join(ROOT, "include", "fame_index.h")).read()
+    names = sorted(set(re.findall(r"^\s*int\s+(fame_\w+)\s*\(", src, re.M)))
+    assert names == ["fame_build_index", "fame_build_index_fasta"]
+    lib = ctypes.CDLL(fidx.LIB_PATH)
+    for name in names:
+        assert hasattr(lib, name), name
