@@ -1030,23 +1030,23 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint4 win = ix.win[id];
     const uint32_t startPos = win.x, chrom = win.y;
     const uint64_t clen = ix.clen[chrom];
-    const uint8_t* cs = ix.seq + ix.coff[chrom];
     const uint64_t span = (uint64_t)ix.winlen + MISCOUNT + ADDMIS - 1;
-    const uint8_t* text;
-    uint64_t tlen;
+    // the text: chromosome positions [tstart, tstart + tlen), scanned
+    // forward, or backward and complemented for the reverse map
+    uint64_t tstart, tlen;
     if (!rev) {
         uint64_t end = (uint64_t)startPos + span;
         if (end > clen) end = clen;
-        text = cs + startPos;
+        tstart = startPos;
         tlen = end - startPos;
     } else {
         const uint64_t e0 = startPos > 0 ? (uint64_t)startPos - 1 : 0;
         uint64_t s0 = (uint64_t)startPos + span;
         if (s0 >= clen) s0 = clen - 1;
-        text = cs + e0 + 1;
+        tstart = e0 + 1;
         tlen = s0 > e0 ? s0 - e0 : 0;
         if (unguarded && startPos == 0) {
-            text = cs;
+            tstart = 0;
             tlen = s0 + 1;
         }
     }
@@ -1077,7 +1077,7 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         if (ovf) ovf[k] = runbest;
         else if (k < cap) outm[k] = runbest;
     };
-    auto step = [&](uint32_t c, int64_t p) {
+    auto step = [&](uint32_t c, int p) {
         myers_step<W>(Pv, Mv, peq + c * (W * 256), score);
         ++ncomp;
         const bool valid = ncomp >= minc; // earlier positions leave the run state alone
@@ -1095,84 +1095,77 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         prevErrs = upd ? e : prevErrs;
         was = valid ? acc : was;
     };
-    // the text is read 16 bytes at a time from 16-byte aligned addresses
-    // (the sequence buffer is padded so that the last chunk stays inside it);
-    // the reverse strand is scanned from its end.  Scan chunk ci is memory
-    // chunk ci (forward) or nchunk-1-ci (reverse).
-    const uintptr_t ta = (uintptr_t)text;
-    const int64_t skip = (int64_t)(ta & 15);
-    // pointer arithmetic from the index's sequence pointer keeps the loads
-    // in the global address space (a round trip through an integer would
-    // make them flat loads, which also count against LDS waits)
-    const uint8_t* a0 = text - skip;
-    const int64_t nchunk = (skip + (int64_t)tlen + 15) >> 4;
+    // The text comes from the packed copy of the genome: one 64-bit word per
+    // 16 letters of a chromosome (ix.pk, from ix.pkoff[chrom]), two bits per
+    // letter (A 0, C 1, G 2, T 3; letter j at bits 2j) and an N mask in bits
+    // 32-47 (every byte other than A, C, G, T, as FAME's reader writes N for
+    // them).  Scan chunk ci is memory chunk ci (forward) or nchunk-1-ci
+    // (reverse, whose letters are reversed and complemented: A<->T, C<->G,
+    // ShiftAnd.h:197-211).  Positions are 32-bit (texts are at most a few
+    // windows long).
+    const int T = (int)tlen;
+    const int skip = (int)(tstart & 15);
+    const uint64_t* __restrict__ pk = ix.pk + ix.pkoff[chrom] + (tstart >> 4);
+    const int nchunk = (skip + T + 15) >> 4;
     struct Chunk {
-        uint32_t wd[4], cw[4], nflag;
-        int lo, hi;     // scan-order range of text bytes in the chunk
-        int64_t pb, pd; // text position of scan-order byte 0, and its step
+        uint32_t x, nm; // codes in scan order (letter ii at bits 2ii), N bits of the text letters
+        int lo, hi;     // scan-order range of text letters in the chunk
+        int pb, pd;     // text position of scan-order letter 0, and its step
     };
-    // the 16 bytes of scan chunk ci (zeros beyond the last chunk)
-    auto raw_chunk = [&](int64_t ci) -> uint4 {
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (ci >= 0 && ci < nchunk) v = *(const uint4*)(a0 + 16 * (rev ? nchunk - 1 - ci : ci));
+    // the packed word of scan chunk ci (zero beyond the last chunk)
+    auto raw_chunk = [&](int ci) -> uint64_t {
+        uint64_t v = 0;
+        if (ci >= 0 && ci < nchunk) v = pk[rev ? nchunk - 1 - ci : ci];
         return v;
     };
     // the same, for 0 <= ci (indexes past the end read the last chunk)
-    auto raw_chunk_c = [&](int64_t ci) -> uint4 {
-        const int64_t c = ci < nchunk ? ci : nchunk - 1;
-        return *(const uint4*)(a0 + 16 * (rev ? nchunk - 1 - c : c));
+    auto raw_chunk_c = [&](int ci) -> uint64_t {
+        const int c = ci < nchunk ? ci : nchunk - 1;
+        return pk[rev ? nchunk - 1 - c : c];
     };
-    auto load_chunk = [&](int64_t ci, const uint4 v, Chunk& k) {
-        const int64_t chunk = rev ? nchunk - 1 - ci : ci;
+    auto load_chunk = [&](int ci, const uint64_t v, Chunk& k) {
+        const int chunk = rev ? nchunk - 1 - ci : ci;
+        uint32_t x = (uint32_t)v, nmk = (uint32_t)(v >> 32) & 0xFFFFu;
         if (rev) {
-            k.wd[0] = __builtin_bswap32(v.w);
-            k.wd[1] = __builtin_bswap32(v.z);
-            k.wd[2] = __builtin_bswap32(v.y);
-            k.wd[3] = __builtin_bswap32(v.x);
-        } else {
-            k.wd[0] = v.x;
-            k.wd[1] = v.y;
-            k.wd[2] = v.z;
-            k.wd[3] = v.w;
+            // letter order reversed (bit reversal, then the two bits of each
+            // letter swapped back) and complemented
+            const uint32_t y = __brev(x);
+            x = ~(((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1));
+            nmk = __brev(nmk) >> 16;
         }
-        const int64_t base = 16 * chunk - skip; // text position of the chunk's first memory byte
-        const int mlo = base < 0 ? (int)(-base) : 0;
-        const int64_t rem = (int64_t)tlen - base;
-        const int mhi = rem < 16 ? (int)rem : 16;
+        const int base = 16 * chunk - skip; // text position of the chunk's first memory letter
+        const int mlo = base < 0 ? -base : 0;
+        const int rem = T - base;
+        const int mhi = rem < 16 ? rem : 16;
         k.lo = rev ? 16 - mhi : mlo;
         k.hi = rev ? 16 - mlo : mhi;
-        // A0 C1 G2 T3 = ((b >> 1) ^ (b >> 2)) & 3 per byte, complemented on
-        // the reverse strand (ShiftAnd.h:197-211); 'N' bytes flagged
-        k.nflag = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            k.cw[q] = (((k.wd[q] >> 1) ^ (k.wd[q] >> 2)) & 0x03030303u) ^ (rev ? 0x03030303u : 0u);
-            const uint32_t t = k.wd[q] ^ 0x4E4E4E4Eu;
-            k.nflag |= (t - 0x01010101u) & ~t & 0x80808080u;
-        }
+        k.x = x;
+        k.nm = nmk & ((0xFFFFu >> (16 - k.hi)) & (0xFFFFu << k.lo)); // N among the text letters only
         k.pb = rev ? base + 15 : base;
         k.pd = rev ? -1 : 1;
     };
+    auto reset = [&]() {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            Pv[w] = ~0ull;
+            Mv[w] = 0;
+        }
+        score = score0;
+    };
     // the bit-vector scan over one chunk
     auto scan_chunk = [&](const Chunk& k) {
-        if (k.lo == 0 && k.hi == 16 && k.nflag == 0) {
+        if (k.lo == 0 && k.hi == 16 && k.nm == 0) {
 #pragma unroll
-            for (int ii = 0; ii < 16; ++ii) step((k.cw[ii >> 2] >> (8 * (ii & 3))) & 3u, k.pb + k.pd * ii);
+            for (int ii = 0; ii < 16; ++ii) step((k.x >> (2 * ii)) & 3u, k.pb + k.pd * ii);
         } else {
 #pragma unroll
             for (int ii = 0; ii < 16; ++ii) {
                 if (ii < k.lo || ii >= k.hi) continue;
-                const uint32_t b = (k.wd[ii >> 2] >> (8 * (ii & 3))) & 0xFFu;
-                if (b == 'N') { // the automaton restarts (ShiftAnd.h:142-146, 212-214)
-#pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        Pv[w] = ~0ull;
-                        Mv[w] = 0;
-                    }
-                    score = score0;
+                if ((k.nm >> ii) & 1u) { // an N restarts the automaton (ShiftAnd.h:142-146, 212-214)
+                    reset();
                     continue;
                 }
-                step((k.cw[ii >> 2] >> (8 * (ii & 3))) & 3u, k.pb + k.pd * ii);
+                step((k.x >> (2 * ii)) & 3u, k.pb + k.pd * ii);
             }
         }
     };
@@ -1182,46 +1175,41 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     for (int q = 0; q < FILT_MASKW; ++q) cmask[q] = 0;
     if (filt && nchunk > 64 * FILT_MASKW) filt = false;
     if (filt) {
-        int64_t x = 0; // scan index of the next text byte
+        int x = 0; // scan index of the next text letter
         // marks the scan chunks holding scan positions [s0, s1]
-        auto mark_range = [&](int64_t s0, int64_t s1) {
+        auto mark_range = [&](int s0, int s1) {
             if (s0 < 0) s0 = 0;
-            if (s1 > (int64_t)tlen - 1) s1 = (int64_t)tlen - 1;
+            if (s1 > T - 1) s1 = T - 1;
             if (s0 > s1) return;
-            int64_t c0, c1;
+            int c0, c1;
             if (!rev) {
                 c0 = (s0 + skip) >> 4;
                 c1 = (s1 + skip) >> 4;
             } else {
-                c0 = nchunk - 1 - (((int64_t)tlen - 1 - s0 + skip) >> 4);
-                c1 = nchunk - 1 - (((int64_t)tlen - 1 - s1 + skip) >> 4);
+                c0 = nchunk - 1 - ((T - 1 - s0 + skip) >> 4);
+                c1 = nchunk - 1 - ((T - 1 - s1 + skip) >> 4);
             }
 #pragma unroll
             for (int q2 = 0; q2 < FILT_MASKW; ++q2) {
-                const int64_t lo = c0 > 64 * q2 ? c0 : 64 * q2;
-                const int64_t hi = c1 < 64 * q2 + 63 ? c1 : 64 * q2 + 63;
+                const int lo = c0 > 64 * q2 ? c0 : 64 * q2;
+                const int hi = c1 < 64 * q2 + 63 ? c1 : 64 * q2 + 63;
                 if (lo <= hi) cmask[q2] |= (~0ull >> (63 - (hi - lo))) << (lo - 64 * q2);
             }
         };
         // Per text letter q: the 16 letters ending at q, top-aligned, from
-        // this chunk's and the previous chunk's packed letters (2 bits each,
+        // this chunk's and the previous chunk's reduced codes (2 bits each,
         // A 0, G 2, C/T 3, scan order from bit 0), looked up in the piece
         // table.  Letters outside the window only ever add spurious hits.
-        // The scan of a chunk is short next to a random load: chunks go in
-        // blocks of two, the next block loaded while this one is scanned
-        // (the empty asm keeps the loads where they are issued).
+        // Chunks go in blocks of two, the next block loaded while this one is
+        // worked on (the empty asm keeps the loads where they are issued).
         uint32_t pprev = 0;
         const uint32_t tbase = threadIdx.x * 8u;
-        auto filter_chunk = [&](int64_t ci, const uint4 v) -> bool {
+        auto filter_chunk = [&](int ci, const uint64_t v) -> bool {
             Chunk k;
             load_chunk(ci, v, k);
-            if (k.nflag) return false; // an N restarts the automaton: scan the window whole
-            uint32_t P = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t r = k.cw[q] | ((k.cw[q] << 1) & 0x02020202u);
-                P |= ((r | (r >> 6) | (r >> 12) | (r >> 18)) & 0xFFu) << (8 * q);
-            }
+            if (k.nm) return false; // an N restarts the automaton: scan the window whole
+            // reduced codes: C (1) and T (3) both become 3
+            const uint32_t P = k.x | ((k.x & 0x55555555u) << 1);
             // byte offset of letter ii's bucket in the table: bucket * 2048
             // (= bits 29-31 of the product moved to bits 11-13) | lane * 8.
             // The masked code XOR an entry is its piece index (< 16) on a
@@ -1233,42 +1221,43 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 const uint32_t xm = xc & tmask;
                 return min(xm ^ e.x, xm ^ e.y);
             };
-            // lowest piece index among the chunk's hits, and 15 - the highest
-            uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
-            if (k.lo == 0 && k.hi == 16) {
+            // any hit at the chunk's 16 positions (the common case is none);
+            // the range of text letters and the piece indexes are looked at
+            // only then
+            uint32_t any = 0xFFFFFFFFu;
+            uint32_t r[16];
 #pragma unroll
-                for (int ii = 0; ii < 16; ++ii) {
-                    const uint32_t r = probe(ii);
-                    alo = min(alo, r);
-                    ahi = min(ahi, r ^ 15u);
-                }
-            } else {
-#pragma unroll
-                for (int ii = 0; ii < 16; ++ii) {
-                    const uint32_t r = probe(ii);
-                    const bool in = ii >= k.lo && ii < k.hi;
-                    alo = in ? min(alo, r) : alo;
-                    ahi = in ? min(ahi, r ^ 15u) : ahi;
-                }
+            for (int ii = 0; ii < 16; ++ii) {
+                r[ii] = probe(ii);
+                any = min(any, r[ii]);
             }
             pprev = P;
+            uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
+            if (any < 16u) {
+                // lowest piece index among the chunk's hits, and 15 - the highest
+#pragma unroll
+                for (int ii = 0; ii < 16; ++ii) {
+                    const bool in = ii >= k.lo && ii < k.hi;
+                    alo = in ? min(alo, r[ii]) : alo;
+                    ahi = in ? min(ahi, r[ii] ^ 15u) : ahi;
+                }
+            }
             if (alo < 16u) {
                 // pieces ilo..ihi end in this chunk (every piece when codes
                 // repeat): the union of their regions.  Piece i ending at
                 // scan position e starts the match at >= e - (i+1)*FP + 1 -
                 // NERR and ends it at <= e - (i+1)*FP + m + NERR.
-                const int64_t ilo = pdup ? 0 : (int64_t)alo;
-                const int64_t ihi = pdup ? (int64_t)NP - 1 : 15 - (int64_t)ahi;
-                mark_range(x - (ihi + 1) * FP + 1 - (int64_t)NERR,
-                           x + (k.hi - k.lo) - 1 - (ilo + 1) * FP + (int64_t)m + (int64_t)NERR);
+                const int ilo = pdup ? 0 : (int)alo;
+                const int ihi = pdup ? NP - 1 : 15 - (int)ahi;
+                mark_range(x - (ihi + 1) * FP + 1 - (int)NERR, x + (k.hi - k.lo) - 1 - (ilo + 1) * FP + (int)m + (int)NERR);
             }
             x += k.hi - k.lo;
             return true;
         };
         if (nchunk > 0) {
-            uint4 c1 = raw_chunk_c(0), c2 = raw_chunk_c(1);
-            for (int64_t ci = 0; ci < nchunk; ci += 2) {
-                const uint4 n1 = raw_chunk_c(ci + 2), n2 = raw_chunk_c(ci + 3);
+            uint64_t c1 = raw_chunk_c(0), c2 = raw_chunk_c(1);
+            for (int ci = 0; ci < nchunk; ci += 2) {
+                const uint64_t n1 = raw_chunk_c(ci + 2), n2 = raw_chunk_c(ci + 3);
                 asm volatile("" ::: "memory");
                 if (!filter_chunk(ci, c1) || (ci + 1 < nchunk && !filter_chunk(ci + 1, c2))) {
                     filt = false;
@@ -1282,8 +1271,8 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     if (filt) {
         // the marked chunks in scan order, the next one loaded ahead; a fresh
         // automaton at the start of every run of consecutive chunks
-        auto next_marked = [&]() -> int64_t {
-            int64_t ci = -1;
+        auto next_marked = [&]() -> int {
+            int ci = -1;
 #pragma unroll
             for (int q = 0; q < FILT_MASKW; ++q)
                 if (ci < 0 && cmask[q]) {
@@ -1292,37 +1281,32 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
                 }
             return ci;
         };
-        int64_t prevci = -2;
-        int64_t ci = next_marked();
-        uint4 v1 = raw_chunk(ci);
+        int prevci = -2;
+        int ci = next_marked();
+        uint64_t v1 = raw_chunk(ci);
         while (ci >= 0) {
-            const int64_t cn = next_marked();
-            const uint4 v = v1;
+            const int cn = next_marked();
+            const uint64_t v = v1;
             v1 = raw_chunk(cn);
             asm volatile("" ::: "memory");
             if (ci != prevci + 1) {
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    Pv[w] = ~0ull;
-                    Mv[w] = 0;
-                }
-                score = score0;
+                reset();
                 if (was) flush();
                 was = false;
             }
             prevci = ci;
             Chunk k;
             load_chunk(ci, v, k);
-            // scan index of the chunk's first text byte (no N in the window)
-            const int64_t p0 = k.pb + k.pd * k.lo;
-            ncomp = (uint32_t)(rev ? (int64_t)tlen - 1 - p0 : p0);
+            // scan index of the chunk's first text letter (no N in the window)
+            const int p0 = k.pb + k.pd * k.lo;
+            ncomp = (uint32_t)(rev ? T - 1 - p0 : p0);
             scan_chunk(k);
             ci = cn;
         }
     } else {
-        uint4 v1 = raw_chunk(0);
-        for (int64_t ci = 0; ci < nchunk; ++ci) {
-            const uint4 v = v1;
+        uint64_t v1 = raw_chunk(0);
+        for (int ci = 0; ci < nchunk; ++ci) {
+            const uint64_t v = v1;
             v1 = raw_chunk(ci + 1);
             asm volatile("" ::: "memory");
             Chunk k;
@@ -2072,6 +2056,31 @@ __global__ void k_build_slots(const uint32_t* __restrict__ tab, const uint32_t* 
     }
 }
 
+// the packed text of the verify stage: word w holds letters 16w .. 16w+15 of
+// the chromosome, two bits each, and the mask of those that are not A, C, G
+// or T; letters past the end are 0 and unmasked
+__global__ void k_pack_seq(const uint8_t* __restrict__ seq, uint64_t len, uint64_t nwords, uint64_t* __restrict__ out)
+{
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint32_t x = 0, nm = 0;
+    for (uint32_t j = 0; j < 16; ++j) {
+        const uint64_t p = 16 * w + j;
+        if (p >= len) break;
+        const uint8_t b = seq[p];
+        uint32_t c;
+        switch (b) {
+        case 'A': c = 0; break;
+        case 'C': c = 1; break;
+        case 'G': c = 2; break;
+        case 'T': c = 3; break;
+        default: c = 0; nm |= 1u << j; break;
+        }
+        x |= c << (2 * j);
+    }
+    out[w] = ((uint64_t)nm << 32) | x;
+}
+
 __global__ void k_meth_u16(const uint32_t* __restrict__ in, uint64_t n, uint16_t* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2300,6 +2309,12 @@ void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab
 {
     if (!htab) return;
     hipLaunchKernelGGL(k_build_slots, dim3(16384), dim3(256), 0, st, tab, kent, htab, slot, bad);
+}
+
+void launch_pack_seq(const uint8_t* seq, uint64_t len, uint64_t nwords, uint64_t* out, hipStream_t st)
+{
+    if (!nwords) return;
+    hipLaunchKernelGGL(k_pack_seq, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, st, seq, len, nwords, out);
 }
 
 void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st)

@@ -83,7 +83,8 @@ struct fame_ctx {
     std::vector<uint8_t> cpg_chrom;
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
-    DevBuf d_tab, d_slot, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom;
+    DevBuf d_tab, d_slot, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom, d_pk,
+        d_pkoff;
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
@@ -223,6 +224,22 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(c->d_clen.reserve(8 * c->nchrom + 8));
         HIPC(hipMemcpy(c->d_coff.p, coff.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
         HIPC(hipMemcpy(c->d_clen.p, clen.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
+        // the verify stage's packed copy: 16 letters per word, two words of
+        // padding after each chromosome
+        std::vector<uint64_t> pkoff(c->nchrom);
+        uint64_t words = 0;
+        for (uint64_t i = 0; i < c->nchrom; ++i) {
+            pkoff[i] = words;
+            words += (clen[i] + 15) / 16 + 2;
+        }
+        HIPC(c->d_pk.reserve(8 * words + 8));
+        HIPC(hipMemset(c->d_pk.p, 0, 8 * words + 8));
+        HIPC(c->d_pkoff.reserve(8 * c->nchrom + 8));
+        HIPC(hipMemcpy(c->d_pkoff.p, pkoff.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
+        for (uint64_t i = 0; i < c->nchrom; ++i)
+            launch_pack_seq(c->d_seq.as<uint8_t>() + coff[i], clen[i], (clen[i] + 15) / 16,
+                            c->d_pk.as<uint64_t>() + pkoff[i], st);
+        HIPC(hipStreamSynchronize(st));
     }
     // tabIndex: u64[htab + 1], narrowed to u32 when every prefix fits
     const uint64_t tablen = f.get<uint64_t>();
@@ -363,6 +380,8 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.seq = c->d_seq.as<uint8_t>();
     ix.coff = c->d_coff.as<uint64_t>();
     ix.clen = c->d_clen.as<uint64_t>();
+    ix.pk = c->d_pk.as<uint64_t>();
+    ix.pkoff = c->d_pkoff.as<uint64_t>();
     ix.cpg_pos = c->d_cpg_pos.as<uint32_t>();
     ix.cpg_chrom = c->d_cpg_chrom.as<uint8_t>();
     ix.readlen = c->readlen;

@@ -52,6 +52,7 @@ enum : uint32_t {
 //            looks up a 32-bit k-mer code there, ReadQueue.cpp:2527)
 //   win      metaWindow as {startPos, chrom, startInd, endInd}
 //   seq      all chromosomes back to back, coff / clen per chromosome
+//   pk       the same packed two bits per letter with an N mask (verify)
 //   cpg_pos  cpgTable[i].pos, cpg_chrom cpgTable[i].chrom
 struct DevIndex {
     const void* tab;
@@ -71,6 +72,12 @@ struct DevIndex {
     const uint8_t* seq;
     const uint64_t* coff;
     const uint64_t* clen;
+    // the verify stage's copy of the chromosomes: per 16 letters one u64,
+    // letter j's code (A 0, C 1, G 2, T 3) at bits 2j, bit 32 + j set when
+    // letter j is not A, C, G or T; chromosome c from word pkoff[c], two
+    // words of padding after each
+    const uint64_t* pk;
+    const uint64_t* pkoff;
     const uint32_t* cpg_pos;
     const uint8_t* cpg_chrom;
     uint32_t readlen, winlen, seedbits;
@@ -265,6 +272,8 @@ void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, ui
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,
                        hipStream_t st);
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
+// nwords packed words (16 letters each) of one chromosome of len letters
+void launch_pack_seq(const uint8_t* seq, uint64_t len, uint64_t nwords, uint64_t* out, hipStream_t st);
 void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab, uint64_t* slot, uint32_t* bad,
                         hipStream_t st);
 void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st);
