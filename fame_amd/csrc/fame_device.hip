@@ -1204,68 +1204,91 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         // worked on (the empty asm keeps the loads where they are issued).
         uint32_t pprev = 0;
         const uint32_t tbase = threadIdx.x * 8u;
-        auto filter_chunk = [&](int ci, const uint64_t v) -> bool {
-            Chunk k;
-            load_chunk(ci, v, k);
-            if (k.nm) return false; // an N restarts the automaton: scan the window whole
-            // reduced codes: C (1) and T (3) both become 3
-            const uint32_t P = k.x | ((k.x & 0x55555555u) << 1);
-            // byte offset of letter ii's bucket in the table: bucket * 2048
-            // (= bits 29-31 of the product moved to bits 11-13) | lane * 8.
-            // The masked code XOR an entry is its piece index (< 16) on a
-            // hit, at least 16 otherwise.
-            auto probe = [&](int ii) -> uint32_t {
-                const uint32_t xc = ii < 15 ? __builtin_amdgcn_alignbit(P, pprev, 2 * (ii + 1)) : P;
-                const uint32_t off = ((mul24u(xc >> 12, hmul) >> 18) & 0x3800u) | tbase;
-                const uint2 e = *(const uint2*)((const char*)s_ptab + off);
-                const uint32_t xm = xc & tmask;
-                return min(xm ^ e.x, xm ^ e.y);
-            };
-            // any hit at the chunk's 16 positions (the common case is none);
-            // the range of text letters and the piece indexes are looked at
-            // only then
+        // byte offset of letter ii's bucket in the table: bucket * 2048
+        // (= bits 29-31 of the product moved to bits 11-13) | lane * 8.
+        // The masked code XOR an entry is its piece index (< 16) on a hit, at
+        // least 16 otherwise.
+        auto probe = [&](uint32_t P, int ii) -> uint32_t {
+            const uint32_t xc = ii < 15 ? __builtin_amdgcn_alignbit(P, pprev, 2 * (ii + 1)) : P;
+            const uint32_t off = ((mul24u(xc >> 12, hmul) >> 18) & 0x3800u) | tbase;
+            const uint2 e = *(const uint2*)((const char*)s_ptab + off);
+            const uint32_t xm = xc & tmask;
+            return min(xm ^ e.x, xm ^ e.y);
+        };
+        // the chunk's 16 positions (scan-order letters lo .. hi-1 are text);
+        // x is the scan index of letter lo.  Any hit (the common case is
+        // none) leads to the piece indexes and the regions they mark.
+        auto probe_chunk = [&](uint32_t P, int lo, int hi) {
             uint32_t any = 0xFFFFFFFFu;
             uint32_t r[16];
 #pragma unroll
             for (int ii = 0; ii < 16; ++ii) {
-                r[ii] = probe(ii);
+                r[ii] = probe(P, ii);
                 any = min(any, r[ii]);
             }
             pprev = P;
-            uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
             if (any < 16u) {
-                // lowest piece index among the chunk's hits, and 15 - the highest
+                // lowest piece index among the hits at text letters, and 15 -
+                // the highest
+                uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
 #pragma unroll
                 for (int ii = 0; ii < 16; ++ii) {
-                    const bool in = ii >= k.lo && ii < k.hi;
+                    const bool in = ii >= lo && ii < hi;
                     alo = in ? min(alo, r[ii]) : alo;
                     ahi = in ? min(ahi, r[ii] ^ 15u) : ahi;
                 }
+                if (alo < 16u) {
+                    // pieces ilo..ihi end in this chunk (every piece when codes
+                    // repeat): the union of their regions.  Piece i ending at
+                    // scan position e starts the match at >= e - (i+1)*FP + 1 -
+                    // NERR and ends it at <= e - (i+1)*FP + m + NERR.
+                    const int ilo = pdup ? 0 : (int)alo;
+                    const int ihi = pdup ? NP - 1 : 15 - (int)ahi;
+                    mark_range(x - (ihi + 1) * FP + 1 - (int)NERR,
+                               x + (hi - lo) - 1 - (ilo + 1) * FP + (int)m + (int)NERR);
+                }
             }
-            if (alo < 16u) {
-                // pieces ilo..ihi end in this chunk (every piece when codes
-                // repeat): the union of their regions.  Piece i ending at
-                // scan position e starts the match at >= e - (i+1)*FP + 1 -
-                // NERR and ends it at <= e - (i+1)*FP + m + NERR.
-                const int ilo = pdup ? 0 : (int)alo;
-                const int ihi = pdup ? NP - 1 : 15 - (int)ahi;
-                mark_range(x - (ihi + 1) * FP + 1 - (int)NERR, x + (k.hi - k.lo) - 1 - (ilo + 1) * FP + (int)m + (int)NERR);
-            }
-            x += k.hi - k.lo;
+            x += hi - lo;
+        };
+        // reduced codes (C and T both 3) of a chunk's letters in scan order
+        auto reduced = [](uint32_t c) { return c | ((c & 0x55555555u) << 1); };
+        // a chunk with text letters outside [lo, hi) or an N: the general
+        // form; false when the window holds an N
+        auto filter_chunk = [&](int ci, const uint64_t v) -> bool {
+            Chunk k;
+            load_chunk(ci, v, k);
+            if (k.nm) return false; // an N restarts the automaton: scan the window whole
+            probe_chunk(reduced(k.x), k.lo, k.hi);
             return true;
         };
-        if (nchunk > 0) {
-            uint64_t c1 = raw_chunk_c(0), c2 = raw_chunk_c(1);
-            for (int ci = 0; ci < nchunk; ci += 2) {
-                const uint64_t n1 = raw_chunk_c(ci + 2), n2 = raw_chunk_c(ci + 3);
+        if (nchunk > 0 && filter_chunk(0, raw_chunk_c(0))) {
+            // interior chunks are whole: their words follow one another in
+            // memory (backwards for the reverse map), two loaded ahead
+            const int last = nchunk - 1;
+            const uint64_t* wp = pk + (rev ? last - 1 : 1);
+            const int dw = rev ? -1 : 1;
+            uint64_t c1 = last > 1 ? wp[0] : 0, c2 = last > 2 ? wp[dw] : 0;
+            for (int ci = 1; ci < last; ++ci) {
+                const uint64_t v = c1;
+                c1 = c2;
+                if (ci + 2 < last) c2 = wp[(ci + 1) * dw];
                 asm volatile("" ::: "memory");
-                if (!filter_chunk(ci, c1) || (ci + 1 < nchunk && !filter_chunk(ci + 1, c2))) {
+                uint32_t xw = (uint32_t)v;
+                uint32_t nmk = (uint32_t)(v >> 32) & 0xFFFFu;
+                // reverse map: letter order reversed, complemented (any N
+                // bit means the same whichever way round)
+                const uint32_t y = __brev(xw);
+                const uint32_t xr = ~(((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1));
+                xw = rev ? xr : xw;
+                if (nmk) {
                     filt = false;
                     break;
                 }
-                c1 = n1;
-                c2 = n2;
+                probe_chunk(reduced(xw), 0, 16);
             }
+            if (filt && last > 0 && !filter_chunk(last, raw_chunk_c(last))) filt = false;
+        } else {
+            filt = false;
         }
     }
     if (filt) {
