@@ -848,10 +848,24 @@ __device__ __forceinline__ uint32_t mul24u(uint32_t a, uint32_t b) { return (uin
 // product of its last 10 letters (bits 12-31) with the multiplier
 __device__ __forceinline__ uint32_t piece_bucket(uint32_t x, uint32_t mul) { return mul24u(x >> 12, mul) >> 29; }
 
+// A job whose filter marked chunks can hand its runs of marked chunks to the
+// block's run queue (k_verify): verify_one then stops after the filter and
+// returns the job's text context and chunk mask here.  ctx.x = index of the
+// text's memory chunk 0 in ix.pk; ctx.y = T | skip << 13 | rev << 17 |
+// m << 18.
+struct VerifyDefer {
+    bool queue;        // hand the runs over (false: scan in place)
+    bool deferred;     // the filter marked nr runs: ctx and cm hold the job
+    uint32_t nr;
+    uint2 ctx;
+    uint64_t cm[4];
+};
+
 template <int W>
 __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDev& bd, uint32_t pool,
                                                const uint32_t* __restrict__ only, uint32_t tid,
-                                               uint64_t* __restrict__ s_peq, uint2* __restrict__ s_ptab)
+                                               uint64_t* __restrict__ s_peq, uint2* __restrict__ s_ptab,
+                                               VerifyDefer& df)
 {
     if (only && bd.ovf_pool[tid] != pool) return 0;
     const uint32_t j = only ? only[tid] : tid;
@@ -1291,6 +1305,23 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
             filt = false;
         }
     }
+    if (filt && df.queue) {
+        // the runs of marked chunks go to the block's queue (the caller
+        // scans them, and stores the job's matches)
+        uint32_t nr = 0;
+#pragma unroll
+        for (int q = 0; q < FILT_MASKW; ++q) {
+            const uint64_t prevw = q ? cmask[q - 1] : 0ull;
+            nr += (uint32_t)__popcll(cmask[q] & ~((cmask[q] << 1) | (prevw >> 63)));
+        }
+        df.deferred = true;
+        df.nr = nr;
+        df.ctx = make_uint2((uint32_t)(pk - ix.pk),
+                            (uint32_t)T | ((uint32_t)skip << 13) | ((rev ? 1u : 0u) << 17) | (m << 18));
+#pragma unroll
+        for (int q = 0; q < FILT_MASKW; ++q) df.cm[q] = cmask[q];
+        return tlen ? tlen : 1;
+    }
     if (filt) {
         // the marked chunks in scan order, the next one loaded ahead; a fresh
         // automaton at the start of every run of consecutive chunks
@@ -1353,15 +1384,123 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     return tlen ? tlen : 1;
 }
 
+// One run of marked chunks (scan chunks c0 .. c0+len-1) of another lane's job,
+// from the job's context (VerifyDefer::ctx) and its Peq masks (column owner of
+// s_peq): the same bit-vector scan and run logic as verify_one, from a fresh
+// automaton.  Returns the matches found; res[0 .. cap) holds the first of them.
 template <int W>
-__global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
+__device__ __forceinline__ uint32_t scan_run(const DevIndex& ix, uint2 ctx, const uint64_t* __restrict__ s_peq,
+                                         uint32_t owner, int c0, int len, uint32_t cap, uint32_t* __restrict__ res)
+{
+    const uint64_t* __restrict__ pk = ix.pk + ctx.x;
+    const int T = (int)(ctx.y & 0x1FFFu);
+    const int skip = (int)((ctx.y >> 13) & 15u);
+    const bool rev = ((ctx.y >> 17) & 1u) != 0;
+    const uint32_t m = ctx.y >> 18;
+    const int nchunk = (skip + T + 15) >> 4;
+    const int z = 64 * W - (int)m;
+    const uint64_t* peq = s_peq + owner;
+    uint64_t Pv[W], Mv[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        Pv[w] = ~0ull;
+        Mv[w] = 0;
+    }
+    const int thr = (int)NERR + z;
+    int score = 64 * W;
+    bool was = false;
+    uint32_t prevErrs = MISCOUNT + 1;
+    uint32_t ncomp = 0;
+    const uint32_t minc = m - NERR;
+    uint32_t nm = 0;
+    uint32_t runbest = 0;
+    auto flush = [&]() {
+        const uint32_t k = nm - 1;
+        if (k < cap) res[k] = runbest;
+    };
+    auto step = [&](uint32_t c, int p) {
+        myers_step<W>(Pv, Mv, peq + c * (W * 256), score);
+        ++ncomp;
+        const bool valid = ncomp >= minc;
+        const bool acc = valid && score <= thr;
+        const uint32_t e = (uint32_t)(score - z);
+        const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
+        const uint32_t packed = (pos << 8) | e;
+        const bool newrun = acc && !was;
+        const bool upd = acc && (newrun || e <= prevErrs);
+        if (valid && !acc && was) flush();
+        nm += newrun ? 1u : 0u;
+        runbest = upd ? packed : runbest;
+        prevErrs = upd ? e : prevErrs;
+        was = valid ? acc : was;
+    };
+    auto word = [&](int ci) -> uint64_t { return pk[rev ? nchunk - 1 - ci : ci]; };
+    const int c1 = c0 + len;
+    uint64_t vn = word(c0);
+    for (int ci = c0; ci < c1; ++ci) {
+        const uint64_t v = vn;
+        if (ci + 1 < c1) vn = word(ci + 1);
+        asm volatile("" ::: "memory");
+        const int chunk = rev ? nchunk - 1 - ci : ci;
+        uint32_t x = (uint32_t)v;
+        if (rev) {
+            const uint32_t y = __brev(x);
+            x = ~(((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1));
+        }
+        const int base = 16 * chunk - skip;
+        const int mlo = base < 0 ? -base : 0;
+        const int rem = T - base;
+        const int mhi = rem < 16 ? rem : 16;
+        const int lo = rev ? 16 - mhi : mlo, hi = rev ? 16 - mlo : mhi;
+        const int pb = rev ? base + 15 : base, pd = rev ? -1 : 1;
+        const int p0 = pb + pd * lo;
+        ncomp = (uint32_t)(rev ? T - 1 - p0 : p0);
+        if (lo == 0 && hi == 16) {
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) step((x >> (2 * ii)) & 3u, pb + pd * ii);
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) {
+                if (ii < lo || ii >= hi) continue;
+                step((x >> (2 * ii)) & 3u, pb + pd * ii);
+            }
+        }
+    }
+    if (was) flush();
+    return nm;
+}
+
+// The first chunk at or after `from` whose bit in the chunk mask is `set`
+// (64 * FILT_MASKW if none); indexes the mask words with constants only.
+__device__ __forceinline__ int cmask_first(const uint64_t (&cm)[4], bool set, int from)
+{
+    int r = 64 * FILT_MASKW;
+    const int fq = from >> 6;
+#pragma unroll
+    for (int q = FILT_MASKW - 1; q >= 0; --q) {
+        uint64_t w = set ? cm[q] : ~cm[q];
+        if (q == fq) w &= ~0ull << (from & 63);
+        if (q >= fq && w) r = 64 * q + __ffsll((long long)w) - 1;
+    }
+    return r;
+}
+
+// Jobs of at least two pattern words run in block rounds of 256: every lane
+// prepares and filters its job; the runs of marked chunks then go to a
+// queue in the LDS the filters' tables used (long runs from its top, short
+// ones from its bottom), which the block's waves take 64 runs at a time,
+// long ones first, so that a wave's scan no longer lasts as long as its
+// lane with the most marked chunks; each owner finally gathers its runs'
+// matches in scan order.  A run's matches do not depend on the others' (each
+// starts from a fresh automaton), so the results equal verify_one's.
+// Overflow re-verification (`only`) and one-word patterns scan in place.
+template <int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
                                                 const uint32_t* __restrict__ only, uint32_t n_only)
 {
+    // exactly 32 KiB of LDS for two pattern words: five blocks per CU
     __shared__ uint64_t s_peq[4 * W * 256];
     __shared__ uint2 s_ptab[W >= 2 ? 8 * 256 : 1];
-    __shared__ unsigned long long s_work[2];
-    if (threadIdx.x < 2) s_work[threadIdx.x] = 0;
-    __syncthreads();
     // plain form: one thread per job (the host sizes the grid); batched form:
     // persistent blocks over the device's job count, nothing after an abort
     uint32_t njobs;
@@ -1369,17 +1508,125 @@ __global__ __launch_bounds__(256) void k_verify(DevIndex ix, BatchDev bd, uint32
     else njobs = min(bd.pool_n[pool], pc_get_cap(bd, pool));
     if (bd.n_dev && (*bd.flags & FLAG_ABORT)) njobs = 0;
     uint64_t jobs = 0, bytes = 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < njobs; t += gridDim.x * blockDim.x) {
-        const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab);
-        jobs += b ? 1 : 0;
-        bytes += b;
+    if (W < 2 || only) {
+        VerifyDefer df{};
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < njobs; t += gridDim.x * blockDim.x) {
+            const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab, df);
+            jobs += b ? 1 : 0;
+            bytes += b;
+        }
+    } else {
+        // the queue in s_ptab: owners' contexts [256] (uint2), their first
+        // result slot [256] (u16), run descriptors [Q] (u32: owner | first
+        // chunk << 8 | (length - 1) << 16 | index of the run in its job << 24),
+        // results [Q][1 + cap] (u32: count, matches), and in its last 16
+        // bytes (bucket 7 of lanes 254 and 255 while the tables live) the
+        // counters: result slots taken, short runs, long runs, runs handed out
+        const uint32_t cap = bd.match_cap;
+        uint2* s_ctx = s_ptab;
+        uint16_t* s_rb = (uint16_t*)(s_ptab + 256);
+        uint32_t* s_desc = (uint32_t*)(s_ptab + 320);
+        uint32_t* s_q = (uint32_t*)(s_ptab + 8 * 256 - 2);
+        constexpr uint32_t QBYTES = 8 * 256 * 8 - 320 * 8 - 16;
+        const uint32_t Qmax = QBYTES / (4 + 4 * (1 + cap));
+        uint32_t* s_resv = s_desc + Qmax;
+        uint32_t Q = Qmax < 4096 ? Qmax : 4096;
+        if (bd.verify_runs_cap && bd.verify_runs_cap < Q) Q = bd.verify_runs_cap;
+        constexpr int LONG_RUN = 12; // chunks
+        const int lane = threadIdx.x & 63;
+        // a job whose runs did not fit the queue is verified again, in
+        // place, in a repeat of its round in which only such jobs run
+        bool redo = false, again = false;
+        for (uint32_t base = blockIdx.x * 256; base < njobs; base += again ? 0u : gridDim.x * 256) {
+            const uint32_t t = base + threadIdx.x;
+            VerifyDefer df{};
+            df.queue = !redo;
+            if (t < njobs && redo == again) {
+                const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab, df);
+                jobs += redo ? 0 : (b ? 1 : 0);
+                bytes += redo ? 0 : b;
+            }
+            // this lane's table is dead: lanes 254 and 255 clear the counters
+            if (threadIdx.x >= 254) *(uint2*)(s_q + 2 * (threadIdx.x - 254)) = make_uint2(0u, 0u);
+            __syncthreads(); // the tables are dead: the queue is written
+            uint32_t rb = 0;
+            if (df.deferred) {
+                rb = atomicAdd(&s_q[0], df.nr);
+                redo = rb + df.nr > Q;
+            } else {
+                redo = false;
+            }
+            if (df.deferred && !redo) {
+                s_ctx[threadIdx.x] = df.ctx;
+                s_rb[threadIdx.x] = (uint16_t)rb;
+                // runs in scan order: first set bit, first clear bit after it
+                uint32_t k = 0;
+                for (int c0 = cmask_first(df.cm, true, 0); c0 < 64 * FILT_MASKW; ++k) {
+                    const int c1 = cmask_first(df.cm, false, c0);
+                    const int len = c1 - c0;
+                    const uint32_t pos = len > LONG_RUN ? Q - 1 - atomicAdd(&s_q[2], 1u) : atomicAdd(&s_q[1], 1u);
+                    s_desc[pos] = threadIdx.x | ((uint32_t)c0 << 8) | ((uint32_t)(len - 1) << 16) | (k << 24);
+                    c0 = cmask_first(df.cm, true, c1);
+                }
+            }
+            __syncthreads(); // the queue is complete
+            again = s_q[0] > Q; // some job's runs did not fit
+            {
+                const uint32_t ns = s_q[1], nl = s_q[2], nq = ns + nl;
+                for (;;) {
+                    uint32_t g = 0;
+                    if (lane == 0) g = atomicAdd(&s_q[3], 64u);
+                    g = (uint32_t)__builtin_amdgcn_readfirstlane((int)lane_u32(g, 0));
+                    if (g >= nq) break;
+                    const uint32_t v = g + lane;
+                    if (v < nq) {
+                        const uint32_t pos = v < nl ? Q - 1 - v : v - nl;
+                        const uint32_t d = s_desc[pos];
+                        const uint32_t owner = d & 0xFFu;
+                        uint32_t* r = s_resv + (size_t)((uint32_t)s_rb[owner] + (d >> 24)) * (1 + cap);
+                        r[0] = scan_run<W>(ix, s_ctx[owner], s_peq, owner, (int)((d >> 8) & 0xFFu),
+                                           (int)((d >> 16) & 0xFFu) + 1, cap, r + 1);
+                    }
+                }
+            }
+            __syncthreads(); // every run is scanned
+            if (df.deferred && !redo) {
+                // the owner's matches in scan order, then as verify_one ends
+                const uint32_t j = t;
+                uint32_t* outm = bd.job_match[pool] + (size_t)j * cap;
+                uint32_t nm = 0;
+                for (uint32_t k = 0; k < df.nr; ++k) {
+                    const uint32_t* r = s_resv + (size_t)(rb + k) * (1 + cap);
+                    const uint32_t cnt = r[0];
+                    for (uint32_t q = 0; q < cnt && q < cap; ++q)
+                        if (nm + q < cap) outm[nm + q] = r[1 + q];
+                    nm += cnt;
+                }
+                bd.job_nm[pool][j] = nm;
+                if (nm > cap) {
+                    const uint32_t o = atomicAdd(bd.ovf_n, 1u);
+                    if (o < bd.ovf_list_cap) {
+                        bd.ovf_pool[o] = pool;
+                        bd.ovf_job[o] = j;
+                    } else {
+                        atomicOr(bd.flags, FLAG_OVF_FULL);
+                    }
+                }
+            }
+            __syncthreads(); // the queue's LDS is free for the next round's tables
+        }
     }
-    if (jobs && !only) {
-        atomicAdd(&s_work[0], (unsigned long long)jobs);
-        atomicAdd(&s_work[1], (unsigned long long)bytes);
+    if (!only) {
+        // one atomic per wave
+        for (int o = 32; o > 0; o >>= 1) {
+            jobs += __shfl_xor(jobs, o, 64);
+            bytes += __shfl_xor(bytes, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && jobs) {
+            atomicAdd(&bd.stats[3], (unsigned long long)jobs);
+            atomicAdd(&bd.stats[4], (unsigned long long)bytes);
+        }
     }
-    __syncthreads();
-    if (threadIdx.x < 2 && s_work[threadIdx.x]) atomicAdd(&bd.stats[3 + threadIdx.x], s_work[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------

@@ -420,6 +420,17 @@ struct Timer {
     }
 };
 
+// FAME_VERIFY_QUEUE_LIMIT (tests only): verify's per-block run queue takes
+// at most this many runs per round of 256 jobs, so that small inputs reach
+// the in-place re-verification of jobs whose runs do not fit; 0 or unset:
+// what the queue's LDS holds
+uint32_t verify_queue_limit()
+{
+    const char* v = getenv("FAME_VERIFY_QUEUE_LIMIT");
+    if (!v || !*v) return 0;
+    return (uint32_t)strtoul(v, nullptr, 10);
+}
+
 int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
               uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st)
 {
@@ -441,6 +452,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.cand_nr = grow<uint32_t>(c->b_cand_nr, n_ro + 1);
     bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
     bd.match_cap = c->match_cap;
+    bd.verify_runs_cap = verify_queue_limit();
     // room for the slots persistent seed waves leave unused at chunk ends
     const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n) + 64u * FAME_SEED_WAVES;
     for (int p = 0; p < FAME_NPOOL; ++p)
@@ -863,6 +875,7 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
         bd.job_ovf_off[p] = pl->job_ovf_off[p].as<uint32_t>();
     }
     bd.match_cap = c->match_cap;
+    bd.verify_runs_cap = verify_queue_limit();
     bd.ovf_n = pl->ovf_n.as<uint32_t>();
     bd.ovf_pool = pl->ovf_pool.as<uint32_t>();
     bd.ovf_job = pl->ovf_job.as<uint32_t>();
@@ -1239,6 +1252,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.codes = grow<uint8_t>(c->b_codes, (size_t)2 * nr * 256 + 256);
     bd.pool_n = grow<uint32_t>(c->b_pool_n, FAME_NPOOL);
     bd.match_cap = c->match_cap;
+    bd.verify_runs_cap = verify_queue_limit();
     const uint32_t seed_grid = std::min<uint32_t>(std::max<uint32_t>(nitems, 1), 8192);
     const uint32_t want_cap = std::max<uint32_t>(1u << 16, 4 * n) + 64u * seed_grid;
     for (int p = 0; p < FAME_NPOOL; ++p) c->pool_cap[p] = std::max(c->pool_cap[p], want_cap);
@@ -1648,6 +1662,7 @@ int na_run(fame_ctx* c, uint8_t* base, const NaLayout& L, const uint8_t* d_seqs,
         bd.job_ovf_off[p] = (uint32_t*)at(L.job_ovf_off[p]);
     }
     bd.match_cap = c->match_cap;
+    bd.verify_runs_cap = verify_queue_limit();
     bd.ovf_n = (uint32_t*)at(L.ovf_n);
     bd.ovf_pool = (uint32_t*)at(L.ovf_pool);
     bd.ovf_job = (uint32_t*)at(L.ovf_job);

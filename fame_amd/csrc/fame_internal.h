@@ -108,6 +108,7 @@ struct BatchDev {
     uint32_t* job_match[FAME_NPOOL]; // first match_cap matches: pos << 8 | err
     uint32_t* job_ovf_off[FAME_NPOOL];
     uint32_t match_cap;
+    uint32_t verify_runs_cap; // verify's run queue: runs per block round (0: what its LDS holds)
     uint32_t* ovf_n;
     uint32_t* ovf_pool;
     uint32_t* ovf_job;

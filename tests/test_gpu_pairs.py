@@ -93,3 +93,18 @@ def test_hip_pairs_wave_walks_match_reference(golden_pair, monkeypatch, walk_env
     assert cnt.tolist() == c.counters
     assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
+
+
+def test_hip_pairs_verify_queue_full_match_reference(golden_pair, monkeypatch):
+    """Verify's run queue held to one run per round of 256 jobs
+    (FAME_VERIFY_QUEUE_LIMIT, test-only): mates' jobs re-verified in place
+    give FAME's paired results."""
+    monkeypatch.setenv("FAME_VERIFY_QUEUE_LIMIT", "1")
+    c = golden_pair
+    print(f"case {c.name}: verify queue of 1 run", flush=True)
+    ctx, v1, m1, v2, m2, cnt, _ = run_hip_pairs(c)
+    bad = np.nonzero((v1 != c.valid1) | (v2 != c.valid2) | (m1 != c.mat1) | (m2 != c.mat2))[0]
+    assert bad.size == 0, f"{bad.size} pairs differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == c.counters
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()

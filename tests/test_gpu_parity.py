@@ -75,3 +75,37 @@ def test_hip_tabindex_path_matches_reference(golden_case, monkeypatch):
     assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
     assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
+
+
+@pytest.mark.parametrize("limit", [1, 7])
+def test_hip_verify_queue_full_matches_reference(golden_case, monkeypatch, limit):
+    """Verify's per-block run queue held to a few runs per round of 256 jobs
+    (FAME_VERIFY_QUEUE_LIMIT, a test-only limit): jobs whose runs do not fit
+    are verified again in place in a repeat of their round; plain and
+    batched forms still give FAME's results."""
+    monkeypatch.setenv("FAME_VERIFY_QUEUE_LIMIT", str(limit))
+    c = golden_case
+    print(f"case {c.name}: verify queue of {limit} runs", flush=True)
+    ctx, valid, mat, cnt, _ = run_hip(c)
+    bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
+    assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    ctx.set_form("batched")
+    n = len(c.lens)
+    valid = np.zeros(n, np.uint8)
+    mat = np.zeros(n, np.uint64)
+    cnt = np.zeros(3, np.uint64)
+    s = 0
+    while s < n:
+        e = min(n, s + c.chunk)
+        stranded = s == 0 and not c.unord
+        valid[s:e], mat[s:e] = ctx.match(c.seqs, c.offs[s:e], c.lens[s:e], stranded, cnt)
+        if stranded:
+            ctx.decide_strand()
+        s = e
+    assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    ctx.close()
