@@ -153,9 +153,49 @@ def main():
                     help="single-end forms to time (plain: one kernel per stage; batched: one graph per batch)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--gather-probe", type=int, default=0,
+                    help="only run a random 8-byte gather of this many lookups from a 2^30-word table "
+                         "(HBM counter calibration for random reads)")
+    ap.add_argument("--pmc-summary", nargs="+", default=None, metavar=("OUT.md", "CSV"),
+                    help="only write OUT.md: per-launch rocprofv3 counters of the last dispatch of each "
+                         "kernel named by --pmc-kernels, from the counter_collection CSVs given")
+    ap.add_argument("--pmc-kernels", default="k_verify<2>;k_seed_light;k_meth_dp",
+                    help="';'-separated parts of kernel names")
+    ap.add_argument("--pmc-command", default="", help="the profiled command, for the summaries' header")
+    ap.add_argument("--traffic-summary", nargs=6, default=None,
+                    metavar=("OUT.md", "KERNEL", "FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="only write OUT.md: FETCH_SIZE / WRITE_SIZE of KERNEL's last dispatch, scaled by "
+                         "what the counters read for the --copy-probe-mib copy (size --copy-mib)")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--gather-summary", nargs=3, default=None, metavar=("OUT.md", "GATHER_FETCH_CSV", "COPY_FETCH_CSV"),
+                    help="only write OUT.md: FETCH_SIZE per lookup of the --gather-probe run (lookups: "
+                         "--gather-lookups) against the copy's calibration")
+    ap.add_argument("--gather-lookups", type=int, default=138_000_000)
+    ap.add_argument("--stats-summary", nargs=2, default=None, metavar=("OUT.md", "DIR"),
+                    help="only write OUT.md: the kernel table of a rocprofv3 --kernel-trace --stats run "
+                         "(its CSVs in DIR) and the per-launch durations of the kernels --pmc-kernels names")
+    ap.add_argument("--extras", default="uneven_se,paired,serve",
+                    help="genome_se only: other workloads run after it (1 step each, no CPU leg) and "
+                         "summarised under other_workloads; empty = none")
+    ap.add_argument("--no-host-io", action="store_true", help="skip the host-buffer (fame_match_reads) timing")
     args = ap.parse_args()
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
+        return
+    if args.gather_probe:
+        gather_probe(args.gather_probe, 1 << 30, reps=2)
+        return
+    if args.gather_summary:
+        gather_summary(*args.gather_summary, args.copy_mib, args.gather_lookups, args.pmc_command)
+        return
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary, args.copy_mib, args.pmc_command)
+        return
+    if args.stats_summary:
+        stats_summary(args.stats_summary[0], args.stats_summary[1], args.pmc_kernels.split(";"), args.pmc_command)
+        return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1:], args.pmc_kernels.split(";"), args.pmc_command)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by read "
@@ -163,7 +203,7 @@ def main():
         raise SystemExit(2)
     wl = dict(WORKLOADS[args.workload])
     if not args.cpu_sample:
-        args.cpu_sample = 300_000 if wl["paired"] else 600_000
+        args.cpu_sample = 300_000 if wl["paired"] else 2_400_000
 
     import torch  # device memory and synchronisation only
 
@@ -326,9 +366,9 @@ def main():
             # the counter summaries under profiles/ were taken on the default
             # workload's 2 M-read launches
             same = args.workload == "genome_se" and args.batch == 2_000_000 and args.genome_scale == 1.0
-            # `roofline` is the kernel of the larger of the two dominant stages
-            # (seed, verify); both are also given under their own names
-            r_seed, r_verify = roofline("seed", full, same), roofline("verify", full, same)
+            r_seed = roofline_seed(full, same)
+            r_verify = roofline_verify(full, same)
+            r_dp = roofline_dp(full, same)
             # the seed kernel's bucket lookups set against a plain random gather
             # of as many 8-byte words from a table of the slot table's size
             lookups = int(sum(c[2]["kmers"] for c in full) / len(full))
@@ -338,14 +378,19 @@ def main():
             r_seed["lookups_per_s"] = round(seed_rate, 1)
             r_seed["gather_probe_lookups_per_s"] = round(probe_rate, 1)
             r_seed["frac_of_gather_probe"] = round(seed_rate / probe_rate, 3)
-            dominant = max(dev_stages, key=dev_stages.get)
-            roof = dict(r_verify if dev_stages["verify"] > dev_stages["seed"] else r_seed)
-            roof["dominant_stage"] = dominant
+            # `roofline` (the contract's object) is the HBM roofline of the
+            # seed kernel, the largest stage that moves memory; verify and the
+            # banded DP are integer-ALU bound and carry their own objects
+            roof = dict(r_seed)
+            roof["dominant_stage"] = max(dev_stages, key=dev_stages.get)
             verify_rate = per_work["window_bytes"] / (per_stage["verify"] / 1e3) if per_stage.get("verify") else 0
             out["verify_window_GBps"] = round(verify_rate / 1e9, 2)
             out["roofline"] = roof
             out["roofline_seed"] = r_seed
             out["roofline_verify"] = r_verify
+            out["roofline_dp"] = r_dp
+        if not paired and not args.no_host_io:
+            out["host_buffers"] = host_leg(ctx, seqs, offs, lens, n, forms)
         # ---------------- CPU baseline + parity on a sample ----------------
         if not args.no_cpu:
             if paired:
@@ -354,54 +399,271 @@ def main():
             else:
                 out["cpu_baseline"], out["parity"] = cpu_leg(args, wl, ctx, reads, idx_path, wd, len(lens_c))
         ctx.close()
+        del d_seqs, d_offs, d_lens, d_valid, d_mat
+        torch.cuda.empty_cache()
+        if args.extras and args.workload == "genome_se":
+            shutil.rmtree(wd, ignore_errors=True)
+            out["other_workloads"] = run_extras(args)
         out["wall_s"] = round(time.time() - t_all, 1)
         print(json.dumps(out), flush=True)
     finally:
         shutil.rmtree(wd, ignore_errors=True)
 
 
-def algorithmic_bytes(stage, st):
-    """Bytes the stage's algorithm must move for one call, from the kernels'
-    own work counters: seed = the bucket lookup of every k-mer (one 8-byte
-    slot, which holds buckets of one or two entries), the index entries read
-    from the k-mer table (buckets of three or more), the read's letters and
-    the candidate records written; verify = the window text of every
-    candidate (WINLEN + MISCOUNT + ADDMIS - 1 letters, as FAME's ShiftAnd
-    scans it)."""
-    if stage == "seed":
-        return st["tab_bytes"] * st["kmers"] + 4 * st["entries"] + st["read_bytes"] + 8 * st["jobs"]
-    return st["window_bytes"]
+# MI355X figures the rooflines are set against: HBM3E bandwidth; 256 CUs x 4
+# SIMDs of 16 lanes at 2.4 GHz, so 39.3 T 32-bit integer lane-ops/s, and one
+# wave64 VALU instruction per SIMD every 4 cycles (6.14e11 wave-instructions/s)
+CUS, SIMDS, CLOCK_HZ = 256, 1024, 2.4e9
+INT_OPS_PEAK = SIMDS * 16 * CLOCK_HZ
+VALU_ISSUE_PEAK = SIMDS * CLOCK_HZ / 4
+
+# kernel, its HBM traffic summary, the VALU counter summary (all three kernels)
+VALU_PROFILE = "seed_verify_dp_genome300M_2Mreads_valu_counters.md"
+PROFILES = {
+    "seed": ("k_seed_light", "k_seed_light_genome300M_2Mreads_traffic.md"),
+    "verify": ("k_verify<2>", "k_verify2_genome300M_2Mreads_traffic.md"),
+    "dp": ("k_meth_dp", None),
+}
 
 
-TRAFFIC = {"seed": ("k_seed_light", "k_seed_light_genome300M_2Mreads_traffic.md"),
-           "verify": ("k_verify<2>", "k_verify2_genome300M_2Mreads_traffic.md")}
-
-
-def measured_traffic(stage):
-    """HBM bytes per launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", TRAFFIC[stage][1])
+def profile_value(fname, key):
+    """A `key: value` line of a committed profile summary (JSON value), or None."""
+    if not fname:
+        return None
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1].strip())
+        if line.startswith(key + ":"):
+            return json.loads(line.split(":", 1)[1])
     return None
 
 
-def roofline(stage, calls, traffic_known=True):
-    """traffic_known: the committed counter summary was taken on this
-    workload (the default one); otherwise traffic is null."""
-    n = len(calls)
-    ms = sum(c[1][stage] for c in calls) / n
-    nbytes = sum(algorithmic_bytes(stage, c[2]) for c in calls) / n
-    gbs = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-    traffic = measured_traffic(stage) if traffic_known else None
-    return {"kernel": TRAFFIC[stage][0], "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+def mean_over(calls, f):
+    return sum(f(c) for c in calls) / len(calls)
+
+
+def roofline_seed(calls, same):
+    """HBM roofline of k_seed_light (the seed stage of a plain call): the
+    algorithmic bytes per (read, orientation) are FAME's own lookups, 16 B
+    of tabIndex per k-mer, 8.125 B per index entry of every bucket named
+    (kmerTableSmall entry + strand bit) and the read's letters; `design_bytes`
+    is what this design must read (one 8-byte slot per k-mer, the k-mer table
+    only for buckets of three or more, the read, the candidate records
+    written).  traffic: HBM bytes per launch from the committed counter
+    summary (null off the default workload)."""
+    ms = mean_over(calls, lambda c: c[1]["seed"])
+    alg = mean_over(calls, lambda c: 16 * c[2]["kmers"] + 8.125 * c[2]["bucket_entries"] + c[2]["read_bytes"])
+    design = mean_over(calls, lambda c: c[2]["tab_bytes"] * c[2]["kmers"] + 4 * c[2]["entries"]
+                       + c[2]["read_bytes"] + 8 * c[2]["jobs"])
+    gbs = alg / (ms / 1e3) / 1e9
+    traffic = profile_value(PROFILES["seed"][1], "traffic_bytes_per_launch") if same else None
+    return {"kernel": "k_seed_light", "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_GBps": round(traffic / (ms / 1e3) / 1e9, 1) if traffic and ms > 0 else None,
-            "algorithmic_bytes_per_launch": int(nbytes), "avg_launch_ms": round(ms, 3),
-            "launches": n, "reads_per_launch": int(sum(c[0] for c in calls) / n)}
+            "traffic_GBps": round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None,
+            "algorithmic_bytes_per_launch": int(alg), "design_bytes_per_launch": int(design),
+            "design_GBps": round(design / (ms / 1e3) / 1e9, 2),
+            "bucket_entries_per_launch": int(mean_over(calls, lambda c: c[2]["bucket_entries"])),
+            "avg_launch_ms": round(ms, 3), "launches": len(calls),
+            "reads_per_launch": int(mean_over(calls, lambda c: c[0]))}
+
+
+def valu_fields(stage, ms, same):
+    """VALU issue fraction, lane utilisation and integer-op rate of a kernel
+    from its committed counter summary (per-launch counts, which do not
+    depend on timing) over the launch time measured here."""
+    cnt = profile_value(VALU_PROFILE, f"counters_per_launch[{PROFILES[stage][0]}]") if same else None
+    if not cnt:
+        return {"valu_counters": None}
+    s = ms / 1e3
+    out = {"valu_insts_per_launch": int(cnt["SQ_INSTS_VALU"]),
+           "valu_issue_frac": round(cnt["SQ_INSTS_VALU"] / s / VALU_ISSUE_PEAK, 3)}
+    if "SQ_THREAD_CYCLES_VALU" in cnt and cnt.get("SQ_ACTIVE_INST_VALU"):
+        # AMD's VALUUtilization: active lanes per issued VALU instruction
+        util = cnt["SQ_THREAD_CYCLES_VALU"] / (64 * cnt["SQ_ACTIVE_INST_VALU"])
+        out["lane_utilisation"] = round(util, 3)
+        # active-lane operations per second against 39.3 T lane-ops/s
+        out["lane_ops_per_s"] = round(cnt["SQ_INSTS_VALU"] * 64 * util / s, 1)
+        out["lane_ops_frac"] = round(cnt["SQ_INSTS_VALU"] * 64 * util / s / INT_OPS_PEAK, 3)
+    if "SQ_INSTS_VALU_INT32" in cnt:
+        out["int_valu_insts_per_launch"] = int(cnt["SQ_INSTS_VALU_INT32"] + cnt.get("SQ_INSTS_VALU_INT64", 0))
+    return out
+
+
+def roofline_verify(calls, same):
+    """k_verify<2> is bound by integer VALU issue.  achieved = the bit-parallel
+    scan work FAME's ShiftAnd would do over every candidate window (7 64-bit
+    word updates per letter and pattern word, a 64-bit op counted as two
+    32-bit ops) per second of this kernel, against 39.3 T 32-bit lane-ops/s;
+    the filter lets this kernel skip most of that scan, so frac can pass 1.
+    The counter fields give what the kernel itself issues."""
+    ms = mean_over(calls, lambda c: c[1]["verify"])
+    ops = mean_over(calls, lambda c: 7 * 2 * c[2]["word_letters"])
+    r = {"kernel": "k_verify<2>", "bound": "valu", "achieved": round(ops / (ms / 1e3) / 1e12, 3),
+         "peak": round(INT_OPS_PEAK / 1e12, 1), "unit": "T int32 ops/s",
+         "frac": round(ops / (ms / 1e3) / INT_OPS_PEAK, 4), "algorithmic_ops_per_launch": int(ops),
+         "window_bytes_per_launch": int(mean_over(calls, lambda c: c[2]["window_bytes"])),
+         "windows_per_launch": int(mean_over(calls, lambda c: c[2]["jobs"])),
+         "traffic": profile_value(PROFILES["verify"][1], "traffic_bytes_per_launch") if same else None,
+         "avg_launch_ms": round(ms, 3), "launches": len(calls),
+         "reads_per_launch": int(mean_over(calls, lambda c: c[0]))}
+    r.update(valu_fields("verify", ms, same))
+    return r
+
+
+def roofline_dp(calls, same):
+    """k_meth_dp (banded DP of the reads matched with errors): (L + 1) x 17
+    band cells per read, as GCUPS over this kernel's own time."""
+    ms = mean_over(calls, lambda c: c[1]["dp"])
+    cells = mean_over(calls, lambda c: 17 * (c[2]["dp_rows"] + c[2]["dp"]))
+    r = {"kernel": "k_meth_dp", "bound": "valu", "gcups": round(cells / (ms / 1e3) / 1e9, 2) if ms else None,
+         "cells_per_launch": int(cells), "alignments_per_launch": int(mean_over(calls, lambda c: c[2]["dp"])),
+         "avg_launch_ms": round(ms, 3), "launches": len(calls)}
+    r.update(valu_fields("dp", ms, same))
+    return r
+
+
+def stats_summary(out_md, d, kernels, command):
+    """Markdown of a rocprofv3 --kernel-trace --stats run: every kernel's
+    calls and durations, then each named kernel's launches in dispatch order."""
+    import csv
+    import glob
+    st = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
+    tr = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    if not st or not tr:
+        raise SystemExit(f"no kernel_stats / kernel_trace CSV under {d}")
+    rows = list(csv.DictReader(open(st[0])))
+    with open(out_md, "w") as f:
+        f.write(f"# Kernel times (rocprofv3 --kernel-trace --stats)\n\n")
+        if command:
+            f.write("Command (from /tmp with TMPDIR=/tmp):\n\n```\n" + command + "\n```\n\n")
+        f.write("| Kernel | Calls | Total ms | Avg ms | Min ms | Max ms | % |\n|---|---:|---:|---:|---:|---:|---:|\n")
+        for r in rows:
+            name = r["Name"][5:] if r["Name"].startswith("void ") else r["Name"]
+            name = name if len(name) <= 100 else name[:97] + "..."
+            f.write(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                    f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['MinNs']) / 1e6:.3f} | "
+                    f"{float(r['MaxNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |\n")
+        launches = {}
+        for r in csv.DictReader(open(tr[0])):
+            for k in kernels:
+                if k in r["Kernel_Name"]:
+                    launches.setdefault(k, []).append(
+                        (int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6))
+        f.write("\nPer-launch durations (ms), in dispatch order:\n\n")
+        for k, v in launches.items():
+            v.sort()
+            f.write(f"- `{k}`: " + ", ".join(f"{ms:.3f}" for _, ms in v) + "\n")
+    log(f"wrote {out_md}")
+
+
+def _counter_by_dispatch(path, counter, match):
+    import csv
+    import collections
+    d = collections.defaultdict(float)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter and match in r["Kernel_Name"]:
+            d[int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+    return [d[k] for k in sorted(d)]
+
+
+def traffic_summary(out_md, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv, copy_mib, command):
+    """HBM bytes per launch of `kernel` (its last dispatch: the 2 M-read
+    launch) from FETCH_SIZE and WRITE_SIZE (KiB), each scaled by the ratio of
+    the bytes a plain copy moves to what the same counter reads for it."""
+    kib = copy_mib * 1024
+    cf = _counter_by_dispatch(cfetch_csv, "FETCH_SIZE", "neg")
+    cw = _counter_by_dispatch(cwrite_csv, "WRITE_SIZE", "neg")
+    if not cf or not cw:
+        raise SystemExit("no copy-probe dispatches in the calibration CSVs")
+    f_scale = kib / (sum(cf) / len(cf))
+    w_scale = kib / (sum(cw) / len(cw))
+    kf = _counter_by_dispatch(fetch_csv, "FETCH_SIZE", kernel)
+    kw = _counter_by_dispatch(write_csv, "WRITE_SIZE", kernel)
+    if not kf or not kw:
+        raise SystemExit(f"no dispatch of {kernel} in the kernel CSVs")
+    fb, wb = kf[-1] * 1024 * f_scale, kw[-1] * 1024 * w_scale
+    with open(out_md, "w") as f:
+        f.write(f"# HBM traffic of `{kernel}`: its last launch in the profiled run\n\n")
+        if command:
+            f.write("Commands (from /tmp with TMPDIR=/tmp; each counter in its own run):\n\n```\n" + command
+                    + "\n```\n\n")
+        f.write(f"Calibration: a plain copy (`torch.neg` over {copy_mib} MiB of int32: {kib} KiB read and {kib} "
+                f"KiB written) reads FETCH_SIZE = {sum(cf) / len(cf):.0f} KiB and WRITE_SIZE = "
+                f"{sum(cw) / len(cw):.0f} KiB per launch (mean of {len(cf)} / {len(cw)} launches): FETCH_SIZE "
+                f"is scaled by {f_scale:.3f} and WRITE_SIZE by {w_scale:.3f}.\n\n")
+        f.write("| Counter | KiB per launch | scaled bytes |\n|---|---:|---:|\n")
+        f.write(f"| FETCH_SIZE | {kf[-1]:.0f} | {fb:.0f} |\n| WRITE_SIZE | {kw[-1]:.0f} | {wb:.0f} |\n\n")
+        f.write(f"traffic_bytes_per_launch: {int(fb + wb)}\n")
+    log(f"wrote {out_md}")
+
+
+def gather_summary(out_md, gather_csv, cfetch_csv, copy_mib, lookups, command):
+    """FETCH_SIZE of the random-gather probe (torch gather kernel: `lookups`
+    8-byte indexes read in order, as many random 8-byte table words, 8-byte
+    results written), per lookup, beside the copy's calibration."""
+    kib = copy_mib * 1024
+    cf = _counter_by_dispatch(cfetch_csv, "FETCH_SIZE", "neg")
+    f_scale = kib / (sum(cf) / len(cf))
+    g = _counter_by_dispatch(gather_csv, "FETCH_SIZE", "scatter_gather")
+    if not g:
+        raise SystemExit("no gather dispatches")
+    g_kib = sum(g) / len(g)
+    idx_kib = 8 * lookups / 1024 / f_scale  # the sequential index stream, as the counter reads it
+    per = (g_kib - idx_kib) * 1024 / lookups
+    with open(out_md, "w") as f:
+        f.write("# FETCH_SIZE of random 8-byte gathers (calibration check)\n\n")
+        if command:
+            f.write("Commands (from /tmp with TMPDIR=/tmp):\n\n```\n" + command + "\n```\n\n")
+        f.write(f"The probe gathers {lookups} words at uniform random positions of a 2^30-word (8 GiB) "
+                f"table with torch's gather kernel, reading the {8 * lookups} bytes of indexes in order "
+                f"and writing as many bytes of results.\n\n")
+        f.write(f"| | KiB per launch |\n|---|---:|\n| FETCH_SIZE (mean of {len(g)} launches) | {g_kib:.0f} |\n"
+                f"| index stream at the copy's factor {f_scale:.3f} | {idx_kib:.0f} |\n\n")
+        f.write(f"counter_bytes_per_random_lookup: {per:.2f}\n\n")
+        f.write(f"The counter reads {per:.1f} bytes per random lookup: one 64-byte unit, so one read "
+                f"request per lookup.  The copy reads at 1/{f_scale:.0f} of the bytes it moves, one unit per "
+                f"{64 * f_scale:.0f} bytes.  If the read requests to HBM are all of one size, that size is "
+                f"{64 * f_scale:.0f} bytes, and the factor {f_scale:.3f} holds for random gathers too "
+                f"({64 * f_scale:.0f} bytes of HBM traffic per independent lookup); the traffic figures "
+                f"here use it.\n")
+    log(f"wrote {out_md}")
+
+
+def pmc_summary(out_md, csvs, kernels, command):
+    """Per-launch counters of the last dispatch of each kernel (the 2 M-read
+    launch of `--reads 2300000 --steps 1 --warmup 0 --forms plain`), summed
+    over XCDs, shader engines and SIMDs as rocprofv3 reports them."""
+    import csv
+    import collections
+    per = collections.defaultdict(lambda: collections.defaultdict(float))  # (kernel, dispatch) -> counter
+    for path in csvs:
+        for row in csv.DictReader(open(path)):
+            name = row["Kernel_Name"]
+            for k in kernels:
+                if k in name:
+                    per[(k, int(row["Dispatch_Id"]))][row["Counter_Name"]] += float(row["Counter_Value"])
+    with open(out_md, "w") as f:
+        f.write("# VALU counters per launch (rocprofv3 --pmc, one pass per counter group)\n\n")
+        if command:
+            f.write("Command (from /tmp with TMPDIR=/tmp):\n\n```\n" + command + "\n```\n\n")
+        for k in kernels:
+            if not any(kk == k for (kk, d) in per):
+                continue
+            # counters from several runs: the last dispatch of the kernel in each
+            last = {}
+            for (kk, d), cs in per.items():
+                if kk != k:
+                    continue
+                for c, v in cs.items():
+                    if c not in last or d > last[c][0]:
+                        last[c] = (d, v)
+            cnt = {c: v for c, (d, v) in sorted(last.items())}
+            f.write(f"## `{k}`\n\n| Counter | per launch |\n|---|---:|\n")
+            for c, v in cnt.items():
+                f.write(f"| {c} | {v:.6g} |\n")
+            f.write(f"\ncounters_per_launch[{k}]: " + json.dumps({c: round(v) for c, v in cnt.items()}) + "\n\n")
+    log(f"wrote {out_md}")
 
 
 def copy_probe(mib):
@@ -443,6 +705,106 @@ def gather_probe(lookups, words, reps=3):
     return lookups / (best / 1e3)
 
 
+def host_leg(ctx, seqs, offs, lens, n, forms):
+    """The whole job through fame_match_reads with the reads in pinned host
+    memory and FAME's CHUNKSIZE of 300 000 reads per call (what FAME's loop
+    gets through the drop-in): host-to-device copies of the reads and the
+    copies of the per-read results back are inside the timed region.  One
+    untimed pass, then one timed pass per form."""
+    import torch
+    pin = torch.empty(seqs.size, dtype=torch.uint8, pin_memory=True)
+    pin.numpy()[:] = seqs
+    hs = pin.numpy()
+    res = {"reads_per_call": CHUNKSIZE, "reads": n}
+    for form in forms:
+        ctx.set_form(form)
+        for rep in range(2):
+            ctx.reset()
+            cnt = np.zeros(3, np.uint64)
+            t0 = time.perf_counter()
+            e = min(n, CHUNKSIZE)
+            ctx.match(hs, offs[:e], lens[:e], True, cnt)
+            ctx.decide_strand()
+            s = e
+            while s < n:
+                e = min(n, s + CHUNKSIZE)
+                ctx.match(hs, offs[s:e], lens[s:e], False, cnt)
+                s = e
+            dt = time.perf_counter() - t0
+        res[form] = {"ms_per_step": round(dt * 1e3, 3), "aligned_reads_per_s": round(int(cnt[0]) / dt, 1),
+                     "input_reads_per_s": round(n / dt, 1)}
+        log(f"[host buffers, {form}] {dt * 1e3:.1f} ms for {n} reads in calls of {CHUNKSIZE}")
+    del pin
+    return res
+
+
+def host_facts():
+    """CPU model, threads visible and the C++ runtime of this host."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    libstdcxx = None
+    for d in ("/usr/lib/x86_64-linux-gnu", "/usr/lib64", "/lib/x86_64-linux-gnu"):
+        p = os.path.join(d, "libstdc++.so.6")
+        if os.path.exists(p):
+            libstdcxx = os.path.basename(os.path.realpath(p))
+            break
+    gxx = None
+    try:
+        gxx = subprocess.run(["g++", "-dumpfullversion"], capture_output=True, text=True).stdout.strip() or None
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_threads": os.cpu_count(), "libstdcxx": libstdcxx, "gxx": gxx}
+
+
+def chunk_rates(st):
+    """FAME's per-chunk times (the harness's steady_clock around each
+    matchReads call): the first chunk (both orientations) and the stranded
+    ones separately, in input reads/s."""
+    ch = st.get("chunks", [])
+    first = [c for c in ch if c["stranded"]]
+    rest = [c for c in ch if not c["stranded"] and c["n"]]
+    out = {}
+    if first:
+        out["first_chunk_reads_per_s"] = round(sum(c["n"] for c in first) / sum(c["s"] for c in first), 1)
+    if rest:
+        out["stranded_reads_per_s"] = round(sum(c["n"] for c in rest) / sum(c["s"] for c in rest), 1)
+    return out
+
+
+def run_extras(args):
+    """The other workloads, each in a child process (1 warm-up + 1 timed step,
+    no CPU leg), summarised."""
+    out = {}
+    for w in [x for x in args.extras.split(",") if x]:
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", w, "--steps", "1", "--warmup", "1",
+               "--no-cpu", "--no-host-io", "--extras", "", "--seed", str(args.seed)]
+        log(f"other workload {w}: {' '.join(cmd[1:])}")
+        t0 = time.time()
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+        if r.returncode != 0 or not lines:
+            log(f"{w} failed (rc {r.returncode}):", r.stderr[-800:])
+            out[w] = {"error": f"rc {r.returncode}"}
+            continue
+        d = json.loads(lines[-1])
+        keep = ("value", "unit", "ms_per_step", "forms", "batched_vs_plain", "counters", "stage_ms",
+                "p50_ms", "p99_ms", "batched_peak_reads_per_s", "plain_peak_reads_per_s", "host_buffers")
+        out[w] = {"workload": d["config"]["workload"], **{k: d[k] for k in keep if k in d}}
+        if "sweep" in d:
+            out[w]["sweep"] = [{"load": row["load"], "batched_p50_ms": row["batched"]["p50_ms"],
+                                "batched_p99_ms": row["batched"]["p99_ms"], "plain_p50_ms": row["plain"]["p50_ms"],
+                                "plain_p99_ms": row["plain"]["p99_ms"]} for row in d["sweep"]]
+        out[w]["wall_s"] = round(time.time() - t0, 1)
+        log(f"{w}: {out[w].get('value')} {out[w].get('unit')}")
+    return out
+
+
 def cpu_leg(args, wl, ctx, reads, idx_path, wd, nchrom):
     """FAME's own CPU path on the first `cpu_sample` reads of the workload, and
     the HIP path on the same sample for a per-read / report comparison."""
@@ -466,7 +828,13 @@ def cpu_leg(args, wl, ctx, reads, idx_path, wd, nchrom):
         f"wall {time.time() - t0:.1f}s)")
     cpu = {"value": round(st["succ"] / st["match_s"], 1), "unit": "reads/s", "cores": args.cpu_cores,
            "kind": "reference", "sample": sample, "match_s": round(st["match_s"], 3),
-           "reads": int(st["reads"]), "succ": int(st["succ"])}
+           "reads": int(st["reads"]), "succ": int(st["succ"]), **chunk_rates(st), **host_facts()}
+    if "stranded_reads_per_s" in cpu and "first_chunk_reads_per_s" in cpu:
+        # the whole job's mix (one first chunk, the rest stranded) at the
+        # sample's rates and aligned fraction
+        N = args.reads or wl["n"]
+        t = CHUNKSIZE / cpu["first_chunk_reads_per_s"] + (N - CHUNKSIZE) / cpu["stranded_reads_per_s"]
+        cpu["whole_job_estimate_aligned_reads_per_s"] = round(N * st["succ"] / st["reads"] / t, 1)
     # HIP path on the same sample, fresh state
     raw = open(os.path.join(wd, "cpu_reads.bin"), "rb").read()
     nr = int(np.frombuffer(raw[8:16], np.uint64)[0])
@@ -666,7 +1034,7 @@ def cpu_leg_pairs(args, wl, ctx, pairs, idx_path, wd, nchrom):
         f"wall {time.time() - t0:.1f}s)")
     cpu = {"value": round(st["succ"] / st["match_s"], 1), "unit": "reads/s", "cores": args.cpu_cores,
            "kind": "reference", "sample": sample, "match_s": round(st["match_s"], 3),
-           "pairs": int(st["reads"]), "succ": int(st["succ"])}
+           "pairs": int(st["reads"]), "succ": int(st["succ"]), **chunk_rates(st), **host_facts()}
     rv1, rm1, rv2, rm2 = read_pairs_bin(os.path.join(wd, "cpu_pairs.bin"))
     ctx.reset()
     a = r1.subset(np.arange(S)).flat()

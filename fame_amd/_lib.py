@@ -207,15 +207,15 @@ class Context:
         self._check(load().fame_print_methylation_levels(self.h, basename.encode()))
 
     def timings(self):
-        t = np.zeros(6, np.float64)
+        t = np.zeros(7, np.float64)
         self._check(load().fame_last_timings(self.h, ptr(t, _dbl)))
-        return dict(zip(("prep", "seed", "verify", "select", "meth", "total"), t.tolist()))
+        return dict(zip(("prep", "seed", "verify", "select", "meth", "total", "dp"), t.tolist()))
 
     def stats(self):
-        t = np.zeros(8, np.uint64)
+        t = np.zeros(11, np.uint64)
         self._check(load().fame_last_stats(self.h, ptr(t, _u64p)))
         return dict(zip(("kmers", "entries", "seeded", "jobs", "window_bytes", "dp", "read_bytes",
-                         "tab_bytes"), (int(x) for x in t)))
+                         "tab_bytes", "bucket_entries", "dp_rows", "word_letters"), (int(x) for x in t)))
 
     def close(self):
         if getattr(self, "h", None):

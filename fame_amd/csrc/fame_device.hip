@@ -187,7 +187,7 @@ struct SeedWork {
 };
 
 struct SeedStats {
-    uint64_t kmers, entries, seeded, bytes;
+    uint64_t kmers, entries, seeded, bytes, buckets;
 };
 
 // NTPS64 of k-mer k of the reduced-letter row `code` (closed form), reduced
@@ -304,6 +304,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     ss.entries += wave_sum_u32(gathered);
     ss.seeded += 1;
     ss.bytes += L;
+    ss.buckets += E;
     wave_sync();
     // entry gather, one entry per lane: entry j belongs to the k-mer whose
     // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
@@ -591,7 +592,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     w.code = m.code;
     seed_table_init(sh);
     PoolChunk pc = {0, 0, 0};
-    SeedStats ss = {0, 0, 0, 0};
+    SeedStats ss{};
     const uint32_t nw = gridDim.x * FAME_SEED_WPB;
     // the (read, orientation) pairs of the queried orientations only: pair q
     // is row 2q + o when one orientation o is queried (k_select never reads
@@ -689,6 +690,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
+        atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
     }
 }
 
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     w.cand = (uint32_t*)take(8ull * ce);
     w.code = s_code;
     seed_table_init(sh);
-    SeedStats ss = {0, 0, 0, 0};
+    SeedStats ss{};
     const bool tiered = hs.count == FAME_JOBS_ON_DEVICE;
     for (uint32_t hi = blockIdx.x; hi < count; hi += gridDim.x) {
         if (tiered) {
@@ -762,6 +764,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
+        atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
     }
 }
 
@@ -1625,6 +1628,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         if ((threadIdx.x & 63) == 0 && jobs) {
             atomicAdd(&bd.stats[3], (unsigned long long)jobs);
             atomicAdd(&bd.stats[4], (unsigned long long)bytes);
+            atomicAdd(&bd.stats[10], (unsigned long long)bytes * W);
         }
     }
 }
@@ -1956,9 +1960,11 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
     uint64_t* D = dec_scratch + gid;
     uint64_t* TR = trace_scratch + gid;
     if (gid == 0) atomicAdd(&bd.stats[5], (unsigned long long)n);
+    uint32_t rows = 0; // read letters aligned, for the cell count
     for (uint32_t t = gid; t < n; t += gsz) {
         const uint32_t r = bd.dp_list[t];
         const uint64_t mat = bd.mat[r];
+        rows += bd.lens[r];
         const bool isFwd = (mat >> 24) & 1;
         const bool rv = !isFwd;
         const uint32_t metaID = (uint32_t)(mat >> 32);
@@ -2226,6 +2232,8 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) rows += __shfl_xor(rows, o, 64);
+    if ((threadIdx.x & 63) == 0 && rows) atomicAdd(&bd.stats[9], (unsigned long long)rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -2497,11 +2505,15 @@ uint32_t meth_dp_lanes()
 }
 
 void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
-                 uint32_t grid_threads, hipStream_t st)
+                 uint32_t grid_threads, hipStream_t st, hipEvent_t between)
 {
-    if (!n) return;
+    if (!n) {
+        if (between) (void)hipEventRecord(between, st);
+        return;
+    }
     const uint32_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_meth_direct, dim3(g < 8192 ? g : 8192), dim3(256), 0, st, ix, bd);
+    if (between) (void)hipEventRecord(between, st);
     hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dec, trace);
 }
 

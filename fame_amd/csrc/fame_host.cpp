@@ -108,7 +108,7 @@ struct fame_ctx {
     uint32_t pool2_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t pe_map_cap = 0;
     uint32_t meth_threads = 65536; // lanes of the DP kernel: as many as are resident at once
-    double timings[6] = {0, 0, 0, 0, 0, 0};
+    double timings[FAME_TIMING_FIELDS] = {0};
     uint64_t last_heavy = 0, last_ovf = 0;
     // batched form: plans per capacity class, their graphs, and counts
     int form = 0; // 0 plain, 1 batched
@@ -401,7 +401,7 @@ T* grow(DevBuf& b, size_t count)
 }
 
 struct Timer {
-    hipEvent_t e[16];
+    hipEvent_t e[17];
     hipStream_t st;
     explicit Timer(hipStream_t s) : st(s)
     {
@@ -645,7 +645,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         uint64_t* dec = grow<uint64_t>(c->b_dec, (size_t)(FAME_DEV_MAX_READ + 1) * c->meth_threads);
         uint64_t* tr = grow<uint64_t>(c->b_trace, (size_t)FAME_DP_TRACE_WORDS * c->meth_threads);
         tm.mark(8);
-        launch_meth(c->ix, bd, n, dp, dec, tr, c->meth_threads, st);
+        launch_meth(c->ix, bd, n, dp, dec, tr, c->meth_threads, st, tm.e[16]);
         tm.mark(9);
     }
     uint64_t cnt[5];
@@ -669,6 +669,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     c->timings[2] = tm.ms(4, 5) + (c->last_ovf ? tm.ms(14, 15) : 0.0);
     c->timings[3] = tm.ms(6, 7);
     c->timings[4] = tm.ms(8, 9);
+    c->timings[6] = n ? tm.ms(16, 9) : 0.0;
     c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FAME_OK;
 }
@@ -1023,7 +1024,7 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     *out_valid = pl->valid.as<uint8_t>();
     *out_mat = pl->mat.as<uint64_t>();
     if (out_status) *out_status = pl->status.as<uint8_t>();
-    c->timings[0] = c->timings[1] = c->timings[2] = c->timings[3] = c->timings[4] = 0;
+    for (double& x : c->timings) x = 0;
     c->timings[1] = ms; // the whole graph
     c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FAME_OK;
@@ -1127,7 +1128,7 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
     *out_mat = om;
     if (out_status) *out_status = os;
     memset(c->stats, 0, sizeof c->stats);
-    for (int k = 0; k < 5; ++k) c->timings[k] = 0;
+    for (double& x : c->timings) x = 0;
     c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->timings[1] = c->timings[5];
     return FAME_OK;
@@ -1466,7 +1467,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         uint64_t* dec = grow<uint64_t>(c->b_dec, (size_t)(FAME_DEV_MAX_READ + 1) * c->meth_threads);
         uint64_t* tr = grow<uint64_t>(c->b_trace, (size_t)FAME_DP_TRACE_WORDS * c->meth_threads);
         tm.mark(14);
-        launch_meth(c->ix, bd, nr, dp, dec, tr, c->meth_threads, st);
+        launch_meth(c->ix, bd, nr, dp, dec, tr, c->meth_threads, st, tm.e[16]);
         tm.mark(15);
     }
     uint64_t cnt[5];
@@ -1484,6 +1485,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     c->timings[2] = tm.ms(4, 5) + tm.ms(8, 9);
     c->timings[3] = tm.ms(6, 7) + tm.ms(10, 11);
     c->timings[4] = tm.ms(14, 15);
+    c->timings[6] = nr ? tm.ms(16, 15) : 0.0;
     c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FAME_OK;
 }
@@ -2229,7 +2231,7 @@ int fame_last_stats(const fame_ctx* c, uint64_t out[FAME_STATS_FIELDS])
     return FAME_OK;
 }
 
-int fame_last_timings(const fame_ctx* c, double out[6])
+int fame_last_timings(const fame_ctx* c, double out[FAME_TIMING_FIELDS])
 {
     if (!c || !out) return FAME_ERR_ARG;
     memcpy(out, c->timings, sizeof c->timings);

@@ -16,7 +16,7 @@ struct fame_ctx;
 #define FAME_PE_WCAP 8         // windows of an item k_pe_walk1 walks in LDS, one lane per item
 #define FAME_PE_WBIG 1360      // windows of an item k_pe_walk1_big walks in LDS, one wave per item
 #define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
-#define FAME_NSTATS 8
+#define FAME_NSTATS 11
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
 #define FAME_SEED_WAVES 16384      // persistent waves of the seed kernel
 #ifndef FAME_SEED_WPB
@@ -252,8 +252,9 @@ void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_s
 // u64 choice words, FAME_DP_TRACE_WORDS u64 trace words (lane-strided)
 #define FAME_DP_SROWS 12
 #define FAME_DP_TRACE_WORDS ((FAME_DEV_TRACE_CAP + 31) / 32)
+// between (optional): recorded after the direct kernel, before the banded DP
 void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
-                 uint32_t grid_threads, hipStream_t st);
+                 uint32_t grid_threads, hipStream_t st, hipEvent_t between = nullptr);
 uint32_t meth_dp_lanes();
 void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, uint32_t minlen, hipStream_t st);
 void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, uint32_t grid, hipStream_t st);

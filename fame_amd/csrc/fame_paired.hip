@@ -824,6 +824,7 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
         atomicAdd(&bd.stats[1], (unsigned long long)st[1]);
         atomicAdd(&bd.stats[2], (unsigned long long)st[2]);
         atomicAdd(&bd.stats[6], (unsigned long long)st[3]);
+        atomicAdd(&bd.stats[8], (unsigned long long)st[1]); // every bucket's entries are read here
     }
 }
 
@@ -882,6 +883,7 @@ __global__ __launch_bounds__(64) void k_pe_seed_big(DevIndex ix, BatchDev bd, Pa
         atomicAdd(&bd.stats[1], (unsigned long long)st[1]);
         atomicAdd(&bd.stats[2], (unsigned long long)st[2]);
         atomicAdd(&bd.stats[6], (unsigned long long)st[3]);
+        atomicAdd(&bd.stats[8], (unsigned long long)st[1]); // every bucket's entries are read here
     }
 }
 

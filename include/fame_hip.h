@@ -234,16 +234,24 @@ int fame_fastq_next(fame_fastq* fq, uint32_t max_reads, uint32_t* n, const uint8
 int fame_fastq_close(fame_fastq* fq);
 
 /* per-stage device time of the last fame_match_reads* call, milliseconds:
- * prep, seed, verify, select, meth, and the total including host waits */
-int fame_last_timings(const fame_ctx* ctx, double out[6]);
+ * [0] prep, [1] seed, [2] verify, [3] select, [4] meth, [5] the total
+ * including host waits, [6] the banded-DP part of meth.  The batched form
+ * reports its whole graph as seed, and the total.
+ * No counterpart in the reference (instrumentation). */
+#define FAME_TIMING_FIELDS 7
+int fame_last_timings(const fame_ctx* ctx, double out[FAME_TIMING_FIELDS]);
 
 /* work done by the last fame_match_reads* call, for roofline accounting:
- * k-mers hashed, index entries read from the k-mer table (with the slot
- * table: those of buckets of three or more), (read, orientation) pairs
- * seeded, window verifications, window bytes scanned, banded alignments,
- * read bytes seeded, and the bytes read per k-mer bucket lookup.
+ * [0] k-mers hashed, [1] index entries read from the k-mer table (with the
+ * slot table: those of buckets of three or more), [2] (read, orientation)
+ * pairs seeded, [3] window verifications, [4] window bytes scanned, [5]
+ * banded alignments, [6] read bytes seeded, [7] bytes read per k-mer bucket
+ * lookup, [8] the bucket lengths of every k-mer looked up, summed (index
+ * entries the seeds name), [9] read letters aligned by the banded DP (rows
+ * of 17 band cells each), [10] window letters scanned times the pattern's
+ * 64-bit words (bit-parallel word updates of a full scan).
  * No counterpart in the reference (instrumentation). */
-#define FAME_STATS_FIELDS 8
+#define FAME_STATS_FIELDS 11
 int fame_last_stats(const fame_ctx* ctx, uint64_t out[FAME_STATS_FIELDS]);
 
 #ifdef __cplusplus
