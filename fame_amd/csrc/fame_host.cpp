@@ -59,6 +59,33 @@ struct HipError {
         if (_e != hipSuccess) throw HipError{_e, #call};                                                     \
     } while (0)
 
+// pinned host memory that grows on demand (staging of small host arrays, so
+// that their copies to the device are asynchronous)
+struct HostStage {
+    void* p = nullptr;
+    size_t cap = 0;
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    template <typename T>
+    T* get(size_t count)
+    {
+        const size_t bytes = count * sizeof(T) + 64;
+        if (bytes > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            HIPC(hipHostMalloc(&p, bytes, 0));
+            cap = bytes;
+        }
+        return (T*)p;
+    }
+};
+
 struct FameError {
     int code;
     std::string msg;
@@ -116,6 +143,7 @@ struct fame_ctx {
     uint64_t graph_calls = 0, graph_fallbacks = 0, graph_captures = 0;
     uint32_t pipe_batch = 1u << 21; // sub-batch of a large batched call (two lanes)
     DevBuf b_pvalid, b_pmat, b_pstatus;  // outputs of a pipelined call
+    HostStage h_offs;                    // host-buffer calls: read offsets rebased to the span copied
     uint64_t pipe_calls = 0;
     double big_ms = 0; // paired seed runs with global scratch, last call
 };
@@ -262,7 +290,11 @@ void load_index(fame_ctx* c, const std::string& path)
     uint64_t last = 0;
     HIPC(hipMemcpy(&last, tab64.as<uint64_t>() + c->htab, 8, hipMemcpyDeviceToHost));
     if (last > c->nkmer) throw FameError{FAME_ERR_FORMAT, "tabIndex points past the k-mer table"};
-    if (c->nkmer < (1ull << 32)) {
+    // FAME_TAB64=1 (tests only) keeps the 64-bit tabIndex, and with it the
+    // kernels of indexes of 2^32 k-mers or more, on a smaller index
+    const char* tab64_env = getenv("FAME_TAB64");
+    const bool keep64 = tab64_env && tab64_env[0] == '1';
+    if (c->nkmer < (1ull << 32) && !keep64) {
         HIPC(c->d_tab.reserve(4 * tablen));
         launch_narrow_tab(tab64.as<uint64_t>(), tablen, c->d_tab.as<uint32_t>(), st);
         HIPC(hipStreamSynchronize(st));
@@ -1844,21 +1876,28 @@ int fame_match_reads(fame_ctx* c, const uint8_t* seqs, const uint64_t* offs, con
         return FAME_ERR_ARG;
     }
     return guarded(c, [&]() {
-        uint64_t total = 0;
+        // only the span of seqs the reads occupy is copied; the offsets are
+        // rebased to it
+        uint64_t lo = ~0ull, hi = 0;
         for (uint32_t i = 0; i < n; ++i) {
             if (lens[i] > FAME_MAX_READ) {
                 set_err(c, "read " + std::to_string(i) + " is longer than FAME_MAX_READ");
                 return (int)FAME_ERR_ARG;
             }
-            total = std::max<uint64_t>(total, offs[i] + lens[i]);
+            lo = std::min<uint64_t>(lo, offs[i]);
+            hi = std::max<uint64_t>(hi, offs[i] + lens[i]);
         }
+        if (!n) lo = 0;
+        const uint64_t total = hi - lo;
         hipStream_t st = c->stream;
         uint8_t* d_s = grow<uint8_t>(c->b_seqs, total + 64);
         uint64_t* d_o = grow<uint64_t>(c->b_offs, n + 1);
         uint32_t* d_l = grow<uint32_t>(c->b_lens, n + 1);
-        if (total) HIPC(hipMemcpyAsync(d_s, seqs, total, hipMemcpyHostToDevice, st));
+        uint64_t* h_o = c->h_offs.get<uint64_t>(n + 1);
+        for (uint32_t i = 0; i < n; ++i) h_o[i] = offs[i] - lo;
+        if (total) HIPC(hipMemcpyAsync(d_s, seqs + lo, total, hipMemcpyHostToDevice, st));
         if (n) {
-            HIPC(hipMemcpyAsync(d_o, offs, 8ull * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_o, h_o, 8ull * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_l, lens, 4ull * n, hipMemcpyHostToDevice, st));
         }
         const uint8_t* ov = nullptr;
@@ -1983,29 +2022,35 @@ int fame_match_pairs(fame_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, c
     }
     if (n > 0x7FFFFFFFu) return FAME_ERR_ARG;
     return guarded(c, [&]() {
-        uint64_t t1 = 0, t2 = 0;
+        // the spans of seqs1 and seqs2 the mates occupy, copied one after
+        // the other; the offsets are rebased to them
+        uint64_t lo1 = ~0ull, hi1 = 0, lo2 = ~0ull, hi2 = 0;
         for (uint32_t i = 0; i < n; ++i) {
             if (lens1[i] > FAME_MAX_READ || lens2[i] > FAME_MAX_READ) {
                 set_err(c, "mate of pair " + std::to_string(i) + " is longer than FAME_MAX_READ");
                 return (int)FAME_ERR_ARG;
             }
-            t1 = std::max<uint64_t>(t1, offs1[i] + lens1[i]);
-            t2 = std::max<uint64_t>(t2, offs2[i] + lens2[i]);
+            lo1 = std::min<uint64_t>(lo1, offs1[i]);
+            hi1 = std::max<uint64_t>(hi1, offs1[i] + lens1[i]);
+            lo2 = std::min<uint64_t>(lo2, offs2[i]);
+            hi2 = std::max<uint64_t>(hi2, offs2[i] + lens2[i]);
         }
+        if (!n) lo1 = lo2 = 0;
+        const uint64_t t1 = hi1 - lo1, t2 = hi2 - lo2;
         hipStream_t st = c->stream;
         const uint64_t total = t1 + t2;
         uint8_t* d_s = grow<uint8_t>(c->b_seqs, total + 64);
         uint64_t* d_o = grow<uint64_t>(c->b_offs, 2ull * n + 1);
         uint32_t* d_l = grow<uint32_t>(c->b_lens, 2ull * n + 1);
-        std::vector<uint64_t> o(2ull * n);
+        uint64_t* o = c->h_offs.get<uint64_t>(2ull * n + 1);
         for (uint32_t i = 0; i < n; ++i) {
-            o[i] = offs1[i];
-            o[n + i] = offs2[i] + t1;
+            o[i] = offs1[i] - lo1;
+            o[n + i] = offs2[i] - lo2 + t1;
         }
-        if (t1) HIPC(hipMemcpyAsync(d_s, seqs1, t1, hipMemcpyHostToDevice, st));
-        if (t2) HIPC(hipMemcpyAsync(d_s + t1, seqs2, t2, hipMemcpyHostToDevice, st));
+        if (t1) HIPC(hipMemcpyAsync(d_s, seqs1 + lo1, t1, hipMemcpyHostToDevice, st));
+        if (t2) HIPC(hipMemcpyAsync(d_s + t1, seqs2 + lo2, t2, hipMemcpyHostToDevice, st));
         if (n) {
-            HIPC(hipMemcpyAsync(d_o, o.data(), 16ull * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(d_o, o, 16ull * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_l, lens1, 4ull * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_l + n, lens2, 4ull * n, hipMemcpyHostToDevice, st));
         }

@@ -108,3 +108,17 @@ def test_hip_pairs_verify_queue_full_match_reference(golden_pair, monkeypatch):
     assert cnt.tolist() == c.counters
     assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
+
+
+def test_hip_pairs_tab64_match_reference(golden_pair, monkeypatch):
+    """Paired path with the 64-bit tabIndex kept (FAME_TAB64, test-only: the
+    kernels of indexes of 2^32 k-mers or more) gives FAME's results."""
+    monkeypatch.setenv("FAME_TAB64", "1")
+    c = golden_pair
+    print(f"case {c.name}: 64-bit tabIndex", flush=True)
+    ctx, v1, m1, v2, m2, cnt, _ = run_hip_pairs(c)
+    bad = np.nonzero((v1 != c.valid1) | (v2 != c.valid2) | (m1 != c.mat1) | (m2 != c.mat2))[0]
+    assert bad.size == 0, f"{bad.size} pairs differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == c.counters
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()

@@ -63,12 +63,17 @@ def test_hip_matches_oracle_single_batch(golden_case):
     ctx.close()
 
 
-def test_hip_tabindex_path_matches_reference(golden_case, monkeypatch):
-    """The seed stage without the slot table (tabIndex + k-mer table reads,
-    as for indexes whose tabIndex needs 64 bits): the same results."""
+@pytest.mark.parametrize("tab64", [False, True])
+def test_hip_tabindex_path_matches_reference(golden_case, monkeypatch, tab64):
+    """The seed stage without the slot table (tabIndex + k-mer table reads),
+    with the tabIndex narrowed to 32 bits or kept at 64 (FAME_TAB64, a
+    test-only switch: the kernels of indexes of 2^32 k-mers or more): the
+    same results."""
     monkeypatch.setenv("FAME_NO_SLOT_TABLE", "1")
+    if tab64:
+        monkeypatch.setenv("FAME_TAB64", "1")
     c = golden_case
-    print(f"case {c.name}: tabIndex path", flush=True)
+    print(f"case {c.name}: tabIndex path, {'64' if tab64 else '32'}-bit", flush=True)
     ctx, valid, mat, cnt, _ = run_hip(c)
     bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
     assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
