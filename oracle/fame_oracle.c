@@ -38,7 +38,7 @@
 #define NERR (MISCOUNT + ADDMIS) /* ShiftAnd<E>, ReadQueue.cpp:532 */
 #define DPBAND (2u * MISCOUNT + ADDMIS) /* LevenshtDP<uint16_t,8>, ReadQueue.cpp:3640 */
 #define CPGDUMMY 0xFFFFFFFFu
-#define MAXW 17 /* patterns up to 1087 characters */
+#define MAXW 257 /* patterns up to 16 447 characters (FAME has no limit; longer reads abort the oracle) */
 
 struct orc_cpg {
     uint8_t chrom;
@@ -1293,7 +1293,11 @@ static int match_one(orc_ctx* cx, const uint8_t* in, uint32_t L, int getStranded
 {
     const orc_index* ix = cx->ix;
     if (L < ix->readlen - 20) return 3; /* :430-435 */
-    if (L < KMERLEN || L / 64 + 1 > MAXW) return 3;
+    if (L < KMERLEN) return 3;
+    if (L / 64 + 1 > MAXW) {
+        fprintf(stderr, "oracle: a read of %u letters is beyond the oracle's ShiftAnd (MAXW %d)\n", L, MAXW);
+        abort();
+    }
     /* std::string keeps a terminating '\0' at seq[size()], which the rev DP
      * walk reads (seq[readSeqPos+1] with readSeqPos = L-1, :3858) */
     uint8_t* seq = xmalloc(L + 1);
@@ -1983,6 +1987,11 @@ static void match_pair(orc_ctx* cx, const uint8_t* in1, uint32_t L1, const uint8
                        uint64_t* mat2)
 {
     const orc_index* ix = cx->ix;
+    if (L1 / 64 + 1 > MAXW || L2 / 64 + 1 > MAXW) {
+        fprintf(stderr, "oracle: a mate of %u letters is beyond the oracle's ShiftAnd (MAXW %d)\n",
+                L1 > L2 ? L1 : L2, MAXW);
+        abort();
+    }
     const uint32_t minlen = (uint32_t)ceilf((float)ix->readlen * 0.75f);
     int inv1 = L1 < minlen, inv2 = L2 < minlen;
     if (inv1 || inv2) cnt[4] += 1;
