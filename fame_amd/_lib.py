@@ -16,7 +16,8 @@ ERRORS = {-1: "FAME_ERR_IO", -2: "FAME_ERR_FORMAT", -3: "FAME_ERR_HIP", -4: "FAM
           -5: "FAME_ERR_CAPACITY"}
 INFO_FIELDS = ("htab", "readlen", "winlen", "kmerlen", "kmercutoff", "seedbits", "ncpg", "nchrom",
                "nkmer", "nwin", "nfiltered")
-MAX_READ = 255
+MAX_READ = 4096  # single-end (FAME_MAX_READ)
+MAX_MATE = 255  # paired mates, the allocation-free entry point (FAME_MAX_MATE)
 
 # every entry point declared in include/fame_hip.h
 EXPORTS = ("fame_open", "fame_close", "fame_last_error", "fame_index_info", "fame_match_reads",

@@ -40,6 +40,13 @@ __device__ __forceinline__ uint64_t rol64(uint64_t v, int s)
 
 // code of a byte in ntHash's reduced (C->T) table msTab (nthash.hpp:88-121):
 // 0 = seedA, 1 = seedG, 2 = seedT, 3 = zero row
+// verify pool of a read of L letters: 64-bit words - 1 for the short-read
+// kernels, FAME_LONG_POOL beyond FAME_DEV_MAX_READ
+__device__ __forceinline__ uint32_t pool_of_len(uint32_t L)
+{
+    return L > FAME_DEV_MAX_READ ? (uint32_t)FAME_LONG_POOL : (L + 63) / 64 - 1;
+}
+
 __device__ __forceinline__ uint32_t red_code(uint8_t c)
 {
     switch (c) {

@@ -53,7 +53,8 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                                               uint8_t* __restrict__ codes, uint32_t ori_mask,
                                               uint32_t* __restrict__ err_flag, uint32_t* __restrict__ max_len,
                                               uint32_t nsplit, uint32_t minlen, int paired,
-                                              const uint32_t* __restrict__ n_dev)
+                                              const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ long_n,
+                                              uint32_t* __restrict__ long_list)
 {
     // per byte value c: complement (bits 0-7), red_code(c) (8-9),
     // red_code(complement) (10-11), c == 'N' (12), c in A C G T N (13)
@@ -121,12 +122,37 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
             } else if (L < minlen) {
                 st = 1; // readSize < ceil(0.75 READLEN) (ReadQueue.cpp:971-980); minlen >= KMERLEN
             }
-            if (L > FAME_DEV_MAX_READ) {
-                if (lane == 0) atomicOr(err_flag, 1u);
-                st = 3;
-            }
             uint8_t* c0 = codes + (size_t)(2 * r) * 256;
             uint8_t* c1 = c0 + 256;
+            if (L > FAME_DEV_MAX_READ) {
+                // a long read (single-end, plain form): validated and
+                // reverse-complemented here, listed for the long-read path;
+                // no row (its seed-stage rows get length 0).  Otherwise (the
+                // batched form, paired mates) FLAG_TOO_LONG.
+                if (!paired && long_list && st == 0) {
+                    const uint8_t* sp = seqs + off_of(i);
+                    bool nflag = false;
+                    for (uint32_t p = (uint32_t)lane; p < L; p += 64) {
+                        const uint32_t t = s_lt[sp[p]];
+                        nflag = nflag || ((t >> 12) & 1u);
+                        qrev[L - 1 - p] = (uint8_t)t;
+                    }
+                    if (__ballot(nflag)) st = 3;
+                    if (st == 0 && lane == 0) {
+                        long_list[atomicAdd(long_n, 1u)] = r;
+                        atomicMax(long_n + 1, L); // the longest of them
+                    }
+                } else {
+                    if (lane == 0) atomicOr(err_flag, 1u);
+                    st = 3;
+                }
+                if (lane == 0) {
+                    pre[r] = st;
+                    c0[255] = 0;
+                    c1[255] = 0;
+                }
+                continue;
+            }
             if (st == 0 || (paired && st == 1)) {
                 bool nflag = false;
                 // unknown characters stay '\0' in revSeq (branch-free per letter)
@@ -161,14 +187,14 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
 // k_seed: seed + filter + map order for one (read, orientation) per wave
 // ---------------------------------------------------------------------------
 
-template <typename TabT, typename IdxT>
+template <typename TabT, typename IdxT, typename EkmT = uint8_t>
 struct SeedWork {
     TabT* klo;      // [K] bucket start
     uint32_t* klen; // [K] bucket length
     uint32_t* koff; // [K] exclusive prefix of klen
     uint32_t* k2;   // [K] second entry of a two-entry bucket read from ix.slot (NULL: no slots)
     uint32_t* ent;  // [cap_e] packed metaId<<1 | fwd
-    uint8_t* ekm;   // [cap_e] k-mer index of each entry
+    EkmT* ekm;      // [cap_e] k-mer index of each entry
     uint32_t* eaux; // [cap_e] kept / first / count (or hash slot, heavy)
     uint32_t cap_e;
     uint32_t* hkey; // heavy path: hash table keyed by packed entry
@@ -218,17 +244,21 @@ __device__ __forceinline__ void seed_stage(const DevIndex& ix, const SeedShared&
     }
 }
 
-template <typename TabT, typename IdxT, int KPL>
+// KPL > 0: the read's row of reduced letters arrives as this lane's word cw
+// (a 256-byte row, the length in its byte 255); KPL == 0 (long reads): the
+// row is already in w.code, L letters long, and the k-mers are hashed and
+// looked up 64 at a time (no register arrays)
+template <typename TabT, typename IdxT, int KPL, typename EkmT>
 __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
-                        const SeedWork<TabT, IdxT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
+                        const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
-                        const uint64_t* pre = nullptr)
+                        const uint64_t* pre = nullptr, uint32_t Llong = 0)
 {
     const int lane = threadIdx.x & 63;
     // cw: this lane's word of the row of reduced letters (the length is in
     // byte 255, the top byte of lane 63's word)
-    const uint32_t L = lane_u32(cw, 63) >> 24;
-    if (!staged) {
+    const uint32_t L = KPL == 0 ? Llong : lane_u32(cw, 63) >> 24;
+    if (KPL > 0 && !staged) {
         ((uint32_t*)w.code)[lane] = cw;
         wave_sync();
     }
@@ -239,39 +269,62 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     // slot per k-mer, or tabIndex[key], tabIndex[key + 1]; all gathers of a
     // lane are issued before the first is consumed.  staged: the slots were
     // loaded by seed_stage (pre)
-    TabT lo[KPL], hi[KPL];
-    uint64_t sl[KPL];
-#pragma unroll
-    for (int q = 0; q < KPL; ++q) {
-        const uint32_t k = lane + 64 * q;
-        lo[q] = hi[q] = 0;
-        sl[q] = staged ? pre[q] : 0;
-        if (k < K && !staged) {
-            const uint64_t key = seed_key(ix, sh, w.code, k);
-            if (inl) {
-                sl[q] = ix.slot[key];
-            } else {
-                lo[q] = tab[key];
-                hi[q] = tab[key + 1];
+    uint32_t bsum = 0;
+    if constexpr (KPL == 0) {
+        for (uint32_t k0 = 0; k0 < K; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            if (k < K) {
+                const uint64_t key = seed_key(ix, sh, w.code, k);
+                if (inl) {
+                    const uint64_t sv = ix.slot[key];
+                    const uint32_t len = slot_len(sv);
+                    w.klo[k] = (TabT)slot_lo(sv);
+                    w.k2[k] = slot_e1(sv);
+                    w.klen[k] = len;
+                    bsum += len;
+                } else {
+                    const TabT lo = tab[key], hi = tab[key + 1];
+                    w.klo[k] = lo;
+                    const uint64_t len = (uint64_t)(hi - lo);
+                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+                }
             }
         }
-    }
-    uint32_t bsum = 0;
-#pragma unroll
-    for (int q = 0; q < KPL; ++q) {
-        const uint32_t k = lane + 64 * q;
-        if (k < K) {
-            if (inl) {
-                const uint32_t len = slot_len(sl[q]);
-                w.klo[k] = (TabT)slot_lo(sl[q]);
-                w.k2[k] = slot_e1(sl[q]);
-                w.klen[k] = len;
-                bsum += len;
-            } else {
-                w.klo[k] = lo[q];
-                const uint64_t len = (uint64_t)(hi[q] - lo[q]);
-                w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
-                bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+    } else {
+        TabT lo[KPL], hi[KPL];
+        uint64_t sl[KPL];
+    #pragma unroll
+        for (int q = 0; q < KPL; ++q) {
+            const uint32_t k = lane + 64 * q;
+            lo[q] = hi[q] = 0;
+            sl[q] = staged ? pre[q] : 0;
+            if (k < K && !staged) {
+                const uint64_t key = seed_key(ix, sh, w.code, k);
+                if (inl) {
+                    sl[q] = ix.slot[key];
+                } else {
+                    lo[q] = tab[key];
+                    hi[q] = tab[key + 1];
+                }
+            }
+        }
+    #pragma unroll
+        for (int q = 0; q < KPL; ++q) {
+            const uint32_t k = lane + 64 * q;
+            if (k < K) {
+                if (inl) {
+                    const uint32_t len = slot_len(sl[q]);
+                    w.klo[k] = (TabT)slot_lo(sl[q]);
+                    w.k2[k] = slot_e1(sl[q]);
+                    w.klen[k] = len;
+                    bsum += len;
+                } else {
+                    w.klo[k] = lo[q];
+                    const uint64_t len = (uint64_t)(hi[q] - lo[q]);
+                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+                }
             }
         }
     }
@@ -313,7 +366,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     if (scatter) {
         for (uint32_t k = lane; k < K; k += 64) {
             const uint32_t o = w.koff[k], n = w.klen[k];
-            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (uint8_t)k;
+            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
         }
         wave_sync();
     }
@@ -336,7 +389,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
             w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
         else
             w.ent[j] = ix.kent[(uint64_t)w.klo[k] + d] & 0x7FFFFFFFu; // without isStart
-        if (!scatter) w.ekm[j] = (uint8_t)k;
+        if (!scatter) w.ekm[j] = (EkmT)k;
     }
     wave_sync();
     // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
@@ -473,7 +526,7 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     }
     // emit candidates: fwd-map windows then rev-map windows
     const uint32_t nf = ncand[0], nr = ncand[1];
-    const uint32_t pool = (L + 63) / 64 - 1;
+    const uint32_t pool = pool_of_len(L);
     uint32_t base = 0;
     if (nf + nr) {
         base = pool_take(bd, pc, pool, nf + nr, exact, lane);
@@ -717,13 +770,14 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         return p;
     };
     const uint32_t ce = hs.cap_e;
-    SeedWork<TabT, uint32_t> w;
-    w.klo = (TabT*)take(8 * FAME_DEV_MAX_READ);
-    w.klen = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
-    w.koff = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
-    w.k2 = (uint32_t*)take(4 * FAME_DEV_MAX_READ);
+    const size_t km = hs.kmax > FAME_DEV_MAX_READ ? hs.kmax : FAME_DEV_MAX_READ;
+    SeedWork<TabT, uint32_t, uint16_t> w;
+    w.klo = (TabT*)take(8 * km);
+    w.klen = (uint32_t*)take(4 * km);
+    w.koff = (uint32_t*)take(4 * km);
+    w.k2 = (uint32_t*)take(4 * km);
     w.ent = (uint32_t*)take(4ull * ce);
-    w.ekm = (uint8_t*)take(ce);
+    w.ekm = (uint16_t*)take(2ull * ce);
     w.eaux = (uint32_t*)take(4ull * ce);
     w.cap_e = ce;
     w.hsize = hs.hsize;
@@ -747,12 +801,26 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
             const uint32_t e = bd.heavy_e[hi];
             if (e < hs.e_lo || e > hs.e_hi) continue;
         }
-        const uint32_t ro = bd.heavy_ro[hs.first + hi];
+        const uint32_t hro = bd.heavy_ro[hs.first + hi];
         uint64_t E;
         uint32_t bc;
         PoolChunk pc = {0, 0, 0};
-        const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
-        const int rc = seed_one<TabT, uint32_t, SeedCap<256>::KPL>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
+        uint32_t ro;
+        int rc;
+        if (hro >> 31) {
+            // a long item: its row of reduced letters from k_long_measure
+            const uint32_t q = hro & 0x7FFFFFFFu;
+            const uint32_t r = bd.long_list[q >> 1];
+            ro = 2 * r + (q & 1);
+            w.code = bd.long_codes + (size_t)q * bd.long_stride;
+            rc = seed_one<TabT, uint32_t, 0>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, 0u, false, nullptr,
+                                             bd.lens[r]);
+        } else {
+            ro = hro;
+            w.code = s_code;
+            const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[threadIdx.x];
+            rc = seed_one<TabT, uint32_t, SeedCap<256>::KPL>(ix, bd, ro, sh, w, pc, true, ss, &E, &bc, cw);
+        }
         if (rc != 0 && threadIdx.x == 0) {
             atomicOr(bd.flags, FLAG_CAPACITY);
             bd.cand_nf[ro] = 0;
@@ -765,6 +833,71 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
         atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
+    }
+}
+
+// Long reads (more than FAME_DEV_MAX_READ letters, listed by k_prep): one wave
+// per (read, orientation) item q = 2 * (list index) + orientation writes the
+// item's row of reduced letters (the short path's 256-byte rows cannot hold
+// it), sums its bucket lengths as seed_one does (E, and bucketCount from the
+// u32 sum) and lists it for the heavy kernel (heavy_ro = 0x80000000 | q),
+// which the host sizes by E.  A read longer than any window text plus the
+// errors allowed (winlen + MISCOUNT + ADDMIS + NERR letters) can match no
+// window, so it is not seeded: k_select finds no candidate and counts it
+// unmatched, as FAME's ShiftAnd finds nothing.  A read over
+// FAME_LONG_MAX_READ that could match raises FLAG_TOO_LONG.
+template <typename TabT>
+__global__ __launch_bounds__(64) void k_long_measure(DevIndex ix, BatchDev bd)
+{
+    __shared__ SeedShared sh;
+    seed_table_init(sh);
+    const int lane = threadIdx.x;
+    const uint32_t nitems = 2 * *bd.long_n;
+    const TabT* tab = (const TabT*)ix.tab;
+    for (uint32_t q = blockIdx.x; q < nitems; q += gridDim.x) {
+        const uint32_t ori = q & 1;
+        if (!((bd.ori_mask >> ori) & 1)) continue;
+        const uint32_t r = bd.long_list[q >> 1];
+        const uint32_t L = bd.lens[r];
+        if (L > ix.winlen + MISCOUNT + ADDMIS + NERR) continue;
+        if (L > FAME_LONG_MAX_READ) {
+            if (lane == 0) atomicOr(bd.flags, FLAG_TOO_LONG);
+            continue;
+        }
+        const uint8_t* src = (ori ? bd.rev : bd.seqs) + bd.offs[r];
+        uint8_t* row = bd.long_codes + (size_t)q * bd.long_stride;
+        for (uint32_t j = lane; j < L; j += 64) row[j] = (uint8_t)red_code(src[j]);
+        wave_sync();
+        const uint32_t K = L - KMERLEN + 1;
+        uint64_t E = 0;
+        uint32_t bsum = 0;
+        for (uint32_t k = lane; k < K; k += 64) {
+            const uint64_t key = seed_key(ix, sh, row, k);
+            uint32_t len;
+            if (ix.slot) {
+                len = slot_len(ix.slot[key]);
+            } else {
+                const uint64_t d = (uint64_t)(tab[key + 1] - tab[key]);
+                len = (uint32_t)d; // bucketCount's u32 truncation
+                E += d > 0xFFFFFFull ? 0xFFFFFFull : d;
+                bsum += len;
+                continue;
+            }
+            E += len > 0xFFFFFFu ? 0xFFFFFFu : len;
+            bsum += len;
+        }
+        for (int o = 32; o > 0; o >>= 1) E += __shfl_xor(E, o, 64);
+        bsum = wave_sum_u32(bsum);
+        if (lane == 0) {
+            const uint32_t h = atomicAdd(bd.heavy_n, 1u);
+            if (h < bd.heavy_cap) {
+                bd.heavy_ro[h] = 0x80000000u | q;
+                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_bc[h] = bsum / 5;
+            } else {
+                atomicOr(bd.flags, FLAG_HEAVY_FULL);
+            }
+        }
     }
 }
 
@@ -1633,6 +1766,175 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
 }
 
+// Verify jobs of reads over FAME_DEV_MAX_READ letters (pool FAME_LONG_POOL):
+// one wave per job, lane w holding word w of the pattern's bit vectors (at
+// most 64 words: FAME_LONG_MAX_READ letters), the pattern top-aligned in its
+// W words as in verify_one (pattern position i at bit i + z).  The Myers
+// step's addition carries across words through the wave: each word's sum
+// without carry-in generates (overflow) or propagates (all ones) a carry,
+// and with G and P the masks of the two over the words, the carry into each
+// word is bit w of (G + (G | P)) ^ G ^ (G | P); the one-bit shifts take the
+// top bit of the word below.  The whole window is scanned (no piece
+// filter); N letters, run logic and reported positions as in verify_one.
+__global__ __launch_bounds__(64) void k_verify_long(DevIndex ix, BatchDev bd, const uint32_t* __restrict__ only,
+                                                    uint32_t n_only)
+{
+    constexpr uint32_t pool = FAME_LONG_POOL;
+    const int lane = threadIdx.x;
+    uint32_t njobs;
+    if (only) njobs = n_only != FAME_JOBS_ON_DEVICE ? n_only : min(*bd.ovf_n, bd.ovf_list_cap);
+    else njobs = min(bd.pool_n[pool], pc_get_cap(bd, pool));
+    if (bd.n_dev && (*bd.flags & FLAG_ABORT)) njobs = 0;
+    uint64_t jobs = 0, bytes = 0, wletters = 0;
+    const uint32_t cap = bd.match_cap;
+    for (uint32_t tid = blockIdx.x; tid < njobs; tid += gridDim.x) {
+        if (only && bd.ovf_pool[tid] != pool) continue;
+        const uint32_t j = only ? only[tid] : tid;
+        const uint32_t meta = bd.job_meta[pool][j];
+        if (meta == JOB_GAP) continue;
+        const uint32_t ro = bd.job_ro[pool][j];
+        const uint32_t r = ro >> 1, ori = ro & 1;
+        const bool rev = (meta >> 31) != 0;
+        const uint32_t id = meta & 0x3FFFFFFFu;
+        const bool unguarded = ((meta >> 30) & 1) != 0;
+        const uint32_t m = bd.lens[r];
+        const int W = (int)((m + 63) >> 6);
+        const int z = 64 * W - (int)m;
+        const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
+        // Peq of this lane's word (ShiftAnd loadBitmasks ShiftAnd.h:345-432:
+        // pattern T also matches text C)
+        uint64_t pe0 = 0, pe1 = 0, pe2 = 0, pe3 = 0;
+        if (lane < W) {
+            for (int b = 0; b < 64; ++b) {
+                const int i = 64 * lane + b - z;
+                if (i < 0 || i >= (int)m) continue;
+                const uint8_t ch = pat[i];
+                const uint64_t bit = 1ull << b;
+                pe0 |= ch == 'A' ? bit : 0ull;
+                pe1 |= (ch == 'C' || ch == 'T') ? bit : 0ull;
+                pe2 |= ch == 'G' ? bit : 0ull;
+                pe3 |= ch == 'T' ? bit : 0ull;
+            }
+        }
+        // the text (verify_one's bounds)
+        const uint4 win = ix.win[id];
+        const uint32_t startPos = win.x, chrom = win.y;
+        const uint64_t clen = ix.clen[chrom];
+        const uint64_t span = (uint64_t)ix.winlen + MISCOUNT + ADDMIS - 1;
+        uint64_t tstart, tlen;
+        if (!rev) {
+            uint64_t end = (uint64_t)startPos + span;
+            if (end > clen) end = clen;
+            tstart = startPos;
+            tlen = end - startPos;
+        } else {
+            const uint64_t e0 = startPos > 0 ? (uint64_t)startPos - 1 : 0;
+            uint64_t s0 = (uint64_t)startPos + span;
+            if (s0 >= clen) s0 = clen - 1;
+            tstart = e0 + 1;
+            tlen = s0 > e0 ? s0 - e0 : 0;
+            if (unguarded && startPos == 0) {
+                tstart = 0;
+                tlen = s0 + 1;
+            }
+        }
+        const int T = (int)tlen;
+        const uint64_t* __restrict__ pk = ix.pk + ix.pkoff[chrom];
+        const uint64_t wmask = W >= 64 ? ~0ull : ((1ull << W) - 1);
+        uint64_t pv = lane < W ? ~0ull : 0ull, mv = 0;
+        const int score0 = 64 * W;
+        const int thr = (int)NERR + z;
+        int score = score0;
+        bool was = false;
+        uint32_t prevErrs = MISCOUNT + 1;
+        uint32_t ncomp = 0;
+        const uint32_t minc = m - NERR;
+        uint32_t nm = 0;
+        uint32_t runbest = 0;
+        uint32_t* outm = bd.job_match[pool] + (size_t)j * cap;
+        uint32_t* ovf = only ? bd.ovf_buf + bd.job_ovf_off[pool][j] : nullptr;
+        auto flush = [&]() {
+            const uint32_t k = nm - 1;
+            if (lane == 0) {
+                if (ovf) ovf[k] = runbest;
+                else if (k < cap) outm[k] = runbest;
+            }
+        };
+        uint64_t wv = 0;
+        int64_t widx = -1;
+        for (int sidx = 0; sidx < T; ++sidx) {
+            const int p = rev ? T - 1 - sidx : sidx;
+            const uint64_t x = tstart + (uint64_t)p;
+            if ((int64_t)(x >> 4) != widx) {
+                widx = (int64_t)(x >> 4);
+                wv = pk[widx];
+            }
+            const uint32_t sh = (uint32_t)(x & 15);
+            if ((wv >> (32 + sh)) & 1) { // an N restarts the automaton (ShiftAnd.h:142-146, 212-214)
+                pv = lane < W ? ~0ull : 0ull;
+                mv = 0;
+                score = score0;
+                continue;
+            }
+            uint32_t c = (uint32_t)(wv >> (2 * sh)) & 3u;
+            if (rev) c ^= 3u; // the reverse map scans the complement
+            const uint64_t eq = c == 0 ? pe0 : c == 1 ? pe1 : c == 2 ? pe2 : pe3;
+            const uint64_t xv = eq | mv;
+            const uint64_t s0 = (eq & pv) + pv;
+            const uint64_t G = __ballot(s0 < pv) & wmask;
+            const uint64_t P = __ballot(s0 == ~0ull) & wmask;
+            const uint64_t B = G | P;
+            const uint64_t C = (G + B) ^ G ^ B;
+            const uint64_t sum = s0 + ((C >> lane) & 1ull);
+            const uint64_t xh = (sum ^ pv) | eq;
+            const uint64_t ph = mv | ~(xh | pv);
+            const uint64_t mh = pv & xh;
+            const uint32_t phb = (uint32_t)(ph >> 63), mhb = (uint32_t)(mh >> 63);
+            score += (int)lane_u32(phb, W - 1) - (int)lane_u32(mhb, W - 1);
+            const uint32_t phc = __shfl_up(phb, 1, 64), mhc = __shfl_up(mhb, 1, 64);
+            const uint64_t ph2 = (ph << 1) | (lane ? (uint64_t)phc : 0ull);
+            const uint64_t mh2 = (mh << 1) | (lane ? (uint64_t)mhc : 0ull);
+            pv = lane < W ? (mh2 | ~(xv | ph2)) : 0ull;
+            mv = lane < W ? (ph2 & xv) : 0ull;
+            ++ncomp;
+            const bool valid = ncomp >= minc;
+            const bool acc = valid && score <= thr;
+            const uint32_t e = (uint32_t)(score - z);
+            const uint32_t pos = rev ? (uint32_t)(p + 1 + m - 1 + MISCOUNT) : (uint32_t)(p + MISCOUNT);
+            const uint32_t packed = (pos << 8) | e;
+            const bool newrun = acc && !was;
+            const bool upd = acc && (newrun || e <= prevErrs);
+            if (valid && !acc && was) flush();
+            nm += newrun ? 1u : 0u;
+            runbest = upd ? packed : runbest;
+            prevErrs = upd ? e : prevErrs;
+            was = valid ? acc : was;
+        }
+        if (was) flush();
+        if (!only && lane == 0) {
+            bd.job_nm[pool][j] = nm;
+            if (nm > cap) {
+                const uint32_t o = atomicAdd(bd.ovf_n, 1u);
+                if (o < bd.ovf_list_cap) {
+                    bd.ovf_pool[o] = pool;
+                    bd.ovf_job[o] = j;
+                } else {
+                    atomicOr(bd.flags, FLAG_OVF_FULL);
+                }
+            }
+        }
+        const uint64_t b = tlen ? tlen : 1;
+        jobs += 1;
+        bytes += b;
+        wletters += b * (uint64_t)W;
+    }
+    if (!only && lane == 0 && jobs) {
+        atomicAdd(&bd.stats[3], (unsigned long long)jobs);
+        atomicAdd(&bd.stats[4], (unsigned long long)bytes);
+        atomicAdd(&bd.stats[10], (unsigned long long)wletters);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_select: saQuerySeedSetRef uniqueness logic + decision block, per read
 // ---------------------------------------------------------------------------
@@ -1715,7 +2017,7 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
         uint64_t out = 0;
         uint8_t use = 0;
         if (bd.pre[r] == 0) {
-            const uint32_t pool = (bd.lens[r] + 63) / 64 - 1;
+            const uint32_t pool = pool_of_len(bd.lens[r]);
             uint64_t mF = 0, mR = 0;
             int sF = 0, sR = 0;
             if (bd.ori_mask & 1) sF = query_seed_set(ix, bd, 2 * r, pool, &mF);
@@ -1774,6 +2076,8 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
         if (st == 0 && ix.win[(uint32_t)(out >> 32)].z != CPGDUMMY) {
             if (((out >> 16) & 0xFF) == 0)
                 want = 1;
+            else if (bd.dp_long_list && bd.lens[r] > FAME_DEV_MAX_READ)
+                want = 3; // the DP of a long read runs on scratch sized for it
             else
                 want = 2;
         }
@@ -1781,16 +2085,19 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
     {
         const int lane = threadIdx.x & 63;
         const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2);
-        uint32_t basedir = 0, basedp = 0;
+        const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2), blong = __ballot(want == 3);
+        uint32_t basedir = 0, basedp = 0, baselong = 0;
         if (lane == 0) {
             if (bdir) basedir = atomicAdd(bd.direct_n, (uint32_t)__popcll(bdir));
             if (bdp) basedp = atomicAdd(bd.dp_n, (uint32_t)__popcll(bdp));
+            if (blong) baselong = atomicAdd(bd.dp_long_n, (uint32_t)__popcll(blong));
         }
         basedir = lane_u32(basedir, 0);
         basedp = lane_u32(basedp, 0);
+        baselong = lane_u32(baselong, 0);
         if (want == 1) bd.direct_list[basedir + __popcll(bdir & below)] = r;
         if (want == 2) bd.dp_list[basedp + __popcll(bdp & below)] = r;
+        if (want == 3) bd.dp_long_list[baselong + __popcll(blong & below)] = r;
     }
     __syncthreads();
     if (threadIdx.x < 5 && s_cnt[threadIdx.x])
@@ -1947,9 +2254,13 @@ __device__ __forceinline__ uint32_t dp_choice(int up, int left, int dg, int mm)
 // last part of the traceback; for rows >= 11 only the traceback choice of
 // slots 1..17 is kept (2 bits each, one word per row), computed during the
 // fill from the same stored values the traceback would read.
+// dec_scratch holds L + 1 choice words per lane, trace_scratch
+// (trace_cap + 31) / 32 trace words (lane-strided): launched for the reads of
+// at most FAME_DEV_MAX_READ letters with FAME_DEV_TRACE_CAP, and for the long
+// reads' list with scratch and cap sized for the longest of them
 __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint16_t* __restrict__ dp_scratch,
                                                  uint64_t* __restrict__ dec_scratch,
-                                                 uint64_t* __restrict__ trace_scratch)
+                                                 uint64_t* __restrict__ trace_scratch, uint32_t trace_cap)
 {
     const uint32_t n = *bd.dp_n;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2124,7 +2435,7 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
         int tn = 0;
         uint64_t tw = 0;
         auto put = [&](uint32_t tt) {
-            if (tn < FAME_DEV_TRACE_CAP) {
+            if (tn < trace_cap) {
                 tw |= (uint64_t)tt << (2 * (tn & 31));
                 if ((tn & 31) == 31) {
                     TR[(size_t)(tn >> 5) * stride] = tw;
@@ -2160,7 +2471,7 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             }
             col = (unsigned long)(rr + k - 9);
         }
-        while (rr > 0 && !bad && tn <= FAME_DEV_TRACE_CAP) { // rows <= 10 from the kept values
+        while (rr > 0 && !bad && tn <= trace_cap) { // rows <= 10 from the kept values
             const uint8_t rc = rv ? seq[rr - 1] : seq[L - rr];
             const uint8_t gc = col_at(cs, clen, ref_end, (long)col);
             const int mm = rv ? cmp_rev(rc, gc) : cmp_fwd(rc, gc);
@@ -2173,13 +2484,13 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             if (tt != 2) --col;
             put(tt);
         }
-        while (col > 0 && tn <= FAME_DEV_TRACE_CAP) {
+        while (col > 0 && tn <= trace_cap) {
             put(3);
             --col;
         }
         if (rr > 0 || col > 0) bad = true;
-        if ((tn & 31) && tn <= FAME_DEV_TRACE_CAP) TR[(size_t)(tn >> 5) * stride] = tw;
-        if (tn > FAME_DEV_TRACE_CAP || bad) {
+        if ((tn & 31) && tn <= trace_cap) TR[(size_t)(tn >> 5) * stride] = tw;
+        if (tn > trace_cap || bad) {
             atomicOr(bd.flags, FLAG_DP_BROKEN);
             continue;
         }
@@ -2375,7 +2686,7 @@ void launch_prep(const BatchDev& bd, uint32_t n, uint32_t readlen, hipStream_t s
     const uint32_t blocks = (n + 255) / 256; // 4 waves per block, 64 reads per wave
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n, readlen, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
-                       0xFFFFFFFFu, 0u, 0, bd.n_dev);
+                       0xFFFFFFFFu, 0u, 0, bd.n_dev, bd.long_n, bd.long_list);
 }
 
 // paired batches: 2n mate reads, mate 2 from nsplit on; pre[] = 1 short | 2 N
@@ -2386,7 +2697,7 @@ void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, ui
     const uint32_t blocks = (n_reads + 255) / 256;
     hipLaunchKernelGGL(k_prep, dim3(blocks < 16384 ? blocks : 16384), dim3(256), 0, st, bd.seqs, bd.offs,
                        bd.lens, n_reads, 0u, bd.rev, bd.pre, bd.codes, bd.ori_mask, bd.flags, bd.max_len,
-                       nsplit, minlen, 1, (const uint32_t*)nullptr);
+                       nsplit, minlen, 1, (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
 }
 
 void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st)
@@ -2483,7 +2794,13 @@ void launch_verify(const DevIndex& ix, const BatchDev& bd, uint32_t pool, uint32
     case 0: hipLaunchKernelGGL(k_verify<1>, g, b, 0, st, ix, bd, 0u, only, only ? njobs : 0u); break;
     case 1: hipLaunchKernelGGL(k_verify<2>, g, b, 0, st, ix, bd, 1u, only, only ? njobs : 0u); break;
     case 2: hipLaunchKernelGGL(k_verify<3>, g, b, 0, st, ix, bd, 2u, only, only ? njobs : 0u); break;
-    default: hipLaunchKernelGGL(k_verify<4>, g, b, 0, st, ix, bd, 3u, only, only ? njobs : 0u); break;
+    case 3: hipLaunchKernelGGL(k_verify<4>, g, b, 0, st, ix, bd, 3u, only, only ? njobs : 0u); break;
+    default: {
+        // one wave per job
+        const uint32_t nb = njobs == FAME_JOBS_ON_DEVICE ? (blocks ? blocks : 1024u) : (njobs < 65536u ? njobs : 65536u);
+        hipLaunchKernelGGL(k_verify_long, dim3(nb), dim3(64), 0, st, ix, bd, only, only ? njobs : 0u);
+        break;
+    }
     }
 }
 
@@ -2514,7 +2831,27 @@ void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* d
     const uint32_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_meth_direct, dim3(g < 8192 ? g : 8192), dim3(256), 0, st, ix, bd);
     if (between) (void)hipEventRecord(between, st);
-    hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dec, trace);
+    hipLaunchKernelGGL(k_meth_dp, dim3(grid_threads / 256), dim3(256), 0, st, ix, bd, dp, dec, trace,
+                       (uint32_t)FAME_DEV_TRACE_CAP);
+}
+
+void launch_meth_dp_long(const DevIndex& ix, const BatchDev& bd, uint16_t* dp, uint64_t* dec, uint64_t* trace,
+                         uint32_t lanes, uint32_t trace_cap, hipStream_t st)
+{
+    BatchDev b = bd;
+    b.dp_list = bd.dp_long_list;
+    b.dp_n = bd.dp_long_n;
+    hipLaunchKernelGGL(k_meth_dp, dim3(lanes / 256), dim3(256), 0, st, ix, b, dp, dec, trace, trace_cap);
+}
+
+void launch_long_measure(const DevIndex& ix, const BatchDev& bd, uint32_t items, hipStream_t st)
+{
+    if (!items) return;
+    const dim3 g(items < 4096 ? items : 4096), b(64);
+    if (ix.tab_is64)
+        hipLaunchKernelGGL(k_long_measure<uint64_t>, g, b, 0, st, ix, bd);
+    else
+        hipLaunchKernelGGL(k_long_measure<uint32_t>, g, b, 0, st, ix, bd);
 }
 
 void launch_ovf_offsets(const BatchDev& bd, uint32_t n_ovf, uint32_t* total, hipStream_t st)

@@ -143,6 +143,9 @@ struct fame_ctx {
     uint64_t graph_calls = 0, graph_fallbacks = 0, graph_captures = 0;
     uint32_t pipe_batch = 1u << 21; // sub-batch of a large batched call (two lanes)
     DevBuf b_pvalid, b_pmat, b_pstatus;  // outputs of a pipelined call
+    // reads over FAME_DEV_MAX_READ letters: counters (long_n, longest, DP
+    // count), list, rows, DP list and DP scratch
+    DevBuf b_long, b_long_list, b_long_codes, b_dp_long, b_ldp, b_ldec, b_ltrace;
     HostStage h_offs;                    // host-buffer calls: read offsets rebased to the span copied
     uint64_t pipe_calls = 0;
     double big_ms = 0; // paired seed runs with global scratch, last call
@@ -199,7 +202,7 @@ void load_index(fame_ctx* c, const std::string& path)
     if (c->htab == 0) throw FameError{FAME_ERR_FORMAT, "index hash table size is 0"};
     if (c->seedbits != FAME_SEEDBITS)
         throw FameError{FAME_ERR_FORMAT, "index built with another spaced seed (SEEDBITS)"};
-    if (c->readlen <= 32 || c->readlen > FAME_MAX_READ)
+    if (c->readlen <= 32 || c->readlen > FAME_MAX_MATE)
         throw FameError{FAME_ERR_FORMAT, "index READLEN outside (32, 255]"};
     hipStream_t st = c->stream;
     // CpG table: {u8 chrom; 3 pad; u32 pos}
@@ -510,9 +513,14 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     HIPC(hipMemsetAsync(bd.stats, 0, 8 * FAME_NSTATS, st));
     bd.direct_list = grow<uint32_t>(c->b_direct, n + 1);
     bd.dp_list = grow<uint32_t>(c->b_dplist, n + 1);
-    bd.direct_n = grow<uint32_t>(c->b_methn, 2);
+    bd.direct_n = grow<uint32_t>(c->b_methn, 3);
     bd.dp_n = bd.direct_n + 1;
-    HIPC(hipMemsetAsync(bd.direct_n, 0, 8, st));
+    bd.dp_long_n = bd.direct_n + 2;
+    bd.dp_long_list = grow<uint32_t>(c->b_dp_long, n + 1);
+    HIPC(hipMemsetAsync(bd.direct_n, 0, 12, st));
+    bd.long_n = grow<uint32_t>(c->b_long, 4);
+    bd.long_list = grow<uint32_t>(c->b_long_list, n + 1);
+    HIPC(hipMemsetAsync(bd.long_n, 0, 8, st)); // long reads and the longest of them
 
     HIPC(hipMemsetAsync(bd.flags, 0, 8, st)); // flags and the longest read
     tm.mark(0);
@@ -521,6 +529,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     // seed: retried with larger pools when a pool overflowed
     uint32_t pool_n[FAME_NPOOL];
     uint32_t heavy_n = 0, flags = 0;
+    uint32_t long_info[2] = {0, 0}; // long reads, the longest of them
     for (int attempt = 0;; ++attempt) {
         for (int p = 0; p < FAME_NPOOL; ++p) {
             bd.pool_cap[p] = c->pool_cap[p];
@@ -535,9 +544,22 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         HIPC(hipMemcpyAsync(pool_n, bd.pool_n, 4 * FAME_NPOOL, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&heavy_n, bd.heavy_n, 4, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(long_info, bd.long_n, 8, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
+        if (long_info[0]) {
+            // long reads: their rows and bucket sums, then the heavy list
+            // holds them too
+            const uint32_t lmax = std::min<uint32_t>(long_info[1], FAME_LONG_MAX_READ);
+            bd.long_stride = (lmax + 63) & ~63u;
+            bd.long_codes = grow<uint8_t>(c->b_long_codes, 2ull * long_info[0] * bd.long_stride + 64);
+            launch_long_measure(c->ix, bd, 2 * long_info[0], st);
+            HIPC(hipMemcpyAsync(&heavy_n, bd.heavy_n, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+        }
         if (flags & FLAG_TOO_LONG) {
-            set_err(c, "read longer than FAME_MAX_READ");
+            set_err(c, "a read longer than FAME_MAX_READ letters could match a window (WINLEN over "
+                       "FAME_MAX_READ - 12)");
             return FAME_ERR_ARG;
         }
         if (flags & FLAG_HEAVY_FULL) {
@@ -594,6 +616,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
                 }
                 HeavyScratch hs{};
                 hs.cap_e = (uint32_t)emax;
+                if (long_info[0])
+                    hs.kmax = std::min<uint32_t>(long_info[1], FAME_LONG_MAX_READ); // >= its k-mers
                 hs.hsize = (uint32_t)rup(2 * emax);
                 uint64_t tbb = std::max(rup(bcmax), rup(2 * emax));
                 hs.tb_cap = (uint32_t)(2 * tbb + 64);
@@ -678,6 +702,16 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         uint64_t* tr = grow<uint64_t>(c->b_trace, (size_t)FAME_DP_TRACE_WORDS * c->meth_threads);
         tm.mark(8);
         launch_meth(c->ix, bd, n, dp, dec, tr, c->meth_threads, st, tm.e[16]);
+        if (long_info[0]) {
+            // the long reads' DP on scratch sized for the longest of them
+            const uint32_t lmax = std::min<uint32_t>(long_info[1], FAME_LONG_MAX_READ);
+            const uint32_t lanes = std::min<uint32_t>(8192, std::max<uint32_t>(256, (long_info[0] + 255) & ~255u));
+            const uint32_t cap = 2 * lmax + 16;
+            uint16_t* ldp = grow<uint16_t>(c->b_ldp, cells * lanes);
+            uint64_t* ldec = grow<uint64_t>(c->b_ldec, (size_t)(lmax + 1) * lanes);
+            uint64_t* ltr = grow<uint64_t>(c->b_ltrace, (size_t)(cap / 32 + 1) * lanes);
+            launch_meth_dp_long(c->ix, bd, ldp, ldec, ltr, lanes, cap, st);
+        }
         tm.mark(9);
     }
     uint64_t cnt[5];
@@ -1373,7 +1407,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         HIPC(hipMemcpyAsync(&flags, bd.flags, 4, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         if (flags & FLAG_TOO_LONG) {
-            set_err(c, "read longer than FAME_MAX_READ");
+            set_err(c, "a mate is longer than FAME_MAX_MATE");
             return FAME_ERR_ARG;
         }
         if (flags & FLAG_HEAVY_FULL) {
@@ -1752,7 +1786,7 @@ int na_run(fame_ctx* c, uint8_t* base, const NaLayout& L, const uint8_t* d_seqs,
     }
     if (flags & FLAG_TOO_LONG) {
         HIPC(hipStreamSynchronize(st));
-        set_err(c, "a read is longer than FAME_MAX_READ");
+        set_err(c, "reads over FAME_MAX_MATE letters need fame_match_reads or fame_match_reads_device");
         return FAME_ERR_ARG;
     }
     if (flags & FLAG_ABORT) {
@@ -1809,6 +1843,7 @@ int serve_run(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
 hipStream_t serve_stream(fame_ctx* c) { return c->stream; }
 int serve_device(fame_ctx* c) { return c->device; }
 uint32_t serve_readlen(fame_ctx* c) { return c->readlen; }
+uint32_t serve_winlen(fame_ctx* c) { return c->winlen; }
 void serve_set_err(fame_ctx* c, const std::string& m) { set_err(c, m); }
 
 } // namespace fame
@@ -1880,8 +1915,8 @@ int fame_match_reads(fame_ctx* c, const uint8_t* seqs, const uint64_t* offs, con
         // rebased to it
         uint64_t lo = ~0ull, hi = 0;
         for (uint32_t i = 0; i < n; ++i) {
-            if (lens[i] > FAME_MAX_READ) {
-                set_err(c, "read " + std::to_string(i) + " is longer than FAME_MAX_READ");
+            if (lens[i] > FAME_MAX_READ && lens[i] <= c->winlen + 12) {
+                set_err(c, "read " + std::to_string(i) + " is longer than FAME_MAX_READ and could match a window");
                 return (int)FAME_ERR_ARG;
             }
             lo = std::min<uint64_t>(lo, offs[i]);
@@ -2026,8 +2061,8 @@ int fame_match_pairs(fame_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, c
         // the other; the offsets are rebased to them
         uint64_t lo1 = ~0ull, hi1 = 0, lo2 = ~0ull, hi2 = 0;
         for (uint32_t i = 0; i < n; ++i) {
-            if (lens1[i] > FAME_MAX_READ || lens2[i] > FAME_MAX_READ) {
-                set_err(c, "mate of pair " + std::to_string(i) + " is longer than FAME_MAX_READ");
+            if (lens1[i] > FAME_MAX_MATE || lens2[i] > FAME_MAX_MATE) {
+                set_err(c, "mate of pair " + std::to_string(i) + " is longer than FAME_MAX_MATE");
                 return (int)FAME_ERR_ARG;
             }
             lo1 = std::min<uint64_t>(lo1, offs1[i]);

@@ -10,12 +10,18 @@
 
 struct fame_ctx;
 
+// reads of at most FAME_DEV_MAX_READ letters take the short-read kernels;
+// longer ones (up to FAME_LONG_MAX_READ) the long-read path: seeded by the
+// heavy kernel from rows of their own, verified one wave per job (pool
+// FAME_LONG_POOL), aligned by the DP kernel on scratch sized for them
 #define FAME_DEV_MAX_READ 255
+#define FAME_LONG_MAX_READ 4096
+#define FAME_LONG_POOL 4
 #define FAME_DEV_TRACE_CAP (2 * FAME_DEV_MAX_READ + 16)
 #define FAME_PE_MAP_CHUNK 256 // map pool entries a paired seed wave reserves at a time
 #define FAME_PE_WCAP 8         // windows of an item k_pe_walk1 walks in LDS, one lane per item
 #define FAME_PE_WBIG 1360      // windows of an item k_pe_walk1_big walks in LDS, one wave per item
-#define FAME_NPOOL 4 // verify pools by 64-bit words of the read: 1..4
+#define FAME_NPOOL 5 // verify pools by 64-bit words of the read: 1..4, then reads over FAME_DEV_MAX_READ
 #define FAME_NSTATS 11
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
 #define FAME_SEED_WAVES 16384      // persistent waves of the seed kernel
@@ -115,7 +121,18 @@ struct BatchDev {
     uint32_t ovf_list_cap;
     uint32_t* ovf_buf;
     uint32_t ovf_buf_cap; // batched form: entries of ovf_buf (0: sized by the host)
-    // heavy (read, orientation) pairs deferred from the LDS seed kernel
+    // reads over FAME_DEV_MAX_READ letters (plain form; NULL: k_prep flags
+    // them FLAG_TOO_LONG): their list, and per long item q = 2 * (index in
+    // the list) + orientation a row of reduced letters (long_stride bytes)
+    uint32_t* long_n;
+    uint32_t* long_list;
+    uint8_t* long_codes;
+    uint32_t long_stride;
+    // DP reads over FAME_DEV_MAX_READ letters (k_select; NULL: in dp_list)
+    uint32_t* dp_long_list;
+    uint32_t* dp_long_n;
+    // heavy (read, orientation) pairs deferred from the LDS seed kernel, and
+    // long items (heavy_ro = 0x80000000 | q)
     uint32_t* heavy_n;
     uint32_t* heavy_ro;
     uint32_t* heavy_e;
@@ -194,6 +211,7 @@ struct HeavyScratch {
     size_t stride;
     uint32_t first, count;
     uint32_t cap_e, hsize, tb_cap;
+    uint32_t kmax = 0; // k-mers per item at most (0: FAME_DEV_MAX_READ; more for long items)
     uint32_t e_lo, e_hi; // device-list mode: only the items with e_lo <= E <= e_hi
 };
 
@@ -204,12 +222,13 @@ inline size_t heavy_scratch_stride(const HeavyScratch& hs)
     size_t stride = 0;
     auto add = [&](size_t b) { stride += (b + 255) & ~(size_t)255; };
     const size_t e = hs.cap_e;
-    add(8 * FAME_DEV_MAX_READ); // klo
-    add(4 * FAME_DEV_MAX_READ); // klen
-    add(4 * FAME_DEV_MAX_READ); // koff
-    add(4 * FAME_DEV_MAX_READ); // k2
+    const size_t km = hs.kmax > FAME_DEV_MAX_READ ? hs.kmax : FAME_DEV_MAX_READ;
+    add(8 * km);                // klo
+    add(4 * km);                // klen
+    add(4 * km);                // koff
+    add(4 * km);                // k2
     add(4 * e);                 // ent
-    add(e);                     // ekm
+    add(2 * e);                 // ekm (u16)
     add(4 * e);                 // eaux
     for (int i = 0; i < 3; ++i) add(4ull * hs.hsize); // hkey, hfirst, hcnt
     add(8 * e);                 // ins_key
@@ -256,6 +275,13 @@ void launch_select(const DevIndex& ix, const BatchDev& bd, uint32_t n, int get_s
 void launch_meth(const DevIndex& ix, const BatchDev& bd, uint32_t n, uint16_t* dp, uint64_t* dec, uint64_t* trace,
                  uint32_t grid_threads, hipStream_t st, hipEvent_t between = nullptr);
 uint32_t meth_dp_lanes();
+// the banded DP of the long reads' list (bd.dp_long_list / dp_long_n) on
+// `lanes` lanes whose scratch holds L + 1 choice words and trace_cap / 32 + 1
+// trace words for the longest of them
+void launch_meth_dp_long(const DevIndex& ix, const BatchDev& bd, uint16_t* dp, uint64_t* dec, uint64_t* trace,
+                         uint32_t lanes, uint32_t trace_cap, hipStream_t st);
+// rows, bucket sums and heavy-list entries of the 2 * long_n long items
+void launch_long_measure(const DevIndex& ix, const BatchDev& bd, uint32_t items, hipStream_t st);
 void launch_prep_pairs(const BatchDev& bd, uint32_t n_reads, uint32_t nsplit, uint32_t minlen, hipStream_t st);
 void launch_pe_seed(const DevIndex& ix, const BatchDev& bd, const PairDev& pd, uint32_t grid, hipStream_t st);
 void launch_pe_walk1(const DevIndex& ix, const BatchDev& bd1, const BatchDev& bd2, const PairDev& pd, hipStream_t st);
@@ -269,6 +295,7 @@ int serve_run(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
 hipStream_t serve_stream(fame_ctx* c);
 int serve_device(fame_ctx* c);
 uint32_t serve_readlen(fame_ctx* c);
+uint32_t serve_winlen(fame_ctx* c);
 void serve_set_err(fame_ctx* c, const std::string& m);
 void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st);
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,

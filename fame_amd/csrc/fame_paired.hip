@@ -637,7 +637,7 @@ __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& p
                     }
                 }
                 if (nf + nr) {
-                    const uint32_t pool = (LA + 63) / 64 - 1;
+                    const uint32_t pool = pool_of_len(LA);
                     uint32_t jb = nj ? pool_take(bd, pcA, pool, nj, false, lane) : 0;
                     if (jb == ~0u) {
                         if (lane == 0) atomicOr(bd.flags, FLAG_POOL_FULL);
@@ -1058,7 +1058,7 @@ __device__ void pe_sort(PeItem* a, int n, int sec, int* stk)
 // the walks of saQuerySeedSetRefFirst / Second over verified windows
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t pool_of(uint32_t L) { return (L + 63) / 64 - 1; }
+__device__ __forceinline__ uint32_t pool_of(uint32_t L) { return pool_of_len(L); }
 
 __device__ __forceinline__ uint32_t jm_at(const BatchDev& bd, uint32_t pool, uint32_t j, uint32_t k)
 {

@@ -265,7 +265,8 @@ int fame_serve_open(fame_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_reads
         hbm_budget = fr / 2;
     }
     // two staging slots in flight: the budget covers both batches
-    uint64_t by_budget = hbm_budget / (2 * (BYTES_PER_READ + 2 * FAME_MAX_READ));
+    // budgeted at FAME_MAX_MATE letters per read (longer reads take more of the byte budget)
+    uint64_t by_budget = hbm_budget / (2 * (BYTES_PER_READ + 2 * FAME_MAX_MATE));
     uint64_t mr = max_batch_reads ? max_batch_reads : 2000000u;
     mr = std::min<uint64_t>(mr, by_budget);
     if (mr < 1024) {
@@ -273,7 +274,7 @@ int fame_serve_open(fame_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_reads
         return FAME_ERR_CAPACITY;
     }
     s->max_reads = (uint32_t)mr;
-    s->max_bytes = mr * FAME_MAX_READ;
+    s->max_bytes = mr * FAME_MAX_MATE;
     bool ok = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) == hipSuccess;
     for (Slot& sl : s->slot) {
         ok = ok && hipHostMalloc((void**)&sl.h_seqs, s->max_bytes + 64, 0) == hipSuccess;
@@ -313,13 +314,21 @@ int fame_serve_submit(fame_server* s, const uint8_t* seqs, const uint64_t* offs,
     r->valid = valid;
     r->mat = mat;
     bool contig = true;
+    const uint32_t winlen = fame::serve_winlen(s->ctx);
     for (uint32_t i = 0; i < n; ++i) {
-        if (lens[i] > FAME_MAX_READ) return FAME_ERR_ARG;
+        if (lens[i] > FAME_MAX_READ && lens[i] <= winlen + 12) {
+            fame::serve_set_err(s->ctx, "a read longer than FAME_MAX_READ could match a window");
+            return FAME_ERR_ARG;
+        }
         r->bytes += lens[i];
         if (i && offs[i] != offs[i - 1] + lens[i - 1]) contig = false;
     }
     r->contiguous = contig;
-    if (n > s->max_reads || r->bytes > s->max_bytes) return FAME_ERR_CAPACITY;
+    if (n > s->max_reads || r->bytes > s->max_bytes) {
+        fame::serve_set_err(s->ctx, "a request holds more reads or letters than one batch (max_batch_reads, "
+                                    "255 letters per read)");
+        return FAME_ERR_CAPACITY;
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;
     *ticket = r->id;

@@ -33,9 +33,18 @@ enum {
     FAME_ERR_CAPACITY = -5  /* a per-read structure outgrew the device limits */
 };
 
-/* longest read the device path accepts; longer reads make fame_match_reads
- * fail with FAME_ERR_ARG (the reference has no limit) */
-#define FAME_MAX_READ 255
+/* Single-end reads of any length are matched as FAME matches them (it has no
+ * limit): up to 255 letters by the short-read kernels, up to FAME_MAX_READ by
+ * the long-read path (plain form; the batched form re-runs a batch holding
+ * one on the plain path), and longer ones are reported unmatched without
+ * being seeded when they exceed the index's window text plus the errors
+ * allowed (WINLEN + 12 letters: no window can hold a match, as FAME's
+ * ShiftAnd finds none).  A longer read that could match (an index with
+ * WINLEN over FAME_MAX_READ - 12) makes the call fail with FAME_ERR_ARG.
+ * Paired mates are limited to FAME_MAX_MATE letters (FAME_ERR_ARG beyond),
+ * and so is fame_match_reads_noalloc. */
+#define FAME_MAX_READ 4096
+#define FAME_MAX_MATE 255
 
 /* index header / table sizes, in this order:
  * htab, readlen, winlen, kmerlen, kmercutoff, seedbits, ncpg, nchrom, nkmer,

@@ -62,6 +62,12 @@ CASES = {
     "se_long": dict(chroms=[60000, 45000], repeats=0, htab=1 << 16, readlen=100, chunk=300000,
                     n=3000, seed=16, unord=False, flip=False, len_range=(120, 255), gen_len=255,
                     share="se_basic"),
+    # reads of 256..1200 letters (beyond the device's short-read kernels) mixed
+    # with the usual special reads of READLEN letters, and reads of 2 100 to
+    # 5 000 letters, longer than any window text plus the errors allowed
+    "se_xlong": dict(chroms=[60000, 45000], repeats=0, htab=1 << 16, readlen=100, chunk=300000,
+                     n=2500, seed=19, unord=False, flip=False, len_range=(256, 1200), gen_len=1200,
+                     share="se_basic", xlong=True),
     # an index with a blacklist: KMERCUTOFF 24 and planted poly-A / poly-T runs
     # and motif families (16 A then a 16-letter tail), so that filterHashTable
     # (RefGenome.cpp:2267-2352) removes k-mers and filteredKmers holds codes
@@ -222,6 +228,15 @@ def make_reads(cfg, chroms):
         s = c[p:p + RL + 10].copy()
         s[s == ord("C")] = ord("T")
         seqs[i] = bytes(s)
+    if cfg.get("xlong"):
+        # longer than a window text can hold (WINLEN 2048 + errors): never matched
+        for j, i in enumerate(pick[5 * k:5 * k + 12]):
+            c = chroms[j % len(chroms)]
+            L = int(rng.integers(2100, 5000))
+            p = int(rng.integers(0, c.size - L))
+            s = c[p:p + L].copy()
+            s[s == ord("C")] = ord("T")
+            seqs[i] = bytes(s)
     if cfg.get("motifs"):
         sites = []
         make_genome(cfg, sites)

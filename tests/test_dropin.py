@@ -84,7 +84,8 @@ def counters(out, paired):
             g("Fully matched pairs"), g("Reads discarded as too short")]
 
 
-SE = ["se_basic", "se_ga_chunked", "se_filt", "se_noloss", "se_repeats", "se_unord", "se_long", "se_rl150"]
+SE = ["se_basic", "se_ga_chunked", "se_filt", "se_noloss", "se_repeats", "se_unord", "se_long", "se_rl150",
+      "se_xlong"]
 PE = ["pe_basic", "pe_ga_chunked", "pe_filt", "pe_noloss", "pe_repeats", "pe_unord", "pe_rl150"]
 
 

@@ -72,7 +72,9 @@ def test_graph_replays_equal_plain(golden_case):
         ctx.set_form("plain")
         ctx.match(*b, False, np.zeros(3, np.uint64))
     info = ctx.form_info()
-    assert info["graph_launches"] == 6 and info["fallbacks"] == 0
+    # a batch holding a read over 255 letters runs on the plain path
+    long_reads = bool((c.lens[half:] > _lib.MAX_MATE).any())
+    assert info["graph_launches"] == 6 and info["fallbacks"] == (6 if long_reads else 0)
     ctx.close()
 
 

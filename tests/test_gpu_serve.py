@@ -32,7 +32,7 @@ def decided_context(c, first):
     return ctx
 
 
-@pytest.mark.parametrize("name", ["se_repeats", "se_long", "se_unord"])
+@pytest.mark.parametrize("name", ["se_repeats", "se_long", "se_unord", "se_xlong"])
 def test_serve_results_equal_alone_and_in_the_mix(name, tmp_path):
     from conftest import Case
     c = Case(name, str(tmp_path))
@@ -51,7 +51,9 @@ def test_serve_results_equal_alone_and_in_the_mix(name, tmp_path):
     a.close()
     # in the mix: every chunk queued at once, batched by the server
     b = decided_context(c, first)
-    srv = _lib.Server(b, max_batch_reads=2048)
+    # batches of 2048 reads, letters budgeted at 255 per read (more room for
+    # the chunks of reads of up to 5 000 letters)
+    srv = _lib.Server(b, max_batch_reads=16384 if (c.lens > 255).any() else 2048)
     tickets = [srv.submit(c.seqs, c.offs[s:e], c.lens[s:e]) for s, e in parts]
     mixed = [srv.wait(t) for t in tickets]
     st = srv.stats()

@@ -138,6 +138,13 @@ def test_noalloc_rejects_sizes_beyond_its_limits(tmp_path):
     o2[99] = MAX_BYTES + 50
     rc = noalloc(ctx, big, hip.put(o2), lens, 100, 1, cnt, 0, 0)
     assert rc == -4 and "FAME_NOALLOC_MAX_BYTES" in _lib.load().fame_last_error(ctx.h).decode()
+    # a read over 255 letters: the allocating entry points serve it
+    o3 = np.arange(100, dtype=np.uint64) * 100
+    l3 = np.full(100, 100, np.uint32)
+    l3[7] = 300
+    o3[8:] += 200
+    rc = noalloc(ctx, big, hip.put(o3), hip.put(l3), 100, 1, cnt, 0, 0)
+    assert rc == -4 and "FAME_MAX_MATE" in _lib.load().fame_last_error(ctx.h).decode()
     assert cnt.tolist() == [0, 0, 0]
     ctx.close()
     hip.free()
