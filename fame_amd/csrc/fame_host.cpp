@@ -97,6 +97,9 @@ struct GraphPlan;
 
 struct fame_ctx {
     std::string err;
+    // a serving worker: its own batch buffers, stream and counters over the
+    // parent's index and CpG accumulators
+    fame_ctx* parent = nullptr;
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr; // second lane of the batched form
@@ -157,6 +160,9 @@ void set_err(fame_ctx* c, const std::string& m)
 {
     if (c) c->err = m;
 }
+
+// the CpG accumulators the context's batches add to
+uint32_t* meth_buf(fame_ctx* c) { return (c->parent ? c->parent : c)->d_meth.as<uint32_t>(); }
 
 // ---------------------------------------------------------------------------
 // index loading (RefGenome::load RefGenome.cpp:230-387, read_strands :414-433)
@@ -466,6 +472,15 @@ uint32_t verify_queue_limit()
     return (uint32_t)strtoul(v, nullptr, 10);
 }
 
+// the heavy scratch (which the context already holds) can grow to `bytes`
+// with 1 GiB of device memory left over
+bool scratch_fits(fame_ctx* c, size_t bytes)
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+    return bytes + (1ull << 30) <= fr + c->b_heavy_scratch.bytes;
+}
+
 int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
               uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st)
 {
@@ -506,7 +521,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.mat = grow<uint64_t>(c->b_mat, n + 1);
     bd.valid = grow<uint8_t>(c->b_valid, n + 1);
     bd.counters = c->d_counters.as<uint64_t>();
-    bd.meth = c->d_meth.as<uint32_t>();
+    bd.meth = meth_buf(c);
     bd.flags = c->d_flags.as<uint32_t>();
     bd.max_len = bd.flags + 1;
     bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
@@ -574,8 +589,8 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             HIPC(hipMemcpy(hr.data(), bd.heavy_ro, 4 * heavy_n, hipMemcpyDeviceToHost));
             uint64_t emax_all = 1;
             for (uint32_t i = 0; i < heavy_n; ++i) emax_all = std::max<uint64_t>(emax_all, he[i]);
-            if (emax_all > (1ull << 26)) {
-                set_err(c, "a read has more than 2^26 index hits");
+            if (emax_all > 0x3FFFFFFFull) {
+                set_err(c, "a read has more than 2^30 index hits, or a bucket of more than 2^24 entries");
                 return FAME_ERR_CAPACITY;
             }
             // size classes (caps 1024 * 4^k): each run's scratch slices are
@@ -632,6 +647,11 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
                 need = std::max(need, stride * per);
                 classes.push_back({c0, c1, per, hs});
                 c0 = c1;
+            }
+            if (!scratch_fits(c, need)) {
+                set_err(c, "a read's index hits need " + std::to_string(need >> 20) +
+                               " MiB of scratch, more than the device has free");
+                return FAME_ERR_CAPACITY;
             }
             HIPC(c->b_heavy_scratch.reserve(need));
             tm.mark(12);
@@ -961,7 +981,7 @@ BatchDev plan_batch(fame_ctx* c, GraphPlan* pl, const uint8_t* d_seqs, const uin
     // single calls count into the context (strand tallies included); the
     // lanes of a pipelined call count into their own arrays
     bd.counters = pl->lane ? pl->counters.as<uint64_t>() : c->d_counters.as<uint64_t>();
-    bd.meth = c->d_meth.as<uint32_t>();
+    bd.meth = meth_buf(c);
     bd.direct_list = pl->direct.as<uint32_t>();
     bd.dp_list = pl->dplist.as<uint32_t>();
     bd.direct_n = pl->methn.as<uint32_t>();
@@ -1332,7 +1352,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
     bd.mat = grow<uint64_t>(c->b_mat, nr + 1);
     bd.valid = grow<uint8_t>(c->b_valid, nr + 1);
     bd.counters = c->d_counters.as<uint64_t>();
-    bd.meth = c->d_meth.as<uint32_t>();
+    bd.meth = meth_buf(c);
     bd.flags = c->d_flags.as<uint32_t>();
     bd.max_len = bd.flags + 1;
     bd.stats = grow<unsigned long long>(c->d_stats, FAME_NSTATS);
@@ -1421,7 +1441,7 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
         c->last_heavy = big_n;
         uint64_t cap_prev = 0;
         for (int round = 0; big_n && !(flags & FLAG_POOL_FULL); ++round) {
-            if (round >= 16) {
+            if (round >= 24) { // 1024 * 2^23 hits: beyond the 2^30 bound
                 flags |= FLAG_CAPACITY;
                 break;
             }
@@ -1434,13 +1454,20 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             // slices); what still overflows goes to larger runs
             emax = std::min<uint64_t>(emax, std::max<uint64_t>(1024, 2 * cap_prev));
             emax = std::max<uint64_t>(emax, std::max<uint64_t>(2 * cap_prev, 256));
-            if (emax > (1ull << 24)) {
-                set_err(c, "a mate has more than 2^24 index hits");
+            // a mate's hits are bounded by the device memory its scratch
+            // slice needs (FAME's maps grow on the heap)
+            if (emax > 0x3FFFFFFFull) {
+                set_err(c, "a mate has more than 2^30 index hits, or a bucket of more than 2^24 entries");
                 return FAME_ERR_CAPACITY;
             }
             cap_prev = emax;
             PairDev pb = pd;
             pe_scratch_dims(pb, (uint32_t)emax);
+            if (!scratch_fits(c, pb.stride)) {
+                set_err(c, "a mate's index hits need " + std::to_string(pb.stride >> 20) +
+                               " MiB of scratch, more than the device has free");
+                return FAME_ERR_CAPACITY;
+            }
             const size_t budget = 2ull << 30;
             pb.nslices = (uint32_t)std::max<size_t>(1, std::min<size_t>(big_n, budget / pb.stride));
             pb.scratch = grow<uint8_t>(c->b_heavy_scratch, pb.stride * pb.nslices);
@@ -1747,7 +1774,7 @@ int na_run(fame_ctx* c, uint8_t* base, const NaLayout& L, const uint8_t* d_seqs,
     bd.mat = (uint64_t*)at(L.mat);
     bd.valid = at(L.valid);
     bd.counters = c->d_counters.as<uint64_t>();
-    bd.meth = c->d_meth.as<uint32_t>();
+    bd.meth = meth_buf(c);
     bd.direct_list = (uint32_t*)at(L.direct);
     bd.dp_list = (uint32_t*)at(L.dplist);
     bd.direct_n = (uint32_t*)at(L.methn);
@@ -1844,6 +1871,49 @@ hipStream_t serve_stream(fame_ctx* c) { return c->stream; }
 int serve_device(fame_ctx* c) { return c->device; }
 uint32_t serve_readlen(fame_ctx* c) { return c->readlen; }
 uint32_t serve_winlen(fame_ctx* c) { return c->winlen; }
+
+fame_ctx* serve_worker_open(fame_ctx* c)
+{
+    auto w = std::make_unique<fame_ctx>();
+    w->parent = c;
+    int rc = guarded(w.get(), [&]() {
+        w->device = c->device;
+        HIPC(hipSetDevice(c->device));
+        HIPC(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&w->stream2, hipStreamNonBlocking));
+        w->both = c->both;
+        w->match_r1_fwd = c->match_r1_fwd;
+        w->htab = c->htab;
+        w->kmercutoff = c->kmercutoff;
+        w->ncpg = c->ncpg;
+        w->nchrom = c->nchrom;
+        w->nkmer = c->nkmer;
+        w->nwin = c->nwin;
+        w->nfilt = c->nfilt;
+        w->readlen = c->readlen;
+        w->winlen = c->winlen;
+        w->kmerlen = c->kmerlen;
+        w->seedbits = c->seedbits;
+        w->ix = c->ix;
+        w->match_cap = c->match_cap;
+        w->minpd = c->minpd;
+        w->maxpd = c->maxpd;
+        w->meth_threads = c->meth_threads;
+        w->form = c->form;
+        w->pipe_batch = c->pipe_batch;
+        HIPC(w->d_counters.reserve(8 * 8));
+        HIPC(hipMemset(w->d_counters.p, 0, 64));
+        HIPC(w->d_flags.reserve(16));
+        return FAME_OK;
+    });
+    if (rc != FAME_OK) {
+        fame_close(w.release());
+        return nullptr;
+    }
+    return w.release();
+}
+
+void serve_worker_close(fame_ctx* w) { fame_close(w); }
 void serve_set_err(fame_ctx* c, const std::string& m) { set_err(c, m); }
 
 } // namespace fame

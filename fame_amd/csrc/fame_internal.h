@@ -296,6 +296,11 @@ hipStream_t serve_stream(fame_ctx* c);
 int serve_device(fame_ctx* c);
 uint32_t serve_readlen(fame_ctx* c);
 uint32_t serve_winlen(fame_ctx* c);
+// a second context for concurrent batches: own buffers, stream and counters,
+// the parent's index and CpG accumulators, its strand decision and form as
+// of now (NULL on failure)
+fame_ctx* serve_worker_open(fame_ctx* c);
+void serve_worker_close(fame_ctx* w);
 void serve_set_err(fame_ctx* c, const std::string& m);
 void launch_pack_tmask(const uint32_t* raw, const uint32_t* kent, uint64_t n, uint64_t* out, hipStream_t st);
 void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first, uint64_t n, uint32_t* out,

@@ -1,9 +1,12 @@
 // Serving loop over one resident index: requests (chunks of single-end reads)
 // are queued as they arrive, admitted into device batches while a batch fits
-// the HBM budget (continuous batching: requests that arrive while a batch is
-// on the GPU join the next one), copied to the device on a stream of their
-// own while the previous batch computes (two staging slots), matched, and
-// completed with their per-read results, counters and latency.  No
+// the HBM budget (continuous batching), copied to the device on a stream of
+// their own (three staging slots), matched by one of two compute workers
+// (the context and a second context over the same index and CpG
+// accumulators, each with its own batch buffers and stream: requests that
+// arrive while a batch runs start at once on the other worker, and one
+// worker's host waits overlap the other's kernels), and completed with their
+// per-read results, counters and latency.  No
 // counterpart in the reference (FAME matches a file chunk by chunk,
 // main.cpp:363-398); each read's result is the one matchReads gives it,
 // whatever batch it runs in.
@@ -69,13 +72,17 @@ struct Slot {
 
 } // namespace
 
+constexpr int kSlots = 3;   // staging slots: one filling while two compute
+constexpr int kWorkers = 2; // compute workers (contexts)
+
 struct fame_server {
     fame_ctx* ctx = nullptr;
+    fame_ctx* wctx[kWorkers] = {nullptr, nullptr}; // wctx[0] = ctx; wctx[1] owned
     int device = 0;
     hipStream_t copy = nullptr;
     uint32_t max_reads = 0;
     uint64_t max_bytes = 0;
-    Slot slot[2];
+    Slot slot[kSlots];
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Req*> pending;
@@ -83,7 +90,7 @@ struct fame_server {
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
-    std::thread admit, compute;
+    std::thread admit, compute[kWorkers];
     uint64_t batches = 0, reads = 0, requests = 0;
     double busy_ms = 0;
     std::string err;
@@ -169,10 +176,11 @@ void admit_loop(fame_server* s)
     }
 }
 
-void compute_loop(fame_server* s)
+void compute_loop(fame_server* s, int wi)
 {
     (void)hipSetDevice(s->device);
-    hipStream_t st = fame::serve_stream(s->ctx);
+    fame_ctx* ctx = s->wctx[wi];
+    hipStream_t st = fame::serve_stream(ctx);
     for (;;) {
         int si;
         {
@@ -190,7 +198,7 @@ void compute_loop(fame_server* s)
         const uint8_t* d_valid = nullptr;
         const uint64_t* d_mat = nullptr;
         if (rc == FAME_OK)
-            rc = fame::serve_run(s->ctx, sl.d_seqs, sl.d_offs, sl.d_lens, sl.n, sl.bytes, st, &d_status, &d_valid,
+            rc = fame::serve_run(ctx, sl.d_seqs, sl.d_offs, sl.d_lens, sl.n, sl.bytes, st, &d_status, &d_valid,
                                  &d_mat);
         if (rc == FAME_OK && sl.n) {
             if (hipMemcpyAsync(sl.h_status, d_status, sl.n, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -264,9 +272,10 @@ int fame_serve_open(fame_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_reads
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return FAME_ERR_HIP;
         hbm_budget = fr / 2;
     }
-    // two staging slots in flight: the budget covers both batches
-    // budgeted at FAME_MAX_MATE letters per read (longer reads take more of the byte budget)
-    uint64_t by_budget = hbm_budget / (2 * (BYTES_PER_READ + 2 * FAME_MAX_MATE));
+    // the staging slots and the two workers' pipelines share the budget;
+    // letters budgeted at FAME_MAX_MATE per read (longer reads take more of
+    // the byte budget)
+    uint64_t by_budget = hbm_budget / (kSlots * 2 * FAME_MAX_MATE + kWorkers * BYTES_PER_READ);
     uint64_t mr = max_batch_reads ? max_batch_reads : 2000000u;
     mr = std::min<uint64_t>(mr, by_budget);
     if (mr < 1024) {
@@ -293,10 +302,13 @@ int fame_serve_open(fame_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_reads
         fame::serve_set_err(ctx, "cannot allocate the serving slots");
         return FAME_ERR_HIP;
     }
-    s->free_slots = {0, 1};
+    s->free_slots = {0, 1, 2};
+    s->wctx[0] = ctx;
+    s->wctx[1] = fame::serve_worker_open(ctx);
     fame_server* p = s.get();
     p->admit = std::thread(admit_loop, p);
-    p->compute = std::thread(compute_loop, p);
+    for (int w = 0; w < kWorkers; ++w)
+        if (p->wctx[w]) p->compute[w] = std::thread(compute_loop, p, w);
     *out = s.release();
     return FAME_OK;
 }
@@ -381,9 +393,11 @@ int fame_serve_close(fame_server* s)
         s->cv.notify_all();
     }
     if (s->admit.joinable()) s->admit.join();
-    if (s->compute.joinable()) s->compute.join();
+    for (auto& t : s->compute)
+        if (t.joinable()) t.join();
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
+    if (s->wctx[1]) fame::serve_worker_close(s->wctx[1]);
     release_slots(s);
     delete s;
     return FAME_OK;

@@ -53,7 +53,8 @@ def test_serve_results_equal_alone_and_in_the_mix(name, tmp_path):
     b = decided_context(c, first)
     # batches of 2048 reads, letters budgeted at 255 per read (more room for
     # the chunks of reads of up to 5 000 letters)
-    srv = _lib.Server(b, max_batch_reads=16384 if (c.lens > 255).any() else 2048)
+    mb = 16384 if (c.lens > 255).any() else 2048
+    srv = _lib.Server(b, max_batch_reads=mb)
     tickets = [srv.submit(c.seqs, c.offs[s:e], c.lens[s:e]) for s, e in parts]
     mixed = [srv.wait(t) for t in tickets]
     st = srv.stats()
@@ -64,6 +65,15 @@ def test_serve_results_equal_alone_and_in_the_mix(name, tmp_path):
     assert np.array_equal(b.meth(), meth_alone)
     # chunks shared batches, and no batch exceeded the limit
     assert st["batches"] < len(parts)
-    assert max(x[3]["batch_reads"] for x in mixed) <= 2048
+    assert max(x[3]["batch_reads"] for x in mixed) <= mb
     assert max(x[3]["batch_reads"] - (e - s) for x, (s, e) in zip(mixed, parts)) > 0
+    # and the oracle on the same chunking (the first chunk stranded, the rest
+    # in the decided orientation): per-read results and CpG counts
+    import orc
+    ix = orc.Index(c.index)
+    om, ov, omat = orc.match_stream(ix, c.seqs, c.offs, c.lens, chunk=first or len(c.lens),
+                                    both_strands=c.unord)
+    for k, ((s, e), (v2, m2, cnt2, tm)) in enumerate(zip(parts, mixed)):
+        assert np.array_equal(v2, ov[s:e]) and np.array_equal(m2, omat[s:e]), f"chunk {k} against the oracle"
+    assert np.array_equal(b.meth(), om.meth())
     b.close()
