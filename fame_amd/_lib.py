@@ -183,10 +183,10 @@ class Context:
         self._check(load().fame_set_pipeline(self.h, int(sub_batch_reads)))
 
     def form_info(self):
-        t = np.zeros(8, np.uint64)
+        t = np.zeros(9, np.uint64)
         self._check(load().fame_form_info(self.h, ptr(t, _u64p)))
         return dict(zip(("form", "graph_launches", "fallbacks", "graphs", "classes", "verify_blocks",
-                         "pipelined_calls", "sub_batch"), (int(x) for x in t)))
+                         "pipelined_calls", "sub_batch", "graphs_held"), (int(x) for x in t)))
 
     def decide_strand(self):
         f = C.c_uint64()

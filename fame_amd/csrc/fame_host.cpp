@@ -796,7 +796,37 @@ struct GraphPlan {
     DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace, counters;
     uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 1024, heavy_blocks2 = 16;
     HeavyScratch hs{}, hs2{};
+    // captured graphs by input buffers and variant, at most kMaxGraphs: the
+    // least recently used one is destroyed for a new one (callers that pass
+    // new buffers every call would otherwise add one per call)
+    static constexpr size_t kMaxGraphs = 64;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+    std::vector<uint64_t> graph_used;
+    uint64_t tick = 0;
+    hipGraphExec_t find_graph(const GraphKey& k)
+    {
+        for (size_t i = 0; i < graphs.size(); ++i)
+            if (graphs[i].first == k) {
+                graph_used[i] = ++tick;
+                return graphs[i].second;
+            }
+        return nullptr;
+    }
+    // the plan's launches have completed (its stream was synchronized)
+    void add_graph(const GraphKey& k, hipGraphExec_t e)
+    {
+        if (graphs.size() >= kMaxGraphs) {
+            size_t lru = 0;
+            for (size_t i = 1; i < graphs.size(); ++i)
+                if (graph_used[i] < graph_used[lru]) lru = i;
+            (void)hipGraphExecDestroy(graphs[lru].second);
+            graphs[lru] = {k, e};
+            graph_used[lru] = ++tick;
+            return;
+        }
+        graphs.push_back({k, e});
+        graph_used.push_back(++tick);
+    }
     // pinned: [0] n, [1] flags, [2..7] counters (3 x u64), [8..11] pool
     // counts; pipelined calls: [16 + k] n, [32 + k] flags and [64 + 4k ..]
     // pool counts of the lane's k-th sub-batch
@@ -810,6 +840,7 @@ struct GraphPlan {
     {
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         graphs.clear();
+        graph_used.clear();
     }
     ~GraphPlan()
     {
@@ -1036,7 +1067,7 @@ hipGraphExec_t capture_graph(fame_ctx* c, GraphPlan* pl, const GraphKey& key, co
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPC(e);
-    pl->graphs.push_back({key, exec});
+    pl->add_graph(key, exec);
     c->graph_captures += 1;
     return exec;
 }
@@ -1057,9 +1088,7 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
             if (q->chosen >= 0) pl->chosen = q->chosen; // the grid suits the GPU, not the class
     const int cand = pl->chosen >= 0 ? pl->chosen : pl->next_trial;
     GraphKey key{d_seqs, d_offs, d_lens, get_stranded, ori, cand};
-    hipGraphExec_t exec = nullptr;
-    for (auto& g : pl->graphs)
-        if (g.first == key) exec = g.second;
+    hipGraphExec_t exec = pl->find_graph(key);
     if (!exec) exec = capture_graph(c, pl, key, d_seqs, d_offs, d_lens, n, ori, get_stranded, cand, st);
     Timer tm(st);
     pl->h_io[0] = n;
@@ -1155,8 +1184,7 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
             const int l = (int)(k & 1);
             const uint32_t s0 = k * B;
             GraphKey key{d_seqs, d_offs + s0, d_lens + s0, 0, ori, cand};
-            for (auto& g : pl[l]->graphs)
-                if (g.first == key) ex[k - k0] = g.second;
+            ex[k - k0] = pl[l]->find_graph(key);
             if (!ex[k - k0]) {
                 HIPC(hipStreamSynchronize(ls[l]));
                 ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, std::min(n - s0, B), ori, 0,
@@ -2239,6 +2267,8 @@ int fame_form_info(const fame_ctx* c, uint64_t out[FAME_FORM_FIELDS])
     out[5] = 0;
     out[6] = c->pipe_calls;
     out[7] = c->pipe_batch;
+    out[8] = 0;
+    for (auto& p : c->plans) out[8] += p->graphs.size();
     uint32_t big = 0;
     for (auto& p : c->plans)
         if (p->cap >= big && p->chosen >= 0) {

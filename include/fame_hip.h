@@ -194,8 +194,9 @@ int fame_set_pipeline(fame_ctx* ctx, uint32_t sub_batch_reads);
 
 /* form, batched graph launches, of which re-run plain, graphs captured,
  * capacity classes, the autotuned verify grid (blocks) of the largest tuned
- * class, pipelined calls, sub-batch size */
-#define FAME_FORM_FIELDS 8
+ * class, pipelined calls, sub-batch size, graphs held now (each capacity
+ * class keeps at most 64, the least recently used destroyed first) */
+#define FAME_FORM_FIELDS 9
 int fame_form_info(const fame_ctx* ctx, uint64_t out[FAME_FORM_FIELDS]);
 
 /* ---- serving loop (no counterpart in the reference) ----------------------

@@ -1,3 +1,4 @@
+import ctypes as C
 import gzip
 import json
 import os
@@ -127,3 +128,47 @@ def read_fastq_gz(path):
     if len(seqs) > 1:
         offs[1:] = np.cumsum(lens[:-1].astype(np.uint64))
     return np.frombuffer(b"".join(seqs), np.uint8), offs, lens
+
+
+H2D, D2H = 1, 2
+
+
+class Hip:
+    """hipMalloc / hipMemcpy / hipFree of the runtime libfamehip.so links"""
+
+    def __init__(self):
+        from fame_amd import _lib
+
+        _lib.load()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        self.h = h
+        self.live = []
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        assert self.h.hipMalloc(C.byref(p), max(int(nbytes), 1)) == 0
+        assert self.h.hipMemset(p, 0, max(int(nbytes), 1)) == 0
+        self.live.append(p.value)
+        return p.value
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, H2D) == 0
+        return p
+
+    def get(self, p, n, dtype):
+        out = np.zeros(n, dtype)
+        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, D2H) == 0
+        return out
+
+    def free(self):
+        for p in self.live:
+            self.h.hipFree(p)
+        self.live = []
+
+

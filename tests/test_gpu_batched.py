@@ -115,3 +115,42 @@ def test_batched_overflow_reruns_plain(golden_case, monkeypatch):
         assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
         assert np.array_equal(ctx.meth(), c.meth)
         ctx.close()
+
+
+def test_graph_cache_stays_bounded(monkeypatch):
+    """Device-path calls with new input buffers every call (70 of them, one
+    capacity class): each captures a graph, the plan keeps at most 64 (the
+    least recently used destroyed), and every call equals the plain form."""
+    from conftest import Case, Hip
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        c = Case("se_basic", td)
+        hip = Hip()
+        ctx = _lib.Context(c.index)
+        cnt = np.zeros(3, np.uint64)
+        ctx.match(c.seqs, c.offs[:700], c.lens[:700], True, cnt)
+        ctx.decide_strand()
+        n = 300
+        plain = _lib.Context(c.index)
+        plain.match(c.seqs, c.offs[:700], c.lens[:700], True, np.zeros(3, np.uint64))
+        plain.decide_strand()
+        ctx.set_form("batched")
+        for k in range(70):
+            s = 700 + (k * 37) % (len(c.lens) - 700 - n)
+            offs = c.offs[s:s + n] - c.offs[s]
+            lens = c.lens[s:s + n]
+            seqs = c.seqs[int(c.offs[s]):int(c.offs[s + n - 1] + lens[-1])]
+            ds, do, dl = hip.put(seqs), hip.put(offs.astype(np.uint64)), hip.put(lens.astype(np.uint32))
+            dv, dm = hip.alloc(n), hip.alloc(8 * n)
+            cb = np.zeros(3, np.uint64)
+            ctx.match_device(ds, do, dl, n, False, cb, dv, dm)
+            v, m = hip.get(dv, n, np.uint8), hip.get(dm, n, np.uint64)
+            cp = np.zeros(3, np.uint64)
+            pv, pm = plain.match(c.seqs, c.offs[s:s + n], lens, False, cp)
+            assert np.array_equal(v, pv) and np.array_equal(m, pm), f"call {k}"
+            assert cb.tolist() == cp.tolist(), f"call {k} counters"
+        info = ctx.form_info()
+        assert info["graphs"] >= 70 and info["graphs_held"] <= 64 and info["fallbacks"] == 0
+        ctx.close()
+        plain.close()
+        hip.free()

@@ -10,51 +10,13 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from conftest import Case
+from conftest import Case, Hip
 from fame_amd import _lib
 
 pytestmark = pytest.mark.gpu
 
 MAX_READS = 32768
 MAX_BYTES = 8 << 20
-H2D, D2H = 1, 2
-
-
-class Hip:
-    """hipMalloc / hipMemcpy / hipFree of the runtime libfamehip.so links"""
-
-    def __init__(self):
-        _lib.load()
-        h = C.CDLL("libamdhip64.so.7")
-        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        h.hipFree.argtypes = [C.c_void_p]
-        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
-        self.h = h
-        self.live = []
-
-    def alloc(self, nbytes):
-        p = C.c_void_p()
-        assert self.h.hipMalloc(C.byref(p), max(int(nbytes), 1)) == 0
-        assert self.h.hipMemset(p, 0, max(int(nbytes), 1)) == 0
-        self.live.append(p.value)
-        return p.value
-
-    def put(self, a):
-        a = np.ascontiguousarray(a)
-        p = self.alloc(a.nbytes)
-        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, H2D) == 0
-        return p
-
-    def get(self, p, n, dtype):
-        out = np.zeros(n, dtype)
-        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, D2H) == 0
-        return out
-
-    def free(self):
-        for p in self.live:
-            self.h.hipFree(p)
-        self.live = []
 
 
 def noalloc(ctx, d_seqs, d_offs, d_lens, n, stranded, cnt, d_valid, d_mat):
