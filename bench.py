@@ -299,14 +299,13 @@ def main():
                 s = e
             return cnt
 
-        forms = ["plain"] if paired else [f for f in args.forms.split(",") if f]
+        forms = [f for f in args.forms.split(",") if f]
         results = {}
         for form in forms:
             cur["form"] = form
-            if not paired:
-                ctx.set_form(form)
-                if args.sub_batch:
-                    ctx.set_pipeline(args.sub_batch)
+            ctx.set_form(form)
+            if args.sub_batch:
+                ctx.set_pipeline(args.sub_batch)
             for _ in range(args.warmup):
                 step()
             calls.clear()

@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fame_hip.h"
@@ -150,6 +151,10 @@ struct fame_ctx {
     // count), list, rows, DP list and DP scratch
     DevBuf b_long, b_long_list, b_long_codes, b_dp_long, b_ldp, b_ldec, b_ltrace;
     HostStage h_offs;                    // host-buffer calls: read offsets rebased to the span copied
+    // batched form, paired: the second lane's context and the lanes' mate
+    // offsets / lengths of their sub-batch
+    fame_ctx* pair_worker = nullptr;
+    DevBuf b_lane_offs[2], b_lane_lens[2];
     uint64_t pipe_calls = 0;
     double big_ms = 0; // paired seed runs with global scratch, last call
 };
@@ -1942,6 +1947,102 @@ fame_ctx* serve_worker_open(fame_ctx* c)
 }
 
 void serve_worker_close(fame_ctx* w) { fame_close(w); }
+
+// Batched form, paired: a call of more than pipe_batch / 2 pairs (not the
+// stranded first chunk) is split into sub-batches of that many pairs that
+// alternate between two lanes, the context and a worker context over the
+// same index and CpG accumulators, each with its own buffers and stream and
+// a host thread of its own: one lane's host waits between its stages (the
+// paired pipeline sizes its stages from device counts) overlap the other's
+// kernels.  Pairs are independent and the CpG counts are atomic sums, so the
+// split changes no result.  Outputs: 2n mates (mate 1, then mate 2).
+int run_pairs_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_o, const uint32_t* d_l, uint32_t n,
+                        uint64_t total_bytes, uint64_t counters[5], const uint8_t** out_valid,
+                        const uint64_t** out_mat)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t B = std::max<uint32_t>(c->pipe_batch / 2, 128);
+    if (!c->pair_worker) c->pair_worker = serve_worker_open(c);
+    fame_ctx* w = c->pair_worker;
+    if (!w) { // no second lane: the plain pipeline
+        const int rc = run_pairs(c, d_seqs, d_o, d_l, n, total_bytes, 0, counters, c->stream);
+        *out_valid = c->b_valid.as<uint8_t>();
+        *out_mat = c->b_mat.as<uint64_t>();
+        return rc;
+    }
+    // the worker follows the context's strand decision and pair distances
+    w->both = c->both;
+    w->match_r1_fwd = c->match_r1_fwd;
+    w->minpd = c->minpd;
+    w->maxpd = c->maxpd;
+    uint8_t* ov = grow<uint8_t>(c->b_pvalid, 2ull * n + 1);
+    uint64_t* om = grow<uint64_t>(c->b_pmat, 2ull * n + 1);
+    uint64_t* lo[2];
+    uint32_t* ll[2];
+    for (int l = 0; l < 2; ++l) {
+        lo[l] = grow<uint64_t>(c->b_lane_offs[l], 2ull * B + 1);
+        ll[l] = grow<uint32_t>(c->b_lane_lens[l], 2ull * B + 1);
+    }
+    const uint32_t nb = (n + B - 1) / B;
+    uint64_t lcnt[2][5] = {{0}};
+    int lrc[2] = {FAME_OK, FAME_OK};
+    auto lane = [&](int l) {
+        fame_ctx* x = l ? w : c;
+        lrc[l] = guarded(x, [&]() {
+            HIPC(hipSetDevice(c->device));
+            hipStream_t st = x->stream;
+            for (uint32_t k = (uint32_t)l; k < nb; k += 2) {
+                const uint32_t s = k * B, m = std::min(B, n - s);
+                // the sub-batch's mates as 2m reads: mate 1 of its pairs, then mate 2
+                HIPC(hipMemcpyAsync(lo[l], d_o + s, 8ull * m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(lo[l] + m, d_o + n + s, 8ull * m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(ll[l], d_l + s, 4ull * m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(ll[l] + m, d_l + n + s, 4ull * m, hipMemcpyDeviceToDevice, st));
+                uint64_t cnt[5] = {0, 0, 0, 0, 0};
+                const int rc = run_pairs(x, d_seqs, lo[l], ll[l], m, total_bytes, 0, cnt, st);
+                if (rc != FAME_OK) return rc;
+                HIPC(hipMemcpyAsync(ov + s, x->b_valid.p, m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(ov + n + s, x->b_valid.as<uint8_t>() + m, m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(om + s, x->b_mat.p, 8ull * m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipMemcpyAsync(om + n + s, x->b_mat.as<uint64_t>() + m, 8ull * m, hipMemcpyDeviceToDevice, st));
+                HIPC(hipStreamSynchronize(st));
+                for (int q = 0; q < 5; ++q) lcnt[l][q] += cnt[q];
+            }
+            return (int)FAME_OK;
+        });
+    };
+    std::thread t1(lane, 1);
+    lane(0);
+    t1.join();
+    if (lrc[0] != FAME_OK) return lrc[0];
+    if (lrc[1] != FAME_OK) {
+        set_err(c, "second lane: " + w->err);
+        return lrc[1];
+    }
+    for (int q = 0; q < 5; ++q) counters[q] += lcnt[0][q] + lcnt[1][q];
+    c->pipe_calls += 1;
+    *out_valid = ov;
+    *out_mat = om;
+    memset(c->stats, 0, sizeof c->stats);
+    for (double& x : c->timings) x = 0;
+    c->timings[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->timings[1] = c->timings[5];
+    return FAME_OK;
+}
+
+// one paired batch in the context's form; the 2n per-mate outputs stay in
+// device buffers whose addresses are returned
+int match_pairs_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_o, const uint32_t* d_l, uint32_t n,
+                      uint64_t total_bytes, int get_stranded, uint64_t counters[5], hipStream_t st,
+                      const uint8_t** out_valid, const uint64_t** out_mat)
+{
+    if (c->form == 1 && !get_stranded && n > std::max<uint32_t>(c->pipe_batch / 2, 128))
+        return run_pairs_pipelined(c, d_seqs, d_o, d_l, n, total_bytes, counters, out_valid, out_mat);
+    const int rc = run_pairs(c, d_seqs, d_o, d_l, n, total_bytes, get_stranded, counters, st);
+    *out_valid = c->b_valid.as<uint8_t>();
+    *out_mat = c->b_mat.as<uint64_t>();
+    return rc;
+}
 void serve_set_err(fame_ctx* c, const std::string& m) { set_err(c, m); }
 
 } // namespace fame
@@ -1979,6 +2080,8 @@ int fame_open(const char* index_path, int both_strands, fame_ctx** out)
 int fame_close(fame_ctx* c)
 {
     if (!c) return FAME_OK;
+    if (c->pair_worker) fame_close(c->pair_worker);
+    c->pair_worker = nullptr;
     c->plans.clear();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -2187,12 +2290,14 @@ int fame_match_pairs(fame_ctx* c, const uint8_t* seqs1, const uint64_t* offs1, c
             HIPC(hipMemcpyAsync(d_l, lens1, 4ull * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(d_l + n, lens2, 4ull * n, hipMemcpyHostToDevice, st));
         }
-        int rc = run_pairs(c, d_s, d_o, d_l, n, total, get_stranded, counters, st);
+        const uint8_t* ov = nullptr;
+        const uint64_t* om = nullptr;
+        int rc = match_pairs_batch(c, d_s, d_o, d_l, n, total, get_stranded, counters, st, &ov, &om);
         if (rc != FAME_OK) return rc;
-        if (n && valid1) HIPC(hipMemcpyAsync(valid1, c->b_valid.p, n, hipMemcpyDeviceToHost, st));
-        if (n && valid2) HIPC(hipMemcpyAsync(valid2, c->b_valid.as<uint8_t>() + n, n, hipMemcpyDeviceToHost, st));
-        if (n && mat1) HIPC(hipMemcpyAsync(mat1, c->b_mat.p, 8ull * n, hipMemcpyDeviceToHost, st));
-        if (n && mat2) HIPC(hipMemcpyAsync(mat2, c->b_mat.as<uint64_t>() + n, 8ull * n, hipMemcpyDeviceToHost, st));
+        if (n && valid1) HIPC(hipMemcpyAsync(valid1, ov, n, hipMemcpyDeviceToHost, st));
+        if (n && valid2) HIPC(hipMemcpyAsync(valid2, ov + n, n, hipMemcpyDeviceToHost, st));
+        if (n && mat1) HIPC(hipMemcpyAsync(mat1, om, 8ull * n, hipMemcpyDeviceToHost, st));
+        if (n && mat2) HIPC(hipMemcpyAsync(mat2, om + n, 8ull * n, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });
@@ -2229,14 +2334,14 @@ int fame_match_pairs_device(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* 
             HIPC(hipStreamSynchronize(st));
             total = e;
         }
-        int rc = run_pairs(c, d_seqs, d_o, d_l, n, total, get_stranded, counters, st);
+        const uint8_t* ov = nullptr;
+        const uint64_t* om = nullptr;
+        int rc = match_pairs_batch(c, d_seqs, d_o, d_l, n, total, get_stranded, counters, st, &ov, &om);
         if (rc != FAME_OK) return rc;
-        if (n && d_valid1) HIPC(hipMemcpyAsync(d_valid1, c->b_valid.p, n, hipMemcpyDeviceToDevice, st));
-        if (n && d_valid2)
-            HIPC(hipMemcpyAsync(d_valid2, c->b_valid.as<uint8_t>() + n, n, hipMemcpyDeviceToDevice, st));
-        if (n && d_mat1) HIPC(hipMemcpyAsync(d_mat1, c->b_mat.p, 8ull * n, hipMemcpyDeviceToDevice, st));
-        if (n && d_mat2)
-            HIPC(hipMemcpyAsync(d_mat2, c->b_mat.as<uint64_t>() + n, 8ull * n, hipMemcpyDeviceToDevice, st));
+        if (n && d_valid1) HIPC(hipMemcpyAsync(d_valid1, ov, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_valid2) HIPC(hipMemcpyAsync(d_valid2, ov + n, n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat1) HIPC(hipMemcpyAsync(d_mat1, om, 8ull * n, hipMemcpyDeviceToDevice, st));
+        if (n && d_mat2) HIPC(hipMemcpyAsync(d_mat2, om + n, 8ull * n, hipMemcpyDeviceToDevice, st));
         HIPC(hipStreamSynchronize(st));
         return (int)FAME_OK;
     });

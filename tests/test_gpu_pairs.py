@@ -122,3 +122,41 @@ def test_hip_pairs_tab64_match_reference(golden_pair, monkeypatch):
     assert cnt.tolist() == c.counters
     assert np.array_equal(ctx.meth(), c.meth)
     ctx.close()
+
+
+def test_hip_pairs_batched_lanes_match_oracle(golden_pair):
+    """Batched form, paired: after a stranded first chunk of a third of the
+    pairs, the rest in one call that splits into sub-batches of 300 pairs
+    alternating between two lanes (the context and a worker context, each
+    with its own buffers, stream and host thread); the oracle on the same
+    chunking, per pair, by counters and CpG counts."""
+    c = golden_pair
+    print(f"case {c.name}: paired lanes", flush=True)
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    ctx.set_pair_dist(c.minpd, c.maxpd)
+    ctx.set_form("batched")
+    ctx.set_pipeline(600)
+    n = len(c.r1[2])
+    first = 0 if c.unord else max(n // 3, 1)
+    v1 = np.zeros(n, np.uint8)
+    v2 = np.zeros(n, np.uint8)
+    m1 = np.zeros(n, np.uint64)
+    m2 = np.zeros(n, np.uint64)
+    cnt = np.zeros(5, np.uint64)
+    if first:
+        a = (c.r1[0], c.r1[1][:first], c.r1[2][:first])
+        b = (c.r2[0], c.r2[1][:first], c.r2[2][:first])
+        v1[:first], m1[:first], v2[:first], m2[:first] = ctx.match_pairs(a, b, True, cnt)
+        ctx.decide_strand()
+    a = (c.r1[0], c.r1[1][first:], c.r1[2][first:])
+    b = (c.r2[0], c.r2[1][first:], c.r2[2][first:])
+    v1[first:], m1[first:], v2[first:], m2[first:] = ctx.match_pairs(a, b, False, cnt)
+    assert ctx.form_info()["pipelined_calls"] >= 1
+    ix = orc.Index(c.index)
+    m, ov1, om1, ov2, om2 = orc.match_pairs_stream(ix, c.r1, c.r2, chunk=first or n, both_strands=c.unord,
+                                                   minpd=c.minpd, maxpd=c.maxpd)
+    bad = np.nonzero((v1 != ov1) | (v2 != ov2) | (m1 != om1) | (m2 != om2))[0]
+    assert bad.size == 0, f"{bad.size} pairs differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == m.counters5.tolist()
+    assert np.array_equal(ctx.meth(), m.meth())
+    ctx.close()
