@@ -1017,6 +1017,37 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
     const uint32_t m = bd.lens[r]; // 64(W-1) < m <= 64W: the pool holds reads of W words
     const int z = 64 * W - (int)m;
     const uint8_t* pat = (ori ? bd.rev : bd.seqs) + bd.offs[r];
+    // the window's text first: its loads lead the pattern work
+    const uint4 win = ix.win[id];
+    const uint2 wtx = ix.wtext[id]; // the chromosome's first word in ix.pk, its length
+    const uint32_t startPos = win.x;
+    const uint64_t clen = wtx.y;
+    const uint64_t span = (uint64_t)ix.winlen + MISCOUNT + ADDMIS - 1;
+    // the text: chromosome positions [tstart, tstart + tlen), scanned
+    // forward, or backward and complemented for the reverse map
+    uint64_t tstart, tlen;
+    if (!rev) {
+        uint64_t end = (uint64_t)startPos + span;
+        if (end > clen) end = clen;
+        tstart = startPos;
+        tlen = end - startPos;
+    } else {
+        const uint64_t e0 = startPos > 0 ? (uint64_t)startPos - 1 : 0;
+        uint64_t s0 = (uint64_t)startPos + span;
+        if (s0 >= clen) s0 = clen - 1;
+        tstart = e0 + 1;
+        tlen = s0 > e0 ? s0 - e0 : 0;
+        if (unguarded && startPos == 0) {
+            tstart = 0;
+            tlen = s0 + 1;
+        }
+    }
+    const int T = (int)tlen;
+    const int skip = (int)(tstart & 15);
+    const uint64_t* __restrict__ pk = ix.pk + wtx.x + (tstart >> 4);
+    const int nchunk = (skip + T + 15) >> 4;
+    // scan chunk 0's word (the last memory word of the reverse map's text)
+    const uint64_t v0 = nchunk > 0 ? pk[rev ? nchunk - 1 : 0] : 0ull;
     // Peq masks of the pattern, laid out [letter*W + word][lane] so that the
     // lanes of a wave read consecutive LDS words (ShiftAnd loadBitmasks
     // ShiftAnd.h:345-432: pattern T also matches text C), shifted up by z;
@@ -1177,29 +1208,6 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
             }
         }
     }
-    const uint4 win = ix.win[id];
-    const uint32_t startPos = win.x, chrom = win.y;
-    const uint64_t clen = ix.clen[chrom];
-    const uint64_t span = (uint64_t)ix.winlen + MISCOUNT + ADDMIS - 1;
-    // the text: chromosome positions [tstart, tstart + tlen), scanned
-    // forward, or backward and complemented for the reverse map
-    uint64_t tstart, tlen;
-    if (!rev) {
-        uint64_t end = (uint64_t)startPos + span;
-        if (end > clen) end = clen;
-        tstart = startPos;
-        tlen = end - startPos;
-    } else {
-        const uint64_t e0 = startPos > 0 ? (uint64_t)startPos - 1 : 0;
-        uint64_t s0 = (uint64_t)startPos + span;
-        if (s0 >= clen) s0 = clen - 1;
-        tstart = e0 + 1;
-        tlen = s0 > e0 ? s0 - e0 : 0;
-        if (unguarded && startPos == 0) {
-            tstart = 0;
-            tlen = s0 + 1;
-        }
-    }
     uint64_t Pv[W], Mv[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -1246,17 +1254,13 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
         was = valid ? acc : was;
     };
     // The text comes from the packed copy of the genome: one 64-bit word per
-    // 16 letters of a chromosome (ix.pk, from ix.pkoff[chrom]), two bits per
+    // 16 letters of a chromosome (ix.pk, from ix.wtext[window].x), two bits per
     // letter (A 0, C 1, G 2, T 3; letter j at bits 2j) and an N mask in bits
     // 32-47 (every byte other than A, C, G, T, as FAME's reader writes N for
     // them).  Scan chunk ci is memory chunk ci (forward) or nchunk-1-ci
     // (reverse, whose letters are reversed and complemented: A<->T, C<->G,
     // ShiftAnd.h:197-211).  Positions are 32-bit (texts are at most a few
     // windows long).
-    const int T = (int)tlen;
-    const int skip = (int)(tstart & 15);
-    const uint64_t* __restrict__ pk = ix.pk + ix.pkoff[chrom] + (tstart >> 4);
-    const int nchunk = (skip + T + 15) >> 4;
     struct Chunk {
         uint32_t x, nm; // codes in scan order (letter ii at bits 2ii), N bits of the text letters
         int lo, hi;     // scan-order range of text letters in the chunk
@@ -1411,7 +1415,7 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
             probe_chunk(reduced(k.x), k.lo, k.hi);
             return true;
         };
-        if (nchunk > 0 && filter_chunk(0, raw_chunk_c(0))) {
+        if (nchunk > 0 && filter_chunk(0, v0)) {
             // interior chunks are whole: their words follow one another in
             // memory (backwards for the reverse map), two loaded ahead
             const int last = nchunk - 1;
@@ -1629,7 +1633,9 @@ __device__ __forceinline__ int cmask_first(const uint64_t (&cm)[4], bool set, in
 // lane with the most marked chunks; each owner finally gathers its runs'
 // matches in scan order.  A run's matches do not depend on the others' (each
 // starts from a fresh automaton), so the results equal verify_one's.
-// Overflow re-verification (`only`) and one-word patterns scan in place.
+// One-word patterns scan in place, one job per lane; so do the jobs of the
+// overflow re-verification (`only`), in the same block rounds (one copy of
+// verify_one in the kernel keeps its code within the instruction cache).
 template <int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_verify(DevIndex ix, BatchDev bd, uint32_t pool,
                                                 const uint32_t* __restrict__ only, uint32_t n_only)
@@ -1644,7 +1650,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     else njobs = min(bd.pool_n[pool], pc_get_cap(bd, pool));
     if (bd.n_dev && (*bd.flags & FLAG_ABORT)) njobs = 0;
     uint64_t jobs = 0, bytes = 0;
-    if (W < 2 || only) {
+    if constexpr (W < 2) {
         VerifyDefer df{};
         for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < njobs; t += gridDim.x * blockDim.x) {
             const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab, df);
@@ -1676,7 +1682,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         for (uint32_t base = blockIdx.x * 256; base < njobs; base += again ? 0u : gridDim.x * 256) {
             const uint32_t t = base + threadIdx.x;
             VerifyDefer df{};
-            df.queue = !redo;
+            df.queue = !redo && !only; // overflow re-verification scans in place
             if (t < njobs && redo == again) {
                 const uint64_t b = verify_one<W>(ix, bd, pool, only, t, s_peq, s_ptab, df);
                 jobs += redo ? 0 : (b ? 1 : 0);
@@ -2363,27 +2369,31 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             }
         }
         // rows 10..L: every slot 1..17 is a computed cell.  The letters of
-        // the next rows are loaded four rows ahead (b0..b3 / c0..c3).
+        // the next rows are loaded eight rows ahead (bq / cq).
         {
             auto ref_b = [&](long rq) -> uint32_t { return rq <= (long)L ? col_at(cs, clen, ref_end, rq + 8) : 0u; };
             auto read_b = [&](long rq) -> uint32_t {
                 return rq <= (long)L ? (uint32_t)(rv ? seq[rq - 1] : seq[L - rq]) : 0u;
             };
-            uint32_t b0 = ref_b(10), b1 = ref_b(11), b2 = ref_b(12), b3 = ref_b(13);
-            uint32_t c0 = read_b(10), c1 = read_b(11), c2 = read_b(12), c3 = read_b(13);
+            constexpr int PD = 8; // rows loaded ahead
+            uint32_t bq[PD], cq[PD];
+#pragma unroll
+            for (int q = 0; q < PD; ++q) {
+                bq[q] = ref_b(10 + q);
+                cq[q] = read_b(10 + q);
+            }
 #pragma unroll 1
             for (long rr = 10; rr <= (long)L; ++rr) {
-                shift_in(b0);
+                shift_in(bq[0]);
                 uint32_t t1, t2;
-                dp_targets(c0, rv, t1, t2);
-                b0 = b1;
-                b1 = b2;
-                b2 = b3;
-                b3 = ref_b(rr + 4);
-                c0 = c1;
-                c1 = c2;
-                c2 = c3;
-                c3 = read_b(rr + 4);
+                dp_targets(cq[0], rv, t1, t2);
+#pragma unroll
+                for (int q = 0; q < PD - 1; ++q) {
+                    bq[q] = bq[q + 1];
+                    cq[q] = cq[q + 1];
+                }
+                bq[PD - 1] = ref_b(rr + PD);
+                cq[PD - 1] = read_b(rr + PD);
                 uint64_t dw = 0;
                 cur[0] = (int)(0xFFFF - rr);
 #pragma unroll

@@ -115,7 +115,8 @@ struct fame_ctx {
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
     DevBuf d_tab, d_slot, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom, d_pk,
-        d_pkoff;
+        d_pkoff, d_wtext;
+    std::vector<uint64_t> h_pkoff, h_clen; // per chromosome, for d_wtext
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
     DevIndex ix{};
@@ -278,6 +279,9 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(hipMemset(c->d_pk.p, 0, 8 * words + 8));
         HIPC(c->d_pkoff.reserve(8 * c->nchrom + 8));
         HIPC(hipMemcpy(c->d_pkoff.p, pkoff.data(), 8 * c->nchrom, hipMemcpyHostToDevice));
+        if (words >= (1ull << 32)) throw FameError{FAME_ERR_FORMAT, "genome of more than 2^36 letters"};
+        c->h_pkoff = pkoff;
+        c->h_clen.assign(clen.begin(), clen.end());
         for (uint64_t i = 0; i < c->nchrom; ++i)
             launch_pack_seq(c->d_seq.as<uint8_t>() + coff[i], clen[i], (clen[i] + 15) / 16,
                             c->d_pk.as<uint64_t>() + pkoff[i], st);
@@ -378,6 +382,14 @@ void load_index(fame_ctx* c, const std::string& path)
         }
         HIPC(c->d_win.reserve(16 * c->nwin + 16));
         HIPC(hipMemcpy(c->d_win.p, raw.data(), 16 * c->nwin, hipMemcpyHostToDevice));
+        std::vector<uint32_t> wt(2 * c->nwin);
+        for (uint64_t i = 0; i < c->nwin; ++i) {
+            const uint32_t ch = raw[4 * i + 1];
+            wt[2 * i] = (uint32_t)c->h_pkoff[ch];
+            wt[2 * i + 1] = (uint32_t)std::min<uint64_t>(c->h_clen[ch], 0xFFFFFFFFull);
+        }
+        HIPC(c->d_wtext.reserve(8 * c->nwin + 8));
+        HIPC(hipMemcpy(c->d_wtext.p, wt.data(), 8 * c->nwin, hipMemcpyHostToDevice));
     }
     c->nfilt = f.get<uint64_t>();
     {
@@ -428,6 +440,7 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.clen = c->d_clen.as<uint64_t>();
     ix.pk = c->d_pk.as<uint64_t>();
     ix.pkoff = c->d_pkoff.as<uint64_t>();
+    ix.wtext = c->d_wtext.as<uint2>();
     ix.cpg_pos = c->d_cpg_pos.as<uint32_t>();
     ix.cpg_chrom = c->d_cpg_chrom.as<uint8_t>();
     ix.readlen = c->readlen;

@@ -84,6 +84,9 @@ struct DevIndex {
     // words of padding after each
     const uint64_t* pk;
     const uint64_t* pkoff;
+    // per window: its chromosome's first word in pk and length (so that the
+    // verify kernels find a window's text with one load after the job's)
+    const uint2* wtext;
     const uint32_t* cpg_pos;
     const uint8_t* cpg_chrom;
     uint32_t readlen, winlen, seedbits;
