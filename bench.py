@@ -447,14 +447,17 @@ def roofline_seed(calls, same):
     algorithmic bytes per (read, orientation) are FAME's own lookups, 16 B
     of tabIndex per k-mer, 8.125 B per index entry of every bucket named
     (kmerTableSmall entry + strand bit) and the read's letters; `design_bytes`
-    is what this design must read (one 8-byte slot per k-mer, the k-mer table
-    only for buckets of three or more, the read, the candidate records
-    written).  traffic: HBM bytes per launch from the committed counter
-    summary (null off the default workload)."""
+    is what this design must read (one 8-byte slot per k-mer read at random,
+    4 B of key and 8 B of slot per k-mer found in the genome-ordered copy,
+    4 B per key-position lookup, the k-mer table only for buckets of three
+    or more, the read, the candidate records written).  traffic: HBM bytes
+    per launch from the committed counter summary (null off the default
+    workload)."""
     ms = mean_over(calls, lambda c: c[1]["seed"])
     alg = mean_over(calls, lambda c: 16 * c[2]["kmers"] + 8.125 * c[2]["bucket_entries"] + c[2]["read_bytes"])
-    design = mean_over(calls, lambda c: c[2]["tab_bytes"] * c[2]["kmers"] + 4 * c[2]["entries"]
-                       + c[2]["read_bytes"] + 8 * c[2]["jobs"])
+    design = mean_over(calls, lambda c: c[2]["tab_bytes"] * c[2]["random_slots"] + 12 * c[2]["ordered_slots"]
+                       + 4 * c[2]["entries"] + c[2]["read_bytes"] + 8 * c[2]["jobs"])
+    seeded = mean_over(calls, lambda c: c[2]["seeded"]) or 1
     gbs = alg / (ms / 1e3) / 1e9
     traffic = profile_value(PROFILES["seed"][1], "traffic_bytes_per_launch") if same else None
     return {"kernel": "k_seed_light", "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
@@ -463,6 +466,8 @@ def roofline_seed(calls, same):
             "algorithmic_bytes_per_launch": int(alg), "design_bytes_per_launch": int(design),
             "design_GBps": round(design / (ms / 1e3) / 1e9, 2),
             "bucket_entries_per_launch": int(mean_over(calls, lambda c: c[2]["bucket_entries"])),
+            "random_slot_reads_per_read_orientation": round(mean_over(calls, lambda c: c[2]["random_slots"]) / seeded, 2),
+            "ordered_slot_reads_per_read_orientation": round(mean_over(calls, lambda c: c[2]["ordered_slots"]) / seeded, 2),
             "avg_launch_ms": round(ms, 3), "launches": len(calls),
             "reads_per_launch": int(mean_over(calls, lambda c: c[0]))}
 

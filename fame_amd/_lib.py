@@ -213,10 +213,11 @@ class Context:
         return dict(zip(("prep", "seed", "verify", "select", "meth", "total", "dp"), t.tolist()))
 
     def stats(self):
-        t = np.zeros(11, np.uint64)
+        t = np.zeros(13, np.uint64)
         self._check(load().fame_last_stats(self.h, ptr(t, _u64p)))
         return dict(zip(("kmers", "entries", "seeded", "jobs", "window_bytes", "dp", "read_bytes",
-                         "tab_bytes", "bucket_entries", "dp_rows", "word_letters"), (int(x) for x in t)))
+                         "tab_bytes", "bucket_entries", "dp_rows", "word_letters", "random_slots",
+                         "ordered_slots"), (int(x) for x in t)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -554,6 +554,238 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Slot lookups through the genome-ordered copy (DevIndex::gkey)
+// ---------------------------------------------------------------------------
+
+// Prefix form of the spaced-seed hash.  With Q[t] = rol(seed(c_t), -t) and
+// P[j] = Q[0] ^ ... ^ Q[j-1] over a read's reduced letters c_t, the hash of
+// k-mer k,
+//   XOR over care positions i of rol(seed(c_{k+i}), 31 - i),
+// equals rol(XOR over care i of Q[k+i], 31 + k); a run [a, b] of care
+// positions contributes P[k+b+1] ^ P[k+a], so the hash is rol of the XOR of
+// P[k + e] over the ends e of the runs: ten LDS reads per k-mer for the
+// shipped seed instead of 28 letters and 28 table terms.
+struct SeedEnds {
+    int n;
+    int e[KMERLEN + 1];
+};
+constexpr SeedEnds seed_ends()
+{
+    SeedEnds s{};
+    bool prev = false;
+    for (int i = 0; i <= (int)KMERLEN; ++i) {
+        const bool cur = i < (int)KMERLEN && ((SEEDBITS >> (KMERLEN - 1 - i)) & 1);
+        if (cur != prev) s.e[s.n++] = i;
+        prev = cur;
+    }
+    return s;
+}
+
+// inclusive XOR scan of a u64 over the lanes (the DPP pattern of
+// wave_iscan_u32 on both halves)
+__device__ __forceinline__ uint32_t wave_ixor_u32(uint32_t v)
+{
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false); // row_shr:1
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false); // row_shr:2
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false); // row_shr:4
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false); // row_shr:8
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_ixor_u64(uint64_t v)
+{
+    return ((uint64_t)wave_ixor_u32((uint32_t)(v >> 32)) << 32) | wave_ixor_u32((uint32_t)v);
+}
+
+// A (read, orientation) passes three stages before seed_one takes its
+// slots, one per iteration of the seed loop, so that each stage's loads have
+// a whole iteration to arrive:
+//   hashed     keys known; the key positions (kpos) of up to three anchor
+//              k-mers in flight
+//   predicted  the anchors give where the read sits in genome order (one or
+//              two offsets); gkey of every k-mer there in flight
+//   slots      k-mers whose key equals a predicted gkey load gslot there
+//              (contiguous), the others slot[key] at random
+template <int KPL>
+struct GoHashed {
+    uint32_t L; // letters, 0 = nothing to seed
+    uint32_t ky[KPL];
+    uint32_t anc; // kpos of this lane's anchor k-mer, ~0u: none
+};
+template <int KPL>
+struct GoPredicted {
+    uint32_t L;
+    uint32_t ky[KPL];
+    uint32_t gk1[KPL], gk2[KPL];
+    int64_t d1, d2;
+};
+template <int KPL>
+struct GoSlots {
+    uint32_t L;
+    uint64_t sl[KPL];
+};
+
+// anchors of K k-mers: k-mers 0, (K-1)/2 and K-1, at most one per lane (an
+// anchor on a lane already taken is dropped)
+struct GoAnchors {
+    int a[3];
+    bool v[3];
+};
+__device__ __forceinline__ GoAnchors go_anchors(uint32_t K)
+{
+    GoAnchors g;
+    g.a[0] = 0;
+    g.a[1] = (int)(K - 1) / 2;
+    g.a[2] = (int)K - 1;
+    g.v[0] = true;
+    g.v[2] = (g.a[2] & 63) != 0;
+    g.v[1] = (g.a[1] & 63) != 0 && (!g.v[2] || (g.a[1] & 63) != (g.a[2] & 63));
+    return g;
+}
+
+// keys of the k-mers of the row cw (lane l holds letters 4l .. 4l+3) through
+// the prefix array pf (this wave's LDS, L + 1 entries), anchor loads issued
+template <int KPL>
+__device__ __forceinline__ void go_hash(const DevIndex& ix, const SeedShared& sh, uint64_t* pf, uint32_t cw,
+                                        uint32_t L, GoHashed<KPL>& h, uint32_t& rnd)
+{
+    const int lane = threadIdx.x & 63;
+    h.L = L;
+    h.anc = ~0u;
+    if (L < KMERLEN) return; // 0: not seeded (k_prep never writes 1 .. KMERLEN-1)
+    uint64_t x[4], acc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t t = 4u * (uint32_t)lane + (uint32_t)j;
+        acc ^= t < L ? sh.rq[t & 63][(cw >> (8 * j)) & 3u] : 0ull;
+        x[j] = acc;
+    }
+    const uint64_t ex = wave_ixor_u64(acc) ^ acc; // letters below this lane's
+    wave_sync();
+    if (lane == 0) pf[0] = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t t = 4u * (uint32_t)lane + (uint32_t)j;
+        if (t < L) pf[t + 1] = ex ^ x[j];
+    }
+    wave_sync();
+    constexpr SeedEnds se = seed_ends();
+    const uint32_t K = L - KMERLEN + 1;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = (uint32_t)lane + 64u * q;
+        uint32_t key = 0;
+        if (k < K) {
+            uint64_t X = 0;
+#pragma unroll
+            for (int j = 0; j < se.n; ++j) X ^= pf[k + se.e[j]];
+            const uint64_t hv = rol64(X, (int)((KMERLEN - 1 + k) & 63));
+            key = (uint32_t)(ix.hpow2 ? (hv & ix.hmask) : (hv % ix.htab));
+        }
+        h.ky[q] = key;
+    }
+    wave_sync();
+    const GoAnchors g = go_anchors(K);
+    int my = -1;
+    if (lane == 0) my = g.a[0];
+    else if (g.v[2] && lane == (g.a[2] & 63)) my = g.a[2];
+    else if (g.v[1] && lane == (g.a[1] & 63)) my = g.a[1];
+    if (my >= 0) {
+        uint32_t key = h.ky[0];
+#pragma unroll
+        for (int q = 1; q < KPL; ++q)
+            if ((my >> 6) == q) key = h.ky[q];
+        h.anc = ix.kpos[key];
+        ++rnd;
+    }
+}
+
+// offsets from the anchors (one two of them agree on, else the first two
+// distinct ones) and the genome-ordered loads at them
+template <int KPL>
+__device__ __forceinline__ void go_predict(const DevIndex& ix, const GoHashed<KPL>& h, GoPredicted<KPL>& p)
+{
+    const int lane = threadIdx.x & 63;
+    p.L = h.L;
+    p.d1 = p.d2 = 0;
+    if (h.L < KMERLEN) return;
+    const uint32_t K = h.L - KMERLEN + 1;
+    const GoAnchors g = go_anchors(K);
+    int64_t d[3];
+    bool ok[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t pos = g.v[i] ? lane_u32(h.anc, g.a[i] & 63) : ~0u;
+        ok[i] = pos != ~0u;
+        d[i] = (int64_t)pos - g.a[i];
+    }
+    int64_t d1 = 0, d2 = 0;
+    int nd = 0;
+    if (ok[0] && ok[1] && d[0] == d[1]) {
+        d1 = d[0];
+        nd = 1;
+    } else if (ok[0] && ok[2] && d[0] == d[2]) {
+        d1 = d[0];
+        nd = 1;
+    } else if (ok[1] && ok[2] && d[1] == d[2]) {
+        d1 = d[1];
+        nd = 1;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (!ok[i]) continue;
+            if (nd == 0) {
+                d1 = d[i];
+                nd = 1;
+            } else if (nd == 1 && d[i] != d1) {
+                d2 = d[i];
+                nd = 2;
+            }
+        }
+    }
+    p.d1 = d1;
+    p.d2 = d2;
+    const int64_t gn = (int64_t)ix.gn;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = (uint32_t)lane + 64u * q;
+        p.ky[q] = h.ky[q];
+        const int64_t p1 = d1 + (int64_t)k, p2 = d2 + (int64_t)k;
+        const bool in1 = nd >= 1 && k < K && p1 >= 0 && p1 < gn;
+        const bool in2 = nd >= 2 && k < K && p2 >= 0 && p2 < gn;
+        p.gk1[q] = in1 ? ix.gkey[p1] : ~0u;
+        p.gk2[q] = in2 ? ix.gkey[p2] : ~0u;
+    }
+}
+
+// each k-mer's slot: gslot where the predicted key is its own, else the
+// slot table at random
+template <int KPL>
+__device__ __forceinline__ void go_fetch(const DevIndex& ix, const GoPredicted<KPL>& p, GoSlots<KPL>& f,
+                                         uint32_t& rnd, uint32_t& ord)
+{
+    const int lane = threadIdx.x & 63;
+    f.L = p.L;
+    const uint32_t K = p.L >= KMERLEN ? p.L - KMERLEN + 1 : 0u;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) {
+        const uint32_t k = (uint32_t)lane + 64u * q;
+        uint64_t v = 0;
+        if (k < K) {
+            const bool o1 = p.gk1[q] == p.ky[q], o2 = p.gk2[q] == p.ky[q];
+            const uint64_t* src = o1   ? ix.gslot + (p.d1 + (int64_t)k)
+                                : o2 ? ix.gslot + (p.d2 + (int64_t)k)
+                                     : ix.slot + p.ky[q];
+            v = *src;
+            if (o1 || o2) ++ord;
+            else ++rnd;
+        }
+        f.sl[q] = v;
+    }
+}
+
 // Capacities of the light seed kernel for reads of at most ML letters
 // (ML = 128 or 256): index hits ML, keys per map ML/2, counting slots ML.
 // Larger (read, orientation) pairs go to the heavy kernel.
@@ -577,7 +809,9 @@ struct SeedLds {
     uint32_t klen[ML];
     uint32_t koff[ML];
     uint32_t k2[ML];
-    union {
+    // the gather / count arrays; between pairs also the prefix array of the
+    // next pair's hash (go_hash)
+    union alignas(8) {
         struct {
             uint32_t ent[C::E];
             uint32_t eaux[C::E];
@@ -609,7 +843,8 @@ struct SeedLds {
 // holds several waves); each wave works through (read, orientation) pairs
 // in grid-stride order.  Both variants are launched; the one that does not
 // fit the batch's longest read (from k_prep) returns at once.
-template <typename TabT, int ML>
+// GO: the slots through the genome-ordered copy (ix.gkey set)
+template <typename TabT, int ML, bool GO>
 __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
 {
     using C = SeedCap<ML>;
@@ -671,7 +906,55 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             bd.cand_off[ro] = 0;
         }
     };
-    if (ix.slot) {
+    static_assert(sizeof(m.u) >= 8 * (ML + 1), "prefix array of the hash");
+    uint32_t go_rnd = 0, go_ord = 0;
+    if constexpr (GO) {
+        // genome-ordered slots, four pairs in flight per wave: pair qC is
+        // matched while qC + nw has its slot loads in flight, qC + 2nw its
+        // genome-ordered loads, qC + 3nw its anchor loads, and the row of
+        // qC + 4nw loads (the first three iterations only fill the stages)
+        uint64_t* pf = (uint64_t*)&m.u;
+        auto row = [&](uint32_t q) {
+            return ((const uint32_t*)(bd.codes + (size_t)(q < nq ? ro_of(q) : 0u) * 256))[lane];
+        };
+        GoHashed<C::KPL> sH;
+        GoPredicted<C::KPL> sP;
+        GoSlots<C::KPL> sF;
+        sH.L = sP.L = sF.L = 0;
+        sH.anc = ~0u;
+        const uint32_t q0 = blockIdx.x * FAME_SEED_WPB + wv;
+        uint32_t cwH = row(q0);
+        for (int64_t qC = (int64_t)q0 - 3 * (int64_t)nw; qC < (int64_t)nq; qC += nw) {
+            const uint32_t qH = (uint32_t)(qC + 3 * (int64_t)nw);
+            const uint32_t cwN = row(qH + nw);
+            GoHashed<C::KPL> nH;
+            GoPredicted<C::KPL> nP;
+            GoSlots<C::KPL> nF;
+            go_hash<C::KPL>(ix, sh, pf, cwH, qH < nq ? (lane_u32(cwH, 63) >> 24) : 0u, nH, go_rnd);
+            go_predict<C::KPL>(ix, sH, nP);
+            go_fetch<C::KPL>(ix, sP, nF, go_rnd, go_ord);
+            if (qC >= 0) {
+                const uint32_t ro = ro_of((uint32_t)qC);
+                if (sF.L == 0) { // not seeded: invalid read
+                    if (lane == 0) {
+                        bd.cand_nf[ro] = 0;
+                        bd.cand_nr[ro] = 0;
+                        bd.cand_off[ro] = 0;
+                    }
+                } else {
+                    uint64_t E;
+                    uint32_t bc;
+                    const int rc = seed_one<TabT, uint16_t, C::KPL>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc,
+                                                                      sF.L << 24, true, sF.sl);
+                    if (rc != 0) defer(ro, E, bc);
+                }
+            }
+            sH = nH;
+            sP = nP;
+            sF = nF;
+            cwH = cwN;
+        }
+    } else if (ix.slot) {
         // slot-table mode, software-pipelined over this wave's pairs: while
         // pair j is worked on, the slot loads of pair j+1 and the code row of
         // pair j+2 are in flight (rows past the batch read row 0, unused)
@@ -720,7 +1003,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             Lc = Ln;
         }
     }
-    for (uint32_t q = blockIdx.x * FAME_SEED_WPB + wv; !ix.slot && q < nq; q += nw) {
+    for (uint32_t q = blockIdx.x * FAME_SEED_WPB + wv; !GO && !ix.slot && q < nq; q += nw) {
         const uint32_t ro = ro_of(q);
         const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)ro * 256))[lane];
         if ((lane_u32(cw, 63) >> 24) == 0) { // not seeded: invalid read
@@ -738,12 +1021,23 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         if (rc != 0) defer(ro, E, bc);
     }
     if (pc.left) pool_gap_fill(bd, pc.pool, pc.base, pc.left, lane);
+    if constexpr (GO) {
+        go_rnd = wave_sum_u32(go_rnd);
+        go_ord = wave_sum_u32(go_ord);
+    } else {
+        go_rnd = (uint32_t)ss.kmers;
+    }
     if (lane == 0 && ss.seeded) {
         atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
         atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
         atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
+    }
+    // slot reads, also those of pairs deferred to the heavy kernel
+    if (lane == 0 && (go_rnd || go_ord)) {
+        atomicAdd(&bd.stats[11], (unsigned long long)go_rnd);
+        atomicAdd(&bd.stats[12], (unsigned long long)go_ord);
     }
 }
 
@@ -833,6 +1127,7 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
         atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
         atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
         atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
+        atomicAdd(&bd.stats[11], (unsigned long long)ss.kmers); // random slot lookups
     }
 }
 
@@ -2655,6 +2950,69 @@ __global__ void k_build_slots(const uint32_t* __restrict__ tab, const uint32_t* 
     }
 }
 
+// The genome-ordered copy of the slot table (DevIndex::gkey / gslot / kpos).
+// Position p < ng is the forward k-mer at p, letters seq[p .. p+31] reduced
+// C->T as a read's row is (A 0, G 1, C/T 2); p = ng + r is the k-mer at r of
+// the reverse complement, letters comp(seq[ng-1-r-i]) reduced the same way.
+// Its key is the spaced-seed hash of those codes taken % htab, as seed_key
+// hashes a read's row; any other letter, or a k-mer running past the end,
+// gives ~0u.  kpos[key] keeps one position of the key (whichever store lands
+// last: it only steers which lines the seed stage reads).
+__device__ __forceinline__ uint32_t gorder_code(uint8_t c, bool rv)
+{
+    switch (c) {
+    case 'A': case 'a': return rv ? 2u : 0u;
+    case 'C': case 'c': return rv ? 1u : 2u;
+    case 'G': case 'g': return rv ? 2u : 1u;
+    case 'T': case 't': return rv ? 0u : 2u;
+    default: return 3u;
+    }
+}
+
+__global__ void k_gorder_keys(const uint8_t* __restrict__ seq, uint64_t ng, uint64_t htab,
+                              uint32_t* __restrict__ gkey, uint32_t* __restrict__ kpos)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const bool pow2 = (htab & (htab - 1)) == 0;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < 2 * ng; p += step) {
+        const bool rv = p >= ng;
+        const uint64_t r = rv ? p - ng : p;
+        uint32_t key = ~0u;
+        if (r + KMERLEN <= ng) {
+            uint64_t h = 0;
+            bool ok = true;
+#pragma unroll
+            for (uint32_t i = 0; i < KMERLEN; ++i) {
+                const uint8_t c = rv ? seq[ng - 1 - (r + i)] : seq[r + i];
+                const uint32_t code = gorder_code(c, rv);
+                ok = ok && code != 3u;
+                if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= rol64(red_seed(code), (int)(KMERLEN - 1 - i));
+            }
+            if (ok) {
+                key = pow2 ? (uint32_t)(h & (htab - 1)) : (uint32_t)(h % htab);
+                kpos[key] = (uint32_t)p;
+            }
+        }
+        gkey[p] = key;
+    }
+}
+
+__global__ void k_gorder_slots(const uint32_t* __restrict__ gkey, uint64_t gn, const uint64_t* __restrict__ slot,
+                               uint64_t* __restrict__ gslot)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < gn; p += step) {
+        const uint32_t k = gkey[p];
+        gslot[p] = k != ~0u ? slot[k] : 0ull;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
+}
+
 // the packed text of the verify stage: word w holds letters 16w .. 16w+15 of
 // the chromosome, two bits each, and the mask of those that are not A, C, G
 // or T; letters past the end are 0 and unmasked
@@ -2732,11 +3090,20 @@ void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hi
         else gs = 16;
     }
     if (ix.tab_is64) {
-        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint64_t, 256>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT, false>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
+                           bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
+                           n_ro);
+    } else if (ix.slot && ix.gkey) {
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, true>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
+                           bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, true>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
+                           n_ro);
     } else {
-        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint32_t, 256>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, false>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
+                           bd, n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
+                           n_ro);
     }
 }
 
@@ -2938,6 +3305,14 @@ void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab
 {
     if (!htab) return;
     hipLaunchKernelGGL(k_build_slots, dim3(16384), dim3(256), 0, st, tab, kent, htab, slot, bad);
+}
+
+void launch_gorder(const uint8_t* seq, uint64_t ng, uint64_t htab, const uint64_t* slot, uint32_t* gkey,
+                   uint64_t* gslot, uint32_t* kpos, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(16384), dim3(256), 0, st, kpos, htab, ~0u);
+    hipLaunchKernelGGL(k_gorder_keys, dim3(65536), dim3(256), 0, st, seq, ng, htab, gkey, kpos);
+    hipLaunchKernelGGL(k_gorder_slots, dim3(16384), dim3(256), 0, st, gkey, 2 * ng, slot, gslot);
 }
 
 void launch_pack_seq(const uint8_t* seq, uint64_t len, uint64_t nwords, uint64_t* out, hipStream_t st)

@@ -115,7 +115,8 @@ struct fame_ctx {
     std::vector<std::string> chr_name; // 256 entries, "" when unmapped
     // device index
     DevBuf d_tab, d_slot, d_kent, d_kpe, d_filt32, d_win, d_seq, d_coff, d_clen, d_cpg_pos, d_cpg_chrom, d_pk,
-        d_pkoff, d_wtext;
+        d_pkoff, d_wtext, d_gkey, d_gslot, d_kpos;
+    uint64_t seq_total = 0;                // letters of all chromosomes
     std::vector<uint64_t> h_pkoff, h_clen; // per chromosome, for d_wtext
     DevBuf d_meth, d_counters, d_flags, d_stats;
     uint64_t stats[FAME_NSTATS] = {0};
@@ -247,6 +248,7 @@ void load_index(fame_ctx* c, const std::string& path)
             total += clen[i];
             f.skip(clen[i]);
         }
+        c->seq_total = total;
         HIPC(c->d_seq.reserve(total + 64));
         HIPC(hipMemset(c->d_seq.p, 0, total + 64));
         fseeko(f.f, pos0, SEEK_SET);
@@ -368,6 +370,24 @@ void load_index(fame_ctx* c, const std::string& path)
         HIPC(hipStreamSynchronize(st));
         if (b) c->d_slot.release();
     }
+    // the slot table in genome order (DevIndex::gkey): positions and keys
+    // must fit 32 bits with ~0u to spare, and the three arrays must leave
+    // room for the batches; FAME_NO_GORDER=1 keeps every lookup random
+    const char* no_go = getenv("FAME_NO_GORDER");
+    const uint64_t gn = 2 * c->seq_total;
+    if (c->d_slot.p && !(no_go && no_go[0] == '1') && gn >= 64 && gn < 0xFFFFFFFFull &&
+        c->htab <= 0xFFFFFFFFull) {
+        const uint64_t need = 12 * gn + 4 * c->htab;
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > need + (8ull << 30)) {
+            HIPC(c->d_gkey.reserve(4 * gn));
+            HIPC(c->d_gslot.reserve(8 * gn));
+            HIPC(c->d_kpos.reserve(4 * c->htab));
+            launch_gorder(c->d_seq.as<uint8_t>(), c->seq_total, c->htab, c->d_slot.as<uint64_t>(),
+                          c->d_gkey.as<uint32_t>(), c->d_gslot.as<uint64_t>(), c->d_kpos.as<uint32_t>(), st);
+            HIPC(hipStreamSynchronize(st));
+        }
+    }
     f.skip(8 * f.get<uint64_t>()); // metaCpGs (empty in current builds)
     f.skip(8 * f.get<uint64_t>()); // metaStartCpGs
     // windows {u32 startPos; u8 chrom; 3 pad; u32 startInd; u32 endInd}
@@ -432,6 +452,10 @@ void load_index(fame_ctx* c, const std::string& path)
     ix.hmask = c->htab - 1;
     ix.kent = c->d_kent.as<uint32_t>();
     ix.slot = c->d_slot.p ? c->d_slot.as<uint64_t>() : nullptr;
+    ix.gkey = c->d_gkey.p ? c->d_gkey.as<uint32_t>() : nullptr;
+    ix.gslot = c->d_gkey.p ? c->d_gslot.as<uint64_t>() : nullptr;
+    ix.kpos = c->d_gkey.p ? c->d_kpos.as<uint32_t>() : nullptr;
+    ix.gn = c->d_gkey.p ? 2 * c->seq_total : 0;
     ix.kpe = c->d_kpe.as<uint64_t>();
     ix.filt32 = c->d_filt32.as<uint32_t>();
     ix.win = c->d_win.as<uint4>();

@@ -22,7 +22,7 @@ struct fame_ctx;
 #define FAME_PE_WCAP 8         // windows of an item k_pe_walk1 walks in LDS, one lane per item
 #define FAME_PE_WBIG 1360      // windows of an item k_pe_walk1_big walks in LDS, one wave per item
 #define FAME_NPOOL 5 // verify pools by 64-bit words of the read: 1..4, then reads over FAME_DEV_MAX_READ
-#define FAME_NSTATS 11
+#define FAME_NSTATS 13
 #define FAME_SEEDBITS 0xDEFDFFFDu // CONST.h:58 (0b11011110111111011111111111111101)
 #define FAME_SEED_WAVES 16384      // persistent waves of the seed kernel
 #ifndef FAME_SEED_WPB
@@ -71,6 +71,21 @@ struct DevIndex {
     // offset and length in kent; NULL when some bucket does not fit
     // (see k_build_slots)
     const uint64_t* slot;
+    // the slot table in genome order, so that the k-mers of a read that
+    // occur where its other k-mers put it are read from a few contiguous
+    // lines instead of one random line each.  Positions p < gn/2 are the
+    // forward k-mers (letters seq[p..p+31] reduced C->T), positions
+    // gn/2 + r those of the reverse complement (letters comp(seq[gn/2-1-r-i])
+    // reduced C->T).  gkey[p] = the hash key of the k-mer at p (~0u when it
+    // holds a letter other than A/C/G/T or runs past the end), gslot[p] =
+    // slot[gkey[p]], kpos[key] = some p with gkey[p] == key (~0u: none).
+    // Since a slot is a function of its key alone, a k-mer whose key equals
+    // gkey[p] takes gslot[p] whatever p is: the prediction only decides how
+    // many lines are read, never what is found.  NULL when off.
+    const uint32_t* gkey;
+    const uint64_t* gslot;
+    const uint32_t* kpos;
+    uint64_t gn;
     const uint64_t* kpe;
     const uint32_t* filt32;
     uint32_t nfilt32;
@@ -311,6 +326,8 @@ void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
 // nwords packed words (16 letters each) of one chromosome of len letters
 void launch_pack_seq(const uint8_t* seq, uint64_t len, uint64_t nwords, uint64_t* out, hipStream_t st);
+void launch_gorder(const uint8_t* seq, uint64_t ng, uint64_t htab, const uint64_t* slot, uint32_t* gkey,
+                   uint64_t* gslot, uint32_t* kpos, hipStream_t st);
 void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab, uint64_t* slot, uint32_t* bad,
                         hipStream_t st);
 void launch_meth_u16(const uint32_t* in, uint64_t n, uint16_t* out, hipStream_t st);

@@ -259,9 +259,12 @@ int fame_last_timings(const fame_ctx* ctx, double out[FAME_TIMING_FIELDS]);
  * lookup, [8] the bucket lengths of every k-mer looked up, summed (index
  * entries the seeds name), [9] read letters aligned by the banded DP (rows
  * of 17 band cells each), [10] window letters scanned times the pattern's
- * 64-bit words (bit-parallel word updates of a full scan).
+ * 64-bit words (bit-parallel word updates of a full scan), [11] k-mer
+ * slots read at random (the bucket lookups, plus the key-position lookups
+ * of the genome-ordered slot copy), [12] k-mer slots read from the
+ * genome-ordered copy (contiguous per read).
  * No counterpart in the reference (instrumentation). */
-#define FAME_STATS_FIELDS 11
+#define FAME_STATS_FIELDS 13
 int fame_last_stats(const fame_ctx* ctx, uint64_t out[FAME_STATS_FIELDS]);
 
 #ifdef __cplusplus

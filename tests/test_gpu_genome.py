@@ -102,6 +102,12 @@ def test_genome_scale_plain_and_batched(genome_case):
     assert ctx.info["htab"] == HTAB and ctx.info["nchrom"] == 24
     print("plain form", flush=True)
     check(ctx, g, *run_loop(ctx, g), "plain")
+    # the stranded chunk (100 000 reads, one orientation): most k-mers are
+    # read from the genome-ordered slot copy at 2^30 keys
+    st = ctx.stats()
+    print(f"slots: {st}", flush=True)
+    assert st["ordered_slots"] > 0.6 * st["kmers"], st
+    assert st["random_slots"] < 0.4 * st["kmers"], st
     ctx.reset()
     print("batched form, pipelined lanes", flush=True)
     ctx.set_form("batched")
@@ -110,6 +116,15 @@ def test_genome_scale_plain_and_batched(genome_case):
     info = ctx.form_info()
     assert info["pipelined_calls"] >= 1 and info["graph_launches"] >= 4
     check(ctx, g, valid, mat, cnt, "batched")
+    ctx.close()
+
+
+def test_genome_scale_without_genome_order(genome_case, monkeypatch):
+    monkeypatch.setenv("FAME_NO_GORDER", "1")
+    g = genome_case
+    print("slot table read at random", flush=True)
+    ctx = _lib.Context(g["idx"])
+    check(ctx, g, *run_loop(ctx, g), "no_gorder")
     ctx.close()
 
 

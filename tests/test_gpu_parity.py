@@ -82,6 +82,45 @@ def test_hip_tabindex_path_matches_reference(golden_case, monkeypatch, tab64):
     ctx.close()
 
 
+def test_hip_without_genome_order_matches_reference(golden_case, monkeypatch):
+    """The slot table read at random for every k-mer (FAME_NO_GORDER=1, a
+    test-only switch that leaves out the genome-ordered copy): the same
+    results."""
+    monkeypatch.setenv("FAME_NO_GORDER", "1")
+    c = golden_case
+    print(f"case {c.name}: no genome-ordered slots", flush=True)
+    ctx, valid, mat, cnt, _ = run_hip(c)
+    bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
+    assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    st = ctx.stats()
+    assert st["ordered_slots"] == 0 and st["random_slots"] >= st["kmers"]
+    ctx.close()
+
+
+def test_hip_genome_ordered_slots_are_used(golden_case):
+    """With the genome-ordered copy (the default), the k-mers of reads drawn
+    from the reference are mostly read from it: every k-mer is read once,
+    from the copy or at random (plus the anchors' key-position lookups)."""
+    c = golden_case
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    n = min(len(c.lens), 2000)
+    cnt = np.zeros(3, np.uint64)
+    ctx.match(c.seqs, c.offs[:n], c.lens[:n], True, cnt)
+    st = ctx.stats()
+    ctx.close()
+    print(f"case {c.name}: {st}", flush=True)
+    assert st["random_slots"] + st["ordered_slots"] >= st["kmers"]
+    if c.name == "se_basic":
+        # 2^16 keys for about 2e5 positions: a key position is often another
+        # occurrence of the key, so only a part is found in the copy here
+        # (the genome-scale test checks the fraction at 2^30 keys; in the
+        # READLEN 150 and repeat cases most reads exceed the light kernel's
+        # 256 index hits and the heavy kernel reads every slot at random)
+        assert st["ordered_slots"] > 0, st
+
+
 @pytest.mark.parametrize("limit", [1, 7])
 def test_hip_verify_queue_full_matches_reference(golden_case, monkeypatch, limit):
     """Verify's per-block run queue held to a few runs per round of 256 jobs
