@@ -244,93 +244,15 @@ __device__ __forceinline__ void seed_stage(const DevIndex& ix, const SeedShared&
     }
 }
 
-// KPL > 0: the read's row of reduced letters arrives as this lane's word cw
-// (a 256-byte row, the length in its byte 255); KPL == 0 (long reads): the
-// row is already in w.code, L letters long, and the k-mers are hashed and
-// looked up 64 at a time (no register arrays)
-template <typename TabT, typename IdxT, int KPL, typename EkmT>
-__device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
-                        const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
-                        uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
-                        const uint64_t* pre = nullptr, uint32_t Llong = 0)
+// exclusive prefix of the bucket lengths over k-mers in index order (koff),
+// E = all index hits of the (read, orientation); false when it needs the
+// heavy path (E beyond the capacity, or a bucket beyond what the device
+// path serves); else the work counters
+template <typename TabT, typename IdxT, typename EkmT>
+__device__ __forceinline__ bool seed_prefix(const SeedWork<TabT, IdxT, EkmT>& w, uint32_t K, uint32_t L, bool inl,
+                                            SeedStats& ss, uint64_t& E)
 {
     const int lane = threadIdx.x & 63;
-    // cw: this lane's word of the row of reduced letters (the length is in
-    // byte 255, the top byte of lane 63's word)
-    const uint32_t L = KPL == 0 ? Llong : lane_u32(cw, 63) >> 24;
-    if (KPL > 0 && !staged) {
-        ((uint32_t*)w.code)[lane] = cw;
-        wave_sync();
-    }
-    const uint32_t K = L - KMERLEN + 1;
-    const TabT* tab = (const TabT*)ix.tab;
-    const bool inl = ix.slot != nullptr && w.k2 != nullptr;
-    // hashes + bucket gathers (NTPS64 closed form; key = h % HTABSIZE): one
-    // slot per k-mer, or tabIndex[key], tabIndex[key + 1]; all gathers of a
-    // lane are issued before the first is consumed.  staged: the slots were
-    // loaded by seed_stage (pre)
-    uint32_t bsum = 0;
-    if constexpr (KPL == 0) {
-        for (uint32_t k0 = 0; k0 < K; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            if (k < K) {
-                const uint64_t key = seed_key(ix, sh, w.code, k);
-                if (inl) {
-                    const uint64_t sv = ix.slot[key];
-                    const uint32_t len = slot_len(sv);
-                    w.klo[k] = (TabT)slot_lo(sv);
-                    w.k2[k] = slot_e1(sv);
-                    w.klen[k] = len;
-                    bsum += len;
-                } else {
-                    const TabT lo = tab[key], hi = tab[key + 1];
-                    w.klo[k] = lo;
-                    const uint64_t len = (uint64_t)(hi - lo);
-                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
-                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
-                }
-            }
-        }
-    } else {
-        TabT lo[KPL], hi[KPL];
-        uint64_t sl[KPL];
-    #pragma unroll
-        for (int q = 0; q < KPL; ++q) {
-            const uint32_t k = lane + 64 * q;
-            lo[q] = hi[q] = 0;
-            sl[q] = staged ? pre[q] : 0;
-            if (k < K && !staged) {
-                const uint64_t key = seed_key(ix, sh, w.code, k);
-                if (inl) {
-                    sl[q] = ix.slot[key];
-                } else {
-                    lo[q] = tab[key];
-                    hi[q] = tab[key + 1];
-                }
-            }
-        }
-    #pragma unroll
-        for (int q = 0; q < KPL; ++q) {
-            const uint32_t k = lane + 64 * q;
-            if (k < K) {
-                if (inl) {
-                    const uint32_t len = slot_len(sl[q]);
-                    w.klo[k] = (TabT)slot_lo(sl[q]);
-                    w.k2[k] = slot_e1(sl[q]);
-                    w.klen[k] = len;
-                    bsum += len;
-                } else {
-                    w.klo[k] = lo[q];
-                    const uint64_t len = (uint64_t)(hi[q] - lo[q]);
-                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
-                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
-                }
-            }
-        }
-    }
-    bsum = wave_sum_u32(bsum);
-    const uint32_t bucketCount = bsum / 5;
-    // exclusive prefix over k-mers in index order
     uint64_t carry = 0;
     bool big = false;
     uint32_t gathered = 0; // entries read from the k-mer table (slot mode: buckets of three or more)
@@ -349,49 +271,28 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
         if (carry > 0x7FFFFFFFull) big = true;
     }
     big = __ballot(big) != 0;
-    const uint64_t E = carry;
-    *E_out = E;
-    *bc_out = bucketCount;
-    if (big || E > w.cap_e) return 1;
+    E = carry;
+    if (big || E > w.cap_e) return false;
     ss.kmers += K;
     ss.entries += wave_sum_u32(gathered);
     ss.seeded += 1;
     ss.bytes += L;
     ss.buckets += E;
     wave_sync();
-    // entry gather, one entry per lane: entry j belongs to the k-mer whose
-    // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
-    // its index over its entries first; beyond, a binary search over koff.
-    const bool scatter = E <= 256;
-    if (scatter) {
-        for (uint32_t k = lane; k < K; k += 64) {
-            const uint32_t o = w.koff[k], n = w.klen[k];
-            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
-        }
-        wave_sync();
-    }
-    for (uint32_t j = lane; j < E; j += 64) {
-        uint32_t k;
-        if (scatter) {
-            k = w.ekm[j];
-        } else {
-            uint32_t lo = 0, hi = K; // last k with koff[k] <= j and klen[k] > 0
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (w.koff[mid] <= j) lo = mid;
-                else hi = mid;
-            }
-            k = lo;
-            while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
-        }
-        const uint32_t d = j - w.koff[k];
-        if (inl && w.klen[k] <= 2) // the bucket is in its slot
-            w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
-        else
-            w.ent[j] = ix.kent[(uint64_t)w.klo[k] + d] & 0x7FFFFFFFu; // without isStart
-        if (!scatter) w.ekm[j] = (EkmT)k;
-    }
-    wave_sync();
+    return true;
+}
+
+// the rest of getSeedRefs for one (read, orientation) once every index hit
+// is in w.ent / w.ekm (E of them, in k-mer order): consecutive dedup, counts
+// and first occurrences, the two maps' insertion sequences, hopscotch order
+// where it matters, and the candidate windows into the verify pools.
+// Returns 1 when a capacity of the caller's arrays is exceeded (heavy path).
+template <typename TabT, typename IdxT, typename EkmT>
+__device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd, uint32_t ro,
+                                          const SeedWork<TabT, IdxT, EkmT>& w,
+                         PoolChunk& pc, bool exact, uint64_t E, uint32_t bucketCount, uint32_t L)
+{
+    const int lane = threadIdx.x & 63;
     // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
     for (uint32_t j = lane; j < E; j += 64) {
         const bool skip = j > 0 && w.ekm[j] == w.ekm[j - 1] && w.ent[j] == w.ent[j - 1];
@@ -554,6 +455,132 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     return 0;
 }
 
+// KPL > 0: the read's row of reduced letters arrives as this lane's word cw
+// (a 256-byte row, the length in its byte 255); KPL == 0 (long reads): the
+// row is already in w.code, L letters long, and the k-mers are hashed and
+// looked up 64 at a time (no register arrays)
+template <typename TabT, typename IdxT, int KPL, typename EkmT>
+__device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
+                        const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
+                        uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
+                        const uint64_t* pre = nullptr, uint32_t Llong = 0)
+{
+    const int lane = threadIdx.x & 63;
+    // cw: this lane's word of the row of reduced letters (the length is in
+    // byte 255, the top byte of lane 63's word)
+    const uint32_t L = KPL == 0 ? Llong : lane_u32(cw, 63) >> 24;
+    if (KPL > 0 && !staged) {
+        ((uint32_t*)w.code)[lane] = cw;
+        wave_sync();
+    }
+    const uint32_t K = L - KMERLEN + 1;
+    const TabT* tab = (const TabT*)ix.tab;
+    const bool inl = ix.slot != nullptr && w.k2 != nullptr;
+    // hashes + bucket gathers (NTPS64 closed form; key = h % HTABSIZE): one
+    // slot per k-mer, or tabIndex[key], tabIndex[key + 1]; all gathers of a
+    // lane are issued before the first is consumed.  staged: the slots were
+    // loaded by seed_stage (pre)
+    uint32_t bsum = 0;
+    if constexpr (KPL == 0) {
+        for (uint32_t k0 = 0; k0 < K; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            if (k < K) {
+                const uint64_t key = seed_key(ix, sh, w.code, k);
+                if (inl) {
+                    const uint64_t sv = ix.slot[key];
+                    const uint32_t len = slot_len(sv);
+                    w.klo[k] = (TabT)slot_lo(sv);
+                    w.k2[k] = slot_e1(sv);
+                    w.klen[k] = len;
+                    bsum += len;
+                } else {
+                    const TabT lo = tab[key], hi = tab[key + 1];
+                    w.klo[k] = lo;
+                    const uint64_t len = (uint64_t)(hi - lo);
+                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+                }
+            }
+        }
+    } else {
+        TabT lo[KPL], hi[KPL];
+        uint64_t sl[KPL];
+    #pragma unroll
+        for (int q = 0; q < KPL; ++q) {
+            const uint32_t k = lane + 64 * q;
+            lo[q] = hi[q] = 0;
+            sl[q] = staged ? pre[q] : 0;
+            if (k < K && !staged) {
+                const uint64_t key = seed_key(ix, sh, w.code, k);
+                if (inl) {
+                    sl[q] = ix.slot[key];
+                } else {
+                    lo[q] = tab[key];
+                    hi[q] = tab[key + 1];
+                }
+            }
+        }
+    #pragma unroll
+        for (int q = 0; q < KPL; ++q) {
+            const uint32_t k = lane + 64 * q;
+            if (k < K) {
+                if (inl) {
+                    const uint32_t len = slot_len(sl[q]);
+                    w.klo[k] = (TabT)slot_lo(sl[q]);
+                    w.k2[k] = slot_e1(sl[q]);
+                    w.klen[k] = len;
+                    bsum += len;
+                } else {
+                    w.klo[k] = lo[q];
+                    const uint64_t len = (uint64_t)(hi[q] - lo[q]);
+                    w.klen[k] = len > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)len;
+                    bsum += (uint32_t)len; // bucketCount += (u32 truncation of the u64 difference)
+                }
+            }
+        }
+    }
+    const uint32_t bucketCount = wave_sum_u32(bsum) / 5;
+    uint64_t E;
+    const bool fits = seed_prefix(w, K, L, inl, ss, E);
+    *E_out = E;
+    *bc_out = bucketCount;
+    if (!fits) return 1;
+    // entry gather, one entry per lane: entry j belongs to the k-mer whose
+    // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
+    // its index over its entries first; beyond, a binary search over koff.
+    const bool scatter = E <= 256;
+    if (scatter) {
+        for (uint32_t k = lane; k < K; k += 64) {
+            const uint32_t o = w.koff[k], n = w.klen[k];
+            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
+        }
+        wave_sync();
+    }
+    for (uint32_t j = lane; j < E; j += 64) {
+        uint32_t k;
+        if (scatter) {
+            k = w.ekm[j];
+        } else {
+            uint32_t lo = 0, hi = K; // last k with koff[k] <= j and klen[k] > 0
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (w.koff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            k = lo;
+            while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k; // skip empty buckets sharing koff
+        }
+        const uint32_t d = j - w.koff[k];
+        if (inl && w.klen[k] <= 2) // the bucket is in its slot
+            w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
+        else
+            w.ent[j] = ix.kent[(uint64_t)w.klo[k] + d] & 0x7FFFFFFFu; // without isStart
+        if (!scatter) w.ekm[j] = (EkmT)k;
+    }
+    wave_sync();
+    return seed_tail<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L);
+}
+
 // ---------------------------------------------------------------------------
 // Slot lookups through the genome-ordered copy (DevIndex::gkey)
 // ---------------------------------------------------------------------------
@@ -618,8 +645,9 @@ template <int KPL>
 struct GoPredicted {
     uint32_t L;
     uint32_t ky[KPL];
-    uint32_t gk1[KPL], gk2[KPL];
+    uint32_t gk1[KPL], gk2[KPL]; // as loaded: valid where the offset puts k-mer k inside the copy
     int64_t d1, d2;
+    int nd; // offsets found (0-2)
 };
 template <int KPL>
 struct GoSlots {
@@ -654,6 +682,8 @@ __device__ __forceinline__ void go_hash(const DevIndex& ix, const SeedShared& sh
     const int lane = threadIdx.x & 63;
     h.L = L;
     h.anc = ~0u;
+#pragma unroll
+    for (int q = 0; q < KPL; ++q) h.ky[q] = 0;
     if (L < KMERLEN) return; // 0: not seeded (k_prep never writes 1 .. KMERLEN-1)
     uint64_t x[4], acc = 0;
 #pragma unroll
@@ -687,19 +717,30 @@ __device__ __forceinline__ void go_hash(const DevIndex& ix, const SeedShared& sh
         h.ky[q] = key;
     }
     wave_sync();
-    const GoAnchors g = go_anchors(K);
+}
+
+// the anchors' key positions: one load per lane, always issued (lanes
+// without an anchor read entry 0 and drop it), so that the waits of the
+// seed loop count loads exactly
+template <int KPL>
+__device__ __forceinline__ void go_anchor_load(const DevIndex& ix, GoHashed<KPL>& h, uint32_t& rnd)
+{
+    const int lane = threadIdx.x & 63;
+    const bool go = h.L >= KMERLEN;
+    const GoAnchors g = go_anchors(go ? h.L - KMERLEN + 1 : 1u);
     int my = -1;
     if (lane == 0) my = g.a[0];
     else if (g.v[2] && lane == (g.a[2] & 63)) my = g.a[2];
     else if (g.v[1] && lane == (g.a[1] & 63)) my = g.a[1];
-    if (my >= 0) {
-        uint32_t key = h.ky[0];
+    if (!go) my = -1;
+    uint32_t key = h.ky[0];
 #pragma unroll
-        for (int q = 1; q < KPL; ++q)
-            if ((my >> 6) == q) key = h.ky[q];
-        h.anc = ix.kpos[key];
-        ++rnd;
-    }
+    for (int q = 1; q < KPL; ++q)
+        if ((my >> 6) == q) key = h.ky[q];
+    // only the anchor lanes' values are read (go_predict); the loaded value
+    // is not touched here, which would wait for it
+    h.anc = ix.kpos[my >= 0 ? key : 0u];
+    rnd += my >= 0 ? 1u : 0u;
 }
 
 // offsets from the anchors (one two of them agree on, else the first two
@@ -709,10 +750,9 @@ __device__ __forceinline__ void go_predict(const DevIndex& ix, const GoHashed<KP
 {
     const int lane = threadIdx.x & 63;
     p.L = h.L;
-    p.d1 = p.d2 = 0;
-    if (h.L < KMERLEN) return;
-    const uint32_t K = h.L - KMERLEN + 1;
-    const GoAnchors g = go_anchors(K);
+    const bool go = h.L >= KMERLEN;
+    const uint32_t K = go ? h.L - KMERLEN + 1 : 0u;
+    const GoAnchors g = go_anchors(go ? K : 1u);
     int64_t d[3];
     bool ok[3];
 #pragma unroll
@@ -723,7 +763,8 @@ __device__ __forceinline__ void go_predict(const DevIndex& ix, const GoHashed<KP
     }
     int64_t d1 = 0, d2 = 0;
     int nd = 0;
-    if (ok[0] && ok[1] && d[0] == d[1]) {
+    if (!go) {
+    } else if (ok[0] && ok[1] && d[0] == d[1]) {
         d1 = d[0];
         nd = 1;
     } else if (ok[0] && ok[2] && d[0] == d[2]) {
@@ -747,6 +788,7 @@ __device__ __forceinline__ void go_predict(const DevIndex& ix, const GoHashed<KP
     }
     p.d1 = d1;
     p.d2 = d2;
+    p.nd = nd;
     const int64_t gn = (int64_t)ix.gn;
 #pragma unroll
     for (int q = 0; q < KPL; ++q) {
@@ -755,8 +797,10 @@ __device__ __forceinline__ void go_predict(const DevIndex& ix, const GoHashed<KP
         const int64_t p1 = d1 + (int64_t)k, p2 = d2 + (int64_t)k;
         const bool in1 = nd >= 1 && k < K && p1 >= 0 && p1 < gn;
         const bool in2 = nd >= 2 && k < K && p2 >= 0 && p2 < gn;
-        p.gk1[q] = in1 ? ix.gkey[p1] : ~0u;
-        p.gk2[q] = in2 ? ix.gkey[p2] : ~0u;
+        // always issued (entry 0 for lanes without a prediction), and not
+        // touched before go_fetch, which checks the offsets again
+        p.gk1[q] = ix.gkey[in1 ? p1 : 0];
+        p.gk2[q] = ix.gkey[in2 ? p2 : 0];
     }
 }
 
@@ -769,20 +813,23 @@ __device__ __forceinline__ void go_fetch(const DevIndex& ix, const GoPredicted<K
     const int lane = threadIdx.x & 63;
     f.L = p.L;
     const uint32_t K = p.L >= KMERLEN ? p.L - KMERLEN + 1 : 0u;
+    const int64_t gn = (int64_t)ix.gn;
 #pragma unroll
     for (int q = 0; q < KPL; ++q) {
         const uint32_t k = (uint32_t)lane + 64u * q;
-        uint64_t v = 0;
-        if (k < K) {
-            const bool o1 = p.gk1[q] == p.ky[q], o2 = p.gk2[q] == p.ky[q];
-            const uint64_t* src = o1   ? ix.gslot + (p.d1 + (int64_t)k)
-                                : o2 ? ix.gslot + (p.d2 + (int64_t)k)
-                                     : ix.slot + p.ky[q];
-            v = *src;
-            if (o1 || o2) ++ord;
-            else ++rnd;
-        }
-        f.sl[q] = v;
+        const bool in = k < K;
+        const int64_t p1 = p.d1 + (int64_t)k, p2 = p.d2 + (int64_t)k;
+        const bool in1 = in && p.nd >= 1 && p1 >= 0 && p1 < gn;
+        const bool in2 = in && p.nd >= 2 && p2 >= 0 && p2 < gn;
+        const bool o1 = in1 && p.gk1[q] == p.ky[q], o2 = !o1 && in2 && p.gk2[q] == p.ky[q];
+        // always issued (slot 0 for lanes past the k-mers); go_gather reads
+        // only the slots of k-mers below K
+        const uint64_t* src = o1   ? ix.gslot + p1
+                            : o2 ? ix.gslot + p2
+                                 : ix.slot + (in ? p.ky[q] : 0u);
+        f.sl[q] = *src;
+        ord += (o1 || o2) ? 1u : 0u;
+        rnd += (in && !(o1 || o2)) ? 1u : 0u;
     }
 }
 
@@ -854,7 +901,8 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
     __shared__ SeedShared sh;
     __shared__ SeedLds<TabT, ML> lds[FAME_SEED_WPB];
     const int lane = threadIdx.x & 63;
-    const uint32_t wv = threadIdx.x >> 6;
+    // wave-uniform (a scalar): the LDS pointers below then live in SGPRs
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     SeedLds<TabT, ML>& m = lds[wv];
     SeedWork<TabT, uint16_t> w;
     w.klo = m.klo;
@@ -931,6 +979,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             GoPredicted<C::KPL> nP;
             GoSlots<C::KPL> nF;
             go_hash<C::KPL>(ix, sh, pf, cwH, qH < nq ? (lane_u32(cwH, 63) >> 24) : 0u, nH, go_rnd);
+            go_anchor_load<C::KPL>(ix, nH, go_rnd);
             go_predict<C::KPL>(ix, sH, nP);
             go_fetch<C::KPL>(ix, sP, nF, go_rnd, go_ord);
             if (qC >= 0) {
@@ -3095,9 +3144,11 @@ void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hi
         hipLaunchKernelGGL((k_seed_light<uint64_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
                            n_ro);
     } else if (ix.slot && ix.gkey) {
+        // the genome-ordered pipeline for reads of up to 128 letters (the
+        // longer variant's stage registers would not fit without spills)
         hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, true>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
                            bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, true>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
                            n_ro);
     } else {
         hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, false>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
