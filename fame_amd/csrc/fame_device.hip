@@ -460,7 +460,7 @@ __device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd,
 // row is already in w.code, L letters long, and the k-mers are hashed and
 // looked up 64 at a time (no register arrays)
 template <typename TabT, typename IdxT, int KPL, typename EkmT>
-__device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
+__device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
                         const uint64_t* pre = nullptr, uint32_t Llong = 0)
@@ -547,7 +547,38 @@ __device__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, See
     if (!fits) return 1;
     // entry gather, one entry per lane: entry j belongs to the k-mer whose
     // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
-    // its index over its entries first; beyond, a binary search over koff.
+    // its index over its entries first; beyond, each k-mer also writes the
+    // k-mer table position of its entries into eaux (free until the dedup),
+    // or the entries themselves for a bucket held in its slot, and the
+    // gather is one load per entry (a 64-bit tabIndex: a binary search over
+    // koff instead)
+    if constexpr (sizeof(TabT) == 4) {
+        if (E > 256) {
+            for (uint32_t k = lane; k < K; k += 64) {
+                const uint32_t o = w.koff[k], n = w.klen[k];
+                const uint32_t lo = (uint32_t)w.klo[k];
+                if (inl && n <= 2) {
+                    for (uint32_t i = 0; i < n; ++i) {
+                        w.ekm[o + i] = (EkmT)k;
+                        w.ent[o + i] = i == 0 ? lo : w.k2[k];
+                        w.eaux[o + i] = ~0u;
+                    }
+                } else {
+                    for (uint32_t i = 0; i < n; ++i) {
+                        w.ekm[o + i] = (EkmT)k;
+                        w.eaux[o + i] = lo + i;
+                    }
+                }
+            }
+            wave_sync();
+            for (uint32_t j = lane; j < E; j += 64) {
+                const uint32_t at = w.eaux[j];
+                if (at != ~0u) w.ent[j] = ix.kent[at] & 0x7FFFFFFFu; // without isStart
+            }
+            wave_sync();
+            return seed_tail<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L);
+        }
+    }
     const bool scatter = E <= 256;
     if (scatter) {
         for (uint32_t k = lane; k < K; k += 64) {
