@@ -333,9 +333,6 @@ struct SeedShared {
     // rot[i][c] = rol(seed_c, 31 - i): the term of position i of a spaced
     // k-mer (nthash.hpp:480-489); code 3 (N and others) contributes 0
     uint64_t rot[KMERLEN][4];
-    // rq[t][c] = rol(seed_c, -t mod 64): the term of letter t of a read in
-    // the prefix form of the hash (seed_prefix_key)
-    uint64_t rq[64][4];
 };
 
 // filled by the whole block at kernel start
@@ -344,10 +341,6 @@ __device__ __forceinline__ void seed_table_init(SeedShared& sh)
     for (int t = threadIdx.x; t < (int)KMERLEN * 4; t += blockDim.x) {
         const int i = t >> 2, c = t & 3;
         sh.rot[i][c] = rol64(red_seed((uint32_t)c), (int)(KMERLEN - 1 - i));
-    }
-    for (int t = threadIdx.x; t < 64 * 4; t += blockDim.x) {
-        const int i = t >> 2, c = t & 3;
-        sh.rq[i][c] = rol64(red_seed((uint32_t)c), (64 - i) & 63);
     }
     __syncthreads();
 }

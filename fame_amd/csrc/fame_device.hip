@@ -706,8 +706,22 @@ __device__ __forceinline__ GoAnchors go_anchors(uint32_t K)
 
 // keys of the k-mers of the row cw (lane l holds letters 4l .. 4l+3) through
 // the prefix array pf (this wave's LDS, L + 1 entries), anchor loads issued
+// rq[t][c] = rol(seed_c, -t mod 64): the term of letter t of a read in the
+// prefix form of the hash (filled by the block at kernel start)
+struct GoShared {
+    uint64_t rq[64][4];
+};
+__device__ __forceinline__ void go_table_init(GoShared& g)
+{
+    for (int t = threadIdx.x; t < 64 * 4; t += blockDim.x) {
+        const int i = t >> 2, c = t & 3;
+        g.rq[i][c] = rol64(red_seed((uint32_t)c), (64 - i) & 63);
+    }
+    __syncthreads();
+}
+
 template <int KPL>
-__device__ __forceinline__ void go_hash(const DevIndex& ix, const SeedShared& sh, uint64_t* pf, uint32_t cw,
+__device__ __forceinline__ void go_hash(const DevIndex& ix, const GoShared& sh, uint64_t* pf, uint32_t cw,
                                         uint32_t L, GoHashed<KPL>& h, uint32_t& rnd)
 {
     const int lane = threadIdx.x & 63;
@@ -993,6 +1007,8 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         // genome-ordered loads, qC + 3nw its anchor loads, and the row of
         // qC + 4nw loads (the first three iterations only fill the stages)
         uint64_t* pf = (uint64_t*)&m.u;
+        __shared__ GoShared gsh;
+        go_table_init(gsh);
         auto row = [&](uint32_t q) {
             return ((const uint32_t*)(bd.codes + (size_t)(q < nq ? ro_of(q) : 0u) * 256))[lane];
         };
@@ -1009,7 +1025,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             GoHashed<C::KPL> nH;
             GoPredicted<C::KPL> nP;
             GoSlots<C::KPL> nF;
-            go_hash<C::KPL>(ix, sh, pf, cwH, qH < nq ? (lane_u32(cwH, 63) >> 24) : 0u, nH, go_rnd);
+            go_hash<C::KPL>(ix, gsh, pf, cwH, qH < nq ? (lane_u32(cwH, 63) >> 24) : 0u, nH, go_rnd);
             go_anchor_load<C::KPL>(ix, nH, go_rnd);
             go_predict<C::KPL>(ix, sH, nP);
             go_fetch<C::KPL>(ix, sP, nF, go_rnd, go_ord);
