@@ -210,6 +210,7 @@ struct SeedWork {
     IdxT* over;        // overflow list
     uint32_t* cand;    // [2][cap_ins] candidate windows per map
     uint8_t* code;     // [L] reduced base codes of the read
+    uint64_t* srt = nullptr; // [hsize] the streaming counter's sort buffer (seed_tail_stream)
 };
 
 struct SeedStats {
@@ -282,110 +283,18 @@ __device__ __forceinline__ bool seed_prefix(const SeedWork<TabT, IdxT, EkmT>& w,
     return true;
 }
 
-// the rest of getSeedRefs for one (read, orientation) once every index hit
-// is in w.ent / w.ekm (E of them, in k-mer order): consecutive dedup, counts
-// and first occurrences, the two maps' insertion sequences, hopscotch order
-// where it matters, and the candidate windows into the verify pools.
-// Returns 1 when a capacity of the caller's arrays is exceeded (heavy path).
+// the maps of one (read, orientation) once their insertion sequences are in
+// w.ins_key / w.ins_cnt (nins keys each; nq of them at or above QTHRESH, the
+// first such in w.cand when nq == 1): hopscotch order where a map holds two or
+// more such windows, and the candidates into the verify pools (returns 1 when
+// a capacity of the caller's arrays is exceeded)
 template <typename TabT, typename IdxT, typename EkmT>
-__device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd, uint32_t ro,
-                                          const SeedWork<TabT, IdxT, EkmT>& w,
-                         PoolChunk& pc, bool exact, uint64_t E, uint32_t bucketCount, uint32_t L)
+__device__ __forceinline__ int seed_maps(const DevIndex& ix, const BatchDev& bd, uint32_t ro,
+                                         const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact,
+                                         const uint32_t (&nins)[2], const uint32_t (&nq)[2], uint32_t bucketCount,
+                                         uint32_t L)
 {
     const int lane = threadIdx.x & 63;
-    // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
-    for (uint32_t j = lane; j < E; j += 64) {
-        const bool skip = j > 0 && w.ekm[j] == w.ekm[j - 1] && w.ent[j] == w.ent[j - 1];
-        w.eaux[j] = skip ? 0u : 1u;
-    }
-    wave_sync();
-    // first occurrence and total count (u16) of every kept key: an open
-    // addressing table keyed by the packed entry (a power of two of at least
-    // twice the entries, at most hsize >= cap_e: every probe sequence ends;
-    // large scratch tables are only cleared as far as the read needs)
-    {
-        uint32_t hs = 64;
-        while (hs < 2 * (uint32_t)E && hs < w.hsize) hs <<= 1;
-        if (hs > w.hsize) hs = w.hsize;
-        const uint32_t hm = hs - 1;
-        const int hshift = __clz(hs) + 1;
-        for (uint32_t i = lane; i < hs; i += 64) {
-            w.hkey[i] = EMPTY;
-            w.hfirst[i] = 0xFFFFFFFFu;
-            w.hcnt[i] = 0;
-        }
-        wave_sync();
-        for (uint32_t j = lane; j < E; j += 64) {
-            if (!(w.eaux[j] & 1)) continue;
-            const uint32_t key = w.ent[j];
-            uint32_t slot = (key * 2654435761u) >> hshift;
-            for (;;) {
-                const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
-                if (old == EMPTY || old == key) break;
-                slot = (slot + 1) & hm;
-            }
-            atomicMin(&w.hfirst[slot], j);
-            atomicAdd(&w.hcnt[slot], 1u);
-            w.eaux[j] = 1u | (slot << 8);
-        }
-        wave_sync();
-        for (uint32_t j = lane; j < E; j += 64) {
-            const uint32_t a = w.eaux[j];
-            if (!(a & 1)) continue;
-            const uint32_t slot = a >> 8;
-            const bool first = w.hfirst[slot] == j;
-            w.eaux[j] = 1u | (first ? 2u : 0u) | ((w.hcnt[slot] & 0xFFFFu) << 16);
-        }
-    }
-    wave_sync();
-    // insertion sequences: a key enters its map at its first occurrence if
-    // that occurrence may insert (k-mer 0, or k-mer cIdx+1 with cIdx < maxQPos)
-    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
-    uint32_t nins[2] = {0, 0}, nq[2] = {0, 0};
-    for (uint32_t j0 = 0; j0 < E; j0 += 64) {
-        const uint32_t j = j0 + lane;
-        bool take0 = false, take1 = false;
-        uint32_t key = 0, cnt = 0;
-        if (j < E) {
-            const uint32_t a = w.eaux[j];
-            const uint32_t km = w.ekm[j];
-            const bool ins = km == 0 || (km - 1) < maxQPos;
-            if ((a & 2) && ins) {
-                const uint32_t e = w.ent[j];
-                key = e >> 1;
-                cnt = a >> 16;
-                take0 = (e & 1) != 0; // forward-strand windows -> fwdMetaIDs
-                take1 = !take0;
-            }
-        }
-        const uint64_t b0 = __ballot(take0), b1 = __ballot(take1);
-        const uint64_t below = lanes_below(lane);
-        if (take0) {
-            const uint32_t p = nins[0] + __popcll(b0 & below);
-            if (p < w.cap_ins) {
-                w.ins_key[p] = key;
-                w.ins_cnt[p] = (uint16_t)cnt;
-            }
-        }
-        if (take1) {
-            const uint32_t p = nins[1] + __popcll(b1 & below);
-            if (p < w.cap_ins) {
-                w.ins_key[w.cap_ins + p] = key;
-                w.ins_cnt[w.cap_ins + p] = (uint16_t)cnt;
-            }
-        }
-        nins[0] += __popcll(b0);
-        nins[1] += __popcll(b1);
-        // windows at or above QTHRESH; the first one of each map is kept
-        // directly, since a map holding at most one such window yields it
-        // whatever the iteration order
-        const bool q0 = take0 && cnt >= QTHRESH, q1 = take1 && cnt >= QTHRESH;
-        const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
-        if (q0 && nq[0] == 0 && (c0 & below) == 0) w.cand[0] = key;
-        if (q1 && nq[1] == 0 && (c1 & below) == 0) w.cand[w.cap_ins] = key;
-        nq[0] += __popcll(c0);
-        nq[1] += __popcll(c1);
-    }
     if ((nins[0] > w.cap_ins && nq[0] > 1) || (nins[1] > w.cap_ins && nq[1] > 1)) return 1;
     wave_sync();
     // maps with two or more windows at or above QTHRESH: hopscotch placement,
@@ -455,11 +364,300 @@ __device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd,
     return 0;
 }
 
+// getSeedRefs' counting without per-hit arrays (the LDS tier): the hits are
+// visited in index order, 64 per round and four rounds' k-mer table loads in
+// flight, each lane finding its hit's k-mer by walking the bucket offsets from
+// the k-mer where the round starts.  A hit is dropped when it repeats the
+// previous hit of the same bucket (ReadQueue.cpp:2381-2391, 2412-2422); a kept
+// hit counts for its window in an LDS table keyed by the packed entry, whose
+// first occurrence is the least (hit << 8 | k-mer) seen.  The windows whose
+// first occurrence may insert (k-mer 0, or k-mer cIdx+1 with cIdx < maxQPos)
+// enter their map in first-occurrence order: the table's windows are sorted by
+// it (bitonic, in LDS).  Returns 1 when the distinct windows outgrow the table
+// or the maps outgrow their arrays (the heavy tier then takes the pair).
+template <typename TabT, typename IdxT, typename EkmT>
+__device__ __forceinline__ int seed_tail_stream(const DevIndex& ix, const BatchDev& bd, uint32_t ro,
+                                                const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact,
+                                                uint64_t E64, uint32_t bucketCount, uint32_t L, uint32_t K, bool inl)
+{
+    const int lane = threadIdx.x & 63;
+    if (E64 >= (1ull << 24)) return 1; // hit indexes are packed in 24 bits
+    const uint32_t E = (uint32_t)E64;
+    const uint32_t hs = w.hsize, hm = hs - 1;
+    const int hshift = __clz(hs) + 1;
+    for (uint32_t i = lane; i < hs; i += 64) {
+        w.hkey[i] = EMPTY;
+        w.hfirst[i] = 0xFFFFFFFFu;
+        w.hcnt[i] = 0;
+    }
+    wave_sync();
+    constexpr int U = 4;
+    uint32_t kb = 0;                  // k-mer of the round's first hit (no later than any of its hits)
+    uint32_t last_v = 0, last_k = ~0u; // the previous round's last hit
+    bool full = false;
+    for (uint32_t t0 = 0; t0 < E && !full; t0 += 64 * U) {
+        uint32_t kk[U], vv[U], inl_v[U];
+        uint64_t at[U];
+        bool isinl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = t0 + 64u * u + (uint32_t)lane;
+            uint32_t k = kb;
+            at[u] = 0;
+            isinl[u] = false;
+            inl_v[u] = 0;
+            if (j < E) {
+                while (w.koff[k] + w.klen[k] <= j) ++k; // also steps over empty buckets
+                const uint32_t d = j - w.koff[k];
+                if (inl && w.klen[k] <= 2) {
+                    isinl[u] = true;
+                    inl_v[u] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
+                } else {
+                    at[u] = (uint64_t)w.klo[k] + d;
+                }
+            }
+            kk[u] = k;
+            // the next round starts at the k-mer of this round's last hit
+            const uint32_t jl = t0 + 64u * u + 63u;
+            kb = lane_u32(k, jl < E ? 63 : (int)((E - 1) & 63));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) vv[u] = ix.kent[at[u]]; // always issued (entry 0 past the hits)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = t0 + 64u * u + (uint32_t)lane;
+            const bool live = j < E;
+            const uint32_t v = isinl[u] ? inl_v[u] : (vv[u] & 0x7FFFFFFFu); // without isStart
+            const uint32_t k = kk[u];
+            uint32_t pv = (uint32_t)wave_shr1_i32((int)v), pk = (uint32_t)wave_shr1_i32((int)k);
+            if (lane == 0) {
+                pv = last_v;
+                pk = last_k;
+            }
+            last_v = lane_u32(v, 63);
+            last_k = lane_u32(k, 63);
+            const bool kept = live && !(j > 0 && pk == k && pv == v);
+            if (kept && !full) {
+                uint32_t slot = (v * 2654435761u) >> hshift;
+                for (uint32_t probes = 0;; ++probes) {
+                    if (probes == hs) {
+                        full = true;
+                        break;
+                    }
+                    const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, v);
+                    if (old == EMPTY || old == v) break;
+                    slot = (slot + 1) & hm;
+                }
+                if (!full) {
+                    atomicMin(&w.hfirst[slot], (j << 8) | k);
+                    atomicAdd(&w.hcnt[slot], 1u);
+                }
+            }
+        }
+        full = __ballot(full) != 0;
+    }
+    if (full) return 1;
+    wave_sync();
+    // windows whose first occurrence may insert, as (first hit << 32 | slot)
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    uint32_t nd = 0;
+    for (uint32_t i0 = 0; i0 < hs; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        bool take = false;
+        uint32_t fj = 0;
+        if (i < hs && w.hkey[i] != EMPTY) {
+            fj = w.hfirst[i];
+            const uint32_t km = fj & 0xFFu;
+            take = km == 0 || (km - 1) < maxQPos;
+        }
+        const uint64_t b = __ballot(take);
+        if (take) w.srt[nd + __popcll(b & lanes_below(lane))] = ((uint64_t)(fj >> 8) << 32) | i;
+        nd += __popcll(b);
+    }
+    uint32_t n2 = 64;
+    while (n2 < nd) n2 <<= 1;
+    for (uint32_t i = nd + lane; i < n2; i += 64) w.srt[i] = ~0ull;
+    wave_sync();
+    // bitonic sort of n2 (a power of two, at most hs) by first hit
+    for (uint32_t size = 2; size <= n2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t p = lane; p < n2 / 2; p += 64) {
+                const uint32_t lo = 2 * p - (p & (stride - 1)), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = w.srt[lo], c = w.srt[hi];
+                if ((a > c) == up) {
+                    w.srt[lo] = c;
+                    w.srt[hi] = a;
+                }
+            }
+            wave_sync();
+        }
+    }
+    // the two maps' insertion sequences in first-occurrence order
+    uint32_t nins[2] = {0, 0}, nq[2] = {0, 0};
+    for (uint32_t j0 = 0; j0 < nd; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        bool take0 = false, take1 = false;
+        uint32_t key = 0, cnt = 0;
+        if (j < nd) {
+            const uint32_t slot = (uint32_t)w.srt[j];
+            const uint32_t e = w.hkey[slot];
+            key = e >> 1;
+            cnt = w.hcnt[slot] & 0xFFFFu;
+            take0 = (e & 1) != 0; // forward-strand windows -> fwdMetaIDs
+            take1 = !take0;
+        }
+        const uint64_t b0 = __ballot(take0), b1 = __ballot(take1);
+        const uint64_t below = lanes_below(lane);
+        if (take0) {
+            const uint32_t p = nins[0] + __popcll(b0 & below);
+            if (p < w.cap_ins) {
+                w.ins_key[p] = key;
+                w.ins_cnt[p] = (uint16_t)cnt;
+            }
+        }
+        if (take1) {
+            const uint32_t p = nins[1] + __popcll(b1 & below);
+            if (p < w.cap_ins) {
+                w.ins_key[w.cap_ins + p] = key;
+                w.ins_cnt[w.cap_ins + p] = (uint16_t)cnt;
+            }
+        }
+        nins[0] += __popcll(b0);
+        nins[1] += __popcll(b1);
+        const bool q0 = take0 && cnt >= QTHRESH, q1 = take1 && cnt >= QTHRESH;
+        const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
+        if (q0 && nq[0] == 0 && (c0 & below) == 0) w.cand[0] = key;
+        if (q1 && nq[1] == 0 && (c1 & below) == 0) w.cand[w.cap_ins] = key;
+        nq[0] += __popcll(c0);
+        nq[1] += __popcll(c1);
+    }
+    wave_sync();
+    return seed_maps<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, nins, nq, bucketCount, L);
+}
+
+// the rest of getSeedRefs for one (read, orientation) once every index hit
+// is in w.ent / w.ekm (E of them, in k-mer order): consecutive dedup, counts
+// and first occurrences, the two maps' insertion sequences, hopscotch order
+// where it matters, and the candidate windows into the verify pools.
+// Returns 1 when a capacity of the caller's arrays is exceeded (heavy path).
+template <typename TabT, typename IdxT, typename EkmT>
+__device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd, uint32_t ro,
+                                          const SeedWork<TabT, IdxT, EkmT>& w,
+                         PoolChunk& pc, bool exact, uint64_t E, uint32_t bucketCount, uint32_t L)
+{
+    const int lane = threadIdx.x & 63;
+    // consecutive dedup inside a bucket (ReadQueue.cpp:2381-2391, 2412-2422)
+    for (uint32_t j = lane; j < E; j += 64) {
+        const bool skip = j > 0 && w.ekm[j] == w.ekm[j - 1] && w.ent[j] == w.ent[j - 1];
+        w.eaux[j] = skip ? 0u : 1u;
+    }
+    wave_sync();
+    // first occurrence and total count (u16) of every kept key: an open
+    // addressing table keyed by the packed entry (a power of two of at least
+    // twice the entries, at most hsize; large scratch tables are only cleared
+    // as far as the read needs).  With hsize >= 2 cap_e every probe sequence
+    // ends at a free slot; a smaller table (the LDS tier, sized for the
+    // distinct windows of a read) gives up after hsize probes and the read
+    // goes to the next tier
+    {
+        uint32_t hs = 64;
+        while (hs < 2 * (uint32_t)E && hs < w.hsize) hs <<= 1;
+        if (hs > w.hsize) hs = w.hsize;
+        const uint32_t hm = hs - 1;
+        const int hshift = __clz(hs) + 1;
+        for (uint32_t i = lane; i < hs; i += 64) {
+            w.hkey[i] = EMPTY;
+            w.hfirst[i] = 0xFFFFFFFFu;
+            w.hcnt[i] = 0;
+        }
+        wave_sync();
+        bool full = false;
+        for (uint32_t j = lane; j < E && !full; j += 64) {
+            if (!(w.eaux[j] & 1)) continue;
+            const uint32_t key = w.ent[j];
+            uint32_t slot = (key * 2654435761u) >> hshift;
+            for (uint32_t probes = 0;; ++probes) {
+                if (probes == hs) {
+                    full = true;
+                    break;
+                }
+                const uint32_t old = atomicCAS(&w.hkey[slot], EMPTY, key);
+                if (old == EMPTY || old == key) break;
+                slot = (slot + 1) & hm;
+            }
+            if (full) break;
+            atomicMin(&w.hfirst[slot], j);
+            atomicAdd(&w.hcnt[slot], 1u);
+            w.eaux[j] = 1u | (slot << 8);
+        }
+        if (__ballot(full)) return 1;
+        wave_sync();
+        for (uint32_t j = lane; j < E; j += 64) {
+            const uint32_t a = w.eaux[j];
+            if (!(a & 1)) continue;
+            const uint32_t slot = a >> 8;
+            const bool first = w.hfirst[slot] == j;
+            w.eaux[j] = 1u | (first ? 2u : 0u) | ((w.hcnt[slot] & 0xFFFFu) << 16);
+        }
+    }
+    wave_sync();
+    // insertion sequences: a key enters its map at its first occurrence if
+    // that occurrence may insert (k-mer 0, or k-mer cIdx+1 with cIdx < maxQPos)
+    const uint32_t maxQPos = L - KMERLEN + 1 - QTHRESH;
+    uint32_t nins[2] = {0, 0}, nq[2] = {0, 0};
+    for (uint32_t j0 = 0; j0 < E; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        bool take0 = false, take1 = false;
+        uint32_t key = 0, cnt = 0;
+        if (j < E) {
+            const uint32_t a = w.eaux[j];
+            const uint32_t km = w.ekm[j];
+            const bool ins = km == 0 || (km - 1) < maxQPos;
+            if ((a & 2) && ins) {
+                const uint32_t e = w.ent[j];
+                key = e >> 1;
+                cnt = a >> 16;
+                take0 = (e & 1) != 0; // forward-strand windows -> fwdMetaIDs
+                take1 = !take0;
+            }
+        }
+        const uint64_t b0 = __ballot(take0), b1 = __ballot(take1);
+        const uint64_t below = lanes_below(lane);
+        if (take0) {
+            const uint32_t p = nins[0] + __popcll(b0 & below);
+            if (p < w.cap_ins) {
+                w.ins_key[p] = key;
+                w.ins_cnt[p] = (uint16_t)cnt;
+            }
+        }
+        if (take1) {
+            const uint32_t p = nins[1] + __popcll(b1 & below);
+            if (p < w.cap_ins) {
+                w.ins_key[w.cap_ins + p] = key;
+                w.ins_cnt[w.cap_ins + p] = (uint16_t)cnt;
+            }
+        }
+        nins[0] += __popcll(b0);
+        nins[1] += __popcll(b1);
+        // windows at or above QTHRESH; the first one of each map is kept
+        // directly, since a map holding at most one such window yields it
+        // whatever the iteration order
+        const bool q0 = take0 && cnt >= QTHRESH, q1 = take1 && cnt >= QTHRESH;
+        const uint64_t c0 = __ballot(q0), c1 = __ballot(q1);
+        if (q0 && nq[0] == 0 && (c0 & below) == 0) w.cand[0] = key;
+        if (q1 && nq[1] == 0 && (c1 & below) == 0) w.cand[w.cap_ins] = key;
+        nq[0] += __popcll(c0);
+        nq[1] += __popcll(c1);
+    }
+    return seed_maps<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, nins, nq, bucketCount, L);
+}
+
 // KPL > 0: the read's row of reduced letters arrives as this lane's word cw
 // (a 256-byte row, the length in its byte 255); KPL == 0 (long reads): the
 // row is already in w.code, L letters long, and the k-mers are hashed and
 // looked up 64 at a time (no register arrays)
-template <typename TabT, typename IdxT, int KPL, typename EkmT>
+template <typename TabT, typename IdxT, int KPL, typename EkmT, bool STREAM = false>
 __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
@@ -545,6 +743,7 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
     *E_out = E;
     *bc_out = bucketCount;
     if (!fits) return 1;
+    if constexpr (STREAM) return seed_tail_stream<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L, K, inl);
     // entry gather, one entry per lane: entry j belongs to the k-mer whose
     // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
     // its index over its entries first; beyond, each k-mer also writes the
@@ -989,7 +1188,7 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
             const uint32_t h = atomicAdd(bd.heavy_n, 1u);
             if (h < bd.heavy_cap) {
                 bd.heavy_ro[h] = ro;
-                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_e[h] = E > 0x7FFFFFFFull ? 0x7FFFFFFFu : (uint32_t)E; // bit 31: handed on by the LDS tier
                 bd.heavy_bc[h] = bc;
             } else {
                 atomicOr(bd.flags, FLAG_HEAVY_FULL);
@@ -1188,7 +1387,9 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
     const bool tiered = hs.count == FAME_JOBS_ON_DEVICE;
     for (uint32_t hi = blockIdx.x; hi < count; hi += gridDim.x) {
         if (tiered) {
-            const uint32_t e = bd.heavy_e[hi];
+            uint32_t e = bd.heavy_e[hi];
+            if (hs.handed && !(e >> 31)) continue; // the LDS tier counted it
+            e &= 0x7FFFFFFFu;
             if (e < hs.e_lo || e > hs.e_hi) continue;
         }
         const uint32_t hro = bd.heavy_ro[hs.first + hi];
@@ -1237,6 +1438,112 @@ __global__ __launch_bounds__(64) void k_seed_heavy(DevIndex ix, BatchDev bd, Hea
 // window, so it is not seeded: k_select finds no candidate and counts it
 // unmatched, as FAME's ShiftAnd finds nothing.  A read over
 // FAME_LONG_MAX_READ that could match raises FLAG_TOO_LONG.
+// The LDS tier between the light kernel (at most 256 index hits) and the
+// heavy kernel (global scratch): (read, orientation) pairs of reads of at
+// most FAME_DEV_MAX_READ letters, one wave per pair, every array in LDS and
+// none per index hit (seed_tail_stream).  The counting table, the maps' key
+// lists and the hopscotch arrays are sized for the distinct windows a read
+// names (MID_H / MID_INS), not for its hits: repeat reads name each of a few
+// dozen windows hundreds of times.  A pair whose windows do not fit goes to
+// the heavy kernel: plain form, listed in ovl (ovl[0] = count, then positions
+// in the heavy list); device form, bit 31 of its hit count set, which the
+// heavy tiers that follow on the stream select on.
+constexpr uint32_t MID_E = 0xFFFFFFu; // hits (24-bit hit indexes)
+constexpr uint32_t MID_H = 512;
+constexpr uint32_t MID_INS = 256;
+constexpr uint32_t MID_TB = 1152; // two hopscotch bucket arrays' worth for MID_INS keys
+
+struct SeedMidLds {
+    uint32_t klo[FAME_DEV_MAX_READ + 1];
+    uint32_t klen[FAME_DEV_MAX_READ + 1];
+    uint32_t koff[FAME_DEV_MAX_READ + 1];
+    uint32_t k2[FAME_DEV_MAX_READ + 1];
+    // the counting table, dead once the maps' key lists are written; then
+    // the hopscotch arrays
+    union {
+        struct {
+            uint32_t key[MID_H];
+            uint32_t first[MID_H];
+            uint32_t cnt[MID_H];
+        } h;
+        struct {
+            uint16_t tb[MID_TB];
+            uint16_t tmp[MID_INS];
+            uint16_t over[MID_INS];
+        } b;
+    } u;
+    uint64_t srt[MID_H];
+    uint32_t ins_key[2 * MID_INS];
+    uint16_t ins_cnt[2 * MID_INS];
+    uint32_t cand[2 * MID_INS];
+    alignas(4) uint8_t code[256];
+};
+
+__global__ __launch_bounds__(64) void k_seed_mid(DevIndex ix, BatchDev bd, uint32_t first, uint32_t count,
+                                                 uint32_t* __restrict__ ovl, uint32_t hsize)
+{
+    __shared__ SeedShared sh;
+    __shared__ SeedMidLds m;
+    const int lane = threadIdx.x;
+    const bool dev = count == FAME_JOBS_ON_DEVICE;
+    if (dev) {
+        if (*bd.flags & FLAG_ABORT) return;
+        count = min(*bd.heavy_n, bd.heavy_cap);
+    }
+    SeedWork<uint32_t, uint16_t> w;
+    w.klo = m.klo;
+    w.klen = m.klen;
+    w.koff = m.koff;
+    w.k2 = m.k2;
+    w.ent = nullptr; // no per-hit arrays
+    w.ekm = nullptr;
+    w.eaux = nullptr;
+    w.cap_e = MID_E;
+    w.hsize = hsize; // a power of two, 64 .. MID_H
+    w.hkey = m.u.h.key;
+    w.hfirst = m.u.h.first;
+    w.hcnt = m.u.h.cnt;
+    w.srt = m.srt;
+    w.ins_key = m.ins_key;
+    w.ins_cnt = m.ins_cnt;
+    w.cap_ins = MID_INS;
+    w.tb = m.u.b.tb;
+    w.tb_cap = MID_TB;
+    w.tmp = m.u.b.tmp;
+    w.over = m.u.b.over;
+    w.cand = m.cand;
+    w.code = m.code;
+    seed_table_init(sh);
+    SeedStats ss{};
+    for (uint32_t hi = blockIdx.x; hi < count; hi += gridDim.x) {
+        const uint32_t h = first + hi;
+        if (dev && (bd.heavy_e[h] >> 31)) continue; // already handed on
+        const uint32_t hro = bd.heavy_ro[h];
+        int rc = 1;
+        if (!(hro >> 31)) { // long items (own rows) always go to the heavy kernel
+            const uint32_t cw = ((const uint32_t*)(bd.codes + (size_t)hro * 256))[lane];
+            uint64_t E;
+            uint32_t bc;
+            PoolChunk pc = {0, 0, 0};
+            wave_sync();
+            rc = seed_one<uint32_t, uint16_t, SeedCap<256>::KPL, uint8_t, true>(ix, bd, hro, sh, w, pc, true, ss, &E,
+                                                                                &bc, cw);
+        }
+        if (rc != 0 && lane == 0) {
+            if (dev) bd.heavy_e[h] |= 0x80000000u;
+            else ovl[1 + atomicAdd(ovl, 1u)] = h;
+        }
+    }
+    if (lane == 0 && ss.seeded) {
+        atomicAdd(&bd.stats[0], (unsigned long long)ss.kmers);
+        atomicAdd(&bd.stats[1], (unsigned long long)ss.entries);
+        atomicAdd(&bd.stats[2], (unsigned long long)ss.seeded);
+        atomicAdd(&bd.stats[6], (unsigned long long)ss.bytes);
+        atomicAdd(&bd.stats[8], (unsigned long long)ss.buckets);
+        atomicAdd(&bd.stats[11], (unsigned long long)ss.kmers); // random slot lookups
+    }
+}
+
 template <typename TabT>
 __global__ __launch_bounds__(64) void k_long_measure(DevIndex ix, BatchDev bd)
 {
@@ -1283,7 +1590,7 @@ __global__ __launch_bounds__(64) void k_long_measure(DevIndex ix, BatchDev bd)
             const uint32_t h = atomicAdd(bd.heavy_n, 1u);
             if (h < bd.heavy_cap) {
                 bd.heavy_ro[h] = 0x80000000u | q;
-                bd.heavy_e[h] = E > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)E;
+                bd.heavy_e[h] = E > 0x7FFFFFFFull ? 0x7FFFFFFFu : (uint32_t)E; // bit 31: handed on by the LDS tier
                 bd.heavy_bc[h] = bsum / 5;
             } else {
                 atomicOr(bd.flags, FLAG_HEAVY_FULL);
@@ -3245,6 +3552,36 @@ uint32_t seed_heavy_blocks(bool tab64)
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 64, 0) != hipSuccess || nb <= 0)
         return 4096;
+    return (uint32_t)nb * (uint32_t)p.multiProcessorCount;
+}
+
+// the LDS tier: plain form over heavy-list positions [first, first + count)
+// (overflow list ovl), device form (count = FAME_JOBS_ON_DEVICE) over the
+// device list; only for the 32-bit tabIndex
+void launch_seed_mid(const DevIndex& ix, const BatchDev& bd, uint32_t first, uint32_t count, uint32_t* ovl,
+                     uint32_t blocks, hipStream_t st)
+{
+    if (!count || !blocks || ix.tab_is64) return;
+    // FAME_MID_SLOTS (tests): a smaller counting table, so that reads overflow
+    // into the heavy tier
+    uint32_t hsize = MID_H;
+    if (const char* e = getenv("FAME_MID_SLOTS")) {
+        const uint32_t v = (uint32_t)atoi(e);
+        if (v >= 64 && v <= MID_H && !(v & (v - 1))) hsize = v;
+    }
+    hipLaunchKernelGGL(k_seed_mid, dim3(blocks), dim3(64), 0, st, ix, bd, first, count, ovl, hsize);
+}
+
+uint32_t seed_mid_max_e() { return MID_E; }
+
+// blocks of k_seed_mid resident at once on the current device
+uint32_t seed_mid_blocks()
+{
+    int nb = 0, dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_seed_mid, 64, 0) != hipSuccess || nb <= 0)
+        return 1024;
     return (uint32_t)nb * (uint32_t)p.multiProcessorCount;
 }
 

@@ -126,7 +126,7 @@ struct fame_ctx {
     DevBuf b_job_meta[FAME_NPOOL], b_job_ro[FAME_NPOOL], b_job_nm[FAME_NPOOL], b_job_match[FAME_NPOOL],
         b_job_ovf_off[FAME_NPOOL];
     DevBuf b_ovf_n, b_ovf_pool, b_ovf_job, b_ovf_buf, b_ovf_total;
-    DevBuf b_heavy_n, b_heavy_ro, b_heavy_e, b_heavy_bc, b_heavy_scratch;
+    DevBuf b_heavy_n, b_heavy_ro, b_heavy_e, b_heavy_bc, b_heavy_scratch, b_mid_ovl;
     DevBuf b_status, b_use, b_mat, b_valid, b_dp, b_dec, b_trace, b_direct, b_dplist, b_methn;
     uint32_t pool_cap[FAME_NPOOL] = {0, 0, 0, 0};
     uint32_t match_cap = 4;
@@ -516,6 +516,13 @@ uint32_t verify_queue_limit()
 
 // the heavy scratch (which the context already holds) can grow to `bytes`
 // with 1 GiB of device memory left over
+// FAME_NO_MID_TIER=1 (tests): every deferred pair on global scratch
+static bool no_mid_tier()
+{
+    const char* e = getenv("FAME_NO_MID_TIER");
+    return e && e[0] == '1';
+}
+
 bool scratch_fits(fame_ctx* c, size_t bytes)
 {
     size_t fr = 0, tot = 0;
@@ -641,12 +648,46 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             std::vector<uint32_t> ord(heavy_n);
             for (uint32_t i = 0; i < heavy_n; ++i) ord[i] = i;
             std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return he[a] < he[b]; });
+            // the LDS tier first: pairs of up to seed_mid_max_e() index hits
+            // (reads of at most FAME_DEV_MAX_READ letters); those whose
+            // windows do not fit it come back in its overflow list and join
+            // the others in the global-scratch classes below
+            const uint32_t mid_e = c->ix.tab_is64 || no_mid_tier() ? 0u : seed_mid_max_e();
+            std::stable_partition(ord.begin(), ord.end(),
+                                  [&](uint32_t i) { return he[i] <= mid_e && !(hr[i] >> 31); });
+            uint32_t nmid = 0;
+            while (nmid < heavy_n && he[ord[nmid]] <= mid_e && !(hr[ord[nmid]] >> 31)) ++nmid;
             {
                 std::vector<uint32_t> r2(heavy_n);
                 for (uint32_t i = 0; i < heavy_n; ++i) r2[i] = hr[ord[i]];
                 HIPC(hipMemcpyAsync(bd.heavy_ro, r2.data(), 4 * heavy_n, hipMemcpyHostToDevice, st));
                 HIPC(hipStreamSynchronize(st));
             }
+            tm.mark(12);
+            if (nmid) {
+                uint32_t* ovl = grow<uint32_t>(c->b_mid_ovl, nmid + 1);
+                HIPC(hipMemsetAsync(ovl, 0, 4, st));
+                launch_seed_mid(c->ix, bd, 0, nmid, ovl, std::min<uint32_t>(nmid, seed_mid_blocks()), st);
+                uint32_t novl = 0;
+                HIPC(hipMemcpyAsync(&novl, ovl, 4, hipMemcpyDeviceToHost, st));
+                HIPC(hipStreamSynchronize(st));
+                std::vector<uint32_t> back(novl);
+                if (novl) HIPC(hipMemcpy(back.data(), ovl + 1, 4ull * novl, hipMemcpyDeviceToHost));
+                // the global classes: the overflowed pairs and those past the
+                // tier, sorted by hits again
+                std::vector<uint32_t> rest;
+                rest.reserve(novl + heavy_n - nmid);
+                for (uint32_t x : back) rest.push_back(ord[x]);
+                for (uint32_t i = nmid; i < heavy_n; ++i) rest.push_back(ord[i]);
+                std::sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) { return he[a] < he[b]; });
+                ord.swap(rest);
+                std::vector<uint32_t> r3(ord.size());
+                for (size_t i = 0; i < ord.size(); ++i) r3[i] = hr[ord[i]];
+                if (!r3.empty())
+                    HIPC(hipMemcpyAsync(bd.heavy_ro, r3.data(), 4 * r3.size(), hipMemcpyHostToDevice, st));
+                HIPC(hipStreamSynchronize(st));
+            }
+            const uint32_t nglob = (uint32_t)ord.size();
             auto rup = [](uint64_t v) {
                 uint64_t p = 1;
                 while (p < v) p <<= 1;
@@ -661,12 +702,12 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             std::vector<Cls> classes;
             size_t need = 0;
             const uint32_t resident = seed_heavy_blocks(c->ix.tab_is64);
-            for (uint32_t c0 = 0; c0 < heavy_n;) {
+            for (uint32_t c0 = 0; c0 < nglob;) {
                 uint64_t cap = 1024;
                 while (cap < he[ord[c0]]) cap *= 4;
                 uint32_t c1 = c0;
                 uint64_t emax = 1, bcmax = 1;
-                while (c1 < heavy_n && he[ord[c1]] <= cap) {
+                while (c1 < nglob && he[ord[c1]] <= cap) {
                     emax = std::max<uint64_t>(emax, he[ord[c1]]);
                     bcmax = std::max<uint64_t>(bcmax, hb[ord[c1]]);
                     ++c1;
@@ -695,8 +736,7 @@ int run_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
                                " MiB of scratch, more than the device has free");
                 return FAME_ERR_CAPACITY;
             }
-            HIPC(c->b_heavy_scratch.reserve(need));
-            tm.mark(12);
+            if (need) HIPC(c->b_heavy_scratch.reserve(need));
             for (Cls& k : classes) {
                 k.hs.base = c->b_heavy_scratch.as<uint8_t>();
                 k.hs.first = k.c0;
@@ -836,7 +876,8 @@ struct GraphPlan {
     DevBuf ovf_n, ovf_pool, ovf_job, ovf_buf, ovf_total, heavy_n, heavy_ro, heavy_e, heavy_bc, heavy_scratch,
         heavy_scratch2;
     DevBuf status, use, mat, valid, stats, direct, dplist, methn, flags, ndev, dp, dec, trace, counters;
-    uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 1024, heavy_blocks2 = 16;
+    uint32_t pool_cap = 0, ovf_list_cap = 0, ovf_buf_cap = 0, heavy_cap = 0, heavy_blocks = 1024, heavy_blocks2 = 16,
+             mid_blocks = 1024;
     HeavyScratch hs{}, hs2{};
     // captured graphs by input buffers and variant, at most kMaxGraphs: the
     // least recently used one is destroyed for a new one (callers that pass
@@ -964,11 +1005,17 @@ GraphPlan* plan_for(fame_ctx* c, uint32_t n, uint64_t total_bytes, int lane = 0)
         // batches of a million reads or more: the first tier on as many
         // slices as are resident at once (1.25 MB each)
         if (cap >= (1u << 20)) pl->heavy_blocks = std::max<uint32_t>(1024, seed_heavy_blocks(c->ix.tab_is64));
+        // the LDS tier takes the pairs of up to seed_mid_max_e() hits (and
+        // hands those it cannot hold to the first global tier)
+        const bool mid = !c->ix.tab_is64 && !no_mid_tier();
         pl->hs.e_lo = 0;
         pl->hs.e_hi = 16384;
+        pl->hs.handed = mid ? 1u : 0u;
+        if (mid) pl->mid_blocks = seed_mid_blocks();
         dims(pl->hs2, 1u << 20);
         pl->hs2.e_lo = 16385;
         pl->hs2.e_hi = 0xFFFFFFFFu;
+        pl->hs2.handed = pl->hs.handed;
         HIPC(pl->heavy_scratch.reserve(pl->hs.stride * pl->heavy_blocks));
         HIPC(pl->heavy_scratch2.reserve(pl->hs2.stride * pl->heavy_blocks2));
         pl->hs.base = pl->heavy_scratch.as<uint8_t>();
@@ -1090,6 +1137,7 @@ hipGraphExec_t capture_graph(fame_ctx* c, GraphPlan* pl, const GraphKey& key, co
     launch_batch_reset(bd, st);
     launch_prep(bd, cap, c->readlen, st);
     launch_seed_light(c->ix, bd, 2 * cap, st, std::max<uint32_t>(lr[1], 1));
+    if (pl->hs.handed) launch_seed_mid(c->ix, bd, 0, FAME_JOBS_ON_DEVICE, nullptr, pl->mid_blocks, st);
     launch_seed_heavy_dev(c->ix, bd, pl->hs, pl->heavy_blocks, st);
     launch_seed_heavy_dev(c->ix, bd, pl->hs2, pl->heavy_blocks2, st);
     for (int p = 0; p < FAME_NPOOL; ++p) {

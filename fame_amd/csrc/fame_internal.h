@@ -231,6 +231,7 @@ struct HeavyScratch {
     uint32_t cap_e, hsize, tb_cap;
     uint32_t kmax = 0; // k-mers per item at most (0: FAME_DEV_MAX_READ; more for long items)
     uint32_t e_lo, e_hi; // device-list mode: only the items with e_lo <= E <= e_hi
+    uint32_t handed = 0;  // device-list mode: only the items the LDS tier handed on (bit 31 of heavy_e)
 };
 
 // bytes of one block's slice of heavy scratch: the arrays k_seed_heavy takes
@@ -326,6 +327,10 @@ void launch_pack_kmers(const uint32_t* raw, const uint8_t* sbits, uint64_t first
 void launch_narrow_tab(const uint64_t* in, uint64_t n, uint32_t* out, hipStream_t st);
 // nwords packed words (16 letters each) of one chromosome of len letters
 void launch_pack_seq(const uint8_t* seq, uint64_t len, uint64_t nwords, uint64_t* out, hipStream_t st);
+void launch_seed_mid(const DevIndex& ix, const BatchDev& bd, uint32_t first, uint32_t count, uint32_t* ovl,
+                     uint32_t blocks, hipStream_t st);
+uint32_t seed_mid_max_e();
+uint32_t seed_mid_blocks();
 void launch_gorder(const uint8_t* seq, uint64_t ng, uint64_t htab, const uint64_t* slot, uint32_t* gkey,
                    uint64_t* gslot, uint32_t* kpos, hipStream_t st);
 void launch_build_slots(const uint32_t* tab, const uint32_t* kent, uint64_t htab, uint64_t* slot, uint32_t* bad,

@@ -47,6 +47,27 @@ def test_batched_matches_reference(golden_case, tmp_path):
     ctx.close()
 
 
+@pytest.mark.parametrize("env", [{"FAME_NO_MID_TIER": "1"}, {"FAME_MID_SLOTS": "64"}],
+                         ids=["heavy_only", "mid_overflow"])
+def test_batched_heavy_tiers_match_reference(golden_case, monkeypatch, env):
+    """The batched form's in-graph tiers for deferred pairs: the global
+    tiers alone, or the LDS tier with a 64-slot counting table handing reads
+    to the first global tier (test-only switches): FAME's results."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    c = golden_case
+    print(f"case {c.name}: {env}", flush=True)
+    ctx = _lib.Context(c.index, both_strands=c.unord)
+    ctx.set_form("batched")
+    valid, mat, cnt = stream(ctx, c, max(len(c.lens) // 3, 1))
+    if int(c.lens.max()) <= 255:  # longer reads always take the plain form
+        assert ctx.form_info()["fallbacks"] == 0
+    assert np.array_equal(valid, c.valid) and np.array_equal(mat, c.mat)
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()
+
+
 def test_graph_replays_equal_plain(golden_case):
     """The same chunk six times through its graph (captured once per
     autotuner candidate, then replayed), a plain call of another chunk

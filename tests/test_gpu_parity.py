@@ -99,6 +99,26 @@ def test_hip_without_genome_order_matches_reference(golden_case, monkeypatch):
     ctx.close()
 
 
+@pytest.mark.parametrize("env", [{"FAME_NO_MID_TIER": "1"}, {"FAME_MID_SLOTS": "64"}],
+                         ids=["heavy_only", "mid_overflow"])
+def test_hip_heavy_tiers_match_reference(golden_case, monkeypatch, env):
+    """Pairs of more than 256 index hits: every one on global scratch
+    (FAME_NO_MID_TIER=1), or the LDS tier with a 64-slot counting table
+    (FAME_MID_SLOTS=64), so that reads naming more windows than it holds are
+    handed to the heavy kernel (both switches are test-only): FAME's results
+    either way."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    c = golden_case
+    print(f"case {c.name}: {env}", flush=True)
+    ctx, valid, mat, cnt, _ = run_hip(c)
+    bad = np.nonzero((valid != c.valid) | (mat != c.mat))[0]
+    assert bad.size == 0, f"{bad.size} reads differ, first {bad[:10].tolist()}"
+    assert cnt.tolist() == [c.ref["succ"], c.ref["nonunique"], c.ref["unsucc"]]
+    assert np.array_equal(ctx.meth(), c.meth)
+    ctx.close()
+
+
 def test_hip_genome_ordered_slots_are_used(golden_case):
     """With the genome-ordered copy (the default), the k-mers of reads drawn
     from the reference are mostly read from it: every k-mer is read once,
