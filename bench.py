@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs="+", default=None, metavar=("OUT.md", "CSV"),
                     help="only write OUT.md: per-launch rocprofv3 counters of the last dispatch of each "
                          "kernel named by --pmc-kernels, from the counter_collection CSVs given")
-    ap.add_argument("--pmc-kernels", default="k_verify<2>;k_seed_light;k_meth_dp",
+    ap.add_argument("--pmc-kernels", default="k_verify<2>;k_seed_light<unsigned int, 128;k_meth_dp",
                     help="';'-separated parts of kernel names")
     ap.add_argument("--pmc-command", default="", help="the profiled command, for the summaries' header")
     ap.add_argument("--traffic-summary", nargs=6, default=None,
@@ -419,7 +419,8 @@ VALU_ISSUE_PEAK = SIMDS * CLOCK_HZ / 4
 # kernel, its HBM traffic summary, the VALU counter summary (all three kernels)
 VALU_PROFILE = "seed_verify_dp_genome300M_2Mreads_valu_counters.md"
 PROFILES = {
-    "seed": ("k_seed_light", "k_seed_light_genome300M_2Mreads_traffic.md"),
+    # the variant for reads of at most 128 letters (the other returns at once)
+    "seed": ("k_seed_light<unsigned int, 128", "k_seed_light_genome300M_2Mreads_traffic.md"),
     "verify": ("k_verify<2>", "k_verify2_genome300M_2Mreads_traffic.md"),
     "dp": ("k_meth_dp", None),
 }
