@@ -1080,16 +1080,20 @@ __device__ __forceinline__ void go_fetch(const DevIndex& ix, const GoPredicted<K
 // Capacities of the light seed kernel for reads of at most ML letters
 // (ML = 128 or 256): index hits ML, keys per map ML/2, counting slots ML.
 // Larger (read, orientation) pairs go to the heavy kernel.
-template <int ML>
+// (LB: the longest read of the variant, which sets the k-mers per lane;
+// at most ML)
+template <int ML, int LB = ML>
 struct SeedCap {
     static constexpr uint32_t E = ML;
     static constexpr uint32_t INS = ML / 2;
     static constexpr uint32_t TB = ML + NBH - 1;
     static constexpr uint32_t HASH = ML;
-    static constexpr int KPL = (ML - (int)KMERLEN + 63) / 64; // k-mers per lane
+    static constexpr int KPL = (LB - (int)KMERLEN + 63) / 64; // k-mers per lane
     static_assert((HASH & (HASH - 1)) == 0 && HASH >= E, "counting table");
+    static_assert(LB <= ML, "read bound");
 };
 constexpr int SEED_SHORT = 128; // read length bound of the short variant
+constexpr int SEED_GO_MID = 159; // the longest read with at most 128 k-mers (two per lane)
 
 // per-wave LDS of the light seed kernel; the gather / count arrays and the
 // hopscotch arrays share storage
@@ -1135,13 +1139,17 @@ struct SeedLds {
 // in grid-stride order.  Both variants are launched; the one that does not
 // fit the batch's longest read (from k_prep) returns at once.
 // GO: the slots through the genome-ordered copy (ix.gkey set)
-template <typename TabT, int ML, bool GO>
-__global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro)
+// Variants by the batch's longest read to seed (k_prep): the one whose
+// range (len_lo, len_hi] holds it runs, the others return at once.
+template <typename TabT, int ML, bool GO, int LB = ML>
+__global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_seed_light(DevIndex ix, BatchDev bd, uint32_t n_ro, uint32_t len_lo, uint32_t len_hi)
 {
-    using C = SeedCap<ML>;
+    using C = SeedCap<ML, LB>;
     if (bd.n_dev) n_ro = 2 * *bd.n_dev;
-    const bool short_batch = *bd.max_len <= (uint32_t)SEED_SHORT;
-    if ((ML == SEED_SHORT) != short_batch) return;
+    {
+        const uint32_t ml = *bd.max_len;
+        if (ml <= len_lo || ml > len_hi) return;
+    }
     __shared__ SeedShared sh;
     __shared__ SeedLds<TabT, ML> lds[FAME_SEED_WPB];
     const int lane = threadIdx.x & 63;
@@ -3487,28 +3495,24 @@ void launch_seed_light(const DevIndex& ix, const BatchDev& bd, uint32_t n_ro, hi
     // the variant that does not fit the batch returns at once; with a hint of
     // the longest read it gets a small grid (still complete if the hint is
     // wrong: the kernels are persistent)
-    uint32_t gs = g, gl = g;
-    if (lmax_hint) {
-        if (lmax_hint <= (uint32_t)SEED_SHORT) gl = 16;
-        else gs = 16;
-    }
+    const uint32_t S = SEED_SHORT, M = SEED_GO_MID, X = 0xFFFFFFFFu;
+    const dim3 b(64 * FAME_SEED_WPB);
+    // with a hint of the longest read the variants that do not fit it get
+    // small grids
+    auto grid = [&](uint32_t lo, uint32_t hi) { return dim3(lmax_hint && (lmax_hint <= lo || lmax_hint > hi) ? 16 : g); };
     if (ix.tab_is64) {
-        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT, false>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
-                           bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint64_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
-                           n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, SEED_SHORT, false>), grid(0, S), b, 0, st, ix, bd, n_ro, 0u, S);
+        hipLaunchKernelGGL((k_seed_light<uint64_t, 256, false>), grid(S, X), b, 0, st, ix, bd, n_ro, S, X);
     } else if (ix.slot && ix.gkey) {
-        // the genome-ordered pipeline for reads of up to 128 letters (the
-        // longer variant's stage registers would not fit without spills)
-        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, true>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
-                           bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
-                           n_ro);
+        // the genome-ordered pipeline for reads of up to 159 letters (two
+        // k-mers per lane; four would not fit its stage registers without
+        // spills)
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, true>), grid(0, S), b, 0, st, ix, bd, n_ro, 0u, S);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, true, SEED_GO_MID>), grid(S, M), b, 0, st, ix, bd, n_ro, S, M);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), grid(M, X), b, 0, st, ix, bd, n_ro, M, X);
     } else {
-        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, false>), dim3(gs), dim3(64 * FAME_SEED_WPB), 0, st, ix,
-                           bd, n_ro);
-        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), dim3(gl), dim3(64 * FAME_SEED_WPB), 0, st, ix, bd,
-                           n_ro);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, SEED_SHORT, false>), grid(0, S), b, 0, st, ix, bd, n_ro, 0u, S);
+        hipLaunchKernelGGL((k_seed_light<uint32_t, 256, false>), grid(S, X), b, 0, st, ix, bd, n_ro, S, X);
     }
 }
 
