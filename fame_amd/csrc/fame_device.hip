@@ -2940,13 +2940,14 @@ __device__ __forceinline__ uint16_t dp_get(const uint16_t* S, size_t stride, lon
 // the complement, and G for r = T; no target for other reverse letters
 __device__ __forceinline__ void dp_targets(uint32_t rc, bool rv, uint32_t& t1, uint32_t& t2)
 {
-    if (!rv) {
-        t1 = rc;
-        t2 = rc == 'T' ? 'C' : rc;
-    } else {
-        t1 = rc == 'T' ? 'A' : rc == 'G' ? 'C' : rc == 'A' ? 'T' : rc == 'C' ? 'G' : 0x100u;
-        t2 = rc == 'T' ? 'G' : t1;
-    }
+    // without branches: the complements of A, C, T, G sit in byte
+    // (letter >> 1) & 3 (0, 1, 2, 3) of 0x43414754, and a letter is one of
+    // them exactly when the complement of its complement is itself
+    constexpr uint32_t comp = 0x43414754u;
+    const uint32_t c1 = (comp >> (8 * ((rc >> 1) & 3u))) & 0xFFu;
+    const uint32_t c2 = (comp >> (8 * ((c1 >> 1) & 3u))) & 0xFFu;
+    t1 = rv ? (c2 == rc ? c1 : 0x100u) : rc;
+    t2 = rc == 'T' ? (rv ? (uint32_t)'G' : (uint32_t)'C') : t1;
 }
 
 // backtrackDP's choice at slot k of row rr (LevenshtDP.h:329-544) from the
@@ -3015,7 +3016,49 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
         // fill (runDPFill / runDPFillRev LevenshtDP.h:207-323): one row at a
         // time in registers; values are computed in int and kept as u16.
         // gw holds the reference bytes of columns r-8 .. r+8 (byte i = column
-        // r-8+i), shifted by one byte per row.
+        // r-8+i), shifted by one byte per row.  Row r's letters (reference
+        // column r+8 and its read letter) are loaded PD rows ahead into slot
+        // r % PD, which the unrolled loops below name statically (no register
+        // rotation, so a row waits only for its own letters); the loads are
+        // always issued, at positions clamped into the chromosome and the
+        // read, and col_at's bounds (0 outside the chromosome) are applied
+        // where a letter is used.
+        // A letter is loaded as the aligned 32-bit word that holds it (an
+        // aligned word never crosses a page, so the word of a byte inside an
+        // allocation is readable) and taken out of the word where it is used.
+        constexpr int PD = 8;
+        const int64_t cmax = (int64_t)clen - 1;
+        const uint32_t cs_mis = (uint32_t)(uintptr_t)cs & 3u, seq_mis = (uint32_t)(uintptr_t)seq & 3u;
+        auto word_at = [](const uint8_t* a) -> uint32_t {
+            return *(const uint32_t*)(a - ((uintptr_t)a & 3u));
+        };
+        auto gload = [&](long j) -> uint32_t { // the word of column j, clamped
+            int64_t p = ref_end - j + 1;
+            p = p < 0 ? 0 : (p > cmax ? cmax : p);
+            return word_at(cs + p);
+        };
+        auto gkeep = [&](long j, uint32_t w) -> uint32_t { // column j's letter (col_at)
+            const int64_t p = ref_end - j + 1;
+            const uint32_t b = (w >> (8 * ((cs_mis + (uint32_t)p) & 3u))) & 0xFFu;
+            return (p < 0 || (uint64_t)p >= clen) ? 0u : b;
+        };
+        auto ridx = [&](long rq) -> long { return rv ? rq - 1 : (long)L - rq; };
+        auto rload = [&](long rq) -> uint32_t { // the word of row rq's read letter, clamped
+            long i = ridx(rq);
+            i = i < 0 ? 0 : (i > (long)L - 1 ? (long)L - 1 : i);
+            return word_at(seq + i);
+        };
+        auto rkeep = [&](long rq, uint32_t w) -> uint32_t {
+            return (w >> (8 * ((seq_mis + (uint32_t)ridx(rq)) & 3u))) & 0xFFu;
+        };
+        uint32_t bq[PD], cq[PD], g0[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) g0[i] = gload(i + 1); // row 0: columns 1..8
+#pragma unroll
+        for (int rq = 1; rq <= PD; ++rq) {
+            bq[rq % PD] = gload(rq + 8);
+            cq[rq % PD] = rload(rq);
+        }
         int prev[19], cur[19];
         uint32_t gw[5];
 #pragma unroll
@@ -3029,30 +3072,46 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
 #pragma unroll
         for (int i = 0; i < 5; ++i) gw[i] = 0;
 #pragma unroll
-        for (int k = 9; k < 17; ++k) gw[k >> 2] |= col_at(cs, clen, ref_end, k - 8) << (8 * (k & 3)); // row 0: col k-8
+        for (int k = 9; k < 17; ++k) gw[k >> 2] |= gkeep(k - 8, g0[k - 9]) << (8 * (k & 3)); // row 0: col k-8
         auto shift_in = [&](uint32_t b) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) gw[i] = (gw[i] >> 8) | (gw[i + 1] << 24);
             gw[4] = b;
         };
         auto gbyte = [&](int i) -> uint32_t { return (gw[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
-        const long Lg = (long)L < 9 ? (long)L : 9;
-#pragma unroll 1
-        for (long rr = 1; rr <= Lg; ++rr) {
-            shift_in(col_at(cs, clen, ref_end, rr + 8));
+        // The rows are stepped to the longest read of the wave's lanes (Lw,
+        // the same for all of them, so that leaving the unrolled loops is a
+        // scalar branch and the letters in flight need no copies); a lane
+        // past its own L computes rows that it neither stores nor uses.
+        uint32_t Lw = 0;
+        for (uint64_t m = __ballot(true); m;) {
+            Lw = lane_u32(L, (int)__builtin_ctzll(m));
+            m &= ~__ballot(L <= Lw);
+        }
+        // rows 1..9: the general recurrence (the first column and the cells
+        // left of it lie in the band)
+#pragma unroll
+        for (int rr = 1; rr <= 9; ++rr) {
+            if (rr > (long)Lw) break;
+            const int s = rr % PD;
+            const uint32_t gb = gkeep(rr + 8, bq[s]);
+            const uint32_t rc = rkeep(rr, cq[s]);
+            bq[s] = gload(rr + PD + 8);
+            cq[s] = rload(rr + PD);
+            shift_in(gb);
             uint32_t t1, t2;
-            dp_targets(rv ? seq[rr - 1] : seq[L - rr], rv, t1, t2);
+            dp_targets(rc, rv, t1, t2);
             // left band (offset -9): the first column for rows 1..8 is not at
             // offset -9; row 9's slot holds the right band of row 8 (65518)
             cur[0] = rr == 9 ? 0xFFFF - 8 - 9 : 0xFFFF;
 #pragma unroll
             for (int k = 1; k < 18; ++k) {
-                const long o = k - 9, c = rr + o;
+                const int c = rr + k - 9;
                 int v;
                 if (c < 0) {
                     v = 0xFFFF;
                 } else if (c == 0) {
-                    v = (int)rr; // first column (rows 1..8)
+                    v = rr; // first column (rows 1..8)
                 } else {
                     const uint32_t gc = gbyte(k - 1);
                     const int mm = (gc != t1 && gc != t2) ? 1 : 0;
@@ -3066,40 +3125,35 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
                 cur[k] = v;
             }
             cur[18] = rr < (long)L ? (int)(0xFFFF - rr - 9) : 0xFFFF;
-            const int k0 = rr >= 9 ? 0 : (int)(9 - rr);
-            uint16_t* row = S + (size_t)(rr * 19) * stride;
+            const int k0 = rr >= 9 ? 0 : 9 - rr;
+            if (rr <= (long)L) {
+                uint16_t* row = S + (size_t)(rr * 19) * stride;
 #pragma unroll
-            for (int k = 0; k < 19; ++k) {
-                if (k >= k0) row[(size_t)k * stride] = (uint16_t)cur[k];
-                prev[k] = cur[k];
+                for (int k = k0; k < 19; ++k) row[(size_t)k * stride] = (uint16_t)cur[k];
             }
+#pragma unroll
+            for (int k = 0; k < 19; ++k) prev[k] = cur[k];
         }
-        // rows 10..L: every slot 1..17 is a computed cell.  The letters of
-        // the next rows are loaded eight rows ahead (bq / cq).
-        {
-            auto ref_b = [&](long rq) -> uint32_t { return rq <= (long)L ? col_at(cs, clen, ref_end, rq + 8) : 0u; };
-            auto read_b = [&](long rq) -> uint32_t {
-                return rq <= (long)L ? (uint32_t)(rv ? seq[rq - 1] : seq[L - rq]) : 0u;
-            };
-            constexpr int PD = 8; // rows loaded ahead
-            uint32_t bq[PD], cq[PD];
+        // rows 10..L: every slot 1..17 is a computed cell; row r uses slot
+        // r % PD, i.e. (10 + q) % PD in step q of a block of PD rows.  The
+        // backtrace starts at the first minimum of row L over offsets -8..8
+        // (backtrackDP / backtrackDPRev LevenshtDP.h:329-544), taken here
+        // when row L is computed.
+        uint16_t minimum = 0xFFFF;
+        unsigned long col = 0;
+        for (long rb = 10; rb <= (long)Lw; rb += PD) {
 #pragma unroll
             for (int q = 0; q < PD; ++q) {
-                bq[q] = ref_b(10 + q);
-                cq[q] = read_b(10 + q);
-            }
-#pragma unroll 1
-            for (long rr = 10; rr <= (long)L; ++rr) {
-                shift_in(bq[0]);
+                const long rr = rb + q;
+                if (rr > (long)Lw) break;
+                const int s = (10 + q) % PD;
+                const uint32_t gb = gkeep(rr + 8, bq[s]);
+                const uint32_t rc = rkeep(rr, cq[s]);
+                bq[s] = gload(rr + PD + 8);
+                cq[s] = rload(rr + PD);
+                shift_in(gb);
                 uint32_t t1, t2;
-                dp_targets(cq[0], rv, t1, t2);
-#pragma unroll
-                for (int q = 0; q < PD - 1; ++q) {
-                    bq[q] = bq[q + 1];
-                    cq[q] = cq[q + 1];
-                }
-                bq[PD - 1] = ref_b(rr + PD);
-                cq[PD - 1] = read_b(rr + PD);
+                dp_targets(rc, rv, t1, t2);
                 uint64_t dw = 0;
                 cur[0] = (int)(0xFFFF - rr);
 #pragma unroll
@@ -3116,29 +3170,31 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
                     cur[k] = a & 0xFFFF;
                 }
                 cur[18] = rr < (long)L ? (int)(0xFFFF - rr - 9) : 0xFFFF;
-                if (rr < DP_SROWS) {
-                    uint16_t* row = S + (size_t)(rr * 19) * stride;
+                if (rr <= (long)L) {
+                    if (rr < DP_SROWS) {
+                        uint16_t* row = S + (size_t)(rr * 19) * stride;
 #pragma unroll
-                    for (int k = 0; k < 19; ++k) row[(size_t)k * stride] = (uint16_t)cur[k];
+                        for (int k = 0; k < 19; ++k) row[(size_t)k * stride] = (uint16_t)cur[k];
+                    }
+                    D[(size_t)rr * stride] = dw; // row 10's word is never read
                 }
-                if (rr >= 11) D[(size_t)rr * stride] = dw;
+                if (rr == (long)L) {
+#pragma unroll
+                    for (int k = 1; k < 18; ++k)
+                        if (cur[k] < (int)minimum) {
+                            col = (unsigned long)((long)L + k - 9);
+                            minimum = (uint16_t)cur[k];
+                        }
+                }
 #pragma unroll
                 for (int k = 0; k < 19; ++k) prev[k] = cur[k];
             }
         }
         // backtrace (backtrackDP / backtrackDPRev LevenshtDP.h:329-544): the
-        // end column is the first minimum of row L over offsets -8..8
+        // end column is the first minimum of row L over offsets -8..8 (for
+        // L >= 10 taken during the fill)
         bool bad = false;
-        uint16_t minimum = 0xFFFF;
-        unsigned long col = 0;
-        if (L >= 10) {
-#pragma unroll
-            for (int k = 1; k < 18; ++k)
-                if (prev[k] < (int)minimum) {
-                    col = (unsigned long)((long)L + k - 9);
-                    minimum = (uint16_t)prev[k];
-                }
-        } else {
+        if (L < 10) {
             for (long o = -DPBAND; o <= DPBAND; ++o) {
                 const uint16_t v = dp_get(S, stride, L, (long)L, (long)L + o, &bad);
                 if (v < minimum) {
@@ -3147,42 +3203,63 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
                 }
             }
         }
-        // the trace: 2-bit steps, 32 to a word
+        // the trace: 2-bit steps, 32 to a word, gathered in a ring of two
+        // words (a row of the traceback adds at most 18 steps, so it completes
+        // at most one word) and stored outside the step loops
         int tn = 0;
-        uint64_t tw = 0;
+        uint64_t tw[2] = {0, 0};
         auto put = [&](uint32_t tt) {
-            if (tn < trace_cap) {
-                tw |= (uint64_t)tt << (2 * (tn & 31));
-                if ((tn & 31) == 31) {
-                    TR[(size_t)(tn >> 5) * stride] = tw;
-                    tw = 0;
-                }
-            }
+            if (tn < (int)trace_cap) tw[(tn >> 5) & 1] |= (uint64_t)tt << (2 * (tn & 31));
             ++tn;
         };
+        auto flush = [&](int tn0) { // the word that step tn0 was in, once complete
+            const int j = tn0 >> 5;
+            if ((tn >> 5) > j) {
+                if (32 * j + 31 < (int)trace_cap) TR[(size_t)j * stride] = tw[j & 1];
+                tw[j & 1] = 0;
+            }
+        };
         long rr = (long)L;
-        if (rr >= 11) { // rows >= 11 from the kept choices, read four rows ahead
+        if (rr >= 11) {
+            // rows >= 11 from the kept choices: row r's word is loaded four
+            // rows ahead into slot (L - r) % 4, static in the unrolled loop;
+            // the loads are always issued (a lane whose walk has ended, or
+            // rows below 11, read words that are never used)
             long k = (long)col - rr + 9;
-            uint64_t q0 = D[(size_t)rr * stride];
-            uint64_t q1 = rr - 1 >= 11 ? D[(size_t)(rr - 1) * stride] : 0;
-            uint64_t q2 = rr - 2 >= 11 ? D[(size_t)(rr - 2) * stride] : 0;
-            uint64_t q3 = rr - 3 >= 11 ? D[(size_t)(rr - 3) * stride] : 0;
-            while (rr >= 11) {
-                if (k < 1 || k > 17) {
-                    bad = true;
-                    break;
-                }
-                const uint32_t tt = (uint32_t)(q0 >> (2 * (k - 1))) & 3u;
-                put(tt);
-                if (tt == 3) {
-                    --k; // deletion: same row, one column left
-                } else {
-                    if (tt == 2) ++k; // insertion: row above, same column
-                    --rr;
-                    q0 = q1;
-                    q1 = q2;
-                    q2 = q3;
-                    q3 = rr - 3 >= 11 ? D[(size_t)(rr - 3) * stride] : 0;
+            uint64_t dq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dq[u] = D[(size_t)(rr - u) * stride];
+            bool act = true;
+            while (__ballot(act)) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint64_t w = dq[u];
+                    const long rq = rr - 4; // the row this slot takes next
+                    if (act) {
+                        // the steps of row rr: a deletion stays in the row,
+                        // one column left; an insertion moves up in the same
+                        // column
+                        const int tn0 = tn;
+                        uint32_t tt = 0;
+                        do {
+                            if (k < 1 || k > 17) {
+                                bad = true;
+                                break;
+                            }
+                            tt = (uint32_t)(w >> (2 * (k - 1))) & 3u;
+                            put(tt);
+                            if (tt == 3) --k;
+                        } while (tt == 3);
+                        flush(tn0);
+                        if (bad) {
+                            act = false;
+                        } else {
+                            if (tt == 2) ++k;
+                            --rr;
+                            act = rr >= 11;
+                        }
+                    }
+                    dq[u] = D[(size_t)(rq > 0 ? rq : 0) * stride];
                 }
             }
             col = (unsigned long)(rr + k - 9);
@@ -3199,13 +3276,15 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
             if (tt != 3) --rr;
             if (tt != 2) --col;
             put(tt);
+            flush(tn - 1);
         }
         while (col > 0 && tn <= trace_cap) {
             put(3);
+            flush(tn - 1);
             --col;
         }
         if (rr > 0 || col > 0) bad = true;
-        if ((tn & 31) && tn <= trace_cap) TR[(size_t)(tn >> 5) * stride] = tw;
+        if ((tn & 31) && tn <= trace_cap) TR[(size_t)(tn >> 5) * stride] = tw[(tn >> 5) & 1];
         if (tn > trace_cap || bad) {
             atomicOr(bd.flags, FLAG_DP_BROKEN);
             continue;
