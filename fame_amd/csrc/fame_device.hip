@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwave = (gridDim.x * blockDim.x) >> 6;
     // groups of 64 reads per wave: one coalesced load of their lengths and
-    // offsets, then read by read, the next read's bytes loaded while this
-    // one is worked on (lane p holds letters p, p + 64, p + 128, p + 192)
+    // offsets, then read by read, with the bytes of the next reads in flight
+    // while one is worked on (lane p holds letters p, p + 64, p + 128, p + 192)
     for (uint32_t r0 = wave * 64; r0 < n; r0 += nwave * 64) {
         const uint32_t cnt = n - r0 < 64 ? n - r0 : 64;
         const uint32_t myL = (uint32_t)lane < cnt ? lens[r0 + lane] : 0u;
@@ -91,24 +91,10 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
             const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(myO >> 32), (int)i);
             return ((uint64_t)hi << 32) | lo;
         };
-        uint8_t nb[4];
-        auto fetch = [&](uint32_t i) {
-            const uint32_t L = len_of(i);
-            const uint32_t Lm = L <= FAME_DEV_MAX_READ ? L : 0u; // longer reads are flagged, not read
-            const uint8_t* sp = seqs + off_of(i);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t p = (uint32_t)lane + 64u * q;
-                nb[q] = p < Lm ? sp[p] : (uint8_t)0;
-            }
-        };
-        fetch(0);
-        for (uint32_t i = 0; i < cnt; ++i) {
+        // read i of the group, its letters in cb (lane p: letters p, p + 64,
+        // p + 128, p + 192)
+        auto prep_one = [&](uint32_t i, const uint8_t (&cb)[4]) {
             const uint32_t r = r0 + i;
-            uint8_t cb[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) cb[q] = nb[q];
-            if (i + 1 < cnt) fetch(i + 1);
             const uint32_t L = len_of(i);
             uint8_t* qrev = rev + off_of(i);
             uint8_t st = 0;
@@ -151,7 +137,7 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                     c0[255] = 0;
                     c1[255] = 0;
                 }
-                continue;
+                return;
             }
             if (st == 0 || (paired && st == 1)) {
                 bool nflag = false;
@@ -177,6 +163,35 @@ __global__ __launch_bounds__(256) void k_prep(const uint8_t* __restrict__ seqs, 
                 pre[r] = st;
                 c0[255] = (st == 0 && (om & 1)) ? (uint8_t)L : 0;
                 c1[255] = (st == 0 && (om & 2)) ? (uint8_t)L : 0;
+            }
+        };
+        // the letters of read i are loaded U reads ahead into slot i % U, a
+        // register set the unrolled loop below names statically
+        constexpr int U = 4;
+        uint8_t nb[U][4];
+        // (always issued: lanes past the read, or reads too long for the
+        // rows, load a byte of this group's lengths instead, which prep_one
+        // does not use)
+        const uint8_t* dummy = (const uint8_t*)(lens + r0);
+        auto fetch = [&](uint32_t i, uint8_t (&d)[4]) {
+            const uint32_t L = len_of(i);
+            const uint32_t Lm = L <= FAME_DEV_MAX_READ ? L : 0u; // longer reads are flagged, not read
+            const uint8_t* sp = seqs + off_of(i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t p = (uint32_t)lane + 64u * q;
+                d[q] = *(p < Lm ? sp + p : dummy);
+            }
+        };
+#pragma unroll
+        for (int u = 0; u < U; ++u) fetch(min((uint32_t)u, cnt - 1), nb[u]);
+        for (uint32_t i0 = 0; i0 < cnt; i0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u;
+                if (i >= cnt) break;
+                prep_one(i, nb[u]);
+                fetch(min(i + U, cnt - 1), nb[u]);
             }
         }
     }
@@ -2848,6 +2863,45 @@ __device__ __forceinline__ uint8_t col_at(const uint8_t* cs, uint64_t clen, int6
     return (p < 0 || (uint64_t)p >= clen) ? 0 : cs[p];
 }
 
+// The first id in [lo, hi) whose CpG coordinate pos + RL2 is above thr (at
+// or above it when !strict), else hi; the coordinates of a window's CpGs
+// ascend with id (cpgTable is in genome order), so this is where the
+// reference's forward scans over a window's CpGs (ReadQueue.cpp:3355-3484,
+// 3690-3912) stop skipping or stop.  Eight probes per round, loaded
+// together, narrow the range about ninefold; `near` probes lo .. lo + 7
+// first, for an answer expected close to lo.
+__device__ __forceinline__ uint32_t cpg_search(const uint32_t* __restrict__ cpos, uint32_t lo, uint32_t hi,
+                                               uint32_t RL2, uint32_t thr, bool strict, bool near)
+{
+    while (lo < hi) {
+        const uint32_t n = hi - lo;
+        const bool dense = near || n <= 8;
+        uint32_t pr[8];
+        bool t[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pr[i] = dense ? lo + (uint32_t)i : lo + ((uint32_t)(i + 1) * n) / 9u;
+            const bool ok = pr[i] < hi;
+            const uint32_t v = cpos[ok ? pr[i] : lo] + RL2;
+            t[i] = !ok || (strict ? v > thr : v >= thr);
+        }
+        near = false;
+        int c = 8; // the first probe at or past the answer
+#pragma unroll
+        for (int i = 7; i >= 0; --i)
+            if (t[i]) c = i;
+        uint32_t nlo = lo, nhi = hi;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i == c - 1) nlo = pr[i] + 1;
+            if (i == c && pr[i] < hi) nhi = pr[i];
+        }
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_meth_direct(DevIndex ix, BatchDev bd)
 {
     const uint32_t n = *bd.direct_n;
@@ -2865,8 +2919,9 @@ __global__ __launch_bounds__(256) void k_meth_direct(DevIndex ix, BatchDev bd)
         const uint32_t minPos = (uint32_t)((uint64_t)metaPos + offset - ((uint64_t)L - 1));
         const uint32_t maxPos = metaPos + offset;
         uint32_t* meth = bd.meth;
-        // errNum == 0 branch :3355-3484
-        for (uint32_t id = m.z; id <= m.w; ++id) {
+        // errNum == 0 branch :3355-3484 (the CpGs before the read's span
+        // found by cpg_search instead of being stepped over)
+        for (uint32_t id = cpg_search(ix.cpg_pos, m.z, m.w + 1, RL2, minPos, false, false); id <= m.w; ++id) {
             const uint32_t pos = ix.cpg_pos[id];
             if (pos + RL2 < minPos) continue;
             if (isFwd) {
@@ -3003,16 +3058,12 @@ __global__ __launch_bounds__(256) void k_meth_dp(DevIndex ix, BatchDev bd, uint1
         const uint8_t* cs = ix.seq + ix.coff[chrom];
         const uint64_t clen = ix.clen[chrom];
         const int64_t ref_end = (int64_t)metaPos + offset;
-        int32_t minIndex = (int32_t)m.z, maxIndex = (int32_t)m.w;
-        for (int32_t id = (int32_t)m.z; (uint32_t)id <= m.w; ++id) {
-            const uint32_t pos = ix.cpg_pos[id];
-            if (pos + RL2 < minPos) {
-                ++minIndex;
-            } else if (pos + RL2 > maxPos) {
-                maxIndex = id - 1;
-                break;
-            }
-        }
+        // the CpGs of the read's span (the reference steps over the window's
+        // CpGs from its first: minIndex counts those before minPos, maxIndex
+        // ends before the first one past maxPos, m.w if none)
+        const uint32_t lo_id = cpg_search(ix.cpg_pos, m.z, m.w + 1, RL2, minPos, false, false);
+        const int32_t minIndex = (int32_t)lo_id;
+        const int32_t maxIndex = (int32_t)cpg_search(ix.cpg_pos, lo_id, m.w + 1, RL2, maxPos, true, true) - 1;
         // fill (runDPFill / runDPFillRev LevenshtDP.h:207-323): one row at a
         // time in registers; values are computed in int and kept as u16.
         // gw holds the reference bytes of columns r-8 .. r+8 (byte i = column
