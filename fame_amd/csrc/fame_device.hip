@@ -2735,10 +2735,17 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
         n = *bd.n_dev;
     }
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    // per block: the counters, then the three lists' appends (one global
+    // atomic per list and block: the lists' heads are the same address for
+    // every block)
     __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_app[3], s_base[3];
     if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 3) s_app[threadIdx.x] = 0;
     __syncthreads();
     int want = 0;
+    int cnt_at = -1;   // the counter of the read's status (0..2; invalid reads none)
+    int tally_at = -1; // counter 3 / 4 of get_stranded
     if (r < n) {
         uint8_t st = 3; // invalid, no counter
         uint64_t out = 0;
@@ -2790,9 +2797,9 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
                 st = 0;
                 out = useFwd ? mF : mR;
                 use = useFwd ? 1 : 2;
-                if (get_stranded) atomicAdd(&s_cnt[tally > 0 ? 3 : 4], 1u);
+                if (get_stranded) tally_at = tally > 0 ? 3 : 4;
             }
-            atomicAdd(&s_cnt[st], 1u);
+            cnt_at = st;
         }
         bd.status[r] = st;
         bd.use[r] = use;
@@ -2809,26 +2816,32 @@ __global__ __launch_bounds__(256) void k_select(DevIndex ix, BatchDev bd, uint32
                 want = 2;
         }
     }
-    {
-        const int lane = threadIdx.x & 63;
-        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2), blong = __ballot(want == 3);
-        uint32_t basedir = 0, basedp = 0, baselong = 0;
-        if (lane == 0) {
-            if (bdir) basedir = atomicAdd(bd.direct_n, (uint32_t)__popcll(bdir));
-            if (bdp) basedp = atomicAdd(bd.dp_n, (uint32_t)__popcll(bdp));
-            if (blong) baselong = atomicAdd(bd.dp_long_n, (uint32_t)__popcll(blong));
-        }
-        basedir = lane_u32(basedir, 0);
-        basedp = lane_u32(basedp, 0);
-        baselong = lane_u32(baselong, 0);
-        if (want == 1) bd.direct_list[basedir + __popcll(bdir & below)] = r;
-        if (want == 2) bd.dp_list[basedp + __popcll(bdp & below)] = r;
-        if (want == 3) bd.dp_long_list[baselong + __popcll(blong & below)] = r;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lanes_below(lane);
+    // the counters: one LDS atomic per wave and counter
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const uint64_t b = __ballot(cnt_at == i || tally_at == i);
+        if (lane == 0 && b) atomicAdd(&s_cnt[i], (uint32_t)__popcll(b));
+    }
+    const uint64_t bdir = __ballot(want == 1), bdp = __ballot(want == 2), blong = __ballot(want == 3);
+    uint32_t wofs[3] = {0, 0, 0};
+    if (lane == 0) {
+        if (bdir) wofs[0] = atomicAdd(&s_app[0], (uint32_t)__popcll(bdir));
+        if (bdp) wofs[1] = atomicAdd(&s_app[1], (uint32_t)__popcll(bdp));
+        if (blong) wofs[2] = atomicAdd(&s_app[2], (uint32_t)__popcll(blong));
     }
     __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t* head = threadIdx.x == 0 ? bd.direct_n : threadIdx.x == 1 ? bd.dp_n : bd.dp_long_n;
+        s_base[threadIdx.x] = s_app[threadIdx.x] ? atomicAdd(head, s_app[threadIdx.x]) : 0u;
+    }
     if (threadIdx.x < 5 && s_cnt[threadIdx.x])
         atomicAdd((unsigned long long*)&bd.counters[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (want == 1) bd.direct_list[s_base[0] + lane_u32(wofs[0], 0) + __popcll(bdir & below)] = r;
+    if (want == 2) bd.dp_list[s_base[1] + lane_u32(wofs[1], 0) + __popcll(bdp & below)] = r;
+    if (want == 3) bd.dp_long_list[s_base[2] + lane_u32(wofs[2], 0) + __popcll(blong & below)] = r;
 }
 
 // ---------------------------------------------------------------------------
