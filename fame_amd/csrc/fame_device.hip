@@ -672,11 +672,18 @@ __device__ __forceinline__ int seed_tail(const DevIndex& ix, const BatchDev& bd,
 // (a 256-byte row, the length in its byte 255); KPL == 0 (long reads): the
 // row is already in w.code, L letters long, and the k-mers are hashed and
 // looked up 64 at a time (no register arrays)
-template <typename TabT, typename IdxT, int KPL, typename EkmT, bool STREAM = false>
+struct SeedNoMid {
+    __device__ void operator()() const {}
+};
+
+// mid() runs once on every path, after the loads of the entry gather are
+// issued and before their values are used: a caller's loads issued there do
+// not delay the gather's (the memory counter is waited on in issue order)
+template <typename TabT, typename IdxT, int KPL, typename EkmT, bool STREAM = false, typename Mid = SeedNoMid>
 __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, uint32_t ro, SeedShared& sh,
                         const SeedWork<TabT, IdxT, EkmT>& w, PoolChunk& pc, bool exact, SeedStats& ss,
                         uint64_t* E_out, uint32_t* bc_out, uint32_t cw, bool staged = false,
-                        const uint64_t* pre = nullptr, uint32_t Llong = 0)
+                        const uint64_t* pre = nullptr, uint32_t Llong = 0, const Mid& mid = Mid{})
 {
     const int lane = threadIdx.x & 63;
     // cw: this lane's word of the row of reduced letters (the length is in
@@ -757,8 +764,14 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
     const bool fits = seed_prefix(w, K, L, inl, ss, E);
     *E_out = E;
     *bc_out = bucketCount;
-    if (!fits) return 1;
-    if constexpr (STREAM) return seed_tail_stream<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L, K, inl);
+    if (!fits) {
+        mid();
+        return 1;
+    }
+    if constexpr (STREAM) {
+        mid();
+        return seed_tail_stream<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L, K, inl);
+    }
     // entry gather, one entry per lane: entry j belongs to the k-mer whose
     // [koff, koff + klen) contains j.  Up to 256 entries each k-mer writes
     // its index over its entries first; beyond, each k-mer also writes the
@@ -768,6 +781,7 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
     // koff instead)
     if constexpr (sizeof(TabT) == 4) {
         if (E > 256) {
+            mid();
             for (uint32_t k = lane; k < K; k += 64) {
                 const uint32_t o = w.koff[k], n = w.klen[k];
                 const uint32_t lo = (uint32_t)w.klo[k];
@@ -800,7 +814,38 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
             for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
         }
         wave_sync();
+        if constexpr (sizeof(TabT) == 4) {
+            // entry j = lane + 64 q: a bucket held in its slot is written at
+            // once; the others' loads are all issued (lanes without one load
+            // entry 0), then mid(), then their values are written
+            uint32_t gv[4];
+            bool need[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t j = lane + 64u * q;
+                need[q] = false;
+                uint64_t at = 0;
+                if (j < E) {
+                    const uint32_t k = w.ekm[j];
+                    const uint32_t d = j - w.koff[k];
+                    if (inl && w.klen[k] <= 2) // the bucket is in its slot
+                        w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
+                    else {
+                        need[q] = true;
+                        at = (uint64_t)w.klo[k] + d;
+                    }
+                }
+                gv[q] = ix.kent[at];
+            }
+            mid();
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (need[q]) w.ent[lane + 64u * q] = gv[q] & 0x7FFFFFFFu; // without isStart
+            wave_sync();
+            return seed_tail<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L);
+        }
     }
+    mid();
     for (uint32_t j = lane; j < E; j += 64) {
         uint32_t k;
         if (scatter) {
@@ -1243,28 +1288,33 @@ __global__ __launch_bounds__(64 * FAME_SEED_WPB) __attribute__((amdgpu_waves_per
         uint32_t cwH = row(q0);
         for (int64_t qC = (int64_t)q0 - 3 * (int64_t)nw; qC < (int64_t)nq; qC += nw) {
             const uint32_t qH = (uint32_t)(qC + 3 * (int64_t)nw);
-            const uint32_t cwN = row(qH + nw);
+            uint32_t cwN;
             GoHashed<C::KPL> nH;
             GoPredicted<C::KPL> nP;
             GoSlots<C::KPL> nF;
             go_hash<C::KPL>(ix, gsh, pf, cwH, qH < nq ? (lane_u32(cwH, 63) >> 24) : 0u, nH, go_rnd);
-            go_anchor_load<C::KPL>(ix, nH, go_rnd);
-            go_predict<C::KPL>(ix, sH, nP);
-            go_fetch<C::KPL>(ix, sP, nF, go_rnd, go_ord);
-            if (qC >= 0) {
+            // the stages' loads, issued inside seed_one after its entry
+            // gather's loads (or here for a pair without one)
+            auto stages = [&]() {
+                cwN = row(qH + nw);
+                go_anchor_load<C::KPL>(ix, nH, go_rnd);
+                go_predict<C::KPL>(ix, sH, nP);
+                go_fetch<C::KPL>(ix, sP, nF, go_rnd, go_ord);
+            };
+            if (qC >= 0 && sF.L != 0) {
                 const uint32_t ro = ro_of((uint32_t)qC);
-                if (sF.L == 0) { // not seeded: invalid read
-                    if (lane == 0) {
-                        bd.cand_nf[ro] = 0;
-                        bd.cand_nr[ro] = 0;
-                        bd.cand_off[ro] = 0;
-                    }
-                } else {
-                    uint64_t E;
-                    uint32_t bc;
-                    const int rc = seed_one<TabT, uint16_t, C::KPL>(ix, bd, ro, sh, w, pc, false, ss, &E, &bc,
-                                                                      sF.L << 24, true, sF.sl);
-                    if (rc != 0) defer(ro, E, bc);
+                uint64_t E;
+                uint32_t bc;
+                const int rc = seed_one<TabT, uint16_t, C::KPL, uint8_t, false>(
+                    ix, bd, ro, sh, w, pc, false, ss, &E, &bc, sF.L << 24, true, sF.sl, 0u, stages);
+                if (rc != 0) defer(ro, E, bc);
+            } else {
+                stages();
+                if (qC >= 0 && lane == 0) { // not seeded: invalid read
+                    const uint32_t ro = ro_of((uint32_t)qC);
+                    bd.cand_nf[ro] = 0;
+                    bd.cand_nr[ro] = 0;
+                    bd.cand_off[ro] = 0;
                 }
             }
             sH = nH;
