@@ -856,35 +856,30 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
             return seed_tail<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L);
         }
     }
-    const bool scatter = E <= 256;
-    if (scatter) {
-        for (uint32_t k = lane; k < K; k += 64) {
-            const uint32_t o = w.koff[k], n = w.klen[k];
-            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
-        }
-        wave_sync();
-        if constexpr (sizeof(TabT) == 4) {
-            // entry j = lane + 64 q: a bucket held in its slot is written at
-            // once; the others' loads are all issued (lanes without one load
-            // entry 0), then mid(), then their values are written
+    if constexpr (sizeof(TabT) == 4) {
+        if (E <= 256) {
+            // eaux[j] = the k-mer table position of entry j, or ~0u for an
+            // entry held in its slot (written to ent at once); ekm[j] = its
+            // k-mer.  The table entries are then loaded, mid(), and written.
+            for (uint32_t k = lane; k < K; k += 64) {
+                const uint32_t o = w.koff[k], n = w.klen[k], lo = (uint32_t)w.klo[k];
+                const bool held = inl && n <= 2; // the bucket is in its slot
+                const uint32_t e1 = held && n == 2 ? w.k2[k] : 0u;
+                for (uint32_t i = 0; i < n; ++i) {
+                    w.ekm[o + i] = (EkmT)k;
+                    if (held) w.ent[o + i] = i == 0 ? lo : e1;
+                    w.eaux[o + i] = held ? ~0u : lo + i;
+                }
+            }
+            wave_sync();
             uint32_t gv[4];
             bool need[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t j = lane + 64u * q;
-                need[q] = false;
-                uint64_t at = 0;
-                if (j < E) {
-                    const uint32_t k = w.ekm[j];
-                    const uint32_t d = j - w.koff[k];
-                    if (inl && w.klen[k] <= 2) // the bucket is in its slot
-                        w.ent[j] = d == 0 ? (uint32_t)w.klo[k] : w.k2[k];
-                    else {
-                        need[q] = true;
-                        at = (uint64_t)w.klo[k] + d;
-                    }
-                }
-                gv[q] = ix.kent[at];
+                const uint32_t at = j < E ? w.eaux[j] : ~0u;
+                need[q] = at != ~0u;
+                gv[q] = ix.kent[need[q] ? at : 0u];
             }
             mid();
 #pragma unroll
@@ -893,6 +888,14 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
             wave_sync();
             return seed_tail<TabT, IdxT, EkmT>(ix, bd, ro, w, pc, exact, E, bucketCount, L);
         }
+    }
+    const bool scatter = E <= 256;
+    if (scatter) {
+        for (uint32_t k = lane; k < K; k += 64) {
+            const uint32_t o = w.koff[k], n = w.klen[k];
+            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (EkmT)k;
+        }
+        wave_sync();
     }
     mid();
     for (uint32_t j = lane; j < E; j += 64) {
