@@ -2238,30 +2238,87 @@ __device__ __forceinline__ uint64_t verify_one(const DevIndex& ix, const BatchDe
             probe_chunk(reduced(k.x), k.lo, k.hi);
             return true;
         };
+        // a chunk's letters in scan order (reverse map: letter order
+        // reversed, complemented; any N bit means the same whichever way
+        // round)
+        auto scan_codes = [&](uint64_t v) -> uint32_t {
+            const uint32_t xw = (uint32_t)v;
+            const uint32_t y = __brev(xw);
+            const uint32_t xr = ~(((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1));
+            return rev ? xr : xw;
+        };
         if (nchunk > 0 && filter_chunk(0, v0)) {
             // interior chunks are whole: their words follow one another in
-            // memory (backwards for the reverse map), two loaded ahead
+            // memory (backwards for the reverse map), two loaded ahead.  A
+            // chunk only records whether any of its positions hit (bit ci of
+            // hm); the hits' pieces and regions are worked out afterwards,
+            // chunk by chunk, from the same probes.  (Hits are rare per lane
+            // but some lane of a wave has one at almost every chunk, and
+            // working them out in the loop cost every lane the work.)
             const int last = nchunk - 1;
+            const int x1 = x; // scan index of chunk 1's first letter
             const uint64_t* wp = pk + (rev ? last - 1 : 1);
             const int dw = rev ? -1 : 1;
             uint64_t c1 = last > 1 ? wp[0] : 0, c2 = last > 2 ? wp[dw] : 0;
+            uint64_t hm[FILT_MASKW];
+#pragma unroll
+            for (int q = 0; q < FILT_MASKW; ++q) hm[q] = 0;
+            uint64_t hacc = 0;
             for (int ci = 1; ci < last; ++ci) {
                 const uint64_t v = c1;
                 c1 = c2;
                 if (ci + 2 < last) c2 = wp[(ci + 1) * dw];
                 asm volatile("" ::: "memory");
-                uint32_t xw = (uint32_t)v;
-                uint32_t nmk = (uint32_t)(v >> 32) & 0xFFFFu;
-                // reverse map: letter order reversed, complemented (any N
-                // bit means the same whichever way round)
-                const uint32_t y = __brev(xw);
-                const uint32_t xr = ~(((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1));
-                xw = rev ? xr : xw;
-                if (nmk) {
+                if ((uint32_t)(v >> 32) & 0xFFFFu) {
                     filt = false;
                     break;
                 }
-                probe_chunk(reduced(xw), 0, 16);
+                const uint32_t P = reduced(scan_codes(v));
+                uint32_t any = 0xFFFFFFFFu;
+#pragma unroll
+                for (int ii = 0; ii < 16; ++ii) any = min(any, probe(P, ii));
+                pprev = P;
+                hacc |= (uint64_t)(any < 16u) << (ci & 63);
+                if ((ci & 63) == 63) {
+#pragma unroll
+                    for (int q = 0; q < FILT_MASKW; ++q)
+                        if (q == (ci >> 6)) hm[q] = hacc;
+                    hacc = 0;
+                }
+            }
+            x = x1 + 16 * (last > 1 ? last - 1 : 0);
+            if (filt) {
+                const int cl = last - 1; // the last interior chunk
+#pragma unroll
+                for (int q = 0; q < FILT_MASKW; ++q)
+                    if (cl >= 1 && q == (cl >> 6)) hm[q] |= hacc;
+                const uint32_t pprev_last = pprev;
+                // the interior chunks with hits: lowest piece index among
+                // the hits, and 15 - the highest; pieces ilo..ihi end in the
+                // chunk (every piece when codes repeat): the union of their
+                // regions.  Piece i ending at scan position e starts the match
+                // at >= e - (i+1)*FP + 1 - NERR and ends it at <= e - (i+1)*FP +
+                // m + NERR.
+#pragma unroll
+                for (int q = 0; q < FILT_MASKW; ++q) {
+                    for (uint64_t b = hm[q]; b; b &= b - 1) {
+                        const int ci = 64 * q + (int)__builtin_ctzll(b);
+                        pprev = reduced(scan_codes(raw_chunk(ci - 1)));
+                        const uint32_t P = reduced(scan_codes(raw_chunk(ci)));
+                        uint32_t alo = 0xFFFFFFFFu, ahi = 0xFFFFFFFFu;
+#pragma unroll
+                        for (int ii = 0; ii < 16; ++ii) {
+                            const uint32_t r = probe(P, ii);
+                            alo = min(alo, r);
+                            ahi = min(ahi, r ^ 15u);
+                        }
+                        const int xc = x1 + 16 * (ci - 1);
+                        const int ilo = pdup ? 0 : (int)alo;
+                        const int ihi = pdup ? NP - 1 : 15 - (int)ahi;
+                        mark_range(xc - (ihi + 1) * FP + 1 - (int)NERR, xc + 15 - (ilo + 1) * FP + (int)m + (int)NERR);
+                    }
+                }
+                pprev = pprev_last;
             }
             if (filt && last > 0 && !filter_chunk(last, raw_chunk_c(last))) filt = false;
         } else {
