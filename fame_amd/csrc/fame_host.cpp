@@ -1235,30 +1235,34 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
     return FAME_OK;
 }
 
-// a large batched call: sub-batches of pipe_batch reads alternate between two
-// lanes (own graph buffers, own stream), so that one lane's memory-bound
-// seeding overlaps the other's compute-bound verification and DP.  Every
-// read is independent and the shared CpG counts are atomic sums, so the
-// split does not change any result.  Sub-batches that outgrew the graphs are
-// re-run on the plain path at the end.
+// a large batched call: sub-batches of at most pipe_batch reads alternate
+// between two lanes (own graph buffers, own stream), so that one lane's
+// memory-bound seeding overlaps the other's compute-bound verification and
+// DP.  The call is cut into an even number of equal sub-batches, so that the
+// lanes end together (a last sub-batch of its own runs with no other lane's
+// work beside it).  Every read is independent and the shared CpG counts are
+// atomic sums, so the split does not change any result.  Sub-batches that
+// outgrew the graphs are re-run on the plain path at the end.
 int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
                   uint64_t total_bytes, uint64_t counters[3], const uint8_t** out_valid, const uint64_t** out_mat,
                   const uint8_t** out_status)
 {
     auto t0 = std::chrono::steady_clock::now();
-    const uint32_t B = c->pipe_batch;
     const bool o0 = c->both || c->match_r1_fwd;
     const bool o1 = c->both || !c->match_r1_fwd;
     const uint32_t ori = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
     hipStream_t ls[2] = {c->stream, c->stream2};
-    GraphPlan* pl[2] = {plan_for(c, B, total_bytes, 1), plan_for(c, B, total_bytes, 2)};
+    GraphPlan* pl[2] = {plan_for(c, c->pipe_batch, total_bytes, 1), plan_for(c, c->pipe_batch, total_bytes, 2)};
     int cand = 2;
     for (auto& q : c->plans)
         if (q->chosen >= 0) cand = q->chosen;
     uint8_t* ov = grow<uint8_t>(c->b_pvalid, n + 1);
     uint64_t* om = grow<uint64_t>(c->b_pmat, n + 1);
     uint8_t* os = grow<uint8_t>(c->b_pstatus, n + 1);
-    const uint32_t nb = (n + B - 1) / B;
+    uint32_t nb = (n + c->pipe_batch - 1) / c->pipe_batch;
+    if (nb & 1) ++nb; // n > pipe_batch: at least two
+    const uint32_t B = (uint32_t)(((uint64_t)n + nb - 1) / nb);
+    nb = (n + B - 1) / B; // no empty sub-batch
     struct Sub {
         uint32_t s, n, lane, slot;
     };
