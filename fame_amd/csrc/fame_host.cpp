@@ -2038,8 +2038,8 @@ fame_ctx* serve_worker_open(fame_ctx* c)
 void serve_worker_close(fame_ctx* w) { fame_close(w); }
 
 // Batched form, paired: a call of more than pipe_batch / 2 pairs (not the
-// stranded first chunk) is split into sub-batches of that many pairs that
-// alternate between two lanes, the context and a worker context over the
+// stranded first chunk) is split into an even number of equal sub-batches of
+// at most that many pairs that alternate between two lanes, the context and a worker context over the
 // same index and CpG accumulators, each with its own buffers and stream and
 // a host thread of its own: one lane's host waits between its stages (the
 // paired pipeline sizes its stages from device counts) overlap the other's
@@ -2050,7 +2050,13 @@ int run_pairs_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_o,
                         const uint64_t** out_mat)
 {
     auto t0 = std::chrono::steady_clock::now();
-    const uint32_t B = std::max<uint32_t>(c->pipe_batch / 2, 128);
+    // an even number of equal sub-batches of at most pipe_batch / 2 pairs, so
+    // that the lanes end together
+    const uint32_t B0 = std::max<uint32_t>(c->pipe_batch / 2, 128);
+    uint32_t nb = (n + B0 - 1) / B0;
+    if (nb & 1) ++nb;
+    const uint32_t B = (uint32_t)(((uint64_t)n + nb - 1) / nb);
+    nb = (n + B - 1) / B;
     if (!c->pair_worker) c->pair_worker = serve_worker_open(c);
     fame_ctx* w = c->pair_worker;
     if (!w) { // no second lane: the plain pipeline
@@ -2072,7 +2078,6 @@ int run_pairs_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_o,
         lo[l] = grow<uint64_t>(c->b_lane_offs[l], 2ull * B + 1);
         ll[l] = grow<uint32_t>(c->b_lane_lens[l], 2ull * B + 1);
     }
-    const uint32_t nb = (n + B - 1) / B;
     uint64_t lcnt[2][5] = {{0}};
     int lrc[2] = {FAME_OK, FAME_OK};
     auto lane = [&](int l) {
