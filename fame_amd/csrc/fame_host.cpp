@@ -1242,14 +1242,16 @@ int run_batch_graph(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, 
 // lanes end together (a last sub-batch of its own runs with no other lane's
 // work beside it).  Every read is independent and the shared CpG counts are
 // atomic sums, so the split does not change any result.  Sub-batches that
-// outgrew the graphs are re-run on the plain path at the end.
+// outgrew the graphs are re-run on the plain path at the end.  The stranded
+// first chunk (get_stranded: both orientations, strand tallies) is split the
+// same way; the lanes' tallies are added to the context's.
 int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const uint32_t* d_lens, uint32_t n,
-                  uint64_t total_bytes, uint64_t counters[3], const uint8_t** out_valid, const uint64_t** out_mat,
-                  const uint8_t** out_status)
+                  uint64_t total_bytes, int get_stranded, uint64_t counters[3], const uint8_t** out_valid,
+                  const uint64_t** out_mat, const uint8_t** out_status)
 {
     auto t0 = std::chrono::steady_clock::now();
-    const bool o0 = c->both || c->match_r1_fwd;
-    const bool o1 = c->both || !c->match_r1_fwd;
+    const bool o0 = c->both || get_stranded || c->match_r1_fwd;
+    const bool o1 = c->both || get_stranded || !c->match_r1_fwd;
     const uint32_t ori = (o0 ? 1u : 0u) | (o1 ? 2u : 0u);
     hipStream_t ls[2] = {c->stream, c->stream2};
     GraphPlan* pl[2] = {plan_for(c, c->pipe_batch, total_bytes, 1), plan_for(c, c->pipe_batch, total_bytes, 2)};
@@ -1277,12 +1279,12 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
         for (uint32_t k = k0; k < k1; ++k) {
             const int l = (int)(k & 1);
             const uint32_t s0 = k * B;
-            GraphKey key{d_seqs, d_offs + s0, d_lens + s0, 0, ori, cand};
+            GraphKey key{d_seqs, d_offs + s0, d_lens + s0, get_stranded, ori, cand};
             ex[k - k0] = pl[l]->find_graph(key);
             if (!ex[k - k0]) {
                 HIPC(hipStreamSynchronize(ls[l]));
-                ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, std::min(n - s0, B), ori, 0,
-                                           cand, ls[l]);
+                ex[k - k0] = capture_graph(c, pl[l], key, d_seqs, d_offs + s0, d_lens + s0, std::min(n - s0, B), ori,
+                                           get_stranded, cand, ls[l]);
             }
         }
         for (uint32_t k = k0; k < k1; ++k) {
@@ -1322,10 +1324,17 @@ int run_pipelined(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, co
     HIPC(hipMemcpy(cnt[0], pl[0]->counters.p, 64, hipMemcpyDeviceToHost));
     HIPC(hipMemcpy(cnt[1], pl[1]->counters.p, 64, hipMemcpyDeviceToHost));
     for (int k = 0; k < 3; ++k) counters[k] += cnt[0][k] + cnt[1][k];
+    if (get_stranded) { // the strand tallies, where fame_decide_strand reads them
+        uint64_t t[2];
+        HIPC(hipMemcpy(t, c->d_counters.as<uint64_t>() + 3, 16, hipMemcpyDeviceToHost));
+        t[0] += cnt[0][3] + cnt[1][3];
+        t[1] += cnt[0][4] + cnt[1][4];
+        HIPC(hipMemcpy(c->d_counters.as<uint64_t>() + 3, t, 16, hipMemcpyHostToDevice));
+    }
     // the sub-batches that outgrew the graphs, on the plain path
     for (Sub& x : subs) {
         c->graph_fallbacks += 1;
-        int rc = run_batch(c, d_seqs, d_offs + x.s, d_lens + x.s, x.n, total_bytes, 0, counters, c->stream);
+        int rc = run_batch(c, d_seqs, d_offs + x.s, d_lens + x.s, x.n, total_bytes, get_stranded, counters, c->stream);
         if (rc != FAME_OK) return rc;
         HIPC(hipMemcpyAsync(ov + x.s, c->b_valid.p, x.n, hipMemcpyDeviceToDevice, c->stream));
         HIPC(hipMemcpyAsync(om + x.s, c->b_mat.p, 8ull * x.n, hipMemcpyDeviceToDevice, c->stream));
@@ -1348,8 +1357,11 @@ int match_batch(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, cons
                 uint64_t total_bytes, int get_stranded, uint64_t counters[3], hipStream_t st,
                 const uint8_t** out_valid, const uint64_t** out_mat, const uint8_t** out_status)
 {
-    if (c->form == 1 && !get_stranded && n > c->pipe_batch)
-        return run_pipelined(c, d_seqs, d_offs, d_lens, n, total_bytes, counters, out_valid, out_mat, out_status);
+    // a large call on two lanes; the stranded first chunk (300 000 reads in
+    // FAME's loop) also from 2^16 reads on, since nothing else can overlap it
+    if (c->form == 1 && (n > c->pipe_batch || (get_stranded && n >= (1u << 16))))
+        return run_pipelined(c, d_seqs, d_offs, d_lens, n, total_bytes, get_stranded, counters, out_valid, out_mat,
+                             out_status);
     if (c->form == 1 && n) {
         const int rc = run_batch_graph(c, d_seqs, d_offs, d_lens, n, total_bytes, get_stranded, counters, st,
                                        out_valid, out_mat, out_status);

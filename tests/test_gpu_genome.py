@@ -2,7 +2,8 @@
 8 GiB slot table and the tabIndex narrowed to 32 bits), 24 chromosomes
 (CHROMNUM 24), 400 000 reads in FAME's chunk loop (a first chunk of 300 000
 in both orientations, then a stranded one), in the plain form, in the
-batched form with the stranded chunk split over the two pipelined lanes, and
+batched form with both chunks split over the two pipelined lanes (the first
+one's strand tallies summed over the lanes), and
 with the 64-bit tabIndex kept (FAME_TAB64, test-only).  The reference is
 FAME's own CPU path (main.cpp queryRoutine over ReadQueue::matchReads),
 built from its sources into oracle/_ref for these constants; its per-read
