@@ -147,7 +147,7 @@ struct fame_ctx {
     int form = 0; // 0 plain, 1 batched
     std::vector<std::unique_ptr<GraphPlan>> plans;
     uint64_t graph_calls = 0, graph_fallbacks = 0, graph_captures = 0;
-    uint32_t pipe_batch = 1u << 21; // sub-batch of a large batched call (two lanes)
+    uint32_t pipe_batch = 3u << 20; // largest sub-batch of a batched call split over two lanes
     DevBuf b_pvalid, b_pmat, b_pstatus;  // outputs of a pipelined call
     // reads over FAME_DEV_MAX_READ letters: counters (long_n, longest, DP
     // count), list, rows, DP list and DP scratch

@@ -186,7 +186,8 @@ int fame_sc_close(fame_sc* sc);
 int fame_set_form(fame_ctx* ctx, int form);
 
 /* In the batched form a call of more than sub_batch_reads reads (default
- * 2^21) that is not the stranded first chunk is split into sub-batches that
+ * 3 * 2^20), and the stranded first chunk from 2^16 reads on, is split into
+ * an even number of equal sub-batches of at most sub_batch_reads that
  * alternate between two lanes (graph buffers and HIP streams of their own),
  * so that one lane's seeding overlaps the other's verification and DP.
  * FAME_ERR_ARG below 256. */
