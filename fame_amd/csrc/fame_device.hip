@@ -927,47 +927,6 @@ __device__ __forceinline__ int seed_one(const DevIndex& ix, const BatchDev& bd, 
 // Slot lookups through the genome-ordered copy (DevIndex::gkey)
 // ---------------------------------------------------------------------------
 
-// Prefix form of the spaced-seed hash.  With Q[t] = rol(seed(c_t), -t) and
-// P[j] = Q[0] ^ ... ^ Q[j-1] over a read's reduced letters c_t, the hash of
-// k-mer k,
-//   XOR over care positions i of rol(seed(c_{k+i}), 31 - i),
-// equals rol(XOR over care i of Q[k+i], 31 + k); a run [a, b] of care
-// positions contributes P[k+b+1] ^ P[k+a], so the hash is rol of the XOR of
-// P[k + e] over the ends e of the runs: ten LDS reads per k-mer for the
-// shipped seed instead of 28 letters and 28 table terms.
-struct SeedEnds {
-    int n;
-    int e[KMERLEN + 1];
-};
-constexpr SeedEnds seed_ends()
-{
-    SeedEnds s{};
-    bool prev = false;
-    for (int i = 0; i <= (int)KMERLEN; ++i) {
-        const bool cur = i < (int)KMERLEN && ((SEEDBITS >> (KMERLEN - 1 - i)) & 1);
-        if (cur != prev) s.e[s.n++] = i;
-        prev = cur;
-    }
-    return s;
-}
-
-// inclusive XOR scan of a u64 over the lanes (the DPP pattern of
-// wave_iscan_u32 on both halves)
-__device__ __forceinline__ uint32_t wave_ixor_u32(uint32_t v)
-{
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false); // row_shr:1
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false); // row_shr:2
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false); // row_shr:4
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false); // row_shr:8
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
-    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
-    return v;
-}
-__device__ __forceinline__ uint64_t wave_ixor_u64(uint64_t v)
-{
-    return ((uint64_t)wave_ixor_u32((uint32_t)(v >> 32)) << 32) | wave_ixor_u32((uint32_t)v);
-}
-
 // A (read, orientation) passes three stages before seed_one takes its
 // slots, one per iteration of the seed loop, so that each stage's loads have
 // a whole iteration to arrive:

@@ -66,6 +66,8 @@ struct PeWork {
     uint32_t* ent;  // [cap_e] kent of each bucket entry
     uint8_t* ekm;   // [cap_e] its k-mer
     uint32_t* eaux; // [cap_e] bit 0 kept; first / count, or lookup slot
+    uint64_t* pf;   // [256] prefix XOR of the read's hash terms, in LDS (dead after the
+                    // hashing); NULL: the closed form from the shared table
     uint32_t cap_e;
     uint32_t* hkey; // counting table (first read)
     uint32_t* hfirst;
@@ -191,6 +193,26 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
     wave_sync();
     const uint32_t K = L - KMERLEN + 1;
     const TabT* tab = (const TabT*)ix.tab;
+    // NTPS64 in prefix form (SeedEnds): the read's terms rol(seed(c_t), -t),
+    // four letters per lane, XOR-scanned over the wave into w.pf
+    if (w.pf) {
+        uint64_t x[4], acc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t t = 4u * (uint32_t)lane + (uint32_t)j;
+            acc ^= t < L ? rol64(red_seed((cwd >> (8 * j)) & 3u), (int)((64u - (t & 63u)) & 63u)) : 0ull;
+            x[j] = acc;
+        }
+        const uint64_t ex = wave_ixor_u64(acc) ^ acc; // the letters below this lane's
+        if (lane == 0) w.pf[0] = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t t = 4u * (uint32_t)lane + (uint32_t)j;
+            if (t < L) w.pf[t + 1] = ex ^ x[j];
+        }
+        wave_sync();
+    }
+    constexpr SeedEnds se = seed_ends();
     TabT lo[PE_KPL], hi[PE_KPL];
 #pragma unroll
     for (int q = 0; q < PE_KPL; ++q) {
@@ -198,14 +220,22 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
         lo[q] = hi[q] = 0;
         if (k < K) {
             uint64_t h = 0;
+            if (w.pf) {
+                uint64_t X = 0;
 #pragma unroll
-            for (uint32_t i = 0; i < KMERLEN; ++i)
-                if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][m.code[k + i]];
+                for (int j = 0; j < se.n; ++j) X ^= w.pf[k + se.e[j]];
+                h = rol64(X, (int)((KMERLEN - 1 + k) & 63));
+            } else {
+#pragma unroll
+                for (uint32_t i = 0; i < KMERLEN; ++i)
+                    if ((SEEDBITS >> (KMERLEN - 1 - i)) & 1) h ^= sh.rot[i][m.code[k + i]];
+            }
             const uint64_t key = ix.hpow2 ? (h & ix.hmask) : (h % ix.htab);
             lo[q] = tab[key];
             hi[q] = tab[key + 1];
         }
     }
+    wave_sync(); // the prefix array is dead: ent and eaux are written below
     uint32_t bsum = 0;
 #pragma unroll
     for (int q = 0; q < PE_KPL; ++q) {
@@ -256,15 +286,31 @@ __device__ int pe_entries(const DevIndex& ix, const BatchDev& bd, uint32_t ro, c
     if (big || carry > w.cap_e) return 1;
     const uint32_t E = (uint32_t)carry;
     wave_sync();
-    for (uint32_t j = lane; j < E; j += 64) {
-        uint32_t a = 0, b = K;
-        while (b - a > 1) {
-            const uint32_t mid = (a + b) >> 1;
-            if (w.koff[mid] <= j) a = mid;
-            else b = mid;
+    // entry j's k-mer: up to 256 entries each k-mer writes its index over its
+    // entries (one LDS read per entry afterwards), beyond a binary search
+    // over the offsets
+    const bool scatter = E <= 256;
+    if (scatter) {
+        for (uint32_t k = lane; k < K; k += 64) {
+            const uint32_t o = w.koff[k], n = w.klen[k];
+            for (uint32_t i = 0; i < n; ++i) w.ekm[o + i] = (uint8_t)k;
         }
-        uint32_t k = a;
-        while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k;
+        wave_sync();
+    }
+    for (uint32_t j = lane; j < E; j += 64) {
+        uint32_t k;
+        if (scatter) {
+            k = w.ekm[j];
+        } else {
+            uint32_t a = 0, b = K;
+            while (b - a > 1) {
+                const uint32_t mid = (a + b) >> 1;
+                if (w.koff[mid] <= j) a = mid;
+                else b = mid;
+            }
+            k = a;
+            while (w.klen[k] == 0 || w.koff[k] + w.klen[k] <= j) ++k;
+        }
         const uint64_t g = (uint64_t)w.klo[k] + (j - w.koff[k]);
         const uint64_t kp = ix.kpe[g]; // kent | tmask << 32
         const uint32_t e = (uint32_t)kp;
@@ -780,6 +826,10 @@ __global__ __launch_bounds__(64) void k_pe_seed(DevIndex ix, BatchDev bd, PairDe
     w.ent = L.ent;
     w.ekm = L.ekm;
     w.eaux = L.eaux;
+    // the hash's prefix array over ent and eaux (2 KiB, written before them)
+    static_assert(offsetof(PeLight<TabT>, eaux) == offsetof(PeLight<TabT>, ent) + 4 * PE_LE, "ent, eaux adjacent");
+    static_assert(offsetof(PeLight<TabT>, ent) % 8 == 0 && 8 * PE_LE <= 2 * 4 * PE_LE, "prefix array fits");
+    w.pf = (uint64_t*)L.ent;
     w.hsize = 2 * PE_LE;
     w.hkey = L.u.h.key;
     w.hfirst = L.u.h.first;
@@ -853,6 +903,7 @@ __global__ __launch_bounds__(64) void k_pe_seed_big(DevIndex ix, BatchDev bd, Pa
     w.ent = (uint32_t*)take(4ull * pd.cap_e);
     w.ekm = (uint8_t*)take(pd.cap_e);
     w.eaux = (uint32_t*)take(4ull * pd.cap_e);
+    w.pf = nullptr; // global scratch: the closed form from LDS is faster
     w.hsize = pd.hsize;
     w.hkey = (uint32_t*)take(4ull * pd.hsize);
     w.hfirst = (uint32_t*)take(4ull * pd.hsize);
