@@ -132,6 +132,50 @@ __device__ __forceinline__ uint32_t hs_thr(uint64_t b, float f)
     return (uint32_t)x;
 }
 
+// Placement of keys 0..n-1 (key t held by lane t) into a fresh hopscotch table
+// of bucket_count buckets without the table, where the insertions need no
+// rehash (n at most 0.9 B) and no displacement beyond the neighbourhood: the
+// table is then linear probing from each key's home bucket (key & (B - 1)) in
+// insertion order (find_empty_bucket, hopscotch_hash.h:1491-1500), kept as a
+// bitmap of at most 128 buckets (B <= 64, n <= 64) in scalar registers.
+// Lane t receives key t's bucket in mypos.  Returns false where that does not
+// hold (the caller emulates the table).
+__device__ __forceinline__ bool lp_place(uint32_t mykey, uint32_t n, uint64_t bucket_count, uint64_t tb_cap,
+                                         uint32_t& mypos)
+{
+    const uint64_t B = bucket_count ? rup_pow2(bucket_count) : 0;
+    const uint64_t arr = B ? B + NBH - 1 : 0;
+    if (B == 0 || B > 64 || n > 64 || arr > tb_cap || n > hs_thr(B, 0.9f)) return false;
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t occ0 = 0, occ1 = 0; // buckets 0..63, 64..127
+    mypos = 0;
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t ib = lane_u32(mykey, (int)t) & (uint32_t)(B - 1); // < 64
+        const uint64_t f0 = ~occ0 & (~0ull << ib);
+        const uint32_t e = f0 ? (uint32_t)__builtin_ctzll(f0) : (~occ1 ? 64u + (uint32_t)__builtin_ctzll(~occ1) : 128u);
+        if (e >= arr || e - ib >= NBH) return false;
+        if (e < 64) occ0 |= 1ull << e;
+        else occ1 |= 1ull << (e - 64);
+        if (lane == t) mypos = e;
+    }
+    return true;
+}
+
+// bucket-order rank of each selected lane's key (bucket mypos < 128) among the
+// selected keys, and their number (the table's iteration order, by bucket)
+__device__ __forceinline__ uint32_t lp_rank(bool sel, uint32_t mypos, uint32_t& rank)
+{
+    uint64_t k0 = 0, k1 = 0;
+    for (uint64_t b = __ballot(sel); b; b &= b - 1) {
+        const uint32_t p = lane_u32(mypos, (int)__builtin_ctzll(b));
+        if (p < 64) k0 |= 1ull << p;
+        else k1 |= 1ull << (p - 64);
+    }
+    rank = mypos < 64 ? (uint32_t)__popcll(k0 & ((1ull << mypos) - 1))
+                      : (uint32_t)(__popcll(k0) + __popcll(k1 & ((1ull << (mypos - 64)) - 1)));
+    return (uint32_t)(__popcll(k0) + __popcll(k1));
+}
+
 // size_type(std::ceil(float(n) / 0.9f)) (hopscotch_hash.h:1180); the float
 // quotient is obtained exactly as the correctly rounded double quotient
 // rounded to float (checked exhaustively for n < 2^24 on the host)

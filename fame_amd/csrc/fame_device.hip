@@ -299,49 +299,25 @@ __device__ __forceinline__ bool seed_prefix(const SeedWork<TabT, IdxT, EkmT>& w,
 }
 
 // The windows of a map at or above QTHRESH in hopscotch iteration order,
-// without the table, when the map's insertions need no rehash (at most
-// 0.9 B keys) and no displacement beyond the neighbourhood: the table is then
-// linear probing from each key's home bucket (key & (B - 1)) in insertion
-// order (find_empty_bucket, hopscotch_hash.h:1491-1500), kept as a bitmap of
-// at most 128 buckets in scalar registers, and iteration is by bucket.
-// Returns false where that does not hold (the caller emulates the table).
+// without the table, where lp_place applies (returns false elsewhere: the
+// caller emulates the table).
 template <typename TabT, typename IdxT, typename EkmT>
 __device__ __forceinline__ bool seed_order_small(const SeedWork<TabT, IdxT, EkmT>& w, int mp, uint32_t nins,
                                                  uint64_t bucket_count, uint32_t& nc)
 {
     const int lane = threadIdx.x & 63;
-    const uint64_t B = bucket_count ? rup_pow2(bucket_count) : 0;
-    const uint64_t arr = B ? B + NBH - 1 : 0;
-    if (B == 0 || B > 64 || nins > 64 || arr > w.tb_cap || nins > hs_thr(B, 0.9f)) return false;
+    if (nins > 64) return false;
     const uint32_t* key = w.ins_key + mp * w.cap_ins;
     const uint16_t* cnt = w.ins_cnt + mp * w.cap_ins;
     const bool in = (uint32_t)lane < nins;
     const uint32_t mykey = in ? key[lane] : 0u;
     const uint32_t mycnt = in ? cnt[lane] : 0u;
-    uint64_t occ0 = 0, occ1 = 0; // buckets 0..63, 64..127
-    uint32_t mypos = 0;
-    for (uint32_t t = 0; t < nins; ++t) {
-        const uint32_t ib = lane_u32(mykey, (int)t) & (uint32_t)(B - 1); // < 64
-        const uint64_t f0 = ~occ0 & (~0ull << ib);
-        const uint32_t e = f0 ? (uint32_t)__builtin_ctzll(f0) : (~occ1 ? 64u + (uint32_t)__builtin_ctzll(~occ1) : 128u);
-        if (e >= arr || e - ib >= NBH) return false;
-        if (e < 64) occ0 |= 1ull << e;
-        else occ1 |= 1ull << (e - 64);
-        if ((uint32_t)lane == t) mypos = e;
-    }
+    uint32_t mypos;
+    if (!lp_place(mykey, nins, bucket_count, w.tb_cap, mypos)) return false;
     const bool keep = in && mycnt >= QTHRESH;
-    uint64_t k0 = 0, k1 = 0;
-    for (uint64_t b = __ballot(keep); b; b &= b - 1) {
-        const uint32_t p = lane_u32(mypos, (int)__builtin_ctzll(b));
-        if (p < 64) k0 |= 1ull << p;
-        else k1 |= 1ull << (p - 64);
-    }
-    if (keep) {
-        const uint32_t rank = mypos < 64 ? (uint32_t)__popcll(k0 & ((1ull << mypos) - 1))
-                                         : (uint32_t)(__popcll(k0) + __popcll(k1 & ((1ull << (mypos - 64)) - 1)));
-        w.cand[mp * w.cap_ins + rank] = mykey;
-    }
-    nc = (uint32_t)(__popcll(k0) + __popcll(k1));
+    uint32_t rank;
+    nc = lp_rank(keep, mypos, rank);
+    if (keep) w.cand[mp * w.cap_ins + rank] = mykey;
     wave_sync();
     return true;
 }

@@ -441,6 +441,25 @@ __device__ int pe_first(const PeWork<TabT>& w, PeLds& m, uint32_t E, uint32_t L,
             if (lane == 0) m.nmap[mp] = nq[mp];
             continue;
         }
+        if (nins[mp] <= 64) {
+            // small maps: iteration order without the table (lp_place)
+            const bool in = (uint32_t)lane < nins[mp];
+            const uint32_t mykey = in ? w.ins_key[mp * w.cap_e + lane] : 0u;
+            const uint32_t mycnt = in ? w.ins_cnt[mp * w.cap_e + lane] : 0u;
+            uint32_t mypos;
+            if (lp_place(mykey, nins[mp], bucketCount, w.tb_cap, mypos)) {
+                const bool keep = in && mycnt >= QTHRESH;
+                uint32_t rank;
+                const uint32_t nc = lp_rank(keep, mypos, rank);
+                if (keep) {
+                    w.mkey[mp * w.mcap + rank] = mykey;
+                    w.mval[mp * w.mcap + rank] = mycnt;
+                }
+                if (lane == 0) m.nmap[mp] = nc;
+                wave_sync();
+                continue;
+            }
+        }
         Hs<uint32_t> h;
         h.tb = w.tb;
         h.tb_cap = w.tb_cap;
@@ -697,6 +716,36 @@ __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& p
                         // build them with 20 buckets, :2801-3164 insert into them)
                         uint32_t pos = off;
                         for (uint32_t s = 0; s < 2 && rc == 0; ++s) {
+                            const uint32_t ns = m.nmap[s];
+                            if (ns <= 64) {
+                                // small maps: iteration order without the table
+                                const bool occ = (uint32_t)lane < ns;
+                                const uint32_t mk = occ ? w.mkey[s * w.mcap + lane] : 0u;
+                                uint32_t mypos;
+                                if (lp_place(mk, ns, 20, w.tb_cap, mypos)) {
+                                    const uint32_t v = occ ? w.mval[s * w.mcap + lane] : 0u;
+                                    const bool job = occ && (v & PF2) && pe_c0(v) >= QTHRESH;
+                                    uint32_t ro, rj;
+                                    const uint32_t no = lp_rank(occ, mypos, ro);
+                                    const uint32_t nj = lp_rank(job, mypos, rj);
+                                    if (occ) {
+                                        PeItem it;
+                                        it.k = mk;
+                                        it.v = v;
+                                        it.j = PE_NONE;
+                                        if (job) {
+                                            it.j = jb + rj;
+                                            jm[it.j] = mk | (s ? 0x80000000u : 0u);
+                                            jr[it.j] = roA;
+                                        }
+                                        pd.map[pos + ro] = it;
+                                    }
+                                    pos += no;
+                                    jb += nj;
+                                    wave_sync();
+                                    continue;
+                                }
+                            }
                             Hs<uint32_t> h;
                             h.tb = w.tb;
                             h.tb_cap = w.tb_cap;
@@ -706,7 +755,6 @@ __device__ void pe_item(const DevIndex& ix, const BatchDev& bd, const PairDev& p
                             h.tmp_cap = w.cap_t;
                             h.key = w.mkey + s * w.mcap;
                             h.nelem = h.nover = 0;
-                            const uint32_t ns = m.nmap[s];
                             if (!hs_setup(h, 20, lane)) {
                                 rc = 1;
                                 break;
