@@ -1605,11 +1605,23 @@ int run_pairs(fame_ctx* c, const uint8_t* d_seqs, const uint64_t* d_offs, const 
             const size_t budget = 2ull << 30;
             pb.nslices = (uint32_t)std::max<size_t>(1, std::min<size_t>(big_n, budget / pb.stride));
             pb.scratch = grow<uint8_t>(c->b_heavy_scratch, pb.stride * pb.nslices);
+            // largest first (by the bit length of the hit count, a counting
+            // sort), so that no long item starts last
+            uint32_t nbin[33] = {0};
+            for (uint32_t x = 0; x < big_n; ++x) ++nbin[32 - (be[x] ? __builtin_clz(be[x]) : 32)];
+            uint32_t at[33];
+            for (int b = 32, s = 0; b >= 0; --b) {
+                at[b] = (uint32_t)s;
+                s += (int)nbin[b];
+            }
+            std::vector<uint32_t> bl(big_n);
+            for (uint32_t x = 0; x < big_n; ++x) bl[at[32 - (be[x] ? __builtin_clz(be[x]) : 32)]++] = bi[x];
             uint32_t* lst = grow<uint32_t>(c->b_pe_list, big_n + 1);
-            HIPC(hipMemcpyAsync(lst, bi.data(), 4 * big_n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(lst, bl.data(), 4 * big_n, hipMemcpyHostToDevice, st));
             pb.list = lst;
             pb.list_n = big_n;
             HIPC(hipMemsetAsync(pd.big_n, 0, 4, st));
+            HIPC(hipMemsetAsync(pd.next, 0, 4, st));
             tm.mark(12);
             launch_pe_seed(c->ix, bd, pb, pb.nslices, st);
             tm.mark(13);

@@ -974,8 +974,17 @@ __global__ __launch_bounds__(64) void k_pe_seed_big(DevIndex ix, BatchDev bd, Pa
     PoolChunk pcA = {0, 0, 0};
     uint64_t st[4] = {0, 0, 0, 0};
     MapChunk mc = {0, 0};
-    for (uint32_t ti = blockIdx.x; ti < pd.list_n; ti += gridDim.x)
+    // listed items taken one at a time from a device counter (their costs
+    // vary by orders of magnitude; the list holds the largest first), the
+    // next one's number fetched while the current one runs
+    uint32_t nt = 0;
+    if (lane == 0) nt = atomicAdd(pd.next, 1u);
+    uint32_t ti = lane_u32(nt, 0);
+    while (ti < pd.list_n) {
+        if (lane == 0) nt = atomicAdd(pd.next, 1u);
         pe_item<TabT>(ix, bd, pd, sh, m, w, pd.list[ti], pcA, mc, st);
+        ti = lane_u32(nt, 0);
+    }
     if (pcA.left) pool_gap_fill(bd, pcA.pool, pcA.base, pcA.left, lane);
     if (lane == 0 && st[2]) {
         atomicAdd(&bd.stats[0], (unsigned long long)st[0]);
